@@ -1,0 +1,821 @@
+/*
+ * libhexdyn.so C ABI (declared in include/hexdyn.h): handle lifetime, entity
+ * renumbering and row layout, state transfer, phase dispatch and the
+ * publish/validation semantics of one large step.
+ *
+ * Reference boundary: tree/src/hexcore/cuda_driver.py CudaDryDycoreDriver
+ * (from_host :2826-3025, _step_device_v841 :4854-5537, step_device :5751,
+ * step :6112-6149).  A step never mutates the published state: it works on
+ * working copies and publishes by swapping buffers only when the sticky
+ * validation flag stayed clear (cuda_driver.py:5355-5403, :5407-5427).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/hexdyn.h"
+#include "hexdyn_internal.h"
+#include "hexdyn_powf.h"
+
+using namespace hexdyn;
+
+namespace {
+
+#define HX_CHECK(expr)                                                              \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      if (h) h->error = std::string(#expr) + ": " + hipGetErrorString(_e);          \
+      return HEXDYN_E_HIP;                                                          \
+    }                                                                               \
+  } while (0)
+
+struct Schedule {
+  double large[3], acoustic[3];
+  int steps[3];
+};
+
+// integration.py:110-160 (RKSchedule.from_mpas)
+Schedule rk_schedule(double dt, int order, int substeps, int splits) {
+  Schedule s;
+  const double dt_dyn = dt / splits;
+  if (order == 3) {
+    s.large[0] = dt_dyn / 3.0; s.large[1] = dt_dyn / 2.0; s.large[2] = dt_dyn;
+    s.acoustic[0] = dt_dyn / 3.0; s.acoustic[1] = dt_dyn / substeps; s.acoustic[2] = dt_dyn / substeps;
+    s.steps[0] = 1; s.steps[1] = std::max(1, substeps / 2); s.steps[2] = substeps;
+  } else {
+    s.large[0] = dt_dyn / 2.0; s.large[1] = dt_dyn / 2.0; s.large[2] = dt_dyn;
+    for (int i = 0; i < 3; ++i) s.acoustic[i] = dt_dyn / substeps;
+    s.steps[0] = std::max(1, substeps / 2); s.steps[1] = std::max(1, substeps / 2); s.steps[2] = substeps;
+  }
+  return s;
+}
+
+// 3-D Hilbert key (Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707, 2004).
+uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+  uint32_t X[3] = {x, y, z};
+  const uint32_t M = 1u << (bits - 1);
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; ++i) X[i] ^= t;
+  uint64_t key = 0;
+  for (int b = bits - 1; b >= 0; --b)
+    for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+  return key;
+}
+
+uint64_t point_key(double x, double y, double z) {
+  const double r = std::sqrt(x * x + y * y + z * z);
+  const int bits = 16;
+  const double s = (double)((1u << bits) - 1);
+  auto q = [&](double v) { return (uint32_t)std::lround((v / r * 0.5 + 0.5) * s); };
+  return hilbert3(q(x), q(y), q(z), bits);
+}
+
+std::vector<int> order_by_keys(const std::vector<uint64_t> &keys) {
+  std::vector<int> idx(keys.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+  return idx;  // idx[new] = old
+}
+
+__global__ void k_import(float *__restrict__ dst, const float *__restrict__ src, const int *__restrict__ old_of_new,
+                         int n, int nlev) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = threadIdx.x & 63;
+  if (e >= n) return;
+  dst[(size_t)e * HX_ROW + k] = k < nlev ? src[(size_t)k * n + old_of_new[e]] : 0.0f;
+}
+
+__global__ void k_export(float *__restrict__ dst, const float *__restrict__ src, const int *__restrict__ old_of_new,
+                         int n, int nlev) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int k = threadIdx.x & 63;
+  if (e >= n || k >= nlev) return;
+  dst[(size_t)k * n + old_of_new[e]] = src[(size_t)e * HX_ROW + k];
+}
+
+struct ProfEntry {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+};
+
+}  // namespace
+
+namespace hexdyn {
+int g_prof_on = 0;
+static std::map<std::string, ProfEntry> g_prof;
+static std::map<std::string, hipEvent_t> g_prof_open;
+
+void prof_mark(const char *name, hipStream_t s, int start) {
+  hipEvent_t ev;
+  if (hipEventCreate(&ev) != hipSuccess) return;
+  (void)hipEventRecord(ev, s);
+  if (start) {
+    g_prof_open[name] = ev;
+  } else {
+    auto it = g_prof_open.find(name);
+    if (it == g_prof_open.end()) {
+      (void)hipEventDestroy(ev);
+      return;
+    }
+    g_prof[name].events.push_back({it->second, ev});
+    g_prof_open.erase(it);
+  }
+}
+}  // namespace hexdyn
+
+struct hexdyn_handle {
+  Instance I;
+  hexdyn_config cfg;
+  Schedule dyn, sca;
+  int fast = 0;
+  int in_step = 0;
+  int cur_sub = 1;
+  double time_s = 0.0;
+  std::string error;
+  std::vector<void *> allocs;
+  int *old_c = nullptr, *old_e = nullptr, *old_v = nullptr;  // device: old index of each new entity
+  std::vector<int> old_c_h, old_e_h, old_v_h;
+
+  template <class T> int alloc(T **p, size_t count) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+      error = "hipMalloc failed";
+      return HEXDYN_E_HIP;
+    }
+    if (hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return HEXDYN_E_HIP;
+    allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+  }
+  template <class T> int upload(const T **p, const std::vector<T> &h) {
+    T *q = nullptr;
+    int rc = alloc(&q, h.size());
+    if (rc) return rc;
+    if (hipMemcpy(q, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return HEXDYN_E_HIP;
+    *p = q;
+    return 0;
+  }
+};
+
+namespace {
+
+int run_phase_begin_step(hexdyn_handle *h, hipStream_t s);
+
+#define NS_CALL(h, fn, ...) ((h)->fast ? hexdyn::fast::fn(__VA_ARGS__) : hexdyn::strict::fn(__VA_ARGS__))
+
+int set_error(hexdyn_handle *h, int rc, const std::string &msg) {
+  if (h) h->error = msg;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hexdyn_abi_version(void) { return HEXDYN_ABI_VERSION; }
+
+float hexdyn_powf(float x, float y) { return hexdyn_powf_eval(x, y); }
+
+int hexdyn_last_error(const hexdyn_handle *h, char *buf, size_t len) {
+  if (!h || !buf || len == 0) return HEXDYN_E_INVALID;
+  std::snprintf(buf, len, "%s", h->error.c_str());
+  return 0;
+}
+
+double hexdyn_model_time(const hexdyn_handle *h) { return h ? h->time_s : 0.0; }
+
+void hexdyn_destroy(hexdyn_handle *h) {
+  if (!h) return;
+  for (void *p : h->allocs) (void)hipFree(p);
+  delete h;
+}
+
+int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_config *c, hexdyn_handle **out) {
+  if (!m || !vt || !c || !out) return HEXDYN_E_INVALID;
+  *out = nullptr;
+  hexdyn_handle *h = new hexdyn_handle();
+  h->cfg = *c;
+  Sizes &n = h->I.n;
+  n.C = m->n_cells; n.E = m->n_edges; n.V = m->n_vertices; n.L = vt->n_levels;
+  n.ME = m->max_edges; n.ME2 = m->max_edges2; n.MA = m->max_adv; n.NT = c->n_scalars; n.NM = c->n_moist;
+  auto refuse = [&](const std::string &msg) {
+    h->error = msg;
+    int rc = HEXDYN_E_REFUSED;
+    hexdyn_destroy(h);
+    return rc;
+  };
+  if (n.C <= 0 || n.E <= 0 || n.V <= 0) return refuse("empty mesh");
+  if (n.L < 4 || n.L + 1 > HX_ROW) return refuse("n_levels must satisfy 4 <= L and L + 1 <= 64");
+  if (n.ME < 3 || n.ME > HEXDYN_MAX_EDGES) return refuse("max_edges outside [3, 7]");
+  if (m->vertex_degree != 3) return refuse("vertex_degree must be 3");
+  if (n.NT < 0 || n.NM < 0 || n.NM > n.NT) return refuse("n_moist must lie in [0, n_scalars]");
+  if (c->config_time_integration_order != 2 && c->config_time_integration_order != 3)
+    return refuse("config_time_integration_order must be 2 or 3");
+  if (c->config_dynamics_split_steps != 1 && c->config_dynamics_split_steps != 3)
+    return refuse("config_dynamics_split_steps must be 1 or 3");
+  if (c->config_dynamics_split_steps == 3 && !c->config_split_dynamics_transport)
+    return refuse("config_dynamics_split_steps=3 requires config_split_dynamics_transport");
+  if (c->config_number_of_sub_steps < 1) return refuse("config_number_of_sub_steps must be positive");
+  if (!(c->config_dt > 0.0)) return refuse("config_dt must be positive");
+  if (c->config_apvm_upwinding < 0.0f) return refuse("config_apvm_upwinding must be non-negative");
+  if (c->config_divergence_damping && !(c->config_len_disp > 0.0f)) return refuse("config_len_disp must be positive");
+  h->fast = c->numerics == HEXDYN_NUMERICS_FAST;
+  const int C = n.C, E = n.E, V = n.V, L = n.L, ME = n.ME, ME2 = n.ME2, MA = n.MA;
+
+  // ---- renumbering along a 3-D Hilbert curve
+  std::vector<uint64_t> kc(C), ke(E), kv(V);
+  const bool have_xyz = m->x_cell && m->y_cell && m->z_cell;
+  for (int i = 0; i < C; ++i) kc[i] = have_xyz ? point_key(m->x_cell[i], m->y_cell[i], m->z_cell[i]) : (uint64_t)i;
+  for (int e = 0; e < E; ++e) {
+    const int a = m->cells_on_edge[2 * e], b = m->cells_on_edge[2 * e + 1];
+    if (a < 0 || b < 0 || a >= C || b >= C) return refuse("cellsOnEdge out of range");
+    ke[e] = have_xyz ? point_key(m->x_cell[a] + m->x_cell[b], m->y_cell[a] + m->y_cell[b], m->z_cell[a] + m->z_cell[b])
+                     : (uint64_t)e;
+  }
+  for (int v = 0; v < V; ++v) {
+    double x = 0, y = 0, z = 0;
+    for (int j = 0; j < 3; ++j) {
+      const int cc = m->cells_on_vertex[3 * v + j];
+      if (cc < 0 || cc >= C) return refuse("cellsOnVertex out of range");
+      if (have_xyz) { x += m->x_cell[cc]; y += m->y_cell[cc]; z += m->z_cell[cc]; }
+    }
+    kv[v] = have_xyz ? point_key(x, y, z) : (uint64_t)v;
+  }
+  h->old_c_h = order_by_keys(kc);
+  h->old_e_h = order_by_keys(ke);
+  h->old_v_h = order_by_keys(kv);
+  std::vector<int> new_c(C), new_e(E), new_v(V);
+  for (int i = 0; i < C; ++i) new_c[h->old_c_h[i]] = i;
+  for (int i = 0; i < E; ++i) new_e[h->old_e_h[i]] = i;
+  for (int i = 0; i < V; ++i) new_v[h->old_v_h[i]] = i;
+  auto chk = [](int x, int lim) { return x >= 0 && x < lim; };
+
+  // ---- connectivity and geometry in new numbering
+  std::vector<int> coe(2 * E), voe(2 * E), eoc((size_t)C * ME, 0), coc((size_t)C * ME, 0), voc((size_t)C * ME, 0),
+      neoc(C), kfc((size_t)C * ME, 0), eov(3 * V), eoe((size_t)E * ME2, 0), neoe(E), nadv(E), advc((size_t)E * MA, 0);
+  std::vector<float> csign((size_t)C * ME, 0.0f), vsign(3 * V), woe((size_t)E * ME2, 0.0f), dc(E), dv(E), dcdv(E),
+      invdc(E), invdv(E), invarea(C), invtri(V), kite(3 * V), fedge(E), fvert(V), cosang(E), sinang(E),
+      adv((size_t)E * MA, 0.0f), adv3((size_t)E * MA, 0.0f);
+  for (int ne = 0; ne < E; ++ne) {
+    const int e = h->old_e_h[ne];
+    for (int j = 0; j < 2; ++j) {
+      const int cc = m->cells_on_edge[2 * e + j], vv = m->vertices_on_edge[2 * e + j];
+      if (!chk(vv, V)) return refuse("verticesOnEdge out of range");
+      coe[2 * ne + j] = new_c[cc];
+      voe[2 * ne + j] = new_v[vv];
+    }
+    dc[ne] = m->dc_edge[e]; dv[ne] = m->dv_edge[e]; dcdv[ne] = dc[ne] * dv[ne];
+    invdc[ne] = m->inv_dc_edge[e]; invdv[ne] = m->inv_dv_edge[e];
+    fedge[ne] = m->f_edge[e]; cosang[ne] = m->cos_angle_edge[e]; sinang[ne] = m->sin_angle_edge[e];
+    neoe[ne] = m->n_edges_on_edge[e];
+    if (neoe[ne] < 0 || neoe[ne] > ME2) return refuse("nEdgesOnEdge out of range");
+    for (int s = 0; s < neoe[ne]; ++s) {
+      const int o = m->edges_on_edge[(size_t)e * ME2 + s];
+      if (!chk(o, E)) return refuse("edgesOnEdge out of range");
+      eoe[(size_t)ne * ME2 + s] = new_e[o];
+      woe[(size_t)ne * ME2 + s] = m->weights_on_edge[(size_t)e * ME2 + s];
+    }
+    nadv[ne] = m->n_adv_cells_for_edge[e];
+    if (nadv[ne] < 0 || nadv[ne] > MA) return refuse("nAdvCellsForEdge out of range");
+    for (int s = 0; s < nadv[ne]; ++s) {
+      const int o = m->adv_cells_for_edge[(size_t)e * MA + s];
+      if (!chk(o, C)) return refuse("advCellsForEdge out of range");
+      advc[(size_t)ne * MA + s] = new_c[o];
+      adv[(size_t)ne * MA + s] = m->adv_coefs[(size_t)e * MA + s];
+      adv3[(size_t)ne * MA + s] = m->adv_coefs_3rd[(size_t)e * MA + s];
+    }
+  }
+  for (int nc = 0; nc < C; ++nc) {
+    const int cc = h->old_c_h[nc];
+    neoc[nc] = m->n_edges_on_cell[cc];
+    if (neoc[nc] < 3 || neoc[nc] > ME) return refuse("nEdgesOnCell out of range");
+    invarea[nc] = m->inv_area_cell[cc];
+    for (int s = 0; s < neoc[nc]; ++s) {
+      const size_t i = (size_t)cc * ME + s, o = (size_t)nc * ME + s;
+      const int e = m->edges_on_cell[i], nb = m->cells_on_cell[i], vv = m->vertices_on_cell[i];
+      if (!chk(e, E) || !chk(nb, C) || !chk(vv, V)) return refuse("cell connectivity out of range");
+      eoc[o] = new_e[e]; coc[o] = new_c[nb]; voc[o] = new_v[vv];
+      kfc[o] = m->kite_for_cell[i];
+      if (kfc[o] < 0 || kfc[o] > 2) return refuse("kiteForCell out of range");
+      csign[o] = m->cells_on_edge[2 * e] == cc ? 1.0f : -1.0f;
+    }
+  }
+  for (int nv = 0; nv < V; ++nv) {
+    const int vv = h->old_v_h[nv];
+    invtri[nv] = m->inv_area_triangle[vv];
+    fvert[nv] = m->f_vertex[vv];
+    for (int j = 0; j < 3; ++j) {
+      const int e = m->edges_on_vertex[3 * vv + j];
+      if (!chk(e, E)) return refuse("edgesOnVertex out of range");
+      eov[3 * nv + j] = new_e[e];
+      vsign[3 * nv + j] = m->vertices_on_edge[2 * e + 1] == vv ? 1.0f : -1.0f;
+      kite[3 * nv + j] = m->kite_areas_on_vertex[3 * vv + j];
+    }
+  }
+  Mesh &M = h->I.m;
+  int rc = 0;
+  rc |= h->upload(&M.coe, coe); rc |= h->upload(&M.voe, voe); rc |= h->upload(&M.eoc, eoc);
+  rc |= h->upload(&M.coc, coc); rc |= h->upload(&M.voc, voc); rc |= h->upload(&M.neoc, neoc);
+  rc |= h->upload(&M.kfc, kfc); rc |= h->upload(&M.eov, eov); rc |= h->upload(&M.eoe, eoe);
+  rc |= h->upload(&M.neoe, neoe); rc |= h->upload(&M.nadv, nadv); rc |= h->upload(&M.advc, advc);
+  rc |= h->upload(&M.csign, csign); rc |= h->upload(&M.vsign, vsign); rc |= h->upload(&M.woe, woe);
+  rc |= h->upload(&M.dc, dc); rc |= h->upload(&M.dv, dv); rc |= h->upload(&M.dcdv, dcdv);
+  rc |= h->upload(&M.invdc, invdc); rc |= h->upload(&M.invdv, invdv); rc |= h->upload(&M.invarea, invarea);
+  rc |= h->upload(&M.invtri, invtri); rc |= h->upload(&M.kite, kite); rc |= h->upload(&M.fedge, fedge);
+  rc |= h->upload(&M.fvert, fvert); rc |= h->upload(&M.cosang, cosang); rc |= h->upload(&M.sinang, sinang);
+  rc |= h->upload(&M.adv, adv); rc |= h->upload(&M.adv3, adv3);
+  {
+    std::vector<int> a(h->old_c_h), b(h->old_e_h), cvec(h->old_v_h);
+    const int *pa, *pb, *pc;
+    rc |= h->upload(&pa, a); rc |= h->upload(&pb, b); rc |= h->upload(&pc, cvec);
+    h->old_c = (int *)pa; h->old_e = (int *)pb; h->old_v = (int *)pc;
+  }
+  if (rc) { int r = HEXDYN_E_HIP; hexdyn_destroy(h); return r; }
+
+  // ---- vertical metrics, padded per-level arrays and row-layout static fields
+  Vert &Z = h->I.z;
+  auto pad = [&](const float *src, int count) {
+    std::vector<float> v(HX_ROW, 0.0f);
+    for (int i = 0; i < count; ++i) v[i] = src[i];
+    return v;
+  };
+  auto rows_cells = [&](const float *src, int nlev) {  // [nlev][C] old -> [C][64] new
+    std::vector<float> v((size_t)C * HX_ROW, 0.0f);
+    for (int nc = 0; nc < C; ++nc)
+      for (int k = 0; k < nlev; ++k) v[(size_t)nc * HX_ROW + k] = src[(size_t)k * C + h->old_c_h[nc]];
+    return v;
+  };
+  auto rows_edges = [&](const float *src, int nlev) {
+    std::vector<float> v((size_t)E * HX_ROW, 0.0f);
+    for (int ne = 0; ne < E; ++ne)
+      for (int k = 0; k < nlev; ++k) v[(size_t)ne * HX_ROW + k] = src[(size_t)k * E + h->old_e_h[ne]];
+    return v;
+  };
+  rc |= h->upload(&Z.rdzw, pad(vt->rdzw, L)); rc |= h->upload(&Z.rdzu, pad(vt->rdzu, L));
+  rc |= h->upload(&Z.fzm, pad(vt->fzm, L)); rc |= h->upload(&Z.fzp, pad(vt->fzp, L));
+  rc |= h->upload(&Z.etp, pad(vt->etp, L)); rc |= h->upload(&Z.etm, pad(vt->etm, L));
+  rc |= h->upload(&Z.ewp, pad(vt->ewp, L + 1)); rc |= h->upload(&Z.ewm, pad(vt->ewm, L + 1));
+  rc |= h->upload(&Z.uinit, pad(vt->u_init, L)); rc |= h->upload(&Z.vinit, pad(vt->v_init, L));
+  Z.cf1 = vt->cf1; Z.cf2 = vt->cf2; Z.cf3 = vt->cf3;
+  rc |= h->upload(&Z.zz, rows_cells(vt->zz, L)); rc |= h->upload(&Z.dss, rows_cells(vt->dss, L));
+  rc |= h->upload(&Z.rb, rows_cells(vt->rho_base, L)); rc |= h->upload(&Z.rtb, rows_cells(vt->rtheta_base, L));
+  rc |= h->upload(&Z.exb, rows_cells(vt->exner_base, L));
+  rc |= h->upload(&Z.zxu, rows_edges(vt->zxu, L));
+  {
+    std::vector<float> zb((size_t)C * ME * HX_ROW, 0.0f), zb3((size_t)C * ME * HX_ROW, 0.0f);
+    for (int nc = 0; nc < C; ++nc) {
+      const int cc = h->old_c_h[nc];
+      for (int s = 0; s < ME; ++s)
+        for (int k = 0; k <= L; ++k) {
+          const size_t src = ((size_t)k * C + cc) * ME + s;
+          const size_t dst = ((size_t)nc * ME + s) * HX_ROW + k;
+          zb[dst] = vt->zb_cell[src];
+          zb3[dst] = vt->zb3_cell[src];
+        }
+    }
+    rc |= h->upload(&Z.zb, zb); rc |= h->upload(&Z.zb3, zb3);
+  }
+  if (rc) { int r = HEXDYN_E_HIP; hexdyn_destroy(h); return r; }
+
+  // ---- working set
+  Work &B = h->I.b;
+  const size_t cr = (size_t)C * HX_ROW, er = (size_t)E * HX_ROW, vr = (size_t)V * HX_ROW;
+  const size_t sr = (size_t)std::max(n.NT, 1) * cr, ser = (size_t)std::max(n.NT, 1) * er;
+  float **cell_bufs[] = {&B.rho, &B.rt, &B.rw, &B.theta, &B.exner, &B.rho_p, &B.rtheta_p, &B.pp, &B.w,
+                         &B.p_theta, &B.p_exner, &B.p_rho_p, &B.p_rtheta_p, &B.p_pp, &B.p_w,
+                         &B.c_rho, &B.c_rt, &B.c_rw, &B.s_rho, &B.s_rt, &B.s_rw, &B.s_theta, &B.s_rho_p,
+                         &B.s_rtheta_p, &B.s_exner, &B.qtot, &B.cqw, &B.cofwr, &B.cofwz, &B.cofwt, &B.coftz,
+                         &B.atri, &B.alpha, &B.gamma, &B.dpdz, &B.euler_rw, &B.tend_rho, &B.tend_rt, &B.tend_rw,
+                         &B.hdiv, &B.rw_p, &B.ww_avg, &B.rho_pp, &B.rtheta_pp, &B.rtheta_pp_old, &B.ke,
+                         &B.divergence, &B.pv_cell, &B.flux_w, &B.split_w, &B.target};
+  for (float **p : cell_bufs) rc |= h->alloc(p, cr);
+  float **edge_bufs[] = {&B.ru, &B.u, &B.p_u, &B.c_ru, &B.s_ru, &B.cqu, &B.euler_ru, &B.tend_ru, &B.fth,
+                         &B.fcorr, &B.fw, &B.ru_p, &B.ru_avg, &B.v, &B.pv_edge, &B.flux_u, &B.split_u};
+  for (float **p : edge_bufs) rc |= h->alloc(p, er);
+  float **vert_bufs[] = {&B.vorticity, &B.ke_vertex, &B.pv_vertex};
+  for (float **p : vert_bufs) rc |= h->alloc(p, vr);
+  float **scal_bufs[] = {&B.q, &B.q_stage, &B.q_next, &B.fct_mn, &B.fct_mx, &B.fct_mass, &B.fct_sin,
+                         &B.fct_sout, &B.fct_vres};
+  for (float **p : scal_bufs) rc |= h->alloc(p, sr);
+  float **sedge_bufs[] = {&B.ev, &B.fct_hres, &B.fct_upw};
+  for (float **p : sedge_bufs) rc |= h->alloc(p, ser);
+  rc |= h->alloc(&B.flag, 16);
+  if (rc) { int r = HEXDYN_E_HIP; hexdyn_destroy(h); return r; }
+  if (n.NM == 0) {
+    std::vector<float> ones(std::max(cr, er), 1.0f);
+    if (hipMemcpy(B.cqw, ones.data(), cr * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B.cqu, ones.data(), er * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+      hexdyn_destroy(h);
+      return HEXDYN_E_HIP;
+    }
+  }
+
+  // ---- constants in the reference's binary32 forms
+  Consts &K = h->I.k;
+  K.gravity = (float)9.80616;
+  K.rcv = 287.0f / (1004.5f - 287.0f);
+  K.c2 = 1004.5f * K.rcv;
+  K.cfac = (float)(287.0 / 100000.0);
+  K.rgas = 287.0f;
+  if (K.rcv != (float)(287.0 / (1004.5 - 287.0))) return refuse("inconsistent binary32 R/cv");
+  K.coef3 = c->config_coef_3rd_order;
+  K.apvm_on = c->config_apvm_upwinding > 0.0f;
+  K.apvm_scale = c->config_apvm_upwinding * (float)c->config_dt;
+  K.divdamp_base = (2.0f * c->config_smdiv) * c->config_len_disp;
+  h->dyn = rk_schedule(c->config_dt, c->config_time_integration_order, c->config_number_of_sub_steps,
+                       c->config_dynamics_split_steps);
+  h->sca = rk_schedule(c->config_dt, c->config_time_integration_order, c->config_number_of_sub_steps, 1);
+  for (int i = 0; i < 3; ++i) {
+    K.dts[i] = (float)h->dyn.acoustic[i];
+    K.dts_inv_coef[i] = K.divdamp_base * (1.0f / K.dts[i]);
+    K.steps[i] = h->dyn.steps[i];
+    K.inv_steps[i] = 1.0f / (float)h->dyn.steps[i];
+    K.sca_dt[i] = (float)h->sca.large[i];
+    K.wn[i] = 1.0f;
+  }
+  if (c->config_split_dynamics_transport) {
+    K.wn[0] = (float)(1.0 / (c->config_time_integration_order == 3 ? 3.0 : 2.0));
+    K.wn[1] = 0.5f;
+  }
+  K.split_inv = 1.0f / (float)c->config_dynamics_split_steps;
+  K.order = c->config_time_integration_order;
+  K.splits = c->config_dynamics_split_steps;
+  K.monotonic = c->config_monotonic;
+  K.divdamp = c->config_divergence_damping;
+  K.split_transport = c->config_split_dynamics_transport;
+  K.scalar_adv = c->config_scalar_advection;
+  *out = h;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct FieldRef {
+  float *ptr;
+  int n;       // entity count
+  int nlev;    // levels
+  const int *old;
+  int planes;  // scalar planes
+};
+
+FieldRef field_ref(hexdyn_handle *h, int id) {
+  Work &B = h->I.b;
+  const Sizes &n = h->I.n;
+  const int C = n.C, E = n.E, V = n.V, L = n.L;
+  const bool in = h->in_step;
+  auto cell = [&](float *p, int lv) { return FieldRef{p, C, lv, h->old_c, 1}; };
+  auto edge = [&](float *p) { return FieldRef{p, E, L, h->old_e, 1}; };
+  switch (id) {
+    case HEXDYN_F_RHO_ZZ: return cell(in ? B.c_rho : B.rho, L);
+    case HEXDYN_F_RTHETA_M: return cell(in ? B.c_rt : B.rt, L);
+    case HEXDYN_F_RU: return edge(in ? B.c_ru : B.ru);
+    case HEXDYN_F_RW: return cell(in ? B.c_rw : B.rw, L + 1);
+    case HEXDYN_F_SCALARS: return FieldRef{B.q, C, L, h->old_c, n.NT};
+    case HEXDYN_F_SCALARS_STAGE: return FieldRef{B.q_stage, C, L, h->old_c, n.NT};
+    case HEXDYN_F_THETA_M: return cell(in ? B.theta : B.p_theta, L);
+    case HEXDYN_F_EXNER: return cell(in ? B.exner : B.p_exner, L);
+    case HEXDYN_F_RHO_P: return cell(in ? B.rho_p : B.p_rho_p, L);
+    case HEXDYN_F_RTHETA_P: return cell(in ? B.rtheta_p : B.p_rtheta_p, L);
+    case HEXDYN_F_PRESSURE_P: return cell(in ? B.pp : B.p_pp, L);
+    case HEXDYN_F_U: return edge(in ? B.u : B.p_u);
+    case HEXDYN_F_W: return cell(in ? B.w : B.p_w, L + 1);
+    case HEXDYN_F_V: return edge(B.v);
+    case HEXDYN_F_PV_EDGE: return edge(B.pv_edge);
+    case HEXDYN_F_KE: return cell(B.ke, L);
+    case HEXDYN_F_DIVERGENCE: return cell(B.divergence, L);
+    case HEXDYN_F_VORTICITY: return FieldRef{B.vorticity, V, L, h->old_v, 1};
+    case HEXDYN_F_TEND_RU: return edge(B.tend_ru);
+    case HEXDYN_F_TEND_RT: return cell(B.tend_rt, L);
+    case HEXDYN_F_TEND_RW: return cell(B.tend_rw, L + 1);
+    case HEXDYN_F_TEND_RHO: return cell(B.tend_rho, L);
+    case HEXDYN_F_RU_P: return edge(B.ru_p);
+    case HEXDYN_F_RW_P: return cell(B.rw_p, L + 1);
+    case HEXDYN_F_RHO_PP: return cell(B.rho_pp, L);
+    case HEXDYN_F_RTHETA_PP: return cell(B.rtheta_pp, L);
+    case HEXDYN_F_RU_AVG: return edge(B.ru_avg);
+    case HEXDYN_F_WW_AVG: return cell(B.ww_avg, L + 1);
+    case HEXDYN_F_A_TRI: return cell(B.atri, L);
+    case HEXDYN_F_ALPHA_TRI: return cell(B.alpha, L);
+    case HEXDYN_F_GAMMA_TRI: return cell(B.gamma, L);
+    case HEXDYN_F_COFWZ: return cell(B.cofwz, L);
+    case HEXDYN_F_COFWT: return cell(B.cofwt, L);
+    case HEXDYN_F_COFTZ: return cell(B.coftz, L + 1);
+    case HEXDYN_F_EULER_RU: return edge(B.euler_ru);
+    case HEXDYN_F_EULER_RW: return cell(B.euler_rw, L + 1);
+    case HEXDYN_F_FLUX_U: return edge(B.flux_u);
+    case HEXDYN_F_FLUX_W: return cell(B.flux_w, L + 1);
+    case HEXDYN_F_QTOT: return cell(B.qtot, L);
+    case HEXDYN_F_CQW: return cell(B.cqw, L);
+    case HEXDYN_F_CQU: return edge(B.cqu);
+    default: return FieldRef{nullptr, 0, 0, nullptr, 0};
+  }
+}
+
+int transfer(hexdyn_handle *h, const FieldRef &f, float *dev, bool to_internal, hipStream_t s) {
+  for (int p = 0; p < f.planes; ++p) {
+    float *in_rows = f.ptr + (size_t)p * f.n * HX_ROW;
+    float *logical = dev + (size_t)p * f.nlev * f.n;
+    const int nb = (f.n + 3) / 4;
+    if (to_internal)
+      hipLaunchKernelGGL(k_import, dim3(nb), dim3(256), 0, s, in_rows, (const float *)logical, f.old, f.n, f.nlev);
+    else
+      hipLaunchKernelGGL(k_export, dim3(nb), dim3(256), 0, s, logical, (const float *)in_rows, f.old, f.n, f.nlev);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : HEXDYN_E_HIP;
+}
+
+int copy_rows(float *dst, const float *src, size_t rows, hipStream_t s) {
+  return hipMemcpyAsync(dst, src, rows * HX_ROW * sizeof(float), hipMemcpyDeviceToDevice, s) == hipSuccess
+             ? 0
+             : HEXDYN_E_HIP;
+}
+
+int run_phase_begin_step(hexdyn_handle *h, hipStream_t s) {
+  Work &B = h->I.b;
+  const Sizes &n = h->I.n;
+  int rc = 0;
+  rc |= copy_rows(B.c_rho, B.rho, n.C, s);
+  rc |= copy_rows(B.c_rt, B.rt, n.C, s);
+  rc |= copy_rows(B.c_ru, B.ru, n.E, s);
+  rc |= copy_rows(B.c_rw, B.rw, n.C, s);
+  rc |= copy_rows(B.theta, B.p_theta, n.C, s);
+  rc |= copy_rows(B.exner, B.p_exner, n.C, s);
+  rc |= copy_rows(B.rho_p, B.p_rho_p, n.C, s);
+  rc |= copy_rows(B.rtheta_p, B.p_rtheta_p, n.C, s);
+  rc |= copy_rows(B.pp, B.p_pp, n.C, s);
+  rc |= copy_rows(B.u, B.p_u, n.E, s);
+  rc |= copy_rows(B.w, B.p_w, n.C, s);
+  if (hipMemsetAsync(B.flag, 0, sizeof(int), s) != hipSuccess) rc |= HEXDYN_E_HIP;
+  if (rc) return HEXDYN_E_HIP;
+  h->in_step = 1;
+  return NS_CALL(h, begin_step, h->I, s);
+}
+
+int run_end_step(hexdyn_handle *h, hipStream_t s) {
+  Work &B = h->I.b;
+  const Sizes &n = h->I.n;
+  const bool transported = h->cfg.config_scalar_advection && h->cfg.config_split_dynamics_transport && n.NT > 0;
+  if (transported && n.NM > 0 && (!h->cfg.config_monotonic)) {
+    if (NS_CALL(h, clamp_scalars, h->I, B.q_stage, s)) return HEXDYN_E_HIP;
+  }
+  int flag = 0;
+  if (hipMemcpyAsync(&flag, B.flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return HEXDYN_E_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return HEXDYN_E_HIP;
+  h->in_step = 0;
+  if (flag) {
+    h->error = "validation flag set: non-finite or non-positive state; step not published";
+    return HEXDYN_E_NONFINITE;
+  }
+  std::swap(B.rho, B.c_rho); std::swap(B.rt, B.c_rt); std::swap(B.ru, B.c_ru); std::swap(B.rw, B.c_rw);
+  std::swap(B.p_theta, B.theta); std::swap(B.p_exner, B.exner); std::swap(B.p_rho_p, B.rho_p);
+  std::swap(B.p_rtheta_p, B.rtheta_p); std::swap(B.p_pp, B.pp); std::swap(B.p_u, B.u); std::swap(B.p_w, B.w);
+  if (transported) std::swap(B.q, B.q_stage);
+  h->time_s += h->cfg.config_dt;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hexdyn_set_state(hexdyn_handle *h, const hexdyn_state_view *st, int rebuild, void *stream) {
+  if (!h || !st) return HEXDYN_E_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  h->in_step = 0;
+  const struct { int id; float *p; } prog[] = {
+      {HEXDYN_F_RHO_ZZ, st->rho_zz}, {HEXDYN_F_RTHETA_M, st->rtheta_m}, {HEXDYN_F_RU, st->ru},
+      {HEXDYN_F_RW, st->rw}, {HEXDYN_F_SCALARS, st->scalars}};
+  for (auto &f : prog) {
+    if (!f.p) {
+      if (f.id == HEXDYN_F_SCALARS && h->I.n.NT == 0) continue;
+      return set_error(h, HEXDYN_E_INVALID, "missing prognostic field");
+    }
+    if (transfer(h, field_ref(h, f.id), f.p, true, s)) return HEXDYN_E_HIP;
+  }
+  Work &B = h->I.b;
+  if (rebuild) {
+    if (NS_CALL(h, rebuild_saved, h->I, s)) return HEXDYN_E_HIP;
+    const Sizes &n = h->I.n;
+    int rc = 0;
+    rc |= copy_rows(B.p_theta, B.theta, n.C, s); rc |= copy_rows(B.p_exner, B.exner, n.C, s);
+    rc |= copy_rows(B.p_rho_p, B.rho_p, n.C, s); rc |= copy_rows(B.p_rtheta_p, B.rtheta_p, n.C, s);
+    rc |= copy_rows(B.p_pp, B.pp, n.C, s); rc |= copy_rows(B.p_u, B.u, n.E, s);
+    rc |= copy_rows(B.p_w, B.w, n.C, s);
+    if (rc) return HEXDYN_E_HIP;
+  } else {
+    const struct { int id; float *p; } diag[] = {
+        {HEXDYN_F_THETA_M, st->theta_m}, {HEXDYN_F_EXNER, st->exner}, {HEXDYN_F_RHO_P, st->rho_p},
+        {HEXDYN_F_RTHETA_P, st->rtheta_p}, {HEXDYN_F_PRESSURE_P, st->pressure_p}, {HEXDYN_F_U, st->u},
+        {HEXDYN_F_W, st->w}};
+    for (auto &f : diag) {
+      if (!f.p) return set_error(h, HEXDYN_E_INVALID, "missing saved diagnostic field");
+      if (transfer(h, field_ref(h, f.id), f.p, true, s)) return HEXDYN_E_HIP;
+    }
+  }
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : HEXDYN_E_HIP;
+}
+
+int hexdyn_get_state(hexdyn_handle *h, const hexdyn_state_view *st, void *stream) {
+  if (!h || !st) return HEXDYN_E_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  const int saved = h->in_step;
+  h->in_step = 0;
+  const struct { int id; float *p; } all[] = {
+      {HEXDYN_F_RHO_ZZ, st->rho_zz}, {HEXDYN_F_RTHETA_M, st->rtheta_m}, {HEXDYN_F_RU, st->ru},
+      {HEXDYN_F_RW, st->rw}, {HEXDYN_F_SCALARS, st->scalars}, {HEXDYN_F_THETA_M, st->theta_m},
+      {HEXDYN_F_EXNER, st->exner}, {HEXDYN_F_RHO_P, st->rho_p}, {HEXDYN_F_RTHETA_P, st->rtheta_p},
+      {HEXDYN_F_PRESSURE_P, st->pressure_p}, {HEXDYN_F_U, st->u}, {HEXDYN_F_W, st->w}};
+  int rc = 0;
+  for (auto &f : all)
+    if (f.p && transfer(h, field_ref(h, f.id), f.p, false, s)) rc = HEXDYN_E_HIP;
+  h->in_step = saved;
+  if (rc) return rc;
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : HEXDYN_E_HIP;
+}
+
+int hexdyn_field_shape(const hexdyn_handle *h, int field, int64_t *rows, int64_t *cols) {
+  if (!h || !rows || !cols) return HEXDYN_E_INVALID;
+  FieldRef f = field_ref(const_cast<hexdyn_handle *>(h), field);
+  if (!f.ptr) return HEXDYN_E_INVALID;
+  *rows = (int64_t)f.planes * f.nlev;
+  *cols = f.n;
+  return 0;
+}
+
+int hexdyn_export_field(hexdyn_handle *h, int field, float *dst, void *stream) {
+  if (!h || !dst) return HEXDYN_E_INVALID;
+  FieldRef f = field_ref(h, field);
+  if (!f.ptr) return HEXDYN_E_INVALID;
+  if (transfer(h, f, dst, false, (hipStream_t)stream)) return HEXDYN_E_HIP;
+  return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : HEXDYN_E_HIP;
+}
+
+int hexdyn_import_field(hexdyn_handle *h, int field, const float *src, void *stream) {
+  if (!h || !src) return HEXDYN_E_INVALID;
+  FieldRef f = field_ref(h, field);
+  if (!f.ptr) return HEXDYN_E_INVALID;
+  if (transfer(h, f, const_cast<float *>(src), true, (hipStream_t)stream)) return HEXDYN_E_HIP;
+  return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? 0 : HEXDYN_E_HIP;
+}
+
+int hexdyn_begin_step(hexdyn_handle *h, void *stream) {
+  if (!h) return HEXDYN_E_INVALID;
+  return run_phase_begin_step(h, (hipStream_t)stream);
+}
+
+int hexdyn_begin_subcycle(hexdyn_handle *h, int subcycle, void *stream) {
+  if (!h || subcycle < 1 || subcycle > h->cfg.config_dynamics_split_steps) return HEXDYN_E_INVALID;
+  h->cur_sub = subcycle;
+  return NS_CALL(h, begin_subcycle, h->I, subcycle, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+}
+
+int hexdyn_dyn_tend(hexdyn_handle *h, int stage, void *stream) {
+  if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
+  return NS_CALL(h, dyn_tend, h->I, stage, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+}
+
+int hexdyn_acoustic_substep(hexdyn_handle *h, int stage, int small_step, void *stream) {
+  if (!h || stage < 1 || stage > 3 || small_step < 1 || small_step > h->I.k.steps[stage - 1]) return HEXDYN_E_INVALID;
+  return NS_CALL(h, acoustic_substep, h->I, stage, small_step, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+}
+
+int hexdyn_divergence_damping(hexdyn_handle *h, int stage, void *stream) {
+  if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
+  return NS_CALL(h, divergence_damping, h->I, stage, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+}
+
+int hexdyn_recover(hexdyn_handle *h, int stage, void *stream) {
+  if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
+  // At stage 3 the recovered fluxes are also accumulated into the split-transport
+  // sums (copy on the first subcycle, add afterwards: integration.py:27-56).
+  return NS_CALL(h, recover, h->I, stage, h->cur_sub, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+}
+
+int hexdyn_solve_diagnostics(hexdyn_handle *h, int rk_step, void *stream) {
+  if (!h || rk_step < 1 || rk_step > 3) return HEXDYN_E_INVALID;
+  return NS_CALL(h, solve_diagnostics, h->I, rk_step, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+}
+
+int hexdyn_end_subcycle(hexdyn_handle *h, int subcycle, void *stream) {
+  (void)stream;
+  if (!h || subcycle != h->cur_sub) return HEXDYN_E_INVALID;
+  return 0;  // carried diagnostics are updated in place; split sums were accumulated by hexdyn_recover
+}
+
+int hexdyn_scalars(hexdyn_handle *h, int stage, void *stream) {
+  if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  Work &B = h->I.b;
+  if (stage == 1 && copy_rows(B.q_stage, B.q, (size_t)h->I.n.NT * h->I.n.C, s)) return HEXDYN_E_HIP;
+  if (NS_CALL(h, scalars_stage, h->I, stage, s)) return HEXDYN_E_HIP;
+  std::swap(B.q_stage, B.q_next);
+  return 0;
+}
+
+int hexdyn_end_step(hexdyn_handle *h, void *stream) {
+  if (!h) return HEXDYN_E_INVALID;
+  return run_end_step(h, (hipStream_t)stream);
+}
+
+int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream) {
+  if (!h || n_steps < 0) return HEXDYN_E_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  const hexdyn_config &c = h->cfg;
+  for (int it = 0; it < n_steps; ++it) {
+    int rc = run_phase_begin_step(h, s);
+    for (int sub = 1; rc == 0 && sub <= c.config_dynamics_split_steps; ++sub) {
+      h->cur_sub = sub;
+      rc |= NS_CALL(h, begin_subcycle, h->I, sub, s);
+      for (int st = 1; rc == 0 && st <= 3; ++st) {
+        rc |= NS_CALL(h, dyn_tend, h->I, st, s);
+        for (int ss = 1; rc == 0 && ss <= h->I.k.steps[st - 1]; ++ss) {
+          rc |= NS_CALL(h, acoustic_substep, h->I, st, ss, s);
+          if (c.config_divergence_damping) rc |= NS_CALL(h, divergence_damping, h->I, st, s);
+        }
+        rc |= NS_CALL(h, recover, h->I, st, sub, s);
+        rc |= NS_CALL(h, solve_diagnostics, h->I, st, s);
+      }
+    }
+    if (rc == 0 && c.config_scalar_advection && c.config_split_dynamics_transport && h->I.n.NT > 0)
+      for (int st = 1; rc == 0 && st <= 3; ++st) rc |= hexdyn_scalars(h, st, stream);
+    if (rc) {
+      h->in_step = 0;
+      if (h->error.empty()) h->error = "kernel launch failed";
+      return HEXDYN_E_HIP;
+    }
+    rc = run_end_step(h, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int hexdyn_profile_enable(hexdyn_handle *h, int enable) {
+  if (!h) return HEXDYN_E_INVALID;
+  (void)hipDeviceSynchronize();
+  for (auto &kv : g_prof)
+    for (auto &p : kv.second.events) {
+      (void)hipEventDestroy(p.first);
+      (void)hipEventDestroy(p.second);
+    }
+  g_prof.clear();
+  g_prof_open.clear();
+  g_prof_on = enable ? 1 : 0;
+  return 0;
+}
+
+int hexdyn_profile_read(hexdyn_handle *h, const char *kernel, double *avg_ms, int64_t *launches) {
+  if (!h || !kernel || !avg_ms || !launches) return HEXDYN_E_INVALID;
+  (void)hipDeviceSynchronize();
+  auto it = g_prof.find(kernel);
+  if (it == g_prof.end()) {
+    *avg_ms = 0.0;
+    *launches = 0;
+    return HEXDYN_E_INVALID;
+  }
+  double total = 0.0;
+  for (auto &p : it->second.events) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, p.first, p.second);
+    total += ms;
+  }
+  *launches = (int64_t)it->second.events.size();
+  *avg_ms = *launches ? total / (double)*launches : 0.0;
+  return 0;
+}
+
+int hexdyn_profile_kernels(hexdyn_handle *h, char *buf, size_t len) {
+  if (!h || !buf || len == 0) return HEXDYN_E_INVALID;
+  std::string names;
+  for (auto &kv : g_prof) {
+    if (!names.empty()) names += ",";
+    names += kv.first;
+  }
+  std::snprintf(buf, len, "%s", names.c_str());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+/*
+ * Internal device-side description of one dycore instance.
+ *
+ * Layout: every field is entity-major with the level index fastest, one
+ * 64-float row (256 bytes) per entity, entities renumbered along a 3-D
+ * Hilbert curve.  Interface fields (rw, w, ...) use rows 0..L of the same
+ * 64-float row.  Scalars are [T][C][64].  Slot-indexed terrain metrics are
+ * [C][maxEdges][64].  A wavefront owns one entity row and lane k owns level k.
+ */
+#ifndef HEXDYN_INTERNAL_H
+#define HEXDYN_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HX_ROW 64
+
+namespace hexdyn {
+
+struct Sizes {
+  int C, E, V, L, ME, ME2, MA, NT, NM;
+};
+
+struct Mesh {
+  // connectivity (renumbered)
+  const int *coe;    // [E][2]
+  const int *voe;    // [E][2]
+  const int *eoc;    // [C][ME]
+  const int *coc;    // [C][ME]
+  const int *voc;    // [C][ME]
+  const int *neoc;   // [C]
+  const int *kfc;    // [C][ME]
+  const int *eov;    // [V][3]
+  const int *eoe;    // [E][ME2]
+  const int *neoe;   // [E]
+  const int *nadv;   // [E]
+  const int *advc;   // [E][MA]
+  // geometry
+  const float *csign;   // [C][ME] +1 first cell of the edge, -1 otherwise
+  const float *vsign;   // [V][3]
+  const float *woe;     // [E][ME2]
+  const float *dc, *dv, *dcdv, *invdc, *invdv;  // [E]
+  const float *invarea; // [C]
+  const float *invtri;  // [V]
+  const float *kite;    // [V][3]
+  const float *fedge, *fvert, *cosang, *sinang;
+  const float *adv, *adv3;  // [E][MA]
+};
+
+struct Vert {
+  const float *rdzw, *rdzu, *fzm, *fzp, *etp, *etm, *ewp, *ewm, *uinit, *vinit;  // [64] each, padded
+  float cf1, cf2, cf3;
+  const float *zz, *dss, *rb, *rtb, *exb;  // cell rows
+  const float *zxu;                        // edge rows
+  const float *zb, *zb3;                   // [C][ME][64]
+};
+
+// Device buffers of the working set (all row layout).
+struct Work {
+  // published (outer) state
+  float *rho, *rt, *ru, *rw, *q;
+  // carried diagnostics (time level one), working copies updated in place during a step
+  float *theta, *exner, *rho_p, *rtheta_p, *pp, *u, *w;
+  // published carried diagnostics
+  float *p_theta, *p_exner, *p_rho_p, *p_rtheta_p, *p_pp, *p_u, *p_w;
+  // current state
+  float *c_rho, *c_rt, *c_ru, *c_rw;
+  // subcycle start copies
+  float *s_rho, *s_rt, *s_ru, *s_rw, *s_theta, *s_rho_p, *s_rtheta_p, *s_exner;
+  // moist coefficients
+  float *qtot, *cqw, *cqu;
+  // implicit coefficients
+  float *cofwr, *cofwz, *cofwt, *coftz, *atri, *alpha, *gamma;
+  // subcycle pools
+  float *dpdz, *euler_ru, *euler_rw, *tend_rho;
+  // stage tendencies and edge fluxes
+  float *tend_ru, *tend_rt, *tend_rw, *hdiv, *fth, *fcorr, *fw;
+  // acoustic
+  float *ru_p, *ru_avg, *rw_p, *ww_avg, *rho_pp, *rtheta_pp, *rtheta_pp_old;
+  // diagnostics
+  float *v, *pv_edge, *ke, *divergence, *vorticity, *ke_vertex, *pv_vertex, *pv_cell;
+  // fluxes
+  float *flux_u, *flux_w, *split_u, *split_w;
+  // transport
+  float *q_stage, *q_next, *target, *ev, *fct_mn, *fct_mx, *fct_mass, *fct_sin, *fct_sout, *fct_vres,
+      *fct_hres, *fct_upw;
+  int *flag;  // sticky validation flag
+};
+
+struct Consts {
+  float gravity, rcv, c2, cfac, rgas;  // binary32 constants in reference form
+  float coef3, apvm_scale, divdamp_base;
+  float dts[3];            // acoustic timestep per stage
+  float dts_inv_coef[3];   // divergence damping coefficient per stage
+  float inv_steps[3];      // 1/steps per stage
+  int steps[3];
+  float sca_dt[3];         // scalar transport stage timesteps
+  float wn[3];             // split-transport density weights, stages 1-2
+  float split_inv;         // 1/splits
+  int order, splits, monotonic, divdamp, split_transport, scalar_adv;
+  int apvm_on;
+};
+
+// Optional per-kernel timing (events around each launch on its stream).
+void prof_mark(const char *name, hipStream_t s, int start);
+extern int g_prof_on;
+
+struct Instance {
+  Sizes n;
+  Mesh m;
+  Vert z;
+  Work b;
+  Consts k;
+};
+
+// Phase entry points, compiled once with strict binary32 numerics
+// (no contraction) and once with fused multiply-adds allowed.
+#define HEXDYN_PHASES(NS)                                                 \
+  namespace NS {                                                          \
+  int begin_step(const Instance &I, hipStream_t s);                       \
+  int begin_subcycle(const Instance &I, int sub, hipStream_t s);          \
+  int dyn_tend(const Instance &I, int stage, hipStream_t s);              \
+  int acoustic_substep(const Instance &I, int stage, int ss, hipStream_t s); \
+  int divergence_damping(const Instance &I, int stage, hipStream_t s);    \
+  int recover(const Instance &I, int stage, int sub, hipStream_t s);      \
+  int solve_diagnostics(const Instance &I, int rk, hipStream_t s);        \
+  int scalars_stage(const Instance &I, int stage, hipStream_t s);         \
+  int rebuild_saved(const Instance &I, hipStream_t s);                    \
+  int clamp_scalars(const Instance &I, float *q, hipStream_t s);          \
+  }
+
+HEXDYN_PHASES(strict)
+HEXDYN_PHASES(fast)
+
+}  // namespace hexdyn
+
+#endif

@@ -1,0 +1,1051 @@
+/*
+ * hexdyn device kernels and the phase functions that launch them.
+ *
+ * This translation unit is compiled twice: with -ffp-contract=off into
+ * namespace hexdyn::strict (every binary32 multiply and add rounded
+ * separately, in the reference's operand order, so results equal the CPU
+ * restatement bit for bit) and with -ffp-contract=fast into hexdyn::fast.
+ *
+ * Mapping: a block of 256 threads is 4 wavefronts; each wavefront owns one
+ * entity (cell, edge or vertex) row of 64 floats and lane k owns level k.
+ * Connectivity of the entity is wave-uniform (scalar loads); neighbour rows
+ * are read as whole 256-byte lines, so every cellsOnEdge / edgesOnCell /
+ * advCellsForEdge gather is one coalesced access per neighbour.  Blocks are
+ * renumbered so each XCD (hardware assigns block b to XCD b % 8) works on a
+ * contiguous range of the space-filling-curve order and its L2 sees the
+ * neighbour rows it needs.  Vertical neighbours come from cross-lane moves;
+ * the serial column recurrences (Thomas sweep, implicit coefficients) walk
+ * the lanes with readlane so their arithmetic order is the reference's.
+ *
+ * Reference operators (operation order per element):
+ *   solve diagnostics  cuda_horizontal_v841.py:67-200, diagnostics.py:95-306
+ *   dyn tendencies     dynamics.py:86-268, dynamics_v841.py:128-185,
+ *                      driver.py:933-1207, acoustic.py:246-281
+ *   implicit coefs     acoustic_v841.py:81-233
+ *   acoustic substep   acoustic_v841.py:296-565
+ *   divergence damping mixing.py:851-945
+ *   recovery           driver.py:3176-3290, driver.py:819-876, terrain.py:491-571
+ *   scalar transport   transport.py:1649-1826, transport.py:1829-2175
+ */
+#include <hip/hip_runtime.h>
+
+#include "hexdyn_internal.h"
+#include "hexdyn_powf.h"
+
+#ifndef HEXDYN_NS
+#define HEXDYN_NS strict
+#endif
+
+namespace hexdyn {
+namespace HEXDYN_NS {
+namespace {
+
+#define ROW(p, ent) ((p) + (size_t)(ent) * HX_ROW)
+#define KB __global__ void __launch_bounds__(256)
+
+__device__ __forceinline__ int wave_entity() {
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  const int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+  const int lb = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+  return __builtin_amdgcn_readfirstlane(lb * 4 + (int)(threadIdx.x >> 6));
+}
+__device__ __forceinline__ int lane() { return (int)(threadIdx.x & 63); }
+__device__ __forceinline__ float up1(float x) { return __shfl_up(x, 1, 64); }     // value of lane k-1
+__device__ __forceinline__ float dn1(float x) { return __shfl_down(x, 1, 64); }   // value of lane k+1
+__device__ __forceinline__ float rdl(float x, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+}
+__device__ __forceinline__ float cs1(float x) { return __builtin_copysignf(1.0f, x); }
+__device__ __forceinline__ float max0(float x) { return x > 0.0f ? x : 0.0f; }
+__device__ __forceinline__ float min0(float x) { return x < 0.0f ? x : 0.0f; }
+__device__ __forceinline__ float pmin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float pmax(float a, float b) { return b > a ? b : a; }
+__device__ __forceinline__ int cl(int k) { return k < 0 ? 0 : (k > 63 ? 63 : k); }
+
+__device__ __forceinline__ float flux4(float qm2, float qm1, float q0, float qp1, float vel) {
+  return (vel * (7.0f * (q0 + qm1) - (qp1 + qm2))) / 12.0f;
+}
+__device__ __forceinline__ float flux3(float qm2, float qm1, float q0, float qp1, float vel, float coef) {
+  return flux4(qm2, qm1, q0, qp1, vel) + ((coef * fabsf(vel)) * ((qp1 - qm2) - 3.0f * (q0 - qm1))) / 12.0f;
+}
+
+__device__ __forceinline__ void set_flag(int *flag) { atomicOr(flag, 1); }
+
+inline int nblocks(int n) { return (n + 3) / 4; }
+
+// ------------------------------------------------------------------ diagnostics
+KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
+  const int v = wave_entity();
+  if (v >= I.n.V) return;
+  const int k = lane();
+  float vort = 0.0f, kev = 0.0f;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int e = I.m.eov[3 * v + s];
+    const float ue = ROW(u, e)[k];
+    vort = vort + (I.m.vsign[3 * v + s] * I.m.dc[e]) * ue;
+    kev = kev + I.m.dcdv[e] * (ue * ue);
+  }
+  const float it = I.m.invtri[v];
+  vort = vort * it;
+  kev = kev * (0.25f * it);
+  if (k < I.n.L) {
+    ROW(I.b.vorticity, v)[k] = vort;
+    ROW(I.b.ke_vertex, v)[k] = kev;
+    ROW(I.b.pv_vertex, v)[k] = I.m.fvert[v] + vort;
+  }
+}
+
+KB k_diag_cell(const Instance I, const float *__restrict__ u) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int ME = I.n.ME;
+  const int n = I.m.neoc[c];
+  const float ia = I.m.invarea[c];
+  float d = 0.0f, kc = 0.0f, pvc = 0.0f;
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    const float ue = ROW(u, e)[k];
+    d = d + (I.m.csign[c * ME + s] * I.m.dv[e]) * ue;
+    kc = kc + 0.25f * (I.m.dcdv[e] * (ue * ue));
+  }
+  d = d * ia;
+  kc = kc * ia;
+  kc = kc * 0.625f;
+  for (int s = 0; s < n; ++s) {
+    const int vv = I.m.voc[c * ME + s];
+    const int ks = I.m.kfc[c * ME + s];
+    const float kite = I.m.kite[3 * vv + ks];
+    float corr = 0.375f * kite;
+    corr = corr * ROW(I.b.ke_vertex, vv)[k];
+    corr = corr * ia;
+    kc = kc + corr;
+    if (I.k.apvm_on) pvc = pvc + (kite * ROW(I.b.pv_vertex, vv)[k]) * ia;
+  }
+  if (k < I.n.L) {
+    ROW(I.b.divergence, c)[k] = d;
+    ROW(I.b.ke, c)[k] = kc;
+    if (I.k.apvm_on) ROW(I.b.pv_cell, c)[k] = pvc;
+  }
+}
+
+KB k_diag_edge(const Instance I, const float *__restrict__ u, int recon) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const int ME2 = I.n.ME2;
+  float vv;
+  if (recon) {
+    vv = 0.0f;
+    const int n = I.m.neoe[e];
+    for (int s = 0; s < n; ++s) vv = vv + I.m.woe[e * ME2 + s] * ROW(u, I.m.eoe[e * ME2 + s])[k];
+  } else {
+    vv = ROW(I.b.v, e)[k];
+  }
+  const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
+  const float pv0 = ROW(I.b.pv_vertex, v0)[k], pv1 = ROW(I.b.pv_vertex, v1)[k];
+  float pve = 0.5f * (pv0 + pv1);
+  if (I.k.apvm_on) {
+    const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+    const float gt = (pv1 - pv0) * I.m.invdv[e];
+    const float gn = (ROW(I.b.pv_cell, c1)[k] - ROW(I.b.pv_cell, c0)[k]) * I.m.invdc[e];
+    pve = pve - I.k.apvm_scale * (vv * gt + ROW(u, e)[k] * gn);
+  }
+  if (k < I.n.L) {
+    if (recon) ROW(I.b.v, e)[k] = vv;
+    ROW(I.b.pv_edge, e)[k] = pve;
+  }
+}
+
+// ---------------------------------------------------------- moist coefficients
+KB k_moist_cell(const Instance I) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const size_t plane = (size_t)I.n.C * HX_ROW;
+  float t = 0.0f;
+  for (int s = 0; s < I.n.NM; ++s) t = t + ROW(I.b.q + s * plane, c)[k];
+  const float tm = up1(t);
+  const float cqw = k >= 1 ? 1.0f / (1.0f + 0.5f * (t + tm)) : 1.0f;
+  if (k < I.n.L) {
+    ROW(I.b.qtot, c)[k] = t;
+    ROW(I.b.cqw, c)[k] = cqw;
+  }
+}
+
+KB k_moist_edge(const Instance I) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const size_t plane = (size_t)I.n.C * HX_ROW;
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  float t = 0.0f;
+  for (int s = 0; s < I.n.NM; ++s)
+    t = t + 0.5f * (ROW(I.b.q + s * plane, c0)[k] + ROW(I.b.q + s * plane, c1)[k]);
+  if (k < I.n.L) ROW(I.b.cqu, e)[k] = 1.0f / (1.0f + t);
+}
+
+// ------------------------------------------------------ implicit coefficients
+KB k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const float *__restrict__ theta,
+           const float *__restrict__ rtp) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L;
+  const int km = cl(k - 1), kp = cl(k + 1);
+  const float g = I.k.gravity, rcv = I.k.rcv, c2 = I.k.c2;
+  const float dts2 = dts * dts;
+  const float *zzr = ROW(I.z.zz, c);
+  const float zk = zzr[k], zkm = zzr[km];
+  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
+  const float rdzw = I.z.rdzw[k], rdzwm = I.z.rdzw[km];
+  const float etp = I.z.etp[k], etpm = I.z.etp[km];
+  const float exk = ROW(exn, c)[k], exkm = ROW(exn, c)[km];
+  float cofwr = 0.0f, cofwz = 0.0f, coftz = 0.0f;
+  if (k >= 1 && k < L) {
+    const float iz = fzm * zk + fzp * zkm;
+    cofwr = (0.5f * g) * iz;
+    cofwz = (((c2 * iz) * I.z.rdzu[k]) * ROW(I.b.cqw, c)[k]) * (fzm * exk + fzp * exkm);
+    coftz = fzm * ROW(theta, c)[k] + fzp * ROW(theta, c)[km];
+  }
+  float cofwt = 0.0f;
+  if (k < L) {
+    const float rbk = ROW(I.z.rb, c)[k], rtbk = ROW(I.z.rtb, c)[k], exbk = ROW(I.z.exb, c)[k];
+    cofwt = (((((((0.5f * rcv) * zk) * g) * rbk) / (1.0f + ROW(I.b.qtot, c)[k])) * exk) /
+             ((rtbk + ROW(rtp, c)[k]) * exbk));
+  }
+  const float coftz_m = up1(coftz), coftz_p = dn1(coftz), cofwt_m = up1(cofwt);
+  float a = 0.0f, b = 1.0f, cc = 0.0f;
+  if (k >= 1 && k < L) {
+    a = (((-cofwz) * coftz_m) * rdzwm) * zkm + cofwr * rdzwm - (cofwt_m * coftz_m) * rdzwm;
+    a = (a * etpm) * I.z.ewp[km];
+    b = (cofwz * coftz) * ((etp * rdzw) * zk + (etpm * rdzwm) * zkm) -
+        coftz * ((etp * cofwt) * rdzw - (etpm * cofwt_m) * rdzwm) + cofwr * (etp * rdzw - etpm * rdzwm);
+    b = b * I.z.ewp[k];
+    cc = (((-cofwz) * coftz_p) * rdzw) * zk - cofwr * rdzw + (cofwt * coftz_p) * rdzw;
+    cc = (cc * etp) * I.z.ewp[kp];
+    if (k == L - 1) cc = 0.0f;
+  }
+  float alpha = 0.0f, gamma = 0.0f;
+  int singular = 0;
+  for (int j = 1; j < L; ++j) {
+    const float gm = rdl(gamma, j - 1);
+    if (k == j) {
+      const float den = 1.0f + dts2 * (b - a * gm);
+      if (den == 0.0f) singular = 1;
+      alpha = 1.0f / den;
+      gamma = (dts2 * cc) * alpha;
+    }
+  }
+  if (singular) set_flag(I.b.flag);
+  if (k <= L) {
+    ROW(I.b.coftz, c)[k] = coftz;
+    if (k < L) {
+      ROW(I.b.cofwr, c)[k] = cofwr;
+      ROW(I.b.cofwz, c)[k] = cofwz;
+      ROW(I.b.cofwt, c)[k] = cofwt;
+      ROW(I.b.atri, c)[k] = a;
+      ROW(I.b.alpha, c)[k] = alpha;
+      ROW(I.b.gamma, c)[k] = gamma;
+    }
+  }
+}
+
+// ------------------------------------------------------------ subcycle start
+KB k_sub_cell(const Instance I) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L, ME = I.n.ME;
+  // subcycle-start copies
+  const float rho_p = ROW(I.b.rho_p, c)[k];
+  ROW(I.b.s_rho, c)[k] = ROW(I.b.c_rho, c)[k];
+  ROW(I.b.s_rt, c)[k] = ROW(I.b.c_rt, c)[k];
+  ROW(I.b.s_rw, c)[k] = ROW(I.b.c_rw, c)[k];
+  ROW(I.b.s_theta, c)[k] = ROW(I.b.theta, c)[k];
+  ROW(I.b.s_rho_p, c)[k] = rho_p;
+  ROW(I.b.s_rtheta_p, c)[k] = ROW(I.b.rtheta_p, c)[k];
+  ROW(I.b.s_exner, c)[k] = ROW(I.b.exner, c)[k];
+  const float g = I.k.gravity;
+  float dpdz;
+  if (I.n.NM <= 0) {
+    dpdz = (-g) * rho_p;
+  } else {
+    const float qt = ROW(I.b.qtot, c)[k];
+    dpdz = (-g) * (ROW(I.z.rb, c)[k] * qt + rho_p * (1.0f + qt));
+  }
+  // density tendency from the subcycle-start fluxes
+  float acc = 0.0f;
+  const int n = I.m.neoc[c];
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    acc = acc + (I.m.csign[c * ME + s] * I.m.dv[e]) * ROW(I.b.c_ru, e)[k];
+  }
+  acc = acc * I.m.invarea[c];
+  const float *rwr = ROW(I.b.c_rw, c);
+  const float trho = (-acc) - I.z.rdzw[k] * (rwr[cl(k + 1)] - rwr[k]);
+  // Euler w pool
+  const float *ppr = ROW(I.b.pp, c);
+  const float dpdzm = up1(dpdz);
+  float erw = 0.0f;
+  if (k >= 1 && k < L) {
+    const float inner = I.z.rdzu[k] * (ppr[k] - ppr[k - 1]) - (I.z.fzm[k] * dpdz + I.z.fzp[k] * dpdzm);
+    erw = I.n.NM <= 0 ? -inner : -(ROW(I.b.cqw, c)[k] * inner);
+  }
+  if (k < L) {
+    ROW(I.b.dpdz, c)[k] = dpdz;
+    ROW(I.b.tend_rho, c)[k] = trho;
+  }
+  if (k <= L) ROW(I.b.euler_rw, c)[k] = erw;
+}
+
+KB k_sub_edge(const Instance I) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  ROW(I.b.s_ru, e)[k] = ROW(I.b.c_ru, e)[k];
+  const float *pp = I.b.pp, *zz = I.z.zz, *dz = I.b.dpdz;
+  const float a = ((ROW(pp, c1)[k] - ROW(pp, c0)[k]) * I.m.invdc[e]) / (0.5f * (ROW(zz, c1)[k] + ROW(zz, c0)[k]));
+  const float bb = (0.5f * ROW(I.z.zxu, e)[k]) * (ROW(dz, c0)[k] + ROW(dz, c1)[k]);
+  if (k < I.n.L) ROW(I.b.euler_ru, e)[k] = (-ROW(I.b.cqu, e)[k]) * (a - bb);
+}
+
+// ---------------------------------------------------------- stage tendencies
+KB k_hdiv(const Instance I) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int ME = I.n.ME;
+  float acc = 0.0f;
+  const int n = I.m.neoc[c];
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    acc = acc + (I.m.csign[c * ME + s] * I.m.dv[e]) * ROW(I.b.c_ru, e)[k];
+  }
+  if (k < I.n.L) ROW(I.b.hdiv, c)[k] = acc * I.m.invarea[c];
+}
+
+KB k_tend_edge(const Instance I, int stage) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const int L = I.n.L, ME2 = I.n.ME2, MA = I.n.MA;
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  const float *ur = ROW(I.b.u, e);
+  // vertical transport of u (dynamics.py:223-268)
+  const float rwe = 0.5f * (ROW(I.b.c_rw, c0)[k] + ROW(I.b.c_rw, c1)[k]);
+  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
+  float wduz = 0.0f;
+  if (k == 1 || k == L - 1) {
+    wduz = rwe * (fzm * ur[k] + fzp * ur[k - 1]);
+  } else if (k >= 2 && k < L - 1) {
+    wduz = flux3(ur[k - 2], ur[k - 1], ur[k], ur[k + 1], rwe, 1.0f);
+  }
+  const float wduz_p = dn1(wduz);
+  const float initial = (-I.z.rdzw[k]) * (wduz_p - wduz);
+  // v8.4.1 nonlinear Coriolis, perturbation Coriolis, KE gradient, divergence term
+  const float pve = ROW(I.b.pv_edge, e)[k];
+  const int nn = I.m.neoe[e];
+  float qq = 0.0f;
+  for (int s = 0; s < nn; ++s) {
+    const int n = I.m.eoe[e * ME2 + s];
+    const float work_pv = 0.5f * (pve + ROW(I.b.pv_edge, n)[k]);
+    qq = qq + (I.m.woe[e * ME2 + s] * ROW(I.b.u, n)[k]) * work_pv;
+  }
+  const float ui = I.z.uinit[k], vi = I.z.vinit[k], fe = I.m.fedge[e];
+  for (int s = 0; s < nn; ++s) {
+    const int n = I.m.eoe[e * ME2 + s];
+    const float ref_u = ui * I.m.cosang[n] - vi * I.m.sinang[n];
+    qq = qq - (I.m.woe[e * ME2 + s] * ref_u) * fe;
+  }
+  const float rho_edge = 0.5f * (ROW(I.b.c_rho, c0)[k] + ROW(I.b.c_rho, c1)[k]);
+  const float horiz = rho_edge * (qq - (ROW(I.b.ke, c1)[k] - ROW(I.b.ke, c0)[k]) * I.m.invdc[e]);
+  const float t = (initial + horiz) - (ur[k] * 0.5f) * (ROW(I.b.hdiv, c0)[k] + ROW(I.b.hdiv, c1)[k]);
+  const float tend = t + ROW(I.b.euler_ru, e)[k];
+  // theta and w edge fluxes with the 3rd-order upwind-blended stencil
+  const float coef = I.k.coef3;
+  const float rue = ROW(I.b.c_ru, e)[k];
+  const float rue_m = ROW(I.b.c_ru, e)[cl(k - 1)];
+  const float rui = fzm * rue + fzp * rue_m;
+  const float sv_t = cs1(rue), sv_w = cs1(rui);
+  float et = 0.0f, ew = 0.0f;
+  const int na = I.m.nadv[e];
+  for (int s = 0; s < na; ++s) {
+    const int j = e * MA + s;
+    const int cell = I.m.advc[j];
+    const float a = I.m.adv[j], a3 = I.m.adv3[j];
+    et = et + (a + (coef * sv_t) * a3) * ROW(I.b.theta, cell)[k];
+    ew = ew + (a + (coef * sv_w) * a3) * ROW(I.b.w, cell)[k];
+  }
+  if (k < L) {
+    ROW(I.b.tend_ru, e)[k] = tend;
+    ROW(I.b.fth, e)[k] = rue * et;
+    if (stage > 1)
+      ROW(I.b.fcorr, e)[k] = ((I.m.dv[e] * (ROW(I.b.s_ru, e)[k] - rue)) * 0.5f) *
+                             (ROW(I.b.s_theta, c1)[k] + ROW(I.b.s_theta, c0)[k]);
+    if (k >= 1) ROW(I.b.fw, e)[k] = rui * ew;
+  }
+}
+
+KB k_tend_cell(const Instance I, int stage) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L, ME = I.n.ME;
+  const float coef = I.k.coef3;
+  const float *th = ROW(I.b.theta, c), *ths = ROW(I.b.s_theta, c);
+  const float *rwc = ROW(I.b.c_rw, c), *rws = ROW(I.b.s_rw, c), *ww = ROW(I.b.w, c);
+  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
+  const int km = cl(k - 1);
+  // vertical theta flux at interface k
+  float vf = 0.0f;
+  if (k == 1) {
+    vf = rwc[1] * (fzm * th[1] + fzp * th[0]);
+    vf = vf + (rws[1] - rwc[1]) * (fzm * ths[1] + fzp * ths[0]);
+  } else if (k >= 2 && k < L - 1) {
+    vf = flux3(th[k - 2], th[k - 1], th[k], th[k + 1], rwc[k], coef);
+    vf = vf + (rws[k] - rwc[k]) * (fzm * ths[k] + fzp * ths[k - 1]);
+  } else if (k == L - 1) {
+    vf = rws[k] * (fzm * th[k] + fzp * th[k - 1]);
+  }
+  const float vf_p = dn1(vf);
+  // vertical w flux at level k (between interfaces k and k+1 uses w[k], w[k-1])
+  float wf = 0.0f;
+  if (k == 1 || k == L - 1) {
+    wf = (0.25f * (rwc[k] + rwc[k - 1])) * (ww[k] + ww[k - 1]);
+  } else if (k >= 2 && k < L - 1) {
+    wf = flux3(ww[k - 2], ww[k - 1], ww[k], ww[k + 1], 0.5f * (rwc[k] + rwc[k - 1]), 1.0f);
+  }
+  const float wf_p = dn1(wf);
+  const int n = I.m.neoc[c];
+  const float ia = I.m.invarea[c];
+  float r = 0.0f;
+  for (int s = 0; s < n; ++s) r = r - I.m.csign[c * ME + s] * ROW(I.b.fth, I.m.eoc[c * ME + s])[k];
+  if (stage > 1)
+    for (int s = 0; s < n; ++s) r = r - I.m.csign[c * ME + s] * ROW(I.b.fcorr, I.m.eoc[c * ME + s])[k];
+  r = r * ia;
+  const float trt = r - I.z.rdzw[k] * (vf_p - vf);
+  float rw_t = 0.0f;
+  if (k >= 1 && k < L) {
+    float q = 0.0f;
+    for (int s = 0; s < n; ++s) q = q - I.m.csign[c * ME + s] * ROW(I.b.fw, I.m.eoc[c * ME + s])[k];
+    q = q * ia;
+    q = q - I.z.rdzu[k] * (wf_p - wf);
+    q = q + ROW(I.b.euler_rw, c)[k];
+    for (int s = 0; s < n; ++s) {
+      const int e = I.m.eoc[c * ME + s];
+      const float tk = ROW(I.b.tend_ru, e)[k];
+      const float flux = I.m.csign[c * ME + s] * (fzm * tk + fzp * ROW(I.b.tend_ru, e)[km]);
+      const size_t zi = ((size_t)c * ME + s) * HX_ROW + k;
+      q = q - (I.z.zb[zi] + cs1(tk) * I.z.zb3[zi]) * flux;
+    }
+    const float *zzr = ROW(I.z.zz, c);
+    rw_t = q * (fzm * zzr[k] + fzp * zzr[km]);
+  }
+  if (k < L) ROW(I.b.tend_rt, c)[k] = trt;
+  if (k <= L) ROW(I.b.tend_rw, c)[k] = rw_t;
+}
+
+// ----------------------------------------------------------- acoustic substep
+KB k_ac_edge(const Instance I, float dts, int first) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  if (k >= I.n.L) return;
+  float *rp = ROW(I.b.ru_p, e), *ra = ROW(I.b.ru_avg, e);
+  const float tr = ROW(I.b.tend_ru, e)[k];
+  if (first) {
+    const float p = dts * tr;
+    rp[k] = p;
+    ra[k] = p;
+    return;
+  }
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  float pg = ((ROW(I.b.rtheta_pp, c1)[k] - ROW(I.b.rtheta_pp, c0)[k]) * I.m.invdc[e]) /
+             (0.5f * (ROW(I.z.zz, c1)[k] + ROW(I.z.zz, c0)[k]));
+  pg = (((ROW(I.b.cqu, e)[k] * 0.5f) * I.k.c2) * (ROW(I.b.s_exner, c0)[k] + ROW(I.b.s_exner, c1)[k])) * pg;
+  pg = pg + ((0.5f * ROW(I.z.zxu, e)[k]) * I.k.gravity) * (ROW(I.b.rho_pp, c0)[k] + ROW(I.b.rho_pp, c1)[k]);
+  const float p = rp[k] + dts * (tr - pg);
+  rp[k] = p;
+  ra[k] = ra[k] + p;
+}
+
+KB k_ac_cell(const Instance I, float dts, int first) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L, ME = I.n.ME;
+  const int kp = cl(k + 1);
+  float rpp = 0.0f, rtpp = 0.0f, rwp = 0.0f, wwa = 0.0f;
+  if (!first) {
+    rpp = ROW(I.b.rho_pp, c)[k];
+    rtpp = ROW(I.b.rtheta_pp, c)[k];
+    rwp = ROW(I.b.rw_p, c)[k];
+    wwa = ROW(I.b.ww_avg, c)[k];
+  }
+  const float rtpp_old = rtpp;
+  // explicit horizontal divergence of the acoustic flux
+  const float ia = I.m.invarea[c];
+  float r = 0.0f, t = 0.0f;
+  const int n = I.m.neoc[c];
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    const int e0 = I.m.coe[2 * e], e1 = I.m.coe[2 * e + 1];
+    const float flux = (((I.m.csign[c * ME + s] * dts) * I.m.dv[e]) * ROW(I.b.ru_p, e)[k]) * ia;
+    r = r - flux;
+    t = t - (flux * 0.5f) * (ROW(I.b.s_theta, e1)[k] + ROW(I.b.s_theta, e0)[k]);
+  }
+  const float rdzw = I.z.rdzw[k];
+  const float coftz = ROW(I.b.coftz, c)[k];
+  const float coftz_p = ROW(I.b.coftz, c)[kp];
+  const float ewm = I.z.ewm[k], ewm_p = I.z.ewm[kp];
+  const float rwp_p = dn1(rwp);
+  const float rs = ((rpp + dts * ROW(I.b.tend_rho, c)[k]) + r) - (dts * rdzw) * (ewm_p * rwp_p - ewm * rwp);
+  const float ts = ((rtpp + dts * ROW(I.b.tend_rt, c)[k]) + t) -
+                   (dts * rdzw) * ((ewm_p * coftz_p) * rwp_p - (ewm * coftz) * rwp);
+  if (k >= 1 && k < L) wwa = wwa + ewm * rwp;
+  // right-hand side of the implicit w equation
+  const float *zzr = ROW(I.z.zz, c);
+  const float zk = zzr[k];
+  const float etp = I.z.etp[k], etm = I.z.etm[k];
+  const float ts_m = up1(ts), rs_m = up1(rs), rtpp_m = up1(rtpp), rpp_m = up1(rpp), zk_m = up1(zk);
+  const float cofwt = ROW(I.b.cofwt, c)[k];
+  const float cofwt_m = up1(cofwt);
+  const float etp_m = I.z.etp[cl(k - 1)], etm_m = I.z.etm[cl(k - 1)];
+  float col = rwp;
+  if (k >= 1 && k < L) {
+    float x = ROW(I.b.tend_rw, c)[k] -
+              ROW(I.b.cofwz, c)[k] * (((etp * zk) * ts - (etp_m * zk_m) * ts_m) +
+                                      ((etm * zk) * rtpp - (etm_m * zk_m) * rtpp_m));
+    x = x - ROW(I.b.cofwr, c)[k] * ((etp * rs + etp_m * rs_m) + (etm * rpp + etm_m * rpp_m));
+    x = x + cofwt * (etp * ts + etm * rtpp);
+    x = x + cofwt_m * (etp_m * ts_m + etm_m * rtpp_m);
+    col = col + dts * x;
+  }
+  // Thomas sweep in level order (one lane per level, serial dependence)
+  const float dts2 = dts * dts;
+  const float a = ROW(I.b.atri, c)[k], al = ROW(I.b.alpha, c)[k], ga = ROW(I.b.gamma, c)[k];
+  for (int j = 1; j < L; ++j) {
+    const float prev = rdl(col, j - 1);
+    if (k == j) col = (col - (dts2 * a) * prev) * al;
+  }
+  for (int j = L - 1; j >= 0; --j) {
+    const float next = rdl(col, j + 1);
+    if (k == j) col = col - ga * next;
+  }
+  // Rayleigh damping of w
+  if (k >= 1 && k < L) {
+    const float delta = ROW(I.b.s_rw, c)[k] - ROW(I.b.c_rw, c)[k];
+    const float damp = ROW(I.z.dss, c)[k];
+    const float zi = I.z.fzm[k] * zk + I.z.fzp[k] * zzr[k - 1];
+    const float ri = I.z.fzm[k] * ROW(I.b.c_rho, c)[k] + I.z.fzp[k] * ROW(I.b.c_rho, c)[k - 1];
+    col = (col + delta - (((dts * damp) * zi) * ri) * ROW(I.b.w, c)[k]) / (1.0f + dts * damp) - delta;
+  }
+  if (k >= 1 && k < L) wwa = wwa + I.z.ewp[k] * col;
+  const float col_p = dn1(col);
+  const float ewp = I.z.ewp[k], ewp_p = I.z.ewp[kp];
+  const float rpp_new = rs - (dts * rdzw) * (ewp_p * col_p - ewp * col);
+  const float rtpp_new = ts - (dts * rdzw) * ((ewp_p * coftz_p) * col_p - (ewp * coftz) * col);
+  if (k <= L) {
+    ROW(I.b.rw_p, c)[k] = col;
+    ROW(I.b.ww_avg, c)[k] = wwa;
+  }
+  if (k < L) {
+    ROW(I.b.rho_pp, c)[k] = rpp_new;
+    ROW(I.b.rtheta_pp, c)[k] = rtpp_new;
+    ROW(I.b.rtheta_pp_old, c)[k] = rtpp_old;
+  }
+}
+
+KB k_divdamp(const Instance I, float coefficient) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  if (k >= I.n.L) return;
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  const float d0 = ROW(I.b.rtheta_pp, c0)[k] - ROW(I.b.rtheta_pp_old, c0)[k];
+  const float d1 = ROW(I.b.rtheta_pp, c1)[k] - ROW(I.b.rtheta_pp_old, c1)[k];
+  const float den = ROW(I.b.s_theta, c0)[k] + ROW(I.b.s_theta, c1)[k];
+  float *rp = ROW(I.b.ru_p, e);
+  rp[k] = rp[k] + ((coefficient * (d0 - d1)) * (1.0f - 0.0f)) / den;
+}
+
+// --------------------------------------------------------------------- recovery
+// w from rw and the terrain-following horizontal flux (terrain.py:491-571).
+// `ru_of(e, k)` must return the edge momentum at level k.
+template <class RU>
+__device__ __forceinline__ float recover_w_column(const Instance &I, int c, int k, float rho_k, float rw_k, RU ru_of) {
+  const int L = I.n.L, ME = I.n.ME;
+  const float *zzr = ROW(I.z.zz, c);
+  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
+  const int km = cl(k - 1);
+  float col = 0.0f;
+  if (k >= 1 && k < L) col = rw_k / (fzm * zzr[k] + fzp * zzr[km]);
+  const int n = I.m.neoc[c];
+  const float cf1 = I.z.cf1, cf2 = I.z.cf2, cf3 = I.z.cf3;
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    const float sg = I.m.csign[c * ME + s];
+    const size_t zi = ((size_t)c * ME + s) * HX_ROW + k;
+    if (k == 0) {
+      const float flux = (cf1 * ru_of(e, 0) + cf2 * ru_of(e, 1)) + cf3 * ru_of(e, 2);
+      col = col + (sg * (I.z.zb[zi] + cs1(flux) * I.z.zb3[zi])) * flux;
+    } else if (k < L) {
+      const float flux = fzm * ru_of(e, k) + fzp * ru_of(e, k - 1);
+      col = col + (sg * (I.z.zb[zi] + cs1(flux) * I.z.zb3[zi])) * flux;
+    }
+  }
+  const float r0 = rdl(rho_k, 0), r1 = rdl(rho_k, 1), r2 = rdl(rho_k, 2);
+  const float rho_m = up1(rho_k);
+  if (k == 0) {
+    const float bottom = (cf1 * r0 + cf2 * r1) + cf3 * r2;
+    col = col / bottom;
+  } else if (k < L) {
+    col = col / (fzm * rho_k + fzp * rho_m);
+  } else {
+    col = 0.0f;
+  }
+  return col;
+}
+
+KB k_rec_cell(const Instance I, int stage, int sub, float inv_steps) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L;
+  float rho = 1.0f, rtv = 1.0f;
+  int bad = 0;
+  if (k < L) {
+    const float dp = ROW(I.b.s_rho_p, c)[k] + ROW(I.b.rho_pp, c)[k];
+    const float rp = ROW(I.b.s_rtheta_p, c)[k] + ROW(I.b.rtheta_pp, c)[k];
+    ROW(I.b.rho_p, c)[k] = dp;
+    ROW(I.b.rtheta_p, c)[k] = rp;
+    rho = dp + ROW(I.z.rb, c)[k];
+    rtv = rp + ROW(I.z.rtb, c)[k];
+    if (!(rho > 0.0f) || !(rtv > 0.0f)) bad = 1;
+    ROW(I.b.c_rho, c)[k] = rho;
+    ROW(I.b.c_rt, c)[k] = rtv;
+    ROW(I.b.theta, c)[k] = rtv / rho;
+    if (stage == 3) {
+      const float zk = ROW(I.z.zz, c)[k];
+      const float arg = (zk * I.k.cfac) * rtv;
+      if (!(arg > 0.0f)) bad = 1;
+      const float ex = hexdyn_powf_eval(arg, I.k.rcv);
+      ROW(I.b.exner, c)[k] = ex;
+      ROW(I.b.pp, c)[k] = (zk * I.k.rgas) * (ex * rp + ROW(I.z.rtb, c)[k] * (ex - ROW(I.z.exb, c)[k]));
+    }
+  }
+  float rwv = 0.0f;
+  if (k >= 1 && k < L) rwv = ROW(I.b.s_rw, c)[k] + ROW(I.b.rw_p, c)[k];
+  if (k <= L) {
+    ROW(I.b.c_rw, c)[k] = rwv;
+    const float fw = ROW(I.b.s_rw, c)[k] + ROW(I.b.ww_avg, c)[k] * inv_steps;
+    ROW(I.b.flux_w, c)[k] = fw;
+    if (stage == 3) ROW(I.b.split_w, c)[k] = sub == 1 ? fw : fw + ROW(I.b.split_w, c)[k];
+  }
+  const float *sru = I.b.s_ru, *rup = I.b.ru_p;
+  const float wv = recover_w_column(I, c, k, rho, rwv, [&](int e, int kk) {
+    return ROW(sru, e)[kk] + ROW(rup, e)[kk];
+  });
+  if (k <= L) ROW(I.b.w, c)[k] = wv;
+  if (bad) set_flag(I.b.flag);
+}
+
+KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  if (k >= I.n.L) return;
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  const float sru = ROW(I.b.s_ru, e)[k];
+  const float ru = sru + ROW(I.b.ru_p, e)[k];
+  ROW(I.b.c_ru, e)[k] = ru;
+  const float fu = sru + ROW(I.b.ru_avg, e)[k] * inv_steps;
+  ROW(I.b.flux_u, e)[k] = fu;
+  if (stage == 3) ROW(I.b.split_u, e)[k] = sub == 1 ? fu : fu + ROW(I.b.split_u, e)[k];
+  const float r0 = (ROW(I.b.s_rho_p, c0)[k] + ROW(I.b.rho_pp, c0)[k]) + ROW(I.z.rb, c0)[k];
+  const float r1 = (ROW(I.b.s_rho_p, c1)[k] + ROW(I.b.rho_pp, c1)[k]) + ROW(I.z.rb, c1)[k];
+  ROW(I.b.u, e)[k] = (2.0f * ru) / (r0 + r1);
+}
+
+// rebuild of the saved diagnostics from a prognostic state (driver.py:2655-2700)
+KB k_rebuild_cell(const Instance I) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L;
+  float rho = 1.0f;
+  if (k < L) {
+    rho = ROW(I.b.rho, c)[k];
+    const float rt = ROW(I.b.rt, c)[k];
+    const float zk = ROW(I.z.zz, c)[k];
+    const float ex = hexdyn_powf_eval((zk * I.k.cfac) * rt, I.k.rcv);
+    const float rtp = rt - ROW(I.z.rtb, c)[k];
+    ROW(I.b.theta, c)[k] = rt / rho;
+    ROW(I.b.exner, c)[k] = ex;
+    ROW(I.b.rho_p, c)[k] = rho - ROW(I.z.rb, c)[k];
+    ROW(I.b.rtheta_p, c)[k] = rtp;
+    ROW(I.b.pp, c)[k] = (zk * I.k.rgas) * (ex * rtp + ROW(I.z.rtb, c)[k] * (ex - ROW(I.z.exb, c)[k]));
+  }
+  const float rwk = ROW(I.b.rw, c)[k];
+  const float *ru = I.b.ru;
+  const float wv = recover_w_column(I, c, k, rho, rwk, [&](int e, int kk) { return ROW(ru, e)[kk]; });
+  if (k <= L) ROW(I.b.w, c)[k] = wv;
+}
+
+KB k_rebuild_edge(const Instance I) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  if (k >= I.n.L) return;
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  ROW(I.b.u, e)[k] = (2.0f * ROW(I.b.ru, e)[k]) / (ROW(I.b.rho, c0)[k] + ROW(I.b.rho, c1)[k]);
+}
+
+// ------------------------------------------------------------ scalar transport
+__device__ __forceinline__ float vflux(const float *qc, float wk, int k, int L, float fzm, float fzp, float coef) {
+  if (k == 1 || k == L - 1) return wk * (fzm * qc[k] + fzp * qc[k - 1]);
+  const float q0 = qc[k - 2], q1 = qc[k - 1], q2 = qc[k], q3 = qc[k + 1];
+  const float base = (wk * (7.0f * (q2 + q1) - (q3 + q0))) / 12.0f;
+  const float corr = ((coef * fabsf(wk)) * ((q3 - q0) - 3.0f * (q2 - q1))) / 12.0f;
+  return base + corr;
+}
+
+KB k_split_scale(const Instance I, int n, float *p, float inv) {
+  const int e = wave_entity();
+  if (e >= n) return;
+  const int k = lane();
+  ROW(p, e)[k] = ROW(p, e)[k] * inv;
+}
+
+KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__ ev) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const int MA = I.n.MA;
+  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int sp = blockIdx.y;
+  const float *q = qs + sp * cplane;
+  const float sv = cs1(ROW(I.b.split_u, e)[k]);
+  const float coef = I.k.coef3;
+  float r = 0.0f;
+  const int na = I.m.nadv[e];
+  for (int s = 0; s < na; ++s) {
+    const int j = e * MA + s;
+    r = r + (I.m.adv[j] + (coef * sv) * I.m.adv3[j]) * ROW(q, I.m.advc[j])[k];
+  }
+  if (k < I.n.L) ROW(ev + sp * eplane, e)[k] = r;
+}
+
+KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__restrict__ ev, float *__restrict__ out,
+             float dt, float wn) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L, ME = I.n.ME;
+  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int sp = blockIdx.y;
+  const float *q = ROW(qs + sp * cplane, c);
+  float vf = 0.0f;
+  if (k >= 1 && k < L) vf = vflux(q, ROW(I.b.split_w, c)[k], k, L, I.z.fzm[k], I.z.fzp[k], I.k.coef3);
+  const float vf_p = dn1(vf);
+  float tc = 0.0f;
+  const int n = I.m.neoc[c];
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    tc = tc + ((-I.m.csign[c * ME + s]) * ROW(I.b.split_u, e)[k]) * ROW(ev + sp * eplane, e)[k];
+  }
+  tc = tc * I.m.invarea[c] + 0.0f;
+  if (k < L) {
+    const float orho = ROW(I.b.rho, c)[k];
+    const float target = (1.0f - wn) * orho + wn * ROW(I.b.c_rho, c)[k];
+    const float qo = ROW(I.b.q + sp * cplane, c)[k];
+    const float num = qo * orho + dt * (tc - I.z.rdzw[k] * (vf_p - vf));
+    ROW(out + sp * cplane, c)[k] = num * (1.0f / target);
+  }
+}
+
+KB k_fct_target(const Instance I, float dt) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L, ME = I.n.ME;
+  const float ia = I.m.invarea[c];
+  float hd = 0.0f;
+  const int n = I.m.neoc[c];
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    hd = hd + (((-I.m.csign[c * ME + s]) * ROW(I.b.split_u, e)[k]) * I.m.dv[e]) * ia;
+  }
+  const float *wa = ROW(I.b.split_w, c);
+  if (k < L) {
+    const float t = ROW(I.b.rho, c)[k] + dt * (hd - I.z.rdzw[k] * (wa[k + 1] - wa[k]));
+    ROW(I.b.target, c)[k] = t;
+    if (!(t > 0.0f)) set_flag(I.b.flag);
+  }
+}
+
+KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const int MA = I.n.MA;
+  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int sp = blockIdx.y;
+  const float *q = qs + sp * cplane;
+  const float *qold = I.b.q + sp * cplane;
+  const float uu = ROW(I.b.split_u, e)[k];
+  const float sv = cs1(uu);
+  const float coef = I.k.coef3;
+  const int na = I.m.nadv[e];
+  float r;
+  if (na == 10) {
+    const int j0 = e * MA;
+    float val = (I.m.adv[j0] + (coef * sv) * I.m.adv3[j0]) * ROW(q, I.m.advc[j0])[k];
+    for (int s = 1; s < 10; ++s) {
+      const int j = e * MA + s;
+      val = val + (I.m.adv[j] + (coef * sv) * I.m.adv3[j]) * ROW(q, I.m.advc[j])[k];
+    }
+    r = uu * val;
+  } else {
+    r = 0.0f;
+    for (int s = 0; s < na; ++s) {
+      const int j = e * MA + s;
+      r = r + (uu * (I.m.adv[j] + (coef * sv) * I.m.adv3[j])) * ROW(q, I.m.advc[j])[k];
+    }
+  }
+  const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
+  const float q1 = ROW(qold, c1)[k] + 0.0f, q2 = ROW(qold, c2)[k] + 0.0f;
+  const float up = (I.m.dv[e] * dt) * (max0(uu) * q1 + min0(uu) * q2);
+  if (k < I.n.L) {
+    ROW(I.b.fct_upw + sp * eplane, e)[k] = up;
+    ROW(I.b.fct_hres + sp * eplane, e)[k] = dt * r - up;
+  }
+}
+
+KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L, ME = I.n.ME;
+  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int sp = blockIdx.y;
+  const float *qsr = ROW(qs + sp * cplane, c);
+  const float *qold = I.b.q + sp * cplane;
+  const float qo = ROW(qold, c)[k] + 0.0f;
+  const float qo_m = up1(qo), qo_p = dn1(qo);
+  // bounds over k-1..k+1 (clamped) in level order, then the neighbours
+  float a, b;
+  if (k == 0) {
+    a = qo; b = qo;
+    a = pmin(a, qo_p); b = pmax(b, qo_p);
+  } else if (k == L - 1) {
+    a = qo_m; b = qo_m;
+    a = pmin(a, qo); b = pmax(b, qo);
+  } else {
+    a = qo_m; b = qo_m;
+    a = pmin(a, qo); b = pmax(b, qo);
+    a = pmin(a, qo_p); b = pmax(b, qo_p);
+  }
+  const int n = I.m.neoc[c];
+  for (int s = 0; s < n; ++s) {
+    const float qn = ROW(qold, I.m.coc[c * ME + s])[k] + 0.0f;
+    a = pmin(a, qn);
+    b = pmax(b, qn);
+  }
+  const float orho = ROW(I.b.rho, c)[k];
+  float mass = qo * orho;
+  const float wk = ROW(I.b.split_w, c)[k];
+  float vres = 0.0f;
+  float fcol = 0.0f;
+  if (k >= 1 && k < L) {
+    vres = dt * vflux(qsr, wk, k, L, I.z.fzm[k], I.z.fzp[k], I.k.coef3);
+    fcol = dt * (max0(wk) * qo_m + min0(wk) * qo);
+  } else {
+    vres = dt * 0.0f;
+  }
+  const float fcol_p = dn1(fcol);
+  const float rdzw = I.z.rdzw[k];
+  if (k <= L - 2) mass = mass - fcol_p * rdzw;
+  if (k >= 1 && k < L) {
+    mass = mass + fcol * rdzw;
+    vres = vres - fcol;
+  }
+  const float vres_p = dn1(vres);
+  float sin_ = (-rdzw) * (min0(vres_p) - max0(vres));
+  float sout = (-rdzw) * (max0(vres_p) - min0(vres));
+  const float ia = I.m.invarea[c];
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    const float so = -I.m.csign[c * ME + s];
+    mass = mass + (so * ROW(I.b.fct_upw + sp * eplane, e)[k]) * ia;
+    const float sr = (-so) * ROW(I.b.fct_hres + sp * eplane, e)[k];
+    sout = sout - max0(sr) * ia;
+    sin_ = sin_ - min0(sr) * ia;
+  }
+  const float target = ROW(I.b.target, c)[k];
+  float f = (b * target - mass) / (sin_ + 1.0e-20f);
+  sin_ = pmin(1.0f, max0(f));
+  f = (a * target - mass) / (sout - 1.0e-20f);
+  sout = pmin(1.0f, max0(f));
+  if (k < L) {
+    ROW(I.b.fct_mass + sp * cplane, c)[k] = mass;
+    ROW(I.b.fct_sin + sp * cplane, c)[k] = sin_;
+    ROW(I.b.fct_sout + sp * cplane, c)[k] = sout;
+  }
+  if (k <= L) ROW(I.b.fct_vres + sp * cplane, c)[k] = vres;
+}
+
+KB k_fct_edge2(const Instance I) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  if (k >= I.n.L) return;
+  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int sp = blockIdx.y;
+  const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
+  const float *si = I.b.fct_sin + sp * cplane, *so = I.b.fct_sout + sp * cplane;
+  float *h = ROW(I.b.fct_hres + sp * eplane, e);
+  const float f = h[k];
+  h[k] = max0(f) * pmin(ROW(so, c1)[k], ROW(si, c2)[k]) + min0(f) * pmin(ROW(si, c1)[k], ROW(so, c2)[k]);
+}
+
+KB k_fct_cell2(const Instance I, float *__restrict__ out) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L, ME = I.n.ME;
+  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int sp = blockIdx.y;
+  const float si = ROW(I.b.fct_sin + sp * cplane, c)[k], so = ROW(I.b.fct_sout + sp * cplane, c)[k];
+  const float si_m = up1(si), so_m = up1(so);
+  float v = ROW(I.b.fct_vres + sp * cplane, c)[k];
+  if (k >= 1 && k < L) v = max0(v) * pmin(so_m, si) + min0(v) * pmin(so, si_m);
+  const float v_p = dn1(v);
+  float mass = ROW(I.b.fct_mass + sp * cplane, c)[k];
+  const float ia = I.m.invarea[c];
+  const int n = I.m.neoc[c];
+  for (int s = 0; s < n; ++s) {
+    const int e = I.m.eoc[c * ME + s];
+    const float sgn = -I.m.csign[c * ME + s];
+    mass = mass + (sgn * ROW(I.b.fct_hres + sp * eplane, e)[k]) * ia;
+  }
+  mass = mass - I.z.rdzw[k] * (v_p - v);
+  if (k < L) ROW(out + sp * cplane, c)[k] = max0(mass / ROW(I.b.target, c)[k]);
+}
+
+KB k_clamp(const Instance I, float *q, int rows) {
+  const int r = wave_entity();
+  if (r >= rows) return;
+  const int k = lane();
+  float *p = ROW(q, r);
+  if (p[k] < 0.0f) p[k] = 0.0f;
+}
+
+template <typename K, typename... A>
+inline void launch_named(const char *name, K kernel, int n, int gy, hipStream_t s, A... args) {
+  if (n <= 0 || gy <= 0) return;
+  if (g_prof_on) prof_mark(name, s, 1);
+  hipLaunchKernelGGL(kernel, dim3(nblocks(n), gy), dim3(256), 0, s, args...);
+  if (g_prof_on) prof_mark(name, s, 0);
+}
+#define launch(kernel, ...) launch_named(#kernel, kernel, __VA_ARGS__)
+
+inline int check() {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : -1;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ phases
+int solve_diagnostics(const Instance &I, int rk, hipStream_t s) {
+  launch(k_diag_vertex, I.n.V, 1, s, I, (const float *)I.b.u);
+  launch(k_diag_cell, I.n.C, 1, s, I, (const float *)I.b.u);
+  launch(k_diag_edge, I.n.E, 1, s, I, (const float *)I.b.u, rk == 3 ? 1 : 0);
+  return check();
+}
+
+int begin_step(const Instance &I, hipStream_t s) {
+  if (I.n.NM > 0) {
+    launch(k_moist_cell, I.n.C, 1, s, I);
+    launch(k_moist_edge, I.n.E, 1, s, I);
+  }
+  return solve_diagnostics(I, 3, s);
+}
+
+int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
+  (void)sub;
+  launch(k_sub_cell, I.n.C, 1, s, I);
+  launch(k_sub_edge, I.n.E, 1, s, I);
+  launch(k_vcoef, I.n.C, 1, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
+         (const float *)I.b.s_rtheta_p);
+  return check();
+}
+
+int dyn_tend(const Instance &I, int stage, hipStream_t s) {
+  if (stage == 2 && I.k.order == 3)
+    launch(k_vcoef, I.n.C, 1, s, I, I.k.dts[1], (const float *)I.b.s_exner, (const float *)I.b.theta,
+           (const float *)I.b.rtheta_p);
+  launch(k_hdiv, I.n.C, 1, s, I);
+  launch(k_tend_edge, I.n.E, 1, s, I, stage);
+  launch(k_tend_cell, I.n.C, 1, s, I, stage);
+  return check();
+}
+
+int acoustic_substep(const Instance &I, int stage, int ss, hipStream_t s) {
+  const float dts = I.k.dts[stage - 1];
+  launch(k_ac_edge, I.n.E, 1, s, I, dts, ss == 1 ? 1 : 0);
+  launch(k_ac_cell, I.n.C, 1, s, I, dts, ss == 1 ? 1 : 0);
+  return check();
+}
+
+int divergence_damping(const Instance &I, int stage, hipStream_t s) {
+  launch(k_divdamp, I.n.E, 1, s, I, I.k.dts_inv_coef[stage - 1]);
+  return check();
+}
+
+int recover(const Instance &I, int stage, int sub, hipStream_t s) {
+  const float inv = I.k.inv_steps[stage - 1];
+  launch(k_rec_cell, I.n.C, 1, s, I, stage, sub, inv);
+  launch(k_rec_edge, I.n.E, 1, s, I, stage, sub, inv);
+  return check();
+}
+
+int scalars_stage(const Instance &I, int stage, hipStream_t s) {
+  const int NT = I.n.NT;
+  const float dt = I.k.sca_dt[stage - 1];
+  if (stage == 1) {
+    launch(k_split_scale, I.n.E, 1, s, I, I.n.E, I.b.split_u, I.k.split_inv);
+    launch(k_split_scale, I.n.C, 1, s, I, I.n.C, I.b.split_w, I.k.split_inv);
+  }
+  if (stage < 3 || !I.k.monotonic) {
+    launch(k_sc_edge, I.n.E, NT, s, I, (const float *)I.b.q_stage, I.b.ev);
+    launch(k_sc_cell, I.n.C, NT, s, I, (const float *)I.b.q_stage, (const float *)I.b.ev, I.b.q_next, dt,
+           I.k.wn[stage - 1]);
+    return check();
+  }
+  launch(k_fct_target, I.n.C, 1, s, I, dt);
+  launch(k_fct_edge1, I.n.E, NT, s, I, (const float *)I.b.q_stage, dt);
+  launch(k_fct_cell1, I.n.C, NT, s, I, (const float *)I.b.q_stage, dt);
+  launch(k_fct_edge2, I.n.E, NT, s, I);
+  launch(k_fct_cell2, I.n.C, NT, s, I, I.b.q_next);
+  return check();
+}
+
+int rebuild_saved(const Instance &I, hipStream_t s) {
+  launch(k_rebuild_cell, I.n.C, 1, s, I);
+  launch(k_rebuild_edge, I.n.E, 1, s, I);
+  return check();
+}
+
+int clamp_scalars(const Instance &I, float *q, hipStream_t s) {
+  launch(k_clamp, I.n.NT * I.n.C, 1, s, I, q, I.n.NT * I.n.C);
+  return check();
+}
+
+}  // namespace HEXDYN_NS
+}  // namespace hexdyn

@@ -1,0 +1,145 @@
+"""Named synthetic workloads: mesh + vertical grid + reference + initial state.
+
+Workloads (sizes, seeds):
+  x1.2562 dry   : icosahedron subdivided 4x, 20 seeded Lloyd iterations, L55 to
+                  30 km, random terrain 0-4 km, JW jet + theta bubble,
+                  dt 1800 s, RK3, 2 acoustic substeps, 3 dynamics subcycles.
+  x1.40962 moist: subdivided 6x, same recipe, qv from RH ~ U(0.3, 0.9), five
+                  log-normal condensates where RH > 0.8, FCT transport, dt 720 s.
+  x1.10242 moist: subdivided 5x, dt 1440 s.
+Seeds: mesh 1, terrain 2, theta perturbation 3, RH 4, condensates 5.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import pickle
+from dataclasses import dataclass, replace
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from .config import DycoreConfig
+from .mesh import Mesh, icosahedral_mesh
+from . import initial as hinit
+from . import setup as hsetup
+
+F32 = np.float32
+
+
+@dataclass
+class Case:
+    name: str
+    mesh: Mesh
+    grid: hinit.VerticalGrid
+    reference: hinit.ReferenceState
+    state: hinit.InitialState
+    config: DycoreConfig
+    n_scalars: int
+    n_moist: int
+    adv_coefs: np.ndarray
+    adv_coefs_3rd: np.ndarray
+    n_adv: np.ndarray
+    adv_cells: np.ndarray
+    inv_area_cell: np.ndarray
+    inv_area_triangle: np.ndarray
+    inv_dc_edge: np.ndarray
+    inv_dv_edge: np.ndarray
+    cos_angle_edge: np.ndarray
+    sin_angle_edge: np.ndarray
+    etp: np.ndarray
+    etm: np.ndarray
+    ewp: np.ndarray
+    ewm: np.ndarray
+    dss: np.ndarray
+    u_init: np.ndarray
+    v_init: np.ndarray
+    len_disp: float
+
+    @property
+    def n_levels(self) -> int:
+        return int(self.grid.zz.shape[0])
+
+
+WORKLOADS = {
+    "x1.2562-dry": dict(level=4, moist=False, n_scalars=0, dt=1800.0),
+    "x1.2562-moist": dict(level=4, moist=True, n_scalars=6, dt=1800.0),
+    "x1.10242-moist": dict(level=5, moist=True, n_scalars=6, dt=1440.0),
+    "x1.40962-moist": dict(level=6, moist=True, n_scalars=6, dt=720.0),
+    "x1.40962-dry": dict(level=6, moist=False, n_scalars=0, dt=720.0),
+    "x1.642-dry": dict(level=3, moist=False, n_scalars=0, dt=3600.0),
+    "x1.642-moist": dict(level=3, moist=True, n_scalars=6, dt=3600.0),
+}
+
+
+def _cache_dir() -> Path:
+    root = os.environ.get("HEXDYN_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "hexdyn")
+    p = Path(root)
+    try:
+        p.mkdir(parents=True, exist_ok=True)
+    except OSError:
+        import tempfile
+
+        p = Path(tempfile.gettempdir()) / "hexdyn-cache"
+        p.mkdir(parents=True, exist_ok=True)
+    return p
+
+
+def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[DycoreConfig] = None,
+               use_cache: bool = True, **overrides) -> Case:
+    spec = dict(WORKLOADS[name])
+    spec.update(overrides)
+    cfg = config or DycoreConfig(config_dt=spec["dt"])
+    key = hashlib.sha256(repr((name, nlev, sorted(spec.items()), 3)).encode()).hexdigest()[:16]
+    path = _cache_dir() / f"case-{name}-{key}.pkl"
+    if use_cache and path.exists():
+        with open(path, "rb") as f:
+            case = pickle.load(f)
+        return replace(case, config=cfg, len_disp=_len_disp(case.mesh, cfg))
+    mesh = icosahedral_mesh(spec["level"], seed=1, lloyd_iterations=20)
+    deriv = hsetup.deriv_two(mesh)
+    terrain = hinit.random_terrain(mesh, seed=2, hmax=spec.get("hmax", 4000.0)) * spec.get("terrain_scale", 1.0)
+    grid = hinit.build_vertical_grid(mesh, terrain, nlev=nlev, coef_3rd_order=cfg.config_coef_3rd_order, deriv=deriv)
+    reference, state = hinit.build_reference_and_state(mesh, grid, moist=spec["moist"], n_scalars=spec["n_scalars"],
+                                                       theta_amplitude=spec.get("theta_amplitude", 0.5))
+    adv, adv3, nadv, advc = hsetup.advection_coefficients(mesh, deriv)
+    etp, etm, ewp, ewm = hsetup.acoustic_offcentering(
+        grid.rdzw, minimum=cfg.config_epssm_minimum, maximum=cfg.config_epssm_maximum,
+        transition_bottom_z=cfg.config_epssm_transition_bottom_z,
+        transition_top_z=cfg.config_epssm_transition_top_z)
+    dss = hsetup.vertical_velocity_damping(grid.zgrid, xnutr=cfg.config_xnutr, zd=cfg.config_zd)
+    angle = mesh.angleEdge.astype(F32)
+    case = Case(
+        name=name, mesh=mesh, grid=grid, reference=reference, state=state, config=cfg,
+        n_scalars=spec["n_scalars"], n_moist=spec["n_scalars"] if spec["moist"] else 0,
+        adv_coefs=adv, adv_coefs_3rd=adv3, n_adv=nadv, adv_cells=advc,
+        inv_area_cell=hsetup.stored_inverse(mesh.areaCell),
+        inv_area_triangle=hsetup.stored_inverse(mesh.areaTriangle),
+        inv_dc_edge=hsetup.stored_inverse(mesh.dcEdge),
+        inv_dv_edge=hsetup.stored_inverse(mesh.dvEdge),
+        cos_angle_edge=np.cos(angle.astype(np.float64)).astype(F32),
+        sin_angle_edge=np.sin(angle.astype(np.float64)).astype(F32),
+        etp=etp, etm=etm, ewp=ewp, ewm=ewm, dss=dss,
+        u_init=np.zeros(nlev, dtype=F32), v_init=np.zeros(nlev, dtype=F32),
+        len_disp=_len_disp(mesh, cfg),
+    )
+    if use_cache:
+        tmp = path.with_suffix(".tmp")
+        with open(tmp, "wb") as f:
+            pickle.dump(case, f, protocol=pickle.HIGHEST_PROTOCOL)
+        os.replace(tmp, path)
+    return case
+
+
+def _len_disp(mesh: Mesh, cfg: DycoreConfig) -> float:
+    if cfg.config_len_disp > 0.0:
+        return float(F32(cfg.config_len_disp))
+    return float(F32(mesh.arrays["nominalMinDc"][0]))
+
+
+def initial_state_arrays(case: Case) -> dict:
+    """Prognostic initial fields (binary32, logical layout)."""
+    s = case.state
+    return {"rho_zz": s.rho_zz, "rtheta_m": s.rtheta_m, "ru": s.ru, "rw": s.rw, "scalars": s.scalars}

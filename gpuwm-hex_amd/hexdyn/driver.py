@@ -1,0 +1,203 @@
+"""Host-side dycore driver over libhexdyn.so, shaped like the reference's
+``CudaDryDycoreDriver`` (``cuda_driver.py:2658``):
+
+* ``HexDycoreDriver.from_host(case, state=None, saved=None)`` uploads the
+  mesh, vertical grid, reference state and initial state once
+  (``from_host`` at ``cuda_driver.py:2826-3025``);
+* ``step_device()`` advances one large step on the device and returns a
+  receipt (``step_device`` at ``:5751``); a set validation flag raises
+  ``FloatingPointError`` and nothing is published (``:5390-5403``);
+* ``step()`` also downloads the state (``:6112-6149``).
+
+Fields live in PyTorch-ROCm tensors in MPAS logical layout ([L][entity]);
+the library keeps its own level-fastest padded layout internally.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import time
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import capi
+from .config import ConfigurationRefusal
+from .lib import lib
+
+STATE_FIELDS = capi.STATE_FIELDS
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+@dataclass
+class StepReceipt:
+    start_time_seconds: float
+    end_time_seconds: float
+    wall_seconds: float
+    stage_acoustic_steps: tuple
+    dynamics_split_steps: int
+
+
+@dataclass
+class HostStepResult:
+    state: Dict[str, np.ndarray]
+    receipt: StepReceipt
+
+
+def _check(h, rc: int, what: str) -> None:
+    if rc == 0:
+        return
+    buf = ct.create_string_buffer(512)
+    if h:
+        lib().hexdyn_last_error(h, buf, 512)
+    msg = f"{what} failed (rc={rc}): {buf.value.decode(errors='replace')}"
+    if rc == capi.HEXDYN_E_NONFINITE:
+        raise FloatingPointError(msg)
+    if rc == capi.HEXDYN_E_REFUSED:
+        raise ConfigurationRefusal(what, rc, msg, "a supported configuration")
+    raise RuntimeError(msg)
+
+
+class HexDycoreDriver:
+    """One dycore instance on the current HIP device."""
+
+    def __init__(self, case, handle: int, bundle) -> None:
+        self.case = case
+        self._h = ct.c_void_p(handle)
+        self._bundle = bundle
+        self.shapes = self._state_shapes(case)
+
+    @staticmethod
+    def _state_shapes(case) -> Dict[str, tuple]:
+        L, C, E = case.n_levels, case.mesh.n_cells, case.mesh.n_edges
+        return {"rho_zz": (L, C), "rtheta_m": (L, C), "ru": (L, E), "rw": (L + 1, C),
+                "scalars": (case.n_scalars, L, C), "theta_m": (L, C), "exner": (L, C), "rho_p": (L, C),
+                "rtheta_p": (L, C), "pressure_p": (L, C), "u": (L, E), "w": (L + 1, C)}
+
+    @classmethod
+    def from_host(cls, case, state: Optional[Dict[str, np.ndarray]] = None,
+                  saved: Optional[Dict[str, np.ndarray]] = None) -> "HexDycoreDriver":
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("HexDycoreDriver needs a HIP device")
+        mesh_s, vert_s, cfg_s, bundle = capi.build_structs(case)
+        h = ct.c_void_p()
+        rc = lib().hexdyn_create(ct.byref(mesh_s), ct.byref(vert_s), ct.byref(cfg_s), ct.byref(h))
+        if rc != 0:
+            raise ConfigurationRefusal("hexdyn_create", rc, "the library refused the mesh or configuration",
+                                       "a supported configuration")
+        drv = cls(case, h.value, bundle)
+        if state is None:
+            s = case.state
+            state = {"rho_zz": s.rho_zz, "rtheta_m": s.rtheta_m, "ru": s.ru, "rw": s.rw, "scalars": s.scalars}
+        drv.set_state(state, saved)
+        return drv
+
+    def close(self) -> None:
+        if self._h:
+            lib().hexdyn_destroy(self._h)
+            self._h = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> ct.c_void_p:
+        return self._h
+
+    @staticmethod
+    def stream_ptr() -> int:
+        torch = _torch()
+        return torch.cuda.current_stream().cuda_stream
+
+    def _view(self, tensors: Dict[str, object]) -> capi.HexdynStateView:
+        return capi.state_view_from_ptrs({k: (t.data_ptr() if t is not None else None) for k, t in tensors.items()})
+
+    def set_state(self, state: Dict[str, np.ndarray], saved: Optional[Dict[str, np.ndarray]] = None) -> None:
+        torch = _torch()
+        tens = {}
+        for name in ("rho_zz", "rtheta_m", "ru", "rw", "scalars"):
+            a = np.ascontiguousarray(state[name], dtype=np.float32)
+            if a.shape != self.shapes[name]:
+                raise ValueError(f"{name} shape {a.shape} != {self.shapes[name]}")
+            if not np.isfinite(a).all():
+                raise FloatingPointError(f"{name} contains non-finite values")
+            tens[name] = torch.from_numpy(a).to("cuda") if a.size else None
+        if np.any(np.asarray(state["rho_zz"]) <= 0.0):
+            raise ValueError("rho must be strictly positive")
+        rebuild = 1
+        if saved is not None:
+            rebuild = 0
+            for name in ("theta_m", "exner", "rho_p", "rtheta_p", "pressure_p", "u", "w"):
+                tens[name] = torch.from_numpy(np.ascontiguousarray(saved[name], dtype=np.float32)).to("cuda")
+        view = self._view(tens)
+        torch.cuda.synchronize()
+        _check(self._h, lib().hexdyn_set_state(self._h, ct.byref(view), rebuild, ct.c_void_p(self.stream_ptr())),
+               "hexdyn_set_state")
+
+    def state_tensors(self) -> Dict[str, object]:
+        torch = _torch()
+        out = {name: torch.empty(shape, dtype=torch.float32, device="cuda") for name, shape in self.shapes.items()}
+        view = self._view({k: (v if v.numel() else None) for k, v in out.items()})
+        _check(self._h, lib().hexdyn_get_state(self._h, ct.byref(view), ct.c_void_p(self.stream_ptr())),
+               "hexdyn_get_state")
+        return out
+
+    def get_state(self) -> Dict[str, np.ndarray]:
+        return {k: v.cpu().numpy() for k, v in self.state_tensors().items()}
+
+    def export_field(self, name: str):
+        torch = _torch()
+        fid = capi.FIELD_ID[name]
+        rows, cols = ct.c_int64(), ct.c_int64()
+        _check(self._h, lib().hexdyn_field_shape(self._h, fid, ct.byref(rows), ct.byref(cols)), "hexdyn_field_shape")
+        out = torch.empty((rows.value, cols.value), dtype=torch.float32, device="cuda")
+        _check(self._h, lib().hexdyn_export_field(self._h, fid, ct.c_void_p(out.data_ptr()),
+                                                  ct.c_void_p(self.stream_ptr())), "hexdyn_export_field")
+        return out
+
+    def step_device(self, n_steps: int = 1) -> StepReceipt:
+        t0 = lib().hexdyn_model_time(self._h)
+        w0 = time.perf_counter()
+        _check(self._h, lib().hexdyn_step(self._h, n_steps, ct.c_void_p(self.stream_ptr())), "hexdyn_step")
+        from .setup import rk_schedule
+
+        c = self.case.config
+        sched = rk_schedule(c.config_dt, order=c.config_time_integration_order,
+                            acoustic_substeps=c.config_number_of_sub_steps,
+                            dynamics_splits=c.config_dynamics_split_steps)
+        return StepReceipt(t0, lib().hexdyn_model_time(self._h), time.perf_counter() - w0,
+                           tuple(s.acoustic_steps for s in sched), c.config_dynamics_split_steps)
+
+    def step(self, n_steps: int = 1) -> HostStepResult:
+        receipt = self.step_device(n_steps)
+        return HostStepResult(self.get_state(), receipt)
+
+    # phase entry points (parity tests walk a step phase by phase)
+    def phase(self, name: str, *args: int) -> None:
+        fn = getattr(lib(), f"hexdyn_{name}")
+        _check(self._h, fn(self._h, *[ct.c_int(a) for a in args], ct.c_void_p(self.stream_ptr())), f"hexdyn_{name}")
+
+    def profile(self, enable: bool) -> None:
+        lib().hexdyn_profile_enable(self._h, 1 if enable else 0)
+
+    def profile_read(self, kernel: str):
+        avg = ct.c_double()
+        n = ct.c_int64()
+        lib().hexdyn_profile_read(self._h, kernel.encode(), ct.byref(avg), ct.byref(n))
+        return avg.value, n.value
+
+    def profile_kernels(self):
+        buf = ct.create_string_buffer(8192)
+        lib().hexdyn_profile_kernels(self._h, buf, 8192)
+        names = buf.value.decode()
+        return [n for n in names.split(",") if n]

@@ -1,0 +1,206 @@
+/*
+ * hexdyn — MPAS-A v8.4.2 split-explicit nonhydrostatic dycore large step on
+ * AMD MI355X (gfx950).  C ABI of libhexdyn.so.
+ *
+ * The library replaces the reference's dycore step object
+ * (tree/src/hexcore/cuda_driver.py CudaDryDycoreDriver):
+ *   hexdyn_create            <- CudaDryDycoreDriver.from_host   (cuda_driver.py:2826-3025)
+ *   hexdyn_set_state         <- DeviceAtmosphere.from_host      (cuda_backend/containers.py:1108)
+ *   hexdyn_get_state         <- step() download half            (cuda_driver.py:6112-6149)
+ *   hexdyn_step              <- _step_device_v841 / step_device (cuda_driver.py:4854-5537, :5751)
+ *   hexdyn_begin_step        <- moist coefficients + first solve_diagnostics (cuda_driver.py:5049-5070, :4232-4245)
+ *   hexdyn_begin_subcycle    <- subcycle-start pools            (cuda_driver.py:4232-4383)
+ *   hexdyn_dyn_tend          <- stage tendencies                (cuda_driver.py:4388-4509)
+ *   hexdyn_acoustic_substep  <- advance_acoustic_step_cuda_v841 (cuda_acoustic_v841.py:801-1040)
+ *   hexdyn_divergence_damping<- divergence_damping_3d           (cuda_horizontal.py:1584-1660)
+ *   hexdyn_recover           <- _recover_candidate + validation (cuda_driver.py:3808-3960, :4692-4718)
+ *   hexdyn_solve_diagnostics <- CudaHorizontalV841.solve_diagnostics (cuda_horizontal_v841.py:1041-1236)
+ *   hexdyn_end_subcycle      <- accumulate_split_flux_cuda_v841 (cuda_dynamics_v841.py:412-442)
+ *   hexdyn_scalars           <- advance_scalar_transport_cuda_v841 (cuda_transport_v841.py:1988)
+ *   hexdyn_end_step          <- finish_split_flux / clamp / validation read / publish (cuda_driver.py:5147-5403)
+ *
+ * Conventions.  Every array crossing this ABI is in MPAS logical order,
+ * level first and entity fastest ([L][entity], [L+1][C] for interface fields,
+ * [T][L][C] for scalars), FP32, 0-based int32 connectivity with -1 padding.
+ * Static mesh/vertical/reference data are host pointers copied at create
+ * time.  State views are device pointers.  Internally the library renumbers
+ * entities along a space-filling curve and stores every field entity-major
+ * with the level index fastest, each entity row padded to 64 floats.
+ *
+ * Return codes: 0 ok, >0 refusal or numerical flag, <0 HIP error.  A set
+ * validation flag makes hexdyn_step/hexdyn_end_step return
+ * HEXDYN_E_NONFINITE and leaves the published state untouched.
+ * Calls on one handle must not overlap; separate handles are independent.
+ */
+#ifndef HEXDYN_H
+#define HEXDYN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HEXDYN_ABI_VERSION 1
+#define HEXDYN_MAX_EDGES 7      /* nEdgesOnCell limit (stencil widths derive from it) */
+#define HEXDYN_ROW 64           /* padded level row; nVertLevels + 1 <= 64 */
+
+enum {
+  HEXDYN_OK = 0,
+  HEXDYN_E_INVALID = 1,       /* malformed argument */
+  HEXDYN_E_NONFINITE = 2,     /* validation flag: non-finite or non-positive state */
+  HEXDYN_E_REFUSED = 3,       /* configuration refusal */
+  HEXDYN_E_HIP = -1
+};
+
+enum { HEXDYN_NUMERICS_STRICT = 0, HEXDYN_NUMERICS_FAST = 1 };
+
+typedef struct hexdyn_mesh {
+  int32_t n_cells, n_edges, n_vertices;
+  int32_t max_edges;       /* row width of edgesOnCell/cellsOnCell/verticesOnCell */
+  int32_t vertex_degree;   /* must be 3 */
+  int32_t max_edges2;      /* row width of edgesOnEdge/weightsOnEdge */
+  int32_t max_adv;         /* row width of advCellsForEdge/adv_coefs */
+  const int32_t *cells_on_edge;        /* [E][2]  normal points cell 0 -> cell 1 */
+  const int32_t *vertices_on_edge;     /* [E][2]  vertex 1 on the left of the normal */
+  const int32_t *edges_on_cell;        /* [C][max_edges] counter-clockwise */
+  const int32_t *cells_on_cell;        /* [C][max_edges] */
+  const int32_t *vertices_on_cell;     /* [C][max_edges] vertex i between edges i and i+1 */
+  const int32_t *n_edges_on_cell;      /* [C] */
+  const int32_t *kite_for_cell;        /* [C][max_edges] slot of the cell in cellsOnVertex */
+  const int32_t *edges_on_vertex;      /* [V][3] */
+  const int32_t *cells_on_vertex;      /* [V][3] */
+  const int32_t *edges_on_edge;        /* [E][max_edges2] */
+  const int32_t *n_edges_on_edge;      /* [E] */
+  const float *weights_on_edge;        /* [E][max_edges2] */
+  const float *dc_edge, *dv_edge;      /* [E] */
+  const float *inv_dc_edge, *inv_dv_edge;          /* [E] stored inverses */
+  const float *area_cell, *inv_area_cell;          /* [C] */
+  const float *area_triangle, *inv_area_triangle;  /* [V] */
+  const float *kite_areas_on_vertex;   /* [V][3] */
+  const float *f_edge, *f_vertex;      /* [E], [V] */
+  const float *cos_angle_edge, *sin_angle_edge;    /* [E] */
+  const int32_t *n_adv_cells_for_edge; /* [E] */
+  const int32_t *adv_cells_for_edge;   /* [E][max_adv] */
+  const float *adv_coefs, *adv_coefs_3rd;          /* [E][max_adv] */
+  const double *x_cell, *y_cell, *z_cell;          /* [C] positions for locality ordering (may be NULL) */
+} hexdyn_mesh;
+
+typedef struct hexdyn_vertical {
+  int32_t n_levels;                    /* L, with L + 1 <= HEXDYN_ROW */
+  const float *zz;                     /* [L][C] */
+  const float *zxu;                    /* [L][E] */
+  const float *rdzw, *rdzu, *fzm, *fzp;/* [L] */
+  float cf1, cf2, cf3;
+  const float *zb_cell, *zb3_cell;     /* [L+1][C][max_edges] */
+  const float *dss;                    /* [L][C] Rayleigh w damping */
+  const float *etp, *etm;              /* [L] off-centring */
+  const float *ewp, *ewm;              /* [L+1] */
+  const float *u_init, *v_init;        /* [L] reference winds */
+  const float *rho_base, *rtheta_base, *exner_base; /* [L][C] */
+} hexdyn_vertical;
+
+typedef struct hexdyn_config {
+  double config_dt;
+  int32_t config_time_integration_order;   /* 2 or 3 */
+  int32_t config_number_of_sub_steps;      /* >= 1 */
+  int32_t config_dynamics_split_steps;     /* 1 or 3 (3 needs split transport) */
+  int32_t config_split_dynamics_transport; /* bool */
+  int32_t config_scalar_advection;         /* bool */
+  int32_t config_monotonic;                /* bool */
+  int32_t config_divergence_damping;       /* bool */
+  float config_coef_3rd_order;
+  float config_apvm_upwinding;             /* >= 0 */
+  float config_smdiv;                      /* >= 0 */
+  float config_len_disp;                   /* > 0 (resolved nominal dc) */
+  int32_t n_scalars;                       /* T */
+  int32_t n_moist;                         /* leading mass-loading species, 0 = dry */
+  int32_t numerics;                        /* HEXDYN_NUMERICS_* */
+} hexdyn_config;
+
+/* Prognostic state and the carried time-level-one diagnostics
+ * (state.py:14-27 PrognosticState; driver.py:661 DrySavedDiagnostics). */
+typedef struct hexdyn_state_view {
+  float *rho_zz;     /* [L][C] */
+  float *rtheta_m;   /* [L][C] */
+  float *ru;         /* [L][E] */
+  float *rw;         /* [L+1][C] */
+  float *scalars;    /* [T][L][C] */
+  float *theta_m;    /* [L][C] */
+  float *exner;      /* [L][C] */
+  float *rho_p;      /* [L][C] density perturbation */
+  float *rtheta_p;   /* [L][C] */
+  float *pressure_p; /* [L][C] */
+  float *u;          /* [L][E] normal velocity */
+  float *w;          /* [L+1][C] vertical velocity */
+} hexdyn_state_view;
+
+/* Internal fields reachable through hexdyn_export_field/hexdyn_import_field
+ * (logical layout, device pointers).  Used by parity tests to compare every
+ * phase against the CPU restatement. */
+enum {
+  HEXDYN_F_RHO_ZZ = 0, HEXDYN_F_RTHETA_M, HEXDYN_F_RU, HEXDYN_F_RW, HEXDYN_F_SCALARS,
+  HEXDYN_F_THETA_M, HEXDYN_F_EXNER, HEXDYN_F_RHO_P, HEXDYN_F_RTHETA_P, HEXDYN_F_PRESSURE_P,
+  HEXDYN_F_U, HEXDYN_F_W,
+  HEXDYN_F_V, HEXDYN_F_PV_EDGE, HEXDYN_F_KE, HEXDYN_F_DIVERGENCE, HEXDYN_F_VORTICITY,
+  HEXDYN_F_TEND_RU, HEXDYN_F_TEND_RT, HEXDYN_F_TEND_RW, HEXDYN_F_TEND_RHO,
+  HEXDYN_F_RU_P, HEXDYN_F_RW_P, HEXDYN_F_RHO_PP, HEXDYN_F_RTHETA_PP, HEXDYN_F_RU_AVG, HEXDYN_F_WW_AVG,
+  HEXDYN_F_A_TRI, HEXDYN_F_ALPHA_TRI, HEXDYN_F_GAMMA_TRI, HEXDYN_F_COFWZ, HEXDYN_F_COFWT, HEXDYN_F_COFTZ,
+  HEXDYN_F_EULER_RU, HEXDYN_F_EULER_RW,
+  HEXDYN_F_FLUX_U, HEXDYN_F_FLUX_W, HEXDYN_F_SCALARS_STAGE,
+  HEXDYN_F_QTOT, HEXDYN_F_CQW, HEXDYN_F_CQU,
+  HEXDYN_F_COUNT
+};
+
+typedef struct hexdyn_handle hexdyn_handle;
+
+int hexdyn_abi_version(void);
+int hexdyn_create(const hexdyn_mesh *mesh, const hexdyn_vertical *vertical,
+                  const hexdyn_config *config, hexdyn_handle **out);
+void hexdyn_destroy(hexdyn_handle *h);
+int hexdyn_last_error(const hexdyn_handle *h, char *buf, size_t len);
+
+/* State transfer (device pointers, logical layout).  With rebuild != 0 the
+ * saved diagnostics are recomputed from the prognostic fields
+ * (driver.py:2655 _rebuild_saved_diagnostics) and the view's diagnostic
+ * pointers are ignored. */
+int hexdyn_set_state(hexdyn_handle *h, const hexdyn_state_view *state, int rebuild, void *stream);
+int hexdyn_get_state(hexdyn_handle *h, const hexdyn_state_view *state, void *stream);
+int hexdyn_export_field(hexdyn_handle *h, int field, float *dst, void *stream);
+int hexdyn_import_field(hexdyn_handle *h, int field, const float *src, void *stream);
+int hexdyn_field_shape(const hexdyn_handle *h, int field, int64_t *rows, int64_t *cols);
+
+/* Whole large steps.  Work is enqueued on `stream` (a hipStream_t, NULL for
+ * the default stream); the call synchronises once per step to read the
+ * 4-byte validation flag. */
+int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream);
+double hexdyn_model_time(const hexdyn_handle *h);
+
+/* Phases of one large step, in the order hexdyn_step runs them. */
+int hexdyn_begin_step(hexdyn_handle *h, void *stream);
+int hexdyn_begin_subcycle(hexdyn_handle *h, int subcycle, void *stream);
+int hexdyn_dyn_tend(hexdyn_handle *h, int stage, void *stream);
+int hexdyn_acoustic_substep(hexdyn_handle *h, int stage, int small_step, void *stream);
+int hexdyn_divergence_damping(hexdyn_handle *h, int stage, void *stream);
+int hexdyn_recover(hexdyn_handle *h, int stage, void *stream);
+int hexdyn_solve_diagnostics(hexdyn_handle *h, int rk_step, void *stream);
+int hexdyn_end_subcycle(hexdyn_handle *h, int subcycle, void *stream);
+int hexdyn_scalars(hexdyn_handle *h, int stage, void *stream);
+int hexdyn_end_step(hexdyn_handle *h, void *stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * every kernel while enabled; read returns the average duration of one
+ * kernel name and its launch count; kernels lists the names seen. */
+int hexdyn_profile_enable(hexdyn_handle *h, int enable);
+int hexdyn_profile_read(hexdyn_handle *h, const char *kernel, double *avg_ms, int64_t *launches);
+int hexdyn_profile_kernels(hexdyn_handle *h, char *buf, size_t len);
+
+/* The exner power function used by recovery, exposed for its accuracy test. */
+float hexdyn_powf(float x, float y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEXDYN_H */

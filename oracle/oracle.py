@@ -1,0 +1,144 @@
+"""TEST INFRASTRUCTURE — Python binding of the CPU oracle (liboracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker.  The product (libhexdyn.so and hexdyn/) never imports
+it.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+import sys
+from pathlib import Path
+from typing import Dict
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent
+sys.path.insert(0, str(ROOT / "gpuwm-hex_amd"))
+
+from hexdyn import capi  # noqa: E402
+
+LIB = HERE / "liboracle.so"
+
+
+def build(force: bool = False) -> Path:
+    if force or not LIB.exists():
+        subprocess.run(["make", "-C", str(HERE), "-s"], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib() -> ct.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = ct.CDLL(str(LIB))
+        _lib.oracle_create.restype = ct.c_void_p
+        _lib.oracle_create.argtypes = [ct.POINTER(capi.HexdynMesh), ct.POINTER(capi.HexdynVertical),
+                                       ct.POINTER(capi.HexdynConfig), ct.c_int]
+        _lib.oracle_destroy.argtypes = [ct.c_void_p]
+        for name in ("oracle_set_state",):
+            getattr(_lib, name).argtypes = [ct.c_void_p, ct.POINTER(capi.HexdynStateView), ct.c_int]
+        _lib.oracle_get_state.argtypes = [ct.c_void_p, ct.POINTER(capi.HexdynStateView)]
+        _lib.oracle_export_field.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int64]
+        _lib.oracle_import_field.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int64]
+        _lib.oracle_field_size.argtypes = [ct.c_void_p, ct.c_int]
+        _lib.oracle_field_size.restype = ct.c_int64
+        _lib.oracle_step.argtypes = [ct.c_void_p, ct.c_int]
+        for name in ("oracle_begin_step", "oracle_end_step"):
+            getattr(_lib, name).argtypes = [ct.c_void_p]
+        for name in ("oracle_begin_subcycle", "oracle_dyn_tend", "oracle_divergence_damping", "oracle_recover",
+                     "oracle_solve_diagnostics", "oracle_end_subcycle", "oracle_scalars"):
+            getattr(_lib, name).argtypes = [ct.c_void_p, ct.c_int]
+        _lib.oracle_acoustic_substep.argtypes = [ct.c_void_p, ct.c_int, ct.c_int]
+        _lib.oracle_powf.argtypes = [ct.c_float, ct.c_float]
+        _lib.oracle_powf.restype = ct.c_float
+        _lib.oracle_model_time.argtypes = [ct.c_void_p]
+        _lib.oracle_model_time.restype = ct.c_double
+    return _lib
+
+
+def state_shapes(case) -> Dict[str, tuple]:
+    L, C, E = case.n_levels, case.mesh.n_cells, case.mesh.n_edges
+    cell = (L, C)
+    return {"rho_zz": cell, "rtheta_m": cell, "ru": (L, E), "rw": (L + 1, C),
+            "scalars": (case.n_scalars, L, C), "theta_m": cell, "exner": cell, "rho_p": cell,
+            "rtheta_p": cell, "pressure_p": cell, "u": (L, E), "w": (L + 1, C)}
+
+
+class Oracle:
+    """CPU restatement of the dycore step; precision 32 (strict) or 64."""
+
+    def __init__(self, case, precision: int = 32) -> None:
+        self.case = case
+        self.mesh_s, self.vert_s, self.cfg_s, self._bundle = capi.build_structs(case)
+        self.h = lib().oracle_create(ct.byref(self.mesh_s), ct.byref(self.vert_s), ct.byref(self.cfg_s), precision)
+        self.shapes = state_shapes(case)
+
+    def __del__(self) -> None:
+        if getattr(self, "h", None):
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def set_state(self, arrays: Dict[str, np.ndarray], rebuild: bool = True) -> None:
+        keep = {}
+        view = capi.HexdynStateView()
+        for name in capi.STATE_FIELDS:
+            if name in arrays and arrays[name] is not None:
+                a = np.ascontiguousarray(arrays[name], dtype=np.float32)
+                keep[name] = a
+                setattr(view, name, a.ctypes.data)
+        lib().oracle_set_state(self.h, ct.byref(view), 1 if rebuild else 0)
+
+    def get_state(self) -> Dict[str, np.ndarray]:
+        out = {name: np.empty(shape, dtype=np.float32) for name, shape in self.shapes.items()}
+        view = capi.HexdynStateView()
+        for name, a in out.items():
+            setattr(view, name, a.ctypes.data)
+        lib().oracle_get_state(self.h, ct.byref(view))
+        return out
+
+    def export_field(self, name: str) -> np.ndarray:
+        fid = capi.FIELD_ID[name]
+        n = lib().oracle_field_size(self.h, fid)
+        out = np.empty(n, dtype=np.float32)
+        rc = lib().oracle_export_field(self.h, fid, out.ctypes.data, n)
+        if rc:
+            raise KeyError(name)
+        return out
+
+    def import_field(self, name: str, values: np.ndarray) -> None:
+        fid = capi.FIELD_ID[name]
+        a = np.ascontiguousarray(values, dtype=np.float32).ravel()
+        rc = lib().oracle_import_field(self.h, fid, a.ctypes.data, a.size)
+        if rc:
+            raise KeyError(name)
+
+    def step(self, n: int = 1) -> int:
+        return lib().oracle_step(self.h, n)
+
+    def model_time(self) -> float:
+        return lib().oracle_model_time(self.h)
+
+    # phases (same names and order as the product's C ABI)
+    def begin_step(self): return lib().oracle_begin_step(self.h)
+    def begin_subcycle(self, sub): return lib().oracle_begin_subcycle(self.h, sub)
+    def dyn_tend(self, stage): return lib().oracle_dyn_tend(self.h, stage)
+    def acoustic_substep(self, stage, ss): return lib().oracle_acoustic_substep(self.h, stage, ss)
+    def divergence_damping(self, stage): return lib().oracle_divergence_damping(self.h, stage)
+    def recover(self, stage): return lib().oracle_recover(self.h, stage)
+    def solve_diagnostics(self, rk): return lib().oracle_solve_diagnostics(self.h, rk)
+    def end_subcycle(self, sub): return lib().oracle_end_subcycle(self.h, sub)
+    def scalars(self, stage): return lib().oracle_scalars(self.h, stage)
+    def end_step(self): return lib().oracle_end_step(self.h)
+
+
+def powf(x: float, y: float) -> float:
+    return float(lib().oracle_powf(x, y))
