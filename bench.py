@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Headline benchmark: simulated days per wall-clock day of the plain dycore
+on x1.40962 L55 moist (T=6, FCT transport), dt 720 s, one MI355X.
+
+A step is one large step (3 dynamics subcycles x RK3 with acoustic substeps,
+then split FCT transport) of the whole mesh with the state resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints one JSON line.  `roofline` reports the dominant kernel (largest total
+time) with its algorithmic bytes per launch over its average launch time,
+timed with HIP events on the launch stream; `step_roofline` reports the whole
+step's algorithmic bytes (phase-level model) over the step time;
+`cpu_baseline` times the CPU restatement of the same step (binary32 strict,
+OpenMP over all host cores) on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "gpuwm-hex_amd"))
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=120)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--workload", default="x1.40962-moist")
+    p.add_argument("--numerics", default="strict", choices=["strict", "fast"])
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-steps", type=int, default=0, help="CPU sample size in large steps (0: auto ~15 s)")
+    p.add_argument("--profile-steps", type=int, default=2)
+    return p.parse_args()
+
+
+def cpu_baseline(case, cpu_steps: int):
+    """Time the CPU restatement (binary32 strict) on a bounded sample."""
+    sys.path.insert(0, str(ROOT))
+    from oracle.oracle import Oracle
+    from hexdyn.case import initial_state_arrays
+
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    t0 = time.perf_counter()
+    o.step(1)
+    first = time.perf_counter() - t0
+    n = cpu_steps if cpu_steps > 0 else max(1, min(8, int(15.0 / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    o.step(n)
+    wall = time.perf_counter() - t0
+    sim = n * case.config.config_dt
+    return {
+        "value": sim / wall,
+        "unit": "sim-days/wall-day",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{n} large steps of {case.name} (after 1 untimed), {wall:.1f} s wall, "
+                  f"binary32 strict CPU restatement, OpenMP",
+        "seconds_per_step": wall / n,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU only; "
+                                   "multi-GPU sharding is designed but not built"}))
+        sys.exit(2)
+    import torch
+
+    from hexdyn.bytes_model import HBM_PEAK_GBS, kernel_bytes, step_bytes
+    from hexdyn.case import build_case
+    from hexdyn.config import DycoreConfig
+    from hexdyn.driver import HexDycoreDriver
+    from hexdyn.case import WORKLOADS
+
+    spec = WORKLOADS[args.workload]
+    cfg = DycoreConfig(config_dt=spec["dt"], numerics=args.numerics)
+    t_setup = time.perf_counter()
+    case = build_case(args.workload, config=cfg)
+    drv = HexDycoreDriver.from_host(case)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+    print(f"# setup {setup_s:.1f}s: C={case.mesh.n_cells} E={case.mesh.n_edges} L={case.n_levels} "
+          f"T={case.n_scalars}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        drv.step_device(1)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    drv.step_device(args.steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = 1000.0 * wall / args.steps
+    sim_seconds = args.steps * case.config.config_dt
+    value = sim_seconds / wall
+    print(f"# timed {args.steps} steps: {wall:.3f}s wall ({gpu_ms:.1f} ms events), {ms_per_step:.3f} ms/step",
+          file=sys.stderr, flush=True)
+
+    # per-kernel timing (events on the launch stream around every kernel)
+    drv.profile(True)
+    drv.step_device(args.profile_steps)
+    torch.cuda.synchronize()
+    C, E, V, L, T = case.mesh.n_cells, case.mesh.n_edges, case.mesh.n_vertices, case.n_levels, case.n_scalars
+    kernels = []
+    for name in drv.profile_kernels():
+        avg_ms, launches = drv.profile_read(name)
+        try:
+            kb = kernel_bytes(name, C=C, E=E, V=V, L=L, T=T, max_edges=case.mesh.max_edges)
+        except KeyError:
+            kb = None
+        kernels.append({"kernel": name, "avg_ms": avg_ms, "launches_per_step": launches / args.profile_steps,
+                        "total_ms_per_step": avg_ms * launches / args.profile_steps,
+                        "alg_bytes": kb, "gbs": (kb / (avg_ms * 1e-3) / 1e9) if kb and avg_ms > 0 else None})
+    drv.profile(False)
+    kernels.sort(key=lambda r: -r["total_ms_per_step"])
+    dom = kernels[0]
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{dom['kernel']}_{args.workload}.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["kernel"],
+                "avg_launch_ms": round(dom["avg_ms"], 5), "alg_bytes_per_launch": dom["alg_bytes"]}
+    from hexdyn.setup import rk_schedule
+
+    sched = rk_schedule(cfg.config_dt, order=cfg.config_time_integration_order,
+                        acoustic_substeps=cfg.config_number_of_sub_steps,
+                        dynamics_splits=cfg.config_dynamics_split_steps)
+    n_ac = sum(s.acoustic_steps for s in sched)
+    b_alg = step_bytes(C=C, L=L, T=T, max_edges=case.mesh.max_edges, n_ac=n_ac, mix=False,
+                       splits=cfg.config_dynamics_split_steps)
+    step_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
+    step_roofline = {"alg_bytes_per_step": b_alg, "achieved": round(step_gbs, 1), "unit": "GB/s",
+                     "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(case, args.cpu_steps)
+        except Exception as exc:  # reported, never silently substituted
+            cpu = {"error": repr(exc)}
+
+    out = {
+        "metric": "sim-days/wall-day x1.40962 L55, 1 GPU: plain vs learned-corrected; %HBM peak",
+        "value": round(value, 1),
+        "unit": "sim-days/wall-day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded icosahedral SCVT mesh, JW baroclinic jet, random terrain, seeded moisture)",
+        "config": {"workload": args.workload, "cells": C, "edges": E, "levels": L, "scalars": T,
+                   "dt": cfg.config_dt, "form": "plain", "numerics": args.numerics,
+                   "dynamics_split_steps": cfg.config_dynamics_split_steps,
+                   "number_of_sub_steps": cfg.config_number_of_sub_steps, "parallelism": "single-gpu"},
+        "roofline": roofline,
+        "step_roofline": step_roofline,
+        "cpu_baseline": cpu,
+        "kernels_top": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for r in kernels[:8]],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,123 @@
+"""Algorithmic HBM bytes of the dycore large step.
+
+Two models, both in binary32 words:
+
+* ``step_bytes``: the phase-level count.  Each named array a phase reads or
+  writes is counted once at its declared shape; in-phase intermediates are
+  not counted (except the FCT scale factors, which need a grid-wide
+  synchronisation); level-independent geometry is excluded.  Floats per
+  (cell, level), with E = 3C and V = 2C:
+
+      solve_diagnostics        20           per call, 4 calls per subcycle
+      implicit coefficients    18           stages 1 and 2
+      subcycle pools           18           once per subcycle
+      mixing                   19           once per subcycle, if on
+      stage tendencies         33 + 2*ME    per stage
+      acoustic substep         50           per substep
+      divergence damping        9           per substep
+      recovery                 38 + 2*ME    per stage
+      split flux               10           per subcycle
+      moist coefficients       T + 5        per step
+      transport stages 1-2     4T + 13      per stage
+      transport stage 3 (FCT)  8T + 13      once
+
+* ``kernel_bytes``: per launch of one HIP kernel, every distinct array the
+  kernel reads or writes counted once at its shape (neighbour gathers of the
+  same array are not counted again).  This is the numerator of a kernel's
+  achieved-bandwidth roofline.
+"""
+
+from __future__ import annotations
+
+from typing import Dict, List, Tuple
+
+WORD = 4
+HBM_PEAK_GBS = 8000.0
+
+
+def floats_per_cell_level(*, T: int, max_edges: int, n_ac: int, mix: bool, splits: int = 3) -> float:
+    per_sub = (4 * 20 + 2 * 18 + 18 + (19 if mix else 0) + 3 * (33 + 2 * max_edges) + n_ac * (50 + 9)
+               + 3 * (38 + 2 * max_edges) + 10)
+    transport = (2 * (4 * T + 13) + (8 * T + 13)) if T > 0 else 0
+    return splits * per_sub + (T + 5 if T > 0 else 0) + transport
+
+
+def step_bytes(*, C: int, L: int, T: int, max_edges: int, n_ac: int, mix: bool = False, splits: int = 3) -> float:
+    """Algorithmic bytes of one large step (phase-level count)."""
+    return WORD * C * L * floats_per_cell_level(T=T, max_edges=max_edges, n_ac=n_ac, mix=mix, splits=splits)
+
+
+# Per-kernel array lists: (entity, levels) with entity in {"C", "E", "V"} and
+# levels "L" (layer) or "W" (interface, L+1).  "S" prefix: one row per scalar.
+_K: Dict[str, List[Tuple[str, str]]] = {
+    "k_diag_vertex": [("E", "L"), ("V", "L"), ("V", "L"), ("V", "L")],
+    "k_diag_cell": [("E", "L"), ("V", "L"), ("V", "L"), ("C", "L"), ("C", "L"), ("C", "L")],
+    "k_diag_edge": [("E", "L"), ("V", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
+    "k_moist_cell": [("SC", "L"), ("C", "L"), ("C", "L")],
+    "k_moist_edge": [("SC", "L"), ("E", "L")],
+    "k_vcoef": [("C", "L")] * 10 + [("C", "W")],
+    "k_sub_cell": [("C", "L")] * 12 + [("C", "W")] * 3 + [("E", "L")],
+    "k_sub_edge": [("E", "L")] * 5 + [("C", "L")] * 3,
+    "k_hdiv": [("E", "L"), ("C", "L")],
+    # u, c_rw, pv_edge, c_rho, ke, hdiv, euler_ru, c_ru, theta, w, s_ru, s_theta | tend_ru, fth, fcorr, fw
+    "k_tend_edge": [("E", "L"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
+                    ("E", "L"), ("C", "L"), ("C", "W"), ("E", "L"), ("C", "L"),
+                    ("E", "L"), ("E", "L"), ("E", "L"), ("E", "W")],
+    # theta, s_theta, c_rw, s_rw, w, fth, fcorr, fw, euler_rw, tend_ru, zb, zb3, zz | tend_rt, tend_rw
+    "k_tend_cell": [("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("C", "W"), ("E", "L"), ("E", "L"),
+                    ("E", "W"), ("C", "W"), ("E", "L"), ("CM", "W"), ("CM", "W"), ("C", "L"),
+                    ("C", "L"), ("C", "W")],
+    # tend_ru, ru_p, ru_avg, rtheta_pp, zz, cqu, s_exner, zxu, rho_pp | ru_p, ru_avg
+    "k_ac_edge": [("E", "L"), ("E", "L"), ("E", "L"), ("C", "L"), ("C", "L"), ("E", "L"), ("C", "L"),
+                  ("E", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
+    # rho_pp, rtheta_pp, rw_p, ww_avg, ru_p, s_theta, tend_rho, tend_rt, tend_rw, coftz, cofwz, cofwr,
+    # cofwt, atri, alpha, gamma, zz, dss, s_rw, c_rw, c_rho, w | rw_p, ww_avg, rho_pp, rtheta_pp, rtheta_pp_old
+    "k_ac_cell": [("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"),
+                  ("C", "L"), ("C", "W"), ("C", "W"), ("C", "L"), ("C", "L"), ("C", "L"), ("C", "L"),
+                  ("C", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("C", "L"),
+                  ("C", "W"), ("C", "W"), ("C", "W"), ("C", "L"), ("C", "L"), ("C", "L")],
+    "k_divdamp": [("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
+    # s_rho_p, rho_pp, s_rtheta_p, rtheta_pp, rb, rtb, s_rw, rw_p, ww_avg, s_ru, ru_p, zz, zb, zb3, exb |
+    # rho_p, rtheta_p, c_rho, c_rt, theta, c_rw, flux_w, w, (exner, pp)
+    "k_rec_cell": [("C", "L")] * 6 + [("C", "W")] * 3 + [("E", "L")] * 2 + [("C", "L"), ("CM", "W"), ("CM", "W"),
+                   ("C", "L")] + [("C", "L")] * 5 + [("C", "W")] * 3 + [("C", "L")] * 2,
+    "k_rec_edge": [("E", "L")] * 3 + [("C", "L")] * 3 + [("E", "L")] * 3,
+    "k_sc_edge": [("SC", "L"), ("E", "L"), ("SE", "L")],
+    "k_sc_cell": [("SC", "L"), ("C", "W"), ("E", "L"), ("SE", "L"), ("C", "L"), ("C", "L"), ("SC", "L"),
+                  ("SC", "L")],
+    "k_fct_target": [("E", "L"), ("C", "W"), ("C", "L"), ("C", "L")],
+    "k_fct_edge1": [("SC", "L"), ("SC", "L"), ("E", "L"), ("SE", "L"), ("SE", "L")],
+    "k_fct_cell1": [("SC", "L"), ("SC", "L"), ("C", "L"), ("C", "W"), ("SE", "L"), ("SE", "L"), ("C", "L"),
+                    ("SC", "L"), ("SC", "L"), ("SC", "L"), ("SC", "W")],
+    "k_fct_edge2": [("SC", "L"), ("SC", "L"), ("SE", "L"), ("SE", "L")],
+    "k_fct_cell2": [("SC", "L"), ("SC", "L"), ("SC", "W"), ("SC", "L"), ("SE", "L"), ("C", "L"), ("SC", "L")],
+    "k_split_scale": [("X", "W"), ("X", "W")],
+}
+
+
+def kernel_bytes(kernel: str, *, C: int, E: int, V: int, L: int, T: int, max_edges: int = 6) -> float:
+    """Algorithmic bytes of one launch of ``kernel`` (scalar kernels: all species of one launch)."""
+    if kernel not in _K:
+        raise KeyError(kernel)
+    total = 0
+    for ent, lev in _K[kernel]:
+        nl = L if lev == "L" else L + 1
+        if ent == "C":
+            rows = C
+        elif ent == "E":
+            rows = E
+        elif ent == "V":
+            rows = V
+        elif ent == "CM":
+            rows = C * max_edges
+        elif ent == "SC":
+            rows = C * T
+        elif ent == "SE":
+            rows = E * T
+        else:
+            rows = C
+        total += rows * nl
+    return float(WORD * total)
+
+
+KNOWN_KERNELS = tuple(_K)

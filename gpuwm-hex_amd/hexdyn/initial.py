@@ -317,9 +317,9 @@ def build_reference_and_state(mesh: Mesh, grid: VerticalGrid, *, moist: bool, n_
 
     # Normal momentum at edges from the JW zonal jet.
     c1, c2 = mesh.cellsOnEdge[:, 0], mesh.cellsOnEdge[:, 1]
-    z_edge = 0.5 * (zmid[:, c1] + zmid[:, c2])
     lat_e = mesh.latEdge[None, :].repeat(nlev, axis=0)
-    eta_e = _eta_from_height(z_edge, lat_e, R)
+    # eta at the edge from its two cells (the coordinate is smooth on the mesh scale)
+    eta_e = 0.5 * (eta[:, c1] + eta[:, c2])
     _, _, u_zonal = _jw_fields(eta_e, lat_e, R)
     u = u_zonal * np.cos(mesh.angleEdge)[None, :]
     ru = 0.5 * (rho_zz[:, c1] + rho_zz[:, c2]) * u

@@ -1208,6 +1208,44 @@ int oracle_end_step(void *h) {
   return rc;
 }
 float oracle_powf(float x, float y) { return hexdyn_powf_eval(x, y); }
+
+// RK schedule of integration.py:110-160 rounded to binary32 (large[3], acoustic[3], steps[3]).
+void oracle_schedule(double dt, int order, int substeps, int splits, float *large, float *acoustic, int *steps) {
+  Stage s[3];
+  schedule(dt, order, substeps, splits, s);
+  for (int i = 0; i < 3; ++i) {
+    large[i] = (float)s[i].large;
+    acoustic[i] = (float)s[i].acoustic;
+    steps[i] = s[i].steps;
+  }
+}
+
+// Tridiagonal coefficients b_tri and c_tri of the last vertical-coefficient evaluation.
+int oracle_export_btri_ctri(void *h, float *b, float *c, int64_t n) {
+  int rc = 0;
+  DISPATCH(h, {
+    if ((int64_t)o.btri.size() != n) rc = 1;
+    else for (int64_t i = 0; i < n; ++i) { b[i] = (float)o.btri[i]; c[i] = (float)o.ctri[i]; }
+  });
+  return rc;
+}
+int oracle_export_btri_ctri_f64(void *h, double *b, double *c, int64_t n) {
+  int rc = 0;
+  DISPATCH(h, {
+    if ((int64_t)o.btri.size() != n) rc = 1;
+    else for (int64_t i = 0; i < n; ++i) { b[i] = (double)o.btri[i]; c[i] = (double)o.ctri[i]; }
+  });
+  return rc;
+}
+int oracle_export_field_f64(void *h, int id, double *dst, int64_t n) {
+  int rc = 0;
+  DISPATCH(h, {
+    auto *f = o.field(id);
+    if (!f || (int64_t)f->size() != n) rc = 1;
+    else for (int64_t i = 0; i < n; ++i) dst[i] = (double)(*f)[i];
+  });
+  return rc;
+}
 double oracle_model_time(void *h) {
   double t = 0;
   DISPATCH(h, t = o.time_s);

@@ -62,6 +62,10 @@ def lib() -> ct.CDLL:
         _lib.oracle_powf.restype = ct.c_float
         _lib.oracle_model_time.argtypes = [ct.c_void_p]
         _lib.oracle_model_time.restype = ct.c_double
+        _lib.oracle_schedule.argtypes = [ct.c_double, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p,
+                                         ct.c_void_p]
+        _lib.oracle_export_btri_ctri_f64.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64]
+        _lib.oracle_export_field_f64.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int64]
     return _lib
 
 
@@ -121,6 +125,22 @@ class Oracle:
         if rc:
             raise KeyError(name)
 
+    def export_field_f64(self, name: str) -> np.ndarray:
+        fid = capi.FIELD_ID[name]
+        n = lib().oracle_field_size(self.h, fid)
+        out = np.empty(n, dtype=np.float64)
+        if lib().oracle_export_field_f64(self.h, fid, out.ctypes.data, n):
+            raise KeyError(name)
+        return out
+
+    def btri_ctri_f64(self):
+        n = self.case.n_levels * self.case.mesh.n_cells
+        b = np.empty(n, dtype=np.float64)
+        c = np.empty(n, dtype=np.float64)
+        if lib().oracle_export_btri_ctri_f64(self.h, b.ctypes.data, c.ctypes.data, n):
+            raise RuntimeError("btri/ctri size mismatch")
+        return b, c
+
     def step(self, n: int = 1) -> int:
         return lib().oracle_step(self.h, n)
 
@@ -142,3 +162,11 @@ class Oracle:
 
 def powf(x: float, y: float) -> float:
     return float(lib().oracle_powf(x, y))
+
+
+def schedule_f32(dt: float, order: int, substeps: int, splits: int):
+    large = np.zeros(3, dtype=np.float32)
+    acoustic = np.zeros(3, dtype=np.float32)
+    steps = np.zeros(3, dtype=np.int32)
+    lib().oracle_schedule(dt, order, substeps, splits, large.ctypes.data, acoustic.ctypes.data, steps.ctypes.data)
+    return large, acoustic, steps

@@ -1,0 +1,14 @@
+from hexdyn.bytes_model import floats_per_cell_level, kernel_bytes, step_bytes, KNOWN_KERNELS
+
+
+def test_phase_table_headline_value():
+    # x1.40962 L55, T=6, maxEdges 6, 4 acoustic substeps per subcycle (1, 1, 2), mixing on
+    f = floats_per_cell_level(T=6, max_edges=6, n_ac=4, mix=True)
+    assert 2150 <= f <= 2250
+    b = step_bytes(C=40962, L=55, T=6, max_edges=6, n_ac=4, mix=True)
+    assert 19.0e9 < b < 20.5e9
+
+
+def test_kernel_bytes_defined():
+    for k in KNOWN_KERNELS:
+        assert kernel_bytes(k, C=2562, E=7680, V=5120, L=55, T=6) > 0

@@ -1,0 +1,22 @@
+import pytest
+
+from hexdyn.config import ConfigurationRefusal, DycoreConfig
+
+
+def test_defaults_valid():
+    DycoreConfig().validate()
+
+
+@pytest.mark.parametrize("raw", [
+    {"config_unknown_knob": 1},
+    {"config_dt": -5.0},
+    {"config_time_integration_order": 4},
+    {"config_dynamics_split_steps": 2},
+    {"config_dynamics_split_steps": 3, "config_split_dynamics_transport": False},
+    {"config_apvm_upwinding": -0.1},
+    {"config_horiz_mixing": "2d_smagorinsky"},
+    {"numerics": "loose"},
+])
+def test_refusals(raw):
+    with pytest.raises(ConfigurationRefusal):
+        DycoreConfig.from_mapping(raw)

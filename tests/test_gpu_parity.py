@@ -1,0 +1,193 @@
+"""HIP dycore (libhexdyn.so through its C ABI) against the CPU oracle.
+
+Strict numerics: every phase and every whole step bit for bit equal to the
+binary32 restatement.  Fast numerics (FMA contraction): within relative L2
+1e-5 of the FP64 restatement on u, w, theta_m and rho_zz after one step.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from hexdyn.case import initial_state_arrays  # noqa: E402
+from hexdyn.config import DycoreConfig  # noqa: E402
+from hexdyn.setup import rk_schedule  # noqa: E402
+from oracle.oracle import Oracle  # noqa: E402
+
+
+def _driver(case):
+    from hexdyn.driver import HexDycoreDriver
+
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return HexDycoreDriver.from_host(case)
+
+
+def _bits_equal(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float32).ravel()
+    b = np.ascontiguousarray(b, dtype=np.float32).ravel()
+    return a.size == b.size and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _diff(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    ne = np.flatnonzero(a.astype(np.float32).view(np.uint32) != b.astype(np.float32).view(np.uint32))
+    return f"{ne.size} words differ, max |d| {np.abs(a - b).max():.3e}"
+
+
+STATE = ["RHO_ZZ", "RTHETA_M", "RU", "RW", "THETA_M", "EXNER", "RHO_P", "RTHETA_P", "PRESSURE_P", "U", "W"]
+
+
+def _phase_walk(case):
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    d = _driver(case)
+    bad = []
+
+    def cmp(tag, fields):
+        for f in fields:
+            a = o.export_field(f)
+            b = d.export_field(f).cpu().numpy()
+            if not _bits_equal(a, b):
+                bad.append(f"{tag}:{f}: {_diff(a, b)}")
+
+    state = STATE + (["SCALARS"] if case.n_scalars else [])
+    cmp("init", state)
+    o.begin_step(); d.phase("begin_step")
+    cmp("begin_step", ["V", "PV_EDGE", "KE", "DIVERGENCE", "VORTICITY"] + (["QTOT", "CQW", "CQU"] if case.n_moist else []))
+    cfg = case.config
+    sched = rk_schedule(cfg.config_dt, acoustic_substeps=cfg.config_number_of_sub_steps,
+                        dynamics_splits=cfg.config_dynamics_split_steps)
+    for sub in range(1, cfg.config_dynamics_split_steps + 1):
+        o.begin_subcycle(sub); d.phase("begin_subcycle", sub)
+        cmp(f"s{sub}", ["A_TRI", "ALPHA_TRI", "GAMMA_TRI", "COFWZ", "COFWT", "COFTZ", "EULER_RU", "EULER_RW",
+                        "TEND_RHO"])
+        for st in (1, 2, 3):
+            o.dyn_tend(st); d.phase("dyn_tend", st)
+            cmp(f"s{sub}t{st}", ["TEND_RU", "TEND_RT", "TEND_RW"])
+            for ss in range(1, sched[st - 1].acoustic_steps + 1):
+                o.acoustic_substep(st, ss); d.phase("acoustic_substep", st, ss)
+                cmp(f"s{sub}t{st}a{ss}", ["RU_P", "RU_AVG", "RW_P", "WW_AVG", "RHO_PP", "RTHETA_PP"])
+                o.divergence_damping(st); d.phase("divergence_damping", st)
+                cmp(f"s{sub}t{st}d{ss}", ["RU_P"])
+            o.recover(st); d.phase("recover", st)
+            cmp(f"s{sub}t{st}r", STATE + ["FLUX_U", "FLUX_W"])
+            o.solve_diagnostics(st); d.phase("solve_diagnostics", st)
+            cmp(f"s{sub}t{st}g", ["V", "PV_EDGE", "KE", "DIVERGENCE", "VORTICITY"])
+        o.end_subcycle(sub); d.phase("end_subcycle", sub)
+    if case.n_scalars:
+        for st in (1, 2, 3):
+            o.scalars(st); d.phase("scalars", st)
+            cmp(f"q{st}", ["SCALARS_STAGE"])
+    o.end_step(); d.phase("end_step")
+    cmp("end", state)
+    return bad
+
+
+@pytest.mark.parametrize("name", ["x1.642-moist", "x1.2562-dry"])
+def test_phase_by_phase_bitwise(get_case, name):
+    print(f"[gpu] phase walk {name}", flush=True)
+    bad = _phase_walk(get_case(name))
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_three_steps_bitwise_moist(get_case):
+    print("[gpu] x1.2562-moist 3 steps", flush=True)
+    case = get_case("x1.2562-moist")
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(3) == 0
+    d = _driver(case)
+    d.step_device(3)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
+def test_headline_mesh_step_bitwise_and_conservative(get_case):
+    print("[gpu] x1.40962-moist 1 step", flush=True)
+    case = get_case("x1.40962-moist")
+    d = _driver(case)
+    s0 = d.get_state()
+    d.step_device(1)
+    got = d.get_state()
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(1) == 0
+    ref = o.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+    # size-independent properties of the GPU result
+    area = 1.0 / case.inv_area_cell.astype(np.float64)[None, :]
+    dz = (1.0 / case.grid.rdzw.astype(np.float64))[:, None]
+    m0 = (area * s0["rho_zz"].astype(np.float64) * dz).sum()
+    m1 = (area * got["rho_zz"].astype(np.float64) * dz).sum()
+    assert abs(m1 / m0 - 1) < 1e-6
+    assert got["scalars"].min() >= 0.0
+    for k, v in got.items():
+        assert np.isfinite(v).all(), k
+
+
+def test_fast_numerics_within_tolerance_of_fp64(get_case):
+    print("[gpu] fast numerics vs FP64", flush=True)
+    base = get_case("x1.2562-moist")
+    import dataclasses
+
+    fast = dataclasses.replace(base, config=dataclasses.replace(base.config, numerics="fast"))
+    d = _driver(fast)
+    d.step_device(1)
+    got = d.get_state()
+    o = Oracle(base, 64)
+    o.set_state(initial_state_arrays(base), rebuild=True)
+    assert o.step(1) == 0
+    ref = o.get_state()
+    for k in ("u", "w", "theta_m", "rho_zz"):
+        rel = np.linalg.norm(got[k].astype(np.float64) - ref[k]) / np.linalg.norm(ref[k].astype(np.float64))
+        print(f"  fast {k}: rel L2 vs FP64 = {rel:.3e}")
+        assert rel <= 1e-5, (k, rel)
+
+
+def test_validation_flag_refuses_and_keeps_state(get_case):
+    print("[gpu] validation flag", flush=True)
+    case = get_case("x1.642-dry")
+    d = _driver(case)
+    before = d.get_state()
+    bad = before["rtheta_m"].copy()
+    bad[3, 7] = np.nan
+    t = torch.from_numpy(bad).cuda()
+    from hexdyn import capi
+    from hexdyn.lib import lib
+    import ctypes as ct
+
+    rc = lib().hexdyn_import_field(d.handle, capi.FIELD_ID["RTHETA_M"], ct.c_void_p(t.data_ptr()),
+                                   ct.c_void_p(d.stream_ptr()))
+    assert rc == 0
+    with pytest.raises(FloatingPointError):
+        d.step_device(1)
+    after = d.get_state()
+    assert _bits_equal(after["rho_zz"], before["rho_zz"])
+    assert _bits_equal(after["ru"], before["ru"])
+    assert np.isnan(after["rtheta_m"][3, 7])
+
+
+def test_determinism_and_restart(get_case):
+    print("[gpu] determinism and restart", flush=True)
+    case = get_case("x1.2562-moist")
+    a = _driver(case)
+    a.step_device(3)
+    sa = a.get_state()
+    b = _driver(case)
+    b.step_device(1)
+    mid = b.get_state()
+    from hexdyn.driver import HexDycoreDriver
+
+    c = HexDycoreDriver.from_host(case, state=mid, saved=mid)
+    c.step_device(2)
+    sc = c.get_state()
+    b.step_device(2)
+    sb = b.get_state()
+    for k in sa:
+        assert _bits_equal(sa[k], sb[k]), f"determinism {k}"
+        assert _bits_equal(sa[k], sc[k]), f"restart {k}"
