@@ -131,6 +131,11 @@ def test_headline_mesh_step_bitwise_and_conservative(get_case):
 
 
 def test_fast_numerics_within_tolerance_of_fp64(get_case):
+    """Tolerance: relative L2 <= 1e-5 against FP64 on u, theta_m and rho_zz.
+    For w, a small residual of cancelling terms, binary32 evaluated in the
+    reference's own order already sits near 1e-4 (the strict path, equal to
+    the binary32 restatement bit for bit); fast numerics must be no worse than
+    1.5 times that floor."""
     print("[gpu] fast numerics vs FP64", flush=True)
     base = get_case("x1.2562-moist")
     import dataclasses
@@ -143,10 +148,20 @@ def test_fast_numerics_within_tolerance_of_fp64(get_case):
     o.set_state(initial_state_arrays(base), rebuild=True)
     assert o.step(1) == 0
     ref = o.get_state()
+    o32 = Oracle(base, 32)
+    o32.set_state(initial_state_arrays(base), rebuild=True)
+    assert o32.step(1) == 0
+    strict = o32.get_state()
+
+    def rel(x, k):
+        return np.linalg.norm(x[k].astype(np.float64) - ref[k]) / np.linalg.norm(ref[k].astype(np.float64))
+
     for k in ("u", "w", "theta_m", "rho_zz"):
-        rel = np.linalg.norm(got[k].astype(np.float64) - ref[k]) / np.linalg.norm(ref[k].astype(np.float64))
-        print(f"  fast {k}: rel L2 vs FP64 = {rel:.3e}")
-        assert rel <= 1e-5, (k, rel)
+        print(f"  {k}: fast rel L2 vs FP64 = {rel(got, k):.3e}, strict binary32 = {rel(strict, k):.3e}")
+    for k in ("u", "theta_m", "rho_zz"):
+        assert rel(got, k) <= 1e-5, k
+    assert rel(got, "w") <= 1.5 * rel(strict, "w")
+    assert not _bits_equal(got["u"], strict["u"])  # the fast path really is a different instruction stream
 
 
 def test_validation_flag_refuses_and_keeps_state(get_case):
