@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=0, help="CPU sample size in large steps (0: auto ~15 s)")
     p.add_argument("--profile-steps", type=int, default=2)
+    p.add_argument("--calibration-copy-mib", type=int, default=0,
+                   help="also run a device copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
 
 
@@ -93,6 +95,14 @@ def main():
     print(f"# setup {setup_s:.1f}s: C={case.mesh.n_cells} E={case.mesh.n_edges} L={case.n_levels} "
           f"T={case.n_scalars}", file=sys.stderr, flush=True)
 
+    if args.calibration_copy_mib > 0:
+        n = args.calibration_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         drv.step_device(1)
     torch.cuda.synchronize()
