@@ -228,7 +228,10 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   };
   if (n.C <= 0 || n.E <= 0 || n.V <= 0) return refuse("empty mesh");
   if (n.L < 4 || n.L + 1 > HX_ROW) return refuse("n_levels must satisfy 4 <= L and L + 1 <= 64");
-  if (n.ME < 3 || n.ME > HEXDYN_MAX_EDGES) return refuse("max_edges outside [3, 7]");
+  if (n.ME < 6 || n.ME > HEXDYN_MAX_EDGES) return refuse("max_edges must be 6 or 7");
+  // edge stencils are stored at the fixed width 2*maxEdges-2 (kernels unroll over it)
+  if (m->max_edges2 > 2 * n.ME - 2 || m->max_adv > 2 * n.ME - 2)
+    return refuse("edgesOnEdge / advCellsForEdge wider than 2*maxEdges-2");
   if (m->vertex_degree != 3) return refuse("vertex_degree must be 3");
   if (n.NT < 0 || n.NM < 0 || n.NM > n.NT) return refuse("n_moist must lie in [0, n_scalars]");
   if (c->config_time_integration_order != 2 && c->config_time_integration_order != 3)
@@ -242,6 +245,8 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   if (c->config_apvm_upwinding < 0.0f) return refuse("config_apvm_upwinding must be non-negative");
   if (c->config_divergence_damping && !(c->config_len_disp > 0.0f)) return refuse("config_len_disp must be positive");
   h->fast = c->numerics == HEXDYN_NUMERICS_FAST;
+  const int in_me2 = m->max_edges2, in_ma = m->max_adv;
+  n.ME2 = n.MA = 2 * n.ME - 2;
   const int C = n.C, E = n.E, V = n.V, L = n.L, ME = n.ME, ME2 = n.ME2, MA = n.MA;
 
   // ---- renumbering along a 3-D Hilbert curve
@@ -290,21 +295,21 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
     invdc[ne] = m->inv_dc_edge[e]; invdv[ne] = m->inv_dv_edge[e];
     fedge[ne] = m->f_edge[e]; cosang[ne] = m->cos_angle_edge[e]; sinang[ne] = m->sin_angle_edge[e];
     neoe[ne] = m->n_edges_on_edge[e];
-    if (neoe[ne] < 0 || neoe[ne] > ME2) return refuse("nEdgesOnEdge out of range");
+    if (neoe[ne] < 0 || neoe[ne] > in_me2) return refuse("nEdgesOnEdge out of range");
     for (int s = 0; s < neoe[ne]; ++s) {
-      const int o = m->edges_on_edge[(size_t)e * ME2 + s];
+      const int o = m->edges_on_edge[(size_t)e * in_me2 + s];
       if (!chk(o, E)) return refuse("edgesOnEdge out of range");
       eoe[(size_t)ne * ME2 + s] = new_e[o];
-      woe[(size_t)ne * ME2 + s] = m->weights_on_edge[(size_t)e * ME2 + s];
+      woe[(size_t)ne * ME2 + s] = m->weights_on_edge[(size_t)e * in_me2 + s];
     }
     nadv[ne] = m->n_adv_cells_for_edge[e];
-    if (nadv[ne] < 0 || nadv[ne] > MA) return refuse("nAdvCellsForEdge out of range");
+    if (nadv[ne] < 0 || nadv[ne] > in_ma) return refuse("nAdvCellsForEdge out of range");
     for (int s = 0; s < nadv[ne]; ++s) {
-      const int o = m->adv_cells_for_edge[(size_t)e * MA + s];
+      const int o = m->adv_cells_for_edge[(size_t)e * in_ma + s];
       if (!chk(o, C)) return refuse("advCellsForEdge out of range");
       advc[(size_t)ne * MA + s] = new_c[o];
-      adv[(size_t)ne * MA + s] = m->adv_coefs[(size_t)e * MA + s];
-      adv3[(size_t)ne * MA + s] = m->adv_coefs_3rd[(size_t)e * MA + s];
+      adv[(size_t)ne * MA + s] = m->adv_coefs[(size_t)e * in_ma + s];
+      adv3[(size_t)ne * MA + s] = m->adv_coefs_3rd[(size_t)e * in_ma + s];
     }
   }
   for (int nc = 0; nc < C; ++nc) {
@@ -700,7 +705,7 @@ int hexdyn_dyn_tend(hexdyn_handle *h, int stage, void *stream) {
 
 int hexdyn_acoustic_substep(hexdyn_handle *h, int stage, int small_step, void *stream) {
   if (!h || stage < 1 || stage > 3 || small_step < 1 || small_step > h->I.k.steps[stage - 1]) return HEXDYN_E_INVALID;
-  return NS_CALL(h, acoustic_substep, h->I, stage, small_step, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  return NS_CALL(h, acoustic_substep, h->I, stage, small_step, 0, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_divergence_damping(hexdyn_handle *h, int stage, void *stream) {
@@ -712,7 +717,7 @@ int hexdyn_recover(hexdyn_handle *h, int stage, void *stream) {
   if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
   // At stage 3 the recovered fluxes are also accumulated into the split-transport
   // sums (copy on the first subcycle, add afterwards: integration.py:27-56).
-  return NS_CALL(h, recover, h->I, stage, h->cur_sub, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  return NS_CALL(h, recover, h->I, stage, h->cur_sub, 0, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_solve_diagnostics(hexdyn_handle *h, int rk_step, void *stream) {
@@ -752,11 +757,12 @@ int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream) {
       rc |= NS_CALL(h, begin_subcycle, h->I, sub, s);
       for (int st = 1; rc == 0 && st <= 3; ++st) {
         rc |= NS_CALL(h, dyn_tend, h->I, st, s);
-        for (int ss = 1; rc == 0 && ss <= h->I.k.steps[st - 1]; ++ss) {
-          rc |= NS_CALL(h, acoustic_substep, h->I, st, ss, s);
-          if (c.config_divergence_damping) rc |= NS_CALL(h, divergence_damping, h->I, st, s);
-        }
-        rc |= NS_CALL(h, recover, h->I, st, sub, s);
+        // divergence damping of substep ss is fused into the next edge pass:
+        // the acoustic edge update of ss+1, or the edge recovery after the last substep
+        const int dd = c.config_divergence_damping ? 1 : 0;
+        for (int ss = 1; rc == 0 && ss <= h->I.k.steps[st - 1]; ++ss)
+          rc |= NS_CALL(h, acoustic_substep, h->I, st, ss, (dd && ss > 1) ? 1 : 0, s);
+        rc |= NS_CALL(h, recover, h->I, st, sub, dd, s);
         rc |= NS_CALL(h, solve_diagnostics, h->I, st, s);
       }
     }

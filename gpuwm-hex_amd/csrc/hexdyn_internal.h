@@ -120,9 +120,9 @@ struct Instance {
   int begin_step(const Instance &I, hipStream_t s);                       \
   int begin_subcycle(const Instance &I, int sub, hipStream_t s);          \
   int dyn_tend(const Instance &I, int stage, hipStream_t s);              \
-  int acoustic_substep(const Instance &I, int stage, int ss, hipStream_t s); \
+  int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStream_t s); \
   int divergence_damping(const Instance &I, int stage, hipStream_t s);    \
-  int recover(const Instance &I, int stage, int sub, hipStream_t s);      \
+  int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s); \
   int solve_diagnostics(const Instance &I, int rk, hipStream_t s);        \
   int scalars_stage(const Instance &I, int stage, hipStream_t s);         \
   int rebuild_saved(const Instance &I, hipStream_t s);                    \

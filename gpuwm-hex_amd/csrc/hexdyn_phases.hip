@@ -74,18 +74,29 @@ __device__ __forceinline__ void set_flag(int *flag) { atomicOr(flag, 1); }
 
 inline int nblocks(int n) { return (n + 3) / 4; }
 
+// Gather helpers.  Connectivity rows are padded to fixed widths (NE = maxEdges
+// slots per cell, NX = 2*maxEdges-2 slots per edge) with index 0, so every
+// slot of an unrolled loop addresses a valid row.  All neighbour rows are
+// loaded before any arithmetic, then combined in slot order for the valid
+// slots only, which keeps the reference's association and lets the memory
+// system overlap every gather of the wave.
+
 // ------------------------------------------------------------------ diagnostics
 KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
   const int v = wave_entity();
   if (v >= I.n.V) return;
   const int k = lane();
+  int e[3];
+  float ue[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) e[s] = I.m.eov[3 * v + s];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) ue[s] = ROW(u, e[s])[k];
   float vort = 0.0f, kev = 0.0f;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    const int e = I.m.eov[3 * v + s];
-    const float ue = ROW(u, e)[k];
-    vort = vort + (I.m.vsign[3 * v + s] * I.m.dc[e]) * ue;
-    kev = kev + I.m.dcdv[e] * (ue * ue);
+    vort = vort + (I.m.vsign[3 * v + s] * I.m.dc[e[s]]) * ue[s];
+    kev = kev + I.m.dcdv[e[s]] * (ue[s] * ue[s]);
   }
   const float it = I.m.invtri[v];
   vort = vort * it;
@@ -97,33 +108,47 @@ KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
   }
 }
 
+template <int NE>
 KB k_diag_cell(const Instance I, const float *__restrict__ u) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int ME = I.n.ME;
   const int n = I.m.neoc[c];
   const float ia = I.m.invarea[c];
-  float d = 0.0f, kc = 0.0f, pvc = 0.0f;
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    const float ue = ROW(u, e)[k];
-    d = d + (I.m.csign[c * ME + s] * I.m.dv[e]) * ue;
-    kc = kc + 0.25f * (I.m.dcdv[e] * (ue * ue));
+  int e[NE], vv[NE], ks[NE];
+  float ue[NE], kv[NE], pv[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    e[s] = I.m.eoc[c * NE + s];
+    vv[s] = I.m.voc[c * NE + s];
+    ks[s] = I.m.kfc[c * NE + s];
   }
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    ue[s] = ROW(u, e[s])[k];
+    kv[s] = ROW(I.b.ke_vertex, vv[s])[k];
+    pv[s] = ROW(I.b.pv_vertex, vv[s])[k];
+  }
+  float d = 0.0f, kc = 0.0f, pvc = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      d = d + (I.m.csign[c * NE + s] * I.m.dv[e[s]]) * ue[s];
+      kc = kc + 0.25f * (I.m.dcdv[e[s]] * (ue[s] * ue[s]));
+    }
   d = d * ia;
   kc = kc * ia;
   kc = kc * 0.625f;
-  for (int s = 0; s < n; ++s) {
-    const int vv = I.m.voc[c * ME + s];
-    const int ks = I.m.kfc[c * ME + s];
-    const float kite = I.m.kite[3 * vv + ks];
-    float corr = 0.375f * kite;
-    corr = corr * ROW(I.b.ke_vertex, vv)[k];
-    corr = corr * ia;
-    kc = kc + corr;
-    if (I.k.apvm_on) pvc = pvc + (kite * ROW(I.b.pv_vertex, vv)[k]) * ia;
-  }
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      const float kite = I.m.kite[3 * vv[s] + ks[s]];
+      float corr = 0.375f * kite;
+      corr = corr * kv[s];
+      corr = corr * ia;
+      kc = kc + corr;
+      if (I.k.apvm_on) pvc = pvc + (kite * pv[s]) * ia;
+    }
   if (k < I.n.L) {
     ROW(I.b.divergence, c)[k] = d;
     ROW(I.b.ke, c)[k] = kc;
@@ -131,16 +156,25 @@ KB k_diag_cell(const Instance I, const float *__restrict__ u) {
   }
 }
 
+template <int NE>
 KB k_diag_edge(const Instance I, const float *__restrict__ u, int recon) {
+  constexpr int NX = 2 * NE - 2;
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
-  const int ME2 = I.n.ME2;
   float vv;
   if (recon) {
-    vv = 0.0f;
+    int nb[NX];
+    float un[NX];
+#pragma unroll
+    for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
+#pragma unroll
+    for (int s = 0; s < NX; ++s) un[s] = ROW(u, nb[s])[k];
     const int n = I.m.neoe[e];
-    for (int s = 0; s < n; ++s) vv = vv + I.m.woe[e * ME2 + s] * ROW(u, I.m.eoe[e * ME2 + s])[k];
+    vv = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NX; ++s)
+      if (s < n) vv = vv + I.m.woe[e * NX + s] * un[s];
   } else {
     vv = ROW(I.b.v, e)[k];
   }
@@ -253,13 +287,30 @@ KB k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const flo
   }
 }
 
+// sum_s csign*dv*x[e_s] / area over the cell's edges, in slot order
+template <int NE>
+__device__ __forceinline__ float cell_flux_div(const Instance &I, int c, int k, const float *__restrict__ x) {
+  const int n = I.m.neoc[c];
+  int e[NE];
+  float xv[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) xv[s] = ROW(x, e[s])[k];
+  float acc = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) acc = acc + (I.m.csign[c * NE + s] * I.m.dv[e[s]]) * xv[s];
+  return acc * I.m.invarea[c];
+}
+
 // ------------------------------------------------------------ subcycle start
+template <int NE>
 KB k_sub_cell(const Instance I) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L, ME = I.n.ME;
-  // subcycle-start copies
+  const int L = I.n.L;
   const float rho_p = ROW(I.b.rho_p, c)[k];
   ROW(I.b.s_rho, c)[k] = ROW(I.b.c_rho, c)[k];
   ROW(I.b.s_rt, c)[k] = ROW(I.b.c_rt, c)[k];
@@ -276,17 +327,9 @@ KB k_sub_cell(const Instance I) {
     const float qt = ROW(I.b.qtot, c)[k];
     dpdz = (-g) * (ROW(I.z.rb, c)[k] * qt + rho_p * (1.0f + qt));
   }
-  // density tendency from the subcycle-start fluxes
-  float acc = 0.0f;
-  const int n = I.m.neoc[c];
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    acc = acc + (I.m.csign[c * ME + s] * I.m.dv[e]) * ROW(I.b.c_ru, e)[k];
-  }
-  acc = acc * I.m.invarea[c];
+  const float acc = cell_flux_div<NE>(I, c, k, I.b.c_ru);
   const float *rwr = ROW(I.b.c_rw, c);
   const float trho = (-acc) - I.z.rdzw[k] * (rwr[cl(k + 1)] - rwr[k]);
-  // Euler w pool
   const float *ppr = ROW(I.b.pp, c);
   const float dpdzm = up1(dpdz);
   float erw = 0.0f;
@@ -297,6 +340,7 @@ KB k_sub_cell(const Instance I) {
   if (k < L) {
     ROW(I.b.dpdz, c)[k] = dpdz;
     ROW(I.b.tend_rho, c)[k] = trho;
+    ROW(I.b.hdiv, c)[k] = acc;  // stage-1 mass-flux divergence of the current ru
   }
   if (k <= L) ROW(I.b.euler_rw, c)[k] = erw;
 }
@@ -314,93 +358,139 @@ KB k_sub_edge(const Instance I) {
 }
 
 // ---------------------------------------------------------- stage tendencies
-KB k_hdiv(const Instance I) {
-  const int c = wave_entity();
-  if (c >= I.n.C) return;
-  const int k = lane();
-  const int ME = I.n.ME;
-  float acc = 0.0f;
-  const int n = I.m.neoc[c];
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    acc = acc + (I.m.csign[c * ME + s] * I.m.dv[e]) * ROW(I.b.c_ru, e)[k];
-  }
-  if (k < I.n.L) ROW(I.b.hdiv, c)[k] = acc * I.m.invarea[c];
-}
 
-KB k_tend_edge(const Instance I, int stage) {
+// momentum tendency of one stage (dynamics.py:223-268, dynamics_v841.py:128-185)
+template <int NE>
+KB k_tend_mom(const Instance I) {
+  constexpr int NX = 2 * NE - 2;
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
-  const int L = I.n.L, ME2 = I.n.ME2, MA = I.n.MA;
+  const int L = I.n.L;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  int nb[NX];
+#pragma unroll
+  for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
   const float *ur = ROW(I.b.u, e);
-  // vertical transport of u (dynamics.py:223-268)
-  const float rwe = 0.5f * (ROW(I.b.c_rw, c0)[k] + ROW(I.b.c_rw, c1)[k]);
+  const float u_k = ur[k];
+  const float u_m1 = ur[cl(k - 1)], u_m2 = ur[cl(k - 2)], u_p1 = ur[cl(k + 1)];
+  const float rw0 = ROW(I.b.c_rw, c0)[k], rw1 = ROW(I.b.c_rw, c1)[k];
+  const float pve = ROW(I.b.pv_edge, e)[k];
+  float un[NX], pvn[NX];
+#pragma unroll
+  for (int s = 0; s < NX; ++s) {
+    un[s] = ROW(I.b.u, nb[s])[k];
+    pvn[s] = ROW(I.b.pv_edge, nb[s])[k];
+  }
+  const float rho0 = ROW(I.b.c_rho, c0)[k], rho1 = ROW(I.b.c_rho, c1)[k];
+  const float ke0 = ROW(I.b.ke, c0)[k], ke1 = ROW(I.b.ke, c1)[k];
+  const float hd0 = ROW(I.b.hdiv, c0)[k], hd1 = ROW(I.b.hdiv, c1)[k];
+  const float eru = ROW(I.b.euler_ru, e)[k];
+  const float rwe = 0.5f * (rw0 + rw1);
   const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
   float wduz = 0.0f;
   if (k == 1 || k == L - 1) {
-    wduz = rwe * (fzm * ur[k] + fzp * ur[k - 1]);
+    wduz = rwe * (fzm * u_k + fzp * u_m1);
   } else if (k >= 2 && k < L - 1) {
-    wduz = flux3(ur[k - 2], ur[k - 1], ur[k], ur[k + 1], rwe, 1.0f);
+    wduz = flux3(u_m2, u_m1, u_k, u_p1, rwe, 1.0f);
   }
   const float wduz_p = dn1(wduz);
   const float initial = (-I.z.rdzw[k]) * (wduz_p - wduz);
-  // v8.4.1 nonlinear Coriolis, perturbation Coriolis, KE gradient, divergence term
-  const float pve = ROW(I.b.pv_edge, e)[k];
   const int nn = I.m.neoe[e];
   float qq = 0.0f;
-  for (int s = 0; s < nn; ++s) {
-    const int n = I.m.eoe[e * ME2 + s];
-    const float work_pv = 0.5f * (pve + ROW(I.b.pv_edge, n)[k]);
-    qq = qq + (I.m.woe[e * ME2 + s] * ROW(I.b.u, n)[k]) * work_pv;
-  }
+#pragma unroll
+  for (int s = 0; s < NX; ++s)
+    if (s < nn) {
+      const float work_pv = 0.5f * (pve + pvn[s]);
+      qq = qq + (I.m.woe[e * NX + s] * un[s]) * work_pv;
+    }
   const float ui = I.z.uinit[k], vi = I.z.vinit[k], fe = I.m.fedge[e];
-  for (int s = 0; s < nn; ++s) {
-    const int n = I.m.eoe[e * ME2 + s];
-    const float ref_u = ui * I.m.cosang[n] - vi * I.m.sinang[n];
-    qq = qq - (I.m.woe[e * ME2 + s] * ref_u) * fe;
+#pragma unroll
+  for (int s = 0; s < NX; ++s)
+    if (s < nn) {
+      const float ref_u = ui * I.m.cosang[nb[s]] - vi * I.m.sinang[nb[s]];
+      qq = qq - (I.m.woe[e * NX + s] * ref_u) * fe;
+    }
+  const float rho_edge = 0.5f * (rho0 + rho1);
+  const float horiz = rho_edge * (qq - (ke1 - ke0) * I.m.invdc[e]);
+  const float t = (initial + horiz) - (u_k * 0.5f) * (hd0 + hd1);
+  if (k < L) ROW(I.b.tend_ru, e)[k] = t + eru;
+}
+
+// theta and w edge fluxes with the 3rd-order upwind-blended stencil (driver.py:1003-1034, 1140-1170)
+template <int NE>
+KB k_tend_flux(const Instance I, int stage) {
+  constexpr int NX = 2 * NE - 2;
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const int L = I.n.L;
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  int ac[NX];
+#pragma unroll
+  for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
+  float thn[NX], wn[NX];
+#pragma unroll
+  for (int s = 0; s < NX; ++s) {
+    thn[s] = ROW(I.b.theta, ac[s])[k];
+    wn[s] = ROW(I.b.w, ac[s])[k];
   }
-  const float rho_edge = 0.5f * (ROW(I.b.c_rho, c0)[k] + ROW(I.b.c_rho, c1)[k]);
-  const float horiz = rho_edge * (qq - (ROW(I.b.ke, c1)[k] - ROW(I.b.ke, c0)[k]) * I.m.invdc[e]);
-  const float t = (initial + horiz) - (ur[k] * 0.5f) * (ROW(I.b.hdiv, c0)[k] + ROW(I.b.hdiv, c1)[k]);
-  const float tend = t + ROW(I.b.euler_ru, e)[k];
-  // theta and w edge fluxes with the 3rd-order upwind-blended stencil
-  const float coef = I.k.coef3;
   const float rue = ROW(I.b.c_ru, e)[k];
-  const float rue_m = ROW(I.b.c_ru, e)[cl(k - 1)];
-  const float rui = fzm * rue + fzp * rue_m;
-  const float sv_t = cs1(rue), sv_w = cs1(rui);
-  float et = 0.0f, ew = 0.0f;
-  const int na = I.m.nadv[e];
-  for (int s = 0; s < na; ++s) {
-    const int j = e * MA + s;
-    const int cell = I.m.advc[j];
-    const float a = I.m.adv[j], a3 = I.m.adv3[j];
-    et = et + (a + (coef * sv_t) * a3) * ROW(I.b.theta, cell)[k];
-    ew = ew + (a + (coef * sv_w) * a3) * ROW(I.b.w, cell)[k];
+  const float rue_m = up1(rue);
+  float sru = 0.0f, sth0 = 0.0f, sth1 = 0.0f;
+  if (stage > 1) {
+    sru = ROW(I.b.s_ru, e)[k];
+    sth0 = ROW(I.b.s_theta, c0)[k];
+    sth1 = ROW(I.b.s_theta, c1)[k];
   }
+  const float coef = I.k.coef3;
+  const float rui = I.z.fzm[k] * rue + I.z.fzp[k] * rue_m;
+  const float sv_t = cs1(rue), sv_w = cs1(rui);
+  const int na = I.m.nadv[e];
+  float et = 0.0f, ew = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NX; ++s)
+    if (s < na) {
+      const float a = I.m.adv[e * NX + s], a3 = I.m.adv3[e * NX + s];
+      et = et + (a + (coef * sv_t) * a3) * thn[s];
+      ew = ew + (a + (coef * sv_w) * a3) * wn[s];
+    }
   if (k < L) {
-    ROW(I.b.tend_ru, e)[k] = tend;
     ROW(I.b.fth, e)[k] = rue * et;
-    if (stage > 1)
-      ROW(I.b.fcorr, e)[k] = ((I.m.dv[e] * (ROW(I.b.s_ru, e)[k] - rue)) * 0.5f) *
-                             (ROW(I.b.s_theta, c1)[k] + ROW(I.b.s_theta, c0)[k]);
+    if (stage > 1) ROW(I.b.fcorr, e)[k] = ((I.m.dv[e] * (sru - rue)) * 0.5f) * (sth1 + sth0);
     if (k >= 1) ROW(I.b.fw, e)[k] = rui * ew;
   }
 }
 
+template <int NE>
 KB k_tend_cell(const Instance I, int stage) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L, ME = I.n.ME;
+  const int L = I.n.L;
   const float coef = I.k.coef3;
+  const int n = I.m.neoc[c];
+  int e[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  float fth[NE], fco[NE], fwv[NE], tru[NE], zb[NE], zb3[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    fth[s] = ROW(I.b.fth, e[s])[k];
+    fwv[s] = ROW(I.b.fw, e[s])[k];
+    tru[s] = ROW(I.b.tend_ru, e[s])[k];
+    const size_t zi = ((size_t)c * NE + s) * HX_ROW + k;
+    zb[s] = I.z.zb[zi];
+    zb3[s] = I.z.zb3[zi];
+  }
+  if (stage > 1) {
+#pragma unroll
+    for (int s = 0; s < NE; ++s) fco[s] = ROW(I.b.fcorr, e[s])[k];
+  }
   const float *th = ROW(I.b.theta, c), *ths = ROW(I.b.s_theta, c);
   const float *rwc = ROW(I.b.c_rw, c), *rws = ROW(I.b.s_rw, c), *ww = ROW(I.b.w, c);
   const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
   const int km = cl(k - 1);
-  // vertical theta flux at interface k
   float vf = 0.0f;
   if (k == 1) {
     vf = rwc[1] * (fzm * th[1] + fzp * th[0]);
@@ -412,7 +502,6 @@ KB k_tend_cell(const Instance I, int stage) {
     vf = rws[k] * (fzm * th[k] + fzp * th[k - 1]);
   }
   const float vf_p = dn1(vf);
-  // vertical w flux at level k (between interfaces k and k+1 uses w[k], w[k-1])
   float wf = 0.0f;
   if (k == 1 || k == L - 1) {
     wf = (0.25f * (rwc[k] + rwc[k - 1])) * (ww[k] + ww[k - 1]);
@@ -420,28 +509,37 @@ KB k_tend_cell(const Instance I, int stage) {
     wf = flux3(ww[k - 2], ww[k - 1], ww[k], ww[k + 1], 0.5f * (rwc[k] + rwc[k - 1]), 1.0f);
   }
   const float wf_p = dn1(wf);
-  const int n = I.m.neoc[c];
   const float ia = I.m.invarea[c];
   float r = 0.0f;
-  for (int s = 0; s < n; ++s) r = r - I.m.csign[c * ME + s] * ROW(I.b.fth, I.m.eoc[c * ME + s])[k];
-  if (stage > 1)
-    for (int s = 0; s < n; ++s) r = r - I.m.csign[c * ME + s] * ROW(I.b.fcorr, I.m.eoc[c * ME + s])[k];
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) r = r - I.m.csign[c * NE + s] * fth[s];
+  if (stage > 1) {
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) r = r - I.m.csign[c * NE + s] * fco[s];
+  }
   r = r * ia;
   const float trt = r - I.z.rdzw[k] * (vf_p - vf);
+  float tru_m[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) tru_m[s] = up1(tru[s]);
   float rw_t = 0.0f;
   if (k >= 1 && k < L) {
     float q = 0.0f;
-    for (int s = 0; s < n; ++s) q = q - I.m.csign[c * ME + s] * ROW(I.b.fw, I.m.eoc[c * ME + s])[k];
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) q = q - I.m.csign[c * NE + s] * fwv[s];
     q = q * ia;
     q = q - I.z.rdzu[k] * (wf_p - wf);
     q = q + ROW(I.b.euler_rw, c)[k];
-    for (int s = 0; s < n; ++s) {
-      const int e = I.m.eoc[c * ME + s];
-      const float tk = ROW(I.b.tend_ru, e)[k];
-      const float flux = I.m.csign[c * ME + s] * (fzm * tk + fzp * ROW(I.b.tend_ru, e)[km]);
-      const size_t zi = ((size_t)c * ME + s) * HX_ROW + k;
-      q = q - (I.z.zb[zi] + cs1(tk) * I.z.zb3[zi]) * flux;
-    }
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) {
+        const float tk = tru[s];
+        const float flux = I.m.csign[c * NE + s] * (fzm * tk + fzp * tru_m[s]);
+        q = q - (zb[s] + cs1(tk) * zb3[s]) * flux;
+      }
     const float *zzr = ROW(I.z.zz, c);
     rw_t = q * (fzm * zzr[k] + fzp * zzr[km]);
   }
@@ -450,13 +548,25 @@ KB k_tend_cell(const Instance I, int stage) {
 }
 
 // ----------------------------------------------------------- acoustic substep
-KB k_ac_edge(const Instance I, float dts, int first) {
+// ru_p + divergence-damping increment of edge e at level k (mixing.py:851-945)
+__device__ __forceinline__ float damped_ru_p(const Instance &I, int e, int k, float rp, float coefficient) {
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  const float d0 = ROW(I.b.rtheta_pp, c0)[k] - ROW(I.b.rtheta_pp_old, c0)[k];
+  const float d1 = ROW(I.b.rtheta_pp, c1)[k] - ROW(I.b.rtheta_pp_old, c1)[k];
+  const float den = ROW(I.b.s_theta, c0)[k] + ROW(I.b.s_theta, c1)[k];
+  return rp + ((coefficient * (d0 - d1)) * (1.0f - 0.0f)) / den;
+}
+
+// damp_prev: apply the previous substep's divergence damping to ru_p first
+// (the same update k_divdamp makes, fused into this pass over the edges)
+KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_coef) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
   float *rp = ROW(I.b.ru_p, e), *ra = ROW(I.b.ru_avg, e);
   const float tr = ROW(I.b.tend_ru, e)[k];
+  const float rp_in = damp_prev ? damped_ru_p(I, e, k, rp[k], damp_coef) : rp[k];
   if (first) {
     const float p = dts * tr;
     rp[k] = p;
@@ -468,17 +578,34 @@ KB k_ac_edge(const Instance I, float dts, int first) {
              (0.5f * (ROW(I.z.zz, c1)[k] + ROW(I.z.zz, c0)[k]));
   pg = (((ROW(I.b.cqu, e)[k] * 0.5f) * I.k.c2) * (ROW(I.b.s_exner, c0)[k] + ROW(I.b.s_exner, c1)[k])) * pg;
   pg = pg + ((0.5f * ROW(I.z.zxu, e)[k]) * I.k.gravity) * (ROW(I.b.rho_pp, c0)[k] + ROW(I.b.rho_pp, c1)[k]);
-  const float p = rp[k] + dts * (tr - pg);
+  const float p = rp_in + dts * (tr - pg);
   rp[k] = p;
   ra[k] = ra[k] + p;
 }
 
+template <int NE>
 KB k_ac_cell(const Instance I, float dts, int first) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L, ME = I.n.ME;
+  const int L = I.n.L;
   const int kp = cl(k + 1);
+  const int n = I.m.neoc[c];
+  int e[NE], o[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    const int a0 = I.m.coe[2 * e[s]], a1 = I.m.coe[2 * e[s] + 1];
+    o[s] = a0 == c ? a1 : a0;  // the other cell of edge s
+  }
+  float rup[NE], sto[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    rup[s] = ROW(I.b.ru_p, e[s])[k];
+    sto[s] = ROW(I.b.s_theta, o[s])[k];
+  }
+  const float sthc = ROW(I.b.s_theta, c)[k];
   float rpp = 0.0f, rtpp = 0.0f, rwp = 0.0f, wwa = 0.0f;
   if (!first) {
     rpp = ROW(I.b.rho_pp, c)[k];
@@ -487,27 +614,28 @@ KB k_ac_cell(const Instance I, float dts, int first) {
     wwa = ROW(I.b.ww_avg, c)[k];
   }
   const float rtpp_old = rtpp;
-  // explicit horizontal divergence of the acoustic flux
   const float ia = I.m.invarea[c];
   float r = 0.0f, t = 0.0f;
-  const int n = I.m.neoc[c];
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    const int e0 = I.m.coe[2 * e], e1 = I.m.coe[2 * e + 1];
-    const float flux = (((I.m.csign[c * ME + s] * dts) * I.m.dv[e]) * ROW(I.b.ru_p, e)[k]) * ia;
-    r = r - flux;
-    t = t - (flux * 0.5f) * (ROW(I.b.s_theta, e1)[k] + ROW(I.b.s_theta, e0)[k]);
-  }
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      const float flux = (((I.m.csign[c * NE + s] * dts) * I.m.dv[e[s]]) * rup[s]) * ia;
+      // theta at the edge: (theta[cell1] + theta[cell0]) in that order
+      const bool first_cell = I.m.csign[c * NE + s] > 0.0f;
+      const float th0 = first_cell ? sthc : sto[s];
+      const float th1 = first_cell ? sto[s] : sthc;
+      r = r - flux;
+      t = t - (flux * 0.5f) * (th1 + th0);
+    }
   const float rdzw = I.z.rdzw[k];
   const float coftz = ROW(I.b.coftz, c)[k];
-  const float coftz_p = ROW(I.b.coftz, c)[kp];
+  const float coftz_p = dn1(coftz);
   const float ewm = I.z.ewm[k], ewm_p = I.z.ewm[kp];
   const float rwp_p = dn1(rwp);
   const float rs = ((rpp + dts * ROW(I.b.tend_rho, c)[k]) + r) - (dts * rdzw) * (ewm_p * rwp_p - ewm * rwp);
   const float ts = ((rtpp + dts * ROW(I.b.tend_rt, c)[k]) + t) -
                    (dts * rdzw) * ((ewm_p * coftz_p) * rwp_p - (ewm * coftz) * rwp);
   if (k >= 1 && k < L) wwa = wwa + ewm * rwp;
-  // right-hand side of the implicit w equation
   const float *zzr = ROW(I.z.zz, c);
   const float zk = zzr[k];
   const float etp = I.z.etp[k], etm = I.z.etm[k];
@@ -525,23 +653,23 @@ KB k_ac_cell(const Instance I, float dts, int first) {
     x = x + cofwt_m * (etp_m * ts_m + etm_m * rtpp_m);
     col = col + dts * x;
   }
-  // Thomas sweep in level order (one lane per level, serial dependence)
   const float dts2 = dts * dts;
   const float a = ROW(I.b.atri, c)[k], al = ROW(I.b.alpha, c)[k], ga = ROW(I.b.gamma, c)[k];
+  const float ad = dts2 * a;
   for (int j = 1; j < L; ++j) {
     const float prev = rdl(col, j - 1);
-    if (k == j) col = (col - (dts2 * a) * prev) * al;
+    if (k == j) col = (col - ad * prev) * al;
   }
   for (int j = L - 1; j >= 0; --j) {
     const float next = rdl(col, j + 1);
     if (k == j) col = col - ga * next;
   }
-  // Rayleigh damping of w
   if (k >= 1 && k < L) {
     const float delta = ROW(I.b.s_rw, c)[k] - ROW(I.b.c_rw, c)[k];
     const float damp = ROW(I.z.dss, c)[k];
-    const float zi = I.z.fzm[k] * zk + I.z.fzp[k] * zzr[k - 1];
-    const float ri = I.z.fzm[k] * ROW(I.b.c_rho, c)[k] + I.z.fzp[k] * ROW(I.b.c_rho, c)[k - 1];
+    const float zi = I.z.fzm[k] * zk + I.z.fzp[k] * zk_m;
+    const float crho = ROW(I.b.c_rho, c)[k];
+    const float ri = I.z.fzm[k] * crho + I.z.fzp[k] * ROW(I.b.c_rho, c)[k - 1];
     col = (col + delta - (((dts * damp) * zi) * ri) * ROW(I.b.w, c)[k]) / (1.0f + dts * damp) - delta;
   }
   if (k >= 1 && k < L) wwa = wwa + I.z.ewp[k] * col;
@@ -565,37 +693,56 @@ KB k_divdamp(const Instance I, float coefficient) {
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
-  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  const float d0 = ROW(I.b.rtheta_pp, c0)[k] - ROW(I.b.rtheta_pp_old, c0)[k];
-  const float d1 = ROW(I.b.rtheta_pp, c1)[k] - ROW(I.b.rtheta_pp_old, c1)[k];
-  const float den = ROW(I.b.s_theta, c0)[k] + ROW(I.b.s_theta, c1)[k];
   float *rp = ROW(I.b.ru_p, e);
-  rp[k] = rp[k] + ((coefficient * (d0 - d1)) * (1.0f - 0.0f)) / den;
+  rp[k] = damped_ru_p(I, e, k, rp[k], coefficient);
 }
 
 // --------------------------------------------------------------------- recovery
 // w from rw and the terrain-following horizontal flux (terrain.py:491-571).
-// `ru_of(e, k)` must return the edge momentum at level k.
-template <class RU>
-__device__ __forceinline__ float recover_w_column(const Instance &I, int c, int k, float rho_k, float rw_k, RU ru_of) {
-  const int L = I.n.L, ME = I.n.ME;
+// ru_row(e) returns lane k's value of the edge momentum row of edge e.
+template <int NE, class RU>
+__device__ __forceinline__ float recover_w_column(const Instance &I, int c, int k, float rho_k, float rw_k, RU ru_row,
+                                                  float *hdiv_out = nullptr) {
+  const int L = I.n.L;
   const float *zzr = ROW(I.z.zz, c);
   const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
   const int km = cl(k - 1);
+  const int n = I.m.neoc[c];
+  int e[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  float ru[NE], zb[NE], zb3[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    ru[s] = ru_row(e[s]);
+    const size_t zi = ((size_t)c * NE + s) * HX_ROW + k;
+    zb[s] = I.z.zb[zi];
+    zb3[s] = I.z.zb3[zi];
+  }
+  if (hdiv_out) {
+    // mass-flux divergence of the recovered ru (dynamics.py:86-121), next stage's hdiv
+    float acc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) acc = acc + (I.m.csign[c * NE + s] * I.m.dv[e[s]]) * ru[s];
+    *hdiv_out = acc * I.m.invarea[c];
+  }
   float col = 0.0f;
   if (k >= 1 && k < L) col = rw_k / (fzm * zzr[k] + fzp * zzr[km]);
-  const int n = I.m.neoc[c];
   const float cf1 = I.z.cf1, cf2 = I.z.cf2, cf3 = I.z.cf3;
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    const float sg = I.m.csign[c * ME + s];
-    const size_t zi = ((size_t)c * ME + s) * HX_ROW + k;
-    if (k == 0) {
-      const float flux = (cf1 * ru_of(e, 0) + cf2 * ru_of(e, 1)) + cf3 * ru_of(e, 2);
-      col = col + (sg * (I.z.zb[zi] + cs1(flux) * I.z.zb3[zi])) * flux;
-    } else if (k < L) {
-      const float flux = fzm * ru_of(e, k) + fzp * ru_of(e, k - 1);
-      col = col + (sg * (I.z.zb[zi] + cs1(flux) * I.z.zb3[zi])) * flux;
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    const float ru_m = up1(ru[s]);
+    const float ru0 = rdl(ru[s], 0), ru1 = rdl(ru[s], 1), ru2 = rdl(ru[s], 2);
+    if (s < n) {
+      const float sg = I.m.csign[c * NE + s];
+      if (k == 0) {
+        const float flux = (cf1 * ru0 + cf2 * ru1) + cf3 * ru2;
+        col = col + (sg * (zb[s] + cs1(flux) * zb3[s])) * flux;
+      } else if (k < L) {
+        const float flux = fzm * ru[s] + fzp * ru_m;
+        col = col + (sg * (zb[s] + cs1(flux) * zb3[s])) * flux;
+      }
     }
   }
   const float r0 = rdl(rho_k, 0), r1 = rdl(rho_k, 1), r2 = rdl(rho_k, 2);
@@ -611,6 +758,7 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
   return col;
 }
 
+template <int NE>
 KB k_rec_cell(const Instance I, int stage, int sub, float inv_steps) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
@@ -646,22 +794,28 @@ KB k_rec_cell(const Instance I, int stage, int sub, float inv_steps) {
     ROW(I.b.flux_w, c)[k] = fw;
     if (stage == 3) ROW(I.b.split_w, c)[k] = sub == 1 ? fw : fw + ROW(I.b.split_w, c)[k];
   }
-  const float *sru = I.b.s_ru, *rup = I.b.ru_p;
-  const float wv = recover_w_column(I, c, k, rho, rwv, [&](int e, int kk) {
-    return ROW(sru, e)[kk] + ROW(rup, e)[kk];
-  });
+  const float *cru = I.b.c_ru;
+  float hd = 0.0f;
+  const float wv = recover_w_column<NE>(I, c, k, rho, rwv, [&](int e) { return ROW(cru, e)[k]; }, &hd);
   if (k <= L) ROW(I.b.w, c)[k] = wv;
+  if (k < L) ROW(I.b.hdiv, c)[k] = hd;
   if (bad) set_flag(I.b.flag);
 }
 
-KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps) {
+// damp: apply the last substep's divergence damping to ru_p first
+KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps, int damp, float damp_coef) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const float sru = ROW(I.b.s_ru, e)[k];
-  const float ru = sru + ROW(I.b.ru_p, e)[k];
+  float rp = ROW(I.b.ru_p, e)[k];
+  if (damp) {
+    rp = damped_ru_p(I, e, k, rp, damp_coef);
+    ROW(I.b.ru_p, e)[k] = rp;
+  }
+  const float ru = sru + rp;
   ROW(I.b.c_ru, e)[k] = ru;
   const float fu = sru + ROW(I.b.ru_avg, e)[k] * inv_steps;
   ROW(I.b.flux_u, e)[k] = fu;
@@ -672,6 +826,7 @@ KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps) {
 }
 
 // rebuild of the saved diagnostics from a prognostic state (driver.py:2655-2700)
+template <int NE>
 KB k_rebuild_cell(const Instance I) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
@@ -692,7 +847,7 @@ KB k_rebuild_cell(const Instance I) {
   }
   const float rwk = ROW(I.b.rw, c)[k];
   const float *ru = I.b.ru;
-  const float wv = recover_w_column(I, c, k, rho, rwk, [&](int e, int kk) { return ROW(ru, e)[kk]; });
+  const float wv = recover_w_column<NE>(I, c, k, rho, rwk, [&](int e) { return ROW(ru, e)[k]; });
   if (k <= L) ROW(I.b.w, c)[k] = wv;
 }
 
@@ -721,43 +876,58 @@ KB k_split_scale(const Instance I, int n, float *p, float inv) {
   ROW(p, e)[k] = ROW(p, e)[k] * inv;
 }
 
+template <int NE>
 KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__ ev) {
+  constexpr int NX = 2 * NE - 2;
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
-  const int MA = I.n.MA;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   const int sp = blockIdx.y;
   const float *q = qs + sp * cplane;
+  int ac[NX];
+  float qv[NX];
+#pragma unroll
+  for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
+#pragma unroll
+  for (int s = 0; s < NX; ++s) qv[s] = ROW(q, ac[s])[k];
   const float sv = cs1(ROW(I.b.split_u, e)[k]);
   const float coef = I.k.coef3;
-  float r = 0.0f;
   const int na = I.m.nadv[e];
-  for (int s = 0; s < na; ++s) {
-    const int j = e * MA + s;
-    r = r + (I.m.adv[j] + (coef * sv) * I.m.adv3[j]) * ROW(q, I.m.advc[j])[k];
-  }
+  float r = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NX; ++s)
+    if (s < na) r = r + (I.m.adv[e * NX + s] + (coef * sv) * I.m.adv3[e * NX + s]) * qv[s];
   if (k < I.n.L) ROW(ev + sp * eplane, e)[k] = r;
 }
 
+template <int NE>
 KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__restrict__ ev, float *__restrict__ out,
              float dt, float wn) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L, ME = I.n.ME;
+  const int L = I.n.L;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   const int sp = blockIdx.y;
   const float *q = ROW(qs + sp * cplane, c);
+  const int n = I.m.neoc[c];
+  int e[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  float uh[NE], evv[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    uh[s] = ROW(I.b.split_u, e[s])[k];
+    evv[s] = ROW(ev + sp * eplane, e[s])[k];
+  }
   float vf = 0.0f;
   if (k >= 1 && k < L) vf = vflux(q, ROW(I.b.split_w, c)[k], k, L, I.z.fzm[k], I.z.fzp[k], I.k.coef3);
   const float vf_p = dn1(vf);
   float tc = 0.0f;
-  const int n = I.m.neoc[c];
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    tc = tc + ((-I.m.csign[c * ME + s]) * ROW(I.b.split_u, e)[k]) * ROW(ev + sp * eplane, e)[k];
-  }
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) tc = tc + ((-I.m.csign[c * NE + s]) * uh[s]) * evv[s];
   tc = tc * I.m.invarea[c] + 0.0f;
   if (k < L) {
     const float orho = ROW(I.b.rho, c)[k];
@@ -768,18 +938,24 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
   }
 }
 
+template <int NE>
 KB k_fct_target(const Instance I, float dt) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L, ME = I.n.ME;
+  const int L = I.n.L;
   const float ia = I.m.invarea[c];
-  float hd = 0.0f;
   const int n = I.m.neoc[c];
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    hd = hd + (((-I.m.csign[c * ME + s]) * ROW(I.b.split_u, e)[k]) * I.m.dv[e]) * ia;
-  }
+  int e[NE];
+  float uh[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) uh[s] = ROW(I.b.split_u, e[s])[k];
+  float hd = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) hd = hd + (((-I.m.csign[c * NE + s]) * uh[s]) * I.m.dv[e[s]]) * ia;
   const float *wa = ROW(I.b.split_w, c);
   if (k < L) {
     const float t = ROW(I.b.rho, c)[k] + dt * (hd - I.z.rdzw[k] * (wa[k + 1] - wa[k]));
@@ -788,37 +964,41 @@ KB k_fct_target(const Instance I, float dt) {
   }
 }
 
+template <int NE>
 KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
+  constexpr int NX = 2 * NE - 2;
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
-  const int MA = I.n.MA;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   const int sp = blockIdx.y;
   const float *q = qs + sp * cplane;
   const float *qold = I.b.q + sp * cplane;
+  int ac[NX];
+  float qv[NX];
+#pragma unroll
+  for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
+#pragma unroll
+  for (int s = 0; s < NX; ++s) qv[s] = ROW(q, ac[s])[k];
+  const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
+  const float q1 = ROW(qold, c1)[k] + 0.0f, q2 = ROW(qold, c2)[k] + 0.0f;
   const float uu = ROW(I.b.split_u, e)[k];
   const float sv = cs1(uu);
   const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
   float r;
   if (na == 10) {
-    const int j0 = e * MA;
-    float val = (I.m.adv[j0] + (coef * sv) * I.m.adv3[j0]) * ROW(q, I.m.advc[j0])[k];
-    for (int s = 1; s < 10; ++s) {
-      const int j = e * MA + s;
-      val = val + (I.m.adv[j] + (coef * sv) * I.m.adv3[j]) * ROW(q, I.m.advc[j])[k];
-    }
+    float val = (I.m.adv[e * NX] + (coef * sv) * I.m.adv3[e * NX]) * qv[0];
+#pragma unroll
+    for (int s = 1; s < NX; ++s)
+      if (s < 10) val = val + (I.m.adv[e * NX + s] + (coef * sv) * I.m.adv3[e * NX + s]) * qv[s];
     r = uu * val;
   } else {
     r = 0.0f;
-    for (int s = 0; s < na; ++s) {
-      const int j = e * MA + s;
-      r = r + (uu * (I.m.adv[j] + (coef * sv) * I.m.adv3[j])) * ROW(q, I.m.advc[j])[k];
-    }
+#pragma unroll
+    for (int s = 0; s < NX; ++s)
+      if (s < na) r = r + (uu * (I.m.adv[e * NX + s] + (coef * sv) * I.m.adv3[e * NX + s])) * qv[s];
   }
-  const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
-  const float q1 = ROW(qold, c1)[k] + 0.0f, q2 = ROW(qold, c2)[k] + 0.0f;
   const float up = (I.m.dv[e] * dt) * (max0(uu) * q1 + min0(uu) * q2);
   if (k < I.n.L) {
     ROW(I.b.fct_upw + sp * eplane, e)[k] = up;
@@ -826,18 +1006,32 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
   }
 }
 
+template <int NE>
 KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L, ME = I.n.ME;
+  const int L = I.n.L;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   const int sp = blockIdx.y;
   const float *qsr = ROW(qs + sp * cplane, c);
   const float *qold = I.b.q + sp * cplane;
+  const int n = I.m.neoc[c];
+  int nb[NE], e[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    nb[s] = I.m.coc[c * NE + s];
+    e[s] = I.m.eoc[c * NE + s];
+  }
+  float qn[NE], upw[NE], hres[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    qn[s] = ROW(qold, nb[s])[k] + 0.0f;
+    upw[s] = ROW(I.b.fct_upw + sp * eplane, e[s])[k];
+    hres[s] = ROW(I.b.fct_hres + sp * eplane, e[s])[k];
+  }
   const float qo = ROW(qold, c)[k] + 0.0f;
   const float qo_m = up1(qo), qo_p = dn1(qo);
-  // bounds over k-1..k+1 (clamped) in level order, then the neighbours
   float a, b;
   if (k == 0) {
     a = qo; b = qo;
@@ -850,12 +1044,12 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
     a = pmin(a, qo); b = pmax(b, qo);
     a = pmin(a, qo_p); b = pmax(b, qo_p);
   }
-  const int n = I.m.neoc[c];
-  for (int s = 0; s < n; ++s) {
-    const float qn = ROW(qold, I.m.coc[c * ME + s])[k] + 0.0f;
-    a = pmin(a, qn);
-    b = pmax(b, qn);
-  }
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      a = pmin(a, qn[s]);
+      b = pmax(b, qn[s]);
+    }
   const float orho = ROW(I.b.rho, c)[k];
   float mass = qo * orho;
   const float wk = ROW(I.b.split_w, c)[k];
@@ -878,14 +1072,15 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
   float sin_ = (-rdzw) * (min0(vres_p) - max0(vres));
   float sout = (-rdzw) * (max0(vres_p) - min0(vres));
   const float ia = I.m.invarea[c];
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    const float so = -I.m.csign[c * ME + s];
-    mass = mass + (so * ROW(I.b.fct_upw + sp * eplane, e)[k]) * ia;
-    const float sr = (-so) * ROW(I.b.fct_hres + sp * eplane, e)[k];
-    sout = sout - max0(sr) * ia;
-    sin_ = sin_ - min0(sr) * ia;
-  }
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      const float so = -I.m.csign[c * NE + s];
+      mass = mass + (so * upw[s]) * ia;
+      const float sr = (-so) * hres[s];
+      sout = sout - max0(sr) * ia;
+      sin_ = sin_ - min0(sr) * ia;
+    }
   const float target = ROW(I.b.target, c)[k];
   float f = (b * target - mass) / (sin_ + 1.0e-20f);
   sin_ = pmin(1.0f, max0(f));
@@ -913,13 +1108,21 @@ KB k_fct_edge2(const Instance I) {
   h[k] = max0(f) * pmin(ROW(so, c1)[k], ROW(si, c2)[k]) + min0(f) * pmin(ROW(si, c1)[k], ROW(so, c2)[k]);
 }
 
+template <int NE>
 KB k_fct_cell2(const Instance I, float *__restrict__ out) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L, ME = I.n.ME;
+  const int L = I.n.L;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   const int sp = blockIdx.y;
+  const int n = I.m.neoc[c];
+  int e[NE];
+  float hres[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) hres[s] = ROW(I.b.fct_hres + sp * eplane, e[s])[k];
   const float si = ROW(I.b.fct_sin + sp * cplane, c)[k], so = ROW(I.b.fct_sout + sp * cplane, c)[k];
   const float si_m = up1(si), so_m = up1(so);
   float v = ROW(I.b.fct_vres + sp * cplane, c)[k];
@@ -927,12 +1130,12 @@ KB k_fct_cell2(const Instance I, float *__restrict__ out) {
   const float v_p = dn1(v);
   float mass = ROW(I.b.fct_mass + sp * cplane, c)[k];
   const float ia = I.m.invarea[c];
-  const int n = I.m.neoc[c];
-  for (int s = 0; s < n; ++s) {
-    const int e = I.m.eoc[c * ME + s];
-    const float sgn = -I.m.csign[c * ME + s];
-    mass = mass + (sgn * ROW(I.b.fct_hres + sp * eplane, e)[k]) * ia;
-  }
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      const float sgn = -I.m.csign[c * NE + s];
+      mass = mass + (sgn * hres[s]) * ia;
+    }
   mass = mass - I.z.rdzw[k] * (v_p - v);
   if (k < L) ROW(out + sp * cplane, c)[k] = max0(mass / ROW(I.b.target, c)[k]);
 }
@@ -962,10 +1165,22 @@ inline int check() {
 }  // namespace
 
 // ------------------------------------------------------------------ phases
+// NE: maxEdges of the mesh (6 or 7); kernels are instantiated for both.
+#define NEDISPATCH(I, call6, call7) \
+  do {                              \
+    if ((I).n.ME == 6) {            \
+      call6;                        \
+    } else {                        \
+      call7;                        \
+    }                               \
+  } while (0)
+#define LAUNCH_NE(I, kernel, ...) \
+  NEDISPATCH(I, launch_named(#kernel, kernel<6>, __VA_ARGS__), launch_named(#kernel, kernel<7>, __VA_ARGS__))
+
 int solve_diagnostics(const Instance &I, int rk, hipStream_t s) {
   launch(k_diag_vertex, I.n.V, 1, s, I, (const float *)I.b.u);
-  launch(k_diag_cell, I.n.C, 1, s, I, (const float *)I.b.u);
-  launch(k_diag_edge, I.n.E, 1, s, I, (const float *)I.b.u, rk == 3 ? 1 : 0);
+  LAUNCH_NE(I, k_diag_cell, I.n.C, 1, s, I, (const float *)I.b.u);
+  LAUNCH_NE(I, k_diag_edge, I.n.E, 1, s, I, (const float *)I.b.u, rk == 3 ? 1 : 0);
   return check();
 }
 
@@ -979,7 +1194,7 @@ int begin_step(const Instance &I, hipStream_t s) {
 
 int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
   (void)sub;
-  launch(k_sub_cell, I.n.C, 1, s, I);
+  LAUNCH_NE(I, k_sub_cell, I.n.C, 1, s, I);
   launch(k_sub_edge, I.n.E, 1, s, I);
   launch(k_vcoef, I.n.C, 1, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
          (const float *)I.b.s_rtheta_p);
@@ -990,16 +1205,16 @@ int dyn_tend(const Instance &I, int stage, hipStream_t s) {
   if (stage == 2 && I.k.order == 3)
     launch(k_vcoef, I.n.C, 1, s, I, I.k.dts[1], (const float *)I.b.s_exner, (const float *)I.b.theta,
            (const float *)I.b.rtheta_p);
-  launch(k_hdiv, I.n.C, 1, s, I);
-  launch(k_tend_edge, I.n.E, 1, s, I, stage);
-  launch(k_tend_cell, I.n.C, 1, s, I, stage);
+  LAUNCH_NE(I, k_tend_mom, I.n.E, 1, s, I);
+  LAUNCH_NE(I, k_tend_flux, I.n.E, 1, s, I, stage);
+  LAUNCH_NE(I, k_tend_cell, I.n.C, 1, s, I, stage);
   return check();
 }
 
-int acoustic_substep(const Instance &I, int stage, int ss, hipStream_t s) {
+int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStream_t s) {
   const float dts = I.k.dts[stage - 1];
-  launch(k_ac_edge, I.n.E, 1, s, I, dts, ss == 1 ? 1 : 0);
-  launch(k_ac_cell, I.n.C, 1, s, I, dts, ss == 1 ? 1 : 0);
+  launch(k_ac_edge, I.n.E, 1, s, I, dts, ss == 1 ? 1 : 0, damp_prev, I.k.dts_inv_coef[stage - 1]);
+  LAUNCH_NE(I, k_ac_cell, I.n.C, 1, s, I, dts, ss == 1 ? 1 : 0);
   return check();
 }
 
@@ -1008,10 +1223,10 @@ int divergence_damping(const Instance &I, int stage, hipStream_t s) {
   return check();
 }
 
-int recover(const Instance &I, int stage, int sub, hipStream_t s) {
+int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
   const float inv = I.k.inv_steps[stage - 1];
-  launch(k_rec_cell, I.n.C, 1, s, I, stage, sub, inv);
-  launch(k_rec_edge, I.n.E, 1, s, I, stage, sub, inv);
+  launch(k_rec_edge, I.n.E, 1, s, I, stage, sub, inv, damp, I.k.dts_inv_coef[stage - 1]);
+  LAUNCH_NE(I, k_rec_cell, I.n.C, 1, s, I, stage, sub, inv);
   return check();
 }
 
@@ -1023,21 +1238,21 @@ int scalars_stage(const Instance &I, int stage, hipStream_t s) {
     launch(k_split_scale, I.n.C, 1, s, I, I.n.C, I.b.split_w, I.k.split_inv);
   }
   if (stage < 3 || !I.k.monotonic) {
-    launch(k_sc_edge, I.n.E, NT, s, I, (const float *)I.b.q_stage, I.b.ev);
-    launch(k_sc_cell, I.n.C, NT, s, I, (const float *)I.b.q_stage, (const float *)I.b.ev, I.b.q_next, dt,
-           I.k.wn[stage - 1]);
+    LAUNCH_NE(I, k_sc_edge, I.n.E, NT, s, I, (const float *)I.b.q_stage, I.b.ev);
+    LAUNCH_NE(I, k_sc_cell, I.n.C, NT, s, I, (const float *)I.b.q_stage, (const float *)I.b.ev, I.b.q_next, dt,
+              I.k.wn[stage - 1]);
     return check();
   }
-  launch(k_fct_target, I.n.C, 1, s, I, dt);
-  launch(k_fct_edge1, I.n.E, NT, s, I, (const float *)I.b.q_stage, dt);
-  launch(k_fct_cell1, I.n.C, NT, s, I, (const float *)I.b.q_stage, dt);
+  LAUNCH_NE(I, k_fct_target, I.n.C, 1, s, I, dt);
+  LAUNCH_NE(I, k_fct_edge1, I.n.E, NT, s, I, (const float *)I.b.q_stage, dt);
+  LAUNCH_NE(I, k_fct_cell1, I.n.C, NT, s, I, (const float *)I.b.q_stage, dt);
   launch(k_fct_edge2, I.n.E, NT, s, I);
-  launch(k_fct_cell2, I.n.C, NT, s, I, I.b.q_next);
+  LAUNCH_NE(I, k_fct_cell2, I.n.C, NT, s, I, I.b.q_next);
   return check();
 }
 
 int rebuild_saved(const Instance &I, hipStream_t s) {
-  launch(k_rebuild_cell, I.n.C, 1, s, I);
+  LAUNCH_NE(I, k_rebuild_cell, I.n.C, 1, s, I);
   launch(k_rebuild_edge, I.n.E, 1, s, I);
   return check();
 }

@@ -59,10 +59,12 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     "k_sub_cell": [("C", "L")] * 12 + [("C", "W")] * 3 + [("E", "L")],
     "k_sub_edge": [("E", "L")] * 5 + [("C", "L")] * 3,
     "k_hdiv": [("E", "L"), ("C", "L")],
-    # u, c_rw, pv_edge, c_rho, ke, hdiv, euler_ru, c_ru, theta, w, s_ru, s_theta | tend_ru, fth, fcorr, fw
-    "k_tend_edge": [("E", "L"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
-                    ("E", "L"), ("C", "L"), ("C", "W"), ("E", "L"), ("C", "L"),
-                    ("E", "L"), ("E", "L"), ("E", "L"), ("E", "W")],
+    # u, c_rw, pv_edge, c_rho, ke, hdiv, euler_ru | tend_ru
+    "k_tend_mom": [("E", "L"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
+                   ("E", "L")],
+    # c_ru, theta, w, s_ru, s_theta | fth, fcorr, fw   (s_ru, s_theta, fcorr at stages 2-3)
+    "k_tend_flux": [("E", "L"), ("C", "L"), ("C", "W"), ("E", "L"), ("C", "L"),
+                    ("E", "L"), ("E", "L"), ("E", "W")],
     # theta, s_theta, c_rw, s_rw, w, fth, fcorr, fw, euler_rw, tend_ru, zb, zb3, zz | tend_rt, tend_rw
     "k_tend_cell": [("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("C", "W"), ("E", "L"), ("E", "L"),
                     ("E", "W"), ("C", "W"), ("E", "L"), ("CM", "W"), ("CM", "W"), ("C", "L"),

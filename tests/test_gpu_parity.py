@@ -169,22 +169,28 @@ def test_validation_flag_refuses_and_keeps_state(get_case):
     case = get_case("x1.642-dry")
     d = _driver(case)
     before = d.get_state()
-    bad = before["rtheta_m"].copy()
+    # the carried theta_m feeds the theta flux; a NaN there must trip the flag
+    bad = before["theta_m"].copy()
     bad[3, 7] = np.nan
     t = torch.from_numpy(bad).cuda()
     from hexdyn import capi
     from hexdyn.lib import lib
     import ctypes as ct
 
-    rc = lib().hexdyn_import_field(d.handle, capi.FIELD_ID["RTHETA_M"], ct.c_void_p(t.data_ptr()),
+    rc = lib().hexdyn_import_field(d.handle, capi.FIELD_ID["THETA_M"], ct.c_void_p(t.data_ptr()),
                                    ct.c_void_p(d.stream_ptr()))
     assert rc == 0
     with pytest.raises(FloatingPointError):
         d.step_device(1)
     after = d.get_state()
-    assert _bits_equal(after["rho_zz"], before["rho_zz"])
-    assert _bits_equal(after["ru"], before["ru"])
-    assert np.isnan(after["rtheta_m"][3, 7])
+    for k in ("rho_zz", "rtheta_m", "ru", "rw", "exner", "u", "w"):
+        assert _bits_equal(after[k], before[k]), k
+    assert np.isnan(after["theta_m"][3, 7])
+    # the published state is intact: with the carried theta restored the step runs
+    tt = torch.from_numpy(before["theta_m"].copy()).cuda()
+    assert lib().hexdyn_import_field(d.handle, capi.FIELD_ID["THETA_M"], ct.c_void_p(tt.data_ptr()),
+                                     ct.c_void_p(d.stream_ptr())) == 0
+    d.step_device(1)
 
 
 def test_determinism_and_restart(get_case):
