@@ -339,8 +339,35 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
       kite[3 * nv + j] = m->kite_areas_on_vertex[3 * vv + j];
     }
   }
+  // per-slot copies of edge/vertex geometry
+  std::vector<float> csdv((size_t)C * ME, 0.0f), cdcdv((size_t)C * ME, 0.0f), ckite((size_t)C * ME, 0.0f),
+      vdc(3 * V), vdcdv(3 * V), ecos((size_t)E * ME2, 0.0f), esin((size_t)E * ME2, 0.0f);
+  std::vector<int> ocell((size_t)C * ME, 0);
+  for (int nc = 0; nc < C; ++nc)
+    for (int s = 0; s < neoc[nc]; ++s) {
+      const size_t o = (size_t)nc * ME + s;
+      const int a0 = coe[2 * eoc[o]], a1 = coe[2 * eoc[o] + 1];
+      ocell[o] = a0 == nc ? a1 : a0;
+      csdv[o] = csign[o] * dv[eoc[o]];
+      cdcdv[o] = dcdv[eoc[o]];
+      ckite[o] = kite[3 * voc[o] + kfc[o]];
+    }
+  for (int nv = 0; nv < V; ++nv)
+    for (int j = 0; j < 3; ++j) {
+      vdc[3 * nv + j] = vsign[3 * nv + j] * dc[eov[3 * nv + j]];
+      vdcdv[3 * nv + j] = dcdv[eov[3 * nv + j]];
+    }
+  for (int ne = 0; ne < E; ++ne)
+    for (int s = 0; s < neoe[ne]; ++s) {
+      const size_t o = (size_t)ne * ME2 + s;
+      ecos[o] = cosang[eoe[o]];
+      esin[o] = sinang[eoe[o]];
+    }
   Mesh &M = h->I.m;
   int rc = 0;
+  rc |= h->upload(&M.csdv, csdv); rc |= h->upload(&M.cdcdv, cdcdv); rc |= h->upload(&M.ckite, ckite);
+  rc |= h->upload(&M.vdc, vdc); rc |= h->upload(&M.vdcdv, vdcdv);
+  rc |= h->upload(&M.ecos, ecos); rc |= h->upload(&M.esin, esin); rc |= h->upload(&M.ocell, ocell);
   rc |= h->upload(&M.coe, coe); rc |= h->upload(&M.voe, voe); rc |= h->upload(&M.eoc, eoc);
   rc |= h->upload(&M.coc, coc); rc |= h->upload(&M.voc, voc); rc |= h->upload(&M.neoc, neoc);
   rc |= h->upload(&M.kfc, kfc); rc |= h->upload(&M.eov, eov); rc |= h->upload(&M.eoe, eoe);

@@ -45,6 +45,16 @@ struct Mesh {
   const float *kite;    // [V][3]
   const float *fedge, *fvert, *cosang, *sinang;
   const float *adv, *adv3;  // [E][MA]
+  // per-slot copies of edge geometry, so a kernel reads them with the
+  // connectivity instead of through it (csign and vsign are +-1, so the
+  // signed products are exact)
+  const float *csdv;    // [C][ME] csign * dv[edge]
+  const float *cdcdv;   // [C][ME] dcdv[edge]
+  const float *ckite;   // [C][ME] kite area of the cell's slot vertex
+  const float *vdc;     // [V][3]  vsign * dc[edge]
+  const float *vdcdv;   // [V][3]  dcdv[edge]
+  const float *ecos, *esin;  // [E][ME2] cos/sin angle of edgesOnEdge
+  const int *ocell;     // [C][ME] the other cell of the slot's edge
 };
 
 struct Vert {

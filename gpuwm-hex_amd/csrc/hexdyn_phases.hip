@@ -72,6 +72,15 @@ __device__ __forceinline__ float flux3(float qm2, float qm1, float q0, float qp1
 
 __device__ __forceinline__ void set_flag(int *flag) { atomicOr(flag, 1); }
 
+// Wave-uniform per-slot coefficients, loaded unconditionally (tables are
+// padded, so every slot is addressable) so the scalar loads issue together
+// instead of one round trip per slot under the slot-count test.
+template <int N, typename T>
+__device__ __forceinline__ void slots(T (&out)[N], const T *__restrict__ p) {
+#pragma unroll
+  for (int s = 0; s < N; ++s) out[s] = p[s];
+}
+
 inline int nblocks(int n) { return (n + 3) / 4; }
 
 // Gather helpers.  Connectivity rows are padded to fixed widths (NE = maxEdges
@@ -95,8 +104,8 @@ KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
   float vort = 0.0f, kev = 0.0f;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    vort = vort + (I.m.vsign[3 * v + s] * I.m.dc[e[s]]) * ue[s];
-    kev = kev + I.m.dcdv[e[s]] * (ue[s] * ue[s]);
+    vort = vort + I.m.vdc[3 * v + s] * ue[s];
+    kev = kev + I.m.vdcdv[3 * v + s] * (ue[s] * ue[s]);
   }
   const float it = I.m.invtri[v];
   vort = vort * it;
@@ -115,13 +124,12 @@ KB k_diag_cell(const Instance I, const float *__restrict__ u) {
   const int k = lane();
   const int n = I.m.neoc[c];
   const float ia = I.m.invarea[c];
-  int e[NE], vv[NE], ks[NE];
+  int e[NE], vv[NE];
   float ue[NE], kv[NE], pv[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     e[s] = I.m.eoc[c * NE + s];
     vv[s] = I.m.voc[c * NE + s];
-    ks[s] = I.m.kfc[c * NE + s];
   }
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
@@ -129,12 +137,16 @@ KB k_diag_cell(const Instance I, const float *__restrict__ u) {
     kv[s] = ROW(I.b.ke_vertex, vv[s])[k];
     pv[s] = ROW(I.b.pv_vertex, vv[s])[k];
   }
+  float sd[NE], dd[NE], kt[NE];
+  slots(sd, I.m.csdv + c * NE);
+  slots(dd, I.m.cdcdv + c * NE);
+  slots(kt, I.m.ckite + c * NE);
   float d = 0.0f, kc = 0.0f, pvc = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) {
-      d = d + (I.m.csign[c * NE + s] * I.m.dv[e[s]]) * ue[s];
-      kc = kc + 0.25f * (I.m.dcdv[e[s]] * (ue[s] * ue[s]));
+      d = d + sd[s] * ue[s];
+      kc = kc + 0.25f * (dd[s] * (ue[s] * ue[s]));
     }
   d = d * ia;
   kc = kc * ia;
@@ -142,7 +154,7 @@ KB k_diag_cell(const Instance I, const float *__restrict__ u) {
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) {
-      const float kite = I.m.kite[3 * vv[s] + ks[s]];
+      const float kite = kt[s];
       float corr = 0.375f * kite;
       corr = corr * kv[s];
       corr = corr * ia;
@@ -171,10 +183,12 @@ KB k_diag_edge(const Instance I, const float *__restrict__ u, int recon) {
 #pragma unroll
     for (int s = 0; s < NX; ++s) un[s] = ROW(u, nb[s])[k];
     const int n = I.m.neoe[e];
+    float wt[NX];
+    slots(wt, I.m.woe + e * NX);
     vv = 0.0f;
 #pragma unroll
     for (int s = 0; s < NX; ++s)
-      if (s < n) vv = vv + I.m.woe[e * NX + s] * un[s];
+      if (s < n) vv = vv + wt[s] * un[s];
   } else {
     vv = ROW(I.b.v, e)[k];
   }
@@ -297,10 +311,12 @@ __device__ __forceinline__ float cell_flux_div(const Instance &I, int c, int k, 
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
 #pragma unroll
   for (int s = 0; s < NE; ++s) xv[s] = ROW(x, e[s])[k];
+  float sd[NE];
+  slots(sd, I.m.csdv + c * NE);
   float acc = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
-    if (s < n) acc = acc + (I.m.csign[c * NE + s] * I.m.dv[e[s]]) * xv[s];
+    if (s < n) acc = acc + sd[s] * xv[s];
   return acc * I.m.invarea[c];
 }
 
@@ -397,19 +413,23 @@ KB k_tend_mom(const Instance I) {
   const float wduz_p = dn1(wduz);
   const float initial = (-I.z.rdzw[k]) * (wduz_p - wduz);
   const int nn = I.m.neoe[e];
+  float wt[NX], ca[NX], sa[NX];
+  slots(wt, I.m.woe + e * NX);
+  slots(ca, I.m.ecos + e * NX);
+  slots(sa, I.m.esin + e * NX);
   float qq = 0.0f;
 #pragma unroll
   for (int s = 0; s < NX; ++s)
     if (s < nn) {
       const float work_pv = 0.5f * (pve + pvn[s]);
-      qq = qq + (I.m.woe[e * NX + s] * un[s]) * work_pv;
+      qq = qq + (wt[s] * un[s]) * work_pv;
     }
   const float ui = I.z.uinit[k], vi = I.z.vinit[k], fe = I.m.fedge[e];
 #pragma unroll
   for (int s = 0; s < NX; ++s)
     if (s < nn) {
-      const float ref_u = ui * I.m.cosang[nb[s]] - vi * I.m.sinang[nb[s]];
-      qq = qq - (I.m.woe[e * NX + s] * ref_u) * fe;
+      const float ref_u = ui * ca[s] - vi * sa[s];
+      qq = qq - (wt[s] * ref_u) * fe;
     }
   const float rho_edge = 0.5f * (rho0 + rho1);
   const float horiz = rho_edge * (qq - (ke1 - ke0) * I.m.invdc[e]);
@@ -447,11 +467,14 @@ KB k_tend_flux(const Instance I, int stage) {
   const float rui = I.z.fzm[k] * rue + I.z.fzp[k] * rue_m;
   const float sv_t = cs1(rue), sv_w = cs1(rui);
   const int na = I.m.nadv[e];
+  float av[NX], a3v[NX];
+  slots(av, I.m.adv + e * NX);
+  slots(a3v, I.m.adv3 + e * NX);
   float et = 0.0f, ew = 0.0f;
 #pragma unroll
   for (int s = 0; s < NX; ++s)
     if (s < na) {
-      const float a = I.m.adv[e * NX + s], a3 = I.m.adv3[e * NX + s];
+      const float a = av[s], a3 = a3v[s];
       et = et + (a + (coef * sv_t) * a3) * thn[s];
       ew = ew + (a + (coef * sv_w) * a3) * wn[s];
     }
@@ -470,6 +493,8 @@ KB k_tend_cell(const Instance I, int stage) {
   const int L = I.n.L;
   const float coef = I.k.coef3;
   const int n = I.m.neoc[c];
+  float sg[NE];
+  slots(sg, I.m.csign + c * NE);
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
@@ -513,11 +538,11 @@ KB k_tend_cell(const Instance I, int stage) {
   float r = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
-    if (s < n) r = r - I.m.csign[c * NE + s] * fth[s];
+    if (s < n) r = r - sg[s] * fth[s];
   if (stage > 1) {
 #pragma unroll
     for (int s = 0; s < NE; ++s)
-      if (s < n) r = r - I.m.csign[c * NE + s] * fco[s];
+      if (s < n) r = r - sg[s] * fco[s];
   }
   r = r * ia;
   const float trt = r - I.z.rdzw[k] * (vf_p - vf);
@@ -529,7 +554,7 @@ KB k_tend_cell(const Instance I, int stage) {
     float q = 0.0f;
 #pragma unroll
     for (int s = 0; s < NE; ++s)
-      if (s < n) q = q - I.m.csign[c * NE + s] * fwv[s];
+      if (s < n) q = q - sg[s] * fwv[s];
     q = q * ia;
     q = q - I.z.rdzu[k] * (wf_p - wf);
     q = q + ROW(I.b.euler_rw, c)[k];
@@ -537,7 +562,7 @@ KB k_tend_cell(const Instance I, int stage) {
     for (int s = 0; s < NE; ++s)
       if (s < n) {
         const float tk = tru[s];
-        const float flux = I.m.csign[c * NE + s] * (fzm * tk + fzp * tru_m[s]);
+        const float flux = sg[s] * (fzm * tk + fzp * tru_m[s]);
         q = q - (zb[s] + cs1(tk) * zb3[s]) * flux;
       }
     const float *zzr = ROW(I.z.zz, c);
@@ -591,13 +616,15 @@ KB k_ac_cell(const Instance I, float dts, int first) {
   const int L = I.n.L;
   const int kp = cl(k + 1);
   const int n = I.m.neoc[c];
+  float sg[NE];
+  slots(sg, I.m.csign + c * NE);
+  float sd[NE];
+  slots(sd, I.m.csdv + c * NE);
   int e[NE], o[NE];
 #pragma unroll
-  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-#pragma unroll
   for (int s = 0; s < NE; ++s) {
-    const int a0 = I.m.coe[2 * e[s]], a1 = I.m.coe[2 * e[s] + 1];
-    o[s] = a0 == c ? a1 : a0;  // the other cell of edge s
+    e[s] = I.m.eoc[c * NE + s];
+    o[s] = I.m.ocell[c * NE + s];  // the other cell of edge s
   }
   float rup[NE], sto[NE];
 #pragma unroll
@@ -619,9 +646,9 @@ KB k_ac_cell(const Instance I, float dts, int first) {
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) {
-      const float flux = (((I.m.csign[c * NE + s] * dts) * I.m.dv[e[s]]) * rup[s]) * ia;
+      const float flux = ((dts * sd[s]) * rup[s]) * ia;  // (csign*dts)*dv, csign = +-1
       // theta at the edge: (theta[cell1] + theta[cell0]) in that order
-      const bool first_cell = I.m.csign[c * NE + s] > 0.0f;
+      const bool first_cell = sg[s] > 0.0f;
       const float th0 = first_cell ? sthc : sto[s];
       const float th1 = first_cell ? sto[s] : sthc;
       r = r - flux;
@@ -708,6 +735,10 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
   const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
   const int km = cl(k - 1);
   const int n = I.m.neoc[c];
+  float sg[NE];
+  slots(sg, I.m.csign + c * NE);
+  float sd[NE];
+  slots(sd, I.m.csdv + c * NE);
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
@@ -724,7 +755,7 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
     float acc = 0.0f;
 #pragma unroll
     for (int s = 0; s < NE; ++s)
-      if (s < n) acc = acc + (I.m.csign[c * NE + s] * I.m.dv[e[s]]) * ru[s];
+      if (s < n) acc = acc + sd[s] * ru[s];
     *hdiv_out = acc * I.m.invarea[c];
   }
   float col = 0.0f;
@@ -735,13 +766,12 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
     const float ru_m = up1(ru[s]);
     const float ru0 = rdl(ru[s], 0), ru1 = rdl(ru[s], 1), ru2 = rdl(ru[s], 2);
     if (s < n) {
-      const float sg = I.m.csign[c * NE + s];
       if (k == 0) {
         const float flux = (cf1 * ru0 + cf2 * ru1) + cf3 * ru2;
-        col = col + (sg * (zb[s] + cs1(flux) * zb3[s])) * flux;
+        col = col + (sg[s] * (zb[s] + cs1(flux) * zb3[s])) * flux;
       } else if (k < L) {
         const float flux = fzm * ru[s] + fzp * ru_m;
-        col = col + (sg * (zb[s] + cs1(flux) * zb3[s])) * flux;
+        col = col + (sg[s] * (zb[s] + cs1(flux) * zb3[s])) * flux;
       }
     }
   }
@@ -894,10 +924,13 @@ KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__
   const float sv = cs1(ROW(I.b.split_u, e)[k]);
   const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
+  float av[NX], a3v[NX];
+  slots(av, I.m.adv + e * NX);
+  slots(a3v, I.m.adv3 + e * NX);
   float r = 0.0f;
 #pragma unroll
   for (int s = 0; s < NX; ++s)
-    if (s < na) r = r + (I.m.adv[e * NX + s] + (coef * sv) * I.m.adv3[e * NX + s]) * qv[s];
+    if (s < na) r = r + (av[s] + (coef * sv) * a3v[s]) * qv[s];
   if (k < I.n.L) ROW(ev + sp * eplane, e)[k] = r;
 }
 
@@ -912,6 +945,8 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
   const int sp = blockIdx.y;
   const float *q = ROW(qs + sp * cplane, c);
   const int n = I.m.neoc[c];
+  float sg[NE];
+  slots(sg, I.m.csign + c * NE);
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
@@ -927,7 +962,7 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
   float tc = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
-    if (s < n) tc = tc + ((-I.m.csign[c * NE + s]) * uh[s]) * evv[s];
+    if (s < n) tc = tc + ((-sg[s]) * uh[s]) * evv[s];
   tc = tc * I.m.invarea[c] + 0.0f;
   if (k < L) {
     const float orho = ROW(I.b.rho, c)[k];
@@ -946,6 +981,8 @@ KB k_fct_target(const Instance I, float dt) {
   const int L = I.n.L;
   const float ia = I.m.invarea[c];
   const int n = I.m.neoc[c];
+  float sd[NE];
+  slots(sd, I.m.csdv + c * NE);
   int e[NE];
   float uh[NE];
 #pragma unroll
@@ -955,7 +992,7 @@ KB k_fct_target(const Instance I, float dt) {
   float hd = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
-    if (s < n) hd = hd + (((-I.m.csign[c * NE + s]) * uh[s]) * I.m.dv[e[s]]) * ia;
+    if (s < n) hd = hd + (-(uh[s] * sd[s])) * ia;  // ((-csign)*uh)*dv, csign = +-1
   const float *wa = ROW(I.b.split_w, c);
   if (k < L) {
     const float t = ROW(I.b.rho, c)[k] + dt * (hd - I.z.rdzw[k] * (wa[k + 1] - wa[k]));
@@ -986,18 +1023,21 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
   const float sv = cs1(uu);
   const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
+  float av[NX], a3v[NX];
+  slots(av, I.m.adv + e * NX);
+  slots(a3v, I.m.adv3 + e * NX);
   float r;
   if (na == 10) {
     float val = (I.m.adv[e * NX] + (coef * sv) * I.m.adv3[e * NX]) * qv[0];
 #pragma unroll
     for (int s = 1; s < NX; ++s)
-      if (s < 10) val = val + (I.m.adv[e * NX + s] + (coef * sv) * I.m.adv3[e * NX + s]) * qv[s];
+      if (s < 10) val = val + (av[s] + (coef * sv) * a3v[s]) * qv[s];
     r = uu * val;
   } else {
     r = 0.0f;
 #pragma unroll
     for (int s = 0; s < NX; ++s)
-      if (s < na) r = r + (uu * (I.m.adv[e * NX + s] + (coef * sv) * I.m.adv3[e * NX + s])) * qv[s];
+      if (s < na) r = r + (uu * (av[s] + (coef * sv) * a3v[s])) * qv[s];
   }
   const float up = (I.m.dv[e] * dt) * (max0(uu) * q1 + min0(uu) * q2);
   if (k < I.n.L) {
@@ -1017,6 +1057,8 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
   const float *qsr = ROW(qs + sp * cplane, c);
   const float *qold = I.b.q + sp * cplane;
   const int n = I.m.neoc[c];
+  float sg[NE];
+  slots(sg, I.m.csign + c * NE);
   int nb[NE], e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
@@ -1075,7 +1117,7 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) {
-      const float so = -I.m.csign[c * NE + s];
+      const float so = -sg[s];
       mass = mass + (so * upw[s]) * ia;
       const float sr = (-so) * hres[s];
       sout = sout - max0(sr) * ia;
@@ -1117,6 +1159,8 @@ KB k_fct_cell2(const Instance I, float *__restrict__ out) {
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   const int sp = blockIdx.y;
   const int n = I.m.neoc[c];
+  float sg[NE];
+  slots(sg, I.m.csign + c * NE);
   int e[NE];
   float hres[NE];
 #pragma unroll
@@ -1133,7 +1177,7 @@ KB k_fct_cell2(const Instance I, float *__restrict__ out) {
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) {
-      const float sgn = -I.m.csign[c * NE + s];
+      const float sgn = -sg[s];
       mass = mass + (sgn * hres[s]) * ia;
     }
   mass = mass - I.z.rdzw[k] * (v_p - v);
