@@ -244,6 +244,19 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   if (!(c->config_dt > 0.0)) return refuse("config_dt must be positive");
   if (c->config_apvm_upwinding < 0.0f) return refuse("config_apvm_upwinding must be non-negative");
   if (c->config_divergence_damping && !(c->config_len_disp > 0.0f)) return refuse("config_len_disp must be positive");
+  if (c->config_horiz_mixing != HEXDYN_MIXING_OFF && c->config_horiz_mixing != HEXDYN_MIXING_2D_SMAGORINSKY)
+    return refuse("config_horiz_mixing must be off or 2d_smagorinsky");
+  const bool mixing = c->config_horiz_mixing == HEXDYN_MIXING_2D_SMAGORINSKY;
+  if (mixing) {
+    if (!(c->config_len_disp > 0.0f)) return refuse("config_len_disp must be positive");
+    if (!(c->config_visc4_2dsmag >= 0.0f) || !(c->config_smagorinsky_coef >= 0.0f))
+      return refuse("config_visc4_2dsmag and config_smagorinsky_coef must be non-negative");
+    if (!(c->config_del4u_div_factor > 0.0f)) return refuse("config_del4u_div_factor must be positive");
+    if (c->config_mpas_cam_coef != 0.0f) return refuse("config_mpas_cam_coef > 0 (CAM viscosity floor) is not implemented");
+    if (!m->deformation_coef_c2 || !m->deformation_coef_s2 || !m->deformation_coef_cs)
+      return refuse("2d_smagorinsky mixing needs the deformation weights");
+    if (!m->area_cell || !m->area_triangle) return refuse("mixing needs areaCell and areaTriangle");
+  }
   h->fast = c->numerics == HEXDYN_NUMERICS_FAST;
   const int in_me2 = m->max_edges2, in_ma = m->max_adv;
   n.ME2 = n.MA = 2 * n.ME - 2;
@@ -368,6 +381,72 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   rc |= h->upload(&M.csdv, csdv); rc |= h->upload(&M.cdcdv, cdcdv); rc |= h->upload(&M.ckite, ckite);
   rc |= h->upload(&M.vdc, vdc); rc |= h->upload(&M.vdcdv, vdcdv);
   rc |= h->upload(&M.ecos, ecos); rc |= h->upload(&M.esin, esin); rc |= h->upload(&M.ocell, ocell);
+  if (mixing) {
+    // binary32 scalars staged as the reference host does (cuda_horizontal_v841.py:1336-1350)
+    const float len = c->config_len_disp, csm = c->config_smagorinsky_coef;
+    const float h4 = c->config_visc4_2dsmag * ((len * len) * len);
+    std::vector<float> del2(E, 1.0f), del4(E, 1.0f);
+    if (c->config_h_ScaleWithMesh) {
+      for (int ne = 0; ne < E; ++ne) {
+        const int e = h->old_e_h[ne];
+        const int a0 = m->cells_on_edge[2 * e], a1 = m->cells_on_edge[2 * e + 1];
+        const float d0 = m->mesh_density ? m->mesh_density[a0] : 1.0f;
+        const float d1 = m->mesh_density ? m->mesh_density[a1] : 1.0f;
+        const float mean = 0.5f * (d0 + d1);
+        if (!(mean > 0.0f)) return refuse("meshDensity must be positive");
+        del2[ne] = 1.0f / hexdyn_powf_eval(mean, 0.25f);
+        del4[ne] = 1.0f / hexdyn_powf_eval(mean, 0.75f);
+      }
+    }
+    std::vector<float> c2s((size_t)C * ME, 0.0f), s2s((size_t)C * ME, 0.0f), css((size_t)C * ME, 0.0f),
+        fdiv((size_t)C * ME, 0.0f), fth2((size_t)C * ME, 0.0f), fw2((size_t)C * ME, 0.0f), f4((size_t)C * ME, 0.0f),
+        fvort(3 * (size_t)V, 0.0f), idc(E), idv(E), rdc(E), rdv(E);
+    for (int nc = 0; nc < C; ++nc) {
+      const int cc = h->old_c_h[nc];
+      const float ia = 1.0f / m->area_cell[cc];
+      for (int s = 0; s < neoc[nc]; ++s) {
+        const size_t i = (size_t)cc * ME + s, o = (size_t)nc * ME + s;
+        const int e = m->edges_on_cell[i];
+        const float sg = m->cells_on_edge[2 * e] == cc ? 1.0f : -1.0f;
+        const float dve = m->dv_edge[e], idce = 1.0f / m->dc_edge[e];
+        c2s[o] = m->deformation_coef_c2[i];
+        s2s[o] = m->deformation_coef_s2[i];
+        css[o] = m->deformation_coef_cs[i];
+        fdiv[o] = (ia * dve) * sg;
+        fth2[o] = ((ia * sg) * dve) * idce;
+        fw2[o] = (((0.5f * ia) * sg) * dve) * idce;
+        f4[o] = (((del4[new_e[e]] * (h4 * ia)) * dve) * sg) * idce;
+      }
+    }
+    for (int nv = 0; nv < V; ++nv) {
+      const int vv = h->old_v_h[nv];
+      const float it = 1.0f / m->area_triangle[vv];
+      for (int j = 0; j < 3; ++j) {
+        const int e = m->edges_on_vertex[3 * vv + j];
+        const float vs = m->vertices_on_edge[2 * e] == vv ? -1.0f : 1.0f;
+        fvort[3 * nv + j] = (it * m->dc_edge[e]) * vs;
+      }
+    }
+    for (int ne = 0; ne < E; ++ne) {
+      const int e = h->old_e_h[ne];
+      idc[ne] = 1.0f / m->dc_edge[e];
+      idv[ne] = std::fmin(1.0f / m->dv_edge[e], 4.0f * idc[ne]);
+      const float scale = del4[ne] * h4;
+      rdc[ne] = (scale * c->config_del4u_div_factor) * idc[ne];
+      rdv[ne] = scale * idv[ne];
+    }
+    rc |= h->upload(&M.defc2, c2s); rc |= h->upload(&M.defs2, s2s); rc |= h->upload(&M.defcs, css);
+    rc |= h->upload(&M.mx_fdiv, fdiv); rc |= h->upload(&M.mx_fth2, fth2); rc |= h->upload(&M.mx_fw2, fw2);
+    rc |= h->upload(&M.mx_f4, f4); rc |= h->upload(&M.mx_fvort, fvort);
+    rc |= h->upload(&M.mx_idc, idc); rc |= h->upload(&M.mx_idv, idv); rc |= h->upload(&M.mx_rdc, rdc);
+    rc |= h->upload(&M.mx_rdv, rdv); rc |= h->upload(&M.mx_del2, del2);
+    Consts &K0 = h->I.k;
+    K0.mixing = 1;
+    K0.mix4 = h4 > 0.0f;
+    K0.strain_scale = (csm * len) * (csm * len);
+    const float inv_dt = 1.0f / (float)c->config_dt;
+    K0.ceiling = (0.01f * (len * len)) * inv_dt;
+  }
   rc |= h->upload(&M.coe, coe); rc |= h->upload(&M.voe, voe); rc |= h->upload(&M.eoc, eoc);
   rc |= h->upload(&M.coc, coc); rc |= h->upload(&M.voc, voc); rc |= h->upload(&M.neoc, neoc);
   rc |= h->upload(&M.kfc, kfc); rc |= h->upload(&M.eov, eov); rc |= h->upload(&M.eoe, eoe);
@@ -453,6 +532,12 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   for (float **p : scal_bufs) rc |= h->alloc(p, sr);
   float **sedge_bufs[] = {&B.ev, &B.fct_hres, &B.fct_upw};
   for (float **p : sedge_bufs) rc |= h->alloc(p, ser);
+  if (mixing) {
+    float **mix_cell[] = {&B.kdiff, &B.delsq_div, &B.delsq_theta, &B.delsq_w, &B.mix_tw, &B.euler_theta};
+    for (float **p : mix_cell) rc |= h->alloc(p, cr);
+    rc |= h->alloc(&B.delsq_u, er);
+    rc |= h->alloc(&B.delsq_vort, vr);
+  }
   rc |= h->alloc(&B.flag, 16);
   if (rc) { int r = HEXDYN_E_HIP; hexdyn_destroy(h); return r; }
   if (n.NM == 0) {
@@ -558,6 +643,8 @@ FieldRef field_ref(hexdyn_handle *h, int id) {
     case HEXDYN_F_COFTZ: return cell(B.coftz, L + 1);
     case HEXDYN_F_EULER_RU: return edge(B.euler_ru);
     case HEXDYN_F_EULER_RW: return cell(B.euler_rw, L + 1);
+    case HEXDYN_F_KDIFF: return cell(B.kdiff, L);
+    case HEXDYN_F_EULER_THETA: return cell(B.euler_theta, L);
     case HEXDYN_F_FLUX_U: return edge(B.flux_u);
     case HEXDYN_F_FLUX_W: return cell(B.flux_w, L + 1);
     case HEXDYN_F_QTOT: return cell(B.qtot, L);

@@ -55,6 +55,14 @@ struct Mesh {
   const float *vdcdv;   // [V][3]  dcdv[edge]
   const float *ecos, *esin;  // [E][ME2] cos/sin angle of edgesOnEdge
   const int *ocell;     // [C][ME] the other cell of the slot's edge
+  // horizontal mixing (binary32 factors in the reference's operation order)
+  const float *defc2, *defs2, *defcs;  // [C][ME] deformation weights
+  const float *mx_fdiv;   // [C][ME] ((1/area)*dv)*sign                        divergence of delsq_u
+  const float *mx_fth2;   // [C][ME] (((1/area)*sign)*dv)*(1/dc)                theta del2
+  const float *mx_fw2;    // [C][ME] (((0.5*(1/area))*sign)*dv)*(1/dc)          w del2
+  const float *mx_f4;     // [C][ME] ((((del4*(h4*(1/area)))*dv)*sign)*(1/dc)   theta and w del4
+  const float *mx_fvort;  // [V][3]  ((1/areaTriangle)*dc)*vertex sign          vorticity of delsq_u
+  const float *mx_idc, *mx_idv, *mx_rdc, *mx_rdv, *mx_del2;  // [E]
 };
 
 struct Vert {
@@ -94,6 +102,8 @@ struct Work {
   // transport
   float *q_stage, *q_next, *target, *ev, *fct_mn, *fct_mx, *fct_mass, *fct_sin, *fct_sout, *fct_vres,
       *fct_hres, *fct_upw;
+  // horizontal mixing: viscosity, Laplacians and the theta Euler pool
+  float *kdiff, *delsq_u, *delsq_div, *delsq_vort, *delsq_theta, *delsq_w, *mix_tw, *euler_theta;
   int *flag;  // sticky validation flag
 };
 
@@ -109,6 +119,8 @@ struct Consts {
   float split_inv;         // 1/splits
   int order, splits, monotonic, divdamp, split_transport, scalar_adv;
   int apvm_on;
+  int mixing, mix4;              // Smagorinsky mixing on; fourth-order part on (h4 > 0)
+  float strain_scale, ceiling;   // (c_s*len)^2, (0.01*len^2)/dt
 };
 
 // Optional per-kernel timing (events around each launch on its stream).

@@ -320,6 +320,235 @@ __device__ __forceinline__ float cell_flux_div(const Instance &I, int c, int k, 
   return acc * I.m.invarea[c];
 }
 
+// ------------------------------------------------------- horizontal mixing
+// v8.4.1 2-D Smagorinsky mixing, once per dynamics subcycle from the
+// subcycle-start state (cuda_horizontal_v841.py:205-583, host sequence
+// :1270-1588; mpas_atm_dissipation_models.F smagorinsky_2d and the u/w/theta
+// del2/del4 filters).  Edge density is 0.5*(rho[cell0]+rho[cell1]) of the
+// current rho_zz, as the subcycle-start solve formed it.
+
+// eddy viscosity from the deformation tensor (smagorinsky_v841_f32)
+template <int NE>
+KB k_mix_kdiff(const Instance I) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int n = I.m.neoc[c];
+  int e[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  float un[NE], vn[NE], c2[NE], s2[NE], cs[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    un[s] = ROW(I.b.u, e[s])[k];
+    vn[s] = ROW(I.b.v, e[s])[k];
+  }
+  slots(c2, I.m.defc2 + c * NE);
+  slots(s2, I.m.defs2 + c * NE);
+  slots(cs, I.m.defcs + c * NE);
+  float dudx = 0.0f, dudy = 0.0f, dvdx = 0.0f, dvdy = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      dudx = (dudx + c2[s] * un[s]) - cs[s] * vn[s];
+      dudy = (dudy + cs[s] * un[s]) - s2[s] * vn[s];
+      dvdx = (dvdx + cs[s] * un[s]) + c2[s] * vn[s];
+      dvdy = (dvdy + s2[s] * un[s]) + cs[s] * vn[s];
+    }
+  const float d11 = 2.0f * dudx, d22 = 2.0f * dvdy, d12 = dudy + dvdx;
+  const float diff = d11 - d22;
+  const float strain = __builtin_sqrtf(0.25f * (diff * diff) + d12 * d12);
+  if (k < I.n.L) ROW(I.b.kdiff, c)[k] = fminf(I.k.strain_scale * strain, I.k.ceiling);
+}
+
+// del2 of u into the Euler pool (momentum_filter_seed_v841_f32): pool + ((rho_e*k_e)*lap)*del2
+KB k_mix_edge(const Instance I) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
+  const float d0 = ROW(I.b.divergence, c0)[k], d1 = ROW(I.b.divergence, c1)[k];
+  const float z0 = ROW(I.b.vorticity, v0)[k], z1 = ROW(I.b.vorticity, v1)[k];
+  const float kd0 = ROW(I.b.kdiff, c0)[k], kd1 = ROW(I.b.kdiff, c1)[k];
+  const float r0 = ROW(I.b.c_rho, c0)[k], r1 = ROW(I.b.c_rho, c1)[k];
+  const float lap = (d1 - d0) * I.m.mx_idc[e] - (z1 - z0) * I.m.mx_idv[e];
+  const float kedge = 0.5f * (kd0 + kd1);
+  const float rho_e = 0.5f * (r0 + r1);
+  if (k < I.n.L) {
+    if (I.k.mix4) ROW(I.b.delsq_u, e)[k] = lap;
+    float *pool = ROW(I.b.euler_ru, e);
+    pool[k] = pool[k] + ((rho_e * kedge) * lap) * I.m.mx_del2[e];
+  }
+}
+
+// divergence of delsq_u (laplacian_divergence_v841_f32)
+template <int NE>
+KB k_mix_lapdiv(const Instance I) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int n = I.m.neoc[c];
+  int e[NE];
+  float x[NE], f[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) x[s] = ROW(I.b.delsq_u, e[s])[k];
+  slots(f, I.m.mx_fdiv + c * NE);
+  float acc = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) acc = acc + f[s] * x[s];
+  if (k < I.n.L) ROW(I.b.delsq_div, c)[k] = acc;
+}
+
+// vorticity of delsq_u (laplacian_vorticity_v841_f32)
+KB k_mix_lapvort(const Instance I) {
+  const int v = wave_entity();
+  if (v >= I.n.V) return;
+  const int k = lane();
+  int e[3];
+  float x[3], f[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) e[s] = I.m.eov[3 * v + s];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) x[s] = ROW(I.b.delsq_u, e[s])[k];
+  slots(f, I.m.mx_fvort + 3 * v);
+  float acc = 0.0f;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) acc = acc + f[s] * x[s];
+  if (k < I.n.L) ROW(I.b.delsq_vort, v)[k] = acc;
+}
+
+// del4 of u (momentum_filter_lap4_v841_f32): pool - rho_e*((ddiv*r_dc) - (dvort*r_dv))
+KB k_mix_edge4(const Instance I) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
+  const float d0 = ROW(I.b.delsq_div, c0)[k], d1 = ROW(I.b.delsq_div, c1)[k];
+  const float z0 = ROW(I.b.delsq_vort, v0)[k], z1 = ROW(I.b.delsq_vort, v1)[k];
+  const float r0 = ROW(I.b.c_rho, c0)[k], r1 = ROW(I.b.c_rho, c1)[k];
+  const float rho_e = 0.5f * (r0 + r1);
+  const float diffusion = rho_e * ((d1 - d0) * I.m.mx_rdc[e] - (z1 - z0) * I.m.mx_rdv[e]);
+  if (k < I.n.L) {
+    float *pool = ROW(I.b.euler_ru, e);
+    pool[k] = pool[k] - diffusion;
+  }
+}
+
+// del2 of theta and of w (theta_filter_lap2_v841_f32, w_filter_lap2_v841_f32).
+// Without the fourth-order part the results go straight to the Euler pools.
+template <int NE>
+KB k_mix_cell(const Instance I) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L;
+  const int n = I.m.neoc[c];
+  int e[NE], o[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    e[s] = I.m.eoc[c * NE + s];
+    o[s] = I.m.ocell[c * NE + s];
+  }
+  float tho[NE], rho[NE], wo[NE], kdo[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    tho[s] = ROW(I.b.theta, o[s])[k];
+    rho[s] = ROW(I.b.c_rho, o[s])[k];
+    wo[s] = ROW(I.b.w, o[s])[k];
+    kdo[s] = ROW(I.b.kdiff, o[s])[k];
+  }
+  const float thc = ROW(I.b.theta, c)[k], rhc = ROW(I.b.c_rho, c)[k], wc = ROW(I.b.w, c)[k];
+  const float kdc = ROW(I.b.kdiff, c)[k];
+  float sg[NE], f2[NE], fw[NE], d2[NE];
+  slots(sg, I.m.csign + c * NE);
+  slots(f2, I.m.mx_fth2 + c * NE);
+  slots(fw, I.m.mx_fw2 + c * NE);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) d2[s] = I.m.mx_del2[e[s]];
+  float lap = 0.0f, tend = 0.0f, lapw = 0.0f, tendw = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    // cell0/cell1 of edge s in cellsOnEdge order
+    const bool first = sg[s] > 0.0f;
+    const float th0 = first ? thc : tho[s], th1 = first ? tho[s] : thc;
+    const float r0 = first ? rhc : rho[s], r1 = first ? rho[s] : rhc;
+    const float w0 = first ? wc : wo[s], w1 = first ? wo[s] : wc;
+    const float k0 = first ? kdc : kdo[s], k1 = first ? kdo[s] : kdc;
+    const float rho_e = 0.5f * (r0 + r1);
+    const float rho_em = up1(rho_e);
+    const float k0m = up1(k0), k1m = up1(k1);
+    if (s < n) {
+      const float flux = (f2[s] * (th1 - th0)) * rho_e;
+      lap = lap + flux;
+      tend = tend + ((flux * 0.5f) * (k0 + k1)) * d2[s];
+      const float fluxw = (fw[s] * (rho_e + rho_em)) * (w1 - w0);
+      lapw = lapw + fluxw;
+      const float sum_k = ((k0 + k1) + k0m) + k1m;
+      tendw = tendw + ((fluxw * d2[s]) * 0.25f) * sum_k;
+    }
+  }
+  if (k == 0 || k >= L) {
+    lapw = 0.0f;
+    tendw = 0.0f;
+  }
+  if (k < L) {
+    ROW(I.b.delsq_theta, c)[k] = lap;
+    ROW(I.b.euler_theta, c)[k] = tend;
+    ROW(I.b.delsq_w, c)[k] = lapw;
+  }
+  if (I.k.mix4) {
+    if (k <= L) ROW(I.b.mix_tw, c)[k] = tendw;
+  } else if (k <= L) {
+    float *pool = ROW(I.b.euler_rw, c);
+    pool[k] = pool[k] + tendw;
+  }
+}
+
+// del4 of theta and of w (theta_filter_lap4_v841_f32, w_filter_lap4_v841_f32), then the w pool
+template <int NE>
+KB k_mix_cell4(const Instance I) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  const int L = I.n.L;
+  const int n = I.m.neoc[c];
+  int o[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) o[s] = I.m.ocell[c * NE + s];
+  float dto[NE], dwo[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    dto[s] = ROW(I.b.delsq_theta, o[s])[k];
+    dwo[s] = ROW(I.b.delsq_w, o[s])[k];
+  }
+  const float dtc = ROW(I.b.delsq_theta, c)[k], dwc = ROW(I.b.delsq_w, c)[k];
+  float sg[NE], f4[NE];
+  slots(sg, I.m.csign + c * NE);
+  slots(f4, I.m.mx_f4 + c * NE);
+  float tend = ROW(I.b.euler_theta, c)[k];
+  float tendw = ROW(I.b.mix_tw, c)[k];
+  const bool wlev = k >= 1 && k < L;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      const bool first = sg[s] > 0.0f;
+      const float t0 = first ? dtc : dto[s], t1 = first ? dto[s] : dtc;
+      const float w0 = first ? dwc : dwo[s], w1 = first ? dwo[s] : dwc;
+      tend = tend - f4[s] * (t1 - t0);
+      if (wlev) tendw = tendw - f4[s] * (w1 - w0);
+    }
+  if (k < L) ROW(I.b.euler_theta, c)[k] = tend;
+  if (k <= L) {
+    float *pool = ROW(I.b.euler_rw, c);
+    pool[k] = pool[k] + tendw;
+  }
+}
+
 // ------------------------------------------------------------ subcycle start
 template <int NE>
 KB k_sub_cell(const Instance I) {
@@ -568,7 +797,7 @@ KB k_tend_cell(const Instance I, int stage) {
     const float *zzr = ROW(I.z.zz, c);
     rw_t = q * (fzm * zzr[k] + fzp * zzr[km]);
   }
-  if (k < L) ROW(I.b.tend_rt, c)[k] = trt;
+  if (k < L) ROW(I.b.tend_rt, c)[k] = I.k.mixing ? trt + ROW(I.b.euler_theta, c)[k] : trt;
   if (k <= L) ROW(I.b.tend_rw, c)[k] = rw_t;
 }
 
@@ -1242,6 +1471,17 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
   launch(k_sub_edge, I.n.E, 1, s, I);
   launch(k_vcoef, I.n.C, 1, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
          (const float *)I.b.s_rtheta_p);
+  if (I.k.mixing) {
+    LAUNCH_NE(I, k_mix_kdiff, I.n.C, 1, s, I);
+    launch(k_mix_edge, I.n.E, 1, s, I);
+    if (I.k.mix4) {
+      LAUNCH_NE(I, k_mix_lapdiv, I.n.C, 1, s, I);
+      launch(k_mix_lapvort, I.n.V, 1, s, I);
+      launch(k_mix_edge4, I.n.E, 1, s, I);
+    }
+    LAUNCH_NE(I, k_mix_cell, I.n.C, 1, s, I);
+    if (I.k.mix4) LAUNCH_NE(I, k_mix_cell4, I.n.C, 1, s, I);
+  }
   return check();
 }
 

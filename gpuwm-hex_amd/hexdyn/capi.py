@@ -34,6 +34,7 @@ FIELDS = [
     "EULER_RU", "EULER_RW",
     "FLUX_U", "FLUX_W", "SCALARS_STAGE",
     "QTOT", "CQW", "CQU",
+    "KDIFF", "EULER_THETA",
 ]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 
@@ -54,6 +55,8 @@ class HexdynMesh(ct.Structure):
         ("n_adv_cells_for_edge", i32p), ("adv_cells_for_edge", i32p),
         ("adv_coefs", f32p), ("adv_coefs_3rd", f32p),
         ("x_cell", f64p), ("y_cell", f64p), ("z_cell", f64p),
+        ("mesh_density", f32p), ("deformation_coef_c2", f32p), ("deformation_coef_s2", f32p),
+        ("deformation_coef_cs", f32p),
     ]
 
 
@@ -86,6 +89,12 @@ class HexdynConfig(ct.Structure):
         ("n_scalars", ct.c_int32),
         ("n_moist", ct.c_int32),
         ("numerics", ct.c_int32),
+        ("config_horiz_mixing", ct.c_int32),
+        ("config_visc4_2dsmag", ct.c_float),
+        ("config_smagorinsky_coef", ct.c_float),
+        ("config_del4u_div_factor", ct.c_float),
+        ("config_h_ScaleWithMesh", ct.c_int32),
+        ("config_mpas_cam_coef", ct.c_float),
     ]
 
 
@@ -166,6 +175,13 @@ def build_structs(case) -> tuple:
     mesh.x_cell = b.f64("xc", m.xCell)
     mesh.y_cell = b.f64("yc", m.yCell)
     mesh.z_cell = b.f64("zc", m.zCell)
+    mesh.mesh_density = b.f32("density", m.arrays.get("meshDensity", np.ones(m.n_cells)))
+    if case.config.config_horiz_mixing != "off":
+        from .setup import deformation_weights
+        c2, s2, cs = deformation_weights(m)
+        mesh.deformation_coef_c2 = b.f32("defc2", c2)
+        mesh.deformation_coef_s2 = b.f32("defs2", s2)
+        mesh.deformation_coef_cs = b.f32("defcs", cs)
 
     vert = HexdynVertical()
     vert.n_levels = g.zz.shape[0]
@@ -206,6 +222,12 @@ def build_structs(case) -> tuple:
     cfg.n_scalars = case.n_scalars
     cfg.n_moist = case.n_moist
     cfg.numerics = 0 if c.numerics == "strict" else 1
+    cfg.config_horiz_mixing = 0 if c.config_horiz_mixing == "off" else 1
+    cfg.config_visc4_2dsmag = c.config_visc4_2dsmag
+    cfg.config_smagorinsky_coef = c.config_smagorinsky_coef
+    cfg.config_del4u_div_factor = c.config_del4u_div_factor
+    cfg.config_h_ScaleWithMesh = int(c.config_h_ScaleWithMesh)
+    cfg.config_mpas_cam_coef = c.config_mpas_cam_coef
     return mesh, vert, cfg, b
 
 

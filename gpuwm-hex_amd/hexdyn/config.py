@@ -46,7 +46,12 @@ class DycoreConfig:
     config_epssm_maximum: float = 0.5
     config_epssm_transition_bottom_z: float = 30000.0
     config_epssm_transition_top_z: float = 50000.0
-    config_horiz_mixing: str = "off"
+    config_horiz_mixing: str = "off"      # or "2d_smagorinsky" (mixing_v841.py:71-85)
+    config_visc4_2dsmag: float = 0.05
+    config_smagorinsky_coef: float = 0.125
+    config_del4u_div_factor: float = 10.0
+    config_h_ScaleWithMesh: bool = True
+    config_mpas_cam_coef: float = 0.0
     config_moist_physics: bool = False
     numerics: str = "strict"
 
@@ -91,8 +96,22 @@ class DycoreConfig:
         if not 0.0 <= self.config_xnutr <= 1.0:
             raise ConfigurationRefusal("config_xnutr", self.config_xnutr, "bounded damping coefficient",
                                        "0 <= config_xnutr <= 1")
-        if self.config_horiz_mixing != "off":
+        if self.config_horiz_mixing not in ("off", "2d_smagorinsky"):
             raise ConfigurationRefusal("config_horiz_mixing", self.config_horiz_mixing,
-                                       "horizontal mixing is not linked into this step", "'off'")
+                                       "only the v8.4.1 2-D Smagorinsky branch is implemented",
+                                       "'off' or '2d_smagorinsky'")
+        if self.config_horiz_mixing == "2d_smagorinsky":
+            for knob in ("config_visc4_2dsmag", "config_smagorinsky_coef"):
+                value = getattr(self, knob)
+                if not math.isfinite(value) or value < 0.0:
+                    raise ConfigurationRefusal(knob, value, "the Registry admits only non-negative values",
+                                               f"{knob}>=0")
+            if not math.isfinite(self.config_del4u_div_factor) or self.config_del4u_div_factor <= 0.0:
+                raise ConfigurationRefusal("config_del4u_div_factor", self.config_del4u_div_factor,
+                                           "the Registry requires a positive divergent hyperdiffusion factor",
+                                           "config_del4u_div_factor>0")
+            if self.config_mpas_cam_coef != 0.0:
+                raise ConfigurationRefusal("config_mpas_cam_coef", self.config_mpas_cam_coef,
+                                           "the CAM viscosity floor is not implemented", "0.0")
         if self.numerics not in ("strict", "fast"):
             raise ConfigurationRefusal("numerics", self.numerics, "two numerics modes exist", "'strict' or 'fast'")

@@ -13,6 +13,8 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libhexdyn.so"
 CSRC = HERE.parent / "csrc"
 
+ABI_VERSION = 2  # HEXDYN_ABI_VERSION in include/hexdyn.h
+
 _lib = None
 
 
@@ -62,7 +64,7 @@ def lib() -> ct.CDLL:
     L.hexdyn_profile_kernels.argtypes = [vp, ct.c_char_p, ct.c_size_t]
     L.hexdyn_powf.argtypes = [ct.c_float, ct.c_float]
     L.hexdyn_powf.restype = ct.c_float
-    if L.hexdyn_abi_version() != 1:
+    if L.hexdyn_abi_version() != ABI_VERSION:
         raise HexdynLibraryMissing("libhexdyn.so ABI version mismatch")
     _lib = L
     return L

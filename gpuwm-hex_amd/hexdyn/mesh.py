@@ -14,7 +14,9 @@ Orientation conventions (the ones the operators rely on):
 * ``verticesOnEdge[e] = (v1, v2)``: v2 lies on the left of the normal
   (k x n), so the circulation around v2 counts +u;
 * ``edgesOnCell``/``cellsOnCell``/``verticesOnCell`` run counter-clockwise;
-  vertex i lies between edge i and edge i+1;
+  edge i joins vertex i and vertex i+1 (the "edge-start" slot convention of
+  the NCAR x1 meshes, ``mesh.py:840-848``), so vertex i lies between edge
+  i-1 and edge i;
 * ``cellsOnVertex`` runs counter-clockwise, ``edgesOnVertex[v, j]`` joins
   ``cellsOnVertex[v, j]`` and ``cellsOnVertex[v, j+1]`` and
   ``kiteAreasOnVertex[v, j]`` is the part of triangle v inside cell
@@ -259,10 +261,10 @@ def build_mesh(points: np.ndarray, *, radius: float = EARTH_RADIUS) -> Mesh:
             cells_on_cell[c, i] = n
             edges_on_cell[c, i] = e
         for i in range(count):
-            nxt = nb[(i + 1) % count][0]
-            # CCW triangle (c, n_i, n_{i+1}) is the vertex between edges i and i+1
-            vertices_on_cell[c, i] = tri_lookup[(c, nb[i][0])]
-            if tri_lookup[(c, nb[i][0])] != tri_lookup[(nb[i][0], nxt)]:
+            prv = nb[(i - 1) % count][0]
+            # CCW triangle (c, n_{i-1}, n_i) is the vertex between edges i-1 and i
+            vertices_on_cell[c, i] = tri_lookup[(c, prv)]
+            if tri_lookup[(c, prv)] != tri_lookup[(prv, nb[i][0])]:
                 raise ValueError("cell neighbourhood is not a fan")
 
     xyz_cell = points * radius
@@ -362,7 +364,7 @@ def _build_trisk_weights(mesh: Mesh) -> None:
             r = 0.0
             for step in range(1, n):
                 j = (start + step) % n
-                vertex_slot = (start + step - 1) % n
+                vertex_slot = (start + step) % n  # the vertex between slots j-1 and j
                 vertex = voc[cell, vertex_slot]
                 r += kite[vertex, kfc[cell, vertex_slot]] / area[cell]
                 ep = eoc[cell, j]

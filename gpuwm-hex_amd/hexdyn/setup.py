@@ -11,6 +11,10 @@ the HIP library and by the oracle alike, so both consume identical bits.
 * Advection coefficient compression in native insertion order:
   ``transport.py:2277`` and ``transport.py:2413`` (core.F:1475-1529).
 * Stored mesh inverses: ``dynamics_v841.py:28`` (``precomputed_mesh_inverse_v841``).
+* Smagorinsky deformation weights ``coef_c2/s2/cs``: ``mixing_v841.py:372-455``
+  (MPAS ``atm_initialize_deformation_weights``, spherical branch), evaluated
+  here in binary64 and rounded once.
+* Mesh scaling of del2/del4: ``mixing.py:271-318`` (``compute_mesh_mixing_scaling``).
 """
 
 from __future__ import annotations
@@ -232,3 +236,94 @@ def stored_inverse(values: np.ndarray) -> np.ndarray:
     if np.any(v == 0):
         raise ValueError("zero geometry value")
     return (F32(1.0) / v).astype(F32)
+
+
+def _arc_length(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Great-circle length between points a and b on the sphere through a (MPAS ``mpas_arc_length``)."""
+    r = np.sqrt(np.sum(a * a, axis=-1))
+    c = np.sqrt(np.sum((b - a) ** 2, axis=-1))
+    return r * 2.0 * np.arcsin(c / (2.0 * r))
+
+
+def _sphere_angle(a: np.ndarray, b: np.ndarray, c: np.ndarray) -> np.ndarray:
+    """Signed spherical angle at a between the arcs a-b and a-c (MPAS ``mpas_sphere_angle``).
+
+    Half-angle form of the spherical law of cosines; positive when
+    ((b-a) x (c-a)) . a >= 0.
+    """
+    sa = _arc_length(b, c)
+    sb = _arc_length(a, c)
+    sc = _arc_length(a, b)
+    d = np.cross(b - a, c - a)
+    s = 0.5 * (sa + sb + sc)
+    ratio = (np.sin(s - sb) * np.sin(s - sc)) / (np.sin(sb) * np.sin(sc))
+    sin_half = np.sqrt(np.minimum(1.0, np.maximum(0.0, ratio)))
+    mag = 2.0 * np.arcsin(np.clip(sin_half, -1.0, 1.0))
+    return np.where(np.sum(d * a, axis=-1) >= 0.0, mag, -mag)
+
+
+def deformation_weights(mesh: Mesh):
+    """Smagorinsky deformation weights ``(coef_c2, coef_s2, coef_cs)``, each (C, maxEdges), binary32.
+
+    Per cell (``mixing_v841.py:372-455``): the cell's vertices are laid out on
+    the tangent plane at polar angles accumulated from the angle to north
+    and at the great-circle distance from the centre; the polygon area comes
+    from the shoelace sum and each side's outward normal angle from atan2 of
+    the side.  For side j with length dl and normal angle t:
+    c2 = dl cos^2 t / A, s2 = dl sin^2 t / A, cs = dl sin t cos t / A, all
+    negated where the cell is not the edge's first cell.
+    """
+    radius = float(mesh.sphere_radius)
+    cell_xyz = np.stack([mesh.xCell, mesh.yCell, mesh.zCell], axis=1) / radius
+    vert_xyz = np.stack([mesh.xVertex, mesh.yVertex, mesh.zVertex], axis=1) / radius
+    counts = mesh.nEdgesOnCell
+    me = mesh.max_edges
+    n_cells = counts.size
+    c2 = np.zeros((n_cells, me), dtype=np.float64)
+    s2 = np.zeros_like(c2)
+    cs = np.zeros_like(c2)
+    north = np.array([0.0, 0.0, 1.0])
+    for n in np.unique(counts):
+        cells = np.flatnonzero(counts == n)
+        p = cell_xyz[cells]                                 # (m, 3)
+        v = vert_xyz[mesh.verticesOnCell[cells, :n]]        # (m, n, 3)
+        polar = p[:, 2] == 1.0
+        theta_abs = np.where(polar, 0.5 * np.pi,
+                             0.5 * np.pi - _sphere_angle(p, v[:, 0], np.broadcast_to(north, p.shape)))
+        vn = np.roll(v, -1, axis=1)
+        pp = np.broadcast_to(p[:, None, :], v.shape)
+        thetav = _sphere_angle(pp, v, vn)                   # (m, n)
+        dl_sphere = radius * _arc_length(pp, v)
+        thetat = np.empty_like(thetav)
+        thetat[:, 0] = theta_abs
+        for j in range(1, n):
+            thetat[:, j] = thetat[:, j - 1] + thetav[:, j - 1]
+        xp = np.cos(thetat) * dl_sphere
+        yp = np.sin(thetat) * dl_sphere
+        xq, yq = np.roll(xp, -1, axis=1), np.roll(yp, -1, axis=1)
+        area = np.zeros(cells.size)
+        for j in range(n):
+            area = (area + 0.25 * (xp[:, j] + xq[:, j]) * (yq[:, j] - yp[:, j])
+                    - 0.25 * (yp[:, j] + yq[:, j]) * (xq[:, j] - xp[:, j]))
+        dx, dy = xq - xp, yq - yp
+        t = np.arctan2(dy, dx) - 0.5 * np.pi
+        dl = np.sqrt(dx * dx + dy * dy)
+        st, ct = np.sin(t), np.cos(t)
+        sign = np.where(mesh.cellsOnEdge[mesh.edgesOnCell[cells, :n], 0] == cells[:, None], 1.0, -1.0)
+        c2[cells, :n] = sign * dl * ct * ct / area[:, None]
+        s2[cells, :n] = sign * dl * st * st / area[:, None]
+        cs[cells, :n] = sign * dl * st * ct / area[:, None]
+    return c2.astype(F32), s2.astype(F32), cs.astype(F32)
+
+
+def mesh_mixing_scaling(mesh: Mesh, *, scale_with_mesh: bool = True):
+    """``meshScalingDel2/Del4`` per edge (``mixing.py:271-318``): 1/rho_e^(1/4) and 1/rho_e^(3/4) of the
+    mean cell mesh density, or ones.  On a quasi-uniform mesh (density 1) both are exactly 1."""
+    n_edges = mesh.cellsOnEdge.shape[0]
+    if not scale_with_mesh:
+        return np.ones(n_edges, F32), np.ones(n_edges, F32)
+    density = np.asarray(mesh.arrays.get("meshDensity", np.ones(mesh.nEdgesOnCell.size)), dtype=F32)
+    mean = F32(0.5) * (density[mesh.cellsOnEdge[:, 0]] + density[mesh.cellsOnEdge[:, 1]])
+    del2 = (F32(1.0) / np.power(mean.astype(np.float64), 0.25).astype(F32)).astype(F32)
+    del4 = (F32(1.0) / np.power(mean.astype(np.float64), 0.75).astype(F32)).astype(F32)
+    return del2, del4

@@ -9,7 +9,8 @@
  *   hexdyn_get_state         <- step() download half            (cuda_driver.py:6112-6149)
  *   hexdyn_step              <- _step_device_v841 / step_device (cuda_driver.py:4854-5537, :5751)
  *   hexdyn_begin_step        <- moist coefficients + first solve_diagnostics (cuda_driver.py:5049-5070, :4232-4245)
- *   hexdyn_begin_subcycle    <- subcycle-start pools            (cuda_driver.py:4232-4383)
+ *   hexdyn_begin_subcycle    <- subcycle-start pools and 2-D Smagorinsky mixing (cuda_driver.py:4232-4383,
+ *                               cuda_horizontal_v841.py:1270-1588)
  *   hexdyn_dyn_tend          <- stage tendencies                (cuda_driver.py:4388-4509)
  *   hexdyn_acoustic_substep  <- advance_acoustic_step_cuda_v841 (cuda_acoustic_v841.py:801-1040)
  *   hexdyn_divergence_damping<- divergence_damping_3d           (cuda_horizontal.py:1584-1660)
@@ -42,7 +43,7 @@
 extern "C" {
 #endif
 
-#define HEXDYN_ABI_VERSION 1
+#define HEXDYN_ABI_VERSION 2
 #define HEXDYN_MAX_EDGES 7      /* nEdgesOnCell limit (stencil widths derive from it) */
 #define HEXDYN_ROW 64           /* padded level row; nVertLevels + 1 <= 64 */
 
@@ -55,6 +56,7 @@ enum {
 };
 
 enum { HEXDYN_NUMERICS_STRICT = 0, HEXDYN_NUMERICS_FAST = 1 };
+enum { HEXDYN_MIXING_OFF = 0, HEXDYN_MIXING_2D_SMAGORINSKY = 1 };
 
 typedef struct hexdyn_mesh {
   int32_t n_cells, n_edges, n_vertices;
@@ -66,7 +68,7 @@ typedef struct hexdyn_mesh {
   const int32_t *vertices_on_edge;     /* [E][2]  vertex 1 on the left of the normal */
   const int32_t *edges_on_cell;        /* [C][max_edges] counter-clockwise */
   const int32_t *cells_on_cell;        /* [C][max_edges] */
-  const int32_t *vertices_on_cell;     /* [C][max_edges] vertex i between edges i and i+1 */
+  const int32_t *vertices_on_cell;     /* [C][max_edges] read only through kite_for_cell */
   const int32_t *n_edges_on_cell;      /* [C] */
   const int32_t *kite_for_cell;        /* [C][max_edges] slot of the cell in cellsOnVertex */
   const int32_t *edges_on_vertex;      /* [V][3] */
@@ -85,6 +87,14 @@ typedef struct hexdyn_mesh {
   const int32_t *adv_cells_for_edge;   /* [E][max_adv] */
   const float *adv_coefs, *adv_coefs_3rd;          /* [E][max_adv] */
   const double *x_cell, *y_cell, *z_cell;          /* [C] positions for locality ordering (may be NULL) */
+  /* Horizontal mixing inputs (read only when config_horiz_mixing is on):
+   * meshDensity (NULL reads as 1) and the v8.4.1 deformation weights
+   * deformation_coef_c2/s2/cs in edgesOnCell slot order, signed by the
+   * cell's side of each edge (mixing_v841.py:372-455). */
+  const float *mesh_density;           /* [C] */
+  const float *deformation_coef_c2;    /* [C][max_edges] */
+  const float *deformation_coef_s2;    /* [C][max_edges] */
+  const float *deformation_coef_cs;    /* [C][max_edges] */
 } hexdyn_mesh;
 
 typedef struct hexdyn_vertical {
@@ -117,6 +127,14 @@ typedef struct hexdyn_config {
   int32_t n_scalars;                       /* T */
   int32_t n_moist;                         /* leading mass-loading species, 0 = dry */
   int32_t numerics;                        /* HEXDYN_NUMERICS_* */
+  /* 2-D Smagorinsky horizontal mixing (mixing_v841.py:71-85 V841MixingConfig);
+   * config_len_disp above is its filter length too. */
+  int32_t config_horiz_mixing;             /* HEXDYN_MIXING_* */
+  float config_visc4_2dsmag;               /* >= 0 */
+  float config_smagorinsky_coef;           /* >= 0 */
+  float config_del4u_div_factor;           /* > 0 */
+  int32_t config_h_ScaleWithMesh;          /* bool */
+  float config_mpas_cam_coef;              /* must be 0 (the CAM floor is refused) */
 } hexdyn_config;
 
 /* Prognostic state and the carried time-level-one diagnostics
@@ -150,6 +168,7 @@ enum {
   HEXDYN_F_EULER_RU, HEXDYN_F_EULER_RW,
   HEXDYN_F_FLUX_U, HEXDYN_F_FLUX_W, HEXDYN_F_SCALARS_STAGE,
   HEXDYN_F_QTOT, HEXDYN_F_CQW, HEXDYN_F_CQU,
+  HEXDYN_F_KDIFF, HEXDYN_F_EULER_THETA,
   HEXDYN_F_COUNT
 };
 

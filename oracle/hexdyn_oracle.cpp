@@ -116,6 +116,10 @@ template <class T> struct Oracle {
   std::vector<T> v, pv_edge, ke, divergence, vorticity, ke_vertex, pv_vertex, pv_cell;
   std::vector<T> flux_u, flux_w, split_u, split_w;
   std::vector<T> q_stage;
+  // horizontal mixing
+  int mixing = 0;
+  std::vector<T> area, tri, density, defc2, defs2, defcs;  // areaCell, areaTriangle, meshDensity, weights
+  std::vector<T> kdiff, euler_theta;
   Stage dyn[3], sca[3];
   int failed = 0;
   int in_step = 0;
@@ -164,6 +168,19 @@ template <class T> struct Oracle {
     adv3 = cast(m->adv_coefs_3rd, (size_t)E * MA);
     dcdv.resize(E);
     for (int e = 0; e < E; ++e) dcdv[e] = dcv[e] * dv[e];
+    mixing = c->config_horiz_mixing == HEXDYN_MIXING_2D_SMAGORINSKY;
+    if (mixing) {
+      if (!m->deformation_coef_c2 || !m->deformation_coef_s2 || !m->deformation_coef_cs || !m->area_cell ||
+          !m->area_triangle)
+        return 1;
+      area = cast(m->area_cell, C);
+      tri = cast(m->area_triangle, V);
+      if (m->mesh_density) density = cast(m->mesh_density, C);
+      else density.assign(C, T(1));
+      defc2 = cast(m->deformation_coef_c2, (size_t)C * ME);
+      defs2 = cast(m->deformation_coef_s2, (size_t)C * ME);
+      defcs = cast(m->deformation_coef_cs, (size_t)C * ME);
+    }
     csign.assign((size_t)C * ME, T(0));
     for (int cc = 0; cc < C; ++cc)
       for (int s = 0; s < neoc[cc]; ++s) {
@@ -193,7 +210,8 @@ template <class T> struct Oracle {
     for (auto *a : {&rho, &rt, &theta_m, &exner, &rho_p, &rtheta_p, &pp, &qtot, &cqw, &o_rho, &c_rho, &c_rt,
                     &s_rho, &s_rt, &c_theta, &c_rho_p, &c_rtheta_p, &s_theta, &s_rho_p, &s_rtheta_p, &s_exner,
                     &cofwr, &cofwz, &cofwt, &atri, &btri, &ctri, &alpha, &gamma, &dpdz, &tend_rho, &tend_rt,
-                    &hdiv, &rho_pp, &rtheta_pp, &rtheta_pp_old, &ke, &divergence, &pv_cell})
+                    &hdiv, &rho_pp, &rtheta_pp, &rtheta_pp_old, &ke, &divergence, &pv_cell, &kdiff,
+                    &euler_theta})
       z(*a, cL());
     for (auto *a : {&ru, &u, &cqu, &c_ru, &s_ru, &euler_ru, &tend_ru, &ru_p, &ru_avg, &v, &pv_edge, &flux_u, &split_u})
       z(*a, eL());
@@ -410,6 +428,161 @@ template <class T> struct Oracle {
         euler_rw[ci(k, cc)] = NM <= 0 ? -inner : -(cqw[ci(k, cc)] * inner);
       }
     }
+    if (mixing) horizontal_mixing();
+  }
+
+  // ------------------------------------------------------ horizontal mixing
+  // v8.4.1 2-D Smagorinsky mixing from the subcycle-start state, in the host
+  // order of compute_dry_mixing_tendencies_v841 (cuda_horizontal_v841.py:1270-1588)
+  // with the per-element arithmetic of its kernels (:205-583): viscosity,
+  // del2 u into the u pool, del4 u, del2/del4 theta (the theta pool), del2/del4 w
+  // added to the w pool.
+  void horizontal_mixing() {
+    const T one = T(1), half = T(0.5), quarter = T(0.25);
+    const T len = T(cfg.config_len_disp), csm = T(cfg.config_smagorinsky_coef);
+    const T strain_scale = (csm * len) * (csm * len);
+    const T inv_dt = one / T(cfg.config_dt);
+    const T ceiling = (T(0.01) * (len * len)) * inv_dt;
+    const T h4 = T(cfg.config_visc4_2dsmag) * ((len * len) * len);
+    const bool mix4 = h4 > T(0);
+    std::vector<T> del2(E, one), del4(E, one);
+    if (cfg.config_h_ScaleWithMesh)
+      for (int e = 0; e < E; ++e) {
+        const T mean = half * (density[coe[2 * (size_t)e]] + density[coe[2 * (size_t)e + 1]]);
+        del2[e] = one / power(mean, T(0.25));
+        del4[e] = one / power(mean, T(0.75));
+      }
+    auto vi = [&](int k, int vx) { return (size_t)k * V + vx; };
+    auto rho_e = [&](int k, int e) {
+      return half * (c_rho[ci(k, coe[2 * (size_t)e])] + c_rho[ci(k, coe[2 * (size_t)e + 1])]);
+    };
+    // viscosity (smagorinsky_v841_f32)
+#pragma omp parallel for
+    for (int cc = 0; cc < C; ++cc)
+      for (int k = 0; k < L; ++k) {
+        T dudx = T(0), dudy = T(0), dvdx = T(0), dvdy = T(0);
+        for (int s = 0; s < neoc[cc]; ++s) {
+          const size_t o = (size_t)cc * ME + s;
+          const int e = eoc[o];
+          const T uu = u[ei(k, e)], vv = v[ei(k, e)];
+          dudx = (dudx + defc2[o] * uu) - defcs[o] * vv;
+          dudy = (dudy + defcs[o] * uu) - defs2[o] * vv;
+          dvdx = (dvdx + defcs[o] * uu) + defc2[o] * vv;
+          dvdy = (dvdy + defs2[o] * uu) + defcs[o] * vv;
+        }
+        const T d11 = T(2) * dudx, d22 = T(2) * dvdy, d12 = dudy + dvdx;
+        const T diff = d11 - d22;
+        const T strain = std::sqrt(quarter * (diff * diff) + d12 * d12);
+        kdiff[ci(k, cc)] = std::fmin(strain_scale * strain, ceiling);
+      }
+    // del2 u (momentum_filter_seed_v841_f32)
+    std::vector<T> delsq_u(eL());
+#pragma omp parallel for
+    for (int e = 0; e < E; ++e) {
+      const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
+      const int v0 = voe[2 * (size_t)e], v1 = voe[2 * (size_t)e + 1];
+      const T idc = one / dcv[e];
+      const T idv = std::fmin(one / dv[e], T(4) * idc);
+      for (int k = 0; k < L; ++k) {
+        const T lap = (divergence[ci(k, c1)] - divergence[ci(k, c0)]) * idc -
+                      (vorticity[vi(k, v1)] - vorticity[vi(k, v0)]) * idv;
+        const T kedge = half * (kdiff[ci(k, c0)] + kdiff[ci(k, c1)]);
+        delsq_u[ei(k, e)] = lap;
+        euler_ru[ei(k, e)] = euler_ru[ei(k, e)] + ((rho_e(k, e) * kedge) * lap) * del2[e];
+      }
+    }
+    if (mix4) {
+      // laplacian_divergence / laplacian_vorticity / momentum_filter_lap4
+      std::vector<T> ddiv(cL()), dvort((size_t)L * V);
+#pragma omp parallel for
+      for (int cc = 0; cc < C; ++cc) {
+        const T ia = one / area[cc];
+        for (int k = 0; k < L; ++k) {
+          T value = T(0);
+          for (int s = 0; s < neoc[cc]; ++s) {
+            const int e = eoc[(size_t)cc * ME + s];
+            value = value + ((ia * dv[e]) * sgn(cc, s)) * delsq_u[ei(k, e)];
+          }
+          ddiv[ci(k, cc)] = value;
+        }
+      }
+#pragma omp parallel for
+      for (int vx = 0; vx < V; ++vx) {
+        const T it = one / tri[vx];
+        for (int k = 0; k < L; ++k) {
+          T value = T(0);
+          for (int s = 0; s < 3; ++s) {
+            const int e = eov[3 * (size_t)vx + s];
+            const T vs = voe[2 * (size_t)e] == vx ? T(-1) : T(1);
+            value = value + ((it * dcv[e]) * vs) * delsq_u[ei(k, e)];
+          }
+          dvort[vi(k, vx)] = value;
+        }
+      }
+#pragma omp parallel for
+      for (int e = 0; e < E; ++e) {
+        const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
+        const int v0 = voe[2 * (size_t)e], v1 = voe[2 * (size_t)e + 1];
+        const T idc = one / dcv[e];
+        const T idv = std::fmin(one / dv[e], T(4) * idc);
+        const T scale = del4[e] * h4;
+        const T r_dc = (scale * T(cfg.config_del4u_div_factor)) * idc;
+        const T r_dv = scale * idv;
+        for (int k = 0; k < L; ++k) {
+          const T diffusion = rho_e(k, e) * ((ddiv[ci(k, c1)] - ddiv[ci(k, c0)]) * r_dc -
+                                             (dvort[vi(k, v1)] - dvort[vi(k, v0)]) * r_dv);
+          euler_ru[ei(k, e)] = euler_ru[ei(k, e)] - diffusion;
+        }
+      }
+    }
+    // theta (theta_filter_lap2/lap4_v841_f32) and w (w_filter_lap2/lap4_v841_f32)
+    std::vector<T> dtheta(cL()), dw(cL(), T(0)), tw(wL(), T(0));
+#pragma omp parallel for
+    for (int cc = 0; cc < C; ++cc) {
+      const T ia = one / area[cc];
+      for (int k = 0; k < L; ++k) {
+        T lap = T(0), tend = T(0), lapw = T(0), tendw = T(0);
+        for (int s = 0; s < neoc[cc]; ++s) {
+          const int e = eoc[(size_t)cc * ME + s];
+          const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
+          const T idc = one / dcv[e];
+          const T factor = ((ia * sgn(cc, s)) * dv[e]) * idc;
+          const T flux = (factor * (s_theta[ci(k, c1)] - s_theta[ci(k, c0)])) * rho_e(k, e);
+          lap = lap + flux;
+          tend = tend + ((flux * half) * (kdiff[ci(k, c0)] + kdiff[ci(k, c1)])) * del2[e];
+          if (k >= 1) {
+            const T fw = (((half * ia) * sgn(cc, s)) * dv[e]) * idc;
+            const T fluxw = (fw * (rho_e(k, e) + rho_e(k - 1, e))) * (w[ci(k, c1)] - w[ci(k, c0)]);
+            lapw = lapw + fluxw;
+            const T sum_k = ((kdiff[ci(k, c0)] + kdiff[ci(k, c1)]) + kdiff[ci(k - 1, c0)]) + kdiff[ci(k - 1, c1)];
+            tendw = tendw + ((fluxw * del2[e]) * quarter) * sum_k;
+          }
+        }
+        dtheta[ci(k, cc)] = lap;
+        euler_theta[ci(k, cc)] = tend;
+        dw[ci(k, cc)] = lapw;
+        tw[ci(k, cc)] = tendw;
+      }
+    }
+    if (mix4) {
+#pragma omp parallel for
+      for (int cc = 0; cc < C; ++cc) {
+        const T ia = one / area[cc];
+        for (int k = 0; k < L; ++k) {
+          T tend = euler_theta[ci(k, cc)], tendw = tw[ci(k, cc)];
+          for (int s = 0; s < neoc[cc]; ++s) {
+            const int e = eoc[(size_t)cc * ME + s];
+            const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
+            const T factor = (((del4[e] * (h4 * ia)) * dv[e]) * sgn(cc, s)) * (one / dcv[e]);
+            tend = tend - factor * (dtheta[ci(k, c1)] - dtheta[ci(k, c0)]);
+            if (k >= 1) tendw = tendw - factor * (dw[ci(k, c1)] - dw[ci(k, c0)]);
+          }
+          euler_theta[ci(k, cc)] = tend;
+          tw[ci(k, cc)] = tendw;
+        }
+      }
+    }
+    for (size_t i = 0; i < wL(); ++i) euler_rw[i] = euler_rw[i] + tw[i];
   }
 
   void mass_flux_divergence(const std::vector<T> &r_u, std::vector<T> &out) {
@@ -513,6 +686,7 @@ template <class T> struct Oracle {
           }
         r = r * invarea[cc];
         tend_rt[ci(k, cc)] = r - rdzw[k] * (vf[k + 1] - vf[k]);
+        if (mixing) tend_rt[ci(k, cc)] = tend_rt[ci(k, cc)] + euler_theta[ci(k, cc)];
       }
     }
     // w transport (stored-inverse branch, driver.py:1140-1207), Euler pool, omega conversion
@@ -1071,6 +1245,8 @@ template <class T> struct Oracle {
       case HEXDYN_F_QTOT: return &qtot;
       case HEXDYN_F_CQW: return &cqw;
       case HEXDYN_F_CQU: return &cqu;
+      case HEXDYN_F_KDIFF: return &kdiff;
+      case HEXDYN_F_EULER_THETA: return &euler_theta;
       default: return nullptr;
     }
   }
@@ -1126,12 +1302,17 @@ extern "C" {
 void *oracle_create(const hexdyn_mesh *m, const hexdyn_vertical *v, const hexdyn_config *c, int precision) {
   Handle *h = new Handle();
   h->precision = precision == 64 ? 64 : 32;
+  int rc;
   if (h->precision == 32) {
     h->f.reset(new Oracle<float>());
-    h->f->init(m, v, c);
+    rc = h->f->init(m, v, c);
   } else {
     h->d.reset(new Oracle<double>());
-    h->d->init(m, v, c);
+    rc = h->d->init(m, v, c);
+  }
+  if (rc) {
+    delete h;
+    return nullptr;
   }
   return h;
 }

@@ -84,6 +84,8 @@ class Oracle:
         self.case = case
         self.mesh_s, self.vert_s, self.cfg_s, self._bundle = capi.build_structs(case)
         self.h = lib().oracle_create(ct.byref(self.mesh_s), ct.byref(self.vert_s), ct.byref(self.cfg_s), precision)
+        if not self.h:
+            raise ValueError("oracle refused the case (missing mixing inputs)")
         self.shapes = state_shapes(case)
 
     def __del__(self) -> None:

@@ -20,10 +20,10 @@ _CASES = {}
 def get_case():
     from hexdyn.case import build_case
 
-    def get(name, **kw):
-        key = (name, tuple(sorted(kw.items())))
+    def get(name, config=None, **kw):
+        key = (name, repr(config), tuple(sorted(kw.items())))
         if key not in _CASES:
-            _CASES[key] = build_case(name, **kw)
+            _CASES[key] = build_case(name, config=config, **kw)
         return _CASES[key]
 
     return get

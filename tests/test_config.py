@@ -14,7 +14,10 @@ def test_defaults_valid():
     {"config_dynamics_split_steps": 2},
     {"config_dynamics_split_steps": 3, "config_split_dynamics_transport": False},
     {"config_apvm_upwinding": -0.1},
-    {"config_horiz_mixing": "2d_smagorinsky"},
+    {"config_horiz_mixing": "2d_fixed"},
+    {"config_horiz_mixing": "2d_smagorinsky", "config_mpas_cam_coef": 0.1},
+    {"config_horiz_mixing": "2d_smagorinsky", "config_visc4_2dsmag": -1.0},
+    {"config_horiz_mixing": "2d_smagorinsky", "config_del4u_div_factor": 0.0},
     {"numerics": "loose"},
 ])
 def test_refusals(raw):

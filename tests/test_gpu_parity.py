@@ -54,6 +54,7 @@ def _phase_walk(case):
                 bad.append(f"{tag}:{f}: {_diff(a, b)}")
 
     state = STATE + (["SCALARS"] if case.n_scalars else [])
+    mixing = case.config.config_horiz_mixing != "off"
     cmp("init", state)
     o.begin_step(); d.phase("begin_step")
     cmp("begin_step", ["V", "PV_EDGE", "KE", "DIVERGENCE", "VORTICITY"] + (["QTOT", "CQW", "CQU"] if case.n_moist else []))
@@ -63,7 +64,7 @@ def _phase_walk(case):
     for sub in range(1, cfg.config_dynamics_split_steps + 1):
         o.begin_subcycle(sub); d.phase("begin_subcycle", sub)
         cmp(f"s{sub}", ["A_TRI", "ALPHA_TRI", "GAMMA_TRI", "COFWZ", "COFWT", "COFTZ", "EULER_RU", "EULER_RW",
-                        "TEND_RHO"])
+                        "TEND_RHO"] + (["KDIFF", "EULER_THETA"] if mixing else []))
         for st in (1, 2, 3):
             o.dyn_tend(st); d.phase("dyn_tend", st)
             cmp(f"s{sub}t{st}", ["TEND_RU", "TEND_RT", "TEND_RW"])
@@ -93,6 +94,20 @@ def test_phase_by_phase_bitwise(get_case, name):
     assert not bad, "\n".join(bad[:20])
 
 
+def _mixing(get_case, name, **kw):
+    import dataclasses
+
+    base = get_case(name)
+    return get_case(name, config=dataclasses.replace(base.config, config_horiz_mixing="2d_smagorinsky", **kw))
+
+
+@pytest.mark.parametrize("name,visc4", [("x1.642-moist", 0.05), ("x1.2562-dry", 0.05), ("x1.2562-dry", 0.0)])
+def test_phase_by_phase_bitwise_with_mixing(get_case, name, visc4):
+    print(f"[gpu] phase walk with 2d_smagorinsky mixing {name} visc4={visc4}", flush=True)
+    bad = _phase_walk(_mixing(get_case, name, config_visc4_2dsmag=visc4))
+    assert not bad, "\n".join(bad[:20])
+
+
 def test_three_steps_bitwise_moist(get_case):
     print("[gpu] x1.2562-moist 3 steps", flush=True)
     case = get_case("x1.2562-moist")
@@ -106,9 +121,10 @@ def test_three_steps_bitwise_moist(get_case):
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
 
 
-def test_headline_mesh_step_bitwise_and_conservative(get_case):
-    print("[gpu] x1.40962-moist 1 step", flush=True)
-    case = get_case("x1.40962-moist")
+@pytest.mark.parametrize("mixing", ["off", "2d_smagorinsky"])
+def test_headline_mesh_step_bitwise_and_conservative(get_case, mixing):
+    print(f"[gpu] x1.40962-moist 1 step, mixing {mixing}", flush=True)
+    case = get_case("x1.40962-moist") if mixing == "off" else _mixing(get_case, "x1.40962-moist")
     d = _driver(case)
     s0 = d.get_state()
     d.step_device(1)

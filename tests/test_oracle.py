@@ -166,3 +166,105 @@ def test_fp32_and_fp64_oracles_agree(get_case):
     for name in ("rho_zz", "rtheta_m", "u", "theta_m"):
         rel = np.linalg.norm(a[name] - b[name]) / np.linalg.norm(b[name])
         assert rel < 1e-5, (name, rel)
+
+
+# ----------------------------------------------------------- horizontal mixing
+def _mixing_config(case, **kw):
+    return dataclasses.replace(case.config, config_horiz_mixing="2d_smagorinsky", **kw)
+
+
+def test_mixing_rest_state_is_exact_fixed_point_without_hyperdiffusion(get_case):
+    # at rest u = v = 0, so the Smagorinsky viscosity, the del2 filters and
+    # (visc4 = 0) the del4 filters all vanish exactly
+    base = get_case("x1.642-dry")
+    case = _rest_case(get_case("x1.642-dry", config=_mixing_config(base, config_visc4_2dsmag=0.0)))
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    before = o.get_state()
+    assert o.step(1) == 0
+    after = o.get_state()
+    for name in ("rho_zz", "rtheta_m", "rw", "exner", "pressure_p", "w"):
+        assert np.array_equal(before[name], after[name]), name
+    assert np.abs(after["u"]).max() == 0.0
+
+
+def test_mixing_theta_filter_is_in_flux_form_fp64(get_case):
+    # the del2 and del4 theta tendencies are divergences of edge fluxes:
+    # their area-weighted sum over the sphere vanishes at every level
+    base = get_case("x1.2562-dry")
+    case = get_case("x1.2562-dry", config=_mixing_config(base))
+    o = Oracle(case, 64)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    o.begin_step()
+    o.begin_subcycle(1)
+    L, C = case.n_levels, case.mesh.n_cells
+    tend = o.export_field_f64("EULER_THETA").reshape(L, C)
+    area = case.mesh.areaCell.astype(np.float32).astype(np.float64)[None, :]
+    total = (area * tend).sum(axis=1)
+    scale = (area * np.abs(tend)).sum(axis=1)
+    assert np.all(scale > 0.0)
+    assert np.all(np.abs(total) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("flow,strain", [("shear_y", 1.0), ("shear_xy", 2.0), ("stretch_x", 1.0),
+                                         ("rotation", 0.0)])
+def test_mixing_viscosity_of_linear_winds_fp64(get_case, flow, strain):
+    # linear winds near (lat, lon) = (0, 0) with x east and y north:
+    # kdiff = (c_s*len)^2 * sqrt(0.25*(D11-D22)^2 + D12^2), D11 = 2 du/dx,
+    # D22 = 2 dv/dy, D12 = du/dy + dv/dx (mpas_atm_dissipation_models.F smagorinsky_2d)
+    base = get_case("x1.2562-dry")
+    case = _rest_case(get_case("x1.2562-dry", config=_mixing_config(base)))
+    m = case.mesh
+    R = m.sphere_radius
+    a = 2e-6
+    y = R * np.arcsin(m.zEdge / R)
+    x = R * np.arctan2(m.yEdge, m.xEdge)
+    east, north = {"shear_y": (a * y, 0 * y), "shear_xy": (a * y, a * x), "stretch_x": (a * x, 0 * y),
+                   "rotation": (a * y, -a * x)}[flow]
+    u = (east * np.cos(m.angleEdge) + north * np.sin(m.angleEdge)).astype(np.float32)
+    o = Oracle(case, 64)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    o.import_field("U", np.broadcast_to(u, (case.n_levels, m.n_edges)))
+    o.begin_step()
+    o.begin_subcycle(1)
+    kd = o.export_field_f64("KDIFF").reshape(case.n_levels, m.n_cells)
+    near = (np.abs(np.arcsin(m.zCell / R)) < 0.15) & (np.abs(np.arctan2(m.yCell, m.xCell)) < 0.15)
+    assert near.sum() >= 10
+    scale = (0.125 * np.float32(case.len_disp)) ** 2 * a
+    err = np.median(np.abs(kd[:, near] / scale - strain))
+    assert err < 0.05 * max(strain, 1.0), (flow, err)
+    assert np.all(kd <= (0.01 * np.float32(case.len_disp) ** 2) / case.config.config_dt * (1 + 1e-6))
+
+
+@pytest.mark.parametrize("mixing", ["off", "2d_smagorinsky"])
+def test_mass_conserved_with_and_without_mixing_fp64(get_case, mixing):
+    base = get_case("x1.642-moist")
+    case = get_case("x1.642-moist", config=dataclasses.replace(base.config, config_horiz_mixing=mixing))
+    o = Oracle(case, 64)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    area = 1.0 / case.inv_area_cell.astype(np.float64)[None, :]
+    shape = (case.n_levels, case.mesh.n_cells)
+    m0 = (area * o.export_field_f64("RHO_ZZ").reshape(shape) * _dz(case)).sum()
+    assert o.step(1) == 0
+    m1 = (area * o.export_field_f64("RHO_ZZ").reshape(shape) * _dz(case)).sum()
+    assert abs(m1 / m0 - 1) < 1e-13
+
+
+def test_mixing_theta_del2_is_dissipative_fp64(get_case):
+    # sum_c A_c theta_c tend_c = -sum_e (dv/dc) rho_e K_e (theta1 - theta0)^2 <= 0 for
+    # the second-order filter (visc4 = 0): a sign or index error in any
+    # flux term makes it positive somewhere
+    base = get_case("x1.2562-dry")
+    case = get_case("x1.2562-dry", config=_mixing_config(base, config_visc4_2dsmag=0.0))
+    o = Oracle(case, 64)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    o.begin_step()
+    o.begin_subcycle(1)
+    L, C = case.n_levels, case.mesh.n_cells
+    tend = o.export_field_f64("EULER_THETA").reshape(L, C)
+    theta = o.export_field_f64("THETA_M").reshape(L, C)
+    area = case.mesh.areaCell.astype(np.float32).astype(np.float64)[None, :]
+    diss = (area * theta * tend).sum(axis=1)
+    assert np.all(diss < 0.0)
+    # and the filter is active: each level loses a visible amount
+    assert np.all(np.abs(diss) > 1e-12 * (area * theta * np.abs(tend)).sum(axis=1))

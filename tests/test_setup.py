@@ -94,3 +94,59 @@ def test_advection_coefficient_properties(mesh):
     exact = mid @ g
     avg = 0.5 * (f[mesh.cellsOnEdge[:, 0]] + f[mesh.cellsOnEdge[:, 1]])
     assert np.abs(val - exact).max() < 0.5 * np.abs(avg - exact).max()
+
+
+def _edge_winds(mesh, east, north):
+    """Normal and tangential edge components of an (east, north) wind."""
+    ang = mesh.angleEdge
+    return east * np.cos(ang) + north * np.sin(ang), -east * np.sin(ang) + north * np.cos(ang)
+
+
+def _deformation(mesh, weights, un, vt):
+    c2, s2, cs = (w.astype(np.float64) for w in weights)
+    out = [np.zeros(mesh.n_cells) for _ in range(4)]
+    for s in range(mesh.max_edges):
+        ok = s < mesh.nEdgesOnCell
+        e = mesh.edgesOnCell[:, s]
+        u, v = np.where(ok, un[e], 0.0), np.where(ok, vt[e], 0.0)
+        out[0] += c2[:, s] * u - cs[:, s] * v      # du/dx
+        out[1] += cs[:, s] * u - s2[:, s] * v      # du/dy
+        out[2] += cs[:, s] * u + c2[:, s] * v      # dv/dx
+        out[3] += s2[:, s] * u + cs[:, s] * v      # dv/dy
+    return out
+
+
+def test_deformation_weights_differentiate_linear_winds():
+    # The weights are the divergence-theorem gradient on the cell's tangent
+    # plane (x east, y north): a linear shear u = a*y must give du/dy = a and
+    # zero elsewhere; a pure stretch v = b*y gives dv/dy = b.
+    m = icosahedral_mesh(4)
+    w = S.deformation_weights(m)
+    R = m.sphere_radius
+    lat_e = np.arcsin(m.zEdge / R)
+    near_eq = np.abs(np.arcsin(m.zCell / R)) < 0.2
+    a = 1e-5
+    un, vt = _edge_winds(m, a * R * lat_e, 0.0 * lat_e)
+    dudx, dudy, dvdx, dvdy = _deformation(m, w, un, vt)
+    assert np.median(np.abs(dudy[near_eq] / a - 1.0)) < 0.01
+    for g in (dudx, dvdx, dvdy):
+        assert np.median(np.abs(g[near_eq])) < 0.02 * a
+    un, vt = _edge_winds(m, 0.0 * lat_e, a * R * lat_e)
+    dudx, dudy, dvdx, dvdy = _deformation(m, w, un, vt)
+    assert np.median(np.abs(dvdy[near_eq] / a - 1.0)) < 0.01
+    assert np.median(np.abs(dudx[near_eq])) < 0.02 * a
+
+
+def test_deformation_weights_sign_follows_edge_orientation():
+    # times the cell's outward sign the weights are orientation free: over a
+    # cell's sides c2 + s2 = dl/A sums to the tangent-plane perimeter over
+    # area, which is the spherical perimeter over area to O(dc^2/R^2)
+    m = icosahedral_mesh(3)
+    c2, s2, cs = (x.astype(np.float64) for x in S.deformation_weights(m))
+    out_sign = np.where(m.cellsOnEdge[m.edgesOnCell, 0] == np.arange(m.n_cells)[:, None], 1.0, -1.0)
+    used = np.arange(m.max_edges)[None, :] < m.nEdgesOnCell[:, None]
+    per_over_area = np.where(used, (c2 + s2) * out_sign, 0.0).sum(axis=1)
+    dl_sum = np.where(used, m.dvEdge[m.edgesOnCell], 0.0).sum(axis=1)
+    assert np.allclose(per_over_area, dl_sum / m.areaCell, rtol=2e-2)
+    # a flipped sign on any one side would move that cell's sum by 2 dl/A
+    assert np.all(per_over_area > 0.5 * dl_sum / m.areaCell)
