@@ -1,6 +1,8 @@
 #!/usr/bin/env python3
 """Headline benchmark: simulated days per wall-clock day of the plain dycore
-on x1.40962 L55 moist (T=6, FCT transport), dt 720 s, one MI355X.
+on x1.40962 L55 moist (T=6, FCT transport), dt 720 s, one MI355X, with the
+v8.4.1 2-D Smagorinsky horizontal mixing (the MPAS default; --mixing off for
+the unmixed step).
 
 A step is one large step (3 dynamics subcycles x RK3 with acoustic substeps,
 then split FCT transport) of the whole mesh with the state resident in HBM.
@@ -35,6 +37,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="x1.40962-moist")
     p.add_argument("--numerics", default="strict", choices=["strict", "fast"])
+    p.add_argument("--mixing", default="2d_smagorinsky", choices=["2d_smagorinsky", "off"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-steps", type=int, default=0, help="CPU sample size in large steps (0: auto ~15 s)")
     p.add_argument("--profile-steps", type=int, default=2)
@@ -86,7 +89,7 @@ def main():
     from hexdyn.case import WORKLOADS
 
     spec = WORKLOADS[args.workload]
-    cfg = DycoreConfig(config_dt=spec["dt"], numerics=args.numerics)
+    cfg = DycoreConfig(config_dt=spec["dt"], numerics=args.numerics, config_horiz_mixing=args.mixing)
     t_setup = time.perf_counter()
     case = build_case(args.workload, config=cfg)
     drv = HexDycoreDriver.from_host(case)
@@ -156,7 +159,7 @@ def main():
                         acoustic_substeps=cfg.config_number_of_sub_steps,
                         dynamics_splits=cfg.config_dynamics_split_steps)
     n_ac = sum(s.acoustic_steps for s in sched)
-    b_alg = step_bytes(C=C, L=L, T=T, max_edges=case.mesh.max_edges, n_ac=n_ac, mix=False,
+    b_alg = step_bytes(C=C, L=L, T=T, max_edges=case.mesh.max_edges, n_ac=n_ac, mix=args.mixing != "off",
                        splits=cfg.config_dynamics_split_steps)
     step_gbs = b_alg / (ms_per_step * 1e-3) / 1e9
     step_roofline = {"alg_bytes_per_step": b_alg, "achieved": round(step_gbs, 1), "unit": "GB/s",
@@ -184,6 +187,7 @@ def main():
         "data": "synthetic (seeded icosahedral SCVT mesh, JW baroclinic jet, random terrain, seeded moisture)",
         "config": {"workload": args.workload, "cells": C, "edges": E, "levels": L, "scalars": T,
                    "dt": cfg.config_dt, "form": "plain", "numerics": args.numerics,
+                   "horiz_mixing": args.mixing,
                    "dynamics_split_steps": cfg.config_dynamics_split_steps,
                    "number_of_sub_steps": cfg.config_number_of_sub_steps, "parallelism": "single-gpu"},
         "roofline": roofline,

@@ -156,6 +156,7 @@ struct hexdyn_handle {
   std::string error;
   std::vector<void *> allocs;
   int *old_c = nullptr, *old_e = nullptr, *old_v = nullptr;  // device: old index of each new entity
+  double *int_partial = nullptr, *int_out = nullptr;          // integral partials and results
   std::vector<int> old_c_h, old_e_h, old_v_h;
 
   template <class T> int alloc(T **p, size_t count) {
@@ -494,6 +495,16 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   rc |= h->upload(&Z.rb, rows_cells(vt->rho_base, L)); rc |= h->upload(&Z.rtb, rows_cells(vt->rtheta_base, L));
   rc |= h->upload(&Z.exb, rows_cells(vt->exner_base, L));
   rc |= h->upload(&Z.zxu, rows_edges(vt->zxu, L));
+  if (vt->zgrid) {
+    std::vector<float> zm((size_t)C * HX_ROW, 0.0f);
+    for (int nc = 0; nc < C; ++nc)
+      for (int k = 0; k < L; ++k) {
+        const int cc = h->old_c_h[nc];
+        zm[(size_t)nc * HX_ROW + k] =
+            (float)(0.5 * ((double)vt->zgrid[(size_t)k * C + cc] + (double)vt->zgrid[(size_t)(k + 1) * C + cc]));
+      }
+    rc |= h->upload(&Z.zmid, zm);
+  }
   {
     std::vector<float> zb((size_t)C * ME * HX_ROW, 0.0f), zb3((size_t)C * ME * HX_ROW, 0.0f);
     for (int nc = 0; nc < C; ++nc) {
@@ -539,6 +550,8 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
     rc |= h->alloc(&B.delsq_vort, vr);
   }
   rc |= h->alloc(&B.flag, 16);
+  rc |= h->alloc(&h->int_partial, (size_t)((C + 3) / 4) * (3 + n.NT));
+  rc |= h->alloc(&h->int_out, (size_t)(3 + n.NT));
   if (rc) { int r = HEXDYN_E_HIP; hexdyn_destroy(h); return r; }
   if (n.NM == 0) {
     std::vector<float> ones(std::max(cr, er), 1.0f);
@@ -877,7 +890,9 @@ int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream) {
         for (int ss = 1; rc == 0 && ss <= h->I.k.steps[st - 1]; ++ss)
           rc |= NS_CALL(h, acoustic_substep, h->I, st, ss, (dd && ss > 1) ? 1 : 0, s);
         rc |= NS_CALL(h, recover, h->I, st, sub, dd, s);
-        rc |= NS_CALL(h, solve_diagnostics, h->I, st, s);
+        // the last stage's diagnostics would only feed a subcycle that does not
+        // follow: the next step's begin_step recomputes them from the published state
+        if (!(sub == c.config_dynamics_split_steps && st == 3)) rc |= NS_CALL(h, solve_diagnostics, h->I, st, s);
       }
     }
     if (rc == 0 && c.config_scalar_advection && c.config_split_dynamics_transport && h->I.n.NT > 0)
@@ -890,6 +905,22 @@ int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream) {
     rc = run_end_step(h, s);
     if (rc) return rc;
   }
+  return 0;
+}
+
+int hexdyn_integrals(hexdyn_handle *h, double *out, int n_out, void *stream) {
+  if (!h || !out) return HEXDYN_E_INVALID;
+  const int NQ = 3 + h->I.n.NT;
+  if (n_out < NQ) return set_error(h, HEXDYN_E_INVALID, "hexdyn_integrals: n_out < 3 + n_scalars");
+  if (h->in_step) return set_error(h, HEXDYN_E_INVALID, "hexdyn_integrals: a step is in progress");
+  hipStream_t s = (hipStream_t)stream;
+  const Work &B = h->I.b;
+  if (hexdyn::integrals(h->I, B.rho, B.rt, B.q, B.p_theta, B.p_exner, B.p_u, h->I.z.zmid, h->int_partial,
+                        h->int_out, s))
+    return set_error(h, HEXDYN_E_HIP, "integral kernels failed to launch");
+  if (hipMemcpyAsync(out, h->int_out, NQ * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return set_error(h, HEXDYN_E_HIP, "integral copy failed");
   return 0;
 }
 

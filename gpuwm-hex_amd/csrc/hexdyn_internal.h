@@ -71,6 +71,7 @@ struct Vert {
   const float *zz, *dss, *rb, *rtb, *exb;  // cell rows
   const float *zxu;                        // edge rows
   const float *zb, *zb3;                   // [C][ME][64]
+  const float *zmid;                       // cell rows: layer-mid heights (null without zgrid)
 };
 
 // Device buffers of the working set (all row layout).
@@ -153,6 +154,11 @@ struct Instance {
 
 HEXDYN_PHASES(strict)
 HEXDYN_PHASES(fast)
+
+// FP64 global integrals (hexdyn_integrals.hip)
+int integrals(const Instance &I, const float *rho, const float *rt, const float *q, const float *theta,
+              const float *exner, const float *u, const float *zmid, double *partial, double *out,
+              hipStream_t s);
 
 }  // namespace hexdyn
 

@@ -43,12 +43,19 @@ namespace {
 #define ROW(p, ent) ((p) + (size_t)(ent) * HX_ROW)
 #define KB __global__ void __launch_bounds__(256)
 
-__device__ __forceinline__ int wave_entity() {
+// First entity of this block: blocks are renumbered so that XCD x (hardware
+// block b runs on XCD b % 8) owns a contiguous range of the curve.
+template <int WPB>
+__device__ __forceinline__ int block_entity0() {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
   const int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
   const int lb = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-  return __builtin_amdgcn_readfirstlane(lb * 4 + (int)(threadIdx.x >> 6));
+  return lb * WPB;
+}
+template <int WPB = 4>
+__device__ __forceinline__ int wave_entity() {
+  return __builtin_amdgcn_readfirstlane(block_entity0<WPB>() + (int)(threadIdx.x >> 6));
 }
 __device__ __forceinline__ int lane() { return (int)(threadIdx.x & 63); }
 __device__ __forceinline__ float up1(float x) { return __shfl_up(x, 1, 64); }     // value of lane k-1
@@ -82,6 +89,14 @@ __device__ __forceinline__ void slots(T (&out)[N], const T *__restrict__ p) {
 }
 
 inline int nblocks(int n) { return (n + 3) / 4; }
+
+// Column kernels: COLW cells (one wavefront each) per block.  The serial
+// vertical recurrences of the block's cells run together on the lanes of
+// one wavefront (lane = cell) through LDS rows padded to COLP floats, so
+// 16 columns cost one column's dependent chain instead of sixteen.
+constexpr int COLW = 16;
+constexpr int COLP = 65;
+#define KBCOL __global__ void __launch_bounds__(64 * COLW)
 
 // Gather helpers.  Connectivity rows are padded to fixed widths (NE = maxEdges
 // slots per cell, NX = 2*maxEdges-2 slots per edge) with index 0, so every
@@ -236,10 +251,13 @@ KB k_moist_edge(const Instance I) {
 }
 
 // ------------------------------------------------------ implicit coefficients
-KB k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const float *__restrict__ theta,
-           const float *__restrict__ rtp) {
-  const int c = wave_entity();
-  if (c >= I.n.C) return;
+KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const float *__restrict__ theta,
+              const float *__restrict__ rtp) {
+  __shared__ float s_a[COLW * COLP], s_b[COLW * COLP], s_c[COLW * COLP];
+  const int w = (int)(threadIdx.x >> 6);
+  const int c0 = wave_entity<COLW>();
+  const bool live = c0 < I.n.C;
+  const int c = live ? c0 : 0;  // every wave reaches the barriers
   const int k = lane();
   const int L = I.n.L;
   const int km = cl(k - 1), kp = cl(k + 1);
@@ -276,19 +294,31 @@ KB k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const flo
     cc = (cc * etp) * I.z.ewp[kp];
     if (k == L - 1) cc = 0.0f;
   }
-  float alpha = 0.0f, gamma = 0.0f;
-  int singular = 0;
-  for (int j = 1; j < L; ++j) {
-    const float gm = rdl(gamma, j - 1);
-    if (k == j) {
-      const float den = 1.0f + dts2 * (b - a * gm);
+  // alpha/gamma recurrence up the column (acoustic_v841.py:81-233), lane = cell
+  s_a[w * COLP + k] = a;
+  s_b[w * COLP + k] = b;
+  s_c[w * COLP + k] = cc;
+  __syncthreads();
+  if (w == 0 && k < COLW) {
+    float *pa = s_a + k * COLP, *pb = s_b + k * COLP;
+    const float *pc = s_c + k * COLP;
+    float gm = 0.0f;
+    int singular = 0;
+    for (int j = 1; j < L; ++j) {
+      const float den = 1.0f + dts2 * (pb[j] - pa[j] * gm);
       if (den == 0.0f) singular = 1;
-      alpha = 1.0f / den;
-      gamma = (dts2 * cc) * alpha;
+      const float al = 1.0f / den;
+      gm = (dts2 * pc[j]) * al;
+      pa[j] = al;
+      pb[j] = gm;
     }
+    if (singular && block_entity0<COLW>() + k < I.n.C) set_flag(I.b.flag);
   }
-  if (singular) set_flag(I.b.flag);
-  if (k <= L) {
+  __syncthreads();
+  const bool inner = k >= 1 && k < L;
+  const float alpha = inner ? s_a[w * COLP + k] : 0.0f;
+  const float gamma = inner ? s_b[w * COLP + k] : 0.0f;
+  if (live && k <= L) {
     ROW(I.b.coftz, c)[k] = coftz;
     if (k < L) {
       ROW(I.b.cofwr, c)[k] = cofwr;
@@ -838,9 +868,12 @@ KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_c
 }
 
 template <int NE>
-KB k_ac_cell(const Instance I, float dts, int first) {
-  const int c = wave_entity();
-  if (c >= I.n.C) return;
+KBCOL k_ac_cell(const Instance I, float dts, int first) {
+  __shared__ float s_x[COLW * COLP], s_d[COLW * COLP], s_l[COLW * COLP], s_g[COLW * COLP];
+  const int w = (int)(threadIdx.x >> 6);
+  const int c0 = wave_entity<COLW>();
+  const bool live = c0 < I.n.C;
+  const int c = live ? c0 : 0;  // every wave reaches the barriers
   const int k = lane();
   const int L = I.n.L;
   const int kp = cl(k + 1);
@@ -912,14 +945,29 @@ KB k_ac_cell(const Instance I, float dts, int first) {
   const float dts2 = dts * dts;
   const float a = ROW(I.b.atri, c)[k], al = ROW(I.b.alpha, c)[k], ga = ROW(I.b.gamma, c)[k];
   const float ad = dts2 * a;
-  for (int j = 1; j < L; ++j) {
-    const float prev = rdl(col, j - 1);
-    if (k == j) col = (col - ad * prev) * al;
+  // Thomas sweeps (acoustic_v841.py:296-565), lane = cell: forward for
+  // levels 1..L-1, back substitution for levels L-1..0
+  s_x[w * COLP + k] = col;
+  s_d[w * COLP + k] = ad;
+  s_l[w * COLP + k] = al;
+  s_g[w * COLP + k] = ga;
+  __syncthreads();
+  if (w == 0 && k < COLW) {
+    float *x = s_x + k * COLP;
+    const float *pd = s_d + k * COLP, *pl = s_l + k * COLP, *pg = s_g + k * COLP;
+    float p = x[0];
+    for (int j = 1; j < L; ++j) {
+      p = (x[j] - pd[j] * p) * pl[j];
+      x[j] = p;
+    }
+    p = x[L];
+    for (int j = L - 1; j >= 0; --j) {
+      p = x[j] - pg[j] * p;
+      x[j] = p;
+    }
   }
-  for (int j = L - 1; j >= 0; --j) {
-    const float next = rdl(col, j + 1);
-    if (k == j) col = col - ga * next;
-  }
+  __syncthreads();
+  col = s_x[w * COLP + k];
   if (k >= 1 && k < L) {
     const float delta = ROW(I.b.s_rw, c)[k] - ROW(I.b.c_rw, c)[k];
     const float damp = ROW(I.z.dss, c)[k];
@@ -933,11 +981,11 @@ KB k_ac_cell(const Instance I, float dts, int first) {
   const float ewp = I.z.ewp[k], ewp_p = I.z.ewp[kp];
   const float rpp_new = rs - (dts * rdzw) * (ewp_p * col_p - ewp * col);
   const float rtpp_new = ts - (dts * rdzw) * ((ewp_p * coftz_p) * col_p - (ewp * coftz) * col);
-  if (k <= L) {
+  if (live && k <= L) {
     ROW(I.b.rw_p, c)[k] = col;
     ROW(I.b.ww_avg, c)[k] = wwa;
   }
-  if (k < L) {
+  if (live && k < L) {
     ROW(I.b.rho_pp, c)[k] = rpp_new;
     ROW(I.b.rtheta_pp, c)[k] = rtpp_new;
     ROW(I.b.rtheta_pp_old, c)[k] = rtpp_old;
@@ -1430,6 +1478,16 @@ inline void launch_named(const char *name, K kernel, int n, int gy, hipStream_t 
 }
 #define launch(kernel, ...) launch_named(#kernel, kernel, __VA_ARGS__)
 
+// column kernels: COLW cells per block of 64*COLW threads
+template <class K, class... A>
+inline void launch_col_named(const char *name, K kernel, int n, hipStream_t s, A... args) {
+  if (n <= 0) return;
+  if (g_prof_on) prof_mark(name, s, 1);
+  hipLaunchKernelGGL(kernel, dim3((n + COLW - 1) / COLW), dim3(64 * COLW), 0, s, args...);
+  if (g_prof_on) prof_mark(name, s, 0);
+}
+#define launch_col(kernel, ...) launch_col_named(#kernel, kernel, __VA_ARGS__)
+
 inline int check() {
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : -1;
@@ -1469,7 +1527,7 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
   (void)sub;
   LAUNCH_NE(I, k_sub_cell, I.n.C, 1, s, I);
   launch(k_sub_edge, I.n.E, 1, s, I);
-  launch(k_vcoef, I.n.C, 1, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
+  launch_col(k_vcoef, I.n.C, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
          (const float *)I.b.s_rtheta_p);
   if (I.k.mixing) {
     LAUNCH_NE(I, k_mix_kdiff, I.n.C, 1, s, I);
@@ -1487,7 +1545,7 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
 
 int dyn_tend(const Instance &I, int stage, hipStream_t s) {
   if (stage == 2 && I.k.order == 3)
-    launch(k_vcoef, I.n.C, 1, s, I, I.k.dts[1], (const float *)I.b.s_exner, (const float *)I.b.theta,
+    launch_col(k_vcoef, I.n.C, s, I, I.k.dts[1], (const float *)I.b.s_exner, (const float *)I.b.theta,
            (const float *)I.b.rtheta_p);
   LAUNCH_NE(I, k_tend_mom, I.n.E, 1, s, I);
   LAUNCH_NE(I, k_tend_flux, I.n.E, 1, s, I, stage);
@@ -1498,7 +1556,8 @@ int dyn_tend(const Instance &I, int stage, hipStream_t s) {
 int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStream_t s) {
   const float dts = I.k.dts[stage - 1];
   launch(k_ac_edge, I.n.E, 1, s, I, dts, ss == 1 ? 1 : 0, damp_prev, I.k.dts_inv_coef[stage - 1]);
-  LAUNCH_NE(I, k_ac_cell, I.n.C, 1, s, I, dts, ss == 1 ? 1 : 0);
+  NEDISPATCH(I, launch_col_named("k_ac_cell", k_ac_cell<6>, I.n.C, s, I, dts, ss == 1 ? 1 : 0),
+             launch_col_named("k_ac_cell", k_ac_cell<7>, I.n.C, s, I, dts, ss == 1 ? 1 : 0));
   return check();
 }
 

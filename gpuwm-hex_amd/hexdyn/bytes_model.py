@@ -58,7 +58,6 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     "k_vcoef": [("C", "L")] * 10 + [("C", "W")],
     "k_sub_cell": [("C", "L")] * 12 + [("C", "W")] * 3 + [("E", "L")],
     "k_sub_edge": [("E", "L")] * 5 + [("C", "L")] * 3,
-    "k_hdiv": [("E", "L"), ("C", "L")],
     # u, c_rw, pv_edge, c_rho, ke, hdiv, euler_ru | tend_ru
     "k_tend_mom": [("E", "L"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
                    ("E", "L")],
@@ -94,6 +93,19 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     "k_fct_edge2": [("SC", "L"), ("SC", "L"), ("SE", "L"), ("SE", "L")],
     "k_fct_cell2": [("SC", "L"), ("SC", "L"), ("SC", "W"), ("SC", "L"), ("SE", "L"), ("C", "L"), ("SC", "L")],
     "k_split_scale": [("X", "W"), ("X", "W")],
+    # horizontal mixing, once per subcycle
+    # u, v | kdiff
+    "k_mix_kdiff": [("E", "L"), ("E", "L"), ("C", "L")],
+    # divergence, vorticity, kdiff, c_rho, euler_ru | euler_ru, delsq_u
+    "k_mix_edge": [("C", "L"), ("V", "L"), ("C", "L"), ("C", "L"), ("E", "L"), ("E", "L"), ("E", "L")],
+    "k_mix_lapdiv": [("E", "L"), ("C", "L")],
+    "k_mix_lapvort": [("E", "L"), ("V", "L")],
+    # delsq_div, delsq_vort, c_rho, euler_ru | euler_ru
+    "k_mix_edge4": [("C", "L"), ("V", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
+    # theta, c_rho, w, kdiff | delsq_theta, euler_theta, delsq_w, tend_w
+    "k_mix_cell": [("C", "L"), ("C", "L"), ("C", "W"), ("C", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("C", "W")],
+    # delsq_theta, delsq_w, euler_theta, tend_w, euler_rw | euler_theta, euler_rw
+    "k_mix_cell4": [("C", "L"), ("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("C", "L"), ("C", "W")],
 }
 
 

@@ -69,6 +69,7 @@ class HexdynVertical(ct.Structure):
         ("etp", f32p), ("etm", f32p), ("ewp", f32p), ("ewm", f32p),
         ("u_init", f32p), ("v_init", f32p),
         ("rho_base", f32p), ("rtheta_base", f32p), ("exner_base", f32p),
+        ("zgrid", f32p),
     ]
 
 
@@ -204,6 +205,7 @@ def build_structs(case) -> tuple:
     vert.rho_base = b.f32("rb", case.reference.rho_base)
     vert.rtheta_base = b.f32("rtb", case.reference.rtheta_base)
     vert.exner_base = b.f32("exb", case.reference.exner_base)
+    vert.zgrid = b.f32("zgrid", g.zgrid)
 
     cfg = HexdynConfig()
     c = case.config

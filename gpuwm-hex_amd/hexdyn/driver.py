@@ -182,6 +182,17 @@ class HexDycoreDriver:
         receipt = self.step_device(n_steps)
         return HostStepResult(self.get_state(), receipt)
 
+    def integrals(self) -> Dict[str, float]:
+        """FP64 global integrals of the published state (hexdyn_integrals)."""
+        nq = 3 + self.case.n_scalars
+        out = (ct.c_double * nq)()
+        _check(self._h, lib().hexdyn_integrals(self._h, out, nq, ct.c_void_p(self.stream_ptr())),
+               "hexdyn_integrals")
+        res = {"dry_mass": out[0], "rho_theta": out[1], "total_energy": out[2]}
+        for s in range(self.case.n_scalars):
+            res[f"tracer_mass_{s}"] = out[3 + s]
+        return res
+
     # phase entry points (parity tests walk a step phase by phase)
     def phase(self, name: str, *args: int) -> None:
         fn = getattr(lib(), f"hexdyn_{name}")

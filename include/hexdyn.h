@@ -109,6 +109,7 @@ typedef struct hexdyn_vertical {
   const float *ewp, *ewm;              /* [L+1] */
   const float *u_init, *v_init;        /* [L] reference winds */
   const float *rho_base, *rtheta_base, *exner_base; /* [L][C] */
+  const float *zgrid;                  /* [L+1][C] interface heights (optional: potential energy) */
 } hexdyn_vertical;
 
 typedef struct hexdyn_config {
@@ -214,6 +215,15 @@ int hexdyn_end_step(hexdyn_handle *h, void *stream);
 int hexdyn_profile_enable(hexdyn_handle *h, int enable);
 int hexdyn_profile_read(hexdyn_handle *h, const char *kernel, double *avg_ms, int64_t *launches);
 int hexdyn_profile_kernels(hexdyn_handle *h, char *buf, size_t len);
+
+/* FP64 global integrals of the published state with fixed-order
+ * compensated reduction (repeatable bits): out[0] dry mass
+ * sum areaCell*dzw*rho_zz, out[1] sum areaCell*dzw*rtheta_m, out[2] total
+ * energy sum areaCell*dzw*rho_zz*(c_v T + g z + K), out[3+s] tracer mass of
+ * scalar s.  n_out >= 3 + n_scalars.  Synchronises `stream`.
+ * (No reference entry point: the reference sums rho*(area*dzw) in float64
+ * with CuPy, cuda_fullstep_lts_active.py:1337-1360.) */
+int hexdyn_integrals(hexdyn_handle *h, double *out, int n_out, void *stream);
 
 /* The exner power function used by recovery, exposed for its accuracy test. */
 float hexdyn_powf(float x, float y);
