@@ -1183,32 +1183,49 @@ KB k_split_scale(const Instance I, int n, float *p, float inv) {
   ROW(p, e)[k] = ROW(p, e)[k] * inv;
 }
 
+// Scalar kernels run one wavefront per entity over all species: the
+// connectivity, stencil weights and the species-independent rows are read
+// once, and two species' neighbour rows are in flight together.
 template <int NE>
 KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__ ev) {
   constexpr int NX = 2 * NE - 2;
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
+  const int NT = I.n.NT;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
-  const int sp = blockIdx.y;
-  const float *q = qs + sp * cplane;
   int ac[NX];
-  float qv[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
-#pragma unroll
-  for (int s = 0; s < NX; ++s) qv[s] = ROW(q, ac[s])[k];
   const float sv = cs1(ROW(I.b.split_u, e)[k]);
   const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
-  float av[NX], a3v[NX];
+  float av[NX], a3v[NX], wt[NX];
   slots(av, I.m.adv + e * NX);
   slots(a3v, I.m.adv3 + e * NX);
-  float r = 0.0f;
 #pragma unroll
-  for (int s = 0; s < NX; ++s)
-    if (s < na) r = r + (av[s] + (coef * sv) * a3v[s]) * qv[s];
-  if (k < I.n.L) ROW(ev + sp * eplane, e)[k] = r;
+  for (int s = 0; s < NX; ++s) wt[s] = av[s] + (coef * sv) * a3v[s];
+  for (int sp = 0; sp < NT; sp += 2) {
+    const int sq = sp + 1 < NT ? sp + 1 : sp;
+    const float *qa = qs + sp * cplane, *qb = qs + sq * cplane;
+    float va[NX], vb[NX];
+#pragma unroll
+    for (int s = 0; s < NX; ++s) {
+      va[s] = ROW(qa, ac[s])[k];
+      vb[s] = ROW(qb, ac[s])[k];
+    }
+    float ra = 0.0f, rb = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NX; ++s)
+      if (s < na) {
+        ra = ra + wt[s] * va[s];
+        rb = rb + wt[s] * vb[s];
+      }
+    if (k < I.n.L) {
+      ROW(ev + sp * eplane, e)[k] = ra;
+      if (sq != sp) ROW(ev + sq * eplane, e)[k] = rb;
+    }
+  }
 }
 
 template <int NE>
@@ -1217,37 +1234,41 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L;
+  const int L = I.n.L, NT = I.n.NT;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
-  const int sp = blockIdx.y;
-  const float *q = ROW(qs + sp * cplane, c);
   const int n = I.m.neoc[c];
   float sg[NE];
   slots(sg, I.m.csign + c * NE);
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-  float uh[NE], evv[NE];
+  float uh[NE];
 #pragma unroll
-  for (int s = 0; s < NE; ++s) {
-    uh[s] = ROW(I.b.split_u, e[s])[k];
-    evv[s] = ROW(ev + sp * eplane, e[s])[k];
-  }
-  float vf = 0.0f;
-  if (k >= 1 && k < L) vf = vflux(q, ROW(I.b.split_w, c)[k], k, L, I.z.fzm[k], I.z.fzp[k], I.k.coef3);
-  const float vf_p = dn1(vf);
-  float tc = 0.0f;
+  for (int s = 0; s < NE; ++s) uh[s] = ROW(I.b.split_u, e[s])[k];
+  const float sw = ROW(I.b.split_w, c)[k];
+  const float orho = ROW(I.b.rho, c)[k];
+  const float target = (1.0f - wn) * orho + wn * ROW(I.b.c_rho, c)[k];
+  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k], rdzw = I.z.rdzw[k], ia = I.m.invarea[c];
+  auto species = [&](int sp) {
+    const float *q = ROW(qs + sp * cplane, c);
+    float evv[NE];
 #pragma unroll
-  for (int s = 0; s < NE; ++s)
-    if (s < n) tc = tc + ((-sg[s]) * uh[s]) * evv[s];
-  tc = tc * I.m.invarea[c] + 0.0f;
-  if (k < L) {
-    const float orho = ROW(I.b.rho, c)[k];
-    const float target = (1.0f - wn) * orho + wn * ROW(I.b.c_rho, c)[k];
+    for (int s = 0; s < NE; ++s) evv[s] = ROW(ev + sp * eplane, e[s])[k];
     const float qo = ROW(I.b.q + sp * cplane, c)[k];
-    const float num = qo * orho + dt * (tc - I.z.rdzw[k] * (vf_p - vf));
-    ROW(out + sp * cplane, c)[k] = num * (1.0f / target);
-  }
+    float vf = 0.0f;
+    if (k >= 1 && k < L) vf = vflux(q, sw, k, L, fzm, fzp, I.k.coef3);
+    const float vf_p = dn1(vf);
+    float tc = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) tc = tc + ((-sg[s]) * uh[s]) * evv[s];
+    tc = tc * ia + 0.0f;
+    if (k < L) {
+      const float num = qo * orho + dt * (tc - rdzw * (vf_p - vf));
+      ROW(out + sp * cplane, c)[k] = num * (1.0f / target);
+    }
+  };
+  for (int sp = 0; sp < NT; ++sp) species(sp);
 }
 
 template <int NE>
@@ -1284,42 +1305,65 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
+  const int NT = I.n.NT;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
-  const int sp = blockIdx.y;
-  const float *q = qs + sp * cplane;
-  const float *qold = I.b.q + sp * cplane;
   int ac[NX];
-  float qv[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
-#pragma unroll
-  for (int s = 0; s < NX; ++s) qv[s] = ROW(q, ac[s])[k];
   const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
-  const float q1 = ROW(qold, c1)[k] + 0.0f, q2 = ROW(qold, c2)[k] + 0.0f;
   const float uu = ROW(I.b.split_u, e)[k];
   const float sv = cs1(uu);
   const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
-  float av[NX], a3v[NX];
+  float av[NX], a3v[NX], wt[NX];
   slots(av, I.m.adv + e * NX);
   slots(a3v, I.m.adv3 + e * NX);
-  float r;
-  if (na == 10) {
-    float val = (I.m.adv[e * NX] + (coef * sv) * I.m.adv3[e * NX]) * qv[0];
 #pragma unroll
-    for (int s = 1; s < NX; ++s)
-      if (s < 10) val = val + (av[s] + (coef * sv) * a3v[s]) * qv[s];
-    r = uu * val;
-  } else {
-    r = 0.0f;
+  for (int s = 0; s < NX; ++s) wt[s] = av[s] + (coef * sv) * a3v[s];
+  const float dvdt = I.m.dv[e] * dt;
+  for (int sp = 0; sp < NT; sp += 2) {
+    const int sq = sp + 1 < NT ? sp + 1 : sp;
+    float va[NX], vb[NX];
 #pragma unroll
-    for (int s = 0; s < NX; ++s)
-      if (s < na) r = r + (uu * (av[s] + (coef * sv) * a3v[s])) * qv[s];
-  }
-  const float up = (I.m.dv[e] * dt) * (max0(uu) * q1 + min0(uu) * q2);
-  if (k < I.n.L) {
-    ROW(I.b.fct_upw + sp * eplane, e)[k] = up;
-    ROW(I.b.fct_hres + sp * eplane, e)[k] = dt * r - up;
+    for (int s = 0; s < NX; ++s) {
+      va[s] = ROW(qs + sp * cplane, ac[s])[k];
+      vb[s] = ROW(qs + sq * cplane, ac[s])[k];
+    }
+    const float *qoa = I.b.q + sp * cplane, *qob = I.b.q + sq * cplane;
+    const float qa1 = ROW(qoa, c1)[k] + 0.0f, qa2 = ROW(qoa, c2)[k] + 0.0f;
+    const float qb1 = ROW(qob, c1)[k] + 0.0f, qb2 = ROW(qob, c2)[k] + 0.0f;
+    float ra, rb;
+    if (na == 10) {
+      // ten-cell stencils associate from the first product (transport.py n_adv == 10 branch)
+      float xa = wt[0] * va[0], xb = wt[0] * vb[0];
+#pragma unroll
+      for (int s = 1; s < NX; ++s)
+        if (s < 10) {
+          xa = xa + wt[s] * va[s];
+          xb = xb + wt[s] * vb[s];
+        }
+      ra = uu * xa;
+      rb = uu * xb;
+    } else {
+      ra = 0.0f;
+      rb = 0.0f;
+#pragma unroll
+      for (int s = 0; s < NX; ++s)
+        if (s < na) {
+          ra = ra + (uu * wt[s]) * va[s];
+          rb = rb + (uu * wt[s]) * vb[s];
+        }
+    }
+    const float upa = dvdt * (max0(uu) * qa1 + min0(uu) * qa2);
+    const float upb = dvdt * (max0(uu) * qb1 + min0(uu) * qb2);
+    if (k < I.n.L) {
+      ROW(I.b.fct_upw + sp * eplane, e)[k] = upa;
+      ROW(I.b.fct_hres + sp * eplane, e)[k] = dt * ra - upa;
+      if (sq != sp) {
+        ROW(I.b.fct_upw + sq * eplane, e)[k] = upb;
+        ROW(I.b.fct_hres + sq * eplane, e)[k] = dt * rb - upb;
+      }
+    }
   }
 }
 
@@ -1330,9 +1374,6 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
   const int k = lane();
   const int L = I.n.L;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
-  const int sp = blockIdx.y;
-  const float *qsr = ROW(qs + sp * cplane, c);
-  const float *qold = I.b.q + sp * cplane;
   const int n = I.m.neoc[c];
   float sg[NE];
   slots(sg, I.m.csign + c * NE);
@@ -1342,75 +1383,79 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
     nb[s] = I.m.coc[c * NE + s];
     e[s] = I.m.eoc[c * NE + s];
   }
-  float qn[NE], upw[NE], hres[NE];
-#pragma unroll
-  for (int s = 0; s < NE; ++s) {
-    qn[s] = ROW(qold, nb[s])[k] + 0.0f;
-    upw[s] = ROW(I.b.fct_upw + sp * eplane, e[s])[k];
-    hres[s] = ROW(I.b.fct_hres + sp * eplane, e[s])[k];
-  }
-  const float qo = ROW(qold, c)[k] + 0.0f;
-  const float qo_m = up1(qo), qo_p = dn1(qo);
-  float a, b;
-  if (k == 0) {
-    a = qo; b = qo;
-    a = pmin(a, qo_p); b = pmax(b, qo_p);
-  } else if (k == L - 1) {
-    a = qo_m; b = qo_m;
-    a = pmin(a, qo); b = pmax(b, qo);
-  } else {
-    a = qo_m; b = qo_m;
-    a = pmin(a, qo); b = pmax(b, qo);
-    a = pmin(a, qo_p); b = pmax(b, qo_p);
-  }
-#pragma unroll
-  for (int s = 0; s < NE; ++s)
-    if (s < n) {
-      a = pmin(a, qn[s]);
-      b = pmax(b, qn[s]);
-    }
   const float orho = ROW(I.b.rho, c)[k];
-  float mass = qo * orho;
   const float wk = ROW(I.b.split_w, c)[k];
-  float vres = 0.0f;
-  float fcol = 0.0f;
-  if (k >= 1 && k < L) {
-    vres = dt * vflux(qsr, wk, k, L, I.z.fzm[k], I.z.fzp[k], I.k.coef3);
-    fcol = dt * (max0(wk) * qo_m + min0(wk) * qo);
-  } else {
-    vres = dt * 0.0f;
-  }
-  const float fcol_p = dn1(fcol);
-  const float rdzw = I.z.rdzw[k];
-  if (k <= L - 2) mass = mass - fcol_p * rdzw;
-  if (k >= 1 && k < L) {
-    mass = mass + fcol * rdzw;
-    vres = vres - fcol;
-  }
-  const float vres_p = dn1(vres);
-  float sin_ = (-rdzw) * (min0(vres_p) - max0(vres));
-  float sout = (-rdzw) * (max0(vres_p) - min0(vres));
+  const float rdzw = I.z.rdzw[k], fzm = I.z.fzm[k], fzp = I.z.fzp[k];
   const float ia = I.m.invarea[c];
-#pragma unroll
-  for (int s = 0; s < NE; ++s)
-    if (s < n) {
-      const float so = -sg[s];
-      mass = mass + (so * upw[s]) * ia;
-      const float sr = (-so) * hres[s];
-      sout = sout - max0(sr) * ia;
-      sin_ = sin_ - min0(sr) * ia;
-    }
   const float target = ROW(I.b.target, c)[k];
-  float f = (b * target - mass) / (sin_ + 1.0e-20f);
-  sin_ = pmin(1.0f, max0(f));
-  f = (a * target - mass) / (sout - 1.0e-20f);
-  sout = pmin(1.0f, max0(f));
-  if (k < L) {
-    ROW(I.b.fct_mass + sp * cplane, c)[k] = mass;
-    ROW(I.b.fct_sin + sp * cplane, c)[k] = sin_;
-    ROW(I.b.fct_sout + sp * cplane, c)[k] = sout;
+  for (int sp = 0; sp < I.n.NT; ++sp) {
+    const float *qsr = ROW(qs + sp * cplane, c);
+    const float *qold = I.b.q + sp * cplane;
+    float qn[NE], upw[NE], hres[NE];
+#pragma unroll
+    for (int s = 0; s < NE; ++s) {
+      qn[s] = ROW(qold, nb[s])[k] + 0.0f;
+      upw[s] = ROW(I.b.fct_upw + sp * eplane, e[s])[k];
+      hres[s] = ROW(I.b.fct_hres + sp * eplane, e[s])[k];
+    }
+    const float qo = ROW(qold, c)[k] + 0.0f;
+    const float qo_m = up1(qo), qo_p = dn1(qo);
+    float a, b;
+    if (k == 0) {
+      a = qo; b = qo;
+      a = pmin(a, qo_p); b = pmax(b, qo_p);
+    } else if (k == L - 1) {
+      a = qo_m; b = qo_m;
+      a = pmin(a, qo); b = pmax(b, qo);
+    } else {
+      a = qo_m; b = qo_m;
+      a = pmin(a, qo); b = pmax(b, qo);
+      a = pmin(a, qo_p); b = pmax(b, qo_p);
+    }
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) {
+        a = pmin(a, qn[s]);
+        b = pmax(b, qn[s]);
+      }
+    float mass = qo * orho;
+    float vres = 0.0f;
+    float fcol = 0.0f;
+    if (k >= 1 && k < L) {
+      vres = dt * vflux(qsr, wk, k, L, fzm, fzp, I.k.coef3);
+      fcol = dt * (max0(wk) * qo_m + min0(wk) * qo);
+    } else {
+      vres = dt * 0.0f;
+    }
+    const float fcol_p = dn1(fcol);
+    if (k <= L - 2) mass = mass - fcol_p * rdzw;
+    if (k >= 1 && k < L) {
+      mass = mass + fcol * rdzw;
+      vres = vres - fcol;
+    }
+    const float vres_p = dn1(vres);
+    float sin_ = (-rdzw) * (min0(vres_p) - max0(vres));
+    float sout = (-rdzw) * (max0(vres_p) - min0(vres));
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) {
+        const float so = -sg[s];
+        mass = mass + (so * upw[s]) * ia;
+        const float sr = (-so) * hres[s];
+        sout = sout - max0(sr) * ia;
+        sin_ = sin_ - min0(sr) * ia;
+      }
+    float f = (b * target - mass) / (sin_ + 1.0e-20f);
+    sin_ = pmin(1.0f, max0(f));
+    f = (a * target - mass) / (sout - 1.0e-20f);
+    sout = pmin(1.0f, max0(f));
+    if (k < L) {
+      ROW(I.b.fct_mass + sp * cplane, c)[k] = mass;
+      ROW(I.b.fct_sin + sp * cplane, c)[k] = sin_;
+      ROW(I.b.fct_sout + sp * cplane, c)[k] = sout;
+    }
+    if (k <= L) ROW(I.b.fct_vres + sp * cplane, c)[k] = vres;
   }
-  if (k <= L) ROW(I.b.fct_vres + sp * cplane, c)[k] = vres;
 }
 
 KB k_fct_edge2(const Instance I) {
@@ -1419,12 +1464,13 @@ KB k_fct_edge2(const Instance I) {
   const int k = lane();
   if (k >= I.n.L) return;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
-  const int sp = blockIdx.y;
   const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
-  const float *si = I.b.fct_sin + sp * cplane, *so = I.b.fct_sout + sp * cplane;
-  float *h = ROW(I.b.fct_hres + sp * eplane, e);
-  const float f = h[k];
-  h[k] = max0(f) * pmin(ROW(so, c1)[k], ROW(si, c2)[k]) + min0(f) * pmin(ROW(si, c1)[k], ROW(so, c2)[k]);
+  for (int sp = 0; sp < I.n.NT; ++sp) {
+    const float *si = I.b.fct_sin + sp * cplane, *so = I.b.fct_sout + sp * cplane;
+    float *h = ROW(I.b.fct_hres + sp * eplane, e);
+    const float f = h[k];
+    h[k] = max0(f) * pmin(ROW(so, c1)[k], ROW(si, c2)[k]) + min0(f) * pmin(ROW(si, c1)[k], ROW(so, c2)[k]);
+  }
 }
 
 template <int NE>
@@ -1434,31 +1480,33 @@ KB k_fct_cell2(const Instance I, float *__restrict__ out) {
   const int k = lane();
   const int L = I.n.L;
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
-  const int sp = blockIdx.y;
   const int n = I.m.neoc[c];
   float sg[NE];
   slots(sg, I.m.csign + c * NE);
   int e[NE];
-  float hres[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  const float ia = I.m.invarea[c], rdzw = I.z.rdzw[k];
+  const float target = ROW(I.b.target, c)[k];
+  for (int sp = 0; sp < I.n.NT; ++sp) {
+    float hres[NE];
 #pragma unroll
-  for (int s = 0; s < NE; ++s) hres[s] = ROW(I.b.fct_hres + sp * eplane, e[s])[k];
-  const float si = ROW(I.b.fct_sin + sp * cplane, c)[k], so = ROW(I.b.fct_sout + sp * cplane, c)[k];
-  const float si_m = up1(si), so_m = up1(so);
-  float v = ROW(I.b.fct_vres + sp * cplane, c)[k];
-  if (k >= 1 && k < L) v = max0(v) * pmin(so_m, si) + min0(v) * pmin(so, si_m);
-  const float v_p = dn1(v);
-  float mass = ROW(I.b.fct_mass + sp * cplane, c)[k];
-  const float ia = I.m.invarea[c];
+    for (int s = 0; s < NE; ++s) hres[s] = ROW(I.b.fct_hres + sp * eplane, e[s])[k];
+    const float si = ROW(I.b.fct_sin + sp * cplane, c)[k], so = ROW(I.b.fct_sout + sp * cplane, c)[k];
+    const float si_m = up1(si), so_m = up1(so);
+    float v = ROW(I.b.fct_vres + sp * cplane, c)[k];
+    if (k >= 1 && k < L) v = max0(v) * pmin(so_m, si) + min0(v) * pmin(so, si_m);
+    const float v_p = dn1(v);
+    float mass = ROW(I.b.fct_mass + sp * cplane, c)[k];
 #pragma unroll
-  for (int s = 0; s < NE; ++s)
-    if (s < n) {
-      const float sgn = -sg[s];
-      mass = mass + (sgn * hres[s]) * ia;
-    }
-  mass = mass - I.z.rdzw[k] * (v_p - v);
-  if (k < L) ROW(out + sp * cplane, c)[k] = max0(mass / ROW(I.b.target, c)[k]);
+    for (int s = 0; s < NE; ++s)
+      if (s < n) {
+        const float sgn = -sg[s];
+        mass = mass + (sgn * hres[s]) * ia;
+      }
+    mass = mass - rdzw * (v_p - v);
+    if (k < L) ROW(out + sp * cplane, c)[k] = max0(mass / target);
+  }
 }
 
 KB k_clamp(const Instance I, float *q, int rows) {
@@ -1574,23 +1622,22 @@ int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
 }
 
 int scalars_stage(const Instance &I, int stage, hipStream_t s) {
-  const int NT = I.n.NT;
   const float dt = I.k.sca_dt[stage - 1];
   if (stage == 1) {
     launch(k_split_scale, I.n.E, 1, s, I, I.n.E, I.b.split_u, I.k.split_inv);
     launch(k_split_scale, I.n.C, 1, s, I, I.n.C, I.b.split_w, I.k.split_inv);
   }
   if (stage < 3 || !I.k.monotonic) {
-    LAUNCH_NE(I, k_sc_edge, I.n.E, NT, s, I, (const float *)I.b.q_stage, I.b.ev);
-    LAUNCH_NE(I, k_sc_cell, I.n.C, NT, s, I, (const float *)I.b.q_stage, (const float *)I.b.ev, I.b.q_next, dt,
+    LAUNCH_NE(I, k_sc_edge, I.n.E, 1, s, I, (const float *)I.b.q_stage, I.b.ev);
+    LAUNCH_NE(I, k_sc_cell, I.n.C, 1, s, I, (const float *)I.b.q_stage, (const float *)I.b.ev, I.b.q_next, dt,
               I.k.wn[stage - 1]);
     return check();
   }
   LAUNCH_NE(I, k_fct_target, I.n.C, 1, s, I, dt);
-  LAUNCH_NE(I, k_fct_edge1, I.n.E, NT, s, I, (const float *)I.b.q_stage, dt);
-  LAUNCH_NE(I, k_fct_cell1, I.n.C, NT, s, I, (const float *)I.b.q_stage, dt);
-  launch(k_fct_edge2, I.n.E, NT, s, I);
-  LAUNCH_NE(I, k_fct_cell2, I.n.C, NT, s, I, I.b.q_next);
+  LAUNCH_NE(I, k_fct_edge1, I.n.E, 1, s, I, (const float *)I.b.q_stage, dt);
+  LAUNCH_NE(I, k_fct_cell1, I.n.C, 1, s, I, (const float *)I.b.q_stage, dt);
+  launch(k_fct_edge2, I.n.E, 1, s, I);
+  LAUNCH_NE(I, k_fct_cell2, I.n.C, 1, s, I, I.b.q_next);
   return check();
 }
 
