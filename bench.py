@@ -143,12 +143,14 @@ def main():
     drv.profile(False)
     kernels.sort(key=lambda r: -r["total_ms_per_step"])
     dom = kernels[0]
+    # rocprofv3 FETCH_SIZE/WRITE_SIZE bytes per launch of the same command (hexdyn.profile_summary)
     traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{dom['kernel']}_{args.workload}.json"
+    tag = args.workload + ("" if args.mixing != "off" else "-nomix") + ("" if args.numerics == "strict" else "-fast")
+    tfile = ROOT / "profiles" / f"traffic_{tag}.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
+            traffic = json.loads(tfile.read_text())["hbm_bytes_per_launch"].get(dom["kernel"])
+        except (ValueError, KeyError):
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(dom["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom["kernel"],
