@@ -844,7 +844,10 @@ int hexdyn_recover(hexdyn_handle *h, int stage, void *stream) {
   if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
   // At stage 3 the recovered fluxes are also accumulated into the split-transport
   // sums (copy on the first subcycle, add afterwards: integration.py:27-56).
-  return NS_CALL(h, recover, h->I, stage, h->cur_sub, 0, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  h->I.k.store_flux = 1;  // the phase API keeps the stage fluxes exportable
+  const int rc = NS_CALL(h, recover, h->I, stage, h->cur_sub, 0, (hipStream_t)stream);
+  h->I.k.store_flux = 0;
+  return rc ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_solve_diagnostics(hexdyn_handle *h, int rk_step, void *stream) {

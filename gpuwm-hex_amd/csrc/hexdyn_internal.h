@@ -121,6 +121,8 @@ struct Consts {
   int order, splits, monotonic, divdamp, split_transport, scalar_adv;
   int apvm_on;
   int mixing, mix4;              // Smagorinsky mixing on; fourth-order part on (h4 > 0)
+  int store_flux;                // recovery also stores the stage's flux_u/flux_w (read only by
+                                 // hexdyn_export_field; stage 3 folds them into the split sums)
   float strain_scale, ceiling;   // (c_s*len)^2, (0.01*len^2)/dt
 };
 

@@ -1098,7 +1098,7 @@ KB k_rec_cell(const Instance I, int stage, int sub, float inv_steps) {
   if (k <= L) {
     ROW(I.b.c_rw, c)[k] = rwv;
     const float fw = ROW(I.b.s_rw, c)[k] + ROW(I.b.ww_avg, c)[k] * inv_steps;
-    ROW(I.b.flux_w, c)[k] = fw;
+    if (I.k.store_flux) ROW(I.b.flux_w, c)[k] = fw;
     if (stage == 3) ROW(I.b.split_w, c)[k] = sub == 1 ? fw : fw + ROW(I.b.split_w, c)[k];
   }
   const float *cru = I.b.c_ru;
@@ -1125,7 +1125,7 @@ KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps, int damp, f
   const float ru = sru + rp;
   ROW(I.b.c_ru, e)[k] = ru;
   const float fu = sru + ROW(I.b.ru_avg, e)[k] * inv_steps;
-  ROW(I.b.flux_u, e)[k] = fu;
+  if (I.k.store_flux) ROW(I.b.flux_u, e)[k] = fu;
   if (stage == 3) ROW(I.b.split_u, e)[k] = sub == 1 ? fu : fu + ROW(I.b.split_u, e)[k];
   const float r0 = (ROW(I.b.s_rho_p, c0)[k] + ROW(I.b.rho_pp, c0)[k]) + ROW(I.z.rb, c0)[k];
   const float r1 = (ROW(I.b.s_rho_p, c1)[k] + ROW(I.b.rho_pp, c1)[k]) + ROW(I.z.rb, c1)[k];
