@@ -152,6 +152,19 @@ struct hexdyn_handle {
   int fast = 0;
   int in_step = 0;
   int cur_sub = 1;
+  // Buffer roles within a step (no copies): the first subcycle starts from
+  // the published buffers, later subcycles from the previous subcycle's
+  // current buffers (s_* and c_* swap roles); until a subcycle's first
+  // recovery the current fields are read where the start fields live, and
+  // until the step's first recovery u and w are read from the published
+  // buffers.  exner and p' change only in a stage-3 recovery: until then
+  // exner reads as the subcycle-start exner and p' as the published p'.
+  // The published state is read-only during a step.
+  int start_is_pub = 0;
+  int fresh_sub = 0;
+  int fresh_step = 0;
+  int exner_start = 0;
+  int pp_pub = 0;
   double time_s = 0.0;
   std::string error;
   std::vector<void *> allocs;
@@ -183,6 +196,50 @@ struct hexdyn_handle {
 namespace {
 
 int run_phase_begin_step(hexdyn_handle *h, hipStream_t s);
+
+// The device pointers the kernels of the current phase see (buffer roles above).
+Instance view(const hexdyn_handle *h, bool recovery) {
+  Instance J = h->I;
+  Work &B = J.b;
+  if (h->start_is_pub) {
+    B.s_rho = B.rho; B.s_rt = B.rt; B.s_ru = B.ru; B.s_rw = B.rw;
+    B.s_theta = B.p_theta; B.s_exner = B.p_exner; B.s_rho_p = B.p_rho_p; B.s_rtheta_p = B.p_rtheta_p;
+  }
+  if (!recovery && h->fresh_sub) {
+    B.c_rho = B.s_rho; B.c_rt = B.s_rt; B.c_ru = B.s_ru; B.c_rw = B.s_rw;
+    B.theta = B.s_theta; B.rho_p = B.s_rho_p; B.rtheta_p = B.s_rtheta_p;
+  }
+  if (!recovery && h->exner_start) B.exner = B.s_exner;
+  if (!recovery && h->pp_pub) B.pp = B.p_pp;
+  if (!recovery && h->fresh_step) {
+    B.u = B.p_u;
+    B.w = B.p_w;
+  }
+  return J;
+}
+
+// Subcycle start: a later subcycle starts from the current buffers, which
+// become the start buffers; the spare start buffers take the next currents.
+void rotate_subcycle(hexdyn_handle *h, int sub) {
+  Work &B = h->I.b;
+  if (sub > 1) {
+    std::swap(B.s_rho, B.c_rho); std::swap(B.s_rt, B.c_rt); std::swap(B.s_ru, B.c_ru); std::swap(B.s_rw, B.c_rw);
+    std::swap(B.s_theta, B.theta); std::swap(B.s_exner, B.exner); std::swap(B.s_rho_p, B.rho_p);
+    std::swap(B.s_rtheta_p, B.rtheta_p);
+    h->start_is_pub = 0;
+  } else {
+    h->start_is_pub = 1;
+  }
+  h->fresh_sub = 1;
+  h->exner_start = 1;
+}
+
+// after a recovery of `stage` the fields it wrote are the current ones
+void recovered(hexdyn_handle *h, int stage) {
+  h->fresh_sub = 0;
+  h->fresh_step = 0;
+  if (stage == 3) h->exner_start = h->pp_pub = 0;
+}
 
 #define NS_CALL(h, fn, ...) ((h)->fast ? hexdyn::fast::fn(__VA_ARGS__) : hexdyn::strict::fn(__VA_ARGS__))
 
@@ -613,7 +670,8 @@ struct FieldRef {
 };
 
 FieldRef field_ref(hexdyn_handle *h, int id) {
-  Work &B = h->I.b;
+  Instance J = view(h, false);  // mid-step fields as the next phase would read them
+  Work &B = J.b;
   const Sizes &n = h->I.n;
   const int C = n.C, E = n.E, V = n.V, L = n.L;
   const bool in = h->in_step;
@@ -688,23 +746,14 @@ int copy_rows(float *dst, const float *src, size_t rows, hipStream_t s) {
 
 int run_phase_begin_step(hexdyn_handle *h, hipStream_t s) {
   Work &B = h->I.b;
-  const Sizes &n = h->I.n;
-  int rc = 0;
-  rc |= copy_rows(B.c_rho, B.rho, n.C, s);
-  rc |= copy_rows(B.c_rt, B.rt, n.C, s);
-  rc |= copy_rows(B.c_ru, B.ru, n.E, s);
-  rc |= copy_rows(B.c_rw, B.rw, n.C, s);
-  rc |= copy_rows(B.theta, B.p_theta, n.C, s);
-  rc |= copy_rows(B.exner, B.p_exner, n.C, s);
-  rc |= copy_rows(B.rho_p, B.p_rho_p, n.C, s);
-  rc |= copy_rows(B.rtheta_p, B.p_rtheta_p, n.C, s);
-  rc |= copy_rows(B.pp, B.p_pp, n.C, s);
-  rc |= copy_rows(B.u, B.p_u, n.E, s);
-  rc |= copy_rows(B.w, B.p_w, n.C, s);
-  if (hipMemsetAsync(B.flag, 0, sizeof(int), s) != hipSuccess) rc |= HEXDYN_E_HIP;
-  if (rc) return HEXDYN_E_HIP;
+  if (hipMemsetAsync(B.flag, 0, sizeof(int), s) != hipSuccess) return HEXDYN_E_HIP;
   h->in_step = 1;
-  return NS_CALL(h, begin_step, h->I, s);
+  h->start_is_pub = 1;
+  h->fresh_sub = 1;
+  h->fresh_step = 1;
+  h->exner_start = 1;
+  h->pp_pub = 1;
+  return NS_CALL(h, begin_step, view(h, false), s);
 }
 
 int run_end_step(hexdyn_handle *h, hipStream_t s) {
@@ -718,6 +767,7 @@ int run_end_step(hexdyn_handle *h, hipStream_t s) {
   if (hipMemcpyAsync(&flag, B.flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return HEXDYN_E_HIP;
   if (hipStreamSynchronize(s) != hipSuccess) return HEXDYN_E_HIP;
   h->in_step = 0;
+  h->start_is_pub = h->fresh_sub = h->fresh_step = h->exner_start = h->pp_pub = 0;
   if (flag) {
     h->error = "validation flag set: non-finite or non-positive state; step not published";
     return HEXDYN_E_NONFINITE;
@@ -738,6 +788,7 @@ int hexdyn_set_state(hexdyn_handle *h, const hexdyn_state_view *st, int rebuild,
   if (!h || !st) return HEXDYN_E_INVALID;
   hipStream_t s = (hipStream_t)stream;
   h->in_step = 0;
+  h->start_is_pub = h->fresh_sub = h->fresh_step = h->exner_start = h->pp_pub = 0;
   const struct { int id; float *p; } prog[] = {
       {HEXDYN_F_RHO_ZZ, st->rho_zz}, {HEXDYN_F_RTHETA_M, st->rtheta_m}, {HEXDYN_F_RU, st->ru},
       {HEXDYN_F_RW, st->rw}, {HEXDYN_F_SCALARS, st->scalars}};
@@ -821,38 +872,42 @@ int hexdyn_begin_step(hexdyn_handle *h, void *stream) {
 
 int hexdyn_begin_subcycle(hexdyn_handle *h, int subcycle, void *stream) {
   if (!h || subcycle < 1 || subcycle > h->cfg.config_dynamics_split_steps) return HEXDYN_E_INVALID;
+  if (!h->in_step) return set_error(h, HEXDYN_E_INVALID, "hexdyn_begin_subcycle outside a step");
   h->cur_sub = subcycle;
-  return NS_CALL(h, begin_subcycle, h->I, subcycle, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  rotate_subcycle(h, subcycle);
+  return NS_CALL(h, begin_subcycle, view(h, false), subcycle, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_dyn_tend(hexdyn_handle *h, int stage, void *stream) {
   if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
-  return NS_CALL(h, dyn_tend, h->I, stage, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  return NS_CALL(h, dyn_tend, view(h, false), stage, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_acoustic_substep(hexdyn_handle *h, int stage, int small_step, void *stream) {
   if (!h || stage < 1 || stage > 3 || small_step < 1 || small_step > h->I.k.steps[stage - 1]) return HEXDYN_E_INVALID;
-  return NS_CALL(h, acoustic_substep, h->I, stage, small_step, 0, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  return NS_CALL(h, acoustic_substep, view(h, false), stage, small_step, 0, (hipStream_t)stream) ? HEXDYN_E_HIP
+                                                                                                   : 0;
 }
 
 int hexdyn_divergence_damping(hexdyn_handle *h, int stage, void *stream) {
   if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
-  return NS_CALL(h, divergence_damping, h->I, stage, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  return NS_CALL(h, divergence_damping, view(h, false), stage, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_recover(hexdyn_handle *h, int stage, void *stream) {
   if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
   // At stage 3 the recovered fluxes are also accumulated into the split-transport
   // sums (copy on the first subcycle, add afterwards: integration.py:27-56).
-  h->I.k.store_flux = 1;  // the phase API keeps the stage fluxes exportable
-  const int rc = NS_CALL(h, recover, h->I, stage, h->cur_sub, 0, (hipStream_t)stream);
-  h->I.k.store_flux = 0;
+  Instance J = view(h, true);
+  J.k.store_flux = 1;  // the phase API keeps the stage fluxes exportable
+  const int rc = NS_CALL(h, recover, J, stage, h->cur_sub, 0, (hipStream_t)stream);
+  recovered(h, stage);
   return rc ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_solve_diagnostics(hexdyn_handle *h, int rk_step, void *stream) {
   if (!h || rk_step < 1 || rk_step > 3) return HEXDYN_E_INVALID;
-  return NS_CALL(h, solve_diagnostics, h->I, rk_step, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  return NS_CALL(h, solve_diagnostics, view(h, false), rk_step, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_end_subcycle(hexdyn_handle *h, int subcycle, void *stream) {
@@ -865,8 +920,8 @@ int hexdyn_scalars(hexdyn_handle *h, int stage, void *stream) {
   if (!h || stage < 1 || stage > 3) return HEXDYN_E_INVALID;
   hipStream_t s = (hipStream_t)stream;
   Work &B = h->I.b;
-  if (stage == 1 && copy_rows(B.q_stage, B.q, (size_t)h->I.n.NT * h->I.n.C, s)) return HEXDYN_E_HIP;
-  if (NS_CALL(h, scalars_stage, h->I, stage, s)) return HEXDYN_E_HIP;
+  // stage 1 reads the published scalars in place; stages 2-3 read the previous stage's result
+  if (NS_CALL(h, scalars_stage, view(h, false), stage, s)) return HEXDYN_E_HIP;
   std::swap(B.q_stage, B.q_next);
   return 0;
 }
@@ -884,24 +939,28 @@ int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream) {
     int rc = run_phase_begin_step(h, s);
     for (int sub = 1; rc == 0 && sub <= c.config_dynamics_split_steps; ++sub) {
       h->cur_sub = sub;
-      rc |= NS_CALL(h, begin_subcycle, h->I, sub, s);
+      rotate_subcycle(h, sub);
+      rc |= NS_CALL(h, begin_subcycle, view(h, false), sub, s);
       for (int st = 1; rc == 0 && st <= 3; ++st) {
-        rc |= NS_CALL(h, dyn_tend, h->I, st, s);
+        rc |= NS_CALL(h, dyn_tend, view(h, false), st, s);
         // divergence damping of substep ss is fused into the next edge pass:
         // the acoustic edge update of ss+1, or the edge recovery after the last substep
         const int dd = c.config_divergence_damping ? 1 : 0;
         for (int ss = 1; rc == 0 && ss <= h->I.k.steps[st - 1]; ++ss)
-          rc |= NS_CALL(h, acoustic_substep, h->I, st, ss, (dd && ss > 1) ? 1 : 0, s);
-        rc |= NS_CALL(h, recover, h->I, st, sub, dd, s);
+          rc |= NS_CALL(h, acoustic_substep, view(h, false), st, ss, (dd && ss > 1) ? 1 : 0, s);
+        rc |= NS_CALL(h, recover, view(h, true), st, sub, dd, s);
+        recovered(h, st);
         // the last stage's diagnostics would only feed a subcycle that does not
         // follow: the next step's begin_step recomputes them from the published state
-        if (!(sub == c.config_dynamics_split_steps && st == 3)) rc |= NS_CALL(h, solve_diagnostics, h->I, st, s);
+        if (!(sub == c.config_dynamics_split_steps && st == 3))
+          rc |= NS_CALL(h, solve_diagnostics, view(h, false), st, s);
       }
     }
     if (rc == 0 && c.config_scalar_advection && c.config_split_dynamics_transport && h->I.n.NT > 0)
       for (int st = 1; rc == 0 && st <= 3; ++st) rc |= hexdyn_scalars(h, st, stream);
     if (rc) {
       h->in_step = 0;
+      h->start_is_pub = h->fresh_sub = h->fresh_step = h->exner_start = h->pp_pub = 0;
       if (h->error.empty()) h->error = "kernel launch failed";
       return HEXDYN_E_HIP;
     }

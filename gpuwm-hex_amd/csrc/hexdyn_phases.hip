@@ -70,11 +70,23 @@ __device__ __forceinline__ float pmin(float a, float b) { return b < a ? b : a; 
 __device__ __forceinline__ float pmax(float a, float b) { return b > a ? b : a; }
 __device__ __forceinline__ int cl(int k) { return k < 0 ? 0 : (k > 63 ? 63 : k); }
 
+// x / 12 with IEEE round-to-nearest.  For quotients in [2^-125, FLT_MAX] the
+// product by RN(1/12) plus one FMA residual correction is the correctly
+// rounded quotient (verified exhaustively over every binary32 x); zeros,
+// tiny, infinite and NaN inputs take the division itself.
+__device__ __forceinline__ float div12(float x) {
+  constexpr float r12 = 0x1.555556p-4f;  // RN(1/12)
+  const float q0 = x * r12;
+  const float a = fabsf(q0);
+  if (a >= 0x1p-125f && a <= 0x1.fffffep127f) return __builtin_fmaf(__builtin_fmaf(-q0, 12.0f, x), r12, q0);
+  return x / 12.0f;
+}
+
 __device__ __forceinline__ float flux4(float qm2, float qm1, float q0, float qp1, float vel) {
-  return (vel * (7.0f * (q0 + qm1) - (qp1 + qm2))) / 12.0f;
+  return div12(vel * (7.0f * (q0 + qm1) - (qp1 + qm2)));
 }
 __device__ __forceinline__ float flux3(float qm2, float qm1, float q0, float qp1, float vel, float coef) {
-  return flux4(qm2, qm1, q0, qp1, vel) + ((coef * fabsf(vel)) * ((qp1 - qm2) - 3.0f * (q0 - qm1))) / 12.0f;
+  return flux4(qm2, qm1, q0, qp1, vel) + div12((coef * fabsf(vel)) * ((qp1 - qm2) - 3.0f * (q0 - qm1)));
 }
 
 __device__ __forceinline__ void set_flag(int *flag) { atomicOr(flag, 1); }
@@ -586,14 +598,7 @@ KB k_sub_cell(const Instance I) {
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
-  const float rho_p = ROW(I.b.rho_p, c)[k];
-  ROW(I.b.s_rho, c)[k] = ROW(I.b.c_rho, c)[k];
-  ROW(I.b.s_rt, c)[k] = ROW(I.b.c_rt, c)[k];
-  ROW(I.b.s_rw, c)[k] = ROW(I.b.c_rw, c)[k];
-  ROW(I.b.s_theta, c)[k] = ROW(I.b.theta, c)[k];
-  ROW(I.b.s_rho_p, c)[k] = rho_p;
-  ROW(I.b.s_rtheta_p, c)[k] = ROW(I.b.rtheta_p, c)[k];
-  ROW(I.b.s_exner, c)[k] = ROW(I.b.exner, c)[k];
+  const float rho_p = ROW(I.b.s_rho_p, c)[k];
   const float g = I.k.gravity;
   float dpdz;
   if (I.n.NM <= 0) {
@@ -625,7 +630,6 @@ KB k_sub_edge(const Instance I) {
   if (e >= I.n.E) return;
   const int k = lane();
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  ROW(I.b.s_ru, e)[k] = ROW(I.b.c_ru, e)[k];
   const float *pp = I.b.pp, *zz = I.z.zz, *dz = I.b.dpdz;
   const float a = ((ROW(pp, c1)[k] - ROW(pp, c0)[k]) * I.m.invdc[e]) / (0.5f * (ROW(zz, c1)[k] + ROW(zz, c0)[k]));
   const float bb = (0.5f * ROW(I.z.zxu, e)[k]) * (ROW(dz, c0)[k] + ROW(dz, c1)[k]);
@@ -1171,8 +1175,8 @@ KB k_rebuild_edge(const Instance I) {
 __device__ __forceinline__ float vflux(const float *qc, float wk, int k, int L, float fzm, float fzp, float coef) {
   if (k == 1 || k == L - 1) return wk * (fzm * qc[k] + fzp * qc[k - 1]);
   const float q0 = qc[k - 2], q1 = qc[k - 1], q2 = qc[k], q3 = qc[k + 1];
-  const float base = (wk * (7.0f * (q2 + q1) - (q3 + q0))) / 12.0f;
-  const float corr = ((coef * fabsf(wk)) * ((q3 - q0) - 3.0f * (q2 - q1))) / 12.0f;
+  const float base = div12(wk * (7.0f * (q2 + q1) - (q3 + q0)));
+  const float corr = div12((coef * fabsf(wk)) * ((q3 - q0) - 3.0f * (q2 - q1)));
   return base + corr;
 }
 
@@ -1623,19 +1627,20 @@ int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
 
 int scalars_stage(const Instance &I, int stage, hipStream_t s) {
   const float dt = I.k.sca_dt[stage - 1];
+  const float *qin = stage == 1 ? I.b.q : I.b.q_stage;  // stage 1 starts from the published scalars
   if (stage == 1) {
     launch(k_split_scale, I.n.E, 1, s, I, I.n.E, I.b.split_u, I.k.split_inv);
     launch(k_split_scale, I.n.C, 1, s, I, I.n.C, I.b.split_w, I.k.split_inv);
   }
   if (stage < 3 || !I.k.monotonic) {
-    LAUNCH_NE(I, k_sc_edge, I.n.E, 1, s, I, (const float *)I.b.q_stage, I.b.ev);
-    LAUNCH_NE(I, k_sc_cell, I.n.C, 1, s, I, (const float *)I.b.q_stage, (const float *)I.b.ev, I.b.q_next, dt,
+    LAUNCH_NE(I, k_sc_edge, I.n.E, 1, s, I, qin, I.b.ev);
+    LAUNCH_NE(I, k_sc_cell, I.n.C, 1, s, I, qin, (const float *)I.b.ev, I.b.q_next, dt,
               I.k.wn[stage - 1]);
     return check();
   }
   LAUNCH_NE(I, k_fct_target, I.n.C, 1, s, I, dt);
-  LAUNCH_NE(I, k_fct_edge1, I.n.E, 1, s, I, (const float *)I.b.q_stage, dt);
-  LAUNCH_NE(I, k_fct_cell1, I.n.C, 1, s, I, (const float *)I.b.q_stage, dt);
+  LAUNCH_NE(I, k_fct_edge1, I.n.E, 1, s, I, qin, dt);
+  LAUNCH_NE(I, k_fct_cell1, I.n.C, 1, s, I, qin, dt);
   launch(k_fct_edge2, I.n.E, 1, s, I);
   LAUNCH_NE(I, k_fct_cell2, I.n.C, 1, s, I, I.b.q_next);
   return check();
