@@ -170,6 +170,19 @@ struct hexdyn_handle {
   std::vector<void *> allocs;
   int *old_c = nullptr, *old_e = nullptr, *old_v = nullptr;  // device: old index of each new entity
   double *int_partial = nullptr, *int_out = nullptr;          // integral partials and results
+  // hipGraph-captured large steps, one executable per buffer-role layout at step start
+  int use_graphs = 1;
+  hipStream_t capture_stream = nullptr;
+  struct Roles {  // host-side buffer roles and step flags
+    Instance I;
+    int in_step, cur_sub, start_is_pub, fresh_sub, fresh_step, exner_start, pp_pub;
+  };
+  std::map<std::string, std::pair<hipGraphExec_t, Roles>> graphs;
+  Roles roles() const { return {I, in_step, cur_sub, start_is_pub, fresh_sub, fresh_step, exner_start, pp_pub}; }
+  void set_roles(const Roles &r) {
+    I = r.I; in_step = r.in_step; cur_sub = r.cur_sub; start_is_pub = r.start_is_pub; fresh_sub = r.fresh_sub;
+    fresh_step = r.fresh_step; exner_start = r.exner_start; pp_pub = r.pp_pub;
+  }
   std::vector<int> old_c_h, old_e_h, old_v_h;
 
   template <class T> int alloc(T **p, size_t count) {
@@ -266,6 +279,8 @@ double hexdyn_model_time(const hexdyn_handle *h) { return h ? h->time_s : 0.0; }
 
 void hexdyn_destroy(hexdyn_handle *h) {
   if (!h) return;
+  for (auto &g : h->graphs) (void)hipGraphExecDestroy(g.second.first);
+  if (h->capture_stream) (void)hipStreamDestroy(h->capture_stream);
   for (void *p : h->allocs) (void)hipFree(p);
   delete h;
 }
@@ -744,9 +759,14 @@ int copy_rows(float *dst, const float *src, size_t rows, hipStream_t s) {
              : HEXDYN_E_HIP;
 }
 
+__global__ void k_clear_flag(int *flag) {
+  if (threadIdx.x == 0) flag[0] = 0;
+}
+
 int run_phase_begin_step(hexdyn_handle *h, hipStream_t s) {
   Work &B = h->I.b;
-  if (hipMemsetAsync(B.flag, 0, sizeof(int), s) != hipSuccess) return HEXDYN_E_HIP;
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s, B.flag);
+  if (hipGetLastError() != hipSuccess) return HEXDYN_E_HIP;
   h->in_step = 1;
   h->start_is_pub = 1;
   h->fresh_sub = 1;
@@ -756,13 +776,20 @@ int run_phase_begin_step(hexdyn_handle *h, hipStream_t s) {
   return NS_CALL(h, begin_step, view(h, false), s);
 }
 
-int run_end_step(hexdyn_handle *h, hipStream_t s) {
+bool transported(const hexdyn_handle *h) {
+  return h->cfg.config_scalar_advection && h->cfg.config_split_dynamics_transport && h->I.n.NT > 0;
+}
+
+// device work that closes a step (before the validation flag is read)
+int end_step_device(hexdyn_handle *h, hipStream_t s) {
+  if (transported(h) && h->I.n.NM > 0 && !h->cfg.config_monotonic)
+    if (NS_CALL(h, clamp_scalars, h->I, h->I.b.q_stage, s)) return HEXDYN_E_HIP;
+  return 0;
+}
+
+// read the flag; publish by swapping buffers when it is clear
+int end_step_publish(hexdyn_handle *h, hipStream_t s) {
   Work &B = h->I.b;
-  const Sizes &n = h->I.n;
-  const bool transported = h->cfg.config_scalar_advection && h->cfg.config_split_dynamics_transport && n.NT > 0;
-  if (transported && n.NM > 0 && (!h->cfg.config_monotonic)) {
-    if (NS_CALL(h, clamp_scalars, h->I, B.q_stage, s)) return HEXDYN_E_HIP;
-  }
   int flag = 0;
   if (hipMemcpyAsync(&flag, B.flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return HEXDYN_E_HIP;
   if (hipStreamSynchronize(s) != hipSuccess) return HEXDYN_E_HIP;
@@ -775,9 +802,14 @@ int run_end_step(hexdyn_handle *h, hipStream_t s) {
   std::swap(B.rho, B.c_rho); std::swap(B.rt, B.c_rt); std::swap(B.ru, B.c_ru); std::swap(B.rw, B.c_rw);
   std::swap(B.p_theta, B.theta); std::swap(B.p_exner, B.exner); std::swap(B.p_rho_p, B.rho_p);
   std::swap(B.p_rtheta_p, B.rtheta_p); std::swap(B.p_pp, B.pp); std::swap(B.p_u, B.u); std::swap(B.p_w, B.w);
-  if (transported) std::swap(B.q, B.q_stage);
+  if (transported(h)) std::swap(B.q, B.q_stage);
   h->time_s += h->cfg.config_dt;
   return 0;
+}
+
+int run_end_step(hexdyn_handle *h, hipStream_t s) {
+  if (end_step_device(h, s)) return HEXDYN_E_HIP;
+  return end_step_publish(h, s);
 }
 
 }  // namespace
@@ -931,42 +963,95 @@ int hexdyn_end_step(hexdyn_handle *h, void *stream) {
   return run_end_step(h, (hipStream_t)stream);
 }
 
+namespace {
+
+// Every device operation of one large step, in order, on stream s.
+int enqueue_step(hexdyn_handle *h, hipStream_t s) {
+  const hexdyn_config &c = h->cfg;
+  int rc = run_phase_begin_step(h, s);
+  for (int sub = 1; rc == 0 && sub <= c.config_dynamics_split_steps; ++sub) {
+    h->cur_sub = sub;
+    rotate_subcycle(h, sub);
+    rc |= NS_CALL(h, begin_subcycle, view(h, false), sub, s);
+    for (int st = 1; rc == 0 && st <= 3; ++st) {
+      rc |= NS_CALL(h, dyn_tend, view(h, false), st, s);
+      // divergence damping of substep ss is fused into the next edge pass:
+      // the acoustic edge update of ss+1, or the edge recovery after the last substep
+      const int dd = c.config_divergence_damping ? 1 : 0;
+      for (int ss = 1; rc == 0 && ss <= h->I.k.steps[st - 1]; ++ss)
+        rc |= NS_CALL(h, acoustic_substep, view(h, false), st, ss, (dd && ss > 1) ? 1 : 0, s);
+      rc |= NS_CALL(h, recover, view(h, true), st, sub, dd, s);
+      recovered(h, st);
+      // the last stage's diagnostics would only feed a subcycle that does not
+      // follow: the next step's begin_step recomputes them from the published state
+      if (!(sub == c.config_dynamics_split_steps && st == 3))
+        rc |= NS_CALL(h, solve_diagnostics, view(h, false), st, s);
+    }
+  }
+  if (rc == 0 && c.config_split_dynamics_transport && transported(h))
+    for (int st = 1; rc == 0 && st <= 3; ++st) rc |= hexdyn_scalars(h, st, s);
+  if (rc == 0) rc = end_step_device(h, s);
+  return rc;
+}
+
+// Key of the buffer-role layout a step starts from: the step's launches
+// depend on nothing else, so one captured graph per layout replays it.
+std::string layout_key(const hexdyn_handle *h) {
+  return std::string(reinterpret_cast<const char *>(&h->I.b), sizeof(Work));
+}
+
+// Enqueue one step as a graph launch on s.  The first step from a given
+// buffer-role layout is captured (on a private stream) and stored with the
+// roles it leaves behind; later steps from that layout replay the graph and
+// take the stored roles, which the launches determine completely.
+int enqueue_step_graph(hexdyn_handle *h, hipStream_t s) {
+  const std::string key = layout_key(h);
+  auto it = h->graphs.find(key);
+  if (it == h->graphs.end()) {
+    if (!h->capture_stream && hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking) != hipSuccess)
+      return HEXDYN_E_HIP;
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return HEXDYN_E_HIP;
+    const int rc = enqueue_step(h, h->capture_stream);
+    const hipError_t e = hipStreamEndCapture(h->capture_stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return HEXDYN_E_HIP;
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return HEXDYN_E_HIP;
+    it = h->graphs.emplace(key, std::make_pair(x, h->roles())).first;
+  }
+  if (hipGraphLaunch(it->second.first, s) != hipSuccess) return HEXDYN_E_HIP;
+  h->set_roles(it->second.second);
+  return 0;
+}
+
+}  // namespace
+
 int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream) {
   if (!h || n_steps < 0) return HEXDYN_E_INVALID;
   hipStream_t s = (hipStream_t)stream;
-  const hexdyn_config &c = h->cfg;
   for (int it = 0; it < n_steps; ++it) {
-    int rc = run_phase_begin_step(h, s);
-    for (int sub = 1; rc == 0 && sub <= c.config_dynamics_split_steps; ++sub) {
-      h->cur_sub = sub;
-      rotate_subcycle(h, sub);
-      rc |= NS_CALL(h, begin_subcycle, view(h, false), sub, s);
-      for (int st = 1; rc == 0 && st <= 3; ++st) {
-        rc |= NS_CALL(h, dyn_tend, view(h, false), st, s);
-        // divergence damping of substep ss is fused into the next edge pass:
-        // the acoustic edge update of ss+1, or the edge recovery after the last substep
-        const int dd = c.config_divergence_damping ? 1 : 0;
-        for (int ss = 1; rc == 0 && ss <= h->I.k.steps[st - 1]; ++ss)
-          rc |= NS_CALL(h, acoustic_substep, view(h, false), st, ss, (dd && ss > 1) ? 1 : 0, s);
-        rc |= NS_CALL(h, recover, view(h, true), st, sub, dd, s);
-        recovered(h, st);
-        // the last stage's diagnostics would only feed a subcycle that does not
-        // follow: the next step's begin_step recomputes them from the published state
-        if (!(sub == c.config_dynamics_split_steps && st == 3))
-          rc |= NS_CALL(h, solve_diagnostics, view(h, false), st, s);
-      }
-    }
-    if (rc == 0 && c.config_scalar_advection && c.config_split_dynamics_transport && h->I.n.NT > 0)
-      for (int st = 1; rc == 0 && st <= 3; ++st) rc |= hexdyn_scalars(h, st, stream);
+    const bool graph = h->use_graphs && !g_prof_on;
+    int rc = graph ? enqueue_step_graph(h, s) : enqueue_step(h, s);
     if (rc) {
       h->in_step = 0;
       h->start_is_pub = h->fresh_sub = h->fresh_step = h->exner_start = h->pp_pub = 0;
-      if (h->error.empty()) h->error = "kernel launch failed";
+      if (h->error.empty()) h->error = "kernel launch or graph capture failed";
       return HEXDYN_E_HIP;
     }
-    rc = run_end_step(h, s);
+    rc = end_step_publish(h, s);
     if (rc) return rc;
   }
+  return 0;
+}
+
+int hexdyn_set_graphs(hexdyn_handle *h, int enable) {
+  if (!h) return HEXDYN_E_INVALID;
+  h->use_graphs = enable ? 1 : 0;
   return 0;
 }
 

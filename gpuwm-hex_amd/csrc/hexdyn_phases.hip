@@ -1103,7 +1103,12 @@ KB k_rec_cell(const Instance I, int stage, int sub, float inv_steps) {
     ROW(I.b.c_rw, c)[k] = rwv;
     const float fw = ROW(I.b.s_rw, c)[k] + ROW(I.b.ww_avg, c)[k] * inv_steps;
     if (I.k.store_flux) ROW(I.b.flux_w, c)[k] = fw;
-    if (stage == 3) ROW(I.b.split_w, c)[k] = sub == 1 ? fw : fw + ROW(I.b.split_w, c)[k];
+    if (stage == 3) {
+      // split-transport sums: copy, then accumulate; the last subcycle also
+      // applies the 1/splits average (integration.py:27-82)
+      const float acc = sub == 1 ? fw : fw + ROW(I.b.split_w, c)[k];
+      ROW(I.b.split_w, c)[k] = sub == I.k.splits ? acc * I.k.split_inv : acc;
+    }
   }
   const float *cru = I.b.c_ru;
   float hd = 0.0f;
@@ -1130,7 +1135,10 @@ KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps, int damp, f
   ROW(I.b.c_ru, e)[k] = ru;
   const float fu = sru + ROW(I.b.ru_avg, e)[k] * inv_steps;
   if (I.k.store_flux) ROW(I.b.flux_u, e)[k] = fu;
-  if (stage == 3) ROW(I.b.split_u, e)[k] = sub == 1 ? fu : fu + ROW(I.b.split_u, e)[k];
+  if (stage == 3) {
+    const float acc = sub == 1 ? fu : fu + ROW(I.b.split_u, e)[k];
+    ROW(I.b.split_u, e)[k] = sub == I.k.splits ? acc * I.k.split_inv : acc;
+  }
   const float r0 = (ROW(I.b.s_rho_p, c0)[k] + ROW(I.b.rho_pp, c0)[k]) + ROW(I.z.rb, c0)[k];
   const float r1 = (ROW(I.b.s_rho_p, c1)[k] + ROW(I.b.rho_pp, c1)[k]) + ROW(I.z.rb, c1)[k];
   ROW(I.b.u, e)[k] = (2.0f * ru) / (r0 + r1);
@@ -1178,13 +1186,6 @@ __device__ __forceinline__ float vflux(const float *qc, float wk, int k, int L, 
   const float base = div12(wk * (7.0f * (q2 + q1) - (q3 + q0)));
   const float corr = div12((coef * fabsf(wk)) * ((q3 - q0) - 3.0f * (q2 - q1)));
   return base + corr;
-}
-
-KB k_split_scale(const Instance I, int n, float *p, float inv) {
-  const int e = wave_entity();
-  if (e >= n) return;
-  const int k = lane();
-  ROW(p, e)[k] = ROW(p, e)[k] * inv;
 }
 
 // Scalar kernels run one wavefront per entity over all species: the
@@ -1628,10 +1629,6 @@ int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
 int scalars_stage(const Instance &I, int stage, hipStream_t s) {
   const float dt = I.k.sca_dt[stage - 1];
   const float *qin = stage == 1 ? I.b.q : I.b.q_stage;  // stage 1 starts from the published scalars
-  if (stage == 1) {
-    launch(k_split_scale, I.n.E, 1, s, I, I.n.E, I.b.split_u, I.k.split_inv);
-    launch(k_split_scale, I.n.C, 1, s, I, I.n.C, I.b.split_w, I.k.split_inv);
-  }
   if (stage < 3 || !I.k.monotonic) {
     LAUNCH_NE(I, k_sc_edge, I.n.E, 1, s, I, qin, I.b.ev);
     LAUNCH_NE(I, k_sc_cell, I.n.C, 1, s, I, qin, (const float *)I.b.ev, I.b.q_next, dt,

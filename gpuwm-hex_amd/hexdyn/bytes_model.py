@@ -92,7 +92,6 @@ _K: Dict[str, List[Tuple[str, str]]] = {
                     ("SC", "L"), ("SC", "L"), ("SC", "L"), ("SC", "W")],
     "k_fct_edge2": [("SC", "L"), ("SC", "L"), ("SE", "L"), ("SE", "L")],
     "k_fct_cell2": [("SC", "L"), ("SC", "L"), ("SC", "W"), ("SC", "L"), ("SE", "L"), ("C", "L"), ("SC", "L")],
-    "k_split_scale": [("X", "W"), ("X", "W")],
     # horizontal mixing, once per subcycle
     # u, v | kdiff
     "k_mix_kdiff": [("E", "L"), ("E", "L"), ("C", "L")],

@@ -182,6 +182,29 @@ class HexDycoreDriver:
         receipt = self.step_device(n_steps)
         return HostStepResult(self.get_state(), receipt)
 
+    def write_restart(self, path, xtime: str) -> None:
+        """MPAS-style restart of the published state (hexdyn.mpas_io.write_restart), with the
+        last acoustic ru_p/rw_p words as output-only fields."""
+        from .mpas_io import write_restart
+
+        acoustic = {"ru_p": self.export_field("RU_P").cpu().numpy(), "rw_p": self.export_field("RW_P").cpu().numpy()}
+        write_restart(path, self.get_state(), xtime=xtime, model_time_s=lib().hexdyn_model_time(self._h),
+                      acoustic=acoustic)
+
+    def read_restart(self, path) -> Dict[str, object]:
+        """Load a restart written by write_restart; the next step continues bit for bit."""
+        from .mpas_io import read_restart
+
+        state, meta = read_restart(path)
+        if "scalars" not in state:
+            state["scalars"] = np.zeros(self.shapes["scalars"], dtype=np.float32)
+        self.set_state(state, saved=state)
+        return meta
+
+    def use_graphs(self, enable: bool) -> None:
+        """Step through captured hipGraphs (default) or kernel by kernel."""
+        _check(self._h, lib().hexdyn_set_graphs(self._h, 1 if enable else 0), "hexdyn_set_graphs")
+
     def integrals(self) -> Dict[str, float]:
         """FP64 global integrals of the published state (hexdyn_integrals)."""
         nq = 3 + self.case.n_scalars

@@ -65,6 +65,7 @@ def lib() -> ct.CDLL:
     L.hexdyn_powf.argtypes = [ct.c_float, ct.c_float]
     L.hexdyn_powf.restype = ct.c_float
     L.hexdyn_integrals.argtypes = [vp, ct.POINTER(ct.c_double), ct.c_int, vp]
+    L.hexdyn_set_graphs.argtypes = [vp, ct.c_int]
     if L.hexdyn_abi_version() != ABI_VERSION:
         raise HexdynLibraryMissing("libhexdyn.so ABI version mismatch")
     _lib = L
@@ -77,5 +78,5 @@ EXPORTED = [
     "hexdyn_model_time", "hexdyn_begin_step", "hexdyn_begin_subcycle", "hexdyn_dyn_tend",
     "hexdyn_acoustic_substep", "hexdyn_divergence_damping", "hexdyn_recover", "hexdyn_solve_diagnostics",
     "hexdyn_end_subcycle", "hexdyn_scalars", "hexdyn_end_step", "hexdyn_profile_enable", "hexdyn_profile_read",
-    "hexdyn_profile_kernels", "hexdyn_powf", "hexdyn_integrals",
+    "hexdyn_profile_kernels", "hexdyn_powf", "hexdyn_integrals", "hexdyn_set_graphs",
 ]

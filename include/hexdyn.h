@@ -197,6 +197,12 @@ int hexdyn_field_shape(const hexdyn_handle *h, int field, int64_t *rows, int64_t
 int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream);
 double hexdyn_model_time(const hexdyn_handle *h);
 
+/* hexdyn_step replays each large step as a captured hipGraph (default on;
+ * one graph per buffer layout, captured on first use; not used while
+ * per-kernel profiling is enabled).  enable = 0 launches kernel by kernel.
+ * (The reference captures CUDA graphs per subcycle, cuda_subcycle_graphs.py.) */
+int hexdyn_set_graphs(hexdyn_handle *h, int enable);
+
 /* Phases of one large step, in the order hexdyn_step runs them. */
 int hexdyn_begin_step(hexdyn_handle *h, void *stream);
 int hexdyn_begin_subcycle(hexdyn_handle *h, int subcycle, void *stream);

@@ -267,3 +267,41 @@ def test_fp64_integrals_match_host_sums_and_repeat(get_case):
     assert abs(b["dry_mass"] / a["dry_mass"] - 1) < 1e-6
     for s in range(case.n_scalars):
         assert abs(b[f"tracer_mass_{s}"] / a[f"tracer_mass_{s}"] - 1) < 1e-5, s
+
+
+def test_graph_replays_match_eager_steps(get_case):
+    # two buffer layouts alternate, so steps 1-2 capture and 3-5 replay; eager
+    # host work (state download) runs between replays
+    print("[gpu] hipGraph replays vs eager x1.642-moist", flush=True)
+    case = get_case("x1.642-moist")
+    runs = {}
+    for graphs in (True, False):
+        d = _driver(case)
+        d.use_graphs(graphs)
+        states = []
+        for _ in range(5):
+            d.step_device(1)
+            states.append(d.get_state())
+        runs[graphs] = states
+    for i, (g, e) in enumerate(zip(runs[True], runs[False])):
+        for k in g:
+            assert _bits_equal(g[k], e[k]), f"step {i + 1} {k}: {_diff(g[k], e[k])}"
+
+
+def test_restart_file_continues_bitwise(get_case, tmp_path):
+    print("[gpu] restart file x1.642-moist", flush=True)
+    case = get_case("x1.642-moist")
+    a = _driver(case)
+    a.step_device(3)
+    ref = a.get_state()
+    b = _driver(case)
+    b.step_device(2)
+    p = tmp_path / "restart.nc"
+    b.write_restart(p, xtime="0000-01-01_00:24:00")
+    c = _driver(case)
+    meta = c.read_restart(p)
+    assert meta["model_time_s"] == 2 * case.config.config_dt
+    c.step_device(1)
+    got = c.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
