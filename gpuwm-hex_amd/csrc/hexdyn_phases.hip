@@ -609,13 +609,14 @@ KB k_sub_cell(const Instance I) {
   }
   const float acc = cell_flux_div<NE>(I, c, k, I.b.c_ru);
   const float *rwr = ROW(I.b.c_rw, c);
-  const float trho = (-acc) - I.z.rdzw[k] * (rwr[cl(k + 1)] - rwr[k]);
+  // 0 - x, not -x: a zero divergence gives +0 (density_tendency_f32)
+  const float trho = (0.0f - acc) - I.z.rdzw[k] * (rwr[cl(k + 1)] - rwr[k]);
   const float *ppr = ROW(I.b.pp, c);
   const float dpdzm = up1(dpdz);
   float erw = 0.0f;
   if (k >= 1 && k < L) {
     const float inner = I.z.rdzu[k] * (ppr[k] - ppr[k - 1]) - (I.z.fzm[k] * dpdz + I.z.fzp[k] * dpdzm);
-    erw = I.n.NM <= 0 ? -inner : -(ROW(I.b.cqw, c)[k] * inner);
+    erw = I.n.NM <= 0 ? 0.0f - inner : 0.0f - ROW(I.b.cqw, c)[k] * inner;  // euler_w(_moist_v841)_f32
   }
   if (k < L) {
     ROW(I.b.dpdz, c)[k] = dpdz;
@@ -633,7 +634,7 @@ KB k_sub_edge(const Instance I) {
   const float *pp = I.b.pp, *zz = I.z.zz, *dz = I.b.dpdz;
   const float a = ((ROW(pp, c1)[k] - ROW(pp, c0)[k]) * I.m.invdc[e]) / (0.5f * (ROW(zz, c1)[k] + ROW(zz, c0)[k]));
   const float bb = (0.5f * ROW(I.z.zxu, e)[k]) * (ROW(dz, c0)[k] + ROW(dz, c1)[k]);
-  if (k < I.n.L) ROW(I.b.euler_ru, e)[k] = (-ROW(I.b.cqu, e)[k]) * (a - bb);
+  if (k < I.n.L) ROW(I.b.euler_ru, e)[k] = (0.0f - ROW(I.b.cqu, e)[k]) * (a - bb);
 }
 
 // ---------------------------------------------------------- stage tendencies

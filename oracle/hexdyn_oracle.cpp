@@ -410,14 +410,15 @@ template <class T> struct Oracle {
       for (int k = 0; k < L; ++k) {
         const T a = ((pp[ci(k, c1)] - pp[ci(k, c0)]) * invdc[e]) / (half * (zz[ci(k, c1)] + zz[ci(k, c0)]));
         const T b = (half * zxu[ei(k, e)]) * (dpdz[ci(k, c0)] + dpdz[ci(k, c1)]);
-        euler_ru[ei(k, e)] = (-cqu[ei(k, e)]) * (a - b);
+        euler_ru[ei(k, e)] = (T(0) - cqu[ei(k, e)]) * (a - b);
       }
     }
     mass_flux_divergence(s_ru, tend_rho);
 #pragma omp parallel for
     for (int cc = 0; cc < C; ++cc)
       for (int k = 0; k < L; ++k)
-        tend_rho[ci(k, cc)] = (-tend_rho[ci(k, cc)]) - rdzw[k] * (s_rw[(size_t)(k + 1) * C + cc] - s_rw[ci(k, cc)]);
+        // (0 - div) - ..., density_tendency_f32 (cuda_horizontal.py:412-433): a zero divergence gives +0
+        tend_rho[ci(k, cc)] = (T(0) - tend_rho[ci(k, cc)]) - rdzw[k] * (s_rw[(size_t)(k + 1) * C + cc] - s_rw[ci(k, cc)]);
 #pragma omp parallel for
     for (int cc = 0; cc < C; ++cc) {
       euler_rw[cc] = T(0);
@@ -425,7 +426,8 @@ template <class T> struct Oracle {
       for (int k = 1; k < L; ++k) {
         const T inner = rdzu[k] * (pp[ci(k, cc)] - pp[ci(k - 1, cc)]) -
                         (fzm[k] * dpdz[ci(k, cc)] + fzp[k] * dpdz[ci(k - 1, cc)]);
-        euler_rw[ci(k, cc)] = NM <= 0 ? -inner : -(cqw[ci(k, cc)] * inner);
+        // euler_w_f32 / euler_w_moist_v841_f32 (cuda_driver.py:955-975, 1604-1630)
+        euler_rw[ci(k, cc)] = NM <= 0 ? T(0) - inner : T(0) - cqw[ci(k, cc)] * inner;
       }
     }
     if (mixing) horizontal_mixing();
