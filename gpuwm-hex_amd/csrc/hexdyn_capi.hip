@@ -994,10 +994,12 @@ int enqueue_step(hexdyn_handle *h, hipStream_t s) {
   return rc;
 }
 
-// Key of the buffer-role layout a step starts from: the step's launches
-// depend on nothing else, so one captured graph per layout replays it.
+// Key of what a step's launches depend on besides the static mesh: the
+// buffer-role layout at step start and the run-time switches.  Work holds
+// only pointers and Consts only 4-byte scalars, so neither has padding.
 std::string layout_key(const hexdyn_handle *h) {
-  return std::string(reinterpret_cast<const char *>(&h->I.b), sizeof(Work));
+  return std::string(reinterpret_cast<const char *>(&h->I.b), sizeof(Work)) +
+         std::string(reinterpret_cast<const char *>(&h->I.k), sizeof(Consts));
 }
 
 // Enqueue one step as a graph launch on s.  The first step from a given
@@ -1047,6 +1049,35 @@ int hexdyn_step(hexdyn_handle *h, int n_steps, void *stream) {
     if (rc) return rc;
   }
   return 0;
+}
+
+int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *p, void *stream) {
+  if (!h) return HEXDYN_E_INVALID;
+  if (h->in_step) return set_error(h, HEXDYN_E_INVALID, "physics tendencies change between steps only");
+  Work &B = h->I.b;
+  if (!p) {
+    h->I.k.physics = 0;
+    return 0;
+  }
+  const Sizes &n = h->I.n;
+  if (!B.phys_rho) {
+    int rc = h->alloc(&B.phys_rho, (size_t)n.C * HX_ROW);
+    rc |= h->alloc(&B.phys_ru, (size_t)n.E * HX_ROW);
+    rc |= h->alloc(&B.phys_rt, (size_t)n.C * HX_ROW);
+    if (rc) return set_error(h, HEXDYN_E_HIP, "physics buffer allocation failed");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const struct { const float *src; float *dst; int n; const int *old; } f[] = {
+      {p->rho, B.phys_rho, n.C, h->old_c}, {p->rho_u, B.phys_ru, n.E, h->old_e}, {p->rho_theta, B.phys_rt, n.C, h->old_c}};
+  for (const auto &x : f) {
+    if (x.src) {
+      if (transfer(h, FieldRef{x.dst, x.n, n.L, x.old, 1}, const_cast<float *>(x.src), true, s)) return HEXDYN_E_HIP;
+    } else if (hipMemsetAsync(x.dst, 0, (size_t)x.n * HX_ROW * sizeof(float), s) != hipSuccess) {
+      return HEXDYN_E_HIP;
+    }
+  }
+  h->I.k.physics = 1;
+  return hipStreamSynchronize(s) == hipSuccess ? 0 : HEXDYN_E_HIP;
 }
 
 int hexdyn_set_graphs(hexdyn_handle *h, int enable) {

@@ -105,6 +105,8 @@ struct Work {
       *fct_hres, *fct_upw;
   // horizontal mixing: viscosity, Laplacians and the theta Euler pool
   float *kdiff, *delsq_u, *delsq_div, *delsq_vort, *delsq_theta, *delsq_w, *mix_tw, *euler_theta;
+  // held physics tendencies (rows; allocated when first installed)
+  float *phys_rho, *phys_ru, *phys_rt;
   int *flag;  // sticky validation flag
 };
 
@@ -121,6 +123,7 @@ struct Consts {
   int order, splits, monotonic, divdamp, split_transport, scalar_adv;
   int apvm_on;
   int mixing, mix4;              // Smagorinsky mixing on; fourth-order part on (h4 > 0)
+  int physics;                   // held physics tendencies are added
   int store_flux;                // recovery also stores the stage's flux_u/flux_w (read only by
                                  // hexdyn_export_field; stage 3 folds them into the split sums)
   float strain_scale, ceiling;   // (c_s*len)^2, (0.01*len^2)/dt

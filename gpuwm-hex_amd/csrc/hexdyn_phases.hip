@@ -620,7 +620,7 @@ KB k_sub_cell(const Instance I) {
   }
   if (k < L) {
     ROW(I.b.dpdz, c)[k] = dpdz;
-    ROW(I.b.tend_rho, c)[k] = trho;
+    ROW(I.b.tend_rho, c)[k] = I.k.physics ? trho + ROW(I.b.phys_rho, c)[k] : trho;
     ROW(I.b.hdiv, c)[k] = acc;  // stage-1 mass-flux divergence of the current ru
   }
   if (k <= L) ROW(I.b.euler_rw, c)[k] = erw;
@@ -698,7 +698,7 @@ KB k_tend_mom(const Instance I) {
   const float rho_edge = 0.5f * (rho0 + rho1);
   const float horiz = rho_edge * (qq - (ke1 - ke0) * I.m.invdc[e]);
   const float t = (initial + horiz) - (u_k * 0.5f) * (hd0 + hd1);
-  if (k < L) ROW(I.b.tend_ru, e)[k] = t + eru;
+  if (k < L) ROW(I.b.tend_ru, e)[k] = I.k.physics ? (t + eru) + ROW(I.b.phys_ru, e)[k] : t + eru;
 }
 
 // theta and w edge fluxes with the 3rd-order upwind-blended stencil (driver.py:1003-1034, 1140-1170)
@@ -832,7 +832,10 @@ KB k_tend_cell(const Instance I, int stage) {
     const float *zzr = ROW(I.z.zz, c);
     rw_t = q * (fzm * zzr[k] + fzp * zzr[km]);
   }
-  if (k < L) ROW(I.b.tend_rt, c)[k] = I.k.mixing ? trt + ROW(I.b.euler_theta, c)[k] : trt;
+  if (k < L) {
+    const float tmix = I.k.mixing ? trt + ROW(I.b.euler_theta, c)[k] : trt;
+    ROW(I.b.tend_rt, c)[k] = I.k.physics ? tmix + ROW(I.b.phys_rt, c)[k] : tmix;
+  }
   if (k <= L) ROW(I.b.tend_rw, c)[k] = rw_t;
 }
 

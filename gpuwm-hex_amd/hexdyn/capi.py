@@ -99,6 +99,10 @@ class HexdynConfig(ct.Structure):
     ]
 
 
+class HexdynPhysicsView(ct.Structure):
+    _fields_ = [("rho", ct.c_void_p), ("rho_u", ct.c_void_p), ("rho_theta", ct.c_void_p)]
+
+
 class HexdynStateView(ct.Structure):
     _fields_ = [(name, ct.c_void_p) for name in (
         "rho_zz", "rtheta_m", "ru", "rw", "scalars",

@@ -201,6 +201,29 @@ class HexDycoreDriver:
         self.set_state(state, saved=state)
         return meta
 
+    def set_physics_tendencies(self, rho=None, rho_u=None, rho_theta=None) -> None:
+        """Hold physics tendencies (logical [L][entity] arrays) through the following steps;
+        all None clears them (hexdyn_set_physics_tendencies)."""
+        torch = _torch()
+        if rho is None and rho_u is None and rho_theta is None:
+            _check(self._h, lib().hexdyn_set_physics_tendencies(self._h, None, ct.c_void_p(self.stream_ptr())),
+                   "hexdyn_set_physics_tendencies")
+            return
+        keep = {}
+        view = capi.HexdynPhysicsView()
+        for name, a, shape in (("rho", rho, self.shapes["rho_zz"]), ("rho_u", rho_u, self.shapes["ru"]),
+                               ("rho_theta", rho_theta, self.shapes["rtheta_m"])):
+            if a is None:
+                continue
+            arr = np.ascontiguousarray(a, dtype=np.float32)
+            if arr.shape != shape:
+                raise ValueError(f"{name} shape {arr.shape} != {shape}")
+            keep[name] = torch.from_numpy(arr).to("cuda")
+            setattr(view, name, keep[name].data_ptr())
+        torch.cuda.synchronize()
+        _check(self._h, lib().hexdyn_set_physics_tendencies(self._h, ct.byref(view), ct.c_void_p(self.stream_ptr())),
+               "hexdyn_set_physics_tendencies")
+
     def use_graphs(self, enable: bool) -> None:
         """Step through captured hipGraphs (default) or kernel by kernel."""
         _check(self._h, lib().hexdyn_set_graphs(self._h, 1 if enable else 0), "hexdyn_set_graphs")

@@ -173,6 +173,20 @@ enum {
   HEXDYN_F_COUNT
 };
 
+/* Physics tendencies held through the dynamics of the following steps until
+ * replaced or cleared: the sources the reference's physics_stage_callback
+ * assembles at a step's first stage and holds through every RK and dynamics
+ * substep (cuda_driver.py:4388-4400).  They enter as tend_rho + rho at the
+ * subcycle start (density_tendency_f32), tend_ru = (tend_ru + euler_ru) +
+ * rho_u (add_pair_inplace_f32, :4443-4446) and tend_rt = (tend_rt [+ mixing
+ * theta pool]) + rho_theta (:4484-4494).  Device pointers, logical layout;
+ * NULL members are zero. */
+typedef struct hexdyn_physics_view {
+  const float *rho;        /* [L][C] */
+  const float *rho_u;      /* [L][E] */
+  const float *rho_theta;  /* [L][C] */
+} hexdyn_physics_view;
+
 typedef struct hexdyn_handle hexdyn_handle;
 
 int hexdyn_abi_version(void);
@@ -202,6 +216,9 @@ double hexdyn_model_time(const hexdyn_handle *h);
  * per-kernel profiling is enabled).  enable = 0 launches kernel by kernel.
  * (The reference captures CUDA graphs per subcycle, cuda_subcycle_graphs.py.) */
 int hexdyn_set_graphs(hexdyn_handle *h, int enable);
+
+/* Install (view != NULL) or clear (view == NULL) held physics tendencies. */
+int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *view, void *stream);
 
 /* Phases of one large step, in the order hexdyn_step runs them. */
 int hexdyn_begin_step(hexdyn_handle *h, void *stream);

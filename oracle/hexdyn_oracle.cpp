@@ -120,6 +120,9 @@ template <class T> struct Oracle {
   int mixing = 0;
   std::vector<T> area, tri, density, defc2, defs2, defcs;  // areaCell, areaTriangle, meshDensity, weights
   std::vector<T> kdiff, euler_theta;
+  // held physics tendencies (cuda_driver.py:4294-4302, 4443-4446, 4484-4494)
+  int physics = 0;
+  std::vector<T> phys_rho, phys_ru, phys_rt;
   Stage dyn[3], sca[3];
   int failed = 0;
   int in_step = 0;
@@ -416,9 +419,11 @@ template <class T> struct Oracle {
     mass_flux_divergence(s_ru, tend_rho);
 #pragma omp parallel for
     for (int cc = 0; cc < C; ++cc)
-      for (int k = 0; k < L; ++k)
+      for (int k = 0; k < L; ++k) {
         // (0 - div) - ..., density_tendency_f32 (cuda_horizontal.py:412-433): a zero divergence gives +0
         tend_rho[ci(k, cc)] = (T(0) - tend_rho[ci(k, cc)]) - rdzw[k] * (s_rw[(size_t)(k + 1) * C + cc] - s_rw[ci(k, cc)]);
+        if (physics) tend_rho[ci(k, cc)] = tend_rho[ci(k, cc)] + phys_rho[ci(k, cc)];
+      }
 #pragma omp parallel for
     for (int cc = 0; cc < C; ++cc) {
       euler_rw[cc] = T(0);
@@ -641,6 +646,7 @@ template <class T> struct Oracle {
         const T horiz = rho_edge * (qq - (ke[ci(k, c1)] - ke[ci(k, c0)]) * invdc[e]);
         T t = (initial + horiz) - (u[ei(k, e)] * half) * (hdiv[ci(k, c0)] + hdiv[ci(k, c1)]);
         tend_ru[ei(k, e)] = t + euler_ru[ei(k, e)];
+        if (physics) tend_ru[ei(k, e)] = tend_ru[ei(k, e)] + phys_ru[ei(k, e)];
       }
     }
     // theta: edge fluxes (stored-inverse branch, driver.py:1003-1034)
@@ -689,6 +695,7 @@ template <class T> struct Oracle {
         r = r * invarea[cc];
         tend_rt[ci(k, cc)] = r - rdzw[k] * (vf[k + 1] - vf[k]);
         if (mixing) tend_rt[ci(k, cc)] = tend_rt[ci(k, cc)] + euler_theta[ci(k, cc)];
+        if (physics) tend_rt[ci(k, cc)] = tend_rt[ci(k, cc)] + phys_rt[ci(k, cc)];
       }
     }
     // w transport (stored-inverse branch, driver.py:1140-1207), Euler pool, omega conversion
@@ -1254,6 +1261,19 @@ template <class T> struct Oracle {
   }
 };
 
+template <class O> static void set_physics(O &o, const float *rho, const float *ru, const float *rt) {
+  using T = typename std::remove_reference<decltype(o.rho[0])>::type;
+  auto in = [](std::vector<T> &dst, const float *src, size_t n) {
+    dst.assign(n, T(0));
+    if (src)
+      for (size_t i = 0; i < n; ++i) dst[i] = (T)src[i];
+  };
+  o.physics = (rho || ru || rt) ? 1 : 0;
+  in(o.phys_rho, rho, o.cL());
+  in(o.phys_ru, ru, o.eL());
+  in(o.phys_rt, rt, o.cL());
+}
+
 struct Handle {
   int precision;  // 32 or 64
   std::unique_ptr<Oracle<float>> f;
@@ -1320,6 +1340,12 @@ void *oracle_create(const hexdyn_mesh *m, const hexdyn_vertical *v, const hexdyn
 }
 
 void oracle_destroy(void *h) { delete (Handle *)h; }
+
+
+int oracle_set_physics(void *h, const float *rho, const float *rho_u, const float *rho_theta) {
+  DISPATCH(h, set_physics(o, rho, rho_u, rho_theta));
+  return 0;
+}
 
 int oracle_set_state(void *h, const hexdyn_state_view *s, int rebuild) {
   DISPATCH(h, set_state(o, s, rebuild));

@@ -40,10 +40,13 @@ def _diff(a, b):
 STATE = ["RHO_ZZ", "RTHETA_M", "RU", "RW", "THETA_M", "EXNER", "RHO_P", "RTHETA_P", "PRESSURE_P", "U", "W"]
 
 
-def _phase_walk(case):
+def _phase_walk(case, physics=None):
     o = Oracle(case, 32)
     o.set_state(initial_state_arrays(case), rebuild=True)
     d = _driver(case)
+    if physics is not None:
+        o.set_physics(**physics)
+        d.set_physics_tendencies(**physics)
     bad = []
 
     def cmp(tag, fields):
@@ -305,3 +308,15 @@ def test_restart_file_continues_bitwise(get_case, tmp_path):
     got = c.get_state()
     for k in ref:
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
+def test_phase_by_phase_bitwise_with_physics_tendencies(get_case):
+    print("[gpu] phase walk with held physics tendencies x1.2562-moist + mixing", flush=True)
+    case = _mixing(get_case, "x1.2562-moist")
+    L, C, E = case.n_levels, case.mesh.n_cells, case.mesh.n_edges
+    rng = np.random.default_rng(11)
+    physics = {"rho": (1e-9 * rng.standard_normal((L, C))).astype(np.float32),
+               "rho_u": (1e-6 * rng.standard_normal((L, E))).astype(np.float32),
+               "rho_theta": (1e-6 * rng.standard_normal((L, C))).astype(np.float32)}
+    bad = _phase_walk(case, physics)
+    assert not bad, "\n".join(bad[:20])

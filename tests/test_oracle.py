@@ -268,3 +268,45 @@ def test_mixing_theta_del2_is_dissipative_fp64(get_case):
     assert np.all(diss < 0.0)
     # and the filter is active: each level loses a visible amount
     assert np.all(np.abs(diss) > 1e-12 * (area * theta * np.abs(tend)).sum(axis=1))
+
+
+# ---------------------------------------------------------- physics seam
+def test_physics_tendencies_enter_the_budgets_fp64(get_case):
+    # a held density and rho*theta tendency adds dt * sum(A dz tendency) to the
+    # global dry mass and rho*theta integrals (the dynamics conserve both);
+    # a held rho_u tendency moves the momentum but no mass
+    case = get_case("x1.642-dry")
+    L, C, E = case.n_levels, case.mesh.n_cells, case.mesh.n_edges
+    area = (1.0 / case.inv_area_cell.astype(np.float64))[None, :]
+    dz = _dz(case)
+    rng = np.random.default_rng(7)
+    prho = (1e-9 * rng.standard_normal((L, C))).astype(np.float32)
+    prt = (1e-6 * rng.standard_normal((L, C))).astype(np.float32)
+    pru = (1e-6 * rng.standard_normal((L, E))).astype(np.float32)
+    shape = (L, C)
+
+    def totals(o):
+        return ((area * o.export_field_f64("RHO_ZZ").reshape(shape) * dz).sum(),
+                (area * o.export_field_f64("RTHETA_M").reshape(shape) * dz).sum())
+
+    base = Oracle(case, 64)
+    base.set_state(initial_state_arrays(case), rebuild=True)
+    m0, t0 = totals(base)
+    assert base.step(1) == 0
+    mb, tb = totals(base)
+    phys = Oracle(case, 64)
+    phys.set_state(initial_state_arrays(case), rebuild=True)
+    phys.set_physics(rho=prho, rho_u=pru, rho_theta=prt)
+    assert phys.step(1) == 0
+    mp, tp = totals(phys)
+    dt = case.config.config_dt
+    assert abs((mp - mb) - dt * (area * prho * dz).sum()) <= 1e-9 * abs(dt * (area * np.abs(prho) * dz).sum())
+    assert abs((tp - tb) - dt * (area * prt * dz).sum()) <= 1e-9 * abs(dt * (area * np.abs(prt) * dz).sum())
+    # the momentum tendency changes the flow but not the mass budget
+    mom = Oracle(case, 64)
+    mom.set_state(initial_state_arrays(case), rebuild=True)
+    mom.set_physics(rho_u=pru)
+    assert mom.step(1) == 0
+    mm, _ = totals(mom)
+    assert abs(mm / mb - 1) < 1e-13
+    assert np.abs(mom.export_field_f64("RU") - base.export_field_f64("RU")).max() > 0
