@@ -208,7 +208,7 @@ struct hexdyn_handle {
 
 namespace {
 
-int run_phase_begin_step(hexdyn_handle *h, hipStream_t s);
+int run_phase_begin_step(hexdyn_handle *h, hipStream_t s, bool fused);
 
 // The device pointers the kernels of the current phase see (buffer roles above).
 Instance view(const hexdyn_handle *h, bool recovery) {
@@ -763,7 +763,7 @@ __global__ void k_clear_flag(int *flag) {
   if (threadIdx.x == 0) flag[0] = 0;
 }
 
-int run_phase_begin_step(hexdyn_handle *h, hipStream_t s) {
+int run_phase_begin_step(hexdyn_handle *h, hipStream_t s, bool fused) {
   Work &B = h->I.b;
   hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s, B.flag);
   if (hipGetLastError() != hipSuccess) return HEXDYN_E_HIP;
@@ -773,7 +773,9 @@ int run_phase_begin_step(hexdyn_handle *h, hipStream_t s) {
   h->fresh_step = 1;
   h->exner_start = 1;
   h->pp_pub = 1;
-  return NS_CALL(h, begin_step, view(h, false), s);
+  Instance J = view(h, false);
+  J.k.fused = fused ? 1 : 0;
+  return NS_CALL(h, begin_step, J, s);
 }
 
 bool transported(const hexdyn_handle *h) {
@@ -899,7 +901,7 @@ int hexdyn_import_field(hexdyn_handle *h, int field, const float *src, void *str
 
 int hexdyn_begin_step(hexdyn_handle *h, void *stream) {
   if (!h) return HEXDYN_E_INVALID;
-  return run_phase_begin_step(h, (hipStream_t)stream);
+  return run_phase_begin_step(h, (hipStream_t)stream, false);
 }
 
 int hexdyn_begin_subcycle(hexdyn_handle *h, int subcycle, void *stream) {
@@ -965,27 +967,36 @@ int hexdyn_end_step(hexdyn_handle *h, void *stream) {
 
 namespace {
 
-// Every device operation of one large step, in order, on stream s.
+// The instance as the whole-step sequence sees it: fused launches on.
+Instance step_view(const hexdyn_handle *h, bool recovery) {
+  Instance J = view(h, recovery);
+  J.k.fused = 1;
+  return J;
+}
+
+// Every device operation of one large step, in order, on stream s.  Each
+// stage's theta/w edge fluxes are formed by the diagnostics pass before it
+// (begin_step's for the first stage of the first subcycle).
 int enqueue_step(hexdyn_handle *h, hipStream_t s) {
   const hexdyn_config &c = h->cfg;
-  int rc = run_phase_begin_step(h, s);
+  int rc = run_phase_begin_step(h, s, true);
   for (int sub = 1; rc == 0 && sub <= c.config_dynamics_split_steps; ++sub) {
     h->cur_sub = sub;
     rotate_subcycle(h, sub);
-    rc |= NS_CALL(h, begin_subcycle, view(h, false), sub, s);
+    rc |= NS_CALL(h, begin_subcycle, step_view(h, false), sub, s);
     for (int st = 1; rc == 0 && st <= 3; ++st) {
-      rc |= NS_CALL(h, dyn_tend, view(h, false), st, s);
+      rc |= NS_CALL(h, dyn_tend, step_view(h, false), st, s);
       // divergence damping of substep ss is fused into the next edge pass:
       // the acoustic edge update of ss+1, or the edge recovery after the last substep
       const int dd = c.config_divergence_damping ? 1 : 0;
       for (int ss = 1; rc == 0 && ss <= h->I.k.steps[st - 1]; ++ss)
-        rc |= NS_CALL(h, acoustic_substep, view(h, false), st, ss, (dd && ss > 1) ? 1 : 0, s);
-      rc |= NS_CALL(h, recover, view(h, true), st, sub, dd, s);
+        rc |= NS_CALL(h, acoustic_substep, step_view(h, false), st, ss, (dd && ss > 1) ? 1 : 0, s);
+      rc |= NS_CALL(h, recover, step_view(h, true), st, sub, dd, s);
       recovered(h, st);
       // the last stage's diagnostics would only feed a subcycle that does not
       // follow: the next step's begin_step recomputes them from the published state
       if (!(sub == c.config_dynamics_split_steps && st == 3))
-        rc |= NS_CALL(h, solve_diagnostics, view(h, false), st, s);
+        rc |= NS_CALL(h, solve_diagnostics, step_view(h, false), st, s);
     }
   }
   if (rc == 0 && c.config_split_dynamics_transport && transported(h))

@@ -126,6 +126,9 @@ struct Consts {
   int physics;                   // held physics tendencies are added
   int store_flux;                // recovery also stores the stage's flux_u/flux_w (read only by
                                  // hexdyn_export_field; stage 3 folds them into the split sums)
+  int fused;                     // whole-step launch sequence: the next stage's theta/w edge fluxes
+                                 // ride on the edge diagnostics, the first acoustic edge update on
+                                 // the momentum tendency, and ru_avg is kept only where read back
   float strain_scale, ceiling;   // (c_s*len)^2, (0.01*len^2)/dt
 };
 

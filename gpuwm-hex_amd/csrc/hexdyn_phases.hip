@@ -195,12 +195,11 @@ KB k_diag_cell(const Instance I, const float *__restrict__ u) {
   }
 }
 
+// pv_edge (with APVM) and, at rk step 3, the TRiSK tangential velocity v of edge e
 template <int NE>
-KB k_diag_edge(const Instance I, const float *__restrict__ u, int recon) {
+__device__ __forceinline__ void diag_edge_body(const Instance &I, const float *__restrict__ u, int e, int k,
+                                               int recon) {
   constexpr int NX = 2 * NE - 2;
-  const int e = wave_entity();
-  if (e >= I.n.E) return;
-  const int k = lane();
   float vv;
   if (recon) {
     int nb[NX];
@@ -232,6 +231,29 @@ KB k_diag_edge(const Instance I, const float *__restrict__ u, int recon) {
     if (recon) ROW(I.b.v, e)[k] = vv;
     ROW(I.b.pv_edge, e)[k] = pve;
   }
+}
+
+template <int NE>
+KB k_diag_edge(const Instance I, const float *__restrict__ u, int recon) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  diag_edge_body<NE>(I, u, e, lane(), recon);
+}
+
+// forward declaration: theta/w edge fluxes of a stage (defined with the tendencies)
+template <int NE>
+__device__ __forceinline__ void tend_flux_edge(const Instance &I, int e, int k, int stage);
+
+// the edge diagnostics and, in the same pass, the next stage's theta/w edge
+// fluxes: every input of those fluxes (theta, w, ru after recovery; the
+// subcycle-start ru and theta) is final once recovery has run
+template <int NE>
+KB k_diag_edge_flux(const Instance I, const float *__restrict__ u, int recon, int stage) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  diag_edge_body<NE>(I, u, e, k, recon);
+  tend_flux_edge<NE>(I, e, k, stage);
 }
 
 // ---------------------------------------------------------- moist coefficients
@@ -639,13 +661,11 @@ KB k_sub_edge(const Instance I) {
 
 // ---------------------------------------------------------- stage tendencies
 
-// momentum tendency of one stage (dynamics.py:223-268, dynamics_v841.py:128-185)
+// momentum tendency of one stage (dynamics.py:223-268, dynamics_v841.py:128-185);
+// returns lane k's tend_ru value, stored for levels k < L
 template <int NE>
-KB k_tend_mom(const Instance I) {
+__device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) {
   constexpr int NX = 2 * NE - 2;
-  const int e = wave_entity();
-  if (e >= I.n.E) return;
-  const int k = lane();
   const int L = I.n.L;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   int nb[NX];
@@ -698,16 +718,42 @@ KB k_tend_mom(const Instance I) {
   const float rho_edge = 0.5f * (rho0 + rho1);
   const float horiz = rho_edge * (qq - (ke1 - ke0) * I.m.invdc[e]);
   const float t = (initial + horiz) - (u_k * 0.5f) * (hd0 + hd1);
-  if (k < L) ROW(I.b.tend_ru, e)[k] = I.k.physics ? (t + eru) + ROW(I.b.phys_ru, e)[k] : t + eru;
+  const float tr = I.k.physics ? (t + eru) + ROW(I.b.phys_ru, e)[k] : t + eru;
+  if (k < L) ROW(I.b.tend_ru, e)[k] = tr;
+  return tr;
 }
 
-// theta and w edge fluxes with the 3rd-order upwind-blended stencil (driver.py:1003-1034, 1140-1170)
 template <int NE>
-KB k_tend_flux(const Instance I, int stage) {
-  constexpr int NX = 2 * NE - 2;
+KB k_tend_mom(const Instance I) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  tend_mom_edge<NE>(I, e, lane());
+}
+
+// the momentum tendency and the first acoustic substep's edge update
+// (ru_p = dts * tend_ru, acoustic_v841.py:296-330) in one pass.  The running
+// sum ru_avg equals ru_p after one substep; it is stored only when a later
+// substep reads it back (with two substeps the second forms p1 + p2 from
+// ru_p, with one the recovery reads ru_p itself).
+template <int NE>
+KB k_tend_mom_ac(const Instance I, float dts, int store_avg) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
+  const float tr = tend_mom_edge<NE>(I, e, k);
+  if (k < I.n.L) {
+    const float p = dts * tr;
+    ROW(I.b.ru_p, e)[k] = p;
+    if (store_avg) ROW(I.b.ru_avg, e)[k] = p;
+  }
+}
+
+// theta and w edge fluxes of edge e with the 3rd-order upwind-blended
+// stencil (driver.py:1003-1034, 1140-1170) and, at stages 2-3, the theta RK
+// correction flux (cuda_dynamics_v841.py:209-247)
+template <int NE>
+__device__ __forceinline__ void tend_flux_edge(const Instance &I, int e, int k, int stage) {
+  constexpr int NX = 2 * NE - 2;
   const int L = I.n.L;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   int ac[NX];
@@ -747,6 +793,13 @@ KB k_tend_flux(const Instance I, int stage) {
     if (stage > 1) ROW(I.b.fcorr, e)[k] = ((I.m.dv[e] * (sru - rue)) * 0.5f) * (sth1 + sth0);
     if (k >= 1) ROW(I.b.fw, e)[k] = rui * ew;
   }
+}
+
+template <int NE>
+KB k_tend_flux(const Instance I, int stage) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  tend_flux_edge<NE>(I, e, lane(), stage);
 }
 
 template <int NE>
@@ -851,14 +904,18 @@ __device__ __forceinline__ float damped_ru_p(const Instance &I, int e, int k, fl
 
 // damp_prev: apply the previous substep's divergence damping to ru_p first
 // (the same update k_divdamp makes, fused into this pass over the edges)
-KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_coef) {
+// avg_from_rp: this is the second of two substeps and ru_avg was not stored at
+// the first, where it equals the undamped ru_p read here (p1 + p2, the same
+// operands in the same order as ru_avg + p)
+KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_coef, int avg_from_rp) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
   float *rp = ROW(I.b.ru_p, e), *ra = ROW(I.b.ru_avg, e);
   const float tr = ROW(I.b.tend_ru, e)[k];
-  const float rp_in = damp_prev ? damped_ru_p(I, e, k, rp[k], damp_coef) : rp[k];
+  const float rp_raw = rp[k];
+  const float rp_in = damp_prev ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
   if (first) {
     const float p = dts * tr;
     rp[k] = p;
@@ -872,7 +929,7 @@ KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_c
   pg = pg + ((0.5f * ROW(I.z.zxu, e)[k]) * I.k.gravity) * (ROW(I.b.rho_pp, c0)[k] + ROW(I.b.rho_pp, c1)[k]);
   const float p = rp_in + dts * (tr - pg);
   rp[k] = p;
-  ra[k] = ra[k] + p;
+  ra[k] = (avg_from_rp ? rp_raw : ra[k]) + p;
 }
 
 template <int NE>
@@ -1123,21 +1180,24 @@ KB k_rec_cell(const Instance I, int stage, int sub, float inv_steps) {
 }
 
 // damp: apply the last substep's divergence damping to ru_p first
-KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps, int damp, float damp_coef) {
+// avg_from_rp: one substep and ru_avg not stored, so the average is the
+// undamped ru_p of that substep
+KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps, int damp, float damp_coef, int avg_from_rp) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const float sru = ROW(I.b.s_ru, e)[k];
-  float rp = ROW(I.b.ru_p, e)[k];
+  const float rp_raw = ROW(I.b.ru_p, e)[k];
+  float rp = rp_raw;
   if (damp) {
     rp = damped_ru_p(I, e, k, rp, damp_coef);
     ROW(I.b.ru_p, e)[k] = rp;
   }
   const float ru = sru + rp;
   ROW(I.b.c_ru, e)[k] = ru;
-  const float fu = sru + ROW(I.b.ru_avg, e)[k] * inv_steps;
+  const float fu = sru + (avg_from_rp ? rp_raw : ROW(I.b.ru_avg, e)[k]) * inv_steps;
   if (I.k.store_flux) ROW(I.b.flux_u, e)[k] = fu;
   if (stage == 3) {
     const float acc = sub == 1 ? fu : fu + ROW(I.b.split_u, e)[k];
@@ -1568,7 +1628,10 @@ inline int check() {
 int solve_diagnostics(const Instance &I, int rk, hipStream_t s) {
   launch(k_diag_vertex, I.n.V, 1, s, I, (const float *)I.b.u);
   LAUNCH_NE(I, k_diag_cell, I.n.C, 1, s, I, (const float *)I.b.u);
-  LAUNCH_NE(I, k_diag_edge, I.n.E, 1, s, I, (const float *)I.b.u, rk == 3 ? 1 : 0);
+  if (I.k.fused)  // diagnostics after stage rk feed stage rk % 3 + 1 (the next subcycle's first after rk 3)
+    LAUNCH_NE(I, k_diag_edge_flux, I.n.E, 1, s, I, (const float *)I.b.u, rk == 3 ? 1 : 0, rk % 3 + 1);
+  else
+    LAUNCH_NE(I, k_diag_edge, I.n.E, 1, s, I, (const float *)I.b.u, rk == 3 ? 1 : 0);
   return check();
 }
 
@@ -1604,15 +1667,23 @@ int dyn_tend(const Instance &I, int stage, hipStream_t s) {
   if (stage == 2 && I.k.order == 3)
     launch_col(k_vcoef, I.n.C, s, I, I.k.dts[1], (const float *)I.b.s_exner, (const float *)I.b.theta,
            (const float *)I.b.rtheta_p);
-  LAUNCH_NE(I, k_tend_mom, I.n.E, 1, s, I);
-  LAUNCH_NE(I, k_tend_flux, I.n.E, 1, s, I, stage);
+  if (I.k.fused) {
+    // the edge fluxes came with the preceding diagnostics
+    LAUNCH_NE(I, k_tend_mom_ac, I.n.E, 1, s, I, I.k.dts[stage - 1], I.k.steps[stage - 1] > 2 ? 1 : 0);
+  } else {
+    LAUNCH_NE(I, k_tend_mom, I.n.E, 1, s, I);
+    LAUNCH_NE(I, k_tend_flux, I.n.E, 1, s, I, stage);
+  }
   LAUNCH_NE(I, k_tend_cell, I.n.C, 1, s, I, stage);
   return check();
 }
 
 int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStream_t s) {
   const float dts = I.k.dts[stage - 1];
-  launch(k_ac_edge, I.n.E, 1, s, I, dts, ss == 1 ? 1 : 0, damp_prev, I.k.dts_inv_coef[stage - 1]);
+  const int n = I.k.steps[stage - 1];
+  if (!(I.k.fused && ss == 1))  // fused: the first edge update ran with the momentum tendency
+    launch(k_ac_edge, I.n.E, 1, s, I, dts, ss == 1 ? 1 : 0, damp_prev, I.k.dts_inv_coef[stage - 1],
+           (I.k.fused && n == 2) ? 1 : 0);
   NEDISPATCH(I, launch_col_named("k_ac_cell", k_ac_cell<6>, I.n.C, s, I, dts, ss == 1 ? 1 : 0),
              launch_col_named("k_ac_cell", k_ac_cell<7>, I.n.C, s, I, dts, ss == 1 ? 1 : 0));
   return check();
@@ -1625,7 +1696,8 @@ int divergence_damping(const Instance &I, int stage, hipStream_t s) {
 
 int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
   const float inv = I.k.inv_steps[stage - 1];
-  launch(k_rec_edge, I.n.E, 1, s, I, stage, sub, inv, damp, I.k.dts_inv_coef[stage - 1]);
+  launch(k_rec_edge, I.n.E, 1, s, I, stage, sub, inv, damp, I.k.dts_inv_coef[stage - 1],
+         (I.k.fused && I.k.steps[stage - 1] == 1) ? 1 : 0);
   LAUNCH_NE(I, k_rec_cell, I.n.C, 1, s, I, stage, sub, inv);
   return check();
 }

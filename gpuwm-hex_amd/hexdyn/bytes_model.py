@@ -53,6 +53,10 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     "k_diag_vertex": [("E", "L"), ("V", "L"), ("V", "L"), ("V", "L")],
     "k_diag_cell": [("E", "L"), ("V", "L"), ("V", "L"), ("C", "L"), ("C", "L"), ("C", "L")],
     "k_diag_edge": [("E", "L"), ("V", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
+    # k_diag_edge's arrays, then k_tend_flux's (the whole-step form of the pass)
+    "k_diag_edge_flux": [("E", "L"), ("V", "L"), ("C", "L"), ("E", "L"), ("E", "L"),
+                         ("E", "L"), ("C", "L"), ("C", "W"), ("E", "L"), ("C", "L"),
+                         ("E", "L"), ("E", "L"), ("E", "W")],
     "k_moist_cell": [("SC", "L"), ("C", "L"), ("C", "L")],
     "k_moist_edge": [("SC", "L"), ("E", "L")],
     "k_vcoef": [("C", "L")] * 10 + [("C", "W")],
@@ -61,6 +65,9 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     # u, c_rw, pv_edge, c_rho, ke, hdiv, euler_ru | tend_ru
     "k_tend_mom": [("E", "L"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
                    ("E", "L")],
+    # k_tend_mom's arrays and the first acoustic substep's ru_p (ru_avg only with > 2 substeps)
+    "k_tend_mom_ac": [("E", "L"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
+                      ("E", "L"), ("E", "L")],
     # c_ru, theta, w, s_ru, s_theta | fth, fcorr, fw   (s_ru, s_theta, fcorr at stages 2-3)
     "k_tend_flux": [("E", "L"), ("C", "L"), ("C", "W"), ("E", "L"), ("C", "L"),
                     ("E", "L"), ("E", "L"), ("E", "W")],
@@ -68,9 +75,10 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     "k_tend_cell": [("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("C", "W"), ("E", "L"), ("E", "L"),
                     ("E", "W"), ("C", "W"), ("E", "L"), ("CM", "W"), ("CM", "W"), ("C", "L"),
                     ("C", "L"), ("C", "W")],
-    # tend_ru, ru_p, ru_avg, rtheta_pp, zz, cqu, s_exner, zxu, rho_pp | ru_p, ru_avg
-    "k_ac_edge": [("E", "L"), ("E", "L"), ("E", "L"), ("C", "L"), ("C", "L"), ("E", "L"), ("C", "L"),
-                  ("E", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
+    # the whole-step form (a second substep: the first one's damping applied, ru_avg formed from ru_p):
+    # tend_ru, ru_p, rtheta_pp, rtheta_pp_old, s_theta, zz, cqu, s_exner, zxu, rho_pp | ru_p, ru_avg
+    "k_ac_edge": [("E", "L"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
+                  ("C", "L"), ("E", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
     # rho_pp, rtheta_pp, rw_p, ww_avg, ru_p, s_theta, tend_rho, tend_rt, tend_rw, coftz, cofwz, cofwr,
     # cofwt, atri, alpha, gamma, zz, dss, s_rw, c_rw, c_rho, w | rw_p, ww_avg, rho_pp, rtheta_pp, rtheta_pp_old
     "k_ac_cell": [("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"),
@@ -82,7 +90,9 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     # rho_p, rtheta_p, c_rho, c_rt, theta, c_rw, flux_w, w, (exner, pp)
     "k_rec_cell": [("C", "L")] * 6 + [("C", "W")] * 3 + [("E", "L")] * 2 + [("C", "L"), ("CM", "W"), ("CM", "W"),
                    ("C", "L")] + [("C", "L")] * 5 + [("C", "W")] * 3 + [("C", "L")] * 2,
-    "k_rec_edge": [("E", "L")] * 3 + [("C", "L")] * 3 + [("E", "L")] * 3,
+    # the whole-step form after one damped substep: s_ru, ru_p, s_rho_p, rho_pp, rb, rtheta_pp,
+    # rtheta_pp_old, s_theta | ru_p, c_ru, u
+    "k_rec_edge": [("E", "L")] * 2 + [("C", "L")] * 6 + [("E", "L")] * 3,
     "k_sc_edge": [("SC", "L"), ("E", "L"), ("SE", "L")],
     "k_sc_cell": [("SC", "L"), ("C", "W"), ("E", "L"), ("SE", "L"), ("C", "L"), ("C", "L"), ("SC", "L"),
                   ("SC", "L")],
