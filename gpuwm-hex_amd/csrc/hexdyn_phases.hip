@@ -69,6 +69,10 @@ __device__ __forceinline__ float min0(float x) { return x < 0.0f ? x : 0.0f; }
 __device__ __forceinline__ float pmin(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float pmax(float a, float b) { return b > a ? b : a; }
 __device__ __forceinline__ int cl(int k) { return k < 0 ? 0 : (k > 63 ? 63 : k); }
+// Value of x held by lane cl(k + d): the same value a load of row[cl(k + d)]
+// gives, taken from the row already in registers.  Whole-wave shuffles: call
+// with every lane active (outside lane-dependent branches).
+__device__ __forceinline__ float lane_off(float x, int k, int d) { return __shfl(x, cl(k + d), 64); }
 
 // x / 12 with IEEE round-to-nearest.  For quotients in [2^-125, FLT_MAX] the
 // product by RN(1/12) plus one FMA residual correction is the correctly
@@ -108,7 +112,9 @@ inline int nblocks(int n) { return (n + 3) / 4; }
 // 16 columns cost one column's dependent chain instead of sixteen.
 constexpr int COLW = 16;
 constexpr int COLP = 65;
-#define KBCOL __global__ void __launch_bounds__(64 * COLW)
+// Two column blocks per CU need every SIMD to hold 8 wavefronts: the
+// register budget is capped for that (at 7 a CU would hold one block).
+#define KBCOL __global__ void __launch_bounds__(64 * COLW) __attribute__((amdgpu_waves_per_eu(8, 8)))
 
 // Gather helpers.  Connectivity rows are padded to fixed widths (NE = maxEdges
 // slots per cell, NX = 2*maxEdges-2 slots per edge) with index 0, so every
@@ -137,10 +143,11 @@ KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
   const float it = I.m.invtri[v];
   vort = vort * it;
   kev = kev * (0.25f * it);
+  const float pvv = I.m.fvert[v] + vort;
   if (k < I.n.L) {
     ROW(I.b.vorticity, v)[k] = vort;
     ROW(I.b.ke_vertex, v)[k] = kev;
-    ROW(I.b.pv_vertex, v)[k] = I.m.fvert[v] + vort;
+    ROW(I.b.pv_vertex, v)[k] = pvv;
   }
 }
 
@@ -195,65 +202,115 @@ KB k_diag_cell(const Instance I, const float *__restrict__ u) {
   }
 }
 
-// pv_edge (with APVM) and, at rk step 3, the TRiSK tangential velocity v of edge e
-template <int NE>
-__device__ __forceinline__ void diag_edge_body(const Instance &I, const float *__restrict__ u, int e, int k,
-                                               int recon) {
+// pv_edge (with APVM) and, at rk step 3, the TRiSK tangential velocity v of
+// edge e.  The bodies below only load and compute; kernels store afterwards,
+// so no load of a pass waits behind one of its own stores.
+struct EdgeDiag {
+  float v, pv;
+};
+template <int NE, bool RECON>
+struct EdgeDiagIn {
+  float un[RECON ? 2 * NE - 2 : 1], wt[RECON ? 2 * NE - 2 : 1];
+  float vrow, pv0, pv1, pc0, pc1, ue;
+  int n;
+};
+// every load of the edge diagnostics (pv_cell and u are read whether or not
+// APVM is on: both buffers always exist)
+template <int NE, bool RECON>
+__device__ __forceinline__ EdgeDiagIn<NE, RECON> diag_edge_load(const Instance &I, const float *__restrict__ u, int e,
+                                                                int k) {
   constexpr int NX = 2 * NE - 2;
-  float vv;
-  if (recon) {
+  EdgeDiagIn<NE, RECON> a;
+  if (RECON) {
     int nb[NX];
-    float un[NX];
 #pragma unroll
     for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
 #pragma unroll
-    for (int s = 0; s < NX; ++s) un[s] = ROW(u, nb[s])[k];
-    const int n = I.m.neoe[e];
-    float wt[NX];
-    slots(wt, I.m.woe + e * NX);
+    for (int s = 0; s < NX; ++s) a.un[s] = ROW(u, nb[s])[k];
+    a.n = I.m.neoe[e];
+    slots(a.wt, I.m.woe + e * NX);
+    a.vrow = 0.0f;
+  } else {
+    a.vrow = ROW(I.b.v, e)[k];
+    a.n = 0;
+  }
+  const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
+  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  a.pv0 = ROW(I.b.pv_vertex, v0)[k];
+  a.pv1 = ROW(I.b.pv_vertex, v1)[k];
+  a.pc0 = ROW(I.b.pv_cell, c0)[k];
+  a.pc1 = ROW(I.b.pv_cell, c1)[k];
+  a.ue = ROW(u, e)[k];
+  return a;
+}
+template <int NE, bool RECON>
+__device__ __forceinline__ EdgeDiag diag_edge_compute(const Instance &I, int e, const EdgeDiagIn<NE, RECON> &a) {
+  constexpr int NX = 2 * NE - 2;
+  float vv;
+  if (RECON) {
     vv = 0.0f;
 #pragma unroll
     for (int s = 0; s < NX; ++s)
-      if (s < n) vv = vv + wt[s] * un[s];
+      if (s < a.n) vv = vv + a.wt[s] * a.un[s];
   } else {
-    vv = ROW(I.b.v, e)[k];
+    vv = a.vrow;
   }
-  const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
-  const float pv0 = ROW(I.b.pv_vertex, v0)[k], pv1 = ROW(I.b.pv_vertex, v1)[k];
-  float pve = 0.5f * (pv0 + pv1);
-  if (I.k.apvm_on) {
-    const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-    const float gt = (pv1 - pv0) * I.m.invdv[e];
-    const float gn = (ROW(I.b.pv_cell, c1)[k] - ROW(I.b.pv_cell, c0)[k]) * I.m.invdc[e];
-    pve = pve - I.k.apvm_scale * (vv * gt + ROW(u, e)[k] * gn);
-  }
+  const float pve = 0.5f * (a.pv0 + a.pv1);
+  // computed either way (a select, not a branch, so the loads are not sunk)
+  const float gt = (a.pv1 - a.pv0) * I.m.invdv[e];
+  const float gn = (a.pc1 - a.pc0) * I.m.invdc[e];
+  const float pva = pve - I.k.apvm_scale * (vv * gt + a.ue * gn);
+  return {vv, I.k.apvm_on ? pva : pve};
+}
+template <int NE, bool RECON>
+__device__ __forceinline__ EdgeDiag diag_edge_values(const Instance &I, const float *__restrict__ u, int e, int k) {
+  return diag_edge_compute<NE, RECON>(I, e, diag_edge_load<NE, RECON>(I, u, e, k));
+}
+__device__ __forceinline__ void store_edge_diag(const Instance &I, int e, int k, bool recon, EdgeDiag d) {
   if (k < I.n.L) {
-    if (recon) ROW(I.b.v, e)[k] = vv;
-    ROW(I.b.pv_edge, e)[k] = pve;
+    if (recon) ROW(I.b.v, e)[k] = d.v;
+    ROW(I.b.pv_edge, e)[k] = d.pv;
   }
 }
 
-template <int NE>
-KB k_diag_edge(const Instance I, const float *__restrict__ u, int recon) {
-  const int e = wave_entity();
-  if (e >= I.n.E) return;
-  diag_edge_body<NE>(I, u, e, lane(), recon);
-}
-
-// forward declaration: theta/w edge fluxes of a stage (defined with the tendencies)
-template <int NE>
-__device__ __forceinline__ void tend_flux_edge(const Instance &I, int e, int k, int stage);
-
-// the edge diagnostics and, in the same pass, the next stage's theta/w edge
-// fluxes: every input of those fluxes (theta, w, ru after recovery; the
-// subcycle-start ru and theta) is final once recovery has run
-template <int NE>
-KB k_diag_edge_flux(const Instance I, const float *__restrict__ u, int recon, int stage) {
+// RECON: rk step 3, where v is reconstructed (cuda_horizontal_v841.py:1058-1097)
+template <int NE, bool RECON>
+KB k_diag_edge(const Instance I, const float *__restrict__ u) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
-  diag_edge_body<NE>(I, u, e, k, recon);
-  tend_flux_edge<NE>(I, e, k, stage);
+  store_edge_diag(I, e, k, RECON, diag_edge_values<NE, RECON>(I, u, e, k));
+}
+
+// theta and w edge fluxes of a stage (defined with the tendencies)
+struct EdgeFlux {
+  float th, corr, w;
+};
+template <int NE>
+struct EdgeFluxIn;
+template <int NE, bool CORR>
+__device__ __forceinline__ EdgeFluxIn<NE> tend_flux_load(const Instance &I, int e, int k);
+template <int NE, bool CORR>
+__device__ __forceinline__ EdgeFlux tend_flux_compute(const Instance &I, int e, const EdgeFluxIn<NE> &b);
+template <bool CORR>
+__device__ __forceinline__ void store_edge_flux(const Instance &I, int e, int k, EdgeFlux f);
+
+// the edge diagnostics and, in the same pass, the next stage's theta/w edge
+// fluxes: every input of those fluxes (theta, w, ru after recovery; the
+// subcycle-start ru and theta) is final once recovery has run.  After rk
+// step 3 (RECON) the next stage is a first stage (no RK correction flux);
+// after rk steps 1 and 2 it is stage 2 or 3 (with the correction).
+template <int NE, bool RECON>
+KB k_diag_edge_flux(const Instance I, const float *__restrict__ u) {
+  const int e = wave_entity();
+  if (e >= I.n.E) return;
+  const int k = lane();
+  const EdgeDiagIn<NE, RECON> a = diag_edge_load<NE, RECON>(I, u, e, k);
+  const EdgeFluxIn<NE> b = tend_flux_load<NE, !RECON>(I, e, k);
+  const EdgeDiag d = diag_edge_compute<NE, RECON>(I, e, a);
+  const EdgeFlux f = tend_flux_compute<NE, !RECON>(I, e, b);
+  store_edge_diag(I, e, k, RECON, d);
+  store_edge_flux<!RECON>(I, e, k, f);
 }
 
 // ---------------------------------------------------------- moist coefficients
@@ -294,38 +351,36 @@ KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const 
   const int c = live ? c0 : 0;  // every wave reaches the barriers
   const int k = lane();
   const int L = I.n.L;
-  const int km = cl(k - 1), kp = cl(k + 1);
   const float g = I.k.gravity, rcv = I.k.rcv, c2 = I.k.c2;
   const float dts2 = dts * dts;
-  const float *zzr = ROW(I.z.zz, c);
-  const float zk = zzr[k], zkm = zzr[km];
-  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
-  const float rdzw = I.z.rdzw[k], rdzwm = I.z.rdzw[km];
-  const float etp = I.z.etp[k], etpm = I.z.etp[km];
-  const float exk = ROW(exn, c)[k], exkm = ROW(exn, c)[km];
+  // every load first; the level k-1 and k+1 values come from neighbouring lanes
+  const float zk = ROW(I.z.zz, c)[k], exk = ROW(exn, c)[k], thk = ROW(theta, c)[k];
+  const float cqw = ROW(I.b.cqw, c)[k], qt = ROW(I.b.qtot, c)[k], rtpk = ROW(rtp, c)[k];
+  const float rbk = ROW(I.z.rb, c)[k], rtbk = ROW(I.z.rtb, c)[k], exbk = ROW(I.z.exb, c)[k];
+  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k], rdzu = I.z.rdzu[k];
+  const float rdzw = I.z.rdzw[k], etp = I.z.etp[k], ewp = I.z.ewp[k];
+  const float zkm = lane_off(zk, k, -1), exkm = lane_off(exk, k, -1), thkm = lane_off(thk, k, -1);
+  const float rdzwm = lane_off(rdzw, k, -1), etpm = lane_off(etp, k, -1);
+  const float ewpm = lane_off(ewp, k, -1), ewpp = lane_off(ewp, k, 1);
   float cofwr = 0.0f, cofwz = 0.0f, coftz = 0.0f;
   if (k >= 1 && k < L) {
     const float iz = fzm * zk + fzp * zkm;
     cofwr = (0.5f * g) * iz;
-    cofwz = (((c2 * iz) * I.z.rdzu[k]) * ROW(I.b.cqw, c)[k]) * (fzm * exk + fzp * exkm);
-    coftz = fzm * ROW(theta, c)[k] + fzp * ROW(theta, c)[km];
+    cofwz = (((c2 * iz) * rdzu) * cqw) * (fzm * exk + fzp * exkm);
+    coftz = fzm * thk + fzp * thkm;
   }
   float cofwt = 0.0f;
-  if (k < L) {
-    const float rbk = ROW(I.z.rb, c)[k], rtbk = ROW(I.z.rtb, c)[k], exbk = ROW(I.z.exb, c)[k];
-    cofwt = (((((((0.5f * rcv) * zk) * g) * rbk) / (1.0f + ROW(I.b.qtot, c)[k])) * exk) /
-             ((rtbk + ROW(rtp, c)[k]) * exbk));
-  }
+  if (k < L) cofwt = (((((((0.5f * rcv) * zk) * g) * rbk) / (1.0f + qt)) * exk) / ((rtbk + rtpk) * exbk));
   const float coftz_m = up1(coftz), coftz_p = dn1(coftz), cofwt_m = up1(cofwt);
   float a = 0.0f, b = 1.0f, cc = 0.0f;
   if (k >= 1 && k < L) {
     a = (((-cofwz) * coftz_m) * rdzwm) * zkm + cofwr * rdzwm - (cofwt_m * coftz_m) * rdzwm;
-    a = (a * etpm) * I.z.ewp[km];
+    a = (a * etpm) * ewpm;
     b = (cofwz * coftz) * ((etp * rdzw) * zk + (etpm * rdzwm) * zkm) -
         coftz * ((etp * cofwt) * rdzw - (etpm * cofwt_m) * rdzwm) + cofwr * (etp * rdzw - etpm * rdzwm);
-    b = b * I.z.ewp[k];
+    b = b * ewp;
     cc = (((-cofwz) * coftz_p) * rdzw) * zk - cofwr * rdzw + (cofwt * coftz_p) * rdzw;
-    cc = (cc * etp) * I.z.ewp[kp];
+    cc = (cc * etp) * ewpp;
     if (k == L - 1) cc = 0.0f;
   }
   // alpha/gamma recurrence up the column (acoustic_v841.py:81-233), lane = cell
@@ -440,9 +495,10 @@ KB k_mix_edge(const Instance I) {
   const float kedge = 0.5f * (kd0 + kd1);
   const float rho_e = 0.5f * (r0 + r1);
   if (k < I.n.L) {
-    if (I.k.mix4) ROW(I.b.delsq_u, e)[k] = lap;
     float *pool = ROW(I.b.euler_ru, e);
-    pool[k] = pool[k] + ((rho_e * kedge) * lap) * I.m.mx_del2[e];
+    const float pk = pool[k];
+    if (I.k.mix4) ROW(I.b.delsq_u, e)[k] = lap;
+    pool[k] = pk + ((rho_e * kedge) * lap) * I.m.mx_del2[e];
   }
 }
 
@@ -620,29 +676,31 @@ KB k_sub_cell(const Instance I) {
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
+  const bool moist = I.n.NM > 0;
+  // every load first (the mass-flux divergence gathers among them)
   const float rho_p = ROW(I.b.s_rho_p, c)[k];
-  const float g = I.k.gravity;
-  float dpdz;
-  if (I.n.NM <= 0) {
-    dpdz = (-g) * rho_p;
-  } else {
-    const float qt = ROW(I.b.qtot, c)[k];
-    dpdz = (-g) * (ROW(I.z.rb, c)[k] * qt + rho_p * (1.0f + qt));
-  }
+  const float qt = moist ? ROW(I.b.qtot, c)[k] : 0.0f;
+  const float rb = moist ? ROW(I.z.rb, c)[k] : 0.0f;
+  const float cqw = moist ? ROW(I.b.cqw, c)[k] : 0.0f;
+  const float prho = I.k.physics ? ROW(I.b.phys_rho, c)[k] : 0.0f;
+  const float rw_k = ROW(I.b.c_rw, c)[k];
+  const float pp_k = ROW(I.b.pp, c)[k];
   const float acc = cell_flux_div<NE>(I, c, k, I.b.c_ru);
-  const float *rwr = ROW(I.b.c_rw, c);
+  const float g = I.k.gravity;
+  const float dpdz = moist ? (-g) * (rb * qt + rho_p * (1.0f + qt)) : (-g) * rho_p;
   // 0 - x, not -x: a zero divergence gives +0 (density_tendency_f32)
-  const float trho = (0.0f - acc) - I.z.rdzw[k] * (rwr[cl(k + 1)] - rwr[k]);
-  const float *ppr = ROW(I.b.pp, c);
+  const float trho = (0.0f - acc) - I.z.rdzw[k] * (lane_off(rw_k, k, 1) - rw_k);
+  const float pp_m1 = lane_off(pp_k, k, -1);
   const float dpdzm = up1(dpdz);
   float erw = 0.0f;
   if (k >= 1 && k < L) {
-    const float inner = I.z.rdzu[k] * (ppr[k] - ppr[k - 1]) - (I.z.fzm[k] * dpdz + I.z.fzp[k] * dpdzm);
-    erw = I.n.NM <= 0 ? 0.0f - inner : 0.0f - ROW(I.b.cqw, c)[k] * inner;  // euler_w(_moist_v841)_f32
+    const float inner = I.z.rdzu[k] * (pp_k - pp_m1) - (I.z.fzm[k] * dpdz + I.z.fzp[k] * dpdzm);
+    erw = moist ? 0.0f - cqw * inner : 0.0f - inner;  // euler_w(_moist_v841)_f32
   }
   if (k < L) {
+    const float tr = I.k.physics ? trho + prho : trho;
     ROW(I.b.dpdz, c)[k] = dpdz;
-    ROW(I.b.tend_rho, c)[k] = I.k.physics ? trho + ROW(I.b.phys_rho, c)[k] : trho;
+    ROW(I.b.tend_rho, c)[k] = tr;
     ROW(I.b.hdiv, c)[k] = acc;  // stage-1 mass-flux divergence of the current ru
   }
   if (k <= L) ROW(I.b.euler_rw, c)[k] = erw;
@@ -673,7 +731,7 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
   const float *ur = ROW(I.b.u, e);
   const float u_k = ur[k];
-  const float u_m1 = ur[cl(k - 1)], u_m2 = ur[cl(k - 2)], u_p1 = ur[cl(k + 1)];
+  const float u_m1 = lane_off(u_k, k, -1), u_m2 = lane_off(u_k, k, -2), u_p1 = lane_off(u_k, k, 1);
   const float rw0 = ROW(I.b.c_rw, c0)[k], rw1 = ROW(I.b.c_rw, c1)[k];
   const float pve = ROW(I.b.pv_edge, e)[k];
   float un[NX], pvn[NX];
@@ -686,8 +744,10 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   const float ke0 = ROW(I.b.ke, c0)[k], ke1 = ROW(I.b.ke, c1)[k];
   const float hd0 = ROW(I.b.hdiv, c0)[k], hd1 = ROW(I.b.hdiv, c1)[k];
   const float eru = ROW(I.b.euler_ru, e)[k];
+  const float pru = I.k.physics ? ROW(I.b.phys_ru, e)[k] : 0.0f;
+  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k], rdzw = I.z.rdzw[k];
+  const float ui = I.z.uinit[k], vi = I.z.vinit[k];
   const float rwe = 0.5f * (rw0 + rw1);
-  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
   float wduz = 0.0f;
   if (k == 1 || k == L - 1) {
     wduz = rwe * (fzm * u_k + fzp * u_m1);
@@ -695,7 +755,7 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
     wduz = flux3(u_m2, u_m1, u_k, u_p1, rwe, 1.0f);
   }
   const float wduz_p = dn1(wduz);
-  const float initial = (-I.z.rdzw[k]) * (wduz_p - wduz);
+  const float initial = (-rdzw) * (wduz_p - wduz);
   const int nn = I.m.neoe[e];
   float wt[NX], ca[NX], sa[NX];
   slots(wt, I.m.woe + e * NX);
@@ -708,7 +768,7 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
       const float work_pv = 0.5f * (pve + pvn[s]);
       qq = qq + (wt[s] * un[s]) * work_pv;
     }
-  const float ui = I.z.uinit[k], vi = I.z.vinit[k], fe = I.m.fedge[e];
+  const float fe = I.m.fedge[e];
 #pragma unroll
   for (int s = 0; s < NX; ++s)
     if (s < nn) {
@@ -718,7 +778,7 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   const float rho_edge = 0.5f * (rho0 + rho1);
   const float horiz = rho_edge * (qq - (ke1 - ke0) * I.m.invdc[e]);
   const float t = (initial + horiz) - (u_k * 0.5f) * (hd0 + hd1);
-  const float tr = I.k.physics ? (t + eru) + ROW(I.b.phys_ru, e)[k] : t + eru;
+  const float tr = I.k.physics ? (t + eru) + pru : t + eru;
   if (k < L) ROW(I.b.tend_ru, e)[k] = tr;
   return tr;
 }
@@ -751,55 +811,80 @@ KB k_tend_mom_ac(const Instance I, float dts, int store_avg) {
 // theta and w edge fluxes of edge e with the 3rd-order upwind-blended
 // stencil (driver.py:1003-1034, 1140-1170) and, at stages 2-3, the theta RK
 // correction flux (cuda_dynamics_v841.py:209-247)
+// CORR: stages 2 and 3, which add the theta RK correction flux
 template <int NE>
-__device__ __forceinline__ void tend_flux_edge(const Instance &I, int e, int k, int stage) {
+struct EdgeFluxIn {
+  float thn[2 * NE - 2], wn[2 * NE - 2], av[2 * NE - 2], a3v[2 * NE - 2];
+  float rue, sru, sth0, sth1, fzm, fzp;
+  int na;
+};
+template <int NE, bool CORR>
+__device__ __forceinline__ EdgeFluxIn<NE> tend_flux_load(const Instance &I, int e, int k) {
   constexpr int NX = 2 * NE - 2;
-  const int L = I.n.L;
+  EdgeFluxIn<NE> b;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   int ac[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
-  float thn[NX], wn[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) {
-    thn[s] = ROW(I.b.theta, ac[s])[k];
-    wn[s] = ROW(I.b.w, ac[s])[k];
+    b.thn[s] = ROW(I.b.theta, ac[s])[k];
+    b.wn[s] = ROW(I.b.w, ac[s])[k];
   }
-  const float rue = ROW(I.b.c_ru, e)[k];
-  const float rue_m = up1(rue);
-  float sru = 0.0f, sth0 = 0.0f, sth1 = 0.0f;
-  if (stage > 1) {
-    sru = ROW(I.b.s_ru, e)[k];
-    sth0 = ROW(I.b.s_theta, c0)[k];
-    sth1 = ROW(I.b.s_theta, c1)[k];
+  b.rue = ROW(I.b.c_ru, e)[k];
+  b.sru = b.sth0 = b.sth1 = 0.0f;
+  if (CORR) {
+    b.sru = ROW(I.b.s_ru, e)[k];
+    b.sth0 = ROW(I.b.s_theta, c0)[k];
+    b.sth1 = ROW(I.b.s_theta, c1)[k];
   }
+  b.fzm = I.z.fzm[k];
+  b.fzp = I.z.fzp[k];
+  b.na = I.m.nadv[e];
+  slots(b.av, I.m.adv + e * NX);
+  slots(b.a3v, I.m.adv3 + e * NX);
+  return b;
+}
+template <int NE, bool CORR>
+__device__ __forceinline__ EdgeFlux tend_flux_compute(const Instance &I, int e, const EdgeFluxIn<NE> &b) {
+  constexpr int NX = 2 * NE - 2;
+  const float rue_m = up1(b.rue);
   const float coef = I.k.coef3;
-  const float rui = I.z.fzm[k] * rue + I.z.fzp[k] * rue_m;
-  const float sv_t = cs1(rue), sv_w = cs1(rui);
-  const int na = I.m.nadv[e];
-  float av[NX], a3v[NX];
-  slots(av, I.m.adv + e * NX);
-  slots(a3v, I.m.adv3 + e * NX);
+  const float rui = b.fzm * b.rue + b.fzp * rue_m;
+  const float sv_t = cs1(b.rue), sv_w = cs1(rui);
   float et = 0.0f, ew = 0.0f;
 #pragma unroll
   for (int s = 0; s < NX; ++s)
-    if (s < na) {
-      const float a = av[s], a3 = a3v[s];
-      et = et + (a + (coef * sv_t) * a3) * thn[s];
-      ew = ew + (a + (coef * sv_w) * a3) * wn[s];
+    if (s < b.na) {
+      const float a = b.av[s], a3 = b.a3v[s];
+      et = et + (a + (coef * sv_t) * a3) * b.thn[s];
+      ew = ew + (a + (coef * sv_w) * a3) * b.wn[s];
     }
-  if (k < L) {
-    ROW(I.b.fth, e)[k] = rue * et;
-    if (stage > 1) ROW(I.b.fcorr, e)[k] = ((I.m.dv[e] * (sru - rue)) * 0.5f) * (sth1 + sth0);
-    if (k >= 1) ROW(I.b.fw, e)[k] = rui * ew;
+  EdgeFlux f;
+  f.th = b.rue * et;
+  f.corr = CORR ? ((I.m.dv[e] * (b.sru - b.rue)) * 0.5f) * (b.sth1 + b.sth0) : 0.0f;
+  f.w = rui * ew;
+  return f;
+}
+template <int NE, bool CORR>
+__device__ __forceinline__ EdgeFlux tend_flux_values(const Instance &I, int e, int k) {
+  return tend_flux_compute<NE, CORR>(I, e, tend_flux_load<NE, CORR>(I, e, k));
+}
+template <bool CORR>
+__device__ __forceinline__ void store_edge_flux(const Instance &I, int e, int k, EdgeFlux f) {
+  if (k < I.n.L) {
+    ROW(I.b.fth, e)[k] = f.th;
+    if (CORR) ROW(I.b.fcorr, e)[k] = f.corr;
+    if (k >= 1) ROW(I.b.fw, e)[k] = f.w;
   }
 }
 
-template <int NE>
-KB k_tend_flux(const Instance I, int stage) {
+template <int NE, bool CORR>
+KB k_tend_flux(const Instance I) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
-  tend_flux_edge<NE>(I, e, lane(), stage);
+  const int k = lane();
+  store_edge_flux<CORR>(I, e, k, tend_flux_values<NE, CORR>(I, e, k));
 }
 
 template <int NE>
@@ -829,26 +914,33 @@ KB k_tend_cell(const Instance I, int stage) {
 #pragma unroll
     for (int s = 0; s < NE; ++s) fco[s] = ROW(I.b.fcorr, e[s])[k];
   }
-  const float *th = ROW(I.b.theta, c), *ths = ROW(I.b.s_theta, c);
-  const float *rwc = ROW(I.b.c_rw, c), *rws = ROW(I.b.s_rw, c), *ww = ROW(I.b.w, c);
+  // the column rows, read once; vertical neighbours by shuffles
+  const float th_k = ROW(I.b.theta, c)[k], ths_k = ROW(I.b.s_theta, c)[k];
+  const float rwc_k = ROW(I.b.c_rw, c)[k], rws_k = ROW(I.b.s_rw, c)[k], ww_k = ROW(I.b.w, c)[k];
+  const float zz_k = ROW(I.z.zz, c)[k], erw_k = ROW(I.b.euler_rw, c)[k];
+  const float eth_k = I.k.mixing ? ROW(I.b.euler_theta, c)[k] : 0.0f;
+  const float prt_k = I.k.physics ? ROW(I.b.phys_rt, c)[k] : 0.0f;
   const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
-  const int km = cl(k - 1);
+  const float th_m1 = lane_off(th_k, k, -1), th_m2 = lane_off(th_k, k, -2), th_p1 = lane_off(th_k, k, 1);
+  const float ths_m1 = lane_off(ths_k, k, -1), rwc_m1 = lane_off(rwc_k, k, -1);
+  const float ww_m1 = lane_off(ww_k, k, -1), ww_m2 = lane_off(ww_k, k, -2), ww_p1 = lane_off(ww_k, k, 1);
+  const float zz_m1 = lane_off(zz_k, k, -1);
   float vf = 0.0f;
   if (k == 1) {
-    vf = rwc[1] * (fzm * th[1] + fzp * th[0]);
-    vf = vf + (rws[1] - rwc[1]) * (fzm * ths[1] + fzp * ths[0]);
+    vf = rwc_k * (fzm * th_k + fzp * th_m1);
+    vf = vf + (rws_k - rwc_k) * (fzm * ths_k + fzp * ths_m1);
   } else if (k >= 2 && k < L - 1) {
-    vf = flux3(th[k - 2], th[k - 1], th[k], th[k + 1], rwc[k], coef);
-    vf = vf + (rws[k] - rwc[k]) * (fzm * ths[k] + fzp * ths[k - 1]);
+    vf = flux3(th_m2, th_m1, th_k, th_p1, rwc_k, coef);
+    vf = vf + (rws_k - rwc_k) * (fzm * ths_k + fzp * ths_m1);
   } else if (k == L - 1) {
-    vf = rws[k] * (fzm * th[k] + fzp * th[k - 1]);
+    vf = rws_k * (fzm * th_k + fzp * th_m1);
   }
   const float vf_p = dn1(vf);
   float wf = 0.0f;
   if (k == 1 || k == L - 1) {
-    wf = (0.25f * (rwc[k] + rwc[k - 1])) * (ww[k] + ww[k - 1]);
+    wf = (0.25f * (rwc_k + rwc_m1)) * (ww_k + ww_m1);
   } else if (k >= 2 && k < L - 1) {
-    wf = flux3(ww[k - 2], ww[k - 1], ww[k], ww[k + 1], 0.5f * (rwc[k] + rwc[k - 1]), 1.0f);
+    wf = flux3(ww_m2, ww_m1, ww_k, ww_p1, 0.5f * (rwc_k + rwc_m1), 1.0f);
   }
   const float wf_p = dn1(wf);
   const float ia = I.m.invarea[c];
@@ -874,7 +966,7 @@ KB k_tend_cell(const Instance I, int stage) {
       if (s < n) q = q - sg[s] * fwv[s];
     q = q * ia;
     q = q - I.z.rdzu[k] * (wf_p - wf);
-    q = q + ROW(I.b.euler_rw, c)[k];
+    q = q + erw_k;
 #pragma unroll
     for (int s = 0; s < NE; ++s)
       if (s < n) {
@@ -882,12 +974,11 @@ KB k_tend_cell(const Instance I, int stage) {
         const float flux = sg[s] * (fzm * tk + fzp * tru_m[s]);
         q = q - (zb[s] + cs1(tk) * zb3[s]) * flux;
       }
-    const float *zzr = ROW(I.z.zz, c);
-    rw_t = q * (fzm * zzr[k] + fzp * zzr[km]);
+    rw_t = q * (fzm * zz_k + fzp * zz_m1);
   }
   if (k < L) {
-    const float tmix = I.k.mixing ? trt + ROW(I.b.euler_theta, c)[k] : trt;
-    ROW(I.b.tend_rt, c)[k] = I.k.physics ? tmix + ROW(I.b.phys_rt, c)[k] : tmix;
+    const float tmix = I.k.mixing ? trt + eth_k : trt;
+    ROW(I.b.tend_rt, c)[k] = I.k.physics ? tmix + prt_k : tmix;
   }
   if (k <= L) ROW(I.b.tend_rw, c)[k] = rw_t;
 }
@@ -906,7 +997,7 @@ __device__ __forceinline__ float damped_ru_p(const Instance &I, int e, int k, fl
 // (the same update k_divdamp makes, fused into this pass over the edges)
 // avg_from_rp: this is the second of two substeps and ru_avg was not stored at
 // the first, where it equals the undamped ru_p read here (p1 + p2, the same
-// operands in the same order as ru_avg + p)
+// operands in the same order as ru_avg + p).  Loads first, stores last.
 KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_coef, int avg_from_rp) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
@@ -914,14 +1005,15 @@ KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_c
   if (k >= I.n.L) return;
   float *rp = ROW(I.b.ru_p, e), *ra = ROW(I.b.ru_avg, e);
   const float tr = ROW(I.b.tend_ru, e)[k];
-  const float rp_raw = rp[k];
-  const float rp_in = damp_prev ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
   if (first) {
     const float p = dts * tr;
     rp[k] = p;
     ra[k] = p;
     return;
   }
+  const float rp_raw = rp[k];
+  const float ra_old = avg_from_rp ? rp_raw : ra[k];
+  const float rp_in = damp_prev ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   float pg = ((ROW(I.b.rtheta_pp, c1)[k] - ROW(I.b.rtheta_pp, c0)[k]) * I.m.invdc[e]) /
              (0.5f * (ROW(I.z.zz, c1)[k] + ROW(I.z.zz, c0)[k]));
@@ -929,11 +1021,13 @@ KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_c
   pg = pg + ((0.5f * ROW(I.z.zxu, e)[k]) * I.k.gravity) * (ROW(I.b.rho_pp, c0)[k] + ROW(I.b.rho_pp, c1)[k]);
   const float p = rp_in + dts * (tr - pg);
   rp[k] = p;
-  ra[k] = (avg_from_rp ? rp_raw : ra[k]) + p;
+  ra[k] = ra_old + p;
 }
 
-template <int NE>
-KBCOL k_ac_cell(const Instance I, float dts, int first) {
+// FIRST: the first substep of a stage (the perturbation fields start at
+// zero).  Every global load is issued before the column sweep.
+template <int NE, bool FIRST>
+KBCOL k_ac_cell(const Instance I, float dts) {
   __shared__ float s_x[COLW * COLP], s_d[COLW * COLP], s_l[COLW * COLP], s_g[COLW * COLP];
   const int w = (int)(threadIdx.x >> 6);
   const int c0 = wave_entity<COLW>();
@@ -941,7 +1035,6 @@ KBCOL k_ac_cell(const Instance I, float dts, int first) {
   const int c = live ? c0 : 0;  // every wave reaches the barriers
   const int k = lane();
   const int L = I.n.L;
-  const int kp = cl(k + 1);
   const int n = I.m.neoc[c];
   float sg[NE];
   slots(sg, I.m.csign + c * NE);
@@ -961,12 +1054,21 @@ KBCOL k_ac_cell(const Instance I, float dts, int first) {
   }
   const float sthc = ROW(I.b.s_theta, c)[k];
   float rpp = 0.0f, rtpp = 0.0f, rwp = 0.0f, wwa = 0.0f;
-  if (!first) {
+  if (!FIRST) {
     rpp = ROW(I.b.rho_pp, c)[k];
     rtpp = ROW(I.b.rtheta_pp, c)[k];
     rwp = ROW(I.b.rw_p, c)[k];
     wwa = ROW(I.b.ww_avg, c)[k];
   }
+  const float coftz = ROW(I.b.coftz, c)[k], cofwt = ROW(I.b.cofwt, c)[k];
+  const float cofwz = ROW(I.b.cofwz, c)[k], cofwr = ROW(I.b.cofwr, c)[k];
+  const float trho = ROW(I.b.tend_rho, c)[k], trt = ROW(I.b.tend_rt, c)[k], trw = ROW(I.b.tend_rw, c)[k];
+  const float zk = ROW(I.z.zz, c)[k];
+  const float a = ROW(I.b.atri, c)[k], al = ROW(I.b.alpha, c)[k], ga = ROW(I.b.gamma, c)[k];
+  const float srw = ROW(I.b.s_rw, c)[k], crw = ROW(I.b.c_rw, c)[k], damp = ROW(I.z.dss, c)[k];
+  const float crho = ROW(I.b.c_rho, c)[k], wk = ROW(I.b.w, c)[k];
+  const float rdzw = I.z.rdzw[k], ewm = I.z.ewm[k], ewp = I.z.ewp[k], etp = I.z.etp[k], etm = I.z.etm[k];
+  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
   const float rtpp_old = rtpp;
   const float ia = I.m.invarea[c];
   float r = 0.0f, t = 0.0f;
@@ -981,34 +1083,25 @@ KBCOL k_ac_cell(const Instance I, float dts, int first) {
       r = r - flux;
       t = t - (flux * 0.5f) * (th1 + th0);
     }
-  const float rdzw = I.z.rdzw[k];
-  const float coftz = ROW(I.b.coftz, c)[k];
   const float coftz_p = dn1(coftz);
-  const float ewm = I.z.ewm[k], ewm_p = I.z.ewm[kp];
+  const float ewm_p = lane_off(ewm, k, 1), ewp_p = lane_off(ewp, k, 1);
   const float rwp_p = dn1(rwp);
-  const float rs = ((rpp + dts * ROW(I.b.tend_rho, c)[k]) + r) - (dts * rdzw) * (ewm_p * rwp_p - ewm * rwp);
-  const float ts = ((rtpp + dts * ROW(I.b.tend_rt, c)[k]) + t) -
-                   (dts * rdzw) * ((ewm_p * coftz_p) * rwp_p - (ewm * coftz) * rwp);
+  const float rs = ((rpp + dts * trho) + r) - (dts * rdzw) * (ewm_p * rwp_p - ewm * rwp);
+  const float ts = ((rtpp + dts * trt) + t) - (dts * rdzw) * ((ewm_p * coftz_p) * rwp_p - (ewm * coftz) * rwp);
   if (k >= 1 && k < L) wwa = wwa + ewm * rwp;
-  const float *zzr = ROW(I.z.zz, c);
-  const float zk = zzr[k];
-  const float etp = I.z.etp[k], etm = I.z.etm[k];
   const float ts_m = up1(ts), rs_m = up1(rs), rtpp_m = up1(rtpp), rpp_m = up1(rpp), zk_m = up1(zk);
-  const float cofwt = ROW(I.b.cofwt, c)[k];
   const float cofwt_m = up1(cofwt);
-  const float etp_m = I.z.etp[cl(k - 1)], etm_m = I.z.etm[cl(k - 1)];
+  const float etp_m = lane_off(etp, k, -1), etm_m = lane_off(etm, k, -1);
+  const float crho_m = up1(crho);
   float col = rwp;
   if (k >= 1 && k < L) {
-    float x = ROW(I.b.tend_rw, c)[k] -
-              ROW(I.b.cofwz, c)[k] * (((etp * zk) * ts - (etp_m * zk_m) * ts_m) +
-                                      ((etm * zk) * rtpp - (etm_m * zk_m) * rtpp_m));
-    x = x - ROW(I.b.cofwr, c)[k] * ((etp * rs + etp_m * rs_m) + (etm * rpp + etm_m * rpp_m));
+    float x = trw - cofwz * (((etp * zk) * ts - (etp_m * zk_m) * ts_m) + ((etm * zk) * rtpp - (etm_m * zk_m) * rtpp_m));
+    x = x - cofwr * ((etp * rs + etp_m * rs_m) + (etm * rpp + etm_m * rpp_m));
     x = x + cofwt * (etp * ts + etm * rtpp);
     x = x + cofwt_m * (etp_m * ts_m + etm_m * rtpp_m);
     col = col + dts * x;
   }
   const float dts2 = dts * dts;
-  const float a = ROW(I.b.atri, c)[k], al = ROW(I.b.alpha, c)[k], ga = ROW(I.b.gamma, c)[k];
   const float ad = dts2 * a;
   // Thomas sweeps (acoustic_v841.py:296-565), lane = cell: forward for
   // levels 1..L-1, back substitution for levels L-1..0
@@ -1034,16 +1127,13 @@ KBCOL k_ac_cell(const Instance I, float dts, int first) {
   __syncthreads();
   col = s_x[w * COLP + k];
   if (k >= 1 && k < L) {
-    const float delta = ROW(I.b.s_rw, c)[k] - ROW(I.b.c_rw, c)[k];
-    const float damp = ROW(I.z.dss, c)[k];
-    const float zi = I.z.fzm[k] * zk + I.z.fzp[k] * zk_m;
-    const float crho = ROW(I.b.c_rho, c)[k];
-    const float ri = I.z.fzm[k] * crho + I.z.fzp[k] * ROW(I.b.c_rho, c)[k - 1];
-    col = (col + delta - (((dts * damp) * zi) * ri) * ROW(I.b.w, c)[k]) / (1.0f + dts * damp) - delta;
+    const float delta = srw - crw;
+    const float zi = fzm * zk + fzp * zk_m;
+    const float ri = fzm * crho + fzp * crho_m;
+    col = (col + delta - (((dts * damp) * zi) * ri) * wk) / (1.0f + dts * damp) - delta;
   }
-  if (k >= 1 && k < L) wwa = wwa + I.z.ewp[k] * col;
+  if (k >= 1 && k < L) wwa = wwa + ewp * col;
   const float col_p = dn1(col);
-  const float ewp = I.z.ewp[k], ewp_p = I.z.ewp[kp];
   const float rpp_new = rs - (dts * rdzw) * (ewp_p * col_p - ewp * col);
   const float rtpp_new = ts - (dts * rdzw) * ((ewp_p * coftz_p) * col_p - (ewp * coftz) * col);
   if (live && k <= L) {
@@ -1073,9 +1163,9 @@ template <int NE, class RU>
 __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int k, float rho_k, float rw_k, RU ru_row,
                                                   float *hdiv_out = nullptr) {
   const int L = I.n.L;
-  const float *zzr = ROW(I.z.zz, c);
+  const float zk = ROW(I.z.zz, c)[k];
+  const float zkm = lane_off(zk, k, -1);
   const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
-  const int km = cl(k - 1);
   const int n = I.m.neoc[c];
   float sg[NE];
   slots(sg, I.m.csign + c * NE);
@@ -1101,7 +1191,7 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
     *hdiv_out = acc * I.m.invarea[c];
   }
   float col = 0.0f;
-  if (k >= 1 && k < L) col = rw_k / (fzm * zzr[k] + fzp * zzr[km]);
+  if (k >= 1 && k < L) col = rw_k / (fzm * zk + fzp * zkm);
   const float cf1 = I.z.cf1, cf2 = I.z.cf2, cf3 = I.z.cf3;
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
@@ -1130,59 +1220,78 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
   return col;
 }
 
-template <int NE>
-KB k_rec_cell(const Instance I, int stage, int sub, float inv_steps) {
+// S3: the stage-3 recovery (exner, p' and the split-transport sums).  The
+// loads and the terrain w recovery come first, branch-free, so every gather
+// of the pass is in flight together; stores come last.
+template <int NE, bool S3>
+KB k_rec_cell(const Instance I, int sub, float inv_steps) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
-  float rho = 1.0f, rtv = 1.0f;
-  int bad = 0;
-  if (k < L) {
-    const float dp = ROW(I.b.s_rho_p, c)[k] + ROW(I.b.rho_pp, c)[k];
-    const float rp = ROW(I.b.s_rtheta_p, c)[k] + ROW(I.b.rtheta_pp, c)[k];
-    ROW(I.b.rho_p, c)[k] = dp;
-    ROW(I.b.rtheta_p, c)[k] = rp;
-    rho = dp + ROW(I.z.rb, c)[k];
-    rtv = rp + ROW(I.z.rtb, c)[k];
-    if (!(rho > 0.0f) || !(rtv > 0.0f)) bad = 1;
-    ROW(I.b.c_rho, c)[k] = rho;
-    ROW(I.b.c_rt, c)[k] = rtv;
-    ROW(I.b.theta, c)[k] = rtv / rho;
-    if (stage == 3) {
-      const float zk = ROW(I.z.zz, c)[k];
-      const float arg = (zk * I.k.cfac) * rtv;
-      if (!(arg > 0.0f)) bad = 1;
-      const float ex = hexdyn_powf_eval(arg, I.k.rcv);
-      ROW(I.b.exner, c)[k] = ex;
-      ROW(I.b.pp, c)[k] = (zk * I.k.rgas) * (ex * rp + ROW(I.z.rtb, c)[k] * (ex - ROW(I.z.exb, c)[k]));
-    }
+  const bool lay = k < L, itf = k <= L, inner = k >= 1 && k < L;
+  const float srp = ROW(I.b.s_rho_p, c)[k], rpp = ROW(I.b.rho_pp, c)[k];
+  const float srtp = ROW(I.b.s_rtheta_p, c)[k], rtpp = ROW(I.b.rtheta_pp, c)[k];
+  const float rb = ROW(I.z.rb, c)[k], rtb = ROW(I.z.rtb, c)[k];
+  const float srw = ROW(I.b.s_rw, c)[k], rwp = ROW(I.b.rw_p, c)[k], wwa = ROW(I.b.ww_avg, c)[k];
+  float zk = 0.0f, exb = 0.0f, spw = 0.0f;
+  if (S3) {
+    zk = ROW(I.z.zz, c)[k];
+    exb = ROW(I.z.exb, c)[k];
+    spw = ROW(I.b.split_w, c)[k];
   }
-  float rwv = 0.0f;
-  if (k >= 1 && k < L) rwv = ROW(I.b.s_rw, c)[k] + ROW(I.b.rw_p, c)[k];
-  if (k <= L) {
-    ROW(I.b.c_rw, c)[k] = rwv;
-    const float fw = ROW(I.b.s_rw, c)[k] + ROW(I.b.ww_avg, c)[k] * inv_steps;
-    if (I.k.store_flux) ROW(I.b.flux_w, c)[k] = fw;
-    if (stage == 3) {
-      // split-transport sums: copy, then accumulate; the last subcycle also
-      // applies the 1/splits average (integration.py:27-82)
-      const float acc = sub == 1 ? fw : fw + ROW(I.b.split_w, c)[k];
-      ROW(I.b.split_w, c)[k] = sub == I.k.splits ? acc * I.k.split_inv : acc;
-    }
-  }
+  const float dp = lay ? srp + rpp : 0.0f;
+  const float rp = lay ? srtp + rtpp : 0.0f;
+  const float rho = lay ? dp + rb : 1.0f;
+  const float rtv = lay ? rp + rtb : 1.0f;
+  const float rwv = inner ? srw + rwp : 0.0f;
   const float *cru = I.b.c_ru;
   float hd = 0.0f;
   const float wv = recover_w_column<NE>(I, c, k, rho, rwv, [&](int e) { return ROW(cru, e)[k]; }, &hd);
-  if (k <= L) ROW(I.b.w, c)[k] = wv;
-  if (k < L) ROW(I.b.hdiv, c)[k] = hd;
+  int bad = (lay && (!(rho > 0.0f) || !(rtv > 0.0f))) ? 1 : 0;
+  const float th = rtv / rho;
+  float ex = 0.0f, ppv = 0.0f;
+  if (S3 && lay) {
+    const float arg = (zk * I.k.cfac) * rtv;
+    if (!(arg > 0.0f)) bad = 1;
+    ex = hexdyn_powf_eval(arg, I.k.rcv);
+    ppv = (zk * I.k.rgas) * (ex * rp + rtb * (ex - exb));
+  }
+  const float fw = srw + wwa * inv_steps;
+  float split = 0.0f;
+  if (S3) {
+    // split-transport sums: copy, then accumulate; the last subcycle also
+    // applies the 1/splits average (integration.py:27-82)
+    const float acc = sub == 1 ? fw : fw + spw;
+    split = sub == I.k.splits ? acc * I.k.split_inv : acc;
+  }
+  if (lay) {
+    ROW(I.b.rho_p, c)[k] = dp;
+    ROW(I.b.rtheta_p, c)[k] = rp;
+    ROW(I.b.c_rho, c)[k] = rho;
+    ROW(I.b.c_rt, c)[k] = rtv;
+    ROW(I.b.theta, c)[k] = th;
+    if (S3) {
+      ROW(I.b.exner, c)[k] = ex;
+      ROW(I.b.pp, c)[k] = ppv;
+    }
+    ROW(I.b.hdiv, c)[k] = hd;
+  }
+  if (itf) {
+    ROW(I.b.c_rw, c)[k] = rwv;
+    if (I.k.store_flux) ROW(I.b.flux_w, c)[k] = fw;
+    if (S3) ROW(I.b.split_w, c)[k] = split;
+    ROW(I.b.w, c)[k] = wv;
+  }
   if (bad) set_flag(I.b.flag);
 }
 
-// damp: apply the last substep's divergence damping to ru_p first
-// avg_from_rp: one substep and ru_avg not stored, so the average is the
-// undamped ru_p of that substep
-KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps, int damp, float damp_coef, int avg_from_rp) {
+// DAMP: apply the last substep's divergence damping to ru_p first.
+// AVGRP: one substep and ru_avg not stored, so the average is the undamped
+// ru_p of that substep.  S3: accumulate the split-transport sums.  Loads
+// first, stores last.
+template <bool S3, bool DAMP, bool AVGRP>
+KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
   const int k = lane();
@@ -1190,22 +1299,24 @@ KB k_rec_edge(const Instance I, int stage, int sub, float inv_steps, int damp, f
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const float sru = ROW(I.b.s_ru, e)[k];
   const float rp_raw = ROW(I.b.ru_p, e)[k];
-  float rp = rp_raw;
-  if (damp) {
-    rp = damped_ru_p(I, e, k, rp, damp_coef);
-    ROW(I.b.ru_p, e)[k] = rp;
-  }
-  const float ru = sru + rp;
-  ROW(I.b.c_ru, e)[k] = ru;
-  const float fu = sru + (avg_from_rp ? rp_raw : ROW(I.b.ru_avg, e)[k]) * inv_steps;
-  if (I.k.store_flux) ROW(I.b.flux_u, e)[k] = fu;
-  if (stage == 3) {
-    const float acc = sub == 1 ? fu : fu + ROW(I.b.split_u, e)[k];
-    ROW(I.b.split_u, e)[k] = sub == I.k.splits ? acc * I.k.split_inv : acc;
-  }
+  const float avg = AVGRP ? rp_raw : ROW(I.b.ru_avg, e)[k];
+  const float spu = S3 ? ROW(I.b.split_u, e)[k] : 0.0f;
   const float r0 = (ROW(I.b.s_rho_p, c0)[k] + ROW(I.b.rho_pp, c0)[k]) + ROW(I.z.rb, c0)[k];
   const float r1 = (ROW(I.b.s_rho_p, c1)[k] + ROW(I.b.rho_pp, c1)[k]) + ROW(I.z.rb, c1)[k];
-  ROW(I.b.u, e)[k] = (2.0f * ru) / (r0 + r1);
+  const float rp = DAMP ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
+  const float ru = sru + rp;
+  const float fu = sru + avg * inv_steps;
+  float split = 0.0f;
+  if (S3) {
+    const float acc = sub == 1 ? fu : fu + spu;
+    split = sub == I.k.splits ? acc * I.k.split_inv : acc;
+  }
+  const float uu = (2.0f * ru) / (r0 + r1);
+  if (DAMP) ROW(I.b.ru_p, e)[k] = rp;
+  ROW(I.b.c_ru, e)[k] = ru;
+  if (I.k.store_flux) ROW(I.b.flux_u, e)[k] = fu;
+  if (S3) ROW(I.b.split_u, e)[k] = split;
+  ROW(I.b.u, e)[k] = uu;
 }
 
 // rebuild of the saved diagnostics from a prognostic state (driver.py:2655-2700)
@@ -1244,9 +1355,10 @@ KB k_rebuild_edge(const Instance I) {
 }
 
 // ------------------------------------------------------------ scalar transport
-__device__ __forceinline__ float vflux(const float *qc, float wk, int k, int L, float fzm, float fzp, float coef) {
-  if (k == 1 || k == L - 1) return wk * (fzm * qc[k] + fzp * qc[k - 1]);
-  const float q0 = qc[k - 2], q1 = qc[k - 1], q2 = qc[k], q3 = qc[k + 1];
+// q0..q3: the column's values at levels k-2, k-1, k, k+1
+__device__ __forceinline__ float vflux(float q0, float q1, float q2, float q3, float wk, int k, int L, float fzm,
+                                       float fzp, float coef) {
+  if (k == 1 || k == L - 1) return wk * (fzm * q2 + fzp * q1);
   const float base = div12(wk * (7.0f * (q2 + q1) - (q3 + q0)));
   const float corr = div12((coef * fabsf(wk)) * ((q3 - q0) - 3.0f * (q2 - q1)));
   return base + corr;
@@ -1318,14 +1430,25 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
   const float orho = ROW(I.b.rho, c)[k];
   const float target = (1.0f - wn) * orho + wn * ROW(I.b.c_rho, c)[k];
   const float fzm = I.z.fzm[k], fzp = I.z.fzp[k], rdzw = I.z.rdzw[k], ia = I.m.invarea[c];
-  auto species = [&](int sp) {
-    const float *q = ROW(qs + sp * cplane, c);
-    float evv[NE];
+  // software-pipelined over species: the next species' rows are requested
+  // before the current one is combined (clamped index, in bounds)
+  struct Sp {
+    float qk, qo, evv[NE];
+  };
+  auto load_sp = [&](int sp) {
+    Sp a;
+    a.qk = ROW(qs + sp * cplane, c)[k];
 #pragma unroll
-    for (int s = 0; s < NE; ++s) evv[s] = ROW(ev + sp * eplane, e[s])[k];
-    const float qo = ROW(I.b.q + sp * cplane, c)[k];
+    for (int s = 0; s < NE; ++s) a.evv[s] = ROW(ev + sp * eplane, e[s])[k];
+    a.qo = ROW(I.b.q + sp * cplane, c)[k];
+    return a;
+  };
+  auto species = [&](int sp, const Sp &a) {
+    const float qk = a.qk, qo = a.qo;
+    const float *evv = a.evv;
+    const float q_m2 = lane_off(qk, k, -2), q_m1 = lane_off(qk, k, -1), q_p1 = lane_off(qk, k, 1);
     float vf = 0.0f;
-    if (k >= 1 && k < L) vf = vflux(q, sw, k, L, fzm, fzp, I.k.coef3);
+    if (k >= 1 && k < L) vf = vflux(q_m2, q_m1, qk, q_p1, sw, k, L, fzm, fzp, I.k.coef3);
     const float vf_p = dn1(vf);
     float tc = 0.0f;
 #pragma unroll
@@ -1337,7 +1460,12 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
       ROW(out + sp * cplane, c)[k] = num * (1.0f / target);
     }
   };
-  for (int sp = 0; sp < NT; ++sp) species(sp);
+  Sp cur = load_sp(0);
+  for (int sp = 0; sp < NT; ++sp) {
+    const Sp nxt = load_sp(sp + 1 < NT ? sp + 1 : sp);
+    species(sp, cur);
+    cur = nxt;
+  }
 }
 
 template <int NE>
@@ -1458,7 +1586,8 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
   const float ia = I.m.invarea[c];
   const float target = ROW(I.b.target, c)[k];
   for (int sp = 0; sp < I.n.NT; ++sp) {
-    const float *qsr = ROW(qs + sp * cplane, c);
+    const float qs_k = ROW(qs + sp * cplane, c)[k];
+    const float qs_m2 = lane_off(qs_k, k, -2), qs_m1 = lane_off(qs_k, k, -1), qs_p1 = lane_off(qs_k, k, 1);
     const float *qold = I.b.q + sp * cplane;
     float qn[NE], upw[NE], hres[NE];
 #pragma unroll
@@ -1491,7 +1620,7 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
     float vres = 0.0f;
     float fcol = 0.0f;
     if (k >= 1 && k < L) {
-      vres = dt * vflux(qsr, wk, k, L, fzm, fzp, I.k.coef3);
+      vres = dt * vflux(qs_m2, qs_m1, qs_k, qs_p1, wk, k, L, fzm, fzp, I.k.coef3);
       fcol = dt * (max0(wk) * qo_m + min0(wk) * qo);
     } else {
       vres = dt * 0.0f;
@@ -1628,10 +1757,21 @@ inline int check() {
 int solve_diagnostics(const Instance &I, int rk, hipStream_t s) {
   launch(k_diag_vertex, I.n.V, 1, s, I, (const float *)I.b.u);
   LAUNCH_NE(I, k_diag_cell, I.n.C, 1, s, I, (const float *)I.b.u);
-  if (I.k.fused)  // diagnostics after stage rk feed stage rk % 3 + 1 (the next subcycle's first after rk 3)
-    LAUNCH_NE(I, k_diag_edge_flux, I.n.E, 1, s, I, (const float *)I.b.u, rk == 3 ? 1 : 0, rk % 3 + 1);
-  else
-    LAUNCH_NE(I, k_diag_edge, I.n.E, 1, s, I, (const float *)I.b.u, rk == 3 ? 1 : 0);
+  const float *u = I.b.u;
+  const bool six = I.n.ME == 6;
+  if (I.k.fused) {  // diagnostics after stage rk feed stage rk % 3 + 1 (the next subcycle's first after rk 3)
+    if (rk == 3)
+      launch_named("k_diag_edge_flux", six ? k_diag_edge_flux<6, true> : k_diag_edge_flux<7, true>, I.n.E, 1, s, I,
+                   u);
+    else
+      launch_named("k_diag_edge_flux", six ? k_diag_edge_flux<6, false> : k_diag_edge_flux<7, false>, I.n.E, 1, s,
+                   I, u);
+  } else {
+    if (rk == 3)
+      launch_named("k_diag_edge", six ? k_diag_edge<6, true> : k_diag_edge<7, true>, I.n.E, 1, s, I, u);
+    else
+      launch_named("k_diag_edge", six ? k_diag_edge<6, false> : k_diag_edge<7, false>, I.n.E, 1, s, I, u);
+  }
   return check();
 }
 
@@ -1672,7 +1812,12 @@ int dyn_tend(const Instance &I, int stage, hipStream_t s) {
     LAUNCH_NE(I, k_tend_mom_ac, I.n.E, 1, s, I, I.k.dts[stage - 1], I.k.steps[stage - 1] > 2 ? 1 : 0);
   } else {
     LAUNCH_NE(I, k_tend_mom, I.n.E, 1, s, I);
-    LAUNCH_NE(I, k_tend_flux, I.n.E, 1, s, I, stage);
+    if (stage > 1)
+      NEDISPATCH(I, launch_named("k_tend_flux", k_tend_flux<6, true>, I.n.E, 1, s, I),
+                 launch_named("k_tend_flux", k_tend_flux<7, true>, I.n.E, 1, s, I));
+    else
+      NEDISPATCH(I, launch_named("k_tend_flux", k_tend_flux<6, false>, I.n.E, 1, s, I),
+                 launch_named("k_tend_flux", k_tend_flux<7, false>, I.n.E, 1, s, I));
   }
   LAUNCH_NE(I, k_tend_cell, I.n.C, 1, s, I, stage);
   return check();
@@ -1684,8 +1829,12 @@ int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStr
   if (!(I.k.fused && ss == 1))  // fused: the first edge update ran with the momentum tendency
     launch(k_ac_edge, I.n.E, 1, s, I, dts, ss == 1 ? 1 : 0, damp_prev, I.k.dts_inv_coef[stage - 1],
            (I.k.fused && n == 2) ? 1 : 0);
-  NEDISPATCH(I, launch_col_named("k_ac_cell", k_ac_cell<6>, I.n.C, s, I, dts, ss == 1 ? 1 : 0),
-             launch_col_named("k_ac_cell", k_ac_cell<7>, I.n.C, s, I, dts, ss == 1 ? 1 : 0));
+  if (ss == 1)
+    NEDISPATCH(I, launch_col_named("k_ac_cell", k_ac_cell<6, true>, I.n.C, s, I, dts),
+               launch_col_named("k_ac_cell", k_ac_cell<7, true>, I.n.C, s, I, dts));
+  else
+    NEDISPATCH(I, launch_col_named("k_ac_cell", k_ac_cell<6, false>, I.n.C, s, I, dts),
+               launch_col_named("k_ac_cell", k_ac_cell<7, false>, I.n.C, s, I, dts));
   return check();
 }
 
@@ -1694,11 +1843,31 @@ int divergence_damping(const Instance &I, int stage, hipStream_t s) {
   return check();
 }
 
+template <bool S3>
+void recover_edges(const Instance &I, int sub, float inv, bool damp, bool avgrp, float coef, hipStream_t s) {
+  if (damp && avgrp)
+    launch_named("k_rec_edge", k_rec_edge<S3, true, true>, I.n.E, 1, s, I, sub, inv, coef);
+  else if (damp)
+    launch_named("k_rec_edge", k_rec_edge<S3, true, false>, I.n.E, 1, s, I, sub, inv, coef);
+  else if (avgrp)
+    launch_named("k_rec_edge", k_rec_edge<S3, false, true>, I.n.E, 1, s, I, sub, inv, coef);
+  else
+    launch_named("k_rec_edge", k_rec_edge<S3, false, false>, I.n.E, 1, s, I, sub, inv, coef);
+}
+
 int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
   const float inv = I.k.inv_steps[stage - 1];
-  launch(k_rec_edge, I.n.E, 1, s, I, stage, sub, inv, damp, I.k.dts_inv_coef[stage - 1],
-         (I.k.fused && I.k.steps[stage - 1] == 1) ? 1 : 0);
-  LAUNCH_NE(I, k_rec_cell, I.n.C, 1, s, I, stage, sub, inv);
+  const bool avgrp = I.k.fused && I.k.steps[stage - 1] == 1;
+  const float coef = I.k.dts_inv_coef[stage - 1];
+  if (stage == 3) {
+    recover_edges<true>(I, sub, inv, damp != 0, avgrp, coef, s);
+    NEDISPATCH(I, launch_named("k_rec_cell", k_rec_cell<6, true>, I.n.C, 1, s, I, sub, inv),
+               launch_named("k_rec_cell", k_rec_cell<7, true>, I.n.C, 1, s, I, sub, inv));
+  } else {
+    recover_edges<false>(I, sub, inv, damp != 0, avgrp, coef, s);
+    NEDISPATCH(I, launch_named("k_rec_cell", k_rec_cell<6, false>, I.n.C, 1, s, I, sub, inv),
+               launch_named("k_rec_cell", k_rec_cell<7, false>, I.n.C, 1, s, I, sub, inv));
+  }
   return check();
 }
 
