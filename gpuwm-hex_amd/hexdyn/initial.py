@@ -54,8 +54,31 @@ class VerticalGrid:
     cf2: float
     cf3: float
     zb_cell: np.ndarray     # (L+1, C, maxEdges)
-    zb3_cell: np.ndarray    # (L+1, C, maxEdges)
+    zb3_cell: np.ndarray    # (L+1, C, maxEdges), scaled by coef_3rd_order
     dzw: np.ndarray         # (L,) binary64 layer depths
+    zb_edge: Optional[np.ndarray] = None   # (L+1, 2, E) MPAS zb as an init file holds it
+    zb3_edge: Optional[np.ndarray] = None  # (L+1, 2, E) MPAS zb3, unscaled
+    dzu: Optional[np.ndarray] = None       # (L,) binary32, index 0 unused
+
+
+def terrain_slot_metrics(mesh: Mesh, zb_edge: np.ndarray, zb3_edge: np.ndarray, coef_3rd_order: float):
+    """Per-cell-slot zb, zb3 from MPAS's per-edge (level, side, edge) arrays.
+
+    Slot s of cell c takes side 0 of its edge when c is the edge's first cell,
+    side 1 otherwise; zb3 is scaled by coef_3rd_order once, in binary32, as the
+    model does when it reads an init file (MPAS core.F, atm_mpas_init_block).
+    """
+    nlev1 = zb_edge.shape[0]
+    zb_cell = np.zeros((nlev1, mesh.n_cells, mesh.max_edges), dtype=F32)
+    zb3_cell = np.zeros_like(zb_cell)
+    coef = F32(coef_3rd_order)
+    for s in range(mesh.max_edges):
+        cells = np.flatnonzero(mesh.nEdgesOnCell > s)
+        e = mesh.edgesOnCell[cells, s]
+        side = np.where(mesh.cellsOnEdge[e, 0] == cells, 0, 1)
+        zb_cell[:, cells, s] = zb_edge[:, side, e]
+        zb3_cell[:, cells, s] = coef * zb3_edge[:, side, e].astype(F32)
+    return zb_cell, zb3_cell
 
 
 @dataclass
@@ -159,20 +182,13 @@ def build_vertical_grid(mesh: Mesh, terrain: np.ndarray, *, nlev: int = 55, ztop
     zb[:, 1, :] = (z_edge - zgrid[:, c2]) * mesh.dvEdge / mesh.areaCell[c2]
     zb3[:, 0, :] = z_edge3 * mesh.dvEdge / mesh.areaCell[c1]
     zb3[:, 1, :] = z_edge3 * mesh.dvEdge / mesh.areaCell[c2]
-    max_edges = mesh.max_edges
-    zb_cell = np.zeros((nlev + 1, mesh.n_cells, max_edges))
-    zb3_cell = np.zeros_like(zb_cell)
-    for s in range(max_edges):
-        cells = np.flatnonzero(mesh.nEdgesOnCell > s)
-        e = mesh.edgesOnCell[cells, s]
-        side = np.where(mesh.cellsOnEdge[e, 0] == cells, 0, 1)
-        zb_cell[:, cells, s] = zb[:, side, e]
-        zb3_cell[:, cells, s] = coef_3rd_order * zb3[:, side, e]
+    zb_edge, zb3_edge = zb.astype(F32), zb3.astype(F32)
+    zb_cell, zb3_cell = terrain_slot_metrics(mesh, zb_edge, zb3_edge, coef_3rd_order)
     return VerticalGrid(
         zw=zw, zgrid=zgrid.astype(F32), zz=zz.astype(F32), zxu=zxu.astype(F32),
         rdzw=(1.0 / dzw).astype(F32), rdzu=rdzu.astype(F32), fzm=fzm.astype(F32), fzp=fzp.astype(F32),
         cf1=float(F32(cf1)), cf2=float(F32(cf2)), cf3=float(F32(cf3)),
-        zb_cell=zb_cell.astype(F32), zb3_cell=zb3_cell.astype(F32), dzw=dzw,
+        zb_cell=zb_cell, zb3_cell=zb3_cell, dzw=dzw, zb_edge=zb_edge, zb3_edge=zb3_edge, dzu=dzu.astype(F32),
     )
 
 

@@ -111,6 +111,11 @@ def write_mpas_grid(mesh: Mesh, path, *, unit_sphere: bool = False) -> None:
     """Write the mesh tables as an MPAS grid file (64-bit offset netCDF)."""
     from scipy.io import netcdf_file
 
+    with netcdf_file(str(path), "w", version=2) as f:
+        _write_mesh(f, mesh, unit_sphere=unit_sphere)
+
+
+def _write_mesh(f, mesh: Mesh, *, unit_sphere: bool = False) -> None:
     a = mesh.arrays
     R = mesh.sphere_radius
     dims = {"nCells": mesh.n_cells, "nEdges": mesh.n_edges, "nVertices": mesh.n_vertices,
@@ -126,41 +131,38 @@ def write_mpas_grid(mesh: Mesh, path, *, unit_sphere: bool = False) -> None:
         "edgesOnEdge": "edgeME2", "weightsOnEdge": "edgeME2", "nEdgesOnCell": "cell", "nEdgesOnEdge": "edge",
         "kiteAreasOnVertex": "vertex3",
     }
-    with netcdf_file(str(path), "w", version=2) as f:
-        f.on_a_sphere = "YES"
-        f.sphere_radius = 1.0 if unit_sphere else float(R)
-        for name, n in dims.items():
-            f.createDimension(name, n)
-        f.createDimension("TWO", 2)
-        for name, arr in a.items():
-            if name in ("nominalMinDc",):
-                continue
-            if name in layout:
-                d = shape_dims[layout[name]]
-            elif arr.ndim == 1 and arr.size == mesh.n_cells:
-                d = ("nCells",)
-            elif arr.ndim == 1 and arr.size == mesh.n_edges:
-                d = ("nEdges",)
-            elif arr.ndim == 1 and arr.size == mesh.n_vertices:
-                d = ("nVertices",)
-            elif name == "kiteForCell":
-                continue
-            else:
-                continue
-            if name in CONNECTIVITY:
-                v = f.createVariable(name, "i4", d)
-                v[:] = np.where(arr >= 0, arr + 1, 0).astype(np.int32)
-            elif name in COUNTS:
-                v = f.createVariable(name, "i4", d)
-                v[:] = arr.astype(np.int32)
-            else:
-                x = np.asarray(arr, dtype=np.float64)
-                if unit_sphere and name in LENGTHS:
-                    x = x / R
-                elif unit_sphere and name in AREAS:
-                    x = x / (R * R)
-                v = f.createVariable(name, "f8", d)
-                v[:] = x
+    f.on_a_sphere = "YES"
+    f.sphere_radius = 1.0 if unit_sphere else float(R)
+    for name, n in dims.items():
+        f.createDimension(name, n)
+    f.createDimension("TWO", 2)
+    for name, arr in a.items():
+        if name in ("nominalMinDc",):
+            continue
+        if name in layout:
+            d = shape_dims[layout[name]]
+        elif arr.ndim == 1 and arr.size == mesh.n_cells:
+            d = ("nCells",)
+        elif arr.ndim == 1 and arr.size == mesh.n_edges:
+            d = ("nEdges",)
+        elif arr.ndim == 1 and arr.size == mesh.n_vertices:
+            d = ("nVertices",)
+        else:
+            continue
+        if name in CONNECTIVITY:
+            v = f.createVariable(name, "i4", d)
+            v[:] = np.where(arr >= 0, arr + 1, 0).astype(np.int32)
+        elif name in COUNTS:
+            v = f.createVariable(name, "i4", d)
+            v[:] = arr.astype(np.int32)
+        else:
+            x = np.asarray(arr, dtype=np.float64)
+            if unit_sphere and name in LENGTHS:
+                x = x / R
+            elif unit_sphere and name in AREAS:
+                x = x / (R * R)
+            v = f.createVariable(name, "f8", d)
+            v[:] = x
 
 
 # ------------------------------------------------------------------ restart
@@ -229,3 +231,196 @@ def read_restart(path) -> Tuple[Dict[str, np.ndarray], Dict[str, object]]:
     if "xtime" in arrays:
         meta["xtime"] = b"".join(arrays["xtime"].tolist()).decode().strip()
     return out, meta
+
+
+# ------------------------------------------------------------------ init files
+#: MPAS names of the scalar slots, in the dycore's order (qv first: the moist
+#: coefficients sum the leading moisture species)
+SCALAR_NAMES = ("qv", "qc", "qr", "qi", "qs", "qg")
+#: vertical metrics an initialized file must carry (driver.py:1456-1459)
+VERTICAL_REQUIRED = ("zgrid", "rdzw", "dzu", "rdzu", "fzm", "fzp", "zz", "zxu", "dss", "zb", "zb3",
+                     "cf1", "cf2", "cf3")
+
+
+def write_mpas_init(case, path, *, unit_sphere: bool = False) -> None:
+    """Write a case as an MPAS-style initialized file (mesh + vertical metrics +
+    reference + state), 64-bit offset netCDF.
+
+    The MPAS variables of an init file are written under their MPAS names and
+    dimensions: ``zgrid (nCells, nVertLevelsP1)``, ``zz``/``dss (nCells,
+    nVertLevels)``, ``zxu (nEdges, nVertLevels)``, 1-D ``rdzw dzu rdzu fzm fzp``,
+    ``zb``/``zb3 (nEdges, TWO, nVertLevelsP1)`` (zb3 unscaled), scalars
+    ``cf1 cf2 cf3``, ``rho_base theta_base rho_zz theta_m u w`` and one variable
+    per scalar species, with a leading ``Time`` record.  The momenta ``ru``,
+    ``rw`` and the binary32 ``rtheta_m``, ``rtheta_base``, ``exner_base`` are
+    added as a model dump carries them, so a file read back reproduces the
+    case bit for bit.
+    """
+    from scipy.io import netcdf_file
+
+    g, ref, st, m = case.grid, case.reference, case.state, case.mesh
+    if g.zb_edge is None or g.zb3_edge is None or g.dzu is None:
+        raise MpasFileError("the case's vertical grid carries no per-edge zb/zb3 (rebuild the case)")
+    L = case.n_levels
+    c1, c2 = m.cellsOnEdge[:, 0], m.cellsOnEdge[:, 1]
+    rho = st.rho_zz.astype(np.float32)
+    u = (st.ru / (np.float32(0.5) * (rho[:, c1] + rho[:, c2]))).astype(np.float32)
+    with netcdf_file(str(path), "w", version=2) as f:
+        f.createDimension("Time", None)  # the record dimension comes first in this format
+        _write_mesh(f, m, unit_sphere=unit_sphere)
+        f.createDimension("nVertLevels", L)
+        f.createDimension("nVertLevelsP1", L + 1)
+
+        def var(name, dims, data, kind="f4"):
+            v = f.createVariable(name, kind, dims)
+            if dims and dims[0] == "Time":
+                v[0] = data
+            elif dims:
+                v[:] = data
+            else:
+                v[...] = data  # a 0-d variable
+
+        var("zgrid", ("nCells", "nVertLevelsP1"), g.zgrid.T)
+        var("zz", ("nCells", "nVertLevels"), g.zz.T)
+        var("zxu", ("nEdges", "nVertLevels"), g.zxu.T)
+        var("dss", ("nCells", "nVertLevels"), np.asarray(case.dss, dtype=np.float32).T)
+        for name, a in (("rdzw", g.rdzw), ("dzu", g.dzu), ("rdzu", g.rdzu), ("fzm", g.fzm), ("fzp", g.fzp),
+                        ("u_init", case.u_init), ("v_init", case.v_init)):
+            var(name, ("nVertLevels",), np.asarray(a, dtype=np.float32))
+        var("zb", ("nEdges", "TWO", "nVertLevelsP1"), np.transpose(g.zb_edge, (2, 1, 0)))
+        var("zb3", ("nEdges", "TWO", "nVertLevelsP1"), np.transpose(g.zb3_edge, (2, 1, 0)))
+        for name, x in (("cf1", g.cf1), ("cf2", g.cf2), ("cf3", g.cf3)):
+            var(name, (), np.float32(x))
+        cl = ("Time", "nCells", "nVertLevels")
+        var("rho_base", cl, ref.rho_base.T)
+        var("theta_base", cl, ref.theta_base.T)
+        var("rtheta_base", cl, ref.rtheta_base.T)
+        var("exner_base", cl, ref.exner_base.T)
+        var("rho_zz", cl, rho.T)
+        var("rtheta_m", cl, st.rtheta_m.T)
+        var("theta_m", cl, (st.rtheta_m / rho).astype(np.float32).T)
+        var("ru", ("Time", "nEdges", "nVertLevels"), st.ru.T)
+        var("u", ("Time", "nEdges", "nVertLevels"), u.T)
+        var("rw", ("Time", "nCells", "nVertLevelsP1"), st.rw.T)
+        for i in range(case.n_scalars):
+            var(SCALAR_NAMES[i] if i < len(SCALAR_NAMES) else f"scalar{i}", cl, st.scalars[i].T)
+        f.moist = 1 if case.n_moist > 0 else 0
+        f.config_dt = float(case.config.config_dt)
+
+
+def _levels_first(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a).T)
+
+
+def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=None, name: str = "mpas"):
+    """A dycore case from MPAS files: mesh (grid file, or the init file's own
+    mesh variables), vertical metrics (the invariant file when given, the
+    reference's authority order, driver.py:1418-1455), reference state and
+    prognostic state (driver.py:1583-1700).
+
+    The per-edge ``zb``/``zb3`` become per-cell-slot metrics with zb3 scaled
+    by ``config_coef_3rd_order`` in binary32.  ``rho_zz`` and ``theta_m`` are
+    used as stored (``rtheta_m`` when the file carries it, else
+    ``rho_zz * theta_m`` in binary32); ``ru``/``rw`` are read when present,
+    else ``ru = 0.5 (rho_zz[c1] + rho_zz[c2]) u``; a file without ``rw`` is
+    refused (the terrain-following w -> rw conversion is not built).
+    ``rtheta_base``/``exner_base`` are read when present, else formed as
+    ``rho_base * theta_base`` and ``(zz R_d/p0 rtheta_base)^(R_d/c_v)``
+    (binary64, rounded once: parity unpinned against MPAS's powf).
+    """
+    from .config import DycoreConfig
+    from .case import Case, _len_disp
+    from . import initial as hinit
+    from . import setup as hsetup
+
+    init_path = Path(init_path)
+    mesh = mesh_from_mpas(grid_path if grid_path is not None else init_path)
+    arrays, attrs = _read(init_path)
+    vert = dict(arrays)
+    if invariant_path is not None:
+        inv, _ = _read(Path(invariant_path))
+        vert.update({k: v for k, v in inv.items() if k in VERTICAL_REQUIRED})
+    missing = [n for n in VERTICAL_REQUIRED if n not in vert]
+    if missing:
+        raise MpasFileError(f"native vertical metrics incomplete: missing {missing}")
+    cfg = config or DycoreConfig(config_dt=float(attrs.get("config_dt", 720.0)))
+    f32 = np.float32
+    zgrid = _levels_first(vert["zgrid"]).astype(f32)
+    zz = _levels_first(vert["zz"]).astype(f32)
+    L = zz.shape[0]
+    if zgrid.shape != (L + 1, mesh.n_cells):
+        raise MpasFileError("zgrid dimensions disagree with zz and the mesh")
+    rdzw = np.asarray(vert["rdzw"], dtype=f32)
+    dzw = 1.0 / rdzw.astype(np.float64)
+    zw = np.concatenate([[0.0], np.cumsum(dzw)])
+    zb_edge = np.transpose(np.asarray(vert["zb"], dtype=f32), (2, 1, 0)).copy()
+    zb3_edge = np.transpose(np.asarray(vert["zb3"], dtype=f32), (2, 1, 0)).copy()
+    if zb_edge.shape != (L + 1, 2, mesh.n_edges):
+        raise MpasFileError("zb/zb3 dimensions disagree with the mesh")
+    zb_cell, zb3_cell = hinit.terrain_slot_metrics(mesh, zb_edge, zb3_edge, cfg.config_coef_3rd_order)
+    grid = hinit.VerticalGrid(
+        zw=zw, zgrid=zgrid, zz=zz, zxu=_levels_first(vert["zxu"]).astype(f32), rdzw=rdzw,
+        rdzu=np.asarray(vert["rdzu"], dtype=f32), fzm=np.asarray(vert["fzm"], dtype=f32),
+        fzp=np.asarray(vert["fzp"], dtype=f32), cf1=float(f32(vert["cf1"])), cf2=float(f32(vert["cf2"])),
+        cf3=float(f32(vert["cf3"])), zb_cell=zb_cell, zb3_cell=zb3_cell, dzw=dzw, zb_edge=zb_edge,
+        zb3_edge=zb3_edge, dzu=np.asarray(vert["dzu"], dtype=f32))
+
+    def cell_field(n):
+        if n not in arrays:
+            raise MpasFileError(f"missing MPAS variable {n}")
+        return _levels_first(arrays[n]).astype(f32)
+
+    rho_base, theta_base = cell_field("rho_base"), cell_field("theta_base")
+    if "rtheta_base" in arrays:
+        rtheta_base = cell_field("rtheta_base")
+    else:
+        rtheta_base = (rho_base * theta_base).astype(f32)
+    if "exner_base" in arrays:
+        exner_base = cell_field("exner_base")
+    else:
+        rgas, cp = 287.0, 1004.5
+        arg = zz.astype(np.float64) * (rgas / 1.0e5) * rtheta_base.astype(np.float64)
+        exner_base = np.power(arg, rgas / (cp - rgas)).astype(f32)
+    reference = hinit.ReferenceState(rho_base=rho_base, rtheta_base=rtheta_base, exner_base=exner_base,
+                                     theta_base=theta_base)
+    rho = cell_field("rho_zz")
+    rtheta = cell_field("rtheta_m") if "rtheta_m" in arrays else (rho * cell_field("theta_m")).astype(f32)
+    c1, c2 = mesh.cellsOnEdge[:, 0], mesh.cellsOnEdge[:, 1]
+    if "ru" in arrays:
+        ru = _levels_first(arrays["ru"]).astype(f32)
+    else:
+        u = _levels_first(arrays["u"]).astype(f32)
+        ru = ((f32(0.5) * (rho[:, c1] + rho[:, c2])) * u).astype(f32)
+    if "rw" not in arrays:
+        raise MpasFileError("the file carries no rw: the w -> rw conversion is not built; "
+                            "write rw (a model dump or hexdyn restart carries it)")
+    rw = _levels_first(arrays["rw"]).astype(f32)
+    names = [n for n in SCALAR_NAMES if n in arrays]
+    names += sorted((n for n in arrays if n.startswith("scalar") and n[6:].isdigit()), key=lambda n: int(n[6:]))
+    scalars = (np.stack([cell_field(n) for n in names]) if names
+               else np.zeros((0, L, mesh.n_cells), dtype=f32))
+    moist = bool(int(attrs.get("moist", 1 if "qv" in arrays else 0)))
+    state = hinit.InitialState(rho_zz=rho, rtheta_m=rtheta, ru=ru, rw=rw, scalars=scalars)
+
+    deriv = hsetup.deriv_two(mesh)
+    adv, adv3, nadv, advc = hsetup.advection_coefficients(mesh, deriv)
+    etp, etm, ewp, ewm = hsetup.acoustic_offcentering(
+        grid.rdzw, minimum=cfg.config_epssm_minimum, maximum=cfg.config_epssm_maximum,
+        transition_bottom_z=cfg.config_epssm_transition_bottom_z,
+        transition_top_z=cfg.config_epssm_transition_top_z)
+    angle = mesh.angleEdge.astype(f32)
+    return Case(
+        name=name, mesh=mesh, grid=grid, reference=reference, state=state, config=cfg,
+        n_scalars=scalars.shape[0], n_moist=scalars.shape[0] if moist else 0,
+        adv_coefs=adv, adv_coefs_3rd=adv3, n_adv=nadv, adv_cells=advc,
+        inv_area_cell=hsetup.stored_inverse(mesh.areaCell),
+        inv_area_triangle=hsetup.stored_inverse(mesh.areaTriangle),
+        inv_dc_edge=hsetup.stored_inverse(mesh.dcEdge),
+        inv_dv_edge=hsetup.stored_inverse(mesh.dvEdge),
+        cos_angle_edge=np.cos(angle.astype(np.float64)).astype(f32),
+        sin_angle_edge=np.sin(angle.astype(np.float64)).astype(f32),
+        etp=etp, etm=etm, ewp=ewp, ewm=ewm, dss=_levels_first(vert["dss"]).astype(f32),
+        u_init=np.asarray(arrays.get("u_init", np.zeros(L)), dtype=f32),
+        v_init=np.asarray(arrays.get("v_init", np.zeros(L)), dtype=f32),
+        len_disp=_len_disp(mesh, cfg),
+    )

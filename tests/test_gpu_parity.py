@@ -310,6 +310,24 @@ def test_restart_file_continues_bitwise(get_case, tmp_path):
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
 
 
+def test_case_read_from_an_mpas_init_file_steps_bitwise(get_case, tmp_path):
+    print("[gpu] case from an MPAS init file x1.642-moist, 2 steps", flush=True)
+    from hexdyn import mpas_io
+
+    case = get_case("x1.642-moist")
+    p = tmp_path / "x1.642.init.nc"
+    mpas_io.write_mpas_init(case, p)
+    back = mpas_io.case_from_mpas(p, config=case.config)
+    o = Oracle(back, 32)
+    o.set_state(initial_state_arrays(back), rebuild=True)
+    assert o.step(2) == 0
+    d = _driver(back)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
 def test_phase_by_phase_bitwise_with_physics_tendencies(get_case):
     print("[gpu] phase walk with held physics tendencies x1.2562-moist + mixing", flush=True)
     case = _mixing(get_case, "x1.2562-moist")

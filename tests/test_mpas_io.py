@@ -80,3 +80,74 @@ def test_restart_continues_bitwise_with_the_oracle(get_case, tmp_path):
     got = second.get_state()
     for k in ref:
         assert np.array_equal(ref[k].view(np.uint32), got[k].view(np.uint32)), k
+
+
+def _same_bits(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype.kind == "f" and a.dtype == b.dtype:
+        a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+        return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    return np.array_equal(a, b)
+
+
+def test_init_file_round_trip_rebuilds_the_case_bitwise(get_case, tmp_path):
+    case = get_case("x1.642-moist")
+    p = tmp_path / "x1.642.init.nc"
+    mpas_io.write_mpas_init(case, p)
+    back = mpas_io.case_from_mpas(p, config=case.config)
+    g, h = case.grid, back.grid
+    for name in ("zgrid", "zz", "zxu", "rdzw", "rdzu", "fzm", "fzp", "zb_cell", "zb3_cell", "zb_edge", "zb3_edge",
+                 "dzu"):
+        assert _same_bits(getattr(g, name), getattr(h, name)), name
+    assert (g.cf1, g.cf2, g.cf3) == (h.cf1, h.cf2, h.cf3)
+    for name in ("rho_base", "rtheta_base", "exner_base", "theta_base"):
+        assert _same_bits(getattr(case.reference, name), getattr(back.reference, name)), name
+    for name in ("rho_zz", "rtheta_m", "ru", "rw", "scalars"):
+        assert _same_bits(getattr(case.state, name), getattr(back.state, name)), name
+    for name in ("adv_coefs", "adv_coefs_3rd", "n_adv", "adv_cells", "inv_area_cell", "inv_dc_edge", "etp", "ewm",
+                 "dss", "u_init", "cos_angle_edge"):
+        assert _same_bits(getattr(case, name), getattr(back, name)), name
+    assert (back.n_scalars, back.n_moist) == (case.n_scalars, case.n_moist)
+
+
+def test_init_file_zb3_is_scaled_once_in_binary32(get_case, tmp_path):
+    case = get_case("x1.642-moist")
+    p = tmp_path / "i.nc"
+    mpas_io.write_mpas_init(case, p)
+    from scipy.io import netcdf_file
+
+    with netcdf_file(str(p), "r", mmap=False) as f:
+        zb3 = np.array(f.variables["zb3"].data)          # (nEdges, TWO, nVertLevelsP1), unscaled
+        coe = np.array(f.variables["cellsOnEdge"].data) - 1
+    m = case.mesh
+    c, s = 7, 2
+    e = m.edgesOnCell[c, s]
+    side = 0 if coe[e, 0] == c else 1
+    want = np.float32(case.config.config_coef_3rd_order) * zb3[e, side, :].astype(np.float32)
+    assert _same_bits(case.grid.zb3_cell[:, c, s], want)
+
+
+def test_init_file_steps_like_the_case_on_the_oracle(get_case, tmp_path):
+    from oracle.oracle import Oracle
+
+    case = get_case("x1.642-moist")
+    p = tmp_path / "x1.642.init.nc"
+    mpas_io.write_mpas_init(case, p)
+    back = mpas_io.case_from_mpas(p, config=case.config)
+    a, b = Oracle(case, 32), Oracle(back, 32)
+    a.set_state(initial_state_arrays(case), rebuild=True)
+    b.set_state(initial_state_arrays(back), rebuild=True)
+    assert a.step(1) == 0 and b.step(1) == 0
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        assert _same_bits(sa[k], sb[k]), k
+
+
+def test_init_file_without_rw_or_metrics_is_refused(get_case, tmp_path):
+    from scipy.io import netcdf_file
+
+    case = get_case("x1.642-moist")
+    p = tmp_path / "i.nc"
+    mpas_io.write_mpas_grid(case.mesh, p)
+    with pytest.raises(mpas_io.MpasFileError, match="vertical"):
+        mpas_io.case_from_mpas(p, config=case.config)
