@@ -151,15 +151,23 @@ KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
   }
 }
 
+// divergence, kinetic energy (0.625/0.375 kite blend) and pv_cell of cell c
+struct CellDiag {
+  float div, ke, pv;
+};
 template <int NE>
-KB k_diag_cell(const Instance I, const float *__restrict__ u) {
-  const int c = wave_entity();
-  if (c >= I.n.C) return;
-  const int k = lane();
-  const int n = I.m.neoc[c];
-  const float ia = I.m.invarea[c];
+struct CellDiagIn {
+  float ue[NE], kv[NE], pv[NE], sd[NE], dd[NE], kt[NE];
+  float ia;
+  int n;
+};
+template <int NE>
+__device__ __forceinline__ CellDiagIn<NE> diag_cell_load(const Instance &I, const float *__restrict__ u, int c,
+                                                         int k) {
+  CellDiagIn<NE> a;
+  a.n = I.m.neoc[c];
+  a.ia = I.m.invarea[c];
   int e[NE], vv[NE];
-  float ue[NE], kv[NE], pv[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     e[s] = I.m.eoc[c * NE + s];
@@ -167,20 +175,25 @@ KB k_diag_cell(const Instance I, const float *__restrict__ u) {
   }
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    ue[s] = ROW(u, e[s])[k];
-    kv[s] = ROW(I.b.ke_vertex, vv[s])[k];
-    pv[s] = ROW(I.b.pv_vertex, vv[s])[k];
+    a.ue[s] = ROW(u, e[s])[k];
+    a.kv[s] = ROW(I.b.ke_vertex, vv[s])[k];
+    a.pv[s] = ROW(I.b.pv_vertex, vv[s])[k];
   }
-  float sd[NE], dd[NE], kt[NE];
-  slots(sd, I.m.csdv + c * NE);
-  slots(dd, I.m.cdcdv + c * NE);
-  slots(kt, I.m.ckite + c * NE);
+  slots(a.sd, I.m.csdv + c * NE);
+  slots(a.dd, I.m.cdcdv + c * NE);
+  slots(a.kt, I.m.ckite + c * NE);
+  return a;
+}
+template <int NE>
+__device__ __forceinline__ CellDiag diag_cell_compute(const Instance &I, const CellDiagIn<NE> &a) {
+  const int n = a.n;
+  const float ia = a.ia;
   float d = 0.0f, kc = 0.0f, pvc = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) {
-      d = d + sd[s] * ue[s];
-      kc = kc + 0.25f * (dd[s] * (ue[s] * ue[s]));
+      d = d + a.sd[s] * a.ue[s];
+      kc = kc + 0.25f * (a.dd[s] * (a.ue[s] * a.ue[s]));
     }
   d = d * ia;
   kc = kc * ia;
@@ -188,18 +201,29 @@ KB k_diag_cell(const Instance I, const float *__restrict__ u) {
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) {
-      const float kite = kt[s];
+      const float kite = a.kt[s];
       float corr = 0.375f * kite;
-      corr = corr * kv[s];
+      corr = corr * a.kv[s];
       corr = corr * ia;
       kc = kc + corr;
-      if (I.k.apvm_on) pvc = pvc + (kite * pv[s]) * ia;
+      pvc = pvc + (kite * a.pv[s]) * ia;
     }
+  return {d, kc, pvc};
+}
+__device__ __forceinline__ void store_cell_diag(const Instance &I, int c, int k, CellDiag r) {
   if (k < I.n.L) {
-    ROW(I.b.divergence, c)[k] = d;
-    ROW(I.b.ke, c)[k] = kc;
-    if (I.k.apvm_on) ROW(I.b.pv_cell, c)[k] = pvc;
+    ROW(I.b.divergence, c)[k] = r.div;
+    ROW(I.b.ke, c)[k] = r.ke;
+    if (I.k.apvm_on) ROW(I.b.pv_cell, c)[k] = r.pv;
   }
+}
+
+template <int NE>
+KB k_diag_cell(const Instance I, const float *__restrict__ u) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  store_cell_diag(I, c, k, diag_cell_compute<NE>(I, diag_cell_load<NE>(I, u, c, k)));
 }
 
 // pv_edge (with APVM) and, at rk step 3, the TRiSK tangential velocity v of
@@ -447,11 +471,10 @@ __device__ __forceinline__ float cell_flux_div(const Instance &I, int c, int k, 
 // current rho_zz, as the subcycle-start solve formed it.
 
 // eddy viscosity from the deformation tensor (smagorinsky_v841_f32)
+// 2-D Smagorinsky viscosity of cell c at level k (lane k), from the
+// deformation of the cell's edge winds
 template <int NE>
-KB k_mix_kdiff(const Instance I) {
-  const int c = wave_entity();
-  if (c >= I.n.C) return;
-  const int k = lane();
+__device__ __forceinline__ float mix_kdiff_value(const Instance &I, int c, int k) {
   const int n = I.m.neoc[c];
   int e[NE];
 #pragma unroll
@@ -477,14 +500,14 @@ KB k_mix_kdiff(const Instance I) {
   const float d11 = 2.0f * dudx, d22 = 2.0f * dvdy, d12 = dudy + dvdx;
   const float diff = d11 - d22;
   const float strain = __builtin_sqrtf(0.25f * (diff * diff) + d12 * d12);
-  if (k < I.n.L) ROW(I.b.kdiff, c)[k] = fminf(I.k.strain_scale * strain, I.k.ceiling);
+  return fminf(I.k.strain_scale * strain, I.k.ceiling);
 }
 
-// del2 of u into the Euler pool (momentum_filter_seed_v841_f32): pool + ((rho_e*k_e)*lap)*del2
-KB k_mix_edge(const Instance I) {
-  const int e = wave_entity();
-  if (e >= I.n.E) return;
-  const int k = lane();
+// del2 u of edge e at level k: the increment added to the u pool and delsq_u
+struct MixEdge {
+  float lap, inc;
+};
+__device__ __forceinline__ MixEdge mix_edge_value(const Instance &I, int e, int k) {
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
   const float d0 = ROW(I.b.divergence, c0)[k], d1 = ROW(I.b.divergence, c1)[k];
@@ -494,13 +517,10 @@ KB k_mix_edge(const Instance I) {
   const float lap = (d1 - d0) * I.m.mx_idc[e] - (z1 - z0) * I.m.mx_idv[e];
   const float kedge = 0.5f * (kd0 + kd1);
   const float rho_e = 0.5f * (r0 + r1);
-  if (k < I.n.L) {
-    float *pool = ROW(I.b.euler_ru, e);
-    const float pk = pool[k];
-    if (I.k.mix4) ROW(I.b.delsq_u, e)[k] = lap;
-    pool[k] = pk + ((rho_e * kedge) * lap) * I.m.mx_del2[e];
-  }
+  return {lap, ((rho_e * kedge) * lap) * I.m.mx_del2[e]};
 }
+
+
 
 // divergence of delsq_u (laplacian_divergence_v841_f32)
 template <int NE>
@@ -670,13 +690,15 @@ KB k_mix_cell4(const Instance I) {
 }
 
 // ------------------------------------------------------------ subcycle start
-template <int NE>
+// MIX: also the 2-D Smagorinsky kdiff of the cell
+template <int NE, bool MIX>
 KB k_sub_cell(const Instance I) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
   const bool moist = I.n.NM > 0;
+  const float kd = MIX ? mix_kdiff_value<NE>(I, c, k) : 0.0f;
   // every load first (the mass-flux divergence gathers among them)
   const float rho_p = ROW(I.b.s_rho_p, c)[k];
   const float qt = moist ? ROW(I.b.qtot, c)[k] : 0.0f;
@@ -702,10 +724,14 @@ KB k_sub_cell(const Instance I) {
     ROW(I.b.dpdz, c)[k] = dpdz;
     ROW(I.b.tend_rho, c)[k] = tr;
     ROW(I.b.hdiv, c)[k] = acc;  // stage-1 mass-flux divergence of the current ru
+    if (MIX) ROW(I.b.kdiff, c)[k] = kd;
   }
   if (k <= L) ROW(I.b.euler_rw, c)[k] = erw;
 }
 
+// MIX: also add the del2 u increment of the 2-D Smagorinsky mixing to the
+// pool (mixing.py del2 u term, in the same pass; kdiff is final by then)
+template <bool MIX>
 KB k_sub_edge(const Instance I) {
   const int e = wave_entity();
   if (e >= I.n.E) return;
@@ -714,7 +740,16 @@ KB k_sub_edge(const Instance I) {
   const float *pp = I.b.pp, *zz = I.z.zz, *dz = I.b.dpdz;
   const float a = ((ROW(pp, c1)[k] - ROW(pp, c0)[k]) * I.m.invdc[e]) / (0.5f * (ROW(zz, c1)[k] + ROW(zz, c0)[k]));
   const float bb = (0.5f * ROW(I.z.zxu, e)[k]) * (ROW(dz, c0)[k] + ROW(dz, c1)[k]);
-  if (k < I.n.L) ROW(I.b.euler_ru, e)[k] = (0.0f - ROW(I.b.cqu, e)[k]) * (a - bb);
+  const float pool = (0.0f - ROW(I.b.cqu, e)[k]) * (a - bb);
+  if (MIX) {
+    const MixEdge m = mix_edge_value(I, e, k);
+    if (k < I.n.L) {
+      if (I.k.mix4) ROW(I.b.delsq_u, e)[k] = m.lap;
+      ROW(I.b.euler_ru, e)[k] = pool + m.inc;
+    }
+  } else {
+    if (k < I.n.L) ROW(I.b.euler_ru, e)[k] = pool;
+  }
 }
 
 // ---------------------------------------------------------- stage tendencies
@@ -1785,13 +1820,20 @@ int begin_step(const Instance &I, hipStream_t s) {
 
 int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
   (void)sub;
-  LAUNCH_NE(I, k_sub_cell, I.n.C, 1, s, I);
-  launch(k_sub_edge, I.n.E, 1, s, I);
+  // with mixing, kdiff rides on the cell pass and the del2 u increment on the
+  // edge pass (the pool is written once)
+  if (I.k.mixing) {
+    NEDISPATCH(I, launch_named("k_sub_cell", k_sub_cell<6, true>, I.n.C, 1, s, I),
+               launch_named("k_sub_cell", k_sub_cell<7, true>, I.n.C, 1, s, I));
+    launch_named("k_sub_edge", k_sub_edge<true>, I.n.E, 1, s, I);
+  } else {
+    NEDISPATCH(I, launch_named("k_sub_cell", k_sub_cell<6, false>, I.n.C, 1, s, I),
+               launch_named("k_sub_cell", k_sub_cell<7, false>, I.n.C, 1, s, I));
+    launch_named("k_sub_edge", k_sub_edge<false>, I.n.E, 1, s, I);
+  }
   launch_col(k_vcoef, I.n.C, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
          (const float *)I.b.s_rtheta_p);
   if (I.k.mixing) {
-    LAUNCH_NE(I, k_mix_kdiff, I.n.C, 1, s, I);
-    launch(k_mix_edge, I.n.E, 1, s, I);
     if (I.k.mix4) {
       LAUNCH_NE(I, k_mix_lapdiv, I.n.C, 1, s, I);
       launch(k_mix_lapvort, I.n.V, 1, s, I);

@@ -60,8 +60,11 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     "k_moist_cell": [("SC", "L"), ("C", "L"), ("C", "L")],
     "k_moist_edge": [("SC", "L"), ("E", "L")],
     "k_vcoef": [("C", "L")] * 10 + [("C", "W")],
-    "k_sub_cell": [("C", "L")] * 12 + [("C", "W")] * 3 + [("E", "L")],
-    "k_sub_edge": [("E", "L")] * 5 + [("C", "L")] * 3,
+    # with 2-D Smagorinsky mixing (the default): + u, v | kdiff
+    "k_sub_cell": [("C", "L")] * 12 + [("C", "W")] * 3 + [("E", "L")] * 3 + [("C", "L")],
+    # with mixing: + divergence, vorticity, kdiff, c_rho | delsq_u
+    "k_sub_edge": [("E", "L")] * 5 + [("C", "L")] * 3 + [("C", "L"), ("V", "L"), ("C", "L"), ("C", "L"),
+                                                          ("E", "L")],
     # u, c_rw, pv_edge, c_rho, ke, hdiv, euler_ru | tend_ru
     "k_tend_mom": [("E", "L"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
                    ("E", "L")],
@@ -103,10 +106,6 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     "k_fct_edge2": [("SC", "L"), ("SC", "L"), ("SE", "L"), ("SE", "L")],
     "k_fct_cell2": [("SC", "L"), ("SC", "L"), ("SC", "W"), ("SC", "L"), ("SE", "L"), ("C", "L"), ("SC", "L")],
     # horizontal mixing, once per subcycle
-    # u, v | kdiff
-    "k_mix_kdiff": [("E", "L"), ("E", "L"), ("C", "L")],
-    # divergence, vorticity, kdiff, c_rho, euler_ru | euler_ru, delsq_u
-    "k_mix_edge": [("C", "L"), ("V", "L"), ("C", "L"), ("C", "L"), ("E", "L"), ("E", "L"), ("E", "L")],
     "k_mix_lapdiv": [("E", "L"), ("C", "L")],
     "k_mix_lapvort": [("E", "L"), ("V", "L")],
     # delsq_div, delsq_vort, c_rho, euler_ru | euler_ru
