@@ -116,6 +116,16 @@ constexpr int COLP = 65;
 // register budget is capped for that (at 7 a CU would hold one block).
 #define KBCOL __global__ void __launch_bounds__(64 * COLW) __attribute__((amdgpu_waves_per_eu(8, 8)))
 
+// Per-level coefficient rows are the same for every column of a block: wave
+// j < N of a column block copies row j into LDS once, instead of each of the
+// 16 wavefronts reading every row.  Callers synchronise before reading.
+template <int N>
+__device__ __forceinline__ void stage_levels(float *s_lev, const float *const (&rows)[N], int w, int k) {
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    if (w == j) s_lev[j * 64 + k] = rows[j][k];
+}
+
 // Gather helpers.  Connectivity rows are padded to fixed widths (NE = maxEdges
 // slots per cell, NX = 2*maxEdges-2 slots per edge) with index 0, so every
 // slot of an unrolled loop addresses a valid row.  All neighbour rows are
@@ -369,6 +379,7 @@ KB k_moist_edge(const Instance I) {
 KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const float *__restrict__ theta,
               const float *__restrict__ rtp) {
   __shared__ float s_a[COLW * COLP], s_b[COLW * COLP], s_c[COLW * COLP];
+  __shared__ float s_lev[6 * 64];
   const int w = (int)(threadIdx.x >> 6);
   const int c0 = wave_entity<COLW>();
   const bool live = c0 < I.n.C;
@@ -381,8 +392,13 @@ KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const 
   const float zk = ROW(I.z.zz, c)[k], exk = ROW(exn, c)[k], thk = ROW(theta, c)[k];
   const float cqw = ROW(I.b.cqw, c)[k], qt = ROW(I.b.qtot, c)[k], rtpk = ROW(rtp, c)[k];
   const float rbk = ROW(I.z.rb, c)[k], rtbk = ROW(I.z.rtb, c)[k], exbk = ROW(I.z.exb, c)[k];
-  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k], rdzu = I.z.rdzu[k];
-  const float rdzw = I.z.rdzw[k], etp = I.z.etp[k], ewp = I.z.ewp[k];
+  {
+    const float *const lev[6] = {I.z.fzm, I.z.fzp, I.z.rdzu, I.z.rdzw, I.z.etp, I.z.ewp};
+    stage_levels<6>(s_lev, lev, w, k);
+  }
+  __syncthreads();
+  const float fzm = s_lev[0 * 64 + k], fzp = s_lev[1 * 64 + k], rdzu = s_lev[2 * 64 + k];
+  const float rdzw = s_lev[3 * 64 + k], etp = s_lev[4 * 64 + k], ewp = s_lev[5 * 64 + k];
   const float zkm = lane_off(zk, k, -1), exkm = lane_off(exk, k, -1), thkm = lane_off(thk, k, -1);
   const float rdzwm = lane_off(rdzw, k, -1), etpm = lane_off(etp, k, -1);
   const float ewpm = lane_off(ewp, k, -1), ewpp = lane_off(ewp, k, 1);
@@ -1064,6 +1080,7 @@ KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_c
 template <int NE, bool FIRST>
 KBCOL k_ac_cell(const Instance I, float dts) {
   __shared__ float s_x[COLW * COLP], s_d[COLW * COLP], s_l[COLW * COLP], s_g[COLW * COLP];
+  __shared__ float s_lev[7 * 64];
   const int w = (int)(threadIdx.x >> 6);
   const int c0 = wave_entity<COLW>();
   const bool live = c0 < I.n.C;
@@ -1102,8 +1119,13 @@ KBCOL k_ac_cell(const Instance I, float dts) {
   const float a = ROW(I.b.atri, c)[k], al = ROW(I.b.alpha, c)[k], ga = ROW(I.b.gamma, c)[k];
   const float srw = ROW(I.b.s_rw, c)[k], crw = ROW(I.b.c_rw, c)[k], damp = ROW(I.z.dss, c)[k];
   const float crho = ROW(I.b.c_rho, c)[k], wk = ROW(I.b.w, c)[k];
-  const float rdzw = I.z.rdzw[k], ewm = I.z.ewm[k], ewp = I.z.ewp[k], etp = I.z.etp[k], etm = I.z.etm[k];
-  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
+  {
+    const float *const lev[7] = {I.z.rdzw, I.z.ewm, I.z.ewp, I.z.etp, I.z.etm, I.z.fzm, I.z.fzp};
+    stage_levels<7>(s_lev, lev, w, k);
+  }
+  __syncthreads();
+  const float rdzw = s_lev[0 * 64 + k], ewm = s_lev[1 * 64 + k], ewp = s_lev[2 * 64 + k];
+  const float etp = s_lev[3 * 64 + k], etm = s_lev[4 * 64 + k], fzm = s_lev[5 * 64 + k], fzp = s_lev[6 * 64 + k];
   const float rtpp_old = rtpp;
   const float ia = I.m.invarea[c];
   float r = 0.0f, t = 0.0f;
