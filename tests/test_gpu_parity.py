@@ -310,6 +310,39 @@ def test_restart_file_continues_bitwise(get_case, tmp_path):
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
 
 
+# Configurations that take the other branches of the fused whole-step
+# sequence: more than two acoustic substeps (ru_avg stored and read back), one
+# substep everywhere (the average taken from ru_p), one dynamics subcycle,
+# no divergence damping, unlimited transport with the final clamp, the RK2
+# schedule, and mixing without its fourth-order part.
+STEP_VARIANTS = [
+    dict(config_number_of_sub_steps=6),
+    dict(config_number_of_sub_steps=1),
+    dict(config_dynamics_split_steps=1),
+    dict(config_divergence_damping=False),
+    dict(config_monotonic=False),
+    dict(config_time_integration_order=2),
+    dict(config_horiz_mixing="2d_smagorinsky", config_visc4_2dsmag=0.0),
+]
+
+
+@pytest.mark.parametrize("variant", STEP_VARIANTS, ids=lambda v: ",".join(f"{k[7:]}={x}" for k, x in v.items()))
+def test_whole_step_variants_bitwise(get_case, variant):
+    import dataclasses
+
+    print(f"[gpu] x1.642-moist 2 steps with {variant}", flush=True)
+    base = get_case("x1.642-moist")
+    case = get_case("x1.642-moist", config=dataclasses.replace(base.config, **variant))
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(2) == 0
+    d = _driver(case)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
 def test_case_read_from_an_mpas_init_file_steps_bitwise(get_case, tmp_path):
     print("[gpu] case from an MPAS init file x1.642-moist, 2 steps", flush=True)
     from hexdyn import mpas_io
