@@ -41,6 +41,13 @@ namespace HEXDYN_NS {
 namespace {
 
 #define ROW(p, ent) ((p) + (size_t)(ent) * HX_ROW)
+// Element k of row ent of p, addressed as a 32-bit byte offset from the
+// wave-uniform base (every array is below 4 GiB), so loads and stores take
+// the SGPR-base + VGPR-offset form instead of a per-lane 64-bit address.
+__device__ __forceinline__ uint32_t row_off(int ent, int k) {
+  return (uint32_t)ent * (uint32_t)(HX_ROW * 4) + (uint32_t)k * 4u;
+}
+#define AT(p, ent, k) (*(float *)((char *)(p) + row_off((ent), (k))))
 #define KB __global__ void __launch_bounds__(256)
 
 // First entity of this block: blocks are renumbered so that XCD x (hardware
@@ -143,7 +150,7 @@ KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
 #pragma unroll
   for (int s = 0; s < 3; ++s) e[s] = I.m.eov[3 * v + s];
 #pragma unroll
-  for (int s = 0; s < 3; ++s) ue[s] = ROW(u, e[s])[k];
+  for (int s = 0; s < 3; ++s) ue[s] = AT(u, e[s], k);
   float vort = 0.0f, kev = 0.0f;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
@@ -155,9 +162,9 @@ KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
   kev = kev * (0.25f * it);
   const float pvv = I.m.fvert[v] + vort;
   if (k < I.n.L) {
-    ROW(I.b.vorticity, v)[k] = vort;
-    ROW(I.b.ke_vertex, v)[k] = kev;
-    ROW(I.b.pv_vertex, v)[k] = pvv;
+    AT(I.b.vorticity, v, k) = vort;
+    AT(I.b.ke_vertex, v, k) = kev;
+    AT(I.b.pv_vertex, v, k) = pvv;
   }
 }
 
@@ -185,9 +192,9 @@ __device__ __forceinline__ CellDiagIn<NE> diag_cell_load(const Instance &I, cons
   }
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    a.ue[s] = ROW(u, e[s])[k];
-    a.kv[s] = ROW(I.b.ke_vertex, vv[s])[k];
-    a.pv[s] = ROW(I.b.pv_vertex, vv[s])[k];
+    a.ue[s] = AT(u, e[s], k);
+    a.kv[s] = AT(I.b.ke_vertex, vv[s], k);
+    a.pv[s] = AT(I.b.pv_vertex, vv[s], k);
   }
   slots(a.sd, I.m.csdv + c * NE);
   slots(a.dd, I.m.cdcdv + c * NE);
@@ -222,9 +229,9 @@ __device__ __forceinline__ CellDiag diag_cell_compute(const Instance &I, const C
 }
 __device__ __forceinline__ void store_cell_diag(const Instance &I, int c, int k, CellDiag r) {
   if (k < I.n.L) {
-    ROW(I.b.divergence, c)[k] = r.div;
-    ROW(I.b.ke, c)[k] = r.ke;
-    if (I.k.apvm_on) ROW(I.b.pv_cell, c)[k] = r.pv;
+    AT(I.b.divergence, c, k) = r.div;
+    AT(I.b.ke, c, k) = r.ke;
+    if (I.k.apvm_on) AT(I.b.pv_cell, c, k) = r.pv;
   }
 }
 
@@ -260,21 +267,21 @@ __device__ __forceinline__ EdgeDiagIn<NE, RECON> diag_edge_load(const Instance &
 #pragma unroll
     for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
 #pragma unroll
-    for (int s = 0; s < NX; ++s) a.un[s] = ROW(u, nb[s])[k];
+    for (int s = 0; s < NX; ++s) a.un[s] = AT(u, nb[s], k);
     a.n = I.m.neoe[e];
     slots(a.wt, I.m.woe + e * NX);
     a.vrow = 0.0f;
   } else {
-    a.vrow = ROW(I.b.v, e)[k];
+    a.vrow = AT(I.b.v, e, k);
     a.n = 0;
   }
   const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  a.pv0 = ROW(I.b.pv_vertex, v0)[k];
-  a.pv1 = ROW(I.b.pv_vertex, v1)[k];
-  a.pc0 = ROW(I.b.pv_cell, c0)[k];
-  a.pc1 = ROW(I.b.pv_cell, c1)[k];
-  a.ue = ROW(u, e)[k];
+  a.pv0 = AT(I.b.pv_vertex, v0, k);
+  a.pv1 = AT(I.b.pv_vertex, v1, k);
+  a.pc0 = AT(I.b.pv_cell, c0, k);
+  a.pc1 = AT(I.b.pv_cell, c1, k);
+  a.ue = AT(u, e, k);
   return a;
 }
 template <int NE, bool RECON>
@@ -302,8 +309,8 @@ __device__ __forceinline__ EdgeDiag diag_edge_values(const Instance &I, const fl
 }
 __device__ __forceinline__ void store_edge_diag(const Instance &I, int e, int k, bool recon, EdgeDiag d) {
   if (k < I.n.L) {
-    if (recon) ROW(I.b.v, e)[k] = d.v;
-    ROW(I.b.pv_edge, e)[k] = d.pv;
+    if (recon) AT(I.b.v, e, k) = d.v;
+    AT(I.b.pv_edge, e, k) = d.pv;
   }
 }
 
@@ -354,12 +361,12 @@ KB k_moist_cell(const Instance I) {
   const int k = lane();
   const size_t plane = (size_t)I.n.C * HX_ROW;
   float t = 0.0f;
-  for (int s = 0; s < I.n.NM; ++s) t = t + ROW(I.b.q + s * plane, c)[k];
+  for (int s = 0; s < I.n.NM; ++s) t = t + AT(I.b.q, s * I.n.C + c, k);
   const float tm = up1(t);
   const float cqw = k >= 1 ? 1.0f / (1.0f + 0.5f * (t + tm)) : 1.0f;
   if (k < I.n.L) {
-    ROW(I.b.qtot, c)[k] = t;
-    ROW(I.b.cqw, c)[k] = cqw;
+    AT(I.b.qtot, c, k) = t;
+    AT(I.b.cqw, c, k) = cqw;
   }
 }
 
@@ -371,8 +378,8 @@ KB k_moist_edge(const Instance I) {
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   float t = 0.0f;
   for (int s = 0; s < I.n.NM; ++s)
-    t = t + 0.5f * (ROW(I.b.q + s * plane, c0)[k] + ROW(I.b.q + s * plane, c1)[k]);
-  if (k < I.n.L) ROW(I.b.cqu, e)[k] = 1.0f / (1.0f + t);
+    t = t + 0.5f * (AT(I.b.q, s * I.n.C + c0, k) + AT(I.b.q, s * I.n.C + c1, k));
+  if (k < I.n.L) AT(I.b.cqu, e, k) = 1.0f / (1.0f + t);
 }
 
 // ------------------------------------------------------ implicit coefficients
@@ -389,9 +396,9 @@ KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const 
   const float g = I.k.gravity, rcv = I.k.rcv, c2 = I.k.c2;
   const float dts2 = dts * dts;
   // every load first; the level k-1 and k+1 values come from neighbouring lanes
-  const float zk = ROW(I.z.zz, c)[k], exk = ROW(exn, c)[k], thk = ROW(theta, c)[k];
-  const float cqw = ROW(I.b.cqw, c)[k], qt = ROW(I.b.qtot, c)[k], rtpk = ROW(rtp, c)[k];
-  const float rbk = ROW(I.z.rb, c)[k], rtbk = ROW(I.z.rtb, c)[k], exbk = ROW(I.z.exb, c)[k];
+  const float zk = AT(I.z.zz, c, k), exk = AT(exn, c, k), thk = AT(theta, c, k);
+  const float cqw = AT(I.b.cqw, c, k), qt = AT(I.b.qtot, c, k), rtpk = AT(rtp, c, k);
+  const float rbk = AT(I.z.rb, c, k), rtbk = AT(I.z.rtb, c, k), exbk = AT(I.z.exb, c, k);
   {
     const float *const lev[6] = {I.z.fzm, I.z.fzp, I.z.rdzu, I.z.rdzw, I.z.etp, I.z.ewp};
     stage_levels<6>(s_lev, lev, w, k);
@@ -448,14 +455,14 @@ KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const 
   const float alpha = inner ? s_a[w * COLP + k] : 0.0f;
   const float gamma = inner ? s_b[w * COLP + k] : 0.0f;
   if (live && k <= L) {
-    ROW(I.b.coftz, c)[k] = coftz;
+    AT(I.b.coftz, c, k) = coftz;
     if (k < L) {
-      ROW(I.b.cofwr, c)[k] = cofwr;
-      ROW(I.b.cofwz, c)[k] = cofwz;
-      ROW(I.b.cofwt, c)[k] = cofwt;
-      ROW(I.b.atri, c)[k] = a;
-      ROW(I.b.alpha, c)[k] = alpha;
-      ROW(I.b.gamma, c)[k] = gamma;
+      AT(I.b.cofwr, c, k) = cofwr;
+      AT(I.b.cofwz, c, k) = cofwz;
+      AT(I.b.cofwt, c, k) = cofwt;
+      AT(I.b.atri, c, k) = a;
+      AT(I.b.alpha, c, k) = alpha;
+      AT(I.b.gamma, c, k) = gamma;
     }
   }
 }
@@ -469,7 +476,7 @@ __device__ __forceinline__ float cell_flux_div(const Instance &I, int c, int k, 
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
 #pragma unroll
-  for (int s = 0; s < NE; ++s) xv[s] = ROW(x, e[s])[k];
+  for (int s = 0; s < NE; ++s) xv[s] = AT(x, e[s], k);
   float sd[NE];
   slots(sd, I.m.csdv + c * NE);
   float acc = 0.0f;
@@ -498,8 +505,8 @@ __device__ __forceinline__ float mix_kdiff_value(const Instance &I, int c, int k
   float un[NE], vn[NE], c2[NE], s2[NE], cs[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    un[s] = ROW(I.b.u, e[s])[k];
-    vn[s] = ROW(I.b.v, e[s])[k];
+    un[s] = AT(I.b.u, e[s], k);
+    vn[s] = AT(I.b.v, e[s], k);
   }
   slots(c2, I.m.defc2 + c * NE);
   slots(s2, I.m.defs2 + c * NE);
@@ -526,10 +533,10 @@ struct MixEdge {
 __device__ __forceinline__ MixEdge mix_edge_value(const Instance &I, int e, int k) {
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
-  const float d0 = ROW(I.b.divergence, c0)[k], d1 = ROW(I.b.divergence, c1)[k];
-  const float z0 = ROW(I.b.vorticity, v0)[k], z1 = ROW(I.b.vorticity, v1)[k];
-  const float kd0 = ROW(I.b.kdiff, c0)[k], kd1 = ROW(I.b.kdiff, c1)[k];
-  const float r0 = ROW(I.b.c_rho, c0)[k], r1 = ROW(I.b.c_rho, c1)[k];
+  const float d0 = AT(I.b.divergence, c0, k), d1 = AT(I.b.divergence, c1, k);
+  const float z0 = AT(I.b.vorticity, v0, k), z1 = AT(I.b.vorticity, v1, k);
+  const float kd0 = AT(I.b.kdiff, c0, k), kd1 = AT(I.b.kdiff, c1, k);
+  const float r0 = AT(I.b.c_rho, c0, k), r1 = AT(I.b.c_rho, c1, k);
   const float lap = (d1 - d0) * I.m.mx_idc[e] - (z1 - z0) * I.m.mx_idv[e];
   const float kedge = 0.5f * (kd0 + kd1);
   const float rho_e = 0.5f * (r0 + r1);
@@ -550,13 +557,13 @@ KB k_mix_lapdiv(const Instance I) {
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
 #pragma unroll
-  for (int s = 0; s < NE; ++s) x[s] = ROW(I.b.delsq_u, e[s])[k];
+  for (int s = 0; s < NE; ++s) x[s] = AT(I.b.delsq_u, e[s], k);
   slots(f, I.m.mx_fdiv + c * NE);
   float acc = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) acc = acc + f[s] * x[s];
-  if (k < I.n.L) ROW(I.b.delsq_div, c)[k] = acc;
+  if (k < I.n.L) AT(I.b.delsq_div, c, k) = acc;
 }
 
 // vorticity of delsq_u (laplacian_vorticity_v841_f32)
@@ -569,12 +576,12 @@ KB k_mix_lapvort(const Instance I) {
 #pragma unroll
   for (int s = 0; s < 3; ++s) e[s] = I.m.eov[3 * v + s];
 #pragma unroll
-  for (int s = 0; s < 3; ++s) x[s] = ROW(I.b.delsq_u, e[s])[k];
+  for (int s = 0; s < 3; ++s) x[s] = AT(I.b.delsq_u, e[s], k);
   slots(f, I.m.mx_fvort + 3 * v);
   float acc = 0.0f;
 #pragma unroll
   for (int s = 0; s < 3; ++s) acc = acc + f[s] * x[s];
-  if (k < I.n.L) ROW(I.b.delsq_vort, v)[k] = acc;
+  if (k < I.n.L) AT(I.b.delsq_vort, v, k) = acc;
 }
 
 // del4 of u (momentum_filter_lap4_v841_f32): pool - rho_e*((ddiv*r_dc) - (dvort*r_dv))
@@ -584,14 +591,13 @@ KB k_mix_edge4(const Instance I) {
   const int k = lane();
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
-  const float d0 = ROW(I.b.delsq_div, c0)[k], d1 = ROW(I.b.delsq_div, c1)[k];
-  const float z0 = ROW(I.b.delsq_vort, v0)[k], z1 = ROW(I.b.delsq_vort, v1)[k];
-  const float r0 = ROW(I.b.c_rho, c0)[k], r1 = ROW(I.b.c_rho, c1)[k];
+  const float d0 = AT(I.b.delsq_div, c0, k), d1 = AT(I.b.delsq_div, c1, k);
+  const float z0 = AT(I.b.delsq_vort, v0, k), z1 = AT(I.b.delsq_vort, v1, k);
+  const float r0 = AT(I.b.c_rho, c0, k), r1 = AT(I.b.c_rho, c1, k);
   const float rho_e = 0.5f * (r0 + r1);
   const float diffusion = rho_e * ((d1 - d0) * I.m.mx_rdc[e] - (z1 - z0) * I.m.mx_rdv[e]);
   if (k < I.n.L) {
-    float *pool = ROW(I.b.euler_ru, e);
-    pool[k] = pool[k] - diffusion;
+    AT(I.b.euler_ru, e, k) = AT(I.b.euler_ru, e, k) - diffusion;
   }
 }
 
@@ -613,13 +619,13 @@ KB k_mix_cell(const Instance I) {
   float tho[NE], rho[NE], wo[NE], kdo[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    tho[s] = ROW(I.b.theta, o[s])[k];
-    rho[s] = ROW(I.b.c_rho, o[s])[k];
-    wo[s] = ROW(I.b.w, o[s])[k];
-    kdo[s] = ROW(I.b.kdiff, o[s])[k];
+    tho[s] = AT(I.b.theta, o[s], k);
+    rho[s] = AT(I.b.c_rho, o[s], k);
+    wo[s] = AT(I.b.w, o[s], k);
+    kdo[s] = AT(I.b.kdiff, o[s], k);
   }
-  const float thc = ROW(I.b.theta, c)[k], rhc = ROW(I.b.c_rho, c)[k], wc = ROW(I.b.w, c)[k];
-  const float kdc = ROW(I.b.kdiff, c)[k];
+  const float thc = AT(I.b.theta, c, k), rhc = AT(I.b.c_rho, c, k), wc = AT(I.b.w, c, k);
+  const float kdc = AT(I.b.kdiff, c, k);
   float sg[NE], f2[NE], fw[NE], d2[NE];
   slots(sg, I.m.csign + c * NE);
   slots(f2, I.m.mx_fth2 + c * NE);
@@ -653,15 +659,14 @@ KB k_mix_cell(const Instance I) {
     tendw = 0.0f;
   }
   if (k < L) {
-    ROW(I.b.delsq_theta, c)[k] = lap;
-    ROW(I.b.euler_theta, c)[k] = tend;
-    ROW(I.b.delsq_w, c)[k] = lapw;
+    AT(I.b.delsq_theta, c, k) = lap;
+    AT(I.b.euler_theta, c, k) = tend;
+    AT(I.b.delsq_w, c, k) = lapw;
   }
   if (I.k.mix4) {
-    if (k <= L) ROW(I.b.mix_tw, c)[k] = tendw;
+    if (k <= L) AT(I.b.mix_tw, c, k) = tendw;
   } else if (k <= L) {
-    float *pool = ROW(I.b.euler_rw, c);
-    pool[k] = pool[k] + tendw;
+    AT(I.b.euler_rw, c, k) = AT(I.b.euler_rw, c, k) + tendw;
   }
 }
 
@@ -679,15 +684,15 @@ KB k_mix_cell4(const Instance I) {
   float dto[NE], dwo[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    dto[s] = ROW(I.b.delsq_theta, o[s])[k];
-    dwo[s] = ROW(I.b.delsq_w, o[s])[k];
+    dto[s] = AT(I.b.delsq_theta, o[s], k);
+    dwo[s] = AT(I.b.delsq_w, o[s], k);
   }
-  const float dtc = ROW(I.b.delsq_theta, c)[k], dwc = ROW(I.b.delsq_w, c)[k];
+  const float dtc = AT(I.b.delsq_theta, c, k), dwc = AT(I.b.delsq_w, c, k);
   float sg[NE], f4[NE];
   slots(sg, I.m.csign + c * NE);
   slots(f4, I.m.mx_f4 + c * NE);
-  float tend = ROW(I.b.euler_theta, c)[k];
-  float tendw = ROW(I.b.mix_tw, c)[k];
+  float tend = AT(I.b.euler_theta, c, k);
+  float tendw = AT(I.b.mix_tw, c, k);
   const bool wlev = k >= 1 && k < L;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
@@ -698,11 +703,8 @@ KB k_mix_cell4(const Instance I) {
       tend = tend - f4[s] * (t1 - t0);
       if (wlev) tendw = tendw - f4[s] * (w1 - w0);
     }
-  if (k < L) ROW(I.b.euler_theta, c)[k] = tend;
-  if (k <= L) {
-    float *pool = ROW(I.b.euler_rw, c);
-    pool[k] = pool[k] + tendw;
-  }
+  if (k < L) AT(I.b.euler_theta, c, k) = tend;
+  if (k <= L) AT(I.b.euler_rw, c, k) = AT(I.b.euler_rw, c, k) + tendw;
 }
 
 // ------------------------------------------------------------ subcycle start
@@ -716,33 +718,33 @@ KB k_sub_cell(const Instance I) {
   const bool moist = I.n.NM > 0;
   const float kd = MIX ? mix_kdiff_value<NE>(I, c, k) : 0.0f;
   // every load first (the mass-flux divergence gathers among them)
-  const float rho_p = ROW(I.b.s_rho_p, c)[k];
-  const float qt = moist ? ROW(I.b.qtot, c)[k] : 0.0f;
-  const float rb = moist ? ROW(I.z.rb, c)[k] : 0.0f;
-  const float cqw = moist ? ROW(I.b.cqw, c)[k] : 0.0f;
-  const float prho = I.k.physics ? ROW(I.b.phys_rho, c)[k] : 0.0f;
-  const float rw_k = ROW(I.b.c_rw, c)[k];
-  const float pp_k = ROW(I.b.pp, c)[k];
+  const float rho_p = AT(I.b.s_rho_p, c, k);
+  const float qt = moist ? AT(I.b.qtot, c, k) : 0.0f;
+  const float rb = moist ? AT(I.z.rb, c, k) : 0.0f;
+  const float cqw = moist ? AT(I.b.cqw, c, k) : 0.0f;
+  const float prho = I.k.physics ? AT(I.b.phys_rho, c, k) : 0.0f;
+  const float rw_k = AT(I.b.c_rw, c, k);
+  const float pp_k = AT(I.b.pp, c, k);
   const float acc = cell_flux_div<NE>(I, c, k, I.b.c_ru);
   const float g = I.k.gravity;
   const float dpdz = moist ? (-g) * (rb * qt + rho_p * (1.0f + qt)) : (-g) * rho_p;
   // 0 - x, not -x: a zero divergence gives +0 (density_tendency_f32)
-  const float trho = (0.0f - acc) - I.z.rdzw[k] * (lane_off(rw_k, k, 1) - rw_k);
+  const float trho = (0.0f - acc) - AT(I.z.rdzw, 0, k) * (lane_off(rw_k, k, 1) - rw_k);
   const float pp_m1 = lane_off(pp_k, k, -1);
   const float dpdzm = up1(dpdz);
   float erw = 0.0f;
   if (k >= 1 && k < L) {
-    const float inner = I.z.rdzu[k] * (pp_k - pp_m1) - (I.z.fzm[k] * dpdz + I.z.fzp[k] * dpdzm);
+    const float inner = AT(I.z.rdzu, 0, k) * (pp_k - pp_m1) - (AT(I.z.fzm, 0, k) * dpdz + AT(I.z.fzp, 0, k) * dpdzm);
     erw = moist ? 0.0f - cqw * inner : 0.0f - inner;  // euler_w(_moist_v841)_f32
   }
   if (k < L) {
     const float tr = I.k.physics ? trho + prho : trho;
-    ROW(I.b.dpdz, c)[k] = dpdz;
-    ROW(I.b.tend_rho, c)[k] = tr;
-    ROW(I.b.hdiv, c)[k] = acc;  // stage-1 mass-flux divergence of the current ru
-    if (MIX) ROW(I.b.kdiff, c)[k] = kd;
+    AT(I.b.dpdz, c, k) = dpdz;
+    AT(I.b.tend_rho, c, k) = tr;
+    AT(I.b.hdiv, c, k) = acc;  // stage-1 mass-flux divergence of the current ru
+    if (MIX) AT(I.b.kdiff, c, k) = kd;
   }
-  if (k <= L) ROW(I.b.euler_rw, c)[k] = erw;
+  if (k <= L) AT(I.b.euler_rw, c, k) = erw;
 }
 
 // MIX: also add the del2 u increment of the 2-D Smagorinsky mixing to the
@@ -754,17 +756,17 @@ KB k_sub_edge(const Instance I) {
   const int k = lane();
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const float *pp = I.b.pp, *zz = I.z.zz, *dz = I.b.dpdz;
-  const float a = ((ROW(pp, c1)[k] - ROW(pp, c0)[k]) * I.m.invdc[e]) / (0.5f * (ROW(zz, c1)[k] + ROW(zz, c0)[k]));
-  const float bb = (0.5f * ROW(I.z.zxu, e)[k]) * (ROW(dz, c0)[k] + ROW(dz, c1)[k]);
-  const float pool = (0.0f - ROW(I.b.cqu, e)[k]) * (a - bb);
+  const float a = ((AT(pp, c1, k) - AT(pp, c0, k)) * I.m.invdc[e]) / (0.5f * (AT(zz, c1, k) + AT(zz, c0, k)));
+  const float bb = (0.5f * AT(I.z.zxu, e, k)) * (AT(dz, c0, k) + AT(dz, c1, k));
+  const float pool = (0.0f - AT(I.b.cqu, e, k)) * (a - bb);
   if (MIX) {
     const MixEdge m = mix_edge_value(I, e, k);
     if (k < I.n.L) {
-      if (I.k.mix4) ROW(I.b.delsq_u, e)[k] = m.lap;
-      ROW(I.b.euler_ru, e)[k] = pool + m.inc;
+      if (I.k.mix4) AT(I.b.delsq_u, e, k) = m.lap;
+      AT(I.b.euler_ru, e, k) = pool + m.inc;
     }
   } else {
-    if (k < I.n.L) ROW(I.b.euler_ru, e)[k] = pool;
+    if (k < I.n.L) AT(I.b.euler_ru, e, k) = pool;
   }
 }
 
@@ -780,24 +782,23 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   int nb[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
-  const float *ur = ROW(I.b.u, e);
-  const float u_k = ur[k];
+  const float u_k = AT(I.b.u, e, k);
   const float u_m1 = lane_off(u_k, k, -1), u_m2 = lane_off(u_k, k, -2), u_p1 = lane_off(u_k, k, 1);
-  const float rw0 = ROW(I.b.c_rw, c0)[k], rw1 = ROW(I.b.c_rw, c1)[k];
-  const float pve = ROW(I.b.pv_edge, e)[k];
+  const float rw0 = AT(I.b.c_rw, c0, k), rw1 = AT(I.b.c_rw, c1, k);
+  const float pve = AT(I.b.pv_edge, e, k);
   float un[NX], pvn[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) {
-    un[s] = ROW(I.b.u, nb[s])[k];
-    pvn[s] = ROW(I.b.pv_edge, nb[s])[k];
+    un[s] = AT(I.b.u, nb[s], k);
+    pvn[s] = AT(I.b.pv_edge, nb[s], k);
   }
-  const float rho0 = ROW(I.b.c_rho, c0)[k], rho1 = ROW(I.b.c_rho, c1)[k];
-  const float ke0 = ROW(I.b.ke, c0)[k], ke1 = ROW(I.b.ke, c1)[k];
-  const float hd0 = ROW(I.b.hdiv, c0)[k], hd1 = ROW(I.b.hdiv, c1)[k];
-  const float eru = ROW(I.b.euler_ru, e)[k];
-  const float pru = I.k.physics ? ROW(I.b.phys_ru, e)[k] : 0.0f;
-  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k], rdzw = I.z.rdzw[k];
-  const float ui = I.z.uinit[k], vi = I.z.vinit[k];
+  const float rho0 = AT(I.b.c_rho, c0, k), rho1 = AT(I.b.c_rho, c1, k);
+  const float ke0 = AT(I.b.ke, c0, k), ke1 = AT(I.b.ke, c1, k);
+  const float hd0 = AT(I.b.hdiv, c0, k), hd1 = AT(I.b.hdiv, c1, k);
+  const float eru = AT(I.b.euler_ru, e, k);
+  const float pru = I.k.physics ? AT(I.b.phys_ru, e, k) : 0.0f;
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k), rdzw = AT(I.z.rdzw, 0, k);
+  const float ui = AT(I.z.uinit, 0, k), vi = AT(I.z.vinit, 0, k);
   const float rwe = 0.5f * (rw0 + rw1);
   float wduz = 0.0f;
   if (k == 1 || k == L - 1) {
@@ -830,7 +831,7 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   const float horiz = rho_edge * (qq - (ke1 - ke0) * I.m.invdc[e]);
   const float t = (initial + horiz) - (u_k * 0.5f) * (hd0 + hd1);
   const float tr = I.k.physics ? (t + eru) + pru : t + eru;
-  if (k < L) ROW(I.b.tend_ru, e)[k] = tr;
+  if (k < L) AT(I.b.tend_ru, e, k) = tr;
   return tr;
 }
 
@@ -854,8 +855,8 @@ KB k_tend_mom_ac(const Instance I, float dts, int store_avg) {
   const float tr = tend_mom_edge<NE>(I, e, k);
   if (k < I.n.L) {
     const float p = dts * tr;
-    ROW(I.b.ru_p, e)[k] = p;
-    if (store_avg) ROW(I.b.ru_avg, e)[k] = p;
+    AT(I.b.ru_p, e, k) = p;
+    if (store_avg) AT(I.b.ru_avg, e, k) = p;
   }
 }
 
@@ -879,18 +880,18 @@ __device__ __forceinline__ EdgeFluxIn<NE> tend_flux_load(const Instance &I, int 
   for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
 #pragma unroll
   for (int s = 0; s < NX; ++s) {
-    b.thn[s] = ROW(I.b.theta, ac[s])[k];
-    b.wn[s] = ROW(I.b.w, ac[s])[k];
+    b.thn[s] = AT(I.b.theta, ac[s], k);
+    b.wn[s] = AT(I.b.w, ac[s], k);
   }
-  b.rue = ROW(I.b.c_ru, e)[k];
+  b.rue = AT(I.b.c_ru, e, k);
   b.sru = b.sth0 = b.sth1 = 0.0f;
   if (CORR) {
-    b.sru = ROW(I.b.s_ru, e)[k];
-    b.sth0 = ROW(I.b.s_theta, c0)[k];
-    b.sth1 = ROW(I.b.s_theta, c1)[k];
+    b.sru = AT(I.b.s_ru, e, k);
+    b.sth0 = AT(I.b.s_theta, c0, k);
+    b.sth1 = AT(I.b.s_theta, c1, k);
   }
-  b.fzm = I.z.fzm[k];
-  b.fzp = I.z.fzp[k];
+  b.fzm = AT(I.z.fzm, 0, k);
+  b.fzp = AT(I.z.fzp, 0, k);
   b.na = I.m.nadv[e];
   slots(b.av, I.m.adv + e * NX);
   slots(b.a3v, I.m.adv3 + e * NX);
@@ -924,9 +925,9 @@ __device__ __forceinline__ EdgeFlux tend_flux_values(const Instance &I, int e, i
 template <bool CORR>
 __device__ __forceinline__ void store_edge_flux(const Instance &I, int e, int k, EdgeFlux f) {
   if (k < I.n.L) {
-    ROW(I.b.fth, e)[k] = f.th;
-    if (CORR) ROW(I.b.fcorr, e)[k] = f.corr;
-    if (k >= 1) ROW(I.b.fw, e)[k] = f.w;
+    AT(I.b.fth, e, k) = f.th;
+    if (CORR) AT(I.b.fcorr, e, k) = f.corr;
+    if (k >= 1) AT(I.b.fw, e, k) = f.w;
   }
 }
 
@@ -954,24 +955,23 @@ KB k_tend_cell(const Instance I, int stage) {
   float fth[NE], fco[NE], fwv[NE], tru[NE], zb[NE], zb3[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    fth[s] = ROW(I.b.fth, e[s])[k];
-    fwv[s] = ROW(I.b.fw, e[s])[k];
-    tru[s] = ROW(I.b.tend_ru, e[s])[k];
-    const size_t zi = ((size_t)c * NE + s) * HX_ROW + k;
-    zb[s] = I.z.zb[zi];
-    zb3[s] = I.z.zb3[zi];
+    fth[s] = AT(I.b.fth, e[s], k);
+    fwv[s] = AT(I.b.fw, e[s], k);
+    tru[s] = AT(I.b.tend_ru, e[s], k);
+    zb[s] = AT(I.z.zb, c * NE + s, k);
+    zb3[s] = AT(I.z.zb3, c * NE + s, k);
   }
   if (stage > 1) {
 #pragma unroll
-    for (int s = 0; s < NE; ++s) fco[s] = ROW(I.b.fcorr, e[s])[k];
+    for (int s = 0; s < NE; ++s) fco[s] = AT(I.b.fcorr, e[s], k);
   }
   // the column rows, read once; vertical neighbours by shuffles
-  const float th_k = ROW(I.b.theta, c)[k], ths_k = ROW(I.b.s_theta, c)[k];
-  const float rwc_k = ROW(I.b.c_rw, c)[k], rws_k = ROW(I.b.s_rw, c)[k], ww_k = ROW(I.b.w, c)[k];
-  const float zz_k = ROW(I.z.zz, c)[k], erw_k = ROW(I.b.euler_rw, c)[k];
-  const float eth_k = I.k.mixing ? ROW(I.b.euler_theta, c)[k] : 0.0f;
-  const float prt_k = I.k.physics ? ROW(I.b.phys_rt, c)[k] : 0.0f;
-  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
+  const float th_k = AT(I.b.theta, c, k), ths_k = AT(I.b.s_theta, c, k);
+  const float rwc_k = AT(I.b.c_rw, c, k), rws_k = AT(I.b.s_rw, c, k), ww_k = AT(I.b.w, c, k);
+  const float zz_k = AT(I.z.zz, c, k), erw_k = AT(I.b.euler_rw, c, k);
+  const float eth_k = I.k.mixing ? AT(I.b.euler_theta, c, k) : 0.0f;
+  const float prt_k = I.k.physics ? AT(I.b.phys_rt, c, k) : 0.0f;
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
   const float th_m1 = lane_off(th_k, k, -1), th_m2 = lane_off(th_k, k, -2), th_p1 = lane_off(th_k, k, 1);
   const float ths_m1 = lane_off(ths_k, k, -1), rwc_m1 = lane_off(rwc_k, k, -1);
   const float ww_m1 = lane_off(ww_k, k, -1), ww_m2 = lane_off(ww_k, k, -2), ww_p1 = lane_off(ww_k, k, 1);
@@ -1005,7 +1005,7 @@ KB k_tend_cell(const Instance I, int stage) {
       if (s < n) r = r - sg[s] * fco[s];
   }
   r = r * ia;
-  const float trt = r - I.z.rdzw[k] * (vf_p - vf);
+  const float trt = r - AT(I.z.rdzw, 0, k) * (vf_p - vf);
   float tru_m[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) tru_m[s] = up1(tru[s]);
@@ -1016,7 +1016,7 @@ KB k_tend_cell(const Instance I, int stage) {
     for (int s = 0; s < NE; ++s)
       if (s < n) q = q - sg[s] * fwv[s];
     q = q * ia;
-    q = q - I.z.rdzu[k] * (wf_p - wf);
+    q = q - AT(I.z.rdzu, 0, k) * (wf_p - wf);
     q = q + erw_k;
 #pragma unroll
     for (int s = 0; s < NE; ++s)
@@ -1029,18 +1029,18 @@ KB k_tend_cell(const Instance I, int stage) {
   }
   if (k < L) {
     const float tmix = I.k.mixing ? trt + eth_k : trt;
-    ROW(I.b.tend_rt, c)[k] = I.k.physics ? tmix + prt_k : tmix;
+    AT(I.b.tend_rt, c, k) = I.k.physics ? tmix + prt_k : tmix;
   }
-  if (k <= L) ROW(I.b.tend_rw, c)[k] = rw_t;
+  if (k <= L) AT(I.b.tend_rw, c, k) = rw_t;
 }
 
 // ----------------------------------------------------------- acoustic substep
 // ru_p + divergence-damping increment of edge e at level k (mixing.py:851-945)
 __device__ __forceinline__ float damped_ru_p(const Instance &I, int e, int k, float rp, float coefficient) {
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  const float d0 = ROW(I.b.rtheta_pp, c0)[k] - ROW(I.b.rtheta_pp_old, c0)[k];
-  const float d1 = ROW(I.b.rtheta_pp, c1)[k] - ROW(I.b.rtheta_pp_old, c1)[k];
-  const float den = ROW(I.b.s_theta, c0)[k] + ROW(I.b.s_theta, c1)[k];
+  const float d0 = AT(I.b.rtheta_pp, c0, k) - AT(I.b.rtheta_pp_old, c0, k);
+  const float d1 = AT(I.b.rtheta_pp, c1, k) - AT(I.b.rtheta_pp_old, c1, k);
+  const float den = AT(I.b.s_theta, c0, k) + AT(I.b.s_theta, c1, k);
   return rp + ((coefficient * (d0 - d1)) * (1.0f - 0.0f)) / den;
 }
 
@@ -1054,25 +1054,24 @@ KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_c
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
-  float *rp = ROW(I.b.ru_p, e), *ra = ROW(I.b.ru_avg, e);
-  const float tr = ROW(I.b.tend_ru, e)[k];
+  const float tr = AT(I.b.tend_ru, e, k);
   if (first) {
     const float p = dts * tr;
-    rp[k] = p;
-    ra[k] = p;
+    AT(I.b.ru_p, e, k) = p;
+    AT(I.b.ru_avg, e, k) = p;
     return;
   }
-  const float rp_raw = rp[k];
-  const float ra_old = avg_from_rp ? rp_raw : ra[k];
+  const float rp_raw = AT(I.b.ru_p, e, k);
+  const float ra_old = avg_from_rp ? rp_raw : AT(I.b.ru_avg, e, k);
   const float rp_in = damp_prev ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  float pg = ((ROW(I.b.rtheta_pp, c1)[k] - ROW(I.b.rtheta_pp, c0)[k]) * I.m.invdc[e]) /
-             (0.5f * (ROW(I.z.zz, c1)[k] + ROW(I.z.zz, c0)[k]));
-  pg = (((ROW(I.b.cqu, e)[k] * 0.5f) * I.k.c2) * (ROW(I.b.s_exner, c0)[k] + ROW(I.b.s_exner, c1)[k])) * pg;
-  pg = pg + ((0.5f * ROW(I.z.zxu, e)[k]) * I.k.gravity) * (ROW(I.b.rho_pp, c0)[k] + ROW(I.b.rho_pp, c1)[k]);
+  float pg = ((AT(I.b.rtheta_pp, c1, k) - AT(I.b.rtheta_pp, c0, k)) * I.m.invdc[e]) /
+             (0.5f * (AT(I.z.zz, c1, k) + AT(I.z.zz, c0, k)));
+  pg = (((AT(I.b.cqu, e, k) * 0.5f) * I.k.c2) * (AT(I.b.s_exner, c0, k) + AT(I.b.s_exner, c1, k))) * pg;
+  pg = pg + ((0.5f * AT(I.z.zxu, e, k)) * I.k.gravity) * (AT(I.b.rho_pp, c0, k) + AT(I.b.rho_pp, c1, k));
   const float p = rp_in + dts * (tr - pg);
-  rp[k] = p;
-  ra[k] = ra_old + p;
+  AT(I.b.ru_p, e, k) = p;
+  AT(I.b.ru_avg, e, k) = ra_old + p;
 }
 
 // FIRST: the first substep of a stage (the perturbation fields start at
@@ -1101,24 +1100,24 @@ KBCOL k_ac_cell(const Instance I, float dts) {
   float rup[NE], sto[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    rup[s] = ROW(I.b.ru_p, e[s])[k];
-    sto[s] = ROW(I.b.s_theta, o[s])[k];
+    rup[s] = AT(I.b.ru_p, e[s], k);
+    sto[s] = AT(I.b.s_theta, o[s], k);
   }
-  const float sthc = ROW(I.b.s_theta, c)[k];
+  const float sthc = AT(I.b.s_theta, c, k);
   float rpp = 0.0f, rtpp = 0.0f, rwp = 0.0f, wwa = 0.0f;
   if (!FIRST) {
-    rpp = ROW(I.b.rho_pp, c)[k];
-    rtpp = ROW(I.b.rtheta_pp, c)[k];
-    rwp = ROW(I.b.rw_p, c)[k];
-    wwa = ROW(I.b.ww_avg, c)[k];
+    rpp = AT(I.b.rho_pp, c, k);
+    rtpp = AT(I.b.rtheta_pp, c, k);
+    rwp = AT(I.b.rw_p, c, k);
+    wwa = AT(I.b.ww_avg, c, k);
   }
-  const float coftz = ROW(I.b.coftz, c)[k], cofwt = ROW(I.b.cofwt, c)[k];
-  const float cofwz = ROW(I.b.cofwz, c)[k], cofwr = ROW(I.b.cofwr, c)[k];
-  const float trho = ROW(I.b.tend_rho, c)[k], trt = ROW(I.b.tend_rt, c)[k], trw = ROW(I.b.tend_rw, c)[k];
-  const float zk = ROW(I.z.zz, c)[k];
-  const float a = ROW(I.b.atri, c)[k], al = ROW(I.b.alpha, c)[k], ga = ROW(I.b.gamma, c)[k];
-  const float srw = ROW(I.b.s_rw, c)[k], crw = ROW(I.b.c_rw, c)[k], damp = ROW(I.z.dss, c)[k];
-  const float crho = ROW(I.b.c_rho, c)[k], wk = ROW(I.b.w, c)[k];
+  const float coftz = AT(I.b.coftz, c, k), cofwt = AT(I.b.cofwt, c, k);
+  const float cofwz = AT(I.b.cofwz, c, k), cofwr = AT(I.b.cofwr, c, k);
+  const float trho = AT(I.b.tend_rho, c, k), trt = AT(I.b.tend_rt, c, k), trw = AT(I.b.tend_rw, c, k);
+  const float zk = AT(I.z.zz, c, k);
+  const float a = AT(I.b.atri, c, k), al = AT(I.b.alpha, c, k), ga = AT(I.b.gamma, c, k);
+  const float srw = AT(I.b.s_rw, c, k), crw = AT(I.b.c_rw, c, k), damp = AT(I.z.dss, c, k);
+  const float crho = AT(I.b.c_rho, c, k), wk = AT(I.b.w, c, k);
   {
     const float *const lev[7] = {I.z.rdzw, I.z.ewm, I.z.ewp, I.z.etp, I.z.etm, I.z.fzm, I.z.fzp};
     stage_levels<7>(s_lev, lev, w, k);
@@ -1194,13 +1193,13 @@ KBCOL k_ac_cell(const Instance I, float dts) {
   const float rpp_new = rs - (dts * rdzw) * (ewp_p * col_p - ewp * col);
   const float rtpp_new = ts - (dts * rdzw) * ((ewp_p * coftz_p) * col_p - (ewp * coftz) * col);
   if (live && k <= L) {
-    ROW(I.b.rw_p, c)[k] = col;
-    ROW(I.b.ww_avg, c)[k] = wwa;
+    AT(I.b.rw_p, c, k) = col;
+    AT(I.b.ww_avg, c, k) = wwa;
   }
   if (live && k < L) {
-    ROW(I.b.rho_pp, c)[k] = rpp_new;
-    ROW(I.b.rtheta_pp, c)[k] = rtpp_new;
-    ROW(I.b.rtheta_pp_old, c)[k] = rtpp_old;
+    AT(I.b.rho_pp, c, k) = rpp_new;
+    AT(I.b.rtheta_pp, c, k) = rtpp_new;
+    AT(I.b.rtheta_pp_old, c, k) = rtpp_old;
   }
 }
 
@@ -1209,8 +1208,7 @@ KB k_divdamp(const Instance I, float coefficient) {
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
-  float *rp = ROW(I.b.ru_p, e);
-  rp[k] = damped_ru_p(I, e, k, rp[k], coefficient);
+  AT(I.b.ru_p, e, k) = damped_ru_p(I, e, k, AT(I.b.ru_p, e, k), coefficient);
 }
 
 // --------------------------------------------------------------------- recovery
@@ -1220,9 +1218,9 @@ template <int NE, class RU>
 __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int k, float rho_k, float rw_k, RU ru_row,
                                                   float *hdiv_out = nullptr) {
   const int L = I.n.L;
-  const float zk = ROW(I.z.zz, c)[k];
+  const float zk = AT(I.z.zz, c, k);
   const float zkm = lane_off(zk, k, -1);
-  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k];
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
   const int n = I.m.neoc[c];
   float sg[NE];
   slots(sg, I.m.csign + c * NE);
@@ -1235,9 +1233,8 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     ru[s] = ru_row(e[s]);
-    const size_t zi = ((size_t)c * NE + s) * HX_ROW + k;
-    zb[s] = I.z.zb[zi];
-    zb3[s] = I.z.zb3[zi];
+    zb[s] = AT(I.z.zb, c * NE + s, k);
+    zb3[s] = AT(I.z.zb3, c * NE + s, k);
   }
   if (hdiv_out) {
     // mass-flux divergence of the recovered ru (dynamics.py:86-121), next stage's hdiv
@@ -1287,15 +1284,15 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps) {
   const int k = lane();
   const int L = I.n.L;
   const bool lay = k < L, itf = k <= L, inner = k >= 1 && k < L;
-  const float srp = ROW(I.b.s_rho_p, c)[k], rpp = ROW(I.b.rho_pp, c)[k];
-  const float srtp = ROW(I.b.s_rtheta_p, c)[k], rtpp = ROW(I.b.rtheta_pp, c)[k];
-  const float rb = ROW(I.z.rb, c)[k], rtb = ROW(I.z.rtb, c)[k];
-  const float srw = ROW(I.b.s_rw, c)[k], rwp = ROW(I.b.rw_p, c)[k], wwa = ROW(I.b.ww_avg, c)[k];
+  const float srp = AT(I.b.s_rho_p, c, k), rpp = AT(I.b.rho_pp, c, k);
+  const float srtp = AT(I.b.s_rtheta_p, c, k), rtpp = AT(I.b.rtheta_pp, c, k);
+  const float rb = AT(I.z.rb, c, k), rtb = AT(I.z.rtb, c, k);
+  const float srw = AT(I.b.s_rw, c, k), rwp = AT(I.b.rw_p, c, k), wwa = AT(I.b.ww_avg, c, k);
   float zk = 0.0f, exb = 0.0f, spw = 0.0f;
   if (S3) {
-    zk = ROW(I.z.zz, c)[k];
-    exb = ROW(I.z.exb, c)[k];
-    spw = ROW(I.b.split_w, c)[k];
+    zk = AT(I.z.zz, c, k);
+    exb = AT(I.z.exb, c, k);
+    spw = AT(I.b.split_w, c, k);
   }
   const float dp = lay ? srp + rpp : 0.0f;
   const float rp = lay ? srtp + rtpp : 0.0f;
@@ -1304,7 +1301,7 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps) {
   const float rwv = inner ? srw + rwp : 0.0f;
   const float *cru = I.b.c_ru;
   float hd = 0.0f;
-  const float wv = recover_w_column<NE>(I, c, k, rho, rwv, [&](int e) { return ROW(cru, e)[k]; }, &hd);
+  const float wv = recover_w_column<NE>(I, c, k, rho, rwv, [&](int e) { return AT(cru, e, k); }, &hd);
   int bad = (lay && (!(rho > 0.0f) || !(rtv > 0.0f))) ? 1 : 0;
   const float th = rtv / rho;
   float ex = 0.0f, ppv = 0.0f;
@@ -1323,22 +1320,22 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps) {
     split = sub == I.k.splits ? acc * I.k.split_inv : acc;
   }
   if (lay) {
-    ROW(I.b.rho_p, c)[k] = dp;
-    ROW(I.b.rtheta_p, c)[k] = rp;
-    ROW(I.b.c_rho, c)[k] = rho;
-    ROW(I.b.c_rt, c)[k] = rtv;
-    ROW(I.b.theta, c)[k] = th;
+    AT(I.b.rho_p, c, k) = dp;
+    AT(I.b.rtheta_p, c, k) = rp;
+    AT(I.b.c_rho, c, k) = rho;
+    AT(I.b.c_rt, c, k) = rtv;
+    AT(I.b.theta, c, k) = th;
     if (S3) {
-      ROW(I.b.exner, c)[k] = ex;
-      ROW(I.b.pp, c)[k] = ppv;
+      AT(I.b.exner, c, k) = ex;
+      AT(I.b.pp, c, k) = ppv;
     }
-    ROW(I.b.hdiv, c)[k] = hd;
+    AT(I.b.hdiv, c, k) = hd;
   }
   if (itf) {
-    ROW(I.b.c_rw, c)[k] = rwv;
-    if (I.k.store_flux) ROW(I.b.flux_w, c)[k] = fw;
-    if (S3) ROW(I.b.split_w, c)[k] = split;
-    ROW(I.b.w, c)[k] = wv;
+    AT(I.b.c_rw, c, k) = rwv;
+    if (I.k.store_flux) AT(I.b.flux_w, c, k) = fw;
+    if (S3) AT(I.b.split_w, c, k) = split;
+    AT(I.b.w, c, k) = wv;
   }
   if (bad) set_flag(I.b.flag);
 }
@@ -1354,12 +1351,12 @@ KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef) {
   const int k = lane();
   if (k >= I.n.L) return;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  const float sru = ROW(I.b.s_ru, e)[k];
-  const float rp_raw = ROW(I.b.ru_p, e)[k];
-  const float avg = AVGRP ? rp_raw : ROW(I.b.ru_avg, e)[k];
-  const float spu = S3 ? ROW(I.b.split_u, e)[k] : 0.0f;
-  const float r0 = (ROW(I.b.s_rho_p, c0)[k] + ROW(I.b.rho_pp, c0)[k]) + ROW(I.z.rb, c0)[k];
-  const float r1 = (ROW(I.b.s_rho_p, c1)[k] + ROW(I.b.rho_pp, c1)[k]) + ROW(I.z.rb, c1)[k];
+  const float sru = AT(I.b.s_ru, e, k);
+  const float rp_raw = AT(I.b.ru_p, e, k);
+  const float avg = AVGRP ? rp_raw : AT(I.b.ru_avg, e, k);
+  const float spu = S3 ? AT(I.b.split_u, e, k) : 0.0f;
+  const float r0 = (AT(I.b.s_rho_p, c0, k) + AT(I.b.rho_pp, c0, k)) + AT(I.z.rb, c0, k);
+  const float r1 = (AT(I.b.s_rho_p, c1, k) + AT(I.b.rho_pp, c1, k)) + AT(I.z.rb, c1, k);
   const float rp = DAMP ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
   const float ru = sru + rp;
   const float fu = sru + avg * inv_steps;
@@ -1369,11 +1366,11 @@ KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef) {
     split = sub == I.k.splits ? acc * I.k.split_inv : acc;
   }
   const float uu = (2.0f * ru) / (r0 + r1);
-  if (DAMP) ROW(I.b.ru_p, e)[k] = rp;
-  ROW(I.b.c_ru, e)[k] = ru;
-  if (I.k.store_flux) ROW(I.b.flux_u, e)[k] = fu;
-  if (S3) ROW(I.b.split_u, e)[k] = split;
-  ROW(I.b.u, e)[k] = uu;
+  if (DAMP) AT(I.b.ru_p, e, k) = rp;
+  AT(I.b.c_ru, e, k) = ru;
+  if (I.k.store_flux) AT(I.b.flux_u, e, k) = fu;
+  if (S3) AT(I.b.split_u, e, k) = split;
+  AT(I.b.u, e, k) = uu;
 }
 
 // rebuild of the saved diagnostics from a prognostic state (driver.py:2655-2700)
@@ -1385,21 +1382,21 @@ KB k_rebuild_cell(const Instance I) {
   const int L = I.n.L;
   float rho = 1.0f;
   if (k < L) {
-    rho = ROW(I.b.rho, c)[k];
-    const float rt = ROW(I.b.rt, c)[k];
-    const float zk = ROW(I.z.zz, c)[k];
+    rho = AT(I.b.rho, c, k);
+    const float rt = AT(I.b.rt, c, k);
+    const float zk = AT(I.z.zz, c, k);
     const float ex = hexdyn_powf_eval((zk * I.k.cfac) * rt, I.k.rcv);
-    const float rtp = rt - ROW(I.z.rtb, c)[k];
-    ROW(I.b.theta, c)[k] = rt / rho;
-    ROW(I.b.exner, c)[k] = ex;
-    ROW(I.b.rho_p, c)[k] = rho - ROW(I.z.rb, c)[k];
-    ROW(I.b.rtheta_p, c)[k] = rtp;
-    ROW(I.b.pp, c)[k] = (zk * I.k.rgas) * (ex * rtp + ROW(I.z.rtb, c)[k] * (ex - ROW(I.z.exb, c)[k]));
+    const float rtp = rt - AT(I.z.rtb, c, k);
+    AT(I.b.theta, c, k) = rt / rho;
+    AT(I.b.exner, c, k) = ex;
+    AT(I.b.rho_p, c, k) = rho - AT(I.z.rb, c, k);
+    AT(I.b.rtheta_p, c, k) = rtp;
+    AT(I.b.pp, c, k) = (zk * I.k.rgas) * (ex * rtp + AT(I.z.rtb, c, k) * (ex - AT(I.z.exb, c, k)));
   }
-  const float rwk = ROW(I.b.rw, c)[k];
+  const float rwk = AT(I.b.rw, c, k);
   const float *ru = I.b.ru;
-  const float wv = recover_w_column<NE>(I, c, k, rho, rwk, [&](int e) { return ROW(ru, e)[k]; });
-  if (k <= L) ROW(I.b.w, c)[k] = wv;
+  const float wv = recover_w_column<NE>(I, c, k, rho, rwk, [&](int e) { return AT(ru, e, k); });
+  if (k <= L) AT(I.b.w, c, k) = wv;
 }
 
 KB k_rebuild_edge(const Instance I) {
@@ -1408,7 +1405,7 @@ KB k_rebuild_edge(const Instance I) {
   const int k = lane();
   if (k >= I.n.L) return;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  ROW(I.b.u, e)[k] = (2.0f * ROW(I.b.ru, e)[k]) / (ROW(I.b.rho, c0)[k] + ROW(I.b.rho, c1)[k]);
+  AT(I.b.u, e, k) = (2.0f * AT(I.b.ru, e, k)) / (AT(I.b.rho, c0, k) + AT(I.b.rho, c1, k));
 }
 
 // ------------------------------------------------------------ scalar transport
@@ -1435,7 +1432,7 @@ KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__
   int ac[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
-  const float sv = cs1(ROW(I.b.split_u, e)[k]);
+  const float sv = cs1(AT(I.b.split_u, e, k));
   const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
   float av[NX], a3v[NX], wt[NX];
@@ -1445,12 +1442,12 @@ KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__
   for (int s = 0; s < NX; ++s) wt[s] = av[s] + (coef * sv) * a3v[s];
   for (int sp = 0; sp < NT; sp += 2) {
     const int sq = sp + 1 < NT ? sp + 1 : sp;
-    const float *qa = qs + sp * cplane, *qb = qs + sq * cplane;
+    const int pa = sp * I.n.C, pb = sq * I.n.C;  // species rows start at s * C
     float va[NX], vb[NX];
 #pragma unroll
     for (int s = 0; s < NX; ++s) {
-      va[s] = ROW(qa, ac[s])[k];
-      vb[s] = ROW(qb, ac[s])[k];
+      va[s] = AT(qs, pa + ac[s], k);
+      vb[s] = AT(qs, pb + ac[s], k);
     }
     float ra = 0.0f, rb = 0.0f;
 #pragma unroll
@@ -1460,8 +1457,8 @@ KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__
         rb = rb + wt[s] * vb[s];
       }
     if (k < I.n.L) {
-      ROW(ev + sp * eplane, e)[k] = ra;
-      if (sq != sp) ROW(ev + sq * eplane, e)[k] = rb;
+      AT(ev, sp * I.n.E + e, k) = ra;
+      if (sq != sp) AT(ev, sq * I.n.E + e, k) = rb;
     }
   }
 }
@@ -1482,11 +1479,11 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
   float uh[NE];
 #pragma unroll
-  for (int s = 0; s < NE; ++s) uh[s] = ROW(I.b.split_u, e[s])[k];
-  const float sw = ROW(I.b.split_w, c)[k];
-  const float orho = ROW(I.b.rho, c)[k];
-  const float target = (1.0f - wn) * orho + wn * ROW(I.b.c_rho, c)[k];
-  const float fzm = I.z.fzm[k], fzp = I.z.fzp[k], rdzw = I.z.rdzw[k], ia = I.m.invarea[c];
+  for (int s = 0; s < NE; ++s) uh[s] = AT(I.b.split_u, e[s], k);
+  const float sw = AT(I.b.split_w, c, k);
+  const float orho = AT(I.b.rho, c, k);
+  const float target = (1.0f - wn) * orho + wn * AT(I.b.c_rho, c, k);
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k), rdzw = AT(I.z.rdzw, 0, k), ia = I.m.invarea[c];
   // software-pipelined over species: the next species' rows are requested
   // before the current one is combined (clamped index, in bounds)
   struct Sp {
@@ -1494,10 +1491,10 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
   };
   auto load_sp = [&](int sp) {
     Sp a;
-    a.qk = ROW(qs + sp * cplane, c)[k];
+    a.qk = AT(qs, sp * I.n.C + c, k);
 #pragma unroll
-    for (int s = 0; s < NE; ++s) a.evv[s] = ROW(ev + sp * eplane, e[s])[k];
-    a.qo = ROW(I.b.q + sp * cplane, c)[k];
+    for (int s = 0; s < NE; ++s) a.evv[s] = AT(ev, sp * I.n.E + e[s], k);
+    a.qo = AT(I.b.q, sp * I.n.C + c, k);
     return a;
   };
   auto species = [&](int sp, const Sp &a) {
@@ -1514,7 +1511,7 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
     tc = tc * ia + 0.0f;
     if (k < L) {
       const float num = qo * orho + dt * (tc - rdzw * (vf_p - vf));
-      ROW(out + sp * cplane, c)[k] = num * (1.0f / target);
+      AT(out, sp * I.n.C + c, k) = num * (1.0f / target);
     }
   };
   Sp cur = load_sp(0);
@@ -1540,15 +1537,16 @@ KB k_fct_target(const Instance I, float dt) {
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
 #pragma unroll
-  for (int s = 0; s < NE; ++s) uh[s] = ROW(I.b.split_u, e[s])[k];
+  for (int s = 0; s < NE; ++s) uh[s] = AT(I.b.split_u, e[s], k);
   float hd = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) hd = hd + (-(uh[s] * sd[s])) * ia;  // ((-csign)*uh)*dv, csign = +-1
-  const float *wa = ROW(I.b.split_w, c);
+  const float wa_k = AT(I.b.split_w, c, k);
+  const float wa_p = lane_off(wa_k, k, 1);
   if (k < L) {
-    const float t = ROW(I.b.rho, c)[k] + dt * (hd - I.z.rdzw[k] * (wa[k + 1] - wa[k]));
-    ROW(I.b.target, c)[k] = t;
+    const float t = AT(I.b.rho, c, k) + dt * (hd - AT(I.z.rdzw, 0, k) * (wa_p - wa_k));
+    AT(I.b.target, c, k) = t;
     if (!(t > 0.0f)) set_flag(I.b.flag);
   }
 }
@@ -1565,7 +1563,7 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
 #pragma unroll
   for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
   const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
-  const float uu = ROW(I.b.split_u, e)[k];
+  const float uu = AT(I.b.split_u, e, k);
   const float sv = cs1(uu);
   const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
@@ -1580,12 +1578,12 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
     float va[NX], vb[NX];
 #pragma unroll
     for (int s = 0; s < NX; ++s) {
-      va[s] = ROW(qs + sp * cplane, ac[s])[k];
-      vb[s] = ROW(qs + sq * cplane, ac[s])[k];
+      va[s] = AT(qs, sp * I.n.C + ac[s], k);
+      vb[s] = AT(qs, sq * I.n.C + ac[s], k);
     }
-    const float *qoa = I.b.q + sp * cplane, *qob = I.b.q + sq * cplane;
-    const float qa1 = ROW(qoa, c1)[k] + 0.0f, qa2 = ROW(qoa, c2)[k] + 0.0f;
-    const float qb1 = ROW(qob, c1)[k] + 0.0f, qb2 = ROW(qob, c2)[k] + 0.0f;
+    const int pa = sp * I.n.C, pb = sq * I.n.C;
+    const float qa1 = AT(I.b.q, pa + c1, k) + 0.0f, qa2 = AT(I.b.q, pa + c2, k) + 0.0f;
+    const float qb1 = AT(I.b.q, pb + c1, k) + 0.0f, qb2 = AT(I.b.q, pb + c2, k) + 0.0f;
     float ra, rb;
     if (na == 10) {
       // ten-cell stencils associate from the first product (transport.py n_adv == 10 branch)
@@ -1611,11 +1609,11 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
     const float upa = dvdt * (max0(uu) * qa1 + min0(uu) * qa2);
     const float upb = dvdt * (max0(uu) * qb1 + min0(uu) * qb2);
     if (k < I.n.L) {
-      ROW(I.b.fct_upw + sp * eplane, e)[k] = upa;
-      ROW(I.b.fct_hres + sp * eplane, e)[k] = dt * ra - upa;
+      AT(I.b.fct_upw, sp * I.n.E + e, k) = upa;
+      AT(I.b.fct_hres, sp * I.n.E + e, k) = dt * ra - upa;
       if (sq != sp) {
-        ROW(I.b.fct_upw + sq * eplane, e)[k] = upb;
-        ROW(I.b.fct_hres + sq * eplane, e)[k] = dt * rb - upb;
+        AT(I.b.fct_upw, sq * I.n.E + e, k) = upb;
+        AT(I.b.fct_hres, sq * I.n.E + e, k) = dt * rb - upb;
       }
     }
   }
@@ -1637,23 +1635,23 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
     nb[s] = I.m.coc[c * NE + s];
     e[s] = I.m.eoc[c * NE + s];
   }
-  const float orho = ROW(I.b.rho, c)[k];
-  const float wk = ROW(I.b.split_w, c)[k];
-  const float rdzw = I.z.rdzw[k], fzm = I.z.fzm[k], fzp = I.z.fzp[k];
+  const float orho = AT(I.b.rho, c, k);
+  const float wk = AT(I.b.split_w, c, k);
+  const float rdzw = AT(I.z.rdzw, 0, k), fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
   const float ia = I.m.invarea[c];
-  const float target = ROW(I.b.target, c)[k];
+  const float target = AT(I.b.target, c, k);
   for (int sp = 0; sp < I.n.NT; ++sp) {
-    const float qs_k = ROW(qs + sp * cplane, c)[k];
+    const float qs_k = AT(qs, sp * I.n.C + c, k);
     const float qs_m2 = lane_off(qs_k, k, -2), qs_m1 = lane_off(qs_k, k, -1), qs_p1 = lane_off(qs_k, k, 1);
-    const float *qold = I.b.q + sp * cplane;
+    const int po = sp * I.n.C;
     float qn[NE], upw[NE], hres[NE];
 #pragma unroll
     for (int s = 0; s < NE; ++s) {
-      qn[s] = ROW(qold, nb[s])[k] + 0.0f;
-      upw[s] = ROW(I.b.fct_upw + sp * eplane, e[s])[k];
-      hres[s] = ROW(I.b.fct_hres + sp * eplane, e[s])[k];
+      qn[s] = AT(I.b.q, po + nb[s], k) + 0.0f;
+      upw[s] = AT(I.b.fct_upw, sp * I.n.E + e[s], k);
+      hres[s] = AT(I.b.fct_hres, sp * I.n.E + e[s], k);
     }
-    const float qo = ROW(qold, c)[k] + 0.0f;
+    const float qo = AT(I.b.q, po + c, k) + 0.0f;
     const float qo_m = up1(qo), qo_p = dn1(qo);
     float a, b;
     if (k == 0) {
@@ -1705,11 +1703,11 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
     f = (a * target - mass) / (sout - 1.0e-20f);
     sout = pmin(1.0f, max0(f));
     if (k < L) {
-      ROW(I.b.fct_mass + sp * cplane, c)[k] = mass;
-      ROW(I.b.fct_sin + sp * cplane, c)[k] = sin_;
-      ROW(I.b.fct_sout + sp * cplane, c)[k] = sout;
+      AT(I.b.fct_mass, sp * I.n.C + c, k) = mass;
+      AT(I.b.fct_sin, sp * I.n.C + c, k) = sin_;
+      AT(I.b.fct_sout, sp * I.n.C + c, k) = sout;
     }
-    if (k <= L) ROW(I.b.fct_vres + sp * cplane, c)[k] = vres;
+    if (k <= L) AT(I.b.fct_vres, sp * I.n.C + c, k) = vres;
   }
 }
 
@@ -1721,10 +1719,10 @@ KB k_fct_edge2(const Instance I) {
   const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
   for (int sp = 0; sp < I.n.NT; ++sp) {
-    const float *si = I.b.fct_sin + sp * cplane, *so = I.b.fct_sout + sp * cplane;
-    float *h = ROW(I.b.fct_hres + sp * eplane, e);
-    const float f = h[k];
-    h[k] = max0(f) * pmin(ROW(so, c1)[k], ROW(si, c2)[k]) + min0(f) * pmin(ROW(si, c1)[k], ROW(so, c2)[k]);
+    const int ps = sp * I.n.C;
+    const float f = AT(I.b.fct_hres, sp * I.n.E + e, k);
+    AT(I.b.fct_hres, sp * I.n.E + e, k) = max0(f) * pmin(AT(I.b.fct_sout, ps + c1, k), AT(I.b.fct_sin, ps + c2, k)) +
+                                              min0(f) * pmin(AT(I.b.fct_sin, ps + c1, k), AT(I.b.fct_sout, ps + c2, k));
   }
 }
 
@@ -1741,18 +1739,18 @@ KB k_fct_cell2(const Instance I, float *__restrict__ out) {
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-  const float ia = I.m.invarea[c], rdzw = I.z.rdzw[k];
-  const float target = ROW(I.b.target, c)[k];
+  const float ia = I.m.invarea[c], rdzw = AT(I.z.rdzw, 0, k);
+  const float target = AT(I.b.target, c, k);
   for (int sp = 0; sp < I.n.NT; ++sp) {
     float hres[NE];
 #pragma unroll
-    for (int s = 0; s < NE; ++s) hres[s] = ROW(I.b.fct_hres + sp * eplane, e[s])[k];
-    const float si = ROW(I.b.fct_sin + sp * cplane, c)[k], so = ROW(I.b.fct_sout + sp * cplane, c)[k];
+    for (int s = 0; s < NE; ++s) hres[s] = AT(I.b.fct_hres, sp * I.n.E + e[s], k);
+    const float si = AT(I.b.fct_sin, sp * I.n.C + c, k), so = AT(I.b.fct_sout, sp * I.n.C + c, k);
     const float si_m = up1(si), so_m = up1(so);
-    float v = ROW(I.b.fct_vres + sp * cplane, c)[k];
+    float v = AT(I.b.fct_vres, sp * I.n.C + c, k);
     if (k >= 1 && k < L) v = max0(v) * pmin(so_m, si) + min0(v) * pmin(so, si_m);
     const float v_p = dn1(v);
-    float mass = ROW(I.b.fct_mass + sp * cplane, c)[k];
+    float mass = AT(I.b.fct_mass, sp * I.n.C + c, k);
 #pragma unroll
     for (int s = 0; s < NE; ++s)
       if (s < n) {
@@ -1760,7 +1758,7 @@ KB k_fct_cell2(const Instance I, float *__restrict__ out) {
         mass = mass + (sgn * hres[s]) * ia;
       }
     mass = mass - rdzw * (v_p - v);
-    if (k < L) ROW(out + sp * cplane, c)[k] = max0(mass / target);
+    if (k < L) AT(out, sp * I.n.C + c, k) = max0(mass / target);
   }
 }
 
@@ -1768,8 +1766,7 @@ KB k_clamp(const Instance I, float *q, int rows) {
   const int r = wave_entity();
   if (r >= rows) return;
   const int k = lane();
-  float *p = ROW(q, r);
-  if (p[k] < 0.0f) p[k] = 0.0f;
+  if (AT(q, r, k) < 0.0f) AT(q, r, k) = 0.0f;
 }
 
 template <typename K, typename... A>
