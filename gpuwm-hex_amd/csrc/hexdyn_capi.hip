@@ -172,6 +172,7 @@ struct hexdyn_handle {
   double *int_partial = nullptr, *int_out = nullptr;          // integral partials and results
   // hipGraph-captured large steps, one executable per buffer-role layout at step start
   int use_graphs = 1;
+  int post_clamp = 1;  // end-of-step q < 0 -> 0 (0: scalars handed over unclamped)
   hipStream_t capture_stream = nullptr;
   struct Roles {  // host-side buffer roles and step flags
     Instance I;
@@ -784,7 +785,8 @@ bool transported(const hexdyn_handle *h) {
 
 // device work that closes a step (before the validation flag is read)
 int end_step_device(hexdyn_handle *h, hipStream_t s) {
-  if (transported(h) && h->I.n.NM > 0 && !h->cfg.config_monotonic)
+  // (monotonic transport already ends in max(0, .); the clamp keeps -0.0 and NaN bits)
+  if (h->post_clamp && transported(h) && h->I.n.NM > 0 && !h->cfg.config_monotonic)
     if (NS_CALL(h, clamp_scalars, h->I, h->I.b.q_stage, s)) return HEXDYN_E_HIP;
   return 0;
 }
@@ -1010,7 +1012,8 @@ int enqueue_step(hexdyn_handle *h, hipStream_t s) {
 // only pointers and Consts only 4-byte scalars, so neither has padding.
 std::string layout_key(const hexdyn_handle *h) {
   return std::string(reinterpret_cast<const char *>(&h->I.b), sizeof(Work)) +
-         std::string(reinterpret_cast<const char *>(&h->I.k), sizeof(Consts));
+         std::string(reinterpret_cast<const char *>(&h->I.k), sizeof(Consts)) +
+         std::string(1, h->post_clamp ? '1' : '0');
 }
 
 // Enqueue one step as a graph launch on s.  The first step from a given
@@ -1094,6 +1097,13 @@ int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *p
 int hexdyn_set_graphs(hexdyn_handle *h, int enable) {
   if (!h) return HEXDYN_E_INVALID;
   h->use_graphs = enable ? 1 : 0;
+  return 0;
+}
+
+int hexdyn_set_post_transport_clamp(hexdyn_handle *h, int enable) {
+  if (!h) return HEXDYN_E_INVALID;
+  if (h->in_step) return set_error(h, HEXDYN_E_INVALID, "hexdyn_set_post_transport_clamp inside a step");
+  h->post_clamp = enable ? 1 : 0;
   return 0;
 }
 

@@ -224,6 +224,12 @@ class HexDycoreDriver:
         _check(self._h, lib().hexdyn_set_physics_tendencies(self._h, ct.byref(view), ct.c_void_p(self.stream_ptr())),
                "hexdyn_set_physics_tendencies")
 
+    def set_post_transport_clamp(self, enable: bool) -> None:
+        """False hands the transported scalars to the caller unclamped (the reference's
+        physics_post_transport_callback path, cuda_driver.py:5299-5311)."""
+        _check(self._h, lib().hexdyn_set_post_transport_clamp(self._h, 1 if enable else 0),
+               "hexdyn_set_post_transport_clamp")
+
     def use_graphs(self, enable: bool) -> None:
         """Step through captured hipGraphs (default) or kernel by kernel."""
         _check(self._h, lib().hexdyn_set_graphs(self._h, 1 if enable else 0), "hexdyn_set_graphs")

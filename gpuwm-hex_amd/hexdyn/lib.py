@@ -67,6 +67,7 @@ def lib() -> ct.CDLL:
     L.hexdyn_integrals.argtypes = [vp, ct.POINTER(ct.c_double), ct.c_int, vp]
     L.hexdyn_set_graphs.argtypes = [vp, ct.c_int]
     L.hexdyn_set_physics_tendencies.argtypes = [vp, ct.POINTER(capi.HexdynPhysicsView), vp]
+    L.hexdyn_set_post_transport_clamp.argtypes = [vp, ct.c_int]
     if L.hexdyn_abi_version() != ABI_VERSION:
         raise HexdynLibraryMissing("libhexdyn.so ABI version mismatch")
     _lib = L
@@ -80,5 +81,5 @@ EXPORTED = [
     "hexdyn_acoustic_substep", "hexdyn_divergence_damping", "hexdyn_recover", "hexdyn_solve_diagnostics",
     "hexdyn_end_subcycle", "hexdyn_scalars", "hexdyn_end_step", "hexdyn_profile_enable", "hexdyn_profile_read",
     "hexdyn_profile_kernels", "hexdyn_powf", "hexdyn_integrals", "hexdyn_set_graphs",
-    "hexdyn_set_physics_tendencies",
+    "hexdyn_set_physics_tendencies", "hexdyn_set_post_transport_clamp",
 ]

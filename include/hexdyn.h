@@ -220,6 +220,15 @@ int hexdyn_set_graphs(hexdyn_handle *h, int enable);
 /* Install (view != NULL) or clear (view == NULL) held physics tendencies. */
 int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *view, void *stream);
 
+/* enable = 0 hands the transported scalars over unclamped: the end of a step
+ * skips the q < 0 -> 0 clamp, as the reference does when a
+ * physics_post_transport_callback takes the scalars (cuda_driver.py:5299-5311).
+ * A caller's microphysics then runs between hexdyn_scalars(3) and
+ * hexdyn_end_step on the candidate scalars (hexdyn_export_field /
+ * hexdyn_import_field of HEXDYN_F_SCALARS), before validation and publish.
+ * Default 1 (the clamp, atm_srk3's DO_PHYSICS branch). */
+int hexdyn_set_post_transport_clamp(hexdyn_handle *h, int enable);
+
 /* Phases of one large step, in the order hexdyn_step runs them. */
 int hexdyn_begin_step(hexdyn_handle *h, void *stream);
 int hexdyn_begin_subcycle(hexdyn_handle *h, int subcycle, void *stream);

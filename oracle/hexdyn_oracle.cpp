@@ -1158,9 +1158,11 @@ template <class T> struct Oracle {
     q_stage = out;
   }
 
+  int post_clamp = 1;  // 0: transported scalars handed over unclamped (cuda_driver.py:5299-5311)
+
   void end_step() {
     if (cfg.config_scalar_advection && cfg.config_split_dynamics_transport && NT > 0) q = q_stage;
-    if (NM > 0)
+    if (NM > 0 && post_clamp)
       for (auto &x : q)
         if (x < T(0)) x = T(0);
     rho = c_rho; rt = c_rt; ru = c_ru; rw = c_rw;
@@ -1341,6 +1343,11 @@ void *oracle_create(const hexdyn_mesh *m, const hexdyn_vertical *v, const hexdyn
 
 void oracle_destroy(void *h) { delete (Handle *)h; }
 
+
+int oracle_set_post_transport_clamp(void *h, int enable) {
+  DISPATCH(h, o.post_clamp = enable ? 1 : 0);
+  return 0;
+}
 
 int oracle_set_physics(void *h, const float *rho, const float *rho_u, const float *rho_theta) {
   DISPATCH(h, set_physics(o, rho, rho_u, rho_theta));

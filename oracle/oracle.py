@@ -61,6 +61,7 @@ def lib() -> ct.CDLL:
         _lib.oracle_powf.argtypes = [ct.c_float, ct.c_float]
         _lib.oracle_powf.restype = ct.c_float
         _lib.oracle_set_physics.argtypes = [ct.c_void_p] * 4
+        _lib.oracle_set_post_transport_clamp.argtypes = [ct.c_void_p, ct.c_int]
         _lib.oracle_model_time.argtypes = [ct.c_void_p]
         _lib.oracle_model_time.restype = ct.c_double
         _lib.oracle_schedule.argtypes = [ct.c_double, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p,
@@ -151,6 +152,10 @@ class Oracle:
         """Held physics tendencies (logical [L][entity]); all None clears them."""
         self._phys = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (rho, rho_u, rho_theta)]
         lib().oracle_set_physics(self.h, *[None if a is None else a.ctypes.data for a in self._phys])
+
+    def set_post_transport_clamp(self, enable: bool) -> None:
+        """False hands the transported scalars over unclamped (no q < 0 -> 0 at the step's end)."""
+        lib().oracle_set_post_transport_clamp(self.h, 1 if enable else 0)
 
     def model_time(self) -> float:
         return lib().oracle_model_time(self.h)

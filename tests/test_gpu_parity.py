@@ -343,6 +343,25 @@ def test_whole_step_variants_bitwise(get_case, variant):
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
 
 
+def test_post_transport_handoff_bitwise(get_case):
+    print("[gpu] unclamped post-transport handoff x1.642-moist, unlimited transport, 2 steps", flush=True)
+    import dataclasses
+
+    base = get_case("x1.642-moist")
+    case = get_case("x1.642-moist", config=dataclasses.replace(base.config, config_monotonic=False))
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    o.set_post_transport_clamp(False)
+    assert o.step(2) == 0
+    d = _driver(case)
+    d.set_post_transport_clamp(False)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    assert (ref["scalars"] < 0).any()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
 def test_case_read_from_an_mpas_init_file_steps_bitwise(get_case, tmp_path):
     print("[gpu] case from an MPAS init file x1.642-moist, 2 steps", flush=True)
     from hexdyn import mpas_io

@@ -310,3 +310,27 @@ def test_physics_tendencies_enter_the_budgets_fp64(get_case):
     mm, _ = totals(mom)
     assert abs(mm / mb - 1) < 1e-13
     assert np.abs(mom.export_field_f64("RU") - base.export_field_f64("RU")).max() > 0
+
+
+def test_post_transport_handoff_skips_the_clamp(get_case):
+    """With unlimited transport some mixing ratios go negative; the default
+    end-of-step clamp (atm_srk3's DO_PHYSICS branch) zeroes them, and the
+    post-transport handoff leaves them for the caller's microphysics
+    (cuda_driver.py:5299-5311).  Everything else is identical."""
+    import dataclasses
+
+    base = get_case("x1.642-moist")
+    case = get_case("x1.642-moist", config=dataclasses.replace(base.config, config_monotonic=False))
+    runs = {}
+    for clamp in (True, False):
+        o = Oracle(case, 32)
+        o.set_state(initial_state_arrays(case), rebuild=True)
+        o.set_post_transport_clamp(clamp)
+        assert o.step(1) == 0
+        runs[clamp] = o.get_state()
+    q_on, q_off = runs[True]["scalars"], runs[False]["scalars"]
+    assert (q_off < 0).any(), "unlimited transport produced no negative mixing ratio to hand over"
+    assert not (q_on < 0).any()
+    assert np.array_equal(np.where(q_off < 0, np.float32(0), q_off).view(np.uint32), q_on.view(np.uint32))
+    for k in ("rho_zz", "rtheta_m", "ru", "rw"):
+        assert np.array_equal(runs[True][k].view(np.uint32), runs[False][k].view(np.uint32)), k
