@@ -14,7 +14,9 @@ time) with its algorithmic bytes per launch over its average launch time,
 timed with HIP events on the launch stream; `step_roofline` reports the whole
 step's algorithmic bytes (phase-level model) over the step time;
 `cpu_baseline` times the CPU restatement of the same step (binary32 strict,
-OpenMP over all host cores) on a bounded sample of the same workload.
+OpenMP over all host cores) on a bounded sample of the same workload;
+`fp64_rel_l2` states the measured numerics' error after one large step
+against the FP64 restatement (relative L2 on u, w, theta_m, rho_zz).
 """
 
 from __future__ import annotations
@@ -72,6 +74,33 @@ def cpu_baseline(case, cpu_steps: int):
                   f"binary32 strict CPU restatement, OpenMP",
         "seconds_per_step": wall / n,
     }
+
+
+def fp64_error(case):
+    """Relative L2 error of one GPU large step (the measured numerics) against
+    the FP64 restatement of the same step, on u, w, theta_m and rho_zz."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT))
+    from oracle.oracle import Oracle
+    from hexdyn.case import initial_state_arrays
+    from hexdyn.driver import HexDycoreDriver
+
+    d = HexDycoreDriver.from_host(case)
+    d.step_device(1)
+    got = d.get_state()
+    d.close()
+    o = Oracle(case, 64)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    if o.step(1) != 0:
+        return {"error": "FP64 restatement refused the step"}
+    ref = o.get_state()
+    out = {}
+    for k in ("u", "w", "theta_m", "rho_zz"):
+        r = np.asarray(ref[k], dtype=np.float64)
+        out[k] = float(np.linalg.norm(np.asarray(got[k], dtype=np.float64) - r) / np.linalg.norm(r))
+    out["after"] = "1 large step from the initial state"
+    return out
 
 
 def main():
@@ -168,11 +197,16 @@ def main():
                      "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
     cpu = None
+    err64 = None
     if not args.no_cpu_baseline:
         try:
             cpu = cpu_baseline(case, args.cpu_steps)
         except Exception as exc:  # reported, never silently substituted
             cpu = {"error": repr(exc)}
+        try:
+            err64 = fp64_error(case)
+        except Exception as exc:
+            err64 = {"error": repr(exc)}
 
     out = {
         "metric": "sim-days/wall-day x1.40962 L55, 1 GPU: plain vs learned-corrected; %HBM peak",
@@ -195,6 +229,7 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
+        "fp64_rel_l2": err64,
         "kernels_top": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for r in kernels[:8]],
     }
     print(json.dumps(out))
