@@ -76,20 +76,16 @@ def cpu_baseline(case, cpu_steps: int):
     }
 
 
-def fp64_error(case):
-    """Relative L2 error of one GPU large step (the measured numerics) against
-    the FP64 restatement of the same step, on u, w, theta_m and rho_zz."""
+def fp64_error(case, got):
+    """Relative L2 error of one GPU large step (`got`, the measured numerics)
+    against the FP64 restatement of the same step, on u, w, theta_m, rho_zz.
+    Part of the CPU leg: the restatement is the checker, never the thing timed."""
     import numpy as np
 
     sys.path.insert(0, str(ROOT))
     from oracle.oracle import Oracle
     from hexdyn.case import initial_state_arrays
-    from hexdyn.driver import HexDycoreDriver
 
-    d = HexDycoreDriver.from_host(case)
-    d.step_device(1)
-    got = d.get_state()
-    d.close()
     o = Oracle(case, 64)
     o.set_state(initial_state_arrays(case), rebuild=True)
     if o.step(1) != 0:
@@ -196,15 +192,20 @@ def main():
     step_roofline = {"alg_bytes_per_step": b_alg, "achieved": round(step_gbs, 1), "unit": "GB/s",
                      "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
+    # the CPU leg (rank 0, N=1): the CPU baseline and the FP64 error of one GPU step
     cpu = None
     err64 = None
     if not args.no_cpu_baseline:
+        first = HexDycoreDriver.from_host(case)
+        first.step_device(1)
+        got = first.get_state()
+        first.close()
         try:
             cpu = cpu_baseline(case, args.cpu_steps)
         except Exception as exc:  # reported, never silently substituted
             cpu = {"error": repr(exc)}
         try:
-            err64 = fp64_error(case)
+            err64 = fp64_error(case, got)
         except Exception as exc:
             err64 = {"error": repr(exc)}
 

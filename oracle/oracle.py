@@ -1,7 +1,8 @@
 """TEST INFRASTRUCTURE — Python binding of the CPU oracle (liboracle.so).
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
-module, as the checker.  The product (libhexdyn.so and hexdyn/) never imports
+Only tests/, __graft_entry__.smoke() and bench.py's CPU leg (the CPU
+baseline timing and the FP64 error of one GPU step) use this module, as the
+checker.  The product (libhexdyn.so and hexdyn/) never imports
 it.
 """
 
