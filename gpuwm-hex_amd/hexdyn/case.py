@@ -6,7 +6,9 @@ Workloads (sizes, seeds):
                   dt 1800 s, RK3, 2 acoustic substeps, 3 dynamics subcycles.
   x1.40962 moist: subdivided 6x, same recipe, qv from RH ~ U(0.3, 0.9), five
                   log-normal condensates where RH > 0.8, FCT transport, dt 720 s.
-  x1.10242 moist: subdivided 5x, dt 1440 s.
+  x1.10242 moist: subdivided 5x, dt 1440 s (the ensemble mesh).
+  x1.163842 qv  : subdivided 7x, water vapour only (seeded RH field), dt 360 s
+                  (the scale-up mesh; about 2.5 min of host time to build).
 Seeds: mesh 1, terrain 2, theta perturbation 3, RH 4, condensates 5.
 """
 
@@ -69,6 +71,7 @@ WORKLOADS = {
     "x1.10242-moist": dict(level=5, moist=True, n_scalars=6, dt=1440.0),
     "x1.40962-moist": dict(level=6, moist=True, n_scalars=6, dt=720.0),
     "x1.40962-dry": dict(level=6, moist=False, n_scalars=0, dt=720.0),
+    "x1.163842-qv": dict(level=7, moist=True, n_scalars=1, dt=360.0),
     "x1.642-dry": dict(level=3, moist=False, n_scalars=0, dt=3600.0),
     "x1.642-moist": dict(level=3, moist=True, n_scalars=6, dt=3600.0),
 }
@@ -98,12 +101,15 @@ def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[Dy
         with open(path, "rb") as f:
             case = pickle.load(f)
         return replace(case, config=cfg, len_disp=_len_disp(case.mesh, cfg))
+    say = _progress(name)
     mesh = icosahedral_mesh(spec["level"], seed=1, lloyd_iterations=20)
+    say(f"mesh {mesh.n_cells} cells")
     deriv = hsetup.deriv_two(mesh)
     terrain = hinit.random_terrain(mesh, seed=2, hmax=spec.get("hmax", 4000.0)) * spec.get("terrain_scale", 1.0)
     grid = hinit.build_vertical_grid(mesh, terrain, nlev=nlev, coef_3rd_order=cfg.config_coef_3rd_order, deriv=deriv)
     reference, state = hinit.build_reference_and_state(mesh, grid, moist=spec["moist"], n_scalars=spec["n_scalars"],
                                                        theta_amplitude=spec.get("theta_amplitude", 0.5))
+    say("vertical grid, reference and initial state")
     adv, adv3, nadv, advc = hsetup.advection_coefficients(mesh, deriv)
     etp, etm, ewp, ewm = hsetup.acoustic_offcentering(
         grid.rdzw, minimum=cfg.config_epssm_minimum, maximum=cfg.config_epssm_maximum,
@@ -131,6 +137,21 @@ def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[Dy
             pickle.dump(case, f, protocol=pickle.HIGHEST_PROTOCOL)
         os.replace(tmp, path)
     return case
+
+
+def _progress(name: str):
+    """Stage messages on stderr while a case is built (HEXDYN_VERBOSE=1)."""
+    if not os.environ.get("HEXDYN_VERBOSE"):
+        return lambda msg: None
+    import sys
+    import time
+
+    t0 = time.perf_counter()
+
+    def say(msg: str) -> None:
+        print(f"# case {name}: {msg} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+
+    return say
 
 
 def _len_disp(mesh: Mesh, cfg: DycoreConfig) -> float:
