@@ -358,6 +358,32 @@ STEP_VARIANTS = [
 ]
 
 
+def test_partial_mass_loading_bitwise(get_case):
+    """physics_mass_loading_count below the carried species count
+    (cuda_driver.py:5063-5068): 4 of 6 species load the air, all 6 are
+    transported; 2 steps bit for bit, and qtot is the leading-4 sum."""
+    import dataclasses
+
+    print("[gpu] x1.642-moist, mass loading by 4 of 6 species, 2 steps", flush=True)
+    case = dataclasses.replace(get_case("x1.642-moist"), n_moist=4)
+    d = _driver(case)
+    d.phase("begin_step")
+    q = np.asarray(case.state.scalars, dtype=np.float32)
+    t = np.zeros(q.shape[1:], dtype=np.float32)
+    for s in range(4):
+        t = t + q[s]
+    assert _bits_equal(d.export_field("QTOT").cpu().numpy(), t)
+    d.close()
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(2) == 0
+    d = _driver(case)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
 @pytest.mark.parametrize("variant", STEP_VARIANTS, ids=lambda v: ",".join(f"{k[7:]}={x}" for k, x in v.items()))
 def test_whole_step_variants_bitwise(get_case, variant):
     import dataclasses

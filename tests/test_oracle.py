@@ -334,3 +334,36 @@ def test_post_transport_handoff_skips_the_clamp(get_case):
     assert np.array_equal(np.where(q_off < 0, np.float32(0), q_off).view(np.uint32), q_on.view(np.uint32))
     for k in ("rho_zz", "rtheta_m", "ru", "rw"):
         assert np.array_equal(runs[True][k].view(np.uint32), runs[False][k].view(np.uint32)), k
+
+
+def test_mass_loading_counts_only_the_leading_species(get_case):
+    """physics_mass_loading_count (cuda_driver.py:5063-5068): qtot sums the
+    leading n_moist carried species in species order; cqw and cqu follow from
+    it in closed form (the moist-coefficient kernels, cuda_driver.py:1522-1634).
+    Species after the count are carried but weigh nothing."""
+    base = get_case("x1.642-moist")
+    case = dataclasses.replace(base, n_moist=4)
+    L, C = case.n_levels, case.mesh.n_cells
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    o.begin_step()
+    q = np.asarray(case.state.scalars, dtype=np.float32)
+    t = np.zeros((L, C), dtype=np.float32)
+    for s in range(4):
+        t = t + q[s]
+    assert np.array_equal(o.export_field("QTOT").reshape(L, C).view(np.uint32), t.view(np.uint32))
+    cqw = o.export_field("CQW").reshape(L, C)
+    half = np.float32(0.5)
+    want = np.float32(1.0) / (np.float32(1.0) + half * (t[1:] + t[:-1]))
+    assert np.array_equal(cqw[1:].view(np.uint32), want.view(np.uint32))
+    c0, c1 = case.mesh.cellsOnEdge[:, 0], case.mesh.cellsOnEdge[:, 1]
+    te = np.zeros((L, case.mesh.n_edges), dtype=np.float32)
+    for s in range(4):
+        te = te + half * (q[s][:, c0] + q[s][:, c1])
+    cqu = o.export_field("CQU").reshape(L, -1)
+    assert np.array_equal(cqu.view(np.uint32), (np.float32(1.0) / (np.float32(1.0) + te)).view(np.uint32))
+    # the full count differs: the last two species are not zero here
+    full = Oracle(base, 32)
+    full.set_state(initial_state_arrays(base), rebuild=True)
+    full.begin_step()
+    assert not np.array_equal(full.export_field("QTOT"), o.export_field("QTOT"))
