@@ -16,8 +16,10 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/hexdyn.h"
@@ -209,6 +211,63 @@ struct hexdyn_handle {
 
 namespace {
 
+// Bitmap over the binary32 arguments [HEXDYN_POW_LO, HEXDYN_POW_HI) of the
+// ones where the host C library's powf(x, y) (the GNU powf the reference's
+// v8.4.1 exner selects, cuda_driver.py:1466-1472) is not the correctly rounded
+// power, i.e. differs from hexdyn_powf_eval.  Built once per process, in
+// word-aligned slices over the host's cores; every marked argument is checked
+// to be the neighbour hexdyn_gnu_powf_fix will choose.
+const std::vector<uint32_t> &gnu_powf_ties(float y, std::string *failure) {
+  static std::once_flag once;
+  static std::vector<uint32_t> bits;
+  static std::string fail;
+  static float built_for = 0.0f;
+  std::call_once(once, [y] {
+    built_for = y;
+    const uint32_t n = HEXDYN_POW_HI - HEXDYN_POW_LO;
+    const uint32_t words = (n + 31u) / 32u;
+    bits.assign(words, 0u);
+    const unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::string> errs(nt);
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t)
+      pool.emplace_back([&, t] {
+        const uint32_t w0 = (uint32_t)((uint64_t)words * t / nt), w1 = (uint32_t)((uint64_t)words * (t + 1) / nt);
+        for (uint32_t w = w0; w < w1; ++w) {
+          uint32_t acc = 0;
+          for (uint32_t b = 0; b < 32u; ++b) {
+            const uint32_t i = w * 32u + b;
+            if (i >= n) break;
+            const float x = hexdyn_f32_from_bits(HEXDYN_POW_LO + i);
+            const double d = hexdyn_pow_d(x, y);
+            const float r = (float)d;
+            const float g = ::powf(x, y);
+            if (hexdyn_f32_bits(g) == hexdyn_f32_bits(r)) continue;
+            acc |= 1u << b;
+            const uint32_t ir = hexdyn_f32_bits(r);
+            const uint32_t fixed = d > (double)r ? ir + 1u : ir - 1u;  // hexdyn_gnu_powf_fix's choice
+            if (fixed != hexdyn_f32_bits(g) && errs[t].empty()) {
+              char msg[160];
+              std::snprintf(msg, sizeof msg, "host powf(%a, %a) = %a is not the neighbour of %a the table encodes",
+                            (double)x, (double)y, (double)g, (double)r);
+              errs[t] = msg;
+            }
+          }
+          bits[w] = acc;
+        }
+      });
+    for (auto &th : pool) th.join();
+    for (const auto &e : errs)
+      if (!e.empty()) {
+        fail = e;
+        break;
+      }
+  });
+  if (built_for != y && fail.empty()) fail = "the GNU powf table serves one exponent per process";
+  if (failure) *failure = fail;
+  return bits;
+}
+
 int run_phase_begin_step(hexdyn_handle *h, hipStream_t s, bool fused);
 
 // The device pointers the kernels of the current phase see (buffer roles above).
@@ -268,7 +327,15 @@ extern "C" {
 
 int hexdyn_abi_version(void) { return HEXDYN_ABI_VERSION; }
 
-float hexdyn_powf(float x, float y) { return hexdyn_powf_eval(x, y); }
+float hexdyn_powf(float x, float y) {
+  const float rcv = 287.0f / (1004.5f - 287.0f);
+  const double d = hexdyn_pow_d(x, y);
+  const float r = (float)d;
+  if (y != rcv || !(x > 0.0f)) return r;
+  std::string why;
+  const std::vector<uint32_t> &ties = gnu_powf_ties(rcv, &why);
+  return why.empty() ? hexdyn_gnu_powf_fix(x, r, d, ties.data()) : r;
+}
 
 int hexdyn_last_error(const hexdyn_handle *h, char *buf, size_t len) {
   if (!h || !buf || len == 0) return HEXDYN_E_INVALID;
@@ -468,8 +535,9 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
         const float d1 = m->mesh_density ? m->mesh_density[a1] : 1.0f;
         const float mean = 0.5f * (d0 + d1);
         if (!(mean > 0.0f)) return refuse("meshDensity must be positive");
-        del2[ne] = 1.0f / hexdyn_powf_eval(mean, 0.25f);
-        del4[ne] = 1.0f / hexdyn_powf_eval(mean, 0.75f);
+        // Fortran ** of reals: the C library's powf (host code)
+        del2[ne] = 1.0f / ::powf(mean, 0.25f);
+        del4[ne] = 1.0f / ::powf(mean, 0.75f);
       }
     }
     std::vector<float> c2s((size_t)C * ME, 0.0f), s2s((size_t)C * ME, 0.0f), css((size_t)C * ME, 0.0f),
@@ -643,6 +711,15 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   K.cfac = (float)(287.0 / 100000.0);
   K.rgas = 287.0f;
   if (K.rcv != (float)(287.0 / (1004.5 - 287.0))) return refuse("inconsistent binary32 R/cv");
+  {
+    std::string why;
+    const std::vector<uint32_t> &ties = gnu_powf_ties(K.rcv, &why);
+    if (!why.empty()) return refuse(why.c_str());
+    if (h->upload(&h->I.m.powf_ties, ties)) {
+      hexdyn_destroy(h);
+      return HEXDYN_E_HIP;
+    }
+  }
   K.coef3 = c->config_coef_3rd_order;
   K.apvm_on = c->config_apvm_upwinding > 0.0f;
   K.apvm_scale = c->config_apvm_upwinding * (float)c->config_dt;

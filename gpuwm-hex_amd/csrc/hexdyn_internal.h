@@ -45,6 +45,7 @@ struct Mesh {
   const float *kite;    // [V][3]
   const float *fedge, *fvert, *cosang, *sinang;
   const float *adv, *adv3;  // [E][MA]
+  const uint32_t *powf_ties;  // GNU powf correction bitmap of the exner exponent (hexdyn_powf.h)
   // per-slot copies of edge geometry, so a kernel reads them with the
   // connectivity instead of through it (csign and vsign are +-1, so the
   // signed products are exact)

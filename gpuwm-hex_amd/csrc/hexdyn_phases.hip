@@ -102,6 +102,14 @@ __device__ __forceinline__ float flux3(float qm2, float qm1, float q0, float qp1
 
 __device__ __forceinline__ void set_flag(int *flag) { atomicOr(flag, 1); }
 
+// exner = arg^(R/cv) with the GNU powf bits the reference's v8.4.1 recovery
+// uses (cuda_driver.py:1466-1472): the correctly rounded binary64 method,
+// moved to the other neighbour where the host-built bitmap marks the argument.
+__device__ __forceinline__ float exner_pow(const Instance &I, float arg) {
+  const double d = hexdyn_pow_d(arg, I.k.rcv);
+  return hexdyn_gnu_powf_fix(arg, (float)d, d, I.m.powf_ties);
+}
+
 // Wave-uniform per-slot coefficients, loaded unconditionally (tables are
 // padded, so every slot is addressable) so the scalar loads issue together
 // instead of one round trip per slot under the slot-count test.
@@ -1308,7 +1316,7 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps) {
   if (S3 && lay) {
     const float arg = (zk * I.k.cfac) * rtv;
     if (!(arg > 0.0f)) bad = 1;
-    ex = hexdyn_powf_eval(arg, I.k.rcv);
+    ex = exner_pow(I, arg);
     ppv = (zk * I.k.rgas) * (ex * rp + rtb * (ex - exb));
   }
   const float fw = srw + wwa * inv_steps;
@@ -1385,7 +1393,7 @@ KB k_rebuild_cell(const Instance I) {
     rho = AT(I.b.rho, c, k);
     const float rt = AT(I.b.rt, c, k);
     const float zk = AT(I.z.zz, c, k);
-    const float ex = hexdyn_powf_eval((zk * I.k.cfac) * rt, I.k.rcv);
+    const float ex = exner_pow(I, (zk * I.k.cfac) * rt);
     const float rtp = rt - AT(I.z.rtb, c, k);
     AT(I.b.theta, c, k) = rt / rho;
     AT(I.b.exner, c, k) = ex;

@@ -15,6 +15,14 @@
  * gfx950 (no contraction, IEEE division).  The double result carries a
  * relative error of a few 1e-16, so the binary32 rounding equals the
  * correctly rounded power except within that distance of a rounding tie.
+ *
+ * The reference's v8.4.1 exner uses the GNU C library's powf, which is
+ * within 0.82 ulp, not correctly rounded: for the exner exponent R/cv it
+ * returns the other neighbour of the exact value on 74 241 of the
+ * 117 440 512 binary32 arguments in [2^-12, 4).  hexdyn_gnu_powf_fix turns
+ * the correctly rounded result into the GNU one from a bitmap of those
+ * arguments, which the library builds once per process by calling the host's
+ * powf (the only source of the GNU bits used here) and comparing.
  */
 #ifndef HEXDYN_POWF_H
 #define HEXDYN_POWF_H
@@ -45,12 +53,18 @@ HEXDYN_HD static inline double hexdyn_f64_from_bits(uint64_t u) {
   return x;
 }
 
-HEXDYN_HD static inline float hexdyn_powf_eval(float xf, float yf) {
+/* Arguments of the GNU correction bitmap: [2^-12, 4) as binary32 bits. */
+#define HEXDYN_POW_LO 0x39800000u
+#define HEXDYN_POW_HI 0x40800000u
+
+/* x^y in binary64, before the final rounding to binary32 (special values
+ * are returned exactly). */
+HEXDYN_HD static inline double hexdyn_pow_d(float xf, float yf) {
   const uint32_t ix = hexdyn_f32_bits(xf);
   if (!(xf > 0.0f) || ix >= 0x7f800000u) {
-    if (xf == 0.0f) return yf > 0.0f ? 0.0f : hexdyn_f32_from_bits(0x7f800000u);
-    if (ix == 0x7f800000u) return yf > 0.0f ? xf : 0.0f;
-    return hexdyn_f32_from_bits(0x7fc00000u);
+    if (xf == 0.0f) return yf > 0.0f ? 0.0 : (double)hexdyn_f32_from_bits(0x7f800000u);
+    if (ix == 0x7f800000u) return yf > 0.0f ? (double)xf : 0.0;
+    return (double)hexdyn_f32_from_bits(0x7fc00000u);
   }
   int e;
   uint32_t mbits;
@@ -86,8 +100,8 @@ HEXDYN_HD static inline float hexdyn_powf_eval(float xf, float yf) {
   const double ed = (double)e;
   const double lnx = ed * ln2_hi + (ed * ln2_lo + (2.0 * s) * p);
   const double t = (double)yf * lnx;
-  if (t > 89.0) return hexdyn_f32_from_bits(0x7f800000u);
-  if (t < -104.0) return 0.0f;
+  if (t > 89.0) return (double)hexdyn_f32_from_bits(0x7f800000u);
+  if (t < -104.0) return 0.0;
   const double kd = __builtin_floor(t * 1.4426950408889634 + 0.5);
   const double r = (t - kd * ln2_hi) - kd * ln2_lo;
   double q = 1.0 / 6227020800.0;          /* 1/13! */
@@ -106,7 +120,21 @@ HEXDYN_HD static inline float hexdyn_powf_eval(float xf, float yf) {
   q = q * r + 1.0;
   const int k = (int)kd;
   const double scale = hexdyn_f64_from_bits((uint64_t)(k + 1023) << 52);
-  return (float)(q * scale);
+  return q * scale;
+}
+
+HEXDYN_HD static inline float hexdyn_powf_eval(float xf, float yf) { return (float)hexdyn_pow_d(xf, yf); }
+
+/* The GNU powf bits of a positive argument x whose correctly rounded power
+ * is r (from the binary64 value d): where the bitmap marks x, the GNU result
+ * is the neighbour of r on the side of the exact value. */
+HEXDYN_HD static inline float hexdyn_gnu_powf_fix(float x, float r, double d, const uint32_t *ties) {
+  const uint32_t ix = hexdyn_f32_bits(x);
+  if (ix < HEXDYN_POW_LO || ix >= HEXDYN_POW_HI) return r;
+  const uint32_t i = ix - HEXDYN_POW_LO;
+  if (!((ties[i >> 5] >> (i & 31u)) & 1u)) return r;
+  const uint32_t ir = hexdyn_f32_bits(r);
+  return hexdyn_f32_from_bits(d > (double)r ? ir + 1u : ir - 1u);
 }
 
 #endif /* HEXDYN_POWF_H */

@@ -7,6 +7,7 @@ Workloads (sizes, seeds):
   x1.40962 moist: subdivided 6x, same recipe, qv from RH ~ U(0.3, 0.9), five
                   log-normal condensates where RH > 0.8, FCT transport, dt 720 s.
   x1.10242 moist: subdivided 5x, dt 1440 s (the ensemble mesh).
+  x1.642, x1.162: subdivided 3x and 2x (small test cases), dt 3600 s and 7200 s.
   x1.163842 qv  : subdivided 7x, water vapour only (seeded RH field), dt 360 s
                   (the scale-up mesh; about 2.5 min of host time to build).
 Seeds: mesh 1, terrain 2, theta perturbation 3, RH 4, condensates 5.
@@ -72,6 +73,7 @@ WORKLOADS = {
     "x1.40962-moist": dict(level=6, moist=True, n_scalars=6, dt=720.0),
     "x1.40962-dry": dict(level=6, moist=False, n_scalars=0, dt=720.0),
     "x1.163842-qv": dict(level=7, moist=True, n_scalars=1, dt=360.0),
+    "x1.162-moist": dict(level=2, moist=True, n_scalars=6, dt=7200.0),
     "x1.642-dry": dict(level=3, moist=False, n_scalars=0, dt=3600.0),
     "x1.642-moist": dict(level=3, moist=True, n_scalars=6, dt=3600.0),
 }

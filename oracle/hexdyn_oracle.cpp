@@ -34,9 +34,10 @@
  *   scalar transport        transport.py:1649-1826 (stages 1-2),
  *                           transport.py:1829-2175 (monotonic FCT, stage 3)
  *   post-transport clamp    regional_v841.py:1004 (putmask q<0 -> 0)
- * Parity status: see the test suite; the exner power function is the
- * product's named binary64 method (hexdyn_powf.h), pinned separately against
- * glibc powf and correctly rounded binary64 pow.
+ * Parity status: see the test suite.  The binary32 power function (exner,
+ * mesh scaling) is the host C library's powf: the GNU powf, equal to the
+ * reference's own transcription of it (native_powf.py) on every argument in
+ * [2^-12, 4) by its golden vectors.
  */
 #include <algorithm>
 #include <cmath>
@@ -63,7 +64,9 @@ template <class T> static inline T pmax(T a, T b) { return b > a ? b : a; }     
 template <class T> static inline T copysign1(T x) { return std::signbit(x) ? T(-1) : T(1); }
 
 template <class T> static inline T power(T x, T y);
-template <> inline float power<float>(float x, float y) { return hexdyn_powf_eval(x, y); }
+// binary32: the host C library's powf, the GNU powf the reference's v8.4.1
+// exner and native MPAS use (cuda_driver.py:1466-1472)
+template <> inline float power<float>(float x, float y) { return ::powf(x, y); }
 template <> inline double power<double>(double x, double y) { return std::pow(x, y); }
 
 struct Stage {
@@ -1263,6 +1266,23 @@ template <class T> struct Oracle {
   }
 };
 
+// Working arrays by member name, for tests that feed the same operands to the
+// reference's own operators (the HIP field ids expose only the published ones).
+template <class O> static auto *internal(O &o, const std::string &n) {
+  using V = typename std::remove_reference<decltype(o.rho)>::type;
+  const std::pair<const char *, V *> table[] = {
+      {"s_theta", &o.s_theta}, {"s_exner", &o.s_exner}, {"s_rho_p", &o.s_rho_p}, {"s_rtheta_p", &o.s_rtheta_p},
+      {"s_rho", &o.s_rho}, {"s_rt", &o.s_rt}, {"s_ru", &o.s_ru}, {"s_rw", &o.s_rw},
+      {"c_rho", &o.c_rho}, {"c_rt", &o.c_rt}, {"c_ru", &o.c_ru}, {"c_rw", &o.c_rw},
+      {"c_theta", &o.c_theta}, {"c_rho_p", &o.c_rho_p}, {"c_rtheta_p", &o.c_rtheta_p},
+      {"w", &o.w}, {"u", &o.u}, {"exner", &o.exner}, {"pp", &o.pp},
+      {"cofwr", &o.cofwr}, {"rtheta_pp_old", &o.rtheta_pp_old}, {"tend_rho", &o.tend_rho},
+      {"split_u", &o.split_u}, {"split_w", &o.split_w}, {"o_rho", &o.o_rho}, {"o_q", &o.o_q}};
+  for (const auto &t : table)
+    if (n == t.first) return t.second;
+  return (V *)nullptr;
+}
+
 template <class O> static void set_physics(O &o, const float *rho, const float *ru, const float *rt) {
   using T = typename std::remove_reference<decltype(o.rho[0])>::type;
   auto in = [](std::vector<T> &dst, const float *src, size_t n) {
@@ -1423,7 +1443,45 @@ int oracle_end_step(void *h) {
   DISPATCH(h, { o.end_step(); rc = o.failed ? HEXDYN_E_NONFINITE : 0; });
   return rc;
 }
-float oracle_powf(float x, float y) { return hexdyn_powf_eval(x, y); }
+float oracle_powf(float x, float y) { return power<float>(x, y); }
+
+// Test hooks: a working array by member name (size, export, import), and the
+// implicit coefficients of the imported step-start exner, theta_m and
+// rtheta_p for acoustic step dts.
+int64_t oracle_internal_size(void *h, const char *name) {
+  int64_t n = -1;
+  DISPATCH(h, { auto *f = internal(o, name); n = f ? (int64_t)f->size() : -1; });
+  return n;
+}
+int oracle_export_internal(void *h, const char *name, float *dst, int64_t n) {
+  int rc = 0;
+  DISPATCH(h, {
+    auto *f = internal(o, name);
+    if (!f || (int64_t)f->size() != n) rc = 1;
+    else for (int64_t i = 0; i < n; ++i) dst[i] = (float)(*f)[i];
+  });
+  return rc;
+}
+int oracle_import_internal(void *h, const char *name, const float *src, int64_t n) {
+  int rc = 0;
+  DISPATCH(h, {
+    auto *f = internal(o, name);
+    if (!f || (int64_t)f->size() != n) rc = 1;
+    else for (int64_t i = 0; i < n; ++i) (*f)[i] = src[i];
+  });
+  return rc;
+}
+// The host C library's powf and the product's exner power method over many
+// arguments (tests and golden-vector scripts compare them with the reference's
+// GNU powf).
+void oracle_libm_powf_many(const float *x, float y, float *out, int64_t n) {
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; ++i) out[i] = ::powf(x[i], y);
+}
+void oracle_hexdyn_powf_many(const float *x, float y, float *out, int64_t n) {
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; ++i) out[i] = hexdyn_powf_eval(x[i], y);
+}
 
 // RK schedule of integration.py:110-160 rounded to binary32 (large[3], acoustic[3], steps[3]).
 void oracle_schedule(double dt, int order, int substeps, int splits, float *large, float *acoustic, int *steps) {

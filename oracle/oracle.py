@@ -69,6 +69,13 @@ def lib() -> ct.CDLL:
                                          ct.c_void_p]
         _lib.oracle_export_btri_ctri_f64.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64]
         _lib.oracle_export_field_f64.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int64]
+        _lib.oracle_internal_size.argtypes = [ct.c_void_p, ct.c_char_p]
+        _lib.oracle_internal_size.restype = ct.c_int64
+        _lib.oracle_export_internal.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_void_p, ct.c_int64]
+        _lib.oracle_import_internal.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_void_p, ct.c_int64]
+        for name in ("oracle_libm_powf_many", "oracle_hexdyn_powf_many"):
+            getattr(_lib, name).argtypes = [ct.c_void_p, ct.c_float, ct.c_void_p, ct.c_int64]
+            getattr(_lib, name).restype = None
     return _lib
 
 
@@ -138,6 +145,22 @@ class Oracle:
             raise KeyError(name)
         return out
 
+    def export_internal(self, name: str) -> np.ndarray:
+        """A working array by its member name (s_theta, s_exner, c_rho, s_rw,
+        c_rw, w, cofwr, rtheta_pp_old, ...), flat, logical [level][entity]."""
+        n = lib().oracle_internal_size(self.h, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=np.float32)
+        if lib().oracle_export_internal(self.h, name.encode(), out.ctypes.data, n):
+            raise KeyError(name)
+        return out
+
+    def import_internal(self, name: str, values: np.ndarray) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float32).ravel()
+        if lib().oracle_import_internal(self.h, name.encode(), a.ctypes.data, a.size):
+            raise KeyError(name)
+
     def btri_ctri_f64(self):
         n = self.case.n_levels * self.case.mesh.n_cells
         b = np.empty(n, dtype=np.float64)
@@ -184,3 +207,13 @@ def schedule_f32(dt: float, order: int, substeps: int, splits: int):
     steps = np.zeros(3, dtype=np.int32)
     lib().oracle_schedule(dt, order, substeps, splits, large.ctypes.data, acoustic.ctypes.data, steps.ctypes.data)
     return large, acoustic, steps
+
+
+def powf_many(x: np.ndarray, y: float, method: str) -> np.ndarray:
+    """powf(x, y) elementwise in binary32 by the host C library ("libm") or
+    by the product's exner method ("hexdyn")."""
+    xs = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(xs)
+    fn = {"libm": lib().oracle_libm_powf_many, "hexdyn": lib().oracle_hexdyn_powf_many}[method]
+    fn(xs.ctypes.data, float(np.float32(y)), out.ctypes.data, xs.size)
+    return out

@@ -1,0 +1,334 @@
+#!/usr/bin/env python3
+"""Write golden vectors of the reference's NumPy operator restatement.
+
+Run in a development checkout where the reference tree is present (never at
+test time, never on a GPU host):
+
+    python tests/golden/make_reference_vectors.py /root/reference
+
+Only the reference functions whose binary32 bits nothing else pins are
+called (their exact operation order): solve diagnostics, the v8.4.1 implicit
+coefficients and acoustic step, large-step recovery, the advection
+coefficients and monotonic scalar transport.  Each call's inputs and outputs
+are written to `tests/golden/reference_<name>.npz` (data only) together with
+a digest of the mesh and vertical arrays they were made on;
+`tests/test_reference_vectors.py` feeds the same inputs to the oracle's phase
+and compares bits.
+
+Inputs are this package's own x1.162 moist case at 10 levels (seeded, so the
+test rebuilds the identical case) and states the oracle produces from it, so
+every operand is realistic and non-zero.
+"""
+from __future__ import annotations
+
+import sys
+import types
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT / "gpuwm-hex_amd"))
+sys.path.insert(0, str(ROOT))
+
+from reference_walk import (ACOUSTIC_FORCING, ACOUSTIC_STATE, COEFFICIENTS, DAMPING, RECOVERY_STATE,  # noqa: E402
+                            VCOEF, build_case as build, case_digest, operand_digest, read, read_all, schedule,
+                            walk)
+
+F32 = np.float32
+
+
+def mesh_view(case):
+    """The mesh as the reference's operators read it: MPAS names, 0-based
+    connectivity, binary32 geometry (the values the oracle is given)."""
+    m = case.mesh
+    ns = types.SimpleNamespace()
+    names = ["cellsOnEdge", "verticesOnEdge", "edgesOnCell", "cellsOnCell", "verticesOnCell", "nEdgesOnCell",
+             "kiteForCell", "edgesOnVertex", "cellsOnVertex", "edgesOnEdge", "nEdgesOnEdge", "weightsOnEdge",
+             "dcEdge", "dvEdge", "areaCell", "areaTriangle", "kiteAreasOnVertex", "fEdge", "fVertex",
+             "angleEdge", "xCell", "yCell", "zCell", "xEdge", "yEdge", "zEdge", "xVertex", "yVertex", "zVertex",
+             "latCell", "lonCell", "latEdge", "lonEdge"]
+    for n in names:
+        try:
+            a = np.asarray(getattr(m, n))
+        except AttributeError:
+            if n in m.arrays:
+                a = np.asarray(m.arrays[n])
+            else:
+                continue
+        setattr(ns, n, a.astype(F32) if a.dtype.kind == "f" else a.astype(np.int64))
+    ns.nCells, ns.nEdges, ns.nVertices = m.n_cells, m.n_edges, m.n_vertices
+    ns.maxEdges = m.max_edges
+    ns.arrays = {n: getattr(ns, n) for n in names if hasattr(ns, n)}
+    return ns
+
+
+def oracle_state(case, steps: int):
+    from hexdyn.case import initial_state_arrays
+    from oracle.oracle import Oracle
+
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    if steps:
+        assert o.step(steps) == 0
+    return o
+
+
+def solve_diagnostics_vectors(case):
+    """diagnostics.compute_solve_diagnostics (diagnostics.py:95-306) at rk
+    step 3 (v reconstructed) and rk step 1 (v from the cache)."""
+    from hexcore.diagnostics import compute_solve_diagnostics
+
+    L, C, E = case.n_levels, case.mesh.n_cells, case.mesh.n_edges
+    o = oracle_state(case, 1)
+    u = o.export_field("U").reshape(L, E)
+    rho = o.export_field("RHO_ZZ").reshape(L, C)
+    # the cache an rk step 1 reads: v of the previous large step's start state
+    v_cached = oracle_state(case, 0).export_field("V").reshape(L, E)
+    assert np.count_nonzero(v_cached != o.export_field("V").reshape(L, E)) > 0
+    mesh = mesh_view(case)
+    cfg = case.config
+    out = {"u": u, "rho_zz": rho, "v_cached": v_cached}
+    for rk in (3, 1):  # kinetic energy, divergence and vorticity do not depend on the rk step
+        d = compute_solve_diagnostics(
+            mesh, u, rho, dt=float(F32(cfg.config_dt)), apvm_upwinding=float(cfg.config_apvm_upwinding),
+            rk_step=rk, cached_tangential_velocity=v_cached if rk != 3 else None,
+            f_vertex=mesh.fVertex, inv_area_cell=case.inv_area_cell.astype(F32),
+            inv_area_triangle=case.inv_area_triangle.astype(F32), inv_dc_edge=case.inv_dc_edge.astype(F32),
+            inv_dv_edge=case.inv_dv_edge.astype(F32))
+        fields = [("V", d.tangential_velocity), ("PV_EDGE", d.pv_edge)]
+        if rk == 3:
+            fields += [("KE", d.kinetic_energy), ("DIVERGENCE", d.divergence), ("VORTICITY", d.vorticity)]
+        for name, val in fields:
+            assert val.dtype == F32, name
+            out[f"rk{rk}_{name}"] = np.ascontiguousarray(val)
+    return out
+
+
+def _offcentering(case):
+    from hexcore.offcentering_v841 import AcousticOffcenteringV841
+
+    L = case.n_levels
+    return AcousticOffcenteringV841(etp=case.etp.astype(F32), etm=case.etm.astype(F32), ewp=case.ewp.astype(F32),
+                                    ewm=case.ewm.astype(F32), height_u_levels=np.zeros(L, F32),
+                                    height_w_levels=np.zeros(L + 1, F32))
+
+
+def _levels(case):
+    g = case.grid
+    return {n: np.asarray(getattr(g, n), dtype=F32) for n in ("rdzw", "fzm", "fzp", "rdzu")}
+
+
+def vertical_coefficient_vectors(case):
+    """acoustic_v841.compute_vertical_implicit_coefficients_v841
+    (acoustic_v841.py:81-258) at stage 1 (subcycle-start theta and
+    rtheta_p) and stage 2 (the stage-1 recovered ones), the exner of the
+    subcycle start, dts of each stage."""
+    from hexcore.acoustic_v841 import compute_vertical_implicit_coefficients_v841
+
+    sched = schedule(case)
+    lev = _levels(case)
+    zz = case.grid.zz.astype(F32)
+    ref = case.reference
+    out = {}
+    for label, o in walk(case):
+        if label not in ("vcoef_1", "vcoef_2"):
+            continue
+        stage = int(label[-1])
+        ops = read_all(o, case, VCOEF[stage])
+        res = compute_vertical_implicit_coefficients_v841(
+            dts=sched[stage - 1].acoustic_timestep, offcentering=_offcentering(case), zz=zz,
+            rho_base=ref.rho_base.astype(F32), rho_theta_base=ref.rtheta_base.astype(F32),
+            exner_base=ref.exner_base.astype(F32), **ops, **lev)
+        out[f"{label}.operands"] = operand_digest(ops.values())
+        for name in ("cofwr", "cofwz", "coftz", "cofwt", "a_tri", "b_tri", "c_tri", "alpha_tri", "gamma_tri"):
+            val = np.asarray(getattr(res, name))
+            assert val.dtype == F32, name
+            out[f"{label}.{name}"] = np.ascontiguousarray(val)
+    return out
+
+
+def _coefficients(o, case):
+    from hexcore.acoustic_v841 import VerticalImplicitCoefficientsV841
+
+    L, C = case.n_levels, case.mesh.n_cells
+    vals = {name: read(o, case, src, shp) for name, src, shp in COEFFICIENTS}
+    b64, c64 = o.btri_ctri_f64()
+    vals["b_tri"] = b64.astype(F32).reshape(L, C)
+    vals["c_tri"] = c64.astype(F32).reshape(L, C)
+    vals["cofrz"] = np.asarray(case.grid.rdzw, dtype=F32)
+    return VerticalImplicitCoefficientsV841(**vals), vals
+
+
+def acoustic_vectors(case):
+    """acoustic_v841.advance_acoustic_step_v841 (acoustic_v841.py:299-560)
+    for the stage-2 substep and both stage-3 substeps, fed the oracle's own
+    coefficients, tendencies and acoustic state at each substep."""
+    from hexcore.acoustic import AcousticStepForcing, AcousticStepState
+    from hexcore.acoustic_v841 import advance_acoustic_step_v841
+
+    sched = schedule(case)
+    lev = _levels(case)
+    mesh = mesh_view(case)
+    out = {}
+    for label, o in walk(case):
+        if not label.startswith("acoustic_") or label.endswith("_out"):
+            continue
+        stage, ss = int(label.split("_")[1]), int(label.split("_")[2])
+        state = read_all(o, case, ACOUSTIC_STATE)
+        forcing = read_all(o, case, ACOUSTIC_FORCING)
+        coefs, cvals = _coefficients(o, case)
+        out[f"{label}.operands"] = operand_digest(list(state.values()) + list(forcing.values()) +
+                                                  list(cvals.values()))
+        forcing["zz"] = case.grid.zz.astype(F32)
+        forcing["zxu"] = case.grid.zxu.astype(F32)
+        forcing["dss"] = np.asarray(case.dss, dtype=F32).reshape(shapes_cell(case))
+        res = advance_acoustic_step_v841(mesh, AcousticStepState(**state), AcousticStepForcing(**forcing), coefs,
+                                         dts=sched[stage - 1].acoustic_timestep, small_step=ss,
+                                         offcentering=_offcentering(case), fzm=lev["fzm"], fzp=lev["fzp"],
+                                         rdzw=lev["rdzw"])
+        for name, _, _ in ACOUSTIC_STATE:
+            val = np.asarray(getattr(res, name))
+            assert val.dtype == F32, name
+            out[f"{label}.{name}"] = np.ascontiguousarray(val)
+    return out
+
+
+def shapes_cell(case):
+    return (case.n_levels, case.mesh.n_cells)
+
+
+def damping_vectors(case):
+    """mixing.divergence_damping_3d (mixing.py:851-932) with the stored
+    inverse of dts, after the stage-2 substep and the second stage-3 one."""
+    from hexcore.mixing import divergence_damping_3d
+
+    sched = schedule(case)
+    mesh = mesh_view(case)
+    cfg = case.config
+    out = {}
+    for label, o in walk(case):
+        if label not in ("damping_2", "damping_3"):
+            continue
+        stage = int(label[-1])
+        args = list(read_all(o, case, DAMPING).values())
+        res = divergence_damping_3d(mesh, *args, dts=sched[stage - 1].acoustic_timestep, config_smdiv=cfg.config_smdiv,
+                                    config_len_disp=float(case.len_disp), native_stored_inverse=True)
+        assert res.dtype == F32
+        out[f"{label}.operands"] = operand_digest(args)
+        out[f"{label}.ru_p"] = np.ascontiguousarray(res)
+    return out
+
+
+def recovery_vectors(case):
+    """integration.recover_large_step_variables (integration.py:198-327)
+    at stages 2 and 3 (exner and pressure at stage 3), no diabatic theta
+    tendency, the case's terrain metrics zb/zb3."""
+    from hexcore.integration import RecoveryBackground, RecoveryState, recover_large_step_variables
+
+    sched = schedule(case)
+    lev = _levels(case)
+    mesh = mesh_view(case)
+    g, ref = case.grid, case.reference
+    bg = RecoveryBackground(rho_base=ref.rho_base.astype(F32), exner_base=ref.exner_base.astype(F32),
+                            rtheta_base=ref.rtheta_base.astype(F32), zz=g.zz.astype(F32))
+    out = {}
+    for label, o in walk(case):
+        if label not in ("recover_2", "recover_3"):
+            continue
+        stage = int(label[-1])
+        state = read_all(o, case, RECOVERY_STATE)
+        res = recover_large_step_variables(
+            mesh, RecoveryState(**state), bg, dt=sched[2].large_timestep, acoustic_steps=sched[stage - 1].acoustic_steps,
+            rk_step=stage, rt_diabatic_tendency=np.zeros(shapes_cell(case), F32), fzm=lev["fzm"], fzp=lev["fzp"],
+            cf1=float(F32(g.cf1)), cf2=float(F32(g.cf2)), cf3=float(F32(g.cf3)),
+            zb_cell=np.asarray(g.zb_cell, dtype=F32), zb3_cell=np.asarray(g.zb3_cell, dtype=F32))
+        out[f"{label}.operands"] = operand_digest(state.values())
+        for name in ("rho_p", "rho_zz", "w", "ww_avg", "rw", "rtheta_p", "theta_m", "exner", "pressure_p",
+                     "ru_avg", "ru", "u"):
+            val = np.asarray(getattr(res, name))
+            assert val.dtype == F32, name
+            out[f"{label}.{name}"] = np.ascontiguousarray(val)
+        if stage == 3:
+            # This NumPy restatement raises to the power with NumPy's binary32
+            # `**`, which on this host is not the GNU powf the v8.4.1 path
+            # selects (cuda_driver.py:1466-1472).  Exner and pressure are
+            # restated here from the function's own argument (integration.py
+            # :267-278) with the reference's GNU powf (native_powf.py).
+            from hexcore.native_powf import powf_native_gnu
+
+            full = res.rtheta_p + bg.rtheta_base
+            arg = (bg.zz * F32(287.0 / 100000.0)) * full
+            rcv = F32(287.0 / (1004.5 - 287.0))
+            ex = np.asarray(powf_native_gnu(arg, rcv), dtype=F32)
+            pp = (bg.zz * F32(287.0)) * (ex * res.rtheta_p + bg.rtheta_base * (ex - bg.exner_base))
+            n_pow = int(np.count_nonzero(ex.view(np.uint32) != res.exner.view(np.uint32)))
+            print(f"recovery stage 3: NumPy ** differs from the GNU powf on {n_pow} of {ex.size} exner values")
+            out[f"{label}.exner"] = np.ascontiguousarray(ex)
+            out[f"{label}.pressure_p"] = np.ascontiguousarray(pp.astype(F32))
+            out[f"{label}.numpy_pow_differs"] = np.array([n_pow], dtype=np.int64)
+    return out
+
+
+def powf_vectors(case):
+    """native_powf.powf_native_gnu (native_powf.py:18-60), the GNU powf the
+    v8.4.1 exner recovery uses (cuda_driver.py:1466-1472), at the exner
+    exponent R/cv over every binary32 argument in [2^-12, 4): the arguments
+    where it differs from the product's correctly rounded binary64 method,
+    with the reference's values there.  Also counts where the host C
+    library's powf differs from the reference (the oracle and the device
+    correction use the host's)."""
+    from hexcore.native_powf import powf_native_gnu
+    from oracle.oracle import powf_many
+
+    y = F32(287.0) / (F32(1004.5) - F32(287.0))
+    lo = int(np.float32(2.0 ** -12).view(np.uint32))
+    hi = int(np.float32(4.0).view(np.uint32))
+    xs_bits, refs, libm_differs = [], [], 0
+    for start in range(lo, hi, 1 << 22):
+        x = np.arange(start, min(start + (1 << 22), hi), dtype=np.uint32).view(np.float32)
+        ref = np.asarray(powf_native_gnu(x, y), dtype=F32)
+        ours = powf_many(x, y, "hexdyn")
+        host = powf_many(x, y, "libm")
+        d = ref.view(np.uint32) != ours.view(np.uint32)
+        xs_bits.append(x.view(np.uint32)[d])
+        refs.append(ref[d])
+        libm_differs += int(np.count_nonzero(ref.view(np.uint32) != host.view(np.uint32)))
+    xb = np.concatenate(xs_bits)
+    print(f"powf: {xb.size} of {hi - lo} arguments differ from the binary64 method; "
+          f"host libm differs from the reference on {libm_differs}")
+    return {"exponent": np.array([y], dtype=F32), "range_bits": np.array([lo, hi], dtype=np.uint32),
+            "x_bits": xb, "reference": np.concatenate(refs),
+            "host_libm_differs": np.array([libm_differs], dtype=np.int64)}
+
+
+VECTORS = {
+    "solve_diagnostics": solve_diagnostics_vectors,
+    "vertical_coefficients": vertical_coefficient_vectors,
+    "acoustic": acoustic_vectors,
+    "divergence_damping": damping_vectors,
+    "recovery": recovery_vectors,
+    "powf": powf_vectors,
+}
+
+
+def main(argv):
+    ref_root = Path(argv[1] if len(argv) > 1 else "/root/reference")
+    sys.path.insert(0, str(ref_root / "tree" / "src"))
+    import hexcore  # noqa: F401  (the reference package)
+
+    case = build()
+    digest = case_digest(case)
+    import hexcore.diagnostics  # noqa: F401  (fail early if an operator module does not import)
+    want = argv[2:] or list(VECTORS)
+    for name in want:
+        arrays = VECTORS[name](case)
+        arrays["case_digest"] = digest
+        path = HERE / f"reference_{name}.npz"
+        np.savez_compressed(path, **arrays)
+        print(f"{path.name}: {len(arrays) - 1} arrays, {path.stat().st_size / 1024:.0f} KiB")
+
+
+if __name__ == "__main__":
+    main(sys.argv)

@@ -1,0 +1,160 @@
+"""The oracle walk at whose checkpoints the reference vectors are taken.
+
+Both `make_reference_vectors.py` (which feeds the operands found at each
+checkpoint to the reference's own operator) and `tests/test_reference_vectors.py`
+(which runs the oracle's phase from the same checkpoint) walk one large step
+of the seeded x1.162 moist case at 10 levels, after one developed step so
+that w, rw and every perturbation are non-zero.  The schedule is RK3 with
+acoustic substep counts (1, 1, 2) and 3 dynamics subcycles; the walk covers
+the first subcycle.
+"""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+
+CASE, NLEV = "x1.162-moist", 10
+F32 = np.float32
+
+
+def build_case():
+    from hexdyn.case import build_case as build
+
+    return build(CASE, nlev=NLEV, use_cache=False)
+
+
+def schedule(case):
+    from hexdyn.setup import rk_schedule
+
+    c = case.config
+    return rk_schedule(c.config_dt, order=c.config_time_integration_order,
+                       acoustic_substeps=c.config_number_of_sub_steps, dynamics_splits=c.config_dynamics_split_steps)
+
+
+def walk(case):
+    """Yield (checkpoint, oracle) along the first subcycle of a large step.
+    A checkpoint named X is followed by X_out after the phase it names ran."""
+    from hexdyn.case import initial_state_arrays
+    from oracle.oracle import Oracle
+
+    sched = schedule(case)
+    assert [s.acoustic_steps for s in sched] == [1, 1, 2]
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(1) == 0
+    o.begin_step()
+    o.begin_subcycle(1)
+    yield "vcoef_1", o
+    o.dyn_tend(1)
+    o.acoustic_substep(1, 1)
+    o.divergence_damping(1)
+    assert o.recover(1) == 0
+    o.solve_diagnostics(1)
+    o.dyn_tend(2)
+    yield "vcoef_2", o
+    yield "acoustic_2_1", o
+    o.acoustic_substep(2, 1)
+    yield "acoustic_2_1_out", o
+    yield "damping_2", o
+    o.divergence_damping(2)
+    yield "damping_2_out", o
+    yield "recover_2", o
+    assert o.recover(2) == 0
+    yield "recover_2_out", o
+    o.solve_diagnostics(2)
+    o.dyn_tend(3)
+    yield "acoustic_3_1", o
+    o.acoustic_substep(3, 1)
+    yield "acoustic_3_1_out", o
+    o.divergence_damping(3)
+    yield "acoustic_3_2", o
+    o.acoustic_substep(3, 2)
+    yield "acoustic_3_2_out", o
+    yield "damping_3", o
+    o.divergence_damping(3)
+    yield "damping_3_out", o
+    yield "recover_3", o
+    assert o.recover(3) == 0
+    yield "recover_3_out", o
+
+
+def shapes(case):
+    L, C, E = case.n_levels, case.mesh.n_cells, case.mesh.n_edges
+    return {"cell": (L, C), "edge": (L, E), "w": (L + 1, C)}
+
+
+# what each reference operator reads, as (argument, source, shape); source is a
+# published field id ("F:NAME") or an oracle working array ("I:name")
+ACOUSTIC_STATE = [("ru_p", "F:RU_P", "edge"), ("rw_p", "F:RW_P", "w"), ("rtheta_pp", "F:RTHETA_PP", "cell"),
+                  ("rtheta_pp_old", "I:rtheta_pp_old", "cell"), ("rho_pp", "F:RHO_PP", "cell"),
+                  ("ru_avg", "F:RU_AVG", "edge"), ("ww_avg", "F:WW_AVG", "w")]
+ACOUSTIC_FORCING = [("rho_zz", "I:c_rho", "cell"), ("theta_m", "I:s_theta", "cell"), ("exner", "I:s_exner", "cell"),
+                    ("cqu", "F:CQU", "edge"), ("tend_ru", "F:TEND_RU", "edge"), ("tend_rho", "F:TEND_RHO", "cell"),
+                    ("tend_rt", "F:TEND_RT", "cell"), ("tend_rw", "F:TEND_RW", "w"), ("w", "I:w", "w"),
+                    ("rw", "I:c_rw", "w"), ("rw_save", "I:s_rw", "w")]
+COEFFICIENTS = [("cofwr", "I:cofwr", "cell"), ("cofwz", "F:COFWZ", "cell"), ("coftz", "F:COFTZ", "w"),
+                ("cofwt", "F:COFWT", "cell"), ("a_tri", "F:A_TRI", "cell"), ("alpha_tri", "F:ALPHA_TRI", "cell"),
+                ("gamma_tri", "F:GAMMA_TRI", "cell")]
+RECOVERY_STATE = [("ww_avg", "F:WW_AVG", "w"), ("rw_save", "I:s_rw", "w"), ("w", "I:w", "w"), ("rw", "I:c_rw", "w"),
+                  ("rw_p", "F:RW_P", "w"), ("rtheta_p", "I:c_rtheta_p", "cell"),
+                  ("rtheta_pp", "F:RTHETA_PP", "cell"), ("rtheta_p_save", "I:s_rtheta_p", "cell"),
+                  ("rho_p", "I:c_rho_p", "cell"), ("rho_p_save", "I:s_rho_p", "cell"), ("rho_pp", "F:RHO_PP", "cell"),
+                  ("rho_zz", "I:c_rho", "cell"), ("ru_avg", "F:RU_AVG", "edge"), ("ru_save", "I:s_ru", "edge"),
+                  ("ru_p", "F:RU_P", "edge"), ("u", "I:u", "edge"), ("ru", "I:c_ru", "edge"),
+                  ("exner", "I:exner", "cell"), ("pressure_p", "I:pp", "cell"), ("theta_m", "I:c_theta", "cell")]
+# the oracle array holding each recovered output (the time averages are kept
+# beside the acoustic sums, in FLUX_U / FLUX_W)
+RECOVERY_OUT = [("rho_p", "I:c_rho_p", "cell"), ("rho_zz", "I:c_rho", "cell"), ("w", "I:w", "w"),
+                ("ww_avg", "F:FLUX_W", "w"), ("rw", "I:c_rw", "w"), ("rtheta_p", "I:c_rtheta_p", "cell"),
+                ("theta_m", "I:c_theta", "cell"), ("exner", "I:exner", "cell"), ("pressure_p", "I:pp", "cell"),
+                ("ru_avg", "F:FLUX_U", "edge"), ("ru", "I:c_ru", "edge"), ("u", "I:u", "edge")]
+
+
+VCOEF = {1: [("cqw", "F:CQW", "cell"), ("exner", "I:s_exner", "cell"), ("theta", "I:s_theta", "cell"),
+             ("rho_theta_perturbation", "I:s_rtheta_p", "cell"), ("qtot", "F:QTOT", "cell")],
+         2: [("cqw", "F:CQW", "cell"), ("exner", "I:s_exner", "cell"), ("theta", "I:c_theta", "cell"),
+             ("rho_theta_perturbation", "I:c_rtheta_p", "cell"), ("qtot", "F:QTOT", "cell")]}
+VCOEF_OUT = [("cofwr", "I:cofwr", "cell"), ("cofwz", "F:COFWZ", "cell"), ("coftz", "F:COFTZ", "w"),
+             ("cofwt", "F:COFWT", "cell"), ("a_tri", "F:A_TRI", "cell"), ("alpha_tri", "F:ALPHA_TRI", "cell"),
+             ("gamma_tri", "F:GAMMA_TRI", "cell")]
+DAMPING = [("rho_u_perturbation", "F:RU_P", "edge"), ("theta_m", "I:s_theta", "cell"),
+           ("rtheta_pp", "F:RTHETA_PP", "cell"), ("rtheta_pp_old", "I:rtheta_pp_old", "cell")]
+
+
+def read(o, case, source, shape):
+    kind, name = source.split(":")
+    a = o.export_field(name) if kind == "F" else o.export_internal(name)
+    return a.reshape(shapes(case)[shape])
+
+
+def read_all(o, case, table):
+    return {name: read(o, case, src, shp) for name, src, shp in table}
+
+
+def operand_digest(arrays) -> np.ndarray:
+    """sha256 of the operands read from the oracle at a checkpoint, in table
+    order (the test checks that its own walk reached the same operands)."""
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a, dtype=np.float32).tobytes())
+    return np.frombuffer(h.hexdigest().encode(), dtype=np.uint8)
+
+
+def case_digest(case) -> np.ndarray:
+    """sha256 over every binary32/int32 mesh, vertical and initial array the
+    operators read."""
+    m, g = case.mesh, case.grid
+    h = hashlib.sha256()
+    for a in (m.cellsOnEdge, m.edgesOnCell, m.verticesOnEdge, m.edgesOnEdge, m.nEdgesOnCell, m.nEdgesOnEdge,
+              m.edgesOnVertex, m.verticesOnCell, m.kiteForCell, m.weightsOnEdge, m.dcEdge, m.dvEdge, m.areaCell,
+              m.areaTriangle, m.kiteAreasOnVertex, m.fVertex, case.inv_area_cell, case.inv_area_triangle,
+              case.inv_dc_edge, case.inv_dv_edge, case.adv_coefs, case.adv_coefs_3rd, case.adv_cells, case.n_adv,
+              g.zz, g.zxu, g.rdzw, g.rdzu, g.fzm, g.fzp, g.zb_cell, g.zb3_cell, [g.cf1, g.cf2, g.cf3], case.dss,
+              case.etp, case.etm, case.ewp, case.ewm, case.reference.rho_base, case.reference.rtheta_base,
+              case.reference.exner_base, case.state.rho_zz, case.state.rtheta_m, case.state.ru, case.state.rw,
+              case.state.scalars):
+        arr = np.asarray(a)
+        arr = arr.astype(F32) if arr.dtype.kind == "f" else arr.astype(np.int32)
+        h.update(np.ascontiguousarray(arr).tobytes())
+    return np.frombuffer(h.hexdigest().encode(), dtype=np.uint8)

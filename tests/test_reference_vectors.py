@@ -1,0 +1,161 @@
+"""The oracle against golden vectors of the reference's own NumPy operators.
+
+`tests/golden/reference_<name>.npz` holds the reference's outputs for the
+functions only its own code pins bit for bit, written by
+`tests/golden/make_reference_vectors.py` from diagnostics.py,
+acoustic_v841.py, mixing.py and integration.py on operands taken from the
+oracle at fixed checkpoints of one large step (`tests/golden/reference_walk.py`).
+Each test walks the oracle to the same checkpoint, checks that it holds the
+same operands (sha256), runs the phase that restates the function and
+compares every output word.  The case is this package's seeded x1.162 moist
+case at 10 levels; its digest is checked first, so a changed case generator
+or walk cannot pass silently.
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+sys.path.insert(0, str(GOLDEN))
+
+import reference_walk as rw  # noqa: E402
+
+NAMES = ["solve_diagnostics", "vertical_coefficients", "acoustic", "divergence_damping", "recovery"]
+
+
+def _load(name):
+    return np.load(GOLDEN / f"reference_{name}.npz", allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def case(get_case):
+    return get_case(rw.CASE, nlev=rw.NLEV)
+
+
+def _same_bits(got, want, what):
+    got = np.ascontiguousarray(got, dtype=np.float32).ravel()
+    want = np.ascontiguousarray(want, dtype=np.float32).ravel()
+    assert got.size == want.size, f"{what}: size {got.size} != {want.size}"
+    bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, (f"{what}: {bad.size} of {got.size} words differ, first at {bad[0]}: "
+                           f"{got[bad[0]]!r} vs reference {want[bad[0]]!r}")
+
+
+def _walk_to(case, label):
+    for name, o in rw.walk(case):
+        if name == label:
+            return o, name
+    raise KeyError(label)
+
+
+def _checkpoints(case, z, table_for):
+    """Walk once, yielding (checkpoint, oracle) at every checkpoint that has
+    vectors in z, after checking the operands there."""
+    for name, o in rw.walk(case):
+        key = f"{name}.operands"
+        if key in z.files:
+            ops = rw.read_all(o, case, table_for(name))
+            extra = []
+            if name.startswith("acoustic_"):
+                b64, c64 = o.btri_ctri_f64()
+                extra = [rw.read(o, case, src, shp) for _, src, shp in rw.COEFFICIENTS]
+                extra += [b64.astype(np.float32), c64.astype(np.float32), np.asarray(case.grid.rdzw, np.float32)]
+            assert bytes(z[key]).decode() == bytes(rw.operand_digest(list(ops.values()) + extra)).decode(), \
+                f"{name}: the walk reached different operands than the vectors were made from"
+        yield name, o
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_vectors_were_made_on_this_case(case, name):
+    assert bytes(_load(name)["case_digest"]).decode() == bytes(rw.case_digest(case)).decode()
+
+
+def test_solve_diagnostics_matches_reference_bits(case):
+    """compute_solve_diagnostics (diagnostics.py:95-306), APVM on, stored
+    inverses: rk step 3 reconstructs v, rk step 1 keeps the cached v."""
+    z = _load("solve_diagnostics")
+    o = Oracle(case, 32)
+    from hexdyn.case import initial_state_arrays
+
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    o.import_field("U", z["u"])
+    o.import_field("RHO_ZZ", z["rho_zz"])
+    for rk in (3, 1):
+        o.import_field("V", z["v_cached"])
+        o.solve_diagnostics(rk)
+        names = ["V", "PV_EDGE"] + (["KE", "DIVERGENCE", "VORTICITY"] if rk == 3 else [])
+        for f in names:
+            _same_bits(o.export_field(f), z[f"rk{rk}_{f}"], f"rk{rk} {f}")
+
+
+def test_vertical_coefficients_match_reference_bits(case):
+    """compute_vertical_implicit_coefficients_v841 (acoustic_v841.py:81-258)
+    at stages 1 and 2: cofwr/cofwz/coftz/cofwt, a/b/c and the Thomas
+    factors alpha/gamma."""
+    z = _load("vertical_coefficients")
+    seen = 0
+    for name, o in _checkpoints(case, z, lambda n: rw.VCOEF[int(n[-1])]):
+        if f"{name}.operands" not in z.files:
+            continue
+        seen += 1
+        for out, src, shp in rw.VCOEF_OUT:
+            _same_bits(rw.read(o, case, src, shp), z[f"{name}.{out}"], f"{name} {out}")
+        b64, c64 = o.btri_ctri_f64()
+        _same_bits(b64.astype(np.float32), z[f"{name}.b_tri"], f"{name} b_tri")
+        _same_bits(c64.astype(np.float32), z[f"{name}.c_tri"], f"{name} c_tri")
+    assert seen == 2
+
+
+def test_acoustic_substeps_match_reference_bits(case):
+    """advance_acoustic_step_v841 (acoustic_v841.py:299-560): the stage-2
+    substep and both stage-3 substeps (small_step 1 and 2)."""
+    z = _load("acoustic")
+    seen = 0
+    pending = None
+    for name, o in _checkpoints(case, z, lambda n: rw.ACOUSTIC_STATE + rw.ACOUSTIC_FORCING):
+        if pending and name == f"{pending}_out":
+            for out, src, shp in rw.ACOUSTIC_STATE:
+                _same_bits(rw.read(o, case, src, shp), z[f"{pending}.{out}"], f"{pending} {out}")
+            seen += 1
+            pending = None
+        if f"{name}.operands" in z.files:
+            pending = name
+    assert seen == 3
+
+
+def test_divergence_damping_matches_reference_bits(case):
+    """divergence_damping_3d (mixing.py:851-932) with the stored inverse of
+    dts, at stages 2 and 3."""
+    z = _load("divergence_damping")
+    seen = 0
+    pending = None
+    for name, o in _checkpoints(case, z, lambda n: rw.DAMPING):
+        if pending and name == f"{pending}_out":
+            _same_bits(rw.read(o, case, "F:RU_P", "edge"), z[f"{pending}.ru_p"], f"{pending} ru_p")
+            seen += 1
+            pending = None
+        if f"{name}.operands" in z.files:
+            pending = name
+    assert seen == 2
+
+
+def test_recovery_matches_reference_bits(case):
+    """recover_large_step_variables (integration.py:198-327) at stages 2 and
+    3: densities, theta_m, rw and w with the terrain terms, u, the time
+    averages, and at stage 3 exner (binary32 powf) and pressure."""
+    z = _load("recovery")
+    seen = 0
+    pending = None
+    for name, o in _checkpoints(case, z, lambda n: rw.RECOVERY_STATE):
+        if pending and name == f"{pending}_out":
+            for out, src, shp in rw.RECOVERY_OUT:
+                _same_bits(rw.read(o, case, src, shp), z[f"{pending}.{out}"], f"{pending} {out}")
+            seen += 1
+            pending = None
+        if f"{name}.operands" in z.files:
+            pending = name
+    assert seen == 2
