@@ -97,6 +97,8 @@ def mesh_from_mpas(grid_path, static_path=None, *, radius: float = EARTH_RADIUS)
         for n in AREAS:
             if n in out:
                 out[n] = out[n] * (radius * radius)
+        if "deriv_two" in out:  # second derivatives: 1 / length^2
+            out["deriv_two"] = out["deriv_two"] / (radius * radius)
         sphere = radius
     else:
         sphere = file_radius
@@ -136,8 +138,13 @@ def _write_mesh(f, mesh: Mesh, *, unit_sphere: bool = False) -> None:
     for name, n in dims.items():
         f.createDimension(name, n)
     f.createDimension("TWO", 2)
+    if "deriv_two" in a:
+        d2 = np.asarray(a["deriv_two"], dtype=np.float64)
+        f.createDimension("FIFTEEN", d2.shape[2])
+        v = f.createVariable("deriv_two", "f8", ("nEdges", "TWO", "FIFTEEN"))
+        v[:] = d2 * (R * R) if unit_sphere else d2
     for name, arr in a.items():
-        if name in ("nominalMinDc",):
+        if name in ("nominalMinDc", "deriv_two"):
             continue
         if name in layout:
             d = shape_dims[layout[name]]
@@ -402,7 +409,10 @@ def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=Non
     moist = bool(int(attrs.get("moist", 1 if "qv" in arrays else 0)))
     state = hinit.InitialState(rho_zz=rho, rtheta_m=rtheta, ru=ru, rw=rw, scalars=scalars)
 
-    deriv = hsetup.deriv_two(mesh)
+    # the file's quadratic-fit second derivatives when it carries them (the
+    # reference builds its advection stencils from the static file's
+    # deriv_two), else this package's fit
+    deriv = mesh.arrays["deriv_two"] if "deriv_two" in mesh.arrays else hsetup.deriv_two(mesh)
     adv, adv3, nadv, advc = hsetup.advection_coefficients(mesh, deriv)
     etp, etm, ewp, ewm = hsetup.acoustic_offcentering(
         grid.rdzw, minimum=cfg.config_epssm_minimum, maximum=cfg.config_epssm_maximum,

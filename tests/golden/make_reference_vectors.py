@@ -39,7 +39,7 @@ from reference_walk import (ACOUSTIC_FORCING, ACOUSTIC_STATE, COEFFICIENTS, DAMP
 F32 = np.float32
 
 
-def mesh_view(case):
+def mesh_view(case, geometry=F32):
     """The mesh as the reference's operators read it: MPAS names, 0-based
     connectivity, binary32 geometry (the values the oracle is given)."""
     m = case.mesh
@@ -57,7 +57,7 @@ def mesh_view(case):
                 a = np.asarray(m.arrays[n])
             else:
                 continue
-        setattr(ns, n, a.astype(F32) if a.dtype.kind == "f" else a.astype(np.int64))
+        setattr(ns, n, a.astype(geometry) if a.dtype.kind == "f" else a.astype(np.int64))
     ns.nCells, ns.nEdges, ns.nVertices = m.n_cells, m.n_edges, m.n_vertices
     ns.maxEdges = m.max_edges
     ns.arrays = {n: getattr(ns, n) for n in names if hasattr(ns, n)}
@@ -303,6 +303,42 @@ def powf_vectors(case):
             "host_libm_differs": np.array([libm_differs], dtype=np.int64)}
 
 
+def advection_vectors(case):
+    """The set-up builders of the advection stencils: transport.py
+    initialize_deriv_two (:610-738, the quadratic-fit second derivatives)
+    and the v8.4.1 compression of adv_coefs/adv_coefs_3rd in native
+    insertion order (_native_advection_order, :2264-2403, and
+    _native_advection_compression, :2405-2443; build_advection_coefficients
+    itself calls the absent Rust engine first), fed this package's stencil
+    and deriv_two."""
+    from hexcore.transport import _native_advection_compression, _native_advection_order, initialize_deriv_two
+    from hexdyn import setup as hsetup
+
+    m = case.mesh
+    mesh64 = mesh_view(case, geometry=np.float64)
+    mesh64.sphere_radius = float(m.sphere_radius)
+    ours_d2 = hsetup.deriv_two(m)
+    ref_d2 = np.asarray(initialize_deriv_two(mesh64))
+    w = ours_d2.shape[2]
+    assert not ref_d2[:, :, w:].any()
+    a, b = ref_d2[:, :, :w], ours_d2
+    rel = float(np.abs(a - b).max() / np.abs(a).max())
+    same32 = float(np.mean(a.astype(F32).view(np.uint32) == b.astype(F32).view(np.uint32)))
+    print(f"deriv_two (binary64): max |diff| / max |ref| = {rel:.3e}; equal after rounding to binary32: {same32:.4f}")
+    counts = np.asarray(m.nEdgesOnCell, dtype=np.int64)
+    coc = np.asarray(m.cellsOnCell, dtype=np.int64)
+    coe = np.asarray(m.cellsOnEdge, dtype=np.int64)
+    d2 = np.asarray(ours_d2, dtype=F32)
+    dc, dv = m.dcEdge.astype(F32), m.dvEdge.astype(F32)
+    nadv = np.asarray(case.n_adv, dtype=np.int64)
+    cells = np.asarray(case.adv_cells, dtype=np.int64)
+    zeros = np.zeros(cells.shape, F32)
+    _, _, ordered = _native_advection_order(counts, coc, coe, zeros, zeros, nadv, cells, garbage=-1)
+    adv, adv3 = _native_advection_compression(counts, coc, coe, d2, dc, dv, nadv, cells)
+    return {"deriv_two_reference": ref_d2[:, :, :w], "deriv_two": d2, "adv_cells": np.asarray(ordered),
+            "adv_coefs": np.asarray(adv, F32), "adv_coefs_3rd": np.asarray(adv3, F32)}
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -310,6 +346,7 @@ VECTORS = {
     "divergence_damping": damping_vectors,
     "recovery": recovery_vectors,
     "powf": powf_vectors,
+    "advection": advection_vectors,
 }
 
 

@@ -151,3 +151,33 @@ def test_init_file_without_rw_or_metrics_is_refused(get_case, tmp_path):
     mpas_io.write_mpas_grid(case.mesh, p)
     with pytest.raises(mpas_io.MpasFileError, match="vertical"):
         mpas_io.case_from_mpas(p, config=case.config)
+
+
+@pytest.mark.parametrize("unit_sphere", [False, True])
+def test_file_deriv_two_builds_the_advection_stencils(get_case, tmp_path, unit_sphere):
+    """A file that carries deriv_two (nEdges, TWO, FIFTEEN) feeds it to the
+    advection-coefficient compression, as the reference builds its stencils
+    from the static file's deriv_two; a file without it gets this package's
+    fit.  The same values give the same stencils bit for bit; perturbed
+    values give different ones, so the file's copy is the one used."""
+    import dataclasses
+
+    from hexdyn import setup as hsetup
+    from hexdyn.mesh import Mesh
+
+    case = get_case("x1.642-moist")
+    d2 = np.zeros((case.mesh.n_edges, 2, 15))
+    d2[:, :, :case.mesh.max_edges + 1] = hsetup.deriv_two(case.mesh)
+    for scale, same in ((1.0, True), (1.0 + 2.0 ** -10, False)):
+        mesh = Mesh(arrays={**case.mesh.arrays, "deriv_two": d2 * scale}, sphere_radius=case.mesh.sphere_radius)
+        p = tmp_path / f"x1.642.{scale}.init.nc"
+        mpas_io.write_mpas_init(dataclasses.replace(case, mesh=mesh), p)
+        if unit_sphere:
+            g = tmp_path / f"x1.642.{scale}.grid.nc"
+            mpas_io.write_mpas_grid(mesh, g, unit_sphere=True)
+            back = mpas_io.case_from_mpas(p, grid_path=g, config=case.config)
+        else:
+            back = mpas_io.case_from_mpas(p, config=case.config)
+        assert "deriv_two" in back.mesh.arrays
+        for name in ("adv_coefs", "adv_coefs_3rd"):
+            assert _same_bits(getattr(case, name), getattr(back, name)) == same, (scale, name)

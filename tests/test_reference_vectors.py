@@ -3,7 +3,7 @@
 `tests/golden/reference_<name>.npz` holds the reference's outputs for the
 functions only its own code pins bit for bit, written by
 `tests/golden/make_reference_vectors.py` from diagnostics.py,
-acoustic_v841.py, mixing.py and integration.py on operands taken from the
+acoustic_v841.py, mixing.py, integration.py and transport.py on operands taken from the
 oracle at fixed checkpoints of one large step (`tests/golden/reference_walk.py`).
 Each test walks the oracle to the same checkpoint, checks that it holds the
 same operands (sha256), runs the phase that restates the function and
@@ -24,7 +24,7 @@ sys.path.insert(0, str(GOLDEN))
 
 import reference_walk as rw  # noqa: E402
 
-NAMES = ["solve_diagnostics", "vertical_coefficients", "acoustic", "divergence_damping", "recovery"]
+NAMES = ["solve_diagnostics", "vertical_coefficients", "acoustic", "divergence_damping", "recovery", "advection"]
 
 
 def _load(name):
@@ -159,3 +159,24 @@ def test_recovery_matches_reference_bits(case):
         if f"{name}.operands" in z.files:
             pending = name
     assert seen == 2
+
+
+def test_advection_stencils_match_reference_bits(case):
+    """The v8.4.1 advection-stencil compression in native insertion order
+    (transport.py _native_advection_order :2264-2403 and
+    _native_advection_compression :2405-2443) from the same deriv_two:
+    cells, adv_coefs and adv_coefs_3rd word for word.  The quadratic-fit
+    deriv_two itself (initialize_deriv_two, :610-738) is a different
+    evaluation of the same least-squares problem here: within 1e-10
+    relative in binary64 (a file's deriv_two is used as given)."""
+    from hexdyn import setup as hsetup
+
+    z = _load("advection")
+    d2 = hsetup.deriv_two(case.mesh)
+    assert np.array_equal(d2.astype(np.float32).view(np.uint32), z["deriv_two"].view(np.uint32))
+    adv, adv3, nadv, cells = hsetup.advection_coefficients(case.mesh, z["deriv_two"])
+    assert np.array_equal(cells, z["adv_cells"])
+    _same_bits(adv, z["adv_coefs"], "adv_coefs")
+    _same_bits(adv3, z["adv_coefs_3rd"], "adv_coefs_3rd")
+    ref = z["deriv_two_reference"]
+    assert np.abs(d2 - ref).max() <= 1e-10 * np.abs(ref).max()
