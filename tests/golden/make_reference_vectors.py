@@ -33,8 +33,8 @@ sys.path.insert(0, str(ROOT / "gpuwm-hex_amd"))
 sys.path.insert(0, str(ROOT))
 
 from reference_walk import (ACOUSTIC_FORCING, ACOUSTIC_STATE, COEFFICIENTS, DAMPING, RECOVERY_STATE,  # noqa: E402
-                            VCOEF, build_case as build, case_digest, operand_digest, read, read_all, schedule,
-                            walk)
+                            TRANSPORT, TRANSPORT_FLUX, VCOEF, build_case as build, case_digest, operand_digest, read,
+                            read_all, schedule, walk, walk_transport)
 
 F32 = np.float32
 
@@ -339,6 +339,65 @@ def advection_vectors(case):
             "adv_coefs": np.asarray(adv, F32), "adv_coefs_3rd": np.asarray(adv3, F32)}
 
 
+def transport_vectors(case):
+    """transport.advance_scalars (:1649-1826, stages 1 and 2, split-transport
+    density weights) and advance_scalars_monotonic (:1829-2175, stage 3,
+    Zalesak FCT with the split-transport target density), stored-inverse
+    area, third-order fluxes with the upwind blend, fed the oracle's own
+    time-averaged fluxes and scalars of each stage."""
+    from hexcore.transport import advance_scalars, advance_scalars_monotonic
+    from hexdyn.setup import rk_schedule
+
+    c = case.config
+    sca = rk_schedule(c.config_dt, order=c.config_time_integration_order,
+                      acoustic_substeps=c.config_number_of_sub_steps, dynamics_splits=1)
+    lev = _levels(case)
+    mesh = mesh_view(case)
+    common = dict(adv_coefs=case.adv_coefs.astype(F32), adv_coefs_3rd=case.adv_coefs_3rd.astype(F32),
+                  n_adv_cells_for_edge=np.asarray(case.n_adv), adv_cells_for_edge=np.asarray(case.adv_cells),
+                  fzm=lev["fzm"], fzp=lev["fzp"], rdzw=lev["rdzw"],
+                  config_coef_3rd_order=float(c.config_coef_3rd_order), advance_density=True,
+                  inv_area_cell=case.inv_area_cell.astype(F32))
+    out = {}
+    held = {}
+    for label, o in walk_transport(case):
+        st = int(label.split("_")[1])
+        if label.endswith("_out"):
+            if st == 1:
+                held.update(read_all(o, case, TRANSPORT_FLUX))  # the 1/splits average is taken at stage 1
+            ops = held[st]
+            stage_in = ops.pop("scalar_stage")
+            args = dict(ops, scalar_stage=stage_in, **{k: held[k] for k in ("uh_avg", "ww_avg")})
+            if st < 3:
+                res = advance_scalars(mesh, args["scalar_old"], args["scalar_stage"], args["rho_zz_old"],
+                                      args["rho_zz_new"], args["uh_avg"], args["ww_avg"],
+                                      sca[st - 1].large_timestep, rk_step=st,
+                                      config_time_integration_order=c.config_time_integration_order, **common)
+            else:
+                res = advance_scalars_monotonic(mesh, args["scalar_old"], args["scalar_stage"], args["rho_zz_old"],
+                                                args["rho_zz_new"], args["uh_avg"], args["ww_avg"],
+                                                sca[2].large_timestep, **common)
+                plain = advance_scalars(mesh, args["scalar_old"], args["scalar_stage"], args["rho_zz_old"],
+                                        args["rho_zz_new"], args["uh_avg"], args["ww_avg"], sca[2].large_timestep,
+                                        rk_step=3, config_time_integration_order=c.config_time_integration_order,
+                                        **common)
+                limited = int(np.count_nonzero(np.asarray(res.scalars) != np.asarray(plain.scalars)))
+                print(f"transport stage 3: the limiter changes {limited} of {res.scalars.size} mixing ratios")
+                assert limited > 0
+                out["scalars_3.limited"] = np.array([limited], dtype=np.int64)
+            val = np.asarray(res.scalars)
+            assert val.dtype == F32
+            out[f"{label[:-4]}.operands"] = operand_digest(
+                [args[k] for k in ("scalar_old", "scalar_stage", "rho_zz_old", "rho_zz_new", "uh_avg", "ww_avg")])
+            out[f"{label[:-4]}.scalars"] = np.ascontiguousarray(val)
+            out[f"{label[:-4]}.density"] = np.ascontiguousarray(np.asarray(res.density, dtype=F32))
+        else:
+            ops = read_all(o, case, TRANSPORT)
+            ops["scalar_stage"] = ops["scalar_old"] if st == 1 else read(o, case, "F:SCALARS_STAGE", "tracer")
+            held[st] = ops
+    return out
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -347,6 +406,7 @@ VECTORS = {
     "recovery": recovery_vectors,
     "powf": powf_vectors,
     "advection": advection_vectors,
+    "transport": transport_vectors,
 }
 
 

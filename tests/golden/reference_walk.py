@@ -79,9 +79,41 @@ def walk(case):
     yield "recover_3_out", o
 
 
+def walk_transport(case):
+    """Yield (checkpoint, oracle) around the three transport stages of a
+    large step (after its three dynamics subcycles)."""
+    from hexdyn.case import initial_state_arrays
+    from oracle.oracle import Oracle
+
+    sched = schedule(case)
+    c = case.config
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(1) == 0
+    o.begin_step()
+    for sub in range(1, c.config_dynamics_split_steps + 1):
+        o.begin_subcycle(sub)
+        for st in (1, 2, 3):
+            o.dyn_tend(st)
+            for ss in range(1, sched[st - 1].acoustic_steps + 1):
+                o.acoustic_substep(st, ss)
+                o.divergence_damping(st)
+            assert o.recover(st) == 0
+            o.solve_diagnostics(st)
+        o.end_subcycle(sub)
+    for st in (1, 2, 3):
+        yield f"scalars_{st}", o
+        o.scalars(st)
+        yield f"scalars_{st}_out", o
+
+
+TRANSPORT = [("scalar_old", "I:o_q", "tracer"), ("rho_zz_old", "I:o_rho", "cell"), ("rho_zz_new", "I:c_rho", "cell")]
+TRANSPORT_FLUX = [("uh_avg", "I:split_u", "edge"), ("ww_avg", "I:split_w", "w")]
+
+
 def shapes(case):
     L, C, E = case.n_levels, case.mesh.n_cells, case.mesh.n_edges
-    return {"cell": (L, C), "edge": (L, E), "w": (L + 1, C)}
+    return {"cell": (L, C), "edge": (L, E), "w": (L + 1, C), "tracer": (case.n_scalars, L, C)}
 
 
 # what each reference operator reads, as (argument, source, shape); source is a

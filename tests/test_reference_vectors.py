@@ -180,3 +180,28 @@ def test_advection_stencils_match_reference_bits(case):
     _same_bits(adv3, z["adv_coefs_3rd"], "adv_coefs_3rd")
     ref = z["deriv_two_reference"]
     assert np.abs(d2 - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_transport_stages_match_reference_bits(case):
+    """advance_scalars (transport.py:1649-1826) at stages 1 and 2 and
+    advance_scalars_monotonic (:1829-2175, Zalesak FCT) at stage 3, all six
+    species, split-transport densities: every mixing ratio word for word."""
+    z = _load("transport")
+    assert bytes(z["case_digest"]).decode() == bytes(rw.case_digest(case)).decode()
+    held, seen = {}, 0
+    for name, o in rw.walk_transport(case):
+        st = int(name.split("_")[1])
+        if not name.endswith("_out"):
+            ops = rw.read_all(o, case, rw.TRANSPORT)
+            ops["scalar_stage"] = ops["scalar_old"] if st == 1 else rw.read(o, case, "F:SCALARS_STAGE", "tracer")
+            held[st] = ops
+            continue
+        if st == 1:
+            held.update(rw.read_all(o, case, rw.TRANSPORT_FLUX))
+        ops = held[st]
+        digest = rw.operand_digest([ops["scalar_old"], ops["scalar_stage"], ops["rho_zz_old"], ops["rho_zz_new"],
+                                    held["uh_avg"], held["ww_avg"]])
+        assert bytes(z[f"scalars_{st}.operands"]).decode() == bytes(digest).decode(), f"stage {st} operands"
+        _same_bits(rw.read(o, case, "F:SCALARS_STAGE", "tracer"), z[f"scalars_{st}.scalars"], f"stage {st} scalars")
+        seen += 1
+    assert seen == 3
