@@ -1277,7 +1277,8 @@ template <class O> static auto *internal(O &o, const std::string &n) {
       {"c_theta", &o.c_theta}, {"c_rho_p", &o.c_rho_p}, {"c_rtheta_p", &o.c_rtheta_p},
       {"w", &o.w}, {"u", &o.u}, {"exner", &o.exner}, {"pp", &o.pp},
       {"cofwr", &o.cofwr}, {"rtheta_pp_old", &o.rtheta_pp_old}, {"tend_rho", &o.tend_rho},
-      {"split_u", &o.split_u}, {"split_w", &o.split_w}, {"o_rho", &o.o_rho}, {"o_q", &o.o_q}};
+      {"split_u", &o.split_u}, {"split_w", &o.split_w}, {"o_rho", &o.o_rho}, {"o_q", &o.o_q},
+      {"dpdz", &o.dpdz}};
   for (const auto &t : table)
     if (n == t.first) return t.second;
   return (V *)nullptr;

@@ -32,7 +32,7 @@ ROOT = HERE.parent.parent
 sys.path.insert(0, str(ROOT / "gpuwm-hex_amd"))
 sys.path.insert(0, str(ROOT))
 
-from reference_walk import (ACOUSTIC_FORCING, ACOUSTIC_STATE, COEFFICIENTS, DAMPING, RECOVERY_STATE,  # noqa: E402
+from reference_walk import (ACOUSTIC_FORCING, ACOUSTIC_STATE, COEFFICIENTS, DAMPING, POOLS, RECOVERY_STATE,  # noqa: E402
                             TRANSPORT, TRANSPORT_FLUX, VCOEF, build_case as build, case_digest, operand_digest, read,
                             read_all, schedule, walk, walk_transport)
 
@@ -398,6 +398,33 @@ def transport_vectors(case):
     return out
 
 
+def pool_vectors(case):
+    """The subcycle-start pools: dynamics.density_tendency (:122-148, with
+    mass_flux_divergence :86-119) and dynamics.pressure_gradient_euler_tendency
+    (:188-224), stored inverses, from the subcycle-start rho_u, rho_w,
+    pressure perturbation and moist dp/dz."""
+    from hexcore.dynamics import density_tendency, pressure_gradient_euler_tendency
+
+    mesh = mesh_view(case)
+    g = case.grid
+    out = {}
+    for label, o in walk(case):
+        if label != "vcoef_1":
+            continue
+        ops = read_all(o, case, POOLS)
+        rho = density_tendency(mesh, ops["rho_u"], ops["rho_w"], np.asarray(g.rdzw, F32),
+                               inv_area_cell=case.inv_area_cell.astype(F32))
+        pgf = pressure_gradient_euler_tendency(mesh, ops["pressure_perturbation"], ops["dpdz"], ops["cqu"],
+                                               g.zz.astype(F32), g.zxu.astype(F32),
+                                               inv_dc_edge=case.inv_dc_edge.astype(F32))
+        assert rho.dtype == F32 and pgf.dtype == F32
+        out["pools_1.operands"] = operand_digest(ops.values())
+        out["pools_1.tend_rho"] = np.ascontiguousarray(rho)
+        out["pools_1.euler_ru"] = np.ascontiguousarray(pgf)
+        break
+    return out
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -407,6 +434,7 @@ VECTORS = {
     "powf": powf_vectors,
     "advection": advection_vectors,
     "transport": transport_vectors,
+    "pools": pool_vectors,
 }
 
 

@@ -143,6 +143,8 @@ RECOVERY_OUT = [("rho_p", "I:c_rho_p", "cell"), ("rho_zz", "I:c_rho", "cell"), (
                 ("ru_avg", "F:FLUX_U", "edge"), ("ru", "I:c_ru", "edge"), ("u", "I:u", "edge")]
 
 
+POOLS = [("rho_u", "I:s_ru", "edge"), ("rho_w", "I:s_rw", "w"), ("pressure_perturbation", "I:pp", "cell"),
+         ("dpdz", "I:dpdz", "cell"), ("cqu", "F:CQU", "edge")]
 VCOEF = {1: [("cqw", "F:CQW", "cell"), ("exner", "I:s_exner", "cell"), ("theta", "I:s_theta", "cell"),
              ("rho_theta_perturbation", "I:s_rtheta_p", "cell"), ("qtot", "F:QTOT", "cell")],
          2: [("cqw", "F:CQW", "cell"), ("exner", "I:s_exner", "cell"), ("theta", "I:c_theta", "cell"),

@@ -205,3 +205,23 @@ def test_transport_stages_match_reference_bits(case):
         _same_bits(rw.read(o, case, "F:SCALARS_STAGE", "tracer"), z[f"scalars_{st}.scalars"], f"stage {st} scalars")
         seen += 1
     assert seen == 3
+
+
+def test_subcycle_pools_match_reference_bits(case):
+    """density_tendency (dynamics.py:122-148) and
+    pressure_gradient_euler_tendency (:188-224) at the first subcycle start.
+    The density tendency follows the reference's device kernel
+    (cuda_horizontal.py:412-433, `0 - div`); the NumPy form `-div` differs
+    from it only in the sign of an exactly zero divergence, which these
+    operands do not hold."""
+    z = _load("pools")
+    assert bytes(z["case_digest"]).decode() == bytes(rw.case_digest(case)).decode()
+    for name, o in rw.walk(case):
+        if name == "vcoef_1":
+            ops = rw.read_all(o, case, rw.POOLS)
+            assert bytes(z["pools_1.operands"]).decode() == bytes(rw.operand_digest(ops.values())).decode()
+            assert np.count_nonzero(z["pools_1.tend_rho"] == 0) == 0
+            _same_bits(rw.read(o, case, "F:TEND_RHO", "cell"), z["pools_1.tend_rho"], "tend_rho")
+            _same_bits(rw.read(o, case, "F:EULER_RU", "edge"), z["pools_1.euler_ru"], "euler_ru")
+            return
+    raise AssertionError("checkpoint not reached")
