@@ -97,7 +97,7 @@ def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[Dy
     spec = dict(WORKLOADS[name])
     spec.update(overrides)
     cfg = config or DycoreConfig(config_dt=spec["dt"])
-    key = hashlib.sha256(repr((name, nlev, sorted(spec.items()), 5)).encode()).hexdigest()[:16]
+    key = hashlib.sha256(repr((name, nlev, sorted(spec.items()), 6)).encode()).hexdigest()[:16]
     path = _cache_dir() / f"case-{name}-{key}.pkl"
     if use_cache and path.exists():
         with open(path, "rb") as f:

@@ -244,8 +244,10 @@ def read_restart(path) -> Tuple[Dict[str, np.ndarray], Dict[str, object]]:
 #: MPAS names of the scalar slots, in the dycore's order (qv first: the moist
 #: coefficients sum the leading moisture species)
 SCALAR_NAMES = ("qv", "qc", "qr", "qi", "qs", "qg")
-#: vertical metrics an initialized file must carry (driver.py:1456-1459)
-VERTICAL_REQUIRED = ("zgrid", "rdzw", "dzu", "rdzu", "fzm", "fzp", "zz", "zxu", "dss", "zb", "zb3",
+#: vertical metrics an initialized file must carry (driver.py:1456-1459);
+#: its dss is not read: the v8.4.1 core recomputes the Rayleigh profile from
+#: zgrid at start-up (driver.py:2416-2421)
+VERTICAL_REQUIRED = ("zgrid", "rdzw", "dzu", "rdzu", "fzm", "fzp", "zz", "zxu", "zb", "zb3",
                      "cf1", "cf2", "cf3")
 
 
@@ -429,7 +431,8 @@ def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=Non
         inv_dv_edge=hsetup.stored_inverse(mesh.dvEdge),
         cos_angle_edge=np.cos(angle.astype(np.float64)).astype(f32),
         sin_angle_edge=np.sin(angle.astype(np.float64)).astype(f32),
-        etp=etp, etm=etm, ewp=ewp, ewm=ewm, dss=_levels_first(vert["dss"]).astype(f32),
+        etp=etp, etm=etm, ewp=ewp, ewm=ewm,
+        dss=hsetup.vertical_velocity_damping(grid.zgrid, xnutr=cfg.config_xnutr, zd=cfg.config_zd),
         u_init=np.asarray(arrays.get("u_init", np.zeros(L)), dtype=f32),
         v_init=np.asarray(arrays.get("v_init", np.zeros(L)), dtype=f32),
         len_disp=_len_disp(mesh, cfg),

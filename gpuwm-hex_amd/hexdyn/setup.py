@@ -5,7 +5,7 @@ the HIP library and by the oracle alike, so both consume identical bits.
 
 * RK schedule: ``integration.py:110-160`` (``RKSchedule.from_mpas``).
 * Off-centring ``etp/etm/ewp/ewm``: ``offcentering_v841.py:45-174``.
-* Rayleigh w damping ``dss``: ``damping_v841.py`` (``_vertical_velocity_damping_serial``).
+* Rayleigh w damping ``dss``: ``damping_v841.py:23-107`` (the binary32 path, host sinf).
 * Quadratic-fit second derivatives ``deriv_two``: ``transport.py:610``
   (``initialize_deriv_two``), the MPAS ``helpers.F`` least-squares method.
 * Advection coefficient compression in native insertion order:
@@ -97,8 +97,24 @@ def acoustic_offcentering(rdzw: np.ndarray, *, minimum: float = 0.1, maximum: fl
     return etp, etm, ewp, ewm
 
 
+def _host_sinf(x: np.ndarray) -> np.ndarray:
+    """binary32 sine by the host C library's sinf, elementwise."""
+    import ctypes
+    import ctypes.util
+
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    fn = libm.sinf
+    fn.argtypes = [ctypes.c_float]
+    fn.restype = ctypes.c_float
+    flat = np.asarray(x, dtype=F32).ravel()
+    return np.fromiter((fn(float(v)) for v in flat), dtype=F32, count=flat.size).reshape(np.shape(x))
+
+
 def vertical_velocity_damping(zgrid: np.ndarray, *, xnutr: float, zd: float) -> np.ndarray:
-    """``dss`` (L, C): xnutr*sin^2(pi/2*(zmid-zd)/(ztop-zd)) above ``zd``."""
+    """``dss`` (L, C): xnutr*sin^2(pi/2*(zmid-zd)/(ztop-zd)) above ``zd``,
+    recomputed from zgrid as the v8.4.1 core does at start-up
+    (damping_v841.py:23-107): binary32 phase, the host C library's sinf
+    (the GNU sinf of the native build), then one product for the square."""
     z = np.asarray(zgrid, dtype=F32)
     nlev = z.shape[0] - 1
     dss = np.zeros((nlev, z.shape[1]), dtype=F32)
@@ -111,7 +127,7 @@ def vertical_velocity_damping(zgrid: np.ndarray, *, xnutr: float, zd: float) -> 
     top = z[nlev][None, :].repeat(nlev, axis=0)
     active = mid > start
     phase = (half * pi) * (mid[active] - start) / (top[active] - start)
-    s = np.sin(phase.astype(np.float64)).astype(F32)
+    s = _host_sinf(phase)
     dss[active] = F32(xnutr) * (s * s)
     return dss
 

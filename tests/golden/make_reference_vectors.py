@@ -425,6 +425,22 @@ def pool_vectors(case):
     return out
 
 
+def damping_profile_vectors(case):
+    """damping_v841.build_v841_vertical_velocity_damping (:23-107), the
+    binary32 path (the host C library's sinf, the reference's default
+    numerics), on the zgrid of the x1.162 case at 55 levels to 30 km with
+    the Rayleigh layer from 22 km."""
+    from hexcore.damping_v841 import build_v841_vertical_velocity_damping
+    from hexdyn.case import build_case as build55
+
+    c = build55("x1.162-moist", nlev=55, use_cache=False)
+    z = np.asarray(c.grid.zgrid, dtype=F32)
+    cfg = c.config
+    dss = build_v841_vertical_velocity_damping(z, xnutr=cfg.config_xnutr, damping_start_z=cfg.config_zd)
+    assert dss.dtype == F32 and np.count_nonzero(dss) > 1000
+    return {"zgrid": z, "xnutr_zd": np.array([cfg.config_xnutr, cfg.config_zd]), "dss": np.ascontiguousarray(dss)}
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -435,6 +451,7 @@ VECTORS = {
     "advection": advection_vectors,
     "transport": transport_vectors,
     "pools": pool_vectors,
+    "damping_profile": damping_profile_vectors,
 }
 
 

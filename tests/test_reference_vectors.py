@@ -225,3 +225,13 @@ def test_subcycle_pools_match_reference_bits(case):
             _same_bits(rw.read(o, case, "F:EULER_RU", "edge"), z["pools_1.euler_ru"], "euler_ru")
             return
     raise AssertionError("checkpoint not reached")
+
+
+def test_damping_profile_matches_reference_bits():
+    """build_v841_vertical_velocity_damping (damping_v841.py:23-107): the
+    Rayleigh w damping recomputed from zgrid, word for word."""
+    from hexdyn import setup as hsetup
+
+    z = _load("damping_profile")
+    xnutr, zd = (float(v) for v in z["xnutr_zd"])
+    _same_bits(hsetup.vertical_velocity_damping(z["zgrid"], xnutr=xnutr, zd=zd), z["dss"], "dss")
