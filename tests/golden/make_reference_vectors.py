@@ -441,6 +441,21 @@ def damping_profile_vectors(case):
     return {"zgrid": z, "xnutr_zd": np.array([cfg.config_xnutr, cfg.config_zd]), "dss": np.ascontiguousarray(dss)}
 
 
+def offcentering_vectors(case):
+    """offcentering_v841.build_v841_acoustic_offcentering (:45-174) on a
+    60-level stretched column to 45 km, so the sine transition between 30
+    and 50 km is exercised (the workloads' 30 km tops never reach it)."""
+    from hexcore.offcentering_v841 import build_v841_acoustic_offcentering
+
+    k = np.arange(60, dtype=np.float64)
+    dz = (150.0 + 1500.0 * (k / 59.0) ** 1.5).astype(F32)
+    dz = (dz * F32(45000.0 / float(dz.astype(np.float64).sum()))).astype(F32)
+    rdzw = (F32(1.0) / dz).astype(F32)
+    res = build_v841_acoustic_offcentering(rdzw)
+    assert float(res.height_w_levels[-1]) > 40000.0
+    return {"rdzw": rdzw, **{n: np.asarray(getattr(res, n), dtype=F32) for n in ("etp", "etm", "ewp", "ewm")}}
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -452,6 +467,7 @@ VECTORS = {
     "transport": transport_vectors,
     "pools": pool_vectors,
     "damping_profile": damping_profile_vectors,
+    "offcentering": offcentering_vectors,
 }
 
 

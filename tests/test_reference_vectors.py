@@ -235,3 +235,15 @@ def test_damping_profile_matches_reference_bits():
     z = _load("damping_profile")
     xnutr, zd = (float(v) for v in z["xnutr_zd"])
     _same_bits(hsetup.vertical_velocity_damping(z["zgrid"], xnutr=xnutr, zd=zd), z["dss"], "dss")
+
+
+def test_offcentering_matches_reference_bits():
+    """build_v841_acoustic_offcentering (offcentering_v841.py:45-174) on a
+    column whose top reaches into the 30-50 km sine transition."""
+    from hexdyn import setup as hsetup
+
+    z = _load("offcentering")
+    got = hsetup.acoustic_offcentering(z["rdzw"])
+    for name, val in zip(("etp", "etm", "ewp", "ewm"), got):
+        _same_bits(val, z[name], name)
+    assert len(set(z["etp"].tolist())) > 5  # the transition is exercised
