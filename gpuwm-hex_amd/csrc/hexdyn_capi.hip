@@ -217,11 +217,12 @@ namespace {
 // power, i.e. differs from hexdyn_powf_eval.  Built once per process, in
 // word-aligned slices over the host's cores; every marked argument is checked
 // to be the neighbour hexdyn_gnu_powf_fix will choose.
-const std::vector<uint32_t> &gnu_powf_ties(float y, std::string *failure) {
+const std::vector<uint32_t> &gnu_powf_ties(float y, std::string *failure, double *near_tie = nullptr) {
   static std::once_flag once;
   static std::vector<uint32_t> bits;
   static std::string fail;
   static float built_for = 0.0f;
+  static double nearest = 0.5;  // smallest |d - r| in ulps over the marked arguments
   std::call_once(once, [y] {
     built_for = y;
     const uint32_t n = HEXDYN_POW_HI - HEXDYN_POW_LO;
@@ -229,6 +230,7 @@ const std::vector<uint32_t> &gnu_powf_ties(float y, std::string *failure) {
     bits.assign(words, 0u);
     const unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     std::vector<std::string> errs(nt);
+    std::vector<double> near(nt, 0.5);
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nt; ++t)
       pool.emplace_back([&, t] {
@@ -246,6 +248,8 @@ const std::vector<uint32_t> &gnu_powf_ties(float y, std::string *failure) {
             acc |= 1u << b;
             const uint32_t ir = hexdyn_f32_bits(r);
             const uint32_t fixed = d > (double)r ? ir + 1u : ir - 1u;  // hexdyn_gnu_powf_fix's choice
+            const double ulp = (double)hexdyn_f32_from_bits(ir + 1u) - (double)r;
+            near[t] = std::min(near[t], std::fabs(d - (double)r) / ulp);
             if (fixed != hexdyn_f32_bits(g) && errs[t].empty()) {
               char msg[160];
               std::snprintf(msg, sizeof msg, "host powf(%a, %a) = %a is not the neighbour of %a the table encodes",
@@ -257,6 +261,8 @@ const std::vector<uint32_t> &gnu_powf_ties(float y, std::string *failure) {
         }
       });
     for (auto &th : pool) th.join();
+    for (double v : near) nearest = std::min(nearest, v);
+    nearest = nearest * (1.0 - 1e-6);  // margin for the ulp and difference roundings
     for (const auto &e : errs)
       if (!e.empty()) {
         fail = e;
@@ -265,6 +271,7 @@ const std::vector<uint32_t> &gnu_powf_ties(float y, std::string *failure) {
   });
   if (built_for != y && fail.empty()) fail = "the GNU powf table serves one exponent per process";
   if (failure) *failure = fail;
+  if (near_tie) *near_tie = nearest;
   return bits;
 }
 
@@ -333,8 +340,9 @@ float hexdyn_powf(float x, float y) {
   const float r = (float)d;
   if (y != rcv || !(x > 0.0f)) return r;
   std::string why;
-  const std::vector<uint32_t> &ties = gnu_powf_ties(rcv, &why);
-  return why.empty() ? hexdyn_gnu_powf_fix(x, r, d, ties.data()) : r;
+  double near_tie = 0.0;
+  const std::vector<uint32_t> &ties = gnu_powf_ties(rcv, &why, &near_tie);
+  return why.empty() ? hexdyn_gnu_powf_fix(x, r, d, ties.data(), near_tie) : r;
 }
 
 int hexdyn_last_error(const hexdyn_handle *h, char *buf, size_t len) {
@@ -713,7 +721,7 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   if (K.rcv != (float)(287.0 / (1004.5 - 287.0))) return refuse("inconsistent binary32 R/cv");
   {
     std::string why;
-    const std::vector<uint32_t> &ties = gnu_powf_ties(K.rcv, &why);
+    const std::vector<uint32_t> &ties = gnu_powf_ties(K.rcv, &why, &K.powf_near_tie);
     if (!why.empty()) return refuse(why.c_str());
     if (h->upload(&h->I.m.powf_ties, ties)) {
       hexdyn_destroy(h);

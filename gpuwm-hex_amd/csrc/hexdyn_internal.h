@@ -113,6 +113,7 @@ struct Work {
 
 struct Consts {
   float gravity, rcv, c2, cfac, rgas;  // binary32 constants in reference form
+  double powf_near_tie;                // GNU powf correction: bitmap read only this close to a tie (ulps)
   float coef3, apvm_scale, divdamp_base;
   float dts[3];            // acoustic timestep per stage
   float dts_inv_coef[3];   // divergence damping coefficient per stage

@@ -107,7 +107,7 @@ __device__ __forceinline__ void set_flag(int *flag) { atomicOr(flag, 1); }
 // moved to the other neighbour where the host-built bitmap marks the argument.
 __device__ __forceinline__ float exner_pow(const Instance &I, float arg) {
   const double d = hexdyn_pow_d(arg, I.k.rcv);
-  return hexdyn_gnu_powf_fix(arg, (float)d, d, I.m.powf_ties);
+  return hexdyn_gnu_powf_fix(arg, (float)d, d, I.m.powf_ties, I.k.powf_near_tie);
 }
 
 // Wave-uniform per-slot coefficients, loaded unconditionally (tables are

@@ -127,10 +127,17 @@ HEXDYN_HD static inline float hexdyn_powf_eval(float xf, float yf) { return (flo
 
 /* The GNU powf bits of a positive argument x whose correctly rounded power
  * is r (from the binary64 value d): where the bitmap marks x, the GNU result
- * is the neighbour of r on the side of the exact value. */
-HEXDYN_HD static inline float hexdyn_gnu_powf_fix(float x, float r, double d, const uint32_t *ties) {
+ * is the neighbour of r on the side of the exact value.  Every marked
+ * argument lies within a hair of a rounding midpoint (|d - r| >= 0.4983 ulp
+ * over the whole range for the exner exponent), so the bitmap is read only
+ * where |d - r| >= near_tie ulp, the smallest such distance the host found
+ * when it built the bitmap (about 0.4 % of arguments). */
+HEXDYN_HD static inline float hexdyn_gnu_powf_fix(float x, float r, double d, const uint32_t *ties, double near_tie) {
   const uint32_t ix = hexdyn_f32_bits(x);
   if (ix < HEXDYN_POW_LO || ix >= HEXDYN_POW_HI) return r;
+  const double rd = (double)r;
+  const double ulp = (double)hexdyn_f32_from_bits(hexdyn_f32_bits(r) + 1u) - rd;
+  if (__builtin_fabs(d - rd) < near_tie * ulp) return r;
   const uint32_t i = ix - HEXDYN_POW_LO;
   if (!((ties[i >> 5] >> (i & 31u)) & 1u)) return r;
   const uint32_t ir = hexdyn_f32_bits(r);
