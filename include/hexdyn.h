@@ -257,7 +257,10 @@ int hexdyn_profile_kernels(hexdyn_handle *h, char *buf, size_t len);
  * with CuPy, cuda_fullstep_lts_active.py:1337-1360.) */
 int hexdyn_integrals(hexdyn_handle *h, double *out, int n_out, void *stream);
 
-/* The exner power function used by recovery, exposed for its accuracy test. */
+/* The exner power function of recovery (cuda_driver.py:1466-1472: the GNU
+ * powf), host copy, exposed for its tests: for y = R/cv and x in [2^-12, 4)
+ * the GNU C library's bits, elsewhere the correctly rounded binary64 method.
+ * The first call builds the process's correction bitmap (about 0.7 s). */
 float hexdyn_powf(float x, float y);
 
 #ifdef __cplusplus
