@@ -1156,6 +1156,7 @@ int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *p
   Work &B = h->I.b;
   if (!p) {
     h->I.k.physics = 0;
+    h->I.k.physics_q = 0;
     return 0;
   }
   const Sizes &n = h->I.n;
@@ -1176,6 +1177,17 @@ int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *p
     }
   }
   h->I.k.physics = 1;
+  h->I.k.physics_q = 0;
+  if (p->scalars && n.NT > 0) {
+    if (!B.phys_q) {
+      int rc = h->alloc(&B.phys_q, (size_t)n.NT * n.C * HX_ROW);
+      rc |= h->alloc(&B.q_src, (size_t)n.NT * n.C * HX_ROW);
+      if (rc) return set_error(h, HEXDYN_E_HIP, "physics buffer allocation failed");
+    }
+    if (transfer(h, FieldRef{B.phys_q, n.C, n.L, h->old_c, n.NT}, const_cast<float *>(p->scalars), true, s))
+      return HEXDYN_E_HIP;
+    h->I.k.physics_q = 1;
+  }
   return hipStreamSynchronize(s) == hipSuccess ? 0 : HEXDYN_E_HIP;
 }
 

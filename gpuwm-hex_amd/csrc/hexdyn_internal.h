@@ -108,6 +108,7 @@ struct Work {
   float *kdiff, *delsq_u, *delsq_div, *delsq_vort, *delsq_theta, *delsq_w, *mix_tw, *euler_theta;
   // held physics tendencies (rows; allocated when first installed)
   float *phys_rho, *phys_ru, *phys_rt;
+  float *phys_q, *q_src;   // held scalar tendency [T][C][64]; old scalars + dt*source/rho_old
   int *flag;  // sticky validation flag
 };
 
@@ -126,6 +127,7 @@ struct Consts {
   int apvm_on;
   int mixing, mix4;              // Smagorinsky mixing on; fourth-order part on (h4 > 0)
   int physics;                   // held physics tendencies are added
+  int physics_q;                 // a held scalar tendency enters the transport
   int store_flux;                // recovery also stores the stage's flux_u/flux_w (read only by
                                  // hexdyn_export_field; stage 3 folds them into the split sums)
   int fused;                     // whole-step launch sequence: the next stage's theta/w edge fluxes

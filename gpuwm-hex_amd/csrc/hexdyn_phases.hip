@@ -1473,7 +1473,7 @@ KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__
 
 template <int NE>
 KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__restrict__ ev, float *__restrict__ out,
-             float dt, float wn) {
+             float dt, float wn, const float *__restrict__ src) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
@@ -1495,7 +1495,7 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
   // software-pipelined over species: the next species' rows are requested
   // before the current one is combined (clamped index, in bounds)
   struct Sp {
-    float qk, qo, evv[NE];
+    float qk, qo, src, evv[NE];
   };
   auto load_sp = [&](int sp) {
     Sp a;
@@ -1503,6 +1503,7 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
 #pragma unroll
     for (int s = 0; s < NE; ++s) a.evv[s] = AT(ev, sp * I.n.E + e[s], k);
     a.qo = AT(I.b.q, sp * I.n.C + c, k);
+    a.src = src ? AT(src, sp * I.n.C + c, k) : 0.0f;  // held scalar tendency (transport.py:1800-1808)
     return a;
   };
   auto species = [&](int sp, const Sp &a) {
@@ -1516,7 +1517,7 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
 #pragma unroll
     for (int s = 0; s < NE; ++s)
       if (s < n) tc = tc + ((-sg[s]) * uh[s]) * evv[s];
-    tc = tc * ia + 0.0f;
+    tc = tc * ia + a.src;
     if (k < L) {
       const float num = qo * orho + dt * (tc - rdzw * (vf_p - vf));
       AT(out, sp * I.n.C + c, k) = num * (1.0f / target);
@@ -1527,6 +1528,20 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
     const Sp nxt = load_sp(sp + 1 < NT ? sp + 1 : sp);
     species(sp, cur);
     cur = nxt;
+  }
+}
+
+// old mixing ratio with the held scalar tendency, old + dt*source/rho_old
+// (transport.py:1906), every species of cell c
+KB k_fct_source(const Instance I, float dt) {
+  const int c = wave_entity();
+  if (c >= I.n.C) return;
+  const int k = lane();
+  if (k >= I.n.L) return;
+  const float rho = AT(I.b.rho, c, k);
+  for (int sp = 0; sp < I.n.NT; ++sp) {
+    const int r = sp * I.n.C + c;
+    AT(I.b.q_src, r, k) = AT(I.b.q, r, k) + (dt * AT(I.b.phys_q, r, k)) / rho;
   }
 }
 
@@ -1560,7 +1575,7 @@ KB k_fct_target(const Instance I, float dt) {
 }
 
 template <int NE>
-KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
+KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt, const float *__restrict__ qold, float qadd) {
   constexpr int NX = 2 * NE - 2;
   const int e = wave_entity();
   if (e >= I.n.E) return;
@@ -1590,8 +1605,8 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
       vb[s] = AT(qs, sq * I.n.C + ac[s], k);
     }
     const int pa = sp * I.n.C, pb = sq * I.n.C;
-    const float qa1 = AT(I.b.q, pa + c1, k) + 0.0f, qa2 = AT(I.b.q, pa + c2, k) + 0.0f;
-    const float qb1 = AT(I.b.q, pb + c1, k) + 0.0f, qb2 = AT(I.b.q, pb + c2, k) + 0.0f;
+    const float qa1 = AT(qold, pa + c1, k) + qadd, qa2 = AT(qold, pa + c2, k) + qadd;
+    const float qb1 = AT(qold, pb + c1, k) + qadd, qb2 = AT(qold, pb + c2, k) + qadd;
     float ra, rb;
     if (na == 10) {
       // ten-cell stencils associate from the first product (transport.py n_adv == 10 branch)
@@ -1628,7 +1643,7 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt) {
 }
 
 template <int NE>
-KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
+KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const float *__restrict__ qold, float qadd) {
   const int c = wave_entity();
   if (c >= I.n.C) return;
   const int k = lane();
@@ -1655,11 +1670,11 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt) {
     float qn[NE], upw[NE], hres[NE];
 #pragma unroll
     for (int s = 0; s < NE; ++s) {
-      qn[s] = AT(I.b.q, po + nb[s], k) + 0.0f;
+      qn[s] = AT(qold, po + nb[s], k) + qadd;
       upw[s] = AT(I.b.fct_upw, sp * I.n.E + e[s], k);
       hres[s] = AT(I.b.fct_hres, sp * I.n.E + e[s], k);
     }
-    const float qo = AT(I.b.q, po + c, k) + 0.0f;
+    const float qo = AT(qold, po + c, k) + qadd;
     const float qo_m = up1(qo), qo_p = dn1(qo);
     float a, b;
     if (k == 0) {
@@ -1946,12 +1961,21 @@ int scalars_stage(const Instance &I, int stage, hipStream_t s) {
   if (stage < 3 || !I.k.monotonic) {
     LAUNCH_NE(I, k_sc_edge, I.n.E, 1, s, I, qin, I.b.ev);
     LAUNCH_NE(I, k_sc_cell, I.n.C, 1, s, I, qin, (const float *)I.b.ev, I.b.q_next, dt,
-              I.k.wn[stage - 1]);
+              I.k.wn[stage - 1], (const float *)(I.k.physics_q ? I.b.phys_q : nullptr));
     return check();
   }
+  // the limiter's old mixing ratio: the published one plus +0, or the one
+  // the held scalar tendency moved (x + -0 == x keeps its bits)
+  const float *qold = I.b.q;
+  float qadd = 0.0f;
+  if (I.k.physics_q) {
+    launch(k_fct_source, I.n.C, 1, s, I, dt);
+    qold = I.b.q_src;
+    qadd = -0.0f;
+  }
   LAUNCH_NE(I, k_fct_target, I.n.C, 1, s, I, dt);
-  LAUNCH_NE(I, k_fct_edge1, I.n.E, 1, s, I, qin, dt);
-  LAUNCH_NE(I, k_fct_cell1, I.n.C, 1, s, I, qin, dt);
+  LAUNCH_NE(I, k_fct_edge1, I.n.E, 1, s, I, qin, dt, qold, qadd);
+  LAUNCH_NE(I, k_fct_cell1, I.n.C, 1, s, I, qin, dt, qold, qadd);
   launch(k_fct_edge2, I.n.E, 1, s, I);
   LAUNCH_NE(I, k_fct_cell2, I.n.C, 1, s, I, I.b.q_next);
   return check();

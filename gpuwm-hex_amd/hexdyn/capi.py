@@ -100,7 +100,7 @@ class HexdynConfig(ct.Structure):
 
 
 class HexdynPhysicsView(ct.Structure):
-    _fields_ = [("rho", ct.c_void_p), ("rho_u", ct.c_void_p), ("rho_theta", ct.c_void_p)]
+    _fields_ = [("rho", ct.c_void_p), ("rho_u", ct.c_void_p), ("rho_theta", ct.c_void_p), ("scalars", ct.c_void_p)]
 
 
 class HexdynStateView(ct.Structure):

@@ -201,18 +201,20 @@ class HexDycoreDriver:
         self.set_state(state, saved=state)
         return meta
 
-    def set_physics_tendencies(self, rho=None, rho_u=None, rho_theta=None) -> None:
-        """Hold physics tendencies (logical [L][entity] arrays) through the following steps;
-        all None clears them (hexdyn_set_physics_tendencies)."""
+    def set_physics_tendencies(self, rho=None, rho_u=None, rho_theta=None, scalars=None) -> None:
+        """Hold physics tendencies (logical [L][entity] arrays; scalars [T][L][C],
+        the transport's scalar_tendency) through the following steps; all None
+        clears them (hexdyn_set_physics_tendencies)."""
         torch = _torch()
-        if rho is None and rho_u is None and rho_theta is None:
+        if rho is None and rho_u is None and rho_theta is None and scalars is None:
             _check(self._h, lib().hexdyn_set_physics_tendencies(self._h, None, ct.c_void_p(self.stream_ptr())),
                    "hexdyn_set_physics_tendencies")
             return
         keep = {}
         view = capi.HexdynPhysicsView()
         for name, a, shape in (("rho", rho, self.shapes["rho_zz"]), ("rho_u", rho_u, self.shapes["ru"]),
-                               ("rho_theta", rho_theta, self.shapes["rtheta_m"])):
+                               ("rho_theta", rho_theta, self.shapes["rtheta_m"]),
+                               ("scalars", scalars, self.shapes["scalars"])):
             if a is None:
                 continue
             arr = np.ascontiguousarray(a, dtype=np.float32)

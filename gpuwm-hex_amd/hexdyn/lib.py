@@ -13,7 +13,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libhexdyn.so"
 CSRC = HERE.parent / "csrc"
 
-ABI_VERSION = 2  # HEXDYN_ABI_VERSION in include/hexdyn.h
+ABI_VERSION = 3  # HEXDYN_ABI_VERSION in include/hexdyn.h
 
 _lib = None
 

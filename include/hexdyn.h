@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define HEXDYN_ABI_VERSION 2
+#define HEXDYN_ABI_VERSION 3
 #define HEXDYN_MAX_EDGES 7      /* nEdgesOnCell limit (stencil widths derive from it) */
 #define HEXDYN_ROW 64           /* padded level row; nVertLevels + 1 <= 64 */
 
@@ -185,6 +185,9 @@ typedef struct hexdyn_physics_view {
   const float *rho;        /* [L][C] */
   const float *rho_u;      /* [L][E] */
   const float *rho_theta;  /* [L][C] */
+  const float *scalars;    /* [T][L][C], the transport's scalar_tendency (cuda_driver.py:5162-5163):
+                              added to the flux divergence at transport stages 1-2 and to the old
+                              mixing ratio as dt*source/rho_old before the limiter */
 } hexdyn_physics_view;
 
 typedef struct hexdyn_handle hexdyn_handle;

@@ -126,6 +126,9 @@ template <class T> struct Oracle {
   // held physics tendencies (cuda_driver.py:4294-4302, 4443-4446, 4484-4494)
   int physics = 0;
   std::vector<T> phys_rho, phys_ru, phys_rt;
+  // held scalar tendencies, the transport's scalar_tendency (cuda_driver.py:5162-5163)
+  int physics_q = 0;
+  std::vector<T> phys_q;
   Stage dyn[3], sca[3];
   int failed = 0;
   int in_step = 0;
@@ -1004,7 +1007,7 @@ template <class T> struct Oracle {
               const int e = eoc[(size_t)cc * ME + s];
               tc = tc + ((-sgn(cc, s)) * uh[ei(k, e)]) * ev[ei(k, e)];
             }
-            tc = tc * invarea[cc] + T(0);
+            tc = tc * invarea[cc] + (physics_q ? phys_q[(size_t)sp * cL() + ci(k, cc)] : T(0));
             const size_t i = ci(k, cc);
             const T num = qo[i] * o_rho[i] + dt_s * (tc - rdzw[k] * (vf[k + 1] - vf[k]));
             out[(size_t)sp * cL() + i] = num * (one / target[i]);
@@ -1036,7 +1039,9 @@ template <class T> struct Oracle {
       const T *qs = &q_stage[(size_t)sp * cL()];
       std::vector<T> qo(cL()), mn(cL()), mx(cL()), mass(cL()), sin_(cL()), sout(cL()), vres(wL()), hf(eL()),
           upw(eL()), hres(eL());
-      for (size_t i = 0; i < cL(); ++i) qo[i] = o_q[(size_t)sp * cL() + i] + T(0);
+      // old + dt*source/rho_old (transport.py:1906); a zero source adds +0
+      for (size_t i = 0; i < cL(); ++i)
+        qo[i] = o_q[(size_t)sp * cL() + i] + (physics_q ? (dt_s * phys_q[(size_t)sp * cL() + i]) / o_rho[i] : T(0));
 #pragma omp parallel for
       for (int e = 0; e < E; ++e)
         for (int k = 0; k < L; ++k) {
@@ -1284,7 +1289,8 @@ template <class O> static auto *internal(O &o, const std::string &n) {
   return (V *)nullptr;
 }
 
-template <class O> static void set_physics(O &o, const float *rho, const float *ru, const float *rt) {
+template <class O> static void set_physics(O &o, const float *rho, const float *ru, const float *rt,
+                                           const float *q) {
   using T = typename std::remove_reference<decltype(o.rho[0])>::type;
   auto in = [](std::vector<T> &dst, const float *src, size_t n) {
     dst.assign(n, T(0));
@@ -1295,6 +1301,8 @@ template <class O> static void set_physics(O &o, const float *rho, const float *
   in(o.phys_rho, rho, o.cL());
   in(o.phys_ru, ru, o.eL());
   in(o.phys_rt, rt, o.cL());
+  o.physics_q = q ? 1 : 0;
+  in(o.phys_q, q, (size_t)o.NT * o.cL());
 }
 
 struct Handle {
@@ -1370,8 +1378,8 @@ int oracle_set_post_transport_clamp(void *h, int enable) {
   return 0;
 }
 
-int oracle_set_physics(void *h, const float *rho, const float *rho_u, const float *rho_theta) {
-  DISPATCH(h, set_physics(o, rho, rho_u, rho_theta));
+int oracle_set_physics(void *h, const float *rho, const float *rho_u, const float *rho_theta, const float *q) {
+  DISPATCH(h, set_physics(o, rho, rho_u, rho_theta, q));
   return 0;
 }
 

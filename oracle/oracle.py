@@ -61,7 +61,7 @@ def lib() -> ct.CDLL:
         _lib.oracle_acoustic_substep.argtypes = [ct.c_void_p, ct.c_int, ct.c_int]
         _lib.oracle_powf.argtypes = [ct.c_float, ct.c_float]
         _lib.oracle_powf.restype = ct.c_float
-        _lib.oracle_set_physics.argtypes = [ct.c_void_p] * 4
+        _lib.oracle_set_physics.argtypes = [ct.c_void_p] * 5
         _lib.oracle_set_post_transport_clamp.argtypes = [ct.c_void_p, ct.c_int]
         _lib.oracle_model_time.argtypes = [ct.c_void_p]
         _lib.oracle_model_time.restype = ct.c_double
@@ -172,9 +172,11 @@ class Oracle:
     def step(self, n: int = 1) -> int:
         return lib().oracle_step(self.h, n)
 
-    def set_physics(self, rho=None, rho_u=None, rho_theta=None) -> None:
-        """Held physics tendencies (logical [L][entity]); all None clears them."""
-        self._phys = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (rho, rho_u, rho_theta)]
+    def set_physics(self, rho=None, rho_u=None, rho_theta=None, scalars=None) -> None:
+        """Held physics tendencies (logical [L][entity]; scalars [T][L][C]);
+        all None clears them."""
+        self._phys = [None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+                      for a in (rho, rho_u, rho_theta, scalars)]
         lib().oracle_set_physics(self.h, *[None if a is None else a.ctypes.data for a in self._phys])
 
     def set_post_transport_clamp(self, enable: bool) -> None:

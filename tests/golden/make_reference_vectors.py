@@ -34,7 +34,7 @@ sys.path.insert(0, str(ROOT))
 
 from reference_walk import (ACOUSTIC_FORCING, ACOUSTIC_STATE, COEFFICIENTS, DAMPING, POOLS, RECOVERY_STATE,  # noqa: E402
                             TRANSPORT, TRANSPORT_FLUX, VCOEF, build_case as build, case_digest, operand_digest, read,
-                            read_all, schedule, walk, walk_transport)
+                            read_all, scalar_tendency, schedule, walk, walk_transport)
 
 F32 = np.float32
 
@@ -339,7 +339,7 @@ def advection_vectors(case):
             "adv_coefs": np.asarray(adv, F32), "adv_coefs_3rd": np.asarray(adv3, F32)}
 
 
-def transport_vectors(case):
+def transport_vectors(case, source=None):
     """transport.advance_scalars (:1649-1826, stages 1 and 2, split-transport
     density weights) and advance_scalars_monotonic (:1829-2175, stage 3,
     Zalesak FCT with the split-transport target density), stored-inverse
@@ -358,9 +358,11 @@ def transport_vectors(case):
                   fzm=lev["fzm"], fzp=lev["fzp"], rdzw=lev["rdzw"],
                   config_coef_3rd_order=float(c.config_coef_3rd_order), advance_density=True,
                   inv_area_cell=case.inv_area_cell.astype(F32))
+    if source is not None:
+        common["scalar_tendency"] = source
     out = {}
     held = {}
-    for label, o in walk_transport(case):
+    for label, o in walk_transport(case, source):
         st = int(label.split("_")[1])
         if label.endswith("_out"):
             if st == 1:
@@ -456,6 +458,14 @@ def offcentering_vectors(case):
     return {"rdzw": rdzw, **{n: np.asarray(getattr(res, n), dtype=F32) for n in ("etp", "etm", "ewp", "ewm")}}
 
 
+def transport_source_vectors(case):
+    """The three transport stages as in transport_vectors, with a held
+    scalar tendency (the transport's scalar_tendency, cuda_driver.py:5162-5163):
+    added to the flux divergence at stages 1-2 (transport.py:1800-1808) and
+    to the old mixing ratio as dt*source/rho_old before the limiter (:1906)."""
+    return transport_vectors(case, scalar_tendency(case))
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -468,6 +478,7 @@ VECTORS = {
     "pools": pool_vectors,
     "damping_profile": damping_profile_vectors,
     "offcentering": offcentering_vectors,
+    "transport_source": transport_source_vectors,
 }
 
 

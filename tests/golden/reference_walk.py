@@ -79,9 +79,19 @@ def walk(case):
     yield "recover_3_out", o
 
 
-def walk_transport(case):
+def scalar_tendency(case) -> np.ndarray:
+    """A seeded held scalar tendency [T][L][C] (kg/kg/s) that moves each
+    mixing ratio by up to +-50 % per hour, so some columns go negative before
+    the limiter and the final clamp."""
+    q = np.asarray(case.state.scalars, dtype=F32)
+    r = np.random.default_rng(11).uniform(-0.5, 0.5, q.shape).astype(F32)
+    return ((q * r) / F32(3600.0)).astype(F32)
+
+
+def walk_transport(case, source=None):
     """Yield (checkpoint, oracle) around the three transport stages of a
-    large step (after its three dynamics subcycles)."""
+    large step (after its three dynamics subcycles), with `source` as the
+    held scalar tendency when given."""
     from hexdyn.case import initial_state_arrays
     from oracle.oracle import Oracle
 
@@ -90,6 +100,8 @@ def walk_transport(case):
     o = Oracle(case, 32)
     o.set_state(initial_state_arrays(case), rebuild=True)
     assert o.step(1) == 0
+    if source is not None:
+        o.set_physics(scalars=source)
     o.begin_step()
     for sub in range(1, c.config_dynamics_split_steps + 1):
         o.begin_subcycle(sub)

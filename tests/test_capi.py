@@ -22,7 +22,7 @@ def test_library_exports_header_symbols():
 
 
 def test_abi_version():
-    assert hlib.lib().hexdyn_abi_version() == hlib.ABI_VERSION == 2
+    assert hlib.lib().hexdyn_abi_version() == hlib.ABI_VERSION == 3
 
 
 def test_refuses_bad_config_without_gpu_calls(get_case):

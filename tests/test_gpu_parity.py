@@ -401,6 +401,29 @@ def test_whole_step_variants_bitwise(get_case, variant):
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
 
 
+def test_unlimited_transport_with_held_scalar_tendency_bitwise(get_case):
+    """config_monotonic off: all three transport stages take the unlimited
+    form, each with the held scalar tendency; 2 whole steps bit for bit."""
+    import dataclasses
+
+    print("[gpu] x1.642-moist, unlimited transport with a held scalar tendency, 2 steps", flush=True)
+    base = get_case("x1.642-moist")
+    case = get_case("x1.642-moist", config=dataclasses.replace(base.config, config_monotonic=False))
+    q = np.asarray(case.state.scalars, dtype=np.float32)
+    src = ((q * np.random.default_rng(13).uniform(-0.3, 0.3, q.shape).astype(np.float32))
+           / np.float32(3600.0)).astype(np.float32)
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    o.set_physics(scalars=src)
+    assert o.step(2) == 0
+    d = _driver(case)
+    d.set_physics_tendencies(scalars=src)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
 def test_post_transport_handoff_bitwise(get_case):
     print("[gpu] unclamped post-transport handoff x1.642-moist, unlimited transport, 2 steps", flush=True)
     import dataclasses
@@ -479,5 +502,9 @@ def test_phase_by_phase_bitwise_with_physics_tendencies(get_case):
     physics = {"rho": (1e-9 * rng.standard_normal((L, C))).astype(np.float32),
                "rho_u": (1e-6 * rng.standard_normal((L, E))).astype(np.float32),
                "rho_theta": (1e-6 * rng.standard_normal((L, C))).astype(np.float32)}
+    # a held scalar tendency that pushes some mixing ratios negative before the limiter
+    q = np.asarray(case.state.scalars, dtype=np.float32)
+    physics["scalars"] = ((q * rng.uniform(-0.5, 0.5, q.shape).astype(np.float32)) / np.float32(3600.0)).astype(
+        np.float32)
     bad = _phase_walk(case, physics)
     assert not bad, "\n".join(bad[:20])

@@ -367,3 +367,39 @@ def test_mass_loading_counts_only_the_leading_species(get_case):
     full.set_state(initial_state_arrays(base), rebuild=True)
     full.begin_step()
     assert not np.array_equal(full.export_field("QTOT"), o.export_field("QTOT"))
+
+
+def test_held_scalar_tendency_enters_the_tracer_budget_fp64(get_case):
+    """A held scalar tendency (the transport's scalar_tendency) adds
+    dt * sum(A dz rho_old-weighted source / rho_old) = dt * sum(A dz source)
+    to each tracer mass through the limited transport, where no mixing
+    ratio is clipped: the transport itself conserves tracer mass."""
+    case = get_case("x1.642-moist")
+    L, C = case.n_levels, case.mesh.n_cells
+    area = (1.0 / case.inv_area_cell.astype(np.float64))[None, :]
+    dz = _dz(case)
+    q0 = np.asarray(case.state.scalars, dtype=np.float32)
+    src = (q0 * np.random.default_rng(5).uniform(0.0, 0.2, q0.shape).astype(np.float32)
+           / np.float32(3600.0)).astype(np.float32)
+
+    def masses(o):
+        rho = o.export_field_f64("RHO_ZZ").reshape(L, C)
+        q = o.export_field_f64("SCALARS").reshape(case.n_scalars, L, C)
+        return np.array([(area * rho * q[s] * dz).sum() for s in range(case.n_scalars)])
+
+    base = Oracle(case, 64)
+    base.set_state(initial_state_arrays(case), rebuild=True)
+    assert base.step(1) == 0
+    mb = masses(base)
+    phys = Oracle(case, 64)
+    phys.set_state(initial_state_arrays(case), rebuild=True)
+    phys.set_physics(scalars=src)
+    assert phys.step(1) == 0
+    mp = masses(phys)
+    dt = case.config.config_dt
+    added = np.array([dt * (area * src[s].astype(np.float64) * dz).sum() for s in range(case.n_scalars)])
+    for s in range(case.n_scalars):
+        if added[s] == 0:
+            assert mp[s] == mb[s]
+            continue
+        assert abs((mp[s] - mb[s]) - added[s]) <= 1e-9 * abs(added[s]) + 1e-12 * abs(mb[s]), s

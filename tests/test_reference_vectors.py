@@ -182,14 +182,17 @@ def test_advection_stencils_match_reference_bits(case):
     assert np.abs(d2 - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
-def test_transport_stages_match_reference_bits(case):
+@pytest.mark.parametrize("vectors", ["transport", "transport_source"])
+def test_transport_stages_match_reference_bits(case, vectors):
     """advance_scalars (transport.py:1649-1826) at stages 1 and 2 and
     advance_scalars_monotonic (:1829-2175, Zalesak FCT) at stage 3, all six
-    species, split-transport densities: every mixing ratio word for word."""
-    z = _load("transport")
+    species, split-transport densities: every mixing ratio word for word,
+    without and with a held scalar tendency (scalar_tendency)."""
+    z = _load(vectors)
     assert bytes(z["case_digest"]).decode() == bytes(rw.case_digest(case)).decode()
+    source = rw.scalar_tendency(case) if vectors == "transport_source" else None
     held, seen = {}, 0
-    for name, o in rw.walk_transport(case):
+    for name, o in rw.walk_transport(case, source):
         st = int(name.split("_")[1])
         if not name.endswith("_out"):
             ops = rw.read_all(o, case, rw.TRANSPORT)
