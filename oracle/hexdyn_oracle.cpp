@@ -619,6 +619,20 @@ template <class T> struct Oracle {
   }
 
   // ------------------------------------------------------ stage tendencies
+  // vertical transport of normal momentum of edge e, all levels: out[k] =
+  // -rdzw(k) * (wduz(k+1) - wduz(k)) (dynamics.py:227-270)
+  void vertical_transport_u(int e, T *out) const {
+    const T half = T(0.5), one = T(1);
+    const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
+    std::vector<T> wduz(L + 1, T(0));
+    auto rwe = [&](int k) { return half * (c_rw[(size_t)k * C + c0] + c_rw[(size_t)k * C + c1]); };
+    wduz[1] = rwe(1) * (fzm[1] * u[ei(1, e)] + fzp[1] * u[ei(0, e)]);
+    for (int k = 2; k < L - 1; ++k)
+      wduz[k] = flux3(u[ei(k - 2, e)], u[ei(k - 1, e)], u[ei(k, e)], u[ei(k + 1, e)], rwe(k), one);
+    wduz[L - 1] = rwe(L - 1) * (fzm[L - 1] * u[ei(L - 1, e)] + fzp[L - 1] * u[ei(L - 2, e)]);
+    for (int k = 0; k < L; ++k) out[k] = (-rdzw[k]) * (wduz[k + 1] - wduz[k]);
+  }
+
   void dyn_tend(int stage) {
     const T half = T(0.5), one = T(1);
     const T coef = T(cfg.config_coef_3rd_order);
@@ -629,14 +643,10 @@ template <class T> struct Oracle {
 #pragma omp parallel for
     for (int e = 0; e < E; ++e) {
       const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
-      std::vector<T> wduz(L + 1, T(0));
-      auto rwe = [&](int k) { return half * (c_rw[(size_t)k * C + c0] + c_rw[(size_t)k * C + c1]); };
-      wduz[1] = rwe(1) * (fzm[1] * u[ei(1, e)] + fzp[1] * u[ei(0, e)]);
-      for (int k = 2; k < L - 1; ++k)
-        wduz[k] = flux3(u[ei(k - 2, e)], u[ei(k - 1, e)], u[ei(k, e)], u[ei(k + 1, e)], rwe(k), one);
-      wduz[L - 1] = rwe(L - 1) * (fzm[L - 1] * u[ei(L - 1, e)] + fzp[L - 1] * u[ei(L - 2, e)]);
+      std::vector<T> vtu(L);
+      vertical_transport_u(e, vtu.data());
       for (int k = 0; k < L; ++k) {
-        const T initial = (-rdzw[k]) * (wduz[k + 1] - wduz[k]);
+        const T initial = vtu[k];
         T qq = T(0);
         for (int s = 0; s < neoe[e]; ++s) {
           const int n = eoe[(size_t)e * ME2 + s];
@@ -1480,6 +1490,24 @@ int oracle_import_internal(void *h, const char *name, const float *src, int64_t 
   });
   return rc;
 }
+// Test hook: the vertical transport of normal momentum (the first term of the
+// momentum tendency) of the current u and rw, [L][E].
+int oracle_vertical_transport_u(void *h, float *out, int64_t n) {
+  int rc = 0;
+  DISPATCH(h, {
+    if (n != (int64_t)o.eL()) {
+      rc = 1;
+    } else {
+      std::vector<std::remove_reference<decltype(o.u[0])>::type> col(o.L);
+      for (int e = 0; e < o.E; ++e) {
+        o.vertical_transport_u(e, col.data());
+        for (int k = 0; k < o.L; ++k) out[(size_t)k * o.E + e] = (float)col[k];
+      }
+    }
+  });
+  return rc;
+}
+
 // The host C library's powf and the product's exner power method over many
 // arguments (tests and golden-vector scripts compare them with the reference's
 // GNU powf).

@@ -73,6 +73,7 @@ def lib() -> ct.CDLL:
         _lib.oracle_internal_size.restype = ct.c_int64
         _lib.oracle_export_internal.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_void_p, ct.c_int64]
         _lib.oracle_import_internal.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_void_p, ct.c_int64]
+        _lib.oracle_vertical_transport_u.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64]
         for name in ("oracle_libm_powf_many", "oracle_hexdyn_powf_many"):
             getattr(_lib, name).argtypes = [ct.c_void_p, ct.c_float, ct.c_void_p, ct.c_int64]
             getattr(_lib, name).restype = None
@@ -160,6 +161,15 @@ class Oracle:
         a = np.ascontiguousarray(values, dtype=np.float32).ravel()
         if lib().oracle_import_internal(self.h, name.encode(), a.ctypes.data, a.size):
             raise KeyError(name)
+
+    def vertical_transport_u(self) -> np.ndarray:
+        """The vertical transport term of the momentum tendency for the
+        current u and rw, flat [L][E] (the code dyn_tend runs)."""
+        n = self.case.n_levels * self.case.mesh.n_edges
+        out = np.empty(n, dtype=np.float32)
+        if lib().oracle_vertical_transport_u(self.h, out.ctypes.data, n):
+            raise ValueError("size mismatch")
+        return out
 
     def btri_ctri_f64(self):
         n = self.case.n_levels * self.case.mesh.n_cells

@@ -466,6 +466,24 @@ def transport_source_vectors(case):
     return transport_vectors(case, scalar_tendency(case))
 
 
+def vertical_transport_vectors(case):
+    """dynamics.vertical_transport_u (:227-270), the vertical transport of
+    normal momentum that opens the momentum tendency, with the stage-1
+    recovered u and rw (the operands of the stage-2 tendencies)."""
+    from hexcore.dynamics import vertical_transport_u
+
+    mesh = mesh_view(case)
+    lev = _levels(case)
+    for label, o in walk(case):
+        if label != "vcoef_2":
+            continue
+        ops = read_all(o, case, [("normal_velocity", "I:u", "edge"), ("rho_w", "I:c_rw", "w")])
+        res = vertical_transport_u(mesh, ops["normal_velocity"], ops["rho_w"], fzm=lev["fzm"], fzp=lev["fzp"],
+                                   rdzw=lev["rdzw"])
+        assert res.dtype == F32
+        return {"vtu_2.operands": operand_digest(ops.values()), "vtu_2.result": np.ascontiguousarray(res)}
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -479,6 +497,7 @@ VECTORS = {
     "damping_profile": damping_profile_vectors,
     "offcentering": offcentering_vectors,
     "transport_source": transport_source_vectors,
+    "vertical_transport": vertical_transport_vectors,
 }
 
 

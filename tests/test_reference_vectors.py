@@ -250,3 +250,18 @@ def test_offcentering_matches_reference_bits():
     for name, val in zip(("etp", "etm", "ewp", "ewm"), got):
         _same_bits(val, z[name], name)
     assert len(set(z["etp"].tolist())) > 5  # the transition is exercised
+
+
+def test_vertical_transport_of_momentum_matches_reference_bits(case):
+    """vertical_transport_u (dynamics.py:227-270): the first term of the
+    momentum tendency, as the oracle's dyn_tend computes it, from the stage-1
+    recovered u and rw."""
+    z = _load("vertical_transport")
+    assert bytes(z["case_digest"]).decode() == bytes(rw.case_digest(case)).decode()
+    for name, o in rw.walk(case):
+        if name == "vcoef_2":
+            ops = rw.read_all(o, case, [("normal_velocity", "I:u", "edge"), ("rho_w", "I:c_rw", "w")])
+            assert bytes(z["vtu_2.operands"]).decode() == bytes(rw.operand_digest(ops.values())).decode()
+            _same_bits(o.vertical_transport_u(), z["vtu_2.result"], "vertical transport of u")
+            return
+    raise AssertionError("checkpoint not reached")
