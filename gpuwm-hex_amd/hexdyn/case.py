@@ -8,6 +8,9 @@ Workloads (sizes, seeds):
                   log-normal condensates where RH > 0.8, FCT transport, dt 720 s.
   x1.10242 moist: subdivided 5x, dt 1440 s (the ensemble mesh).
   x1.642, x1.162: subdivided 3x and 2x (small test cases), dt 3600 s and 7200 s.
+  r642 moist    : 642 seeded random generators, 31 Lloyd iterations: a ragged
+                  mesh with 110 pentagons, 434 hexagons and 98 heptagons
+                  (maxEdges 7), dt 3600 s.
   x1.163842 qv  : subdivided 7x, water vapour only (seeded RH field), dt 360 s
                   (the scale-up mesh; about 2.5 min of host time to build).
 Seeds: mesh 1, terrain 2, theta perturbation 3, RH 4, condensates 5.
@@ -25,7 +28,7 @@ from typing import Optional
 import numpy as np
 
 from .config import DycoreConfig
-from .mesh import Mesh, icosahedral_mesh
+from .mesh import Mesh, build_mesh, icosahedral_mesh, random_points
 from . import initial as hinit
 from . import setup as hsetup
 
@@ -74,6 +77,7 @@ WORKLOADS = {
     "x1.40962-dry": dict(level=6, moist=False, n_scalars=0, dt=720.0),
     "x1.163842-qv": dict(level=7, moist=True, n_scalars=1, dt=360.0),
     "x1.162-moist": dict(level=2, moist=True, n_scalars=6, dt=7200.0),
+    "r642-moist": dict(level=0, random_points=642, moist=True, n_scalars=6, dt=3600.0),
     "x1.642-dry": dict(level=3, moist=False, n_scalars=0, dt=3600.0),
     "x1.642-moist": dict(level=3, moist=True, n_scalars=6, dt=3600.0),
 }
@@ -104,7 +108,10 @@ def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[Dy
             case = pickle.load(f)
         return replace(case, config=cfg, len_disp=_len_disp(case.mesh, cfg))
     say = _progress(name)
-    mesh = icosahedral_mesh(spec["level"], seed=1, lloyd_iterations=20)
+    if spec.get("random_points"):
+        mesh = build_mesh(random_points(spec["random_points"], seed=1, lloyd_iterations=31))
+    else:
+        mesh = icosahedral_mesh(spec["level"], seed=1, lloyd_iterations=20)
     say(f"mesh {mesh.n_cells} cells")
     deriv = hsetup.deriv_two(mesh)
     terrain = hinit.random_terrain(mesh, seed=2, hmax=spec.get("hmax", 4000.0)) * spec.get("terrain_scale", 1.0)

@@ -131,7 +131,19 @@ def generate_points(level: int, *, seed: int = 1, lloyd_iterations: int = 20, ji
     rng = np.random.default_rng(seed)
     spacing = np.sqrt(4.0 * np.pi / verts.shape[0])
     points = _normalize(verts + jitter * spacing * rng.standard_normal(verts.shape))
-    for _ in range(lloyd_iterations):
+    return _lloyd(points, lloyd_iterations)
+
+
+def random_points(n: int, *, seed: int = 1, lloyd_iterations: int = 31) -> np.ndarray:
+    """Seeded uniform random generators on the sphere relaxed by a few
+    spherical Lloyd iterations: a ragged Voronoi mesh of pentagons, hexagons
+    and heptagons (maxEdges 7 for n = 642, seed 1, 31 iterations)."""
+    rng = np.random.default_rng(seed)
+    return _lloyd(_normalize(rng.standard_normal((n, 3))), lloyd_iterations)
+
+
+def _lloyd(points: np.ndarray, iterations: int) -> np.ndarray:
+    for _ in range(iterations):
         tri = _oriented_triangles(points)
         cc = _circumcenters(points, tri)
         areas, moments = _kite_parts(points, tri, cc)

@@ -92,8 +92,9 @@ def _phase_walk(case, physics=None):
     return bad
 
 
-@pytest.mark.parametrize("name", ["x1.642-moist", "x1.2562-dry"])
+@pytest.mark.parametrize("name", ["x1.642-moist", "x1.2562-dry", "r642-moist"])
 def test_phase_by_phase_bitwise(get_case, name):
+    # r642: ragged mesh with heptagons, which runs the maxEdges-7 kernels
     print(f"[gpu] phase walk {name}", flush=True)
     bad = _phase_walk(get_case(name))
     assert not bad, "\n".join(bad[:20])
@@ -106,11 +107,28 @@ def _mixing(get_case, name, **kw):
     return get_case(name, config=dataclasses.replace(base.config, config_horiz_mixing="2d_smagorinsky", **kw))
 
 
-@pytest.mark.parametrize("name,visc4", [("x1.642-moist", 0.05), ("x1.2562-dry", 0.05), ("x1.2562-dry", 0.0)])
+@pytest.mark.parametrize("name,visc4", [("x1.642-moist", 0.05), ("x1.2562-dry", 0.05), ("x1.2562-dry", 0.0),
+                                        ("r642-moist", 0.05)])
 def test_phase_by_phase_bitwise_with_mixing(get_case, name, visc4):
     print(f"[gpu] phase walk with 2d_smagorinsky mixing {name} visc4={visc4}", flush=True)
     bad = _phase_walk(_mixing(get_case, name, config_visc4_2dsmag=visc4))
     assert not bad, "\n".join(bad[:20])
+
+
+def test_ragged_mesh_three_steps_bitwise(get_case):
+    """Pentagons, hexagons and heptagons (maxEdges 7): three whole steps on
+    the replayed graph bit for bit equal to the oracle."""
+    print("[gpu] r642-moist (maxEdges 7) 3 steps", flush=True)
+    case = get_case("r642-moist")
+    assert case.mesh.max_edges == 7
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(3) == 0
+    d = _driver(case)
+    d.step_device(3)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
 
 
 def test_three_steps_bitwise_moist(get_case):

@@ -2,18 +2,34 @@
 import numpy as np
 import pytest
 
-from hexdyn.mesh import edge_signs_on_cells, edge_signs_on_vertices, icosahedral_mesh
+from hexdyn.mesh import build_mesh, edge_signs_on_cells, edge_signs_on_vertices, icosahedral_mesh, random_points
 
 
-@pytest.fixture(scope="module")
-def mesh():
-    return icosahedral_mesh(3)
+@pytest.fixture(scope="module", params=["icosahedral", "ragged"])
+def mesh(request):
+    """The quasi-uniform x1.642 mesh, and a ragged one from 642 random
+    generators (pentagons, hexagons and heptagons, maxEdges 7)."""
+    if request.param == "icosahedral":
+        return icosahedral_mesh(3)
+    return build_mesh(random_points(642, seed=1, lloyd_iterations=31))
+
+
+def _accuracy_factor(mesh):
+    """Discretization-error bounds (not the exact identities) are looser on
+    the ragged mesh, whose cells are far from centroidal and regular."""
+    return 1.0 if mesh.max_edges == 6 else 5.0
 
 
 def test_counts_and_euler(mesh):
     assert mesh.n_cells == 642
     assert mesh.n_cells - mesh.n_edges + mesh.n_vertices == 2
-    assert np.bincount(mesh.nEdgesOnCell)[5] == 12
+    counts = np.bincount(mesh.nEdgesOnCell, minlength=8)
+    if mesh.max_edges == 6:
+        assert counts[5] == 12
+    else:
+        assert mesh.max_edges == 7 and counts[7] > 0
+    # Euler on the sphere: sum over cells of (6 - sides) = 12
+    assert int((6 - mesh.nEdgesOnCell).sum()) == 12
 
 
 def test_areas_partition(mesh):
@@ -65,8 +81,9 @@ def test_tangential_reconstruction(mesh, axis):
     for j in range(mesh.edgesOnEdge.shape[1]):
         a = mesh.nEdgesOnEdge > j
         v[a] += mesh.weightsOnEdge[a, j] * u[mesh.edgesOnEdge[a, j]]
-    assert np.sqrt(np.mean((v - vt) ** 2)) < 0.01 * 20.0
-    assert np.abs(v - vt).max() < 0.03 * 20.0
+    f = _accuracy_factor(mesh)
+    assert np.sqrt(np.mean((v - vt) ** 2)) < 0.01 * 20.0 * f
+    assert np.abs(v - vt).max() < 0.03 * 20.0 * f
 
 
 def test_divergence_and_curl_identities(mesh):
@@ -78,7 +95,8 @@ def test_divergence_and_curl_identities(mesh):
         e = mesh.edgesOnCell[c, j]
         div[c] += s[c, j] * mesh.dvEdge[e] * u[e]
     # point-sampled flux of a divergence-free flow: zero up to quadrature error
-    assert np.abs(div / mesh.areaCell).max() < 1e-3 * 20.0 / mesh.dcEdge.mean()
+    f = _accuracy_factor(mesh)
+    assert np.abs(div / mesh.areaCell).max() < 1e-3 * 20.0 / mesh.dcEdge.mean() * f
     # curl of a gradient vanishes: u = grad(phi) along edges
     rng = np.random.default_rng(0)
     phi = rng.standard_normal(mesh.n_cells)
@@ -89,4 +107,4 @@ def test_divergence_and_curl_identities(mesh):
     # solid-body vorticity approximates 2*omega*sin(lat)
     vort = (sv * mesh.dcEdge[mesh.edgesOnVertex] * u[mesh.edgesOnVertex]).sum(1) / mesh.areaTriangle
     exact = 2 * 20.0 / mesh.sphere_radius * np.sin(mesh.latVertex)
-    assert np.abs(vort - exact).max() < 0.02 * np.abs(exact).max()
+    assert np.abs(vort - exact).max() < 0.02 * np.abs(exact).max() * f

@@ -29,8 +29,10 @@ def _rest_case(case):
     return dataclasses.replace(case, reference=ref, state=st)
 
 
-def test_rest_state_is_exact_fixed_point(get_case):
-    case = _rest_case(get_case("x1.642-dry"))
+@pytest.mark.parametrize("name", ["x1.642-dry", "r642-moist"])
+def test_rest_state_is_exact_fixed_point(get_case, name):
+    # r642: the ragged mesh of pentagons, hexagons and heptagons (maxEdges 7)
+    case = _rest_case(get_case(name))
     o = Oracle(case, 32)
     o.set_state(initial_state_arrays(case), rebuild=True)
     before = o.get_state()
@@ -57,8 +59,9 @@ def _dz(case):
     return (1.0 / case.grid.rdzw.astype(np.float64))[:, None]
 
 
-def test_dry_and_tracer_mass_conserved_fp64(get_case):
-    case = get_case("x1.642-moist")
+@pytest.mark.parametrize("name", ["x1.642-moist", "r642-moist"])
+def test_dry_and_tracer_mass_conserved_fp64(get_case, name):
+    case = get_case(name)
     o = Oracle(case, 64)
     o.set_state(initial_state_arrays(case), rebuild=True)
     # the step's own cell measure: the stored binary32 inverse areas
