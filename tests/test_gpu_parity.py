@@ -131,6 +131,27 @@ def test_ragged_mesh_three_steps_bitwise(get_case):
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
 
 
+@pytest.mark.parametrize("nlev", [10, 63])
+def test_level_counts_two_steps_bitwise(get_case, nlev):
+    """A short column and the deepest one a 64-lane row holds (L + 1 = 64,
+    the interface row on the last lane), with mixing: 2 steps bit for bit."""
+    print(f"[gpu] x1.642-moist at {nlev} levels, mixing on, 2 steps", flush=True)
+    import dataclasses
+
+    base = get_case("x1.642-moist", nlev=nlev)
+    case = get_case("x1.642-moist", nlev=nlev,
+                    config=dataclasses.replace(base.config, config_horiz_mixing="2d_smagorinsky"))
+    assert case.n_levels == nlev
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(2) == 0
+    d = _driver(case)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
 def test_three_steps_bitwise_moist(get_case):
     print("[gpu] x1.2562-moist 3 steps", flush=True)
     case = get_case("x1.2562-moist")
