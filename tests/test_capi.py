@@ -2,6 +2,8 @@
 import re
 from pathlib import Path
 
+import pytest
+
 from hexdyn import lib as hlib
 
 HEADER = Path(__file__).resolve().parent.parent / "include" / "hexdyn.h"
@@ -32,6 +34,60 @@ def test_refuses_bad_config_without_gpu_calls(get_case):
     case = get_case("x1.642-dry")
     m, v, c, keep = capi.build_structs(case)
     c.config_dynamics_split_steps = 2
+    h = ct.c_void_p()
+    assert hlib.lib().hexdyn_create(ct.byref(m), ct.byref(v), ct.byref(c), ct.byref(h)) == capi.HEXDYN_E_REFUSED
+    assert not h.value
+
+
+_KEEP = []
+
+
+def _bad_levels(m, v, c, keep):
+    v.n_levels = 64
+
+
+def _bad_max_edges(m, v, c, keep):
+    m.max_edges = 8
+
+
+def _empty_mesh(m, v, c, keep):
+    m.n_cells = 0
+
+
+def _moist_over_scalars(m, v, c, keep):
+    c.n_moist = c.n_scalars + 1
+
+
+def _cam_floor(m, v, c, keep):
+    c.config_horiz_mixing = 1
+    c.config_mpas_cam_coef = 0.1
+
+
+def _cell_index_out_of_range(m, v, c, keep):
+    import ctypes as ct
+    import numpy as np
+
+    coe = np.ctypeslib.as_array(ct.cast(m.cells_on_edge, ct.POINTER(ct.c_int32)), shape=(2 * m.n_edges,))
+    bad = coe.copy()
+    bad[7] = m.n_cells  # one past the last cell
+    _KEEP.append(bad)  # alive until the create call
+    m.cells_on_edge = bad.ctypes.data_as(ct.POINTER(ct.c_int32))
+
+
+@pytest.mark.parametrize("mutate", [_bad_levels, _bad_max_edges, _empty_mesh, _moist_over_scalars, _cam_floor,
+                                    _cell_index_out_of_range], ids=lambda f: f.__name__.strip("_"))
+def test_refuses_inputs_outside_the_contract(get_case, mutate):
+    """Sizes and tables outside what the kernels assume are refused at
+    create (HEXDYN_E_REFUSED), before any device call: 64 levels (L + 1 must
+    fit a 64-lane row), maxEdges 8, an empty mesh, more mass-loading species
+    than carried ones, the CAM viscosity floor, a connectivity index out of
+    range."""
+    import ctypes as ct
+    from hexdyn import capi
+
+    case = get_case("x1.642-moist")
+    m, v, c, keep = capi.build_structs(case)
+    mutate(m, v, c, keep)
     h = ct.c_void_p()
     assert hlib.lib().hexdyn_create(ct.byref(m), ct.byref(v), ct.byref(c), ct.byref(h)) == capi.HEXDYN_E_REFUSED
     assert not h.value
