@@ -363,6 +363,8 @@ def transport_vectors(case, source=None):
     out = {}
     held = {}
     for label, o in walk_transport(case, source):
+        if not label.startswith("scalars_"):
+            continue
         st = int(label.split("_")[1])
         if label.endswith("_out"):
             if st == 1:
@@ -484,6 +486,28 @@ def vertical_transport_vectors(case):
         return {"vtu_2.operands": operand_digest(ops.values()), "vtu_2.result": np.ascontiguousarray(res)}
 
 
+def split_flux_vectors(case):
+    """integration.accumulate_split_flux (:27-55) over the three dynamics
+    subcycles' time-averaged mass fluxes, then finish_split_flux (:58-82):
+    the transport's uh_avg and ww_avg."""
+    from hexcore.integration import accumulate_split_flux, finish_split_flux
+
+    acc_u = acc_w = None
+    fluxes = []
+    for label, o in walk_transport(case):
+        if label.startswith("subcycle_"):
+            fu, fw = read(o, case, "F:FLUX_U", "edge"), read(o, case, "F:FLUX_W", "w")
+            fluxes += [fu, fw]
+            acc_u = accumulate_split_flux(fu, acc_u)
+            acc_w = accumulate_split_flux(fw, acc_w)
+        elif label == "scalars_1":
+            break
+    n = case.config.config_dynamics_split_steps
+    return {"split.operands": operand_digest(fluxes),
+            "split.uh_avg": np.ascontiguousarray(finish_split_flux(acc_u, n)),
+            "split.ww_avg": np.ascontiguousarray(finish_split_flux(acc_w, n))}
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -498,6 +522,7 @@ VECTORS = {
     "offcentering": offcentering_vectors,
     "transport_source": transport_source_vectors,
     "vertical_transport": vertical_transport_vectors,
+    "split_flux": split_flux_vectors,
 }
 
 

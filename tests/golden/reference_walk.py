@@ -113,6 +113,7 @@ def walk_transport(case, source=None):
             assert o.recover(st) == 0
             o.solve_diagnostics(st)
         o.end_subcycle(sub)
+        yield f"subcycle_{sub}", o
     for st in (1, 2, 3):
         yield f"scalars_{st}", o
         o.scalars(st)
