@@ -508,6 +508,24 @@ def split_flux_vectors(case):
             "split.ww_avg": np.ascontiguousarray(finish_split_flux(acc_w, n))}
 
 
+def moist_coefficient_vectors(case):
+    """regional_v841.compute_moist_coefficients (:1180-1219, the
+    atm_compute_moist_coefficients restatement): qtot, cqw, cqu from the
+    step-start scalars, every carried species loading the air."""
+    from hexcore.regional_v841 import compute_moist_coefficients
+
+    for label, o in walk(case):
+        if label != "vcoef_1":
+            continue
+        q = read(o, case, "I:o_q", "tracer")
+        qtot, cqw, cqu = compute_moist_coefficients(
+            q, moist_indices=list(range(case.n_moist)),
+            cells_on_edge_remapped=np.asarray(case.mesh.cellsOnEdge, dtype=np.int64), nlev=case.n_levels)
+        assert qtot.dtype == F32 and cqw.dtype == F32 and cqu.dtype == F32
+        return {"moist.operands": operand_digest([q]), "moist.qtot": np.ascontiguousarray(qtot),
+                "moist.cqw": np.ascontiguousarray(cqw), "moist.cqu": np.ascontiguousarray(cqu)}
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -523,6 +541,7 @@ VECTORS = {
     "transport_source": transport_source_vectors,
     "vertical_transport": vertical_transport_vectors,
     "split_flux": split_flux_vectors,
+    "moist_coefficients": moist_coefficient_vectors,
 }
 
 

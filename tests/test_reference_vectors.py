@@ -285,3 +285,19 @@ def test_split_flux_average_matches_reference_bits(case):
             _same_bits(rw.read(o, case, "I:split_w", "w"), z["split.ww_avg"], "ww_avg")
             return
     raise AssertionError("checkpoint not reached")
+
+
+def test_moist_coefficients_match_reference_bits(case):
+    """compute_moist_coefficients (regional_v841.py:1180-1219): qtot, cqw and
+    cqu of the step-start scalars, six loading species."""
+    z = _load("moist_coefficients")
+    assert bytes(z["case_digest"]).decode() == bytes(rw.case_digest(case)).decode()
+    for name, o in rw.walk(case):
+        if name == "vcoef_1":
+            q = rw.read(o, case, "I:o_q", "tracer")
+            assert bytes(z["moist.operands"]).decode() == bytes(rw.operand_digest([q])).decode()
+            _same_bits(rw.read(o, case, "F:QTOT", "cell"), z["moist.qtot"], "qtot")
+            _same_bits(rw.read(o, case, "F:CQW", "cell"), z["moist.cqw"], "cqw")
+            _same_bits(rw.read(o, case, "F:CQU", "edge"), z["moist.cqu"], "cqu")
+            return
+    raise AssertionError("checkpoint not reached")
