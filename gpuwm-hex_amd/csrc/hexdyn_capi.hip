@@ -345,6 +345,16 @@ float hexdyn_powf(float x, float y) {
   return why.empty() ? hexdyn_gnu_powf_fix(x, r, d, ties.data(), near_tie) : r;
 }
 
+namespace {
+thread_local std::string t_create_error;  // why this thread's last hexdyn_create failed
+}
+
+int hexdyn_last_create_error(char *buf, size_t len) {
+  if (!buf || len == 0) return HEXDYN_E_INVALID;
+  std::snprintf(buf, len, "%s", t_create_error.c_str());
+  return 0;
+}
+
 int hexdyn_last_error(const hexdyn_handle *h, char *buf, size_t len) {
   if (!h || !buf || len == 0) return HEXDYN_E_INVALID;
   std::snprintf(buf, len, "%s", h->error.c_str());
@@ -361,19 +371,35 @@ void hexdyn_destroy(hexdyn_handle *h) {
   delete h;
 }
 
+static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_config *c, hexdyn_handle **out);
+
 int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_config *c, hexdyn_handle **out) {
-  if (!m || !vt || !c || !out) return HEXDYN_E_INVALID;
+  t_create_error.clear();
+  if (!m || !vt || !c || !out) {
+    t_create_error = "hexdyn_create: null argument";
+    return HEXDYN_E_INVALID;
+  }
   *out = nullptr;
+  return create_impl(m, vt, c, out);
+}
+
+static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_config *c, hexdyn_handle **out) {
   hexdyn_handle *h = new hexdyn_handle();
   h->cfg = *c;
   Sizes &n = h->I.n;
   n.C = m->n_cells; n.E = m->n_edges; n.V = m->n_vertices; n.L = vt->n_levels;
   n.ME = m->max_edges; n.ME2 = m->max_edges2; n.MA = m->max_adv; n.NT = c->n_scalars; n.NM = c->n_moist;
   auto refuse = [&](const std::string &msg) {
-    h->error = msg;
+    t_create_error = msg;
     int rc = HEXDYN_E_REFUSED;
     hexdyn_destroy(h);
     return rc;
+  };
+  // a HIP failure during set-up: keep the reason readable after the handle is gone
+  auto hip_fail = [&](const char *what) {
+    t_create_error = std::string(what) + (h->error.empty() ? "" : ": " + h->error);
+    hexdyn_destroy(h);
+    return HEXDYN_E_HIP;
   };
   if (n.C <= 0 || n.E <= 0 || n.V <= 0) return refuse("empty mesh");
   if (n.L < 4 || n.L + 1 > HX_ROW) return refuse("n_levels must satisfy 4 <= L and L + 1 <= 64");
@@ -383,6 +409,19 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
     return refuse("edgesOnEdge / advCellsForEdge wider than 2*maxEdges-2");
   if (m->vertex_degree != 3) return refuse("vertex_degree must be 3");
   if (n.NT < 0 || n.NM < 0 || n.NM > n.NT) return refuse("n_moist must lie in [0, n_scalars]");
+  {
+    // kernels address every row as a 32-bit byte offset from its array's
+    // base (row * 256 + level * 4): the largest row index of any buffer must
+    // stay below 2^32 / 256 rows
+    const int64_t me_max = std::max(m->max_edges, 6);
+    const int64_t rows[] = {(int64_t)n.C, (int64_t)n.E, (int64_t)n.V, (int64_t)n.C * me_max,
+                            (int64_t)std::max(n.NT, 1) * n.C, (int64_t)std::max(n.NT, 1) * n.E};
+    for (int64_t r : rows)
+      if (r * HX_ROW * (int64_t)sizeof(float) >= ((int64_t)1 << 32))
+        return refuse("mesh too large for 32-bit row offsets: an array would reach 4 GiB "
+                      "(rows x 256 B with rows = cells, edges, vertices, cells x maxEdges, "
+                      "scalars x cells or scalars x edges)");
+  }
   if (c->config_time_integration_order != 2 && c->config_time_integration_order != 3)
     return refuse("config_time_integration_order must be 2 or 3");
   if (c->config_dynamics_split_steps != 1 && c->config_dynamics_split_steps != 3)
@@ -401,10 +440,19 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
     if (!(c->config_visc4_2dsmag >= 0.0f) || !(c->config_smagorinsky_coef >= 0.0f))
       return refuse("config_visc4_2dsmag and config_smagorinsky_coef must be non-negative");
     if (!(c->config_del4u_div_factor > 0.0f)) return refuse("config_del4u_div_factor must be positive");
-    if (c->config_mpas_cam_coef != 0.0f) return refuse("config_mpas_cam_coef > 0 (CAM viscosity floor) is not implemented");
+    if (!(c->config_mpas_cam_coef >= 0.0f)) return refuse("config_mpas_cam_coef must be non-negative");
+    if (c->config_mpas_cam_coef > 0.0f &&
+        (c->config_number_cam_damping_levels < 1 || c->config_number_cam_damping_levels > n.L))
+      return refuse("config_number_cam_damping_levels must lie in 1..nVertLevels");
     if (!m->deformation_coef_c2 || !m->deformation_coef_s2 || !m->deformation_coef_cs)
       return refuse("2d_smagorinsky mixing needs the deformation weights");
     if (!m->area_cell || !m->area_triangle) return refuse("mixing needs areaCell and areaTriangle");
+  }
+  if (c->config_rayleigh_damp_u) {
+    if (c->config_number_rayleigh_damp_u_levels < 1 || c->config_number_rayleigh_damp_u_levels > n.L)
+      return refuse("config_number_rayleigh_damp_u_levels must lie in 1..nVertLevels");
+    if (!(c->config_rayleigh_damp_u_timescale_days > 0.0f) || !std::isfinite(c->config_rayleigh_damp_u_timescale_days))
+      return refuse("config_rayleigh_damp_u_timescale_days must be finite and positive");
   }
   h->fast = c->numerics == HEXDYN_NUMERICS_FAST;
   const int in_me2 = m->max_edges2, in_ma = m->max_adv;
@@ -596,6 +644,10 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
     K0.strain_scale = (csm * len) * (csm * len);
     const float inv_dt = 1.0f / (float)c->config_dt;
     K0.ceiling = (0.01f * (len * len)) * inv_dt;
+    if (c->config_mpas_cam_coef > 0.0f) {
+      K0.cam_levels = c->config_number_cam_damping_levels;
+      K0.cam_base = ((4.0f * 2.0833f) * len) * c->config_mpas_cam_coef;
+    }
   }
   rc |= h->upload(&M.coe, coe); rc |= h->upload(&M.voe, voe); rc |= h->upload(&M.eoc, eoc);
   rc |= h->upload(&M.coc, coc); rc |= h->upload(&M.voc, voc); rc |= h->upload(&M.neoc, neoc);
@@ -613,7 +665,7 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
     rc |= h->upload(&pa, a); rc |= h->upload(&pb, b); rc |= h->upload(&pc, cvec);
     h->old_c = (int *)pa; h->old_e = (int *)pb; h->old_v = (int *)pc;
   }
-  if (rc) { int r = HEXDYN_E_HIP; hexdyn_destroy(h); return r; }
+  if (rc) return hip_fail("device allocation or upload failed");
 
   // ---- vertical metrics, padded per-level arrays and row-layout static fields
   Vert &Z = h->I.z;
@@ -668,7 +720,7 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
     }
     rc |= h->upload(&Z.zb, zb); rc |= h->upload(&Z.zb3, zb3);
   }
-  if (rc) { int r = HEXDYN_E_HIP; hexdyn_destroy(h); return r; }
+  if (rc) return hip_fail("device allocation or upload failed");
 
   // ---- working set
   Work &B = h->I.b;
@@ -701,14 +753,12 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
   rc |= h->alloc(&B.flag, 16);
   rc |= h->alloc(&h->int_partial, (size_t)((C + 3) / 4) * (3 + n.NT));
   rc |= h->alloc(&h->int_out, (size_t)(3 + n.NT));
-  if (rc) { int r = HEXDYN_E_HIP; hexdyn_destroy(h); return r; }
+  if (rc) return hip_fail("device allocation or upload failed");
   if (n.NM == 0) {
     std::vector<float> ones(std::max(cr, er), 1.0f);
     if (hipMemcpy(B.cqw, ones.data(), cr * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(B.cqu, ones.data(), er * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
-      hexdyn_destroy(h);
-      return HEXDYN_E_HIP;
-    }
+        hipMemcpy(B.cqu, ones.data(), er * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      return hip_fail("dry moist-coefficient upload failed");
   }
 
   // ---- constants in the reference's binary32 forms
@@ -723,10 +773,12 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
     std::string why;
     const std::vector<uint32_t> &ties = gnu_powf_ties(K.rcv, &why, &K.powf_near_tie);
     if (!why.empty()) return refuse(why.c_str());
-    if (h->upload(&h->I.m.powf_ties, ties)) {
-      hexdyn_destroy(h);
-      return HEXDYN_E_HIP;
-    }
+    if (h->upload(&h->I.m.powf_ties, ties)) return hip_fail("powf correction table upload failed");
+  }
+  if (c->config_rayleigh_damp_u) {
+    const int nr = c->config_number_rayleigh_damp_u_levels;
+    K.ray_levels = nr;
+    K.ray_inverse = 1.0f / ((float)nr * (c->config_rayleigh_damp_u_timescale_days * 86400.0f));
   }
   K.coef3 = c->config_coef_3rd_order;
   K.apvm_on = c->config_apvm_upwinding > 0.0f;
@@ -868,13 +920,25 @@ bool transported(const hexdyn_handle *h) {
   return h->cfg.config_scalar_advection && h->cfg.config_split_dynamics_transport && h->I.n.NT > 0;
 }
 
-// device work that closes a step (before the validation flag is read)
+// device work that closes a step (before the validation flag is read): the
+// q < 0 -> 0 clamp of every species (cuda_driver.py:5306-5311, atm_srk3's
+// unconditional clamp), a where-assignment that keeps -0.0 and NaN bits.
+// Monotonic transport already ends every value in max(0, .) >= +0, so the
+// clamp would change nothing there.  Without transport the published scalars
+// are clamped into the candidate buffer, which the publish then takes.
 int end_step_device(hexdyn_handle *h, hipStream_t s) {
-  // (monotonic transport already ends in max(0, .); the clamp keeps -0.0 and NaN bits)
-  if (h->post_clamp && transported(h) && h->I.n.NM > 0 && !h->cfg.config_monotonic)
-    if (NS_CALL(h, clamp_scalars, h->I, h->I.b.q_stage, s)) return HEXDYN_E_HIP;
+  if (!h->post_clamp || h->I.n.NT == 0) return 0;
+  if (transported(h)) {
+    if (!h->cfg.config_monotonic && NS_CALL(h, clamp_scalars, h->I, h->I.b.q_stage, h->I.b.q_stage, s))
+      return HEXDYN_E_HIP;
+  } else if (NS_CALL(h, clamp_scalars, h->I, h->I.b.q, h->I.b.q_stage, s)) {
+    return HEXDYN_E_HIP;
+  }
   return 0;
 }
+
+// the candidate scalars (q_stage) replace the published ones at publish
+bool scalars_in_stage(const hexdyn_handle *h) { return transported(h) || (h->post_clamp && h->I.n.NT > 0); }
 
 // read the flag; publish by swapping buffers when it is clear
 int end_step_publish(hexdyn_handle *h, hipStream_t s) {
@@ -891,7 +955,7 @@ int end_step_publish(hexdyn_handle *h, hipStream_t s) {
   std::swap(B.rho, B.c_rho); std::swap(B.rt, B.c_rt); std::swap(B.ru, B.c_ru); std::swap(B.rw, B.c_rw);
   std::swap(B.p_theta, B.theta); std::swap(B.p_exner, B.exner); std::swap(B.p_rho_p, B.rho_p);
   std::swap(B.p_rtheta_p, B.rtheta_p); std::swap(B.p_pp, B.pp); std::swap(B.p_u, B.u); std::swap(B.p_w, B.w);
-  if (transported(h)) std::swap(B.q, B.q_stage);
+  if (scalars_in_stage(h)) std::swap(B.q, B.q_stage);
   h->time_s += h->cfg.config_dt;
   return 0;
 }
@@ -1094,7 +1158,8 @@ int enqueue_step(hexdyn_handle *h, hipStream_t s) {
 
 // Key of what a step's launches depend on besides the static mesh: the
 // buffer-role layout at step start and the run-time switches.  Work holds
-// only pointers and Consts only 4-byte scalars, so neither has padding.
+// only pointers and Consts one double and 4-byte scalars (asserted at its
+// definition), so neither has padding bytes.
 std::string layout_key(const hexdyn_handle *h) {
   return std::string(reinterpret_cast<const char *>(&h->I.b), sizeof(Work)) +
          std::string(reinterpret_cast<const char *>(&h->I.k), sizeof(Consts)) +
@@ -1190,6 +1255,8 @@ int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *p
   }
   return hipStreamSynchronize(s) == hipSuccess ? 0 : HEXDYN_E_HIP;
 }
+
+int hexdyn_graph_count(const hexdyn_handle *h) { return h ? (int)h->graphs.size() : 0; }
 
 int hexdyn_set_graphs(hexdyn_handle *h, int enable) {
   if (!h) return HEXDYN_E_INVALID;

@@ -112,9 +112,11 @@ struct Work {
   int *flag;  // sticky validation flag
 };
 
+// Every member is 4 bytes except the leading double, so the struct has no
+// padding and its bytes identify its value (the step-graph cache keys on them).
 struct Consts {
-  float gravity, rcv, c2, cfac, rgas;  // binary32 constants in reference form
   double powf_near_tie;                // GNU powf correction: bitmap read only this close to a tie (ulps)
+  float gravity, rcv, c2, cfac, rgas;  // binary32 constants in reference form
   float coef3, apvm_scale, divdamp_base;
   float dts[3];            // acoustic timestep per stage
   float dts_inv_coef[3];   // divergence damping coefficient per stage
@@ -134,7 +136,12 @@ struct Consts {
                                  // ride on the edge diagnostics, the first acoustic edge update on
                                  // the momentum tendency, and ru_avg is kept only where read back
   float strain_scale, ceiling;   // (c_s*len)^2, (0.01*len^2)/dt
+  int ray_levels;                // Rayleigh damping of u over the top ray_levels levels (0: off)
+  float ray_inverse;             // 1/(levels*(days*86400))
+  int cam_levels;                // CAM kdiff floor over the top cam_levels levels (0: off)
+  float cam_base;                // ((4*2.0833)*len)*cam_coef
 };
+static_assert(sizeof(Consts) == sizeof(double) + 46 * 4, "Consts must have no padding");
 
 // Optional per-kernel timing (events around each launch on its stream).
 void prof_mark(const char *name, hipStream_t s, int start);
@@ -161,7 +168,7 @@ struct Instance {
   int solve_diagnostics(const Instance &I, int rk, hipStream_t s);        \
   int scalars_stage(const Instance &I, int stage, hipStream_t s);         \
   int rebuild_saved(const Instance &I, hipStream_t s);                    \
-  int clamp_scalars(const Instance &I, float *q, hipStream_t s);          \
+  int clamp_scalars(const Instance &I, const float *src, float *dst, hipStream_t s); \
   }
 
 HEXDYN_PHASES(strict)

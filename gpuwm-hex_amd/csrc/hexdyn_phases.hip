@@ -531,7 +531,12 @@ __device__ __forceinline__ float mix_kdiff_value(const Instance &I, int c, int k
   const float d11 = 2.0f * dudx, d22 = 2.0f * dvdy, d12 = dudy + dvdx;
   const float diff = d11 - d22;
   const float strain = __builtin_sqrtf(0.25f * (diff * diff) + d12 * d12);
-  return fminf(I.k.strain_scale * strain, I.k.ceiling);
+  const float kd = fminf(I.k.strain_scale * strain, I.k.ceiling);
+  // CAM floor over the top levels (native/upper_damping.rs:5-16,
+  // cuda_upper_damping_v841.py:5-11): max(kdiff, base*(1 - (L-1-k)/n))
+  const int L = I.n.L, ncam = I.k.cam_levels;
+  if (k >= L - ncam && k < L) return fmaxf(kd, I.k.cam_base * (1.0f - (float)(L - 1 - k) / (float)ncam));
+  return kd;
 }
 
 // del2 u of edge e at level k: the increment added to the u pool and delsq_u
@@ -837,7 +842,11 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
     }
   const float rho_edge = 0.5f * (rho0 + rho1);
   const float horiz = rho_edge * (qq - (ke1 - ke0) * I.m.invdc[e]);
-  const float t = (initial + horiz) - (u_k * 0.5f) * (hd0 + hd1);
+  float t = (initial + horiz) - (u_k * 0.5f) * (hd0 + hd1);
+  // upper-level Rayleigh damping of u, before the Euler pool is added
+  // (native/upper_damping.rs:18-31; cuda_upper_damping_v841.py:12-18)
+  const int nray = I.k.ray_levels;
+  if (k >= L - nray && k < L) t = t - (rho_edge * u_k) * ((float)(k - (L - nray) + 1) * I.k.ray_inverse);
   const float tr = I.k.physics ? (t + eru) + pru : t + eru;
   if (k < L) AT(I.b.tend_ru, e, k) = tr;
   return tr;
@@ -1785,11 +1794,13 @@ KB k_fct_cell2(const Instance I, float *__restrict__ out) {
   }
 }
 
-KB k_clamp(const Instance I, float *q, int rows) {
+// q < 0 -> 0 as a where-assignment (-0.0 and NaN keep their bits); src may be dst
+KB k_clamp(const Instance I, const float *src, float *dst, int rows) {
   const int r = wave_entity();
   if (r >= rows) return;
   const int k = lane();
-  if (AT(q, r, k) < 0.0f) AT(q, r, k) = 0.0f;
+  const float x = AT(src, r, k);
+  AT(dst, r, k) = x < 0.0f ? 0.0f : x;
 }
 
 template <typename K, typename... A>
@@ -1987,8 +1998,8 @@ int rebuild_saved(const Instance &I, hipStream_t s) {
   return check();
 }
 
-int clamp_scalars(const Instance &I, float *q, hipStream_t s) {
-  launch(k_clamp, I.n.NT * I.n.C, 1, s, I, q, I.n.NT * I.n.C);
+int clamp_scalars(const Instance &I, const float *src, float *dst, hipStream_t s) {
+  launch(k_clamp, I.n.NT * I.n.C, 1, s, I, src, dst, I.n.NT * I.n.C);
   return check();
 }
 

@@ -96,6 +96,10 @@ class HexdynConfig(ct.Structure):
         ("config_del4u_div_factor", ct.c_float),
         ("config_h_ScaleWithMesh", ct.c_int32),
         ("config_mpas_cam_coef", ct.c_float),
+        ("config_number_cam_damping_levels", ct.c_int32),
+        ("config_rayleigh_damp_u", ct.c_int32),
+        ("config_number_rayleigh_damp_u_levels", ct.c_int32),
+        ("config_rayleigh_damp_u_timescale_days", ct.c_float),
     ]
 
 
@@ -234,6 +238,10 @@ def build_structs(case) -> tuple:
     cfg.config_del4u_div_factor = c.config_del4u_div_factor
     cfg.config_h_ScaleWithMesh = int(c.config_h_ScaleWithMesh)
     cfg.config_mpas_cam_coef = c.config_mpas_cam_coef
+    cfg.config_number_cam_damping_levels = c.config_number_cam_damping_levels
+    cfg.config_rayleigh_damp_u = int(c.config_rayleigh_damp_u)
+    cfg.config_number_rayleigh_damp_u_levels = c.config_number_rayleigh_damp_u_levels
+    cfg.config_rayleigh_damp_u_timescale_days = c.config_rayleigh_damp_u_timescale_days
     return mesh, vert, cfg, b
 
 

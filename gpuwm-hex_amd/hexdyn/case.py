@@ -13,6 +13,7 @@ Workloads (sizes, seeds):
                   (maxEdges 7), dt 3600 s.
   x1.163842 qv  : subdivided 7x, water vapour only (seeded RH field), dt 360 s
                   (the scale-up mesh; about 2.5 min of host time to build).
+Override reference_wind=(u0, v0) gives a non-zero u_init/v_init profile.
 Seeds: mesh 1, terrain 2, theta perturbation 3, RH 4, condensates 5.
 """
 
@@ -101,7 +102,7 @@ def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[Dy
     spec = dict(WORKLOADS[name])
     spec.update(overrides)
     cfg = config or DycoreConfig(config_dt=spec["dt"])
-    key = hashlib.sha256(repr((name, nlev, sorted(spec.items()), 6)).encode()).hexdigest()[:16]
+    key = hashlib.sha256(repr((name, nlev, sorted(spec.items()), _generator_digest())).encode()).hexdigest()[:16]
     path = _cache_dir() / f"case-{name}-{key}.pkl"
     if use_cache and path.exists():
         with open(path, "rb") as f:
@@ -125,7 +126,16 @@ def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[Dy
         transition_bottom_z=cfg.config_epssm_transition_bottom_z,
         transition_top_z=cfg.config_epssm_transition_top_z)
     dss = hsetup.vertical_velocity_damping(grid.zgrid, xnutr=cfg.config_xnutr, zd=cfg.config_zd)
-    angle = mesh.angleEdge.astype(F32)
+    cos_angle, sin_angle = hsetup.angle_cos_sin(mesh.angleEdge)
+    L = nlev
+    u_init, v_init = np.zeros(L, dtype=F32), np.zeros(L, dtype=F32)
+    if spec.get("reference_wind"):
+        # a non-zero reference wind profile (u0, v0): u_init ramps from u0/2 at
+        # the surface to u0 at the top, v_init from v0 down to 0
+        u0, v0 = spec["reference_wind"]
+        z = np.arange(L, dtype=np.float64) / max(L - 1, 1)
+        u_init = (u0 * (0.5 + 0.5 * z)).astype(F32)
+        v_init = (v0 * (1.0 - z)).astype(F32)
     case = Case(
         name=name, mesh=mesh, grid=grid, reference=reference, state=state, config=cfg,
         n_scalars=spec["n_scalars"], n_moist=spec["n_scalars"] if spec["moist"] else 0,
@@ -134,10 +144,9 @@ def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[Dy
         inv_area_triangle=hsetup.stored_inverse(mesh.areaTriangle),
         inv_dc_edge=hsetup.stored_inverse(mesh.dcEdge),
         inv_dv_edge=hsetup.stored_inverse(mesh.dvEdge),
-        cos_angle_edge=np.cos(angle.astype(np.float64)).astype(F32),
-        sin_angle_edge=np.sin(angle.astype(np.float64)).astype(F32),
+        cos_angle_edge=cos_angle, sin_angle_edge=sin_angle,
         etp=etp, etm=etm, ewp=ewp, ewm=ewm, dss=dss,
-        u_init=np.zeros(nlev, dtype=F32), v_init=np.zeros(nlev, dtype=F32),
+        u_init=u_init, v_init=v_init,
         len_disp=_len_disp(mesh, cfg),
     )
     if use_cache:
@@ -146,6 +155,16 @@ def build_case(name: str = "x1.2562-dry", *, nlev: int = 55, config: Optional[Dy
             pickle.dump(case, f, protocol=pickle.HIGHEST_PROTOCOL)
         os.replace(tmp, path)
     return case
+
+
+def _generator_digest() -> str:
+    """sha256 of the source of every module that builds a case, so a cached
+    case is never served after the generator changed."""
+    here = Path(__file__).resolve().parent
+    h = hashlib.sha256()
+    for mod in ("case.py", "mesh.py", "initial.py", "setup.py", "config.py"):
+        h.update((here / mod).read_bytes())
+    return h.hexdigest()
 
 
 def _progress(name: str):

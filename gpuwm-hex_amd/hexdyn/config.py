@@ -52,6 +52,10 @@ class DycoreConfig:
     config_del4u_div_factor: float = 10.0
     config_h_ScaleWithMesh: bool = True
     config_mpas_cam_coef: float = 0.0
+    config_number_cam_damping_levels: int = 4
+    config_rayleigh_damp_u: bool = False
+    config_number_rayleigh_damp_u_levels: int = 6
+    config_rayleigh_damp_u_timescale_days: float = 5.0
     config_moist_physics: bool = False
     numerics: str = "strict"
 
@@ -110,8 +114,23 @@ class DycoreConfig:
                 raise ConfigurationRefusal("config_del4u_div_factor", self.config_del4u_div_factor,
                                            "the Registry requires a positive divergent hyperdiffusion factor",
                                            "config_del4u_div_factor>0")
-            if self.config_mpas_cam_coef != 0.0:
+            if not math.isfinite(self.config_mpas_cam_coef) or self.config_mpas_cam_coef < 0.0:
                 raise ConfigurationRefusal("config_mpas_cam_coef", self.config_mpas_cam_coef,
-                                           "the CAM viscosity floor is not implemented", "0.0")
+                                           "the CAM viscosity floor needs a non-negative coefficient",
+                                           "config_mpas_cam_coef>=0")
+            if self.config_mpas_cam_coef > 0.0 and self.config_number_cam_damping_levels < 1:
+                raise ConfigurationRefusal("config_number_cam_damping_levels",
+                                           self.config_number_cam_damping_levels,
+                                           "the CAM floor needs at least one level", "1..nVertLevels")
+        if self.config_rayleigh_damp_u:
+            if self.config_number_rayleigh_damp_u_levels < 1:
+                raise ConfigurationRefusal("config_number_rayleigh_damp_u_levels",
+                                           self.config_number_rayleigh_damp_u_levels,
+                                           "the damping layer must hold at least one level", "1..nVertLevels")
+            days = self.config_rayleigh_damp_u_timescale_days
+            if not math.isfinite(days) or days <= 0.0:
+                raise ConfigurationRefusal("config_rayleigh_damp_u_timescale_days", days,
+                                           "enabled Rayleigh damping requires a finite positive timescale",
+                                           "config_rayleigh_damp_u_timescale_days>0")
         if self.numerics not in ("strict", "fast"):
             raise ConfigurationRefusal("numerics", self.numerics, "two numerics modes exist", "'strict' or 'fast'")

@@ -88,10 +88,15 @@ class HexDycoreDriver:
             raise RuntimeError("HexDycoreDriver needs a HIP device")
         mesh_s, vert_s, cfg_s, bundle = capi.build_structs(case)
         h = ct.c_void_p()
-        rc = lib().hexdyn_create(ct.byref(mesh_s), ct.byref(vert_s), ct.byref(cfg_s), ct.byref(h))
+        L = lib()
+        rc = L.hexdyn_create(ct.byref(mesh_s), ct.byref(vert_s), ct.byref(cfg_s), ct.byref(h))
         if rc != 0:
-            raise ConfigurationRefusal("hexdyn_create", rc, "the library refused the mesh or configuration",
-                                       "a supported configuration")
+            buf = ct.create_string_buffer(512)
+            L.hexdyn_last_create_error(buf, len(buf))
+            why = buf.value.decode(errors="replace") or f"hexdyn_create returned {rc}"
+            if rc == capi.HEXDYN_E_REFUSED:
+                raise ConfigurationRefusal("hexdyn_create", rc, why, "a supported mesh and configuration")
+            raise RuntimeError(f"hexdyn_create failed ({rc}): {why}")
         drv = cls(case, h.value, bundle)
         if state is None:
             s = case.state

@@ -13,7 +13,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libhexdyn.so"
 CSRC = HERE.parent / "csrc"
 
-ABI_VERSION = 3  # HEXDYN_ABI_VERSION in include/hexdyn.h
+ABI_VERSION = 4  # HEXDYN_ABI_VERSION in include/hexdyn.h
 
 _lib = None
 
@@ -45,6 +45,8 @@ def lib() -> ct.CDLL:
     L.hexdyn_destroy.argtypes = [vp]
     L.hexdyn_destroy.restype = None
     L.hexdyn_last_error.argtypes = [vp, ct.c_char_p, ct.c_size_t]
+    L.hexdyn_last_create_error.argtypes = [ct.c_char_p, ct.c_size_t]
+    L.hexdyn_graph_count.argtypes = [vp]
     L.hexdyn_set_state.argtypes = [vp, ct.POINTER(capi.HexdynStateView), ct.c_int, vp]
     L.hexdyn_get_state.argtypes = [vp, ct.POINTER(capi.HexdynStateView), vp]
     L.hexdyn_export_field.argtypes = [vp, ct.c_int, vp, vp]
@@ -81,5 +83,6 @@ EXPORTED = [
     "hexdyn_acoustic_substep", "hexdyn_divergence_damping", "hexdyn_recover", "hexdyn_solve_diagnostics",
     "hexdyn_end_subcycle", "hexdyn_scalars", "hexdyn_end_step", "hexdyn_profile_enable", "hexdyn_profile_read",
     "hexdyn_profile_kernels", "hexdyn_powf", "hexdyn_integrals", "hexdyn_set_graphs",
-    "hexdyn_set_physics_tendencies", "hexdyn_set_post_transport_clamp",
+    "hexdyn_set_physics_tendencies", "hexdyn_set_post_transport_clamp", "hexdyn_last_create_error",
+    "hexdyn_graph_count",
 ]

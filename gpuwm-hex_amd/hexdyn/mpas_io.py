@@ -420,7 +420,7 @@ def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=Non
         grid.rdzw, minimum=cfg.config_epssm_minimum, maximum=cfg.config_epssm_maximum,
         transition_bottom_z=cfg.config_epssm_transition_bottom_z,
         transition_top_z=cfg.config_epssm_transition_top_z)
-    angle = mesh.angleEdge.astype(f32)
+    cos_angle, sin_angle = hsetup.angle_cos_sin(mesh.angleEdge)
     return Case(
         name=name, mesh=mesh, grid=grid, reference=reference, state=state, config=cfg,
         n_scalars=scalars.shape[0], n_moist=scalars.shape[0] if moist else 0,
@@ -429,8 +429,7 @@ def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=Non
         inv_area_triangle=hsetup.stored_inverse(mesh.areaTriangle),
         inv_dc_edge=hsetup.stored_inverse(mesh.dcEdge),
         inv_dv_edge=hsetup.stored_inverse(mesh.dvEdge),
-        cos_angle_edge=np.cos(angle.astype(np.float64)).astype(f32),
-        sin_angle_edge=np.sin(angle.astype(np.float64)).astype(f32),
+        cos_angle_edge=cos_angle, sin_angle_edge=sin_angle,
         etp=etp, etm=etm, ewp=ewp, ewm=ewm,
         dss=hsetup.vertical_velocity_damping(grid.zgrid, xnutr=cfg.config_xnutr, zd=cfg.config_zd),
         u_init=np.asarray(arrays.get("u_init", np.zeros(L)), dtype=f32),

@@ -97,17 +97,41 @@ def acoustic_offcentering(rdzw: np.ndarray, *, minimum: float = 0.1, maximum: fl
     return etp, etm, ewp, ewm
 
 
-def _host_sinf(x: np.ndarray) -> np.ndarray:
-    """binary32 sine by the host C library's sinf, elementwise."""
+def _libm_f32(name: str, nargs: int = 1):
+    """The host C library's binary32 function `name` (sinf, cosf, powf)."""
     import ctypes
     import ctypes.util
 
     libm = ctypes.CDLL(ctypes.util.find_library("m"))
-    fn = libm.sinf
-    fn.argtypes = [ctypes.c_float]
+    fn = getattr(libm, name)
+    fn.argtypes = [ctypes.c_float] * nargs
     fn.restype = ctypes.c_float
+    return fn
+
+
+def _host_sinf(x: np.ndarray) -> np.ndarray:
+    """binary32 sine by the host C library's sinf, elementwise."""
+    fn = _libm_f32("sinf")
     flat = np.asarray(x, dtype=F32).ravel()
     return np.fromiter((fn(float(v)) for v in flat), dtype=F32, count=flat.size).reshape(np.shape(x))
+
+
+def _host_cosf(x: np.ndarray) -> np.ndarray:
+    """binary32 cosine by the host C library's cosf, elementwise."""
+    fn = _libm_f32("cosf")
+    flat = np.asarray(x, dtype=F32).ravel()
+    return np.fromiter((fn(float(v)) for v in flat), dtype=F32, count=flat.size).reshape(np.shape(x))
+
+
+def angle_cos_sin(angle_edge: np.ndarray):
+    """cos and sin of the binary32 angleEdge for the perturbation-Coriolis
+    reference wind u_init*cos(angle) - v_init*sin(angle).  Native MPAS
+    (gfortran, RKIND single) calls the C library's cosf/sinf; the reference's
+    NumPy restatement takes np.cos/np.sin of each float32 angle as a scalar
+    (dynamics_v841.py:160-165), which is the same library call.  Both are the
+    host cosf/sinf here, evaluated once per edge."""
+    a = np.asarray(angle_edge, dtype=F32)
+    return _host_cosf(a), _host_sinf(a)
 
 
 def vertical_velocity_damping(zgrid: np.ndarray, *, xnutr: float, zd: float) -> np.ndarray:
@@ -334,12 +358,15 @@ def deformation_weights(mesh: Mesh):
 
 def mesh_mixing_scaling(mesh: Mesh, *, scale_with_mesh: bool = True):
     """``meshScalingDel2/Del4`` per edge (``mixing.py:271-318``): 1/rho_e^(1/4) and 1/rho_e^(3/4) of the
-    mean cell mesh density, or ones.  On a quasi-uniform mesh (density 1) both are exactly 1."""
+    binary32 mean cell mesh density, or ones, with Fortran's real ** as the host C library's powf (the
+    evaluation hexdyn_create makes).  On a quasi-uniform mesh (density 1) both are exactly 1."""
     n_edges = mesh.cellsOnEdge.shape[0]
     if not scale_with_mesh:
         return np.ones(n_edges, F32), np.ones(n_edges, F32)
     density = np.asarray(mesh.arrays.get("meshDensity", np.ones(mesh.nEdgesOnCell.size)), dtype=F32)
     mean = F32(0.5) * (density[mesh.cellsOnEdge[:, 0]] + density[mesh.cellsOnEdge[:, 1]])
-    del2 = (F32(1.0) / np.power(mean.astype(np.float64), 0.25).astype(F32)).astype(F32)
-    del4 = (F32(1.0) / np.power(mean.astype(np.float64), 0.75).astype(F32)).astype(F32)
+    powf = _libm_f32("powf", 2)
+    one = F32(1.0)
+    del2 = np.array([one / F32(powf(float(m), 0.25)) for m in mean], dtype=F32)
+    del4 = np.array([one / F32(powf(float(m), 0.75)) for m in mean], dtype=F32)
     return del2, del4

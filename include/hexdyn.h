@@ -43,7 +43,7 @@
 extern "C" {
 #endif
 
-#define HEXDYN_ABI_VERSION 3
+#define HEXDYN_ABI_VERSION 4
 #define HEXDYN_MAX_EDGES 7      /* nEdgesOnCell limit (stencil widths derive from it) */
 #define HEXDYN_ROW 64           /* padded level row; nVertLevels + 1 <= 64 */
 
@@ -135,7 +135,18 @@ typedef struct hexdyn_config {
   float config_smagorinsky_coef;           /* >= 0 */
   float config_del4u_div_factor;           /* > 0 */
   int32_t config_h_ScaleWithMesh;          /* bool */
-  float config_mpas_cam_coef;              /* must be 0 (the CAM floor is refused) */
+  /* CAM viscosity floor on kdiff over the top levels (mixing_v841.py:746-750,
+   * cuda_upper_damping_v841.py:5-11): kdiff = max(kdiff, base*(1-(L-1-k)/n)),
+   * base = ((4*2.0833)*config_len_disp)*config_mpas_cam_coef. */
+  float config_mpas_cam_coef;              /* >= 0, 0 = off */
+  int32_t config_number_cam_damping_levels;        /* n in 1..L when the floor is on (MPAS default 4) */
+  /* Upper-level Rayleigh damping of the normal wind on every stage's
+   * momentum tendency, before the Euler pool is added (cuda_driver.py:4427-4433,
+   * native/upper_damping.rs:18-31): tend_ru -= (rho_edge*u)*((k-(L-n)+1)*inv),
+   * inv = 1/(n*(days*86400)), for the top n levels. */
+  int32_t config_rayleigh_damp_u;                  /* bool */
+  int32_t config_number_rayleigh_damp_u_levels;    /* n in 1..L when on (MPAS default 6) */
+  float config_rayleigh_damp_u_timescale_days;     /* > 0 when on (MPAS default 5) */
 } hexdyn_config;
 
 /* Prognostic state and the carried time-level-one diagnostics
@@ -197,6 +208,9 @@ int hexdyn_create(const hexdyn_mesh *mesh, const hexdyn_vertical *vertical,
                   const hexdyn_config *config, hexdyn_handle **out);
 void hexdyn_destroy(hexdyn_handle *h);
 int hexdyn_last_error(const hexdyn_handle *h, char *buf, size_t len);
+/* Why the calling thread's last hexdyn_create failed (the handle of a
+ * refused create no longer exists); empty after a successful create. */
+int hexdyn_last_create_error(char *buf, size_t len);
 
 /* State transfer (device pointers, logical layout).  With rebuild != 0 the
  * saved diagnostics are recomputed from the prognostic fields
@@ -219,6 +233,8 @@ double hexdyn_model_time(const hexdyn_handle *h);
  * per-kernel profiling is enabled).  enable = 0 launches kernel by kernel.
  * (The reference captures CUDA graphs per subcycle, cuda_subcycle_graphs.py.) */
 int hexdyn_set_graphs(hexdyn_handle *h, int enable);
+/* Number of captured step graphs the handle holds (one per buffer layout). */
+int hexdyn_graph_count(const hexdyn_handle *h);
 
 /* Install (view != NULL) or clear (view == NULL) held physics tendencies. */
 int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *view, void *stream);

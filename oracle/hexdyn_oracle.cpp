@@ -87,6 +87,20 @@ static void schedule(double dt, int order, int substeps, int splits, Stage out[3
   }
 }
 
+// Planted mistakes (test hook, Oracle::plant): each is a plausible error the
+// tests must catch.
+enum {
+  PLANT_NONE = 0,
+  PLANT_CORIOLIS_TARGET_ANGLE = 1,  // reference wind from the target edge's angle
+  PLANT_CORIOLIS_NEIGHBOUR_F = 2,   // perturbation Coriolis with the neighbour's f
+  PLANT_BLEND_SIGN = 3,             // third-order upwind blend with the wrong sign
+  PLANT_ADV_SLOT = 4,               // advection weight s paired with cell s+1
+  PLANT_CORRECTION_SIGN = 5,        // RK correction flux of theta with the wrong sign
+  PLANT_VERTICAL_CORRECTION = 6,    // (rw_save - rw) reversed in the vertical theta flux
+  PLANT_W_INTERP = 7,               // fzm/fzp swapped in the w-flux edge mass flux
+  PLANT_OMEGA_SIGN = 8,             // omega conversion with the upwind sign reversed
+};
+
 template <class T> struct Oracle {
   // sizes
   int C, E, V, L, ME, ME2, MA, NT, NM;
@@ -129,6 +143,11 @@ template <class T> struct Oracle {
   // held scalar tendencies, the transport's scalar_tendency (cuda_driver.py:5162-5163)
   int physics_q = 0;
   std::vector<T> phys_q;
+  // the stage's edge fluxes (kept for the tests: theta flux, RK correction flux, w flux)
+  std::vector<T> fth_e, corr_e, fw_e;
+  // Test hook: a deliberate mistake planted in one operator, so the tests can
+  // show that their checks catch it (0 = none; see the PLANT_* codes).
+  int plant = 0;
   Stage dyn[3], sca[3];
   int failed = 0;
   int in_step = 0;
@@ -441,8 +460,9 @@ template <class T> struct Oracle {
         euler_rw[ci(k, cc)] = NM <= 0 ? T(0) - inner : T(0) - cqw[ci(k, cc)] * inner;
       }
     }
-    if (mixing) horizontal_mixing();
+    if (mixing && !pools_only) horizontal_mixing();
   }
+  int pools_only = 0;  // test hook: begin_subcycle stops before the mixing
 
   // ------------------------------------------------------ horizontal mixing
   // v8.4.1 2-D Smagorinsky mixing from the subcycle-start state, in the host
@@ -487,6 +507,13 @@ template <class T> struct Oracle {
         const T diff = d11 - d22;
         const T strain = std::sqrt(quarter * (diff * diff) + d12 * d12);
         kdiff[ci(k, cc)] = std::fmin(strain_scale * strain, ceiling);
+        // CAM floor over the top levels (native/upper_damping.rs:5-16)
+        const int ncam = cfg.config_mpas_cam_coef > 0.0f ? cfg.config_number_cam_damping_levels : 0;
+        if (k >= L - ncam) {
+          const T base = ((T(4) * T(2.0833f)) * len) * T(cfg.config_mpas_cam_coef);
+          const T floor = base * (one - T(L - 1 - k) / T(ncam));
+          kdiff[ci(k, cc)] = std::fmax(kdiff[ci(k, cc)], floor);
+        }
       }
     // del2 u (momentum_filter_seed_v841_f32)
     std::vector<T> delsq_u(eL());
@@ -624,7 +651,7 @@ template <class T> struct Oracle {
   void vertical_transport_u(int e, T *out) const {
     const T half = T(0.5), one = T(1);
     const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
-    std::vector<T> wduz(L + 1, T(0));
+    T wduz[HEXDYN_ROW + 1] = {};
     auto rwe = [&](int k) { return half * (c_rw[(size_t)k * C + c0] + c_rw[(size_t)k * C + c1]); };
     wduz[1] = rwe(1) * (fzm[1] * u[ei(1, e)] + fzp[1] * u[ei(0, e)]);
     for (int k = 2; k < L - 1; ++k)
@@ -633,68 +660,115 @@ template <class T> struct Oracle {
     for (int k = 0; k < L; ++k) out[k] = (-rdzw[k]) * (wduz[k + 1] - wduz[k]);
   }
 
+  // Nonlinear PV/KE momentum terms of edge e at level k with the v8.4.1
+  // perturbation-Coriolis subtraction (dynamics_v841.py:128-185: the second
+  // neighbour loop takes the neighbour edge's angle and the target edge's f):
+  //   (initial + rho_edge*(q - dKE/dc)) - (u*0.5)*(hdiv0 + hdiv1).
+  // Without `initial` it is dynamics.py:273-305's
+  //   rho_edge*(q - dKE/dc) - (u*0.5)*(hdiv0 + hdiv1).
+  T momentum_terms(int e, int k, const T *initial) const {
+    const T half = T(0.5);
+    const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
+    T qq = T(0);
+    for (int s = 0; s < neoe[e]; ++s) {
+      const int n = eoe[(size_t)e * ME2 + s];
+      const T work_pv = half * (pv_edge[ei(k, e)] + pv_edge[ei(k, n)]);
+      qq = qq + (woe[(size_t)e * ME2 + s] * u[ei(k, n)]) * work_pv;
+    }
+    for (int s = 0; s < neoe[e]; ++s) {
+      const int n = eoe[(size_t)e * ME2 + s];
+      const int na = plant == PLANT_CORIOLIS_TARGET_ANGLE ? e : n;
+      const T ref_u = uinit[k] * cosang[na] - vinit[k] * sinang[na];
+      const T f = plant == PLANT_CORIOLIS_NEIGHBOUR_F ? fedge[n] : fedge[e];
+      qq = qq - (woe[(size_t)e * ME2 + s] * ref_u) * f;
+    }
+    const T rho_edge = half * (c_rho[ci(k, c0)] + c_rho[ci(k, c1)]);
+    const T horiz = rho_edge * (qq - (ke[ci(k, c1)] - ke[ci(k, c0)]) * invdc[e]);
+    const T lhs = initial ? *initial + horiz : horiz;
+    return lhs - (u[ei(k, e)] * half) * (hdiv[ci(k, c0)] + hdiv[ci(k, c1)]);
+  }
+
+  // Upper-level Rayleigh damping of u at level k (native/upper_damping.rs:18-31):
+  // t - (rho_edge*u)*((k-(L-n)+1)*inverse) on the top n levels.
+  T rayleigh_u(int e, int k, T t) const {
+    if (!cfg.config_rayleigh_damp_u) return t;
+    const int n = cfg.config_number_rayleigh_damp_u_levels;
+    if (k < L - n) return t;
+    const T inverse = T(1) / (T(n) * (T(cfg.config_rayleigh_damp_u_timescale_days) * T(86400)));
+    const T rho_edge = T(0.5) * (c_rho[ci(k, coe[2 * (size_t)e])] + c_rho[ci(k, coe[2 * (size_t)e + 1])]);
+    return t - (rho_edge * u[ei(k, e)]) * (T(k - (L - n) + 1) * inverse);
+  }
+
+  // w -> omega tendency of cell cc at interface k, 1 <= k < L
+  // (acoustic.py:246-281): r - sum_slots (zb + sign(tend_u)*zb3)*flux, then
+  // times the interface zz.
+  T omega(int cc, int k, T r, const std::vector<T> &tru) const {
+    for (int s = 0; s < neoc[cc]; ++s) {
+      const int e = eoc[(size_t)cc * ME + s];
+      const T tk = tru[ei(k, e)];
+      const T flux = sgn(cc, s) * (fzm[k] * tk + fzp[k] * tru[ei(k - 1, e)]);
+      const size_t zi = ((size_t)k * C + cc) * ME + s;
+      const T up = plant == PLANT_OMEGA_SIGN ? -copysign1(tk) : copysign1(tk);
+      r = r - (zbc[zi] + up * zb3c[zi]) * flux;
+    }
+    return r * (fzm[k] * zz[ci(k, cc)] + fzp[k] * zz[ci(k - 1, cc)]);
+  }
+
   void dyn_tend(int stage) {
     const T half = T(0.5), one = T(1);
     const T coef = T(cfg.config_coef_3rd_order);
     if (stage == 2 && cfg.config_time_integration_order == 3)
       vertical_coefficients(dyn[1].acoustic, s_exner, c_theta, c_rtheta_p);
     mass_flux_divergence(c_ru, hdiv);
-    // momentum (vertical transport, then v8.4.1 horizontal terms, then Euler pool)
+    // momentum (vertical transport, then v8.4.1 horizontal terms, Rayleigh
+    // damping, then the Euler pool)
 #pragma omp parallel for
     for (int e = 0; e < E; ++e) {
-      const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
-      std::vector<T> vtu(L);
-      vertical_transport_u(e, vtu.data());
+      T vtu[HEXDYN_ROW];
+      vertical_transport_u(e, vtu);
       for (int k = 0; k < L; ++k) {
-        const T initial = vtu[k];
-        T qq = T(0);
-        for (int s = 0; s < neoe[e]; ++s) {
-          const int n = eoe[(size_t)e * ME2 + s];
-          const T work_pv = half * (pv_edge[ei(k, e)] + pv_edge[ei(k, n)]);
-          qq = qq + (woe[(size_t)e * ME2 + s] * u[ei(k, n)]) * work_pv;
-        }
-        for (int s = 0; s < neoe[e]; ++s) {
-          const int n = eoe[(size_t)e * ME2 + s];
-          const T ref_u = uinit[k] * cosang[n] - vinit[k] * sinang[n];
-          qq = qq - (woe[(size_t)e * ME2 + s] * ref_u) * fedge[e];
-        }
-        const T rho_edge = half * (c_rho[ci(k, c0)] + c_rho[ci(k, c1)]);
-        const T horiz = rho_edge * (qq - (ke[ci(k, c1)] - ke[ci(k, c0)]) * invdc[e]);
-        T t = (initial + horiz) - (u[ei(k, e)] * half) * (hdiv[ci(k, c0)] + hdiv[ci(k, c1)]);
+        const T t = rayleigh_u(e, k, momentum_terms(e, k, &vtu[k]));
         tend_ru[ei(k, e)] = t + euler_ru[ei(k, e)];
         if (physics) tend_ru[ei(k, e)] = tend_ru[ei(k, e)] + phys_ru[ei(k, e)];
       }
     }
     // theta: edge fluxes (stored-inverse branch, driver.py:1003-1034)
-    std::vector<T> fth(eL()), corr(eL());
+    fth_e.assign(eL(), T(0));
+    corr_e.assign(eL(), T(0));
+    std::vector<T> &fth = fth_e, &corr = corr_e;
+    const T sv_sign = plant == PLANT_BLEND_SIGN ? T(-1) : T(1);
 #pragma omp parallel for
     for (int e = 0; e < E; ++e) {
       const int c0 = coe[2 * (size_t)e], c1 = coe[2 * (size_t)e + 1];
       for (int k = 0; k < L; ++k) {
         const T rue = c_ru[ei(k, e)];
-        const T sv = copysign1(rue);
+        const T sv = sv_sign * copysign1(rue);
         T et = T(0);
         for (int s = 0; s < nadv[e]; ++s) {
           const size_t j = (size_t)e * MA + s;
+          const size_t jc = plant == PLANT_ADV_SLOT ? (size_t)e * MA + (s + 1) % nadv[e] : j;
           const T wgt = adv[j] + (coef * sv) * adv3[j];
-          et = et + wgt * c_theta[ci(k, advc[j])];
+          et = et + wgt * c_theta[ci(k, advc[jc])];
         }
         fth[ei(k, e)] = rue * et;
-        corr[ei(k, e)] = ((dv[e] * (s_ru[ei(k, e)] - rue)) * half) * (s_theta[ci(k, c1)] + s_theta[ci(k, c0)]);
+        const T dru = plant == PLANT_CORRECTION_SIGN ? rue - s_ru[ei(k, e)] : s_ru[ei(k, e)] - rue;
+        corr[ei(k, e)] = ((dv[e] * dru) * half) * (s_theta[ci(k, c1)] + s_theta[ci(k, c0)]);
       }
     }
 #pragma omp parallel for
     for (int cc = 0; cc < C; ++cc) {
-      std::vector<T> vf(L + 1, T(0));
+      T vf[HEXDYN_ROW + 1] = {};
       auto th = [&](int k) { return c_theta[ci(k, cc)]; };
       auto ths = [&](int k) { return s_theta[ci(k, cc)]; };
       auto rwc = [&](int k) { return c_rw[(size_t)k * C + cc]; };
       auto rws = [&](int k) { return s_rw[(size_t)k * C + cc]; };
+      // (rws - rwc): the RK correction of the vertical flux (planted mistake: reversed)
+      auto drw = [&](int k) { return plant == PLANT_VERTICAL_CORRECTION ? rwc(k) - rws(k) : rws(k) - rwc(k); };
       vf[1] = rwc(1) * (fzm[1] * th(1) + fzp[1] * th(0));
-      vf[1] = vf[1] + (rws(1) - rwc(1)) * (fzm[1] * ths(1) + fzp[1] * ths(0));
+      vf[1] = vf[1] + drw(1) * (fzm[1] * ths(1) + fzp[1] * ths(0));
       for (int k = 2; k < L - 1; ++k) {
-        vf[k] = flux3(th(k - 2), th(k - 1), th(k), th(k + 1), rwc(k), coef);
-        vf[k] = vf[k] + (rws(k) - rwc(k)) * (fzm[k] * ths(k) + fzp[k] * ths(k - 1));
+        vf[k] = flux3(th(k - 2), th(k - 1), th(k), th(k + 1), rwc(k), plant == PLANT_BLEND_SIGN ? -coef : coef);
+        vf[k] = vf[k] + drw(k) * (fzm[k] * ths(k) + fzp[k] * ths(k - 1));
       }
       vf[L - 1] = rws(L - 1) * (fzm[L - 1] * th(L - 1) + fzp[L - 1] * th(L - 2));
       for (int k = 0; k < L; ++k) {
@@ -715,17 +789,20 @@ template <class T> struct Oracle {
       }
     }
     // w transport (stored-inverse branch, driver.py:1140-1207), Euler pool, omega conversion
-    std::vector<T> fw((size_t)(L + 1) * E, T(0));
+    fw_e.assign((size_t)(L + 1) * E, T(0));
+    std::vector<T> &fw = fw_e;
 #pragma omp parallel for
     for (int e = 0; e < E; ++e) {
       for (int k = 1; k < L; ++k) {
-        const T rui = fzm[k] * c_ru[ei(k, e)] + fzp[k] * c_ru[ei(k - 1, e)];
-        const T sv = copysign1(rui);
+        const T rui = plant == PLANT_W_INTERP ? fzp[k] * c_ru[ei(k, e)] + fzm[k] * c_ru[ei(k - 1, e)]
+                                              : fzm[k] * c_ru[ei(k, e)] + fzp[k] * c_ru[ei(k - 1, e)];
+        const T sv = sv_sign * copysign1(rui);
         T ew = T(0);
         for (int s = 0; s < nadv[e]; ++s) {
           const size_t j = (size_t)e * MA + s;
+          const size_t jc = plant == PLANT_ADV_SLOT ? (size_t)e * MA + (s + 1) % nadv[e] : j;
           const T wgt = adv[j] + (coef * sv) * adv3[j];
-          ew = ew + wgt * w[(size_t)k * C + advc[j]];
+          ew = ew + wgt * w[(size_t)k * C + advc[jc]];
         }
         fw[(size_t)k * E + e] = rui * ew;
       }
@@ -734,10 +811,11 @@ template <class T> struct Oracle {
     for (int cc = 0; cc < C; ++cc) {
       auto rwc = [&](int k) { return c_rw[(size_t)k * C + cc]; };
       auto ww = [&](int k) { return w[(size_t)k * C + cc]; };
-      std::vector<T> vf(L + 1, T(0));
+      T vf[HEXDYN_ROW + 1] = {};
       vf[1] = (T(0.25) * (rwc(1) + rwc(0))) * (ww(1) + ww(0));
       for (int k = 2; k < L - 1; ++k)
-        vf[k] = flux3(ww(k - 2), ww(k - 1), ww(k), ww(k + 1), half * (rwc(k) + rwc(k - 1)), one);
+        vf[k] = flux3(ww(k - 2), ww(k - 1), ww(k), ww(k + 1), half * (rwc(k) + rwc(k - 1)),
+                      plant == PLANT_BLEND_SIGN ? -one : one);
       vf[L - 1] = (T(0.25) * (rwc(L - 1) + rwc(L - 2))) * (ww(L - 1) + ww(L - 2));
       tend_rw[cc] = T(0);
       tend_rw[(size_t)L * C + cc] = T(0);
@@ -750,16 +828,7 @@ template <class T> struct Oracle {
         r = r * invarea[cc];
         r = r - rdzu[k] * (vf[k + 1] - vf[k]);
         r = r + euler_rw[(size_t)k * C + cc];
-        // omega conversion (acoustic.py:272-280)
-        for (int s = 0; s < neoc[cc]; ++s) {
-          const int e = eoc[(size_t)cc * ME + s];
-          const T tk = tend_ru[ei(k, e)];
-          const T flux = sgn(cc, s) * (fzm[k] * tk + fzp[k] * tend_ru[ei(k - 1, e)]);
-          const size_t zi = ((size_t)k * C + cc) * ME + s;
-          r = r - (zbc[zi] + copysign1(tk) * zb3c[zi]) * flux;
-        }
-        r = r * (fzm[k] * zz[ci(k, cc)] + fzp[k] * zz[ci(k - 1, cc)]);
-        tend_rw[(size_t)k * C + cc] = r;
+        tend_rw[(size_t)k * C + cc] = omega(cc, k, r, tend_ru);
       }
     }
   }
@@ -801,7 +870,7 @@ template <class T> struct Oracle {
     const T dts2 = dts_r * dts_r;
 #pragma omp parallel for
     for (int cc = 0; cc < C; ++cc) {
-      std::vector<T> rs(L), ts(L), col(L + 1);
+      T rs[HEXDYN_ROW] = {}, ts[HEXDYN_ROW] = {}, col[HEXDYN_ROW + 1] = {};
       const T ia = invarea[cc];
       for (int k = 0; k < L; ++k) {
         T r = T(0), t = T(0);
@@ -924,7 +993,7 @@ template <class T> struct Oracle {
     const T one = T(1);
 #pragma omp parallel for
     for (int cc = 0; cc < C; ++cc) {
-      std::vector<T> col(L + 1, T(0));
+      T col[HEXDYN_ROW + 1] = {};
       for (int k = 1; k < L; ++k)
         col[k] = r_w[(size_t)k * C + cc] / (fzm[k] * zz[ci(k, cc)] + fzp[k] * zz[ci(k - 1, cc)]);
       for (int s = 0; s < neoc[cc]; ++s) {
@@ -1009,7 +1078,7 @@ template <class T> struct Oracle {
           }
 #pragma omp parallel for
         for (int cc = 0; cc < C; ++cc) {
-          std::vector<T> vf(L + 1, T(0));
+          T vf[HEXDYN_ROW + 1] = {};
           for (int k = 1; k < L; ++k) vf[k] = vflux(qs, wa, cc, k, coef);
           for (int k = 0; k < L; ++k) {
             T tc = T(0);
@@ -1096,7 +1165,7 @@ template <class T> struct Oracle {
         vres[cc] = dt_s * T(0);
         vres[(size_t)L * C + cc] = dt_s * T(0);
         for (int k = 1; k < L; ++k) vres[(size_t)k * C + cc] = dt_s * vflux(qs, wa, cc, k, coef);
-        std::vector<T> fcol(L + 1, zero);
+        T fcol[HEXDYN_ROW + 1] = {};
         for (int k = 1; k < L; ++k) {
           const T vel = wa[(size_t)k * C + cc];
           fcol[k] = dt_s * (pmax0(vel) * qo[ci(k - 1, cc)] + pmin0(vel) * qo[ci(k, cc)]);
@@ -1180,7 +1249,8 @@ template <class T> struct Oracle {
 
   void end_step() {
     if (cfg.config_scalar_advection && cfg.config_split_dynamics_transport && NT > 0) q = q_stage;
-    if (NM > 0 && post_clamp)
+    // the q < 0 -> 0 where-assignment on every species (cuda_driver.py:5306-5311)
+    if (post_clamp)
       for (auto &x : q)
         if (x < T(0)) x = T(0);
     rho = c_rho; rt = c_rt; ru = c_ru; rw = c_rw;
@@ -1293,7 +1363,10 @@ template <class O> static auto *internal(O &o, const std::string &n) {
       {"w", &o.w}, {"u", &o.u}, {"exner", &o.exner}, {"pp", &o.pp},
       {"cofwr", &o.cofwr}, {"rtheta_pp_old", &o.rtheta_pp_old}, {"tend_rho", &o.tend_rho},
       {"split_u", &o.split_u}, {"split_w", &o.split_w}, {"o_rho", &o.o_rho}, {"o_q", &o.o_q},
-      {"dpdz", &o.dpdz}};
+      {"dpdz", &o.dpdz}, {"fth", &o.fth_e}, {"corr", &o.corr_e}, {"fw", &o.fw_e}, {"hdiv", &o.hdiv},
+      {"euler_ru", &o.euler_ru}, {"euler_rw", &o.euler_rw}, {"euler_theta", &o.euler_theta},
+      {"kdiff", &o.kdiff}, {"v", &o.v}, {"divergence", &o.divergence}, {"vorticity", &o.vorticity},
+      {"tend_ru", &o.tend_ru}, {"tend_rt", &o.tend_rt}, {"tend_rw", &o.tend_rw}};
   for (const auto &t : table)
     if (n == t.first) return t.second;
   return (V *)nullptr;
@@ -1463,6 +1536,64 @@ int oracle_end_step(void *h) {
   return rc;
 }
 float oracle_powf(float x, float y) { return power<float>(x, y); }
+
+int oracle_set_plant(void *h, int code) {
+  DISPATCH(h, o.plant = code);
+  return 0;
+}
+
+// begin_subcycle without the horizontal mixing, then the mixing alone: the
+// two halves of oracle_begin_subcycle (the tests compare the pools the mixing
+// starts from and the increments it adds)
+int oracle_begin_subcycle_pools(void *h) {
+  DISPATCH(h, {
+    o.pools_only = 1;
+    o.begin_subcycle();
+    o.pools_only = 0;
+  });
+  return 0;
+}
+int oracle_horizontal_mixing(void *h) {
+  DISPATCH(h, if (o.mixing) o.horizontal_mixing());
+  return 0;
+}
+
+// The momentum terms of every edge and level without the vertical transport
+// and pools (dynamics.py:273-305's association) from the current u, rho_zz,
+// pv_edge, kinetic energy and the mass-flux divergence of the current ru.
+int oracle_momentum_terms(void *h, float *out, int64_t n) {
+  int rc = 0;
+  DISPATCH(h, {
+    if (n != (int64_t)o.eL()) {
+      rc = 1;
+    } else {
+      o.mass_flux_divergence(o.c_ru, o.hdiv);
+      for (int e = 0; e < o.E; ++e)
+        for (int k = 0; k < o.L; ++k) out[o.ei(k, e)] = (float)o.momentum_terms(e, k, nullptr);
+    }
+  });
+  return rc;
+}
+
+// The omega conversion applied to a given w tendency [L+1][C] with a given
+// momentum tendency [L][E] (acoustic.py:246-281); out [L+1][C], rows 0 and L
+// copied.
+int oracle_omega_conversion(void *h, const float *w_tend, const float *u_tend, float *out) {
+  DISPATCH(h, {
+    using V = typename std::remove_reference<decltype(o.tend_ru)>::type;
+    V tru(o.eL());
+    for (size_t i = 0; i < o.eL(); ++i) tru[i] = u_tend[i];
+    for (int cc = 0; cc < o.C; ++cc) {
+      out[cc] = w_tend[cc];
+      out[(size_t)o.L * o.C + cc] = w_tend[(size_t)o.L * o.C + cc];
+      for (int k = 1; k < o.L; ++k) {
+        using TT = typename V::value_type;
+        out[(size_t)k * o.C + cc] = (float)o.omega(cc, k, (TT)w_tend[(size_t)k * o.C + cc], tru);
+      }
+    }
+  });
+  return 0;
+}
 
 // Test hooks: a working array by member name (size, export, import), and the
 // implicit coefficients of the imported step-start exner, theta_m and
