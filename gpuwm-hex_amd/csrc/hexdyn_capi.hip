@@ -591,9 +591,10 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
         const float d1 = m->mesh_density ? m->mesh_density[a1] : 1.0f;
         const float mean = 0.5f * (d0 + d1);
         if (!(mean > 0.0f)) return refuse("meshDensity must be positive");
-        // Fortran ** of reals: the C library's powf (host code)
-        del2[ne] = 1.0f / ::powf(mean, 0.25f);
-        del4[ne] = 1.0f / ::powf(mean, 0.75f);
+        // the reference's mesh-scaling power: binary64 pow of the binary32
+        // mean, rounded once (mixing.py:313-318 through rkind_libm.powf_rkind)
+        del2[ne] = 1.0f / (float)std::pow((double)mean, (double)0.25f);
+        del4[ne] = 1.0f / (float)std::pow((double)mean, (double)0.75f);
       }
     }
     std::vector<float> c2s((size_t)C * ME, 0.0f), s2s((size_t)C * ME, 0.0f), css((size_t)C * ME, 0.0f),

@@ -237,6 +237,10 @@ class HexDycoreDriver:
         _check(self._h, lib().hexdyn_set_post_transport_clamp(self._h, 1 if enable else 0),
                "hexdyn_set_post_transport_clamp")
 
+    def graph_count(self) -> int:
+        """Captured step graphs held (one per buffer layout)."""
+        return int(lib().hexdyn_graph_count(self._h))
+
     def use_graphs(self, enable: bool) -> None:
         """Step through captured hipGraphs (default) or kernel by kernel."""
         _check(self._h, lib().hexdyn_set_graphs(self._h, 1 if enable else 0), "hexdyn_set_graphs")

@@ -21,6 +21,8 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 from typing import Dict, Optional
 
+import os
+
 import numpy as np
 from scipy.special import sph_harm_y
 
@@ -228,16 +230,55 @@ def _jw_fields(eta: np.ndarray, lat: np.ndarray, radius: float):
     return temp, phi, u
 
 
+def _newton_update(eta: np.ndarray, z: np.ndarray, lat: np.ndarray, radius: float):
+    """One Newton update of phi(eta) = g*z, elementwise; returns the new eta
+    and max |step/eta|."""
+    temp, phi, _ = _jw_fields(eta, lat, radius)
+    f = phi - GRAVITY * z
+    dfdeta = -RGAS * temp / eta
+    step = f / dfdeta
+    new = np.clip(eta - step, 1e-5, 1.3)
+    return new, float(np.max(np.abs(step / new))) if new.size else 0.0
+
+
 def _eta_from_height(z: np.ndarray, lat: np.ndarray, radius: float) -> np.ndarray:
+    """Newton iteration for the JW vertical coordinate eta at heights z, all
+    points iterated until the largest relative step is below 1e-13.  Large
+    arrays are split into column chunks updated on threads (numpy releases
+    the GIL in its elementwise loops); every element sees the same operations
+    and the same iteration count as one whole-array update."""
     eta = np.clip(np.exp(-z / 7500.0), 1e-4, 1.2)
-    for _ in range(60):
-        temp, phi, _ = _jw_fields(eta, lat, radius)
-        f = phi - GRAVITY * z
-        dfdeta = -RGAS * temp / eta
-        step = f / dfdeta
-        eta = np.clip(eta - step, 1e-5, 1.3)
-        if np.max(np.abs(step / eta)) < 1e-13:
-            break
+    z = np.broadcast_to(z, eta.shape)
+    lat = np.broadcast_to(lat, eta.shape)
+    cols = eta.shape[-1] if eta.ndim else 1
+    nchunk = 1
+    if eta.size >= (1 << 20) and eta.ndim == 2:
+        nchunk = max(1, min(16, os.cpu_count() or 1))
+    if nchunk == 1:
+        for _ in range(60):
+            eta, rel = _newton_update(eta, z, lat, radius)
+            if rel < 1e-13:
+                break
+        return eta
+    from concurrent.futures import ThreadPoolExecutor
+
+    bounds = np.linspace(0, cols, nchunk + 1, dtype=np.int64)
+    parts = [(slice(None), slice(int(bounds[i]), int(bounds[i + 1]))) for i in range(nchunk)]
+    eta = np.ascontiguousarray(eta)
+    with ThreadPoolExecutor(max_workers=nchunk) as pool:
+        for _ in range(60):
+            def run(sl):
+                new, rel = _newton_update(eta[sl], z[sl], lat[sl], radius)
+                return sl, new, rel
+            results = list(pool.map(run, parts))
+            out = np.empty_like(eta)
+            rel = 0.0
+            for sl, new, r in results:
+                out[sl] = new
+                rel = max(rel, r)
+            eta = out
+            if rel < 1e-13:
+                break
     return eta
 
 

@@ -222,62 +222,77 @@ def build_mesh(points: np.ndarray, *, radius: float = EARTH_RADIUS) -> Mesh:
     cells_on_edge = edges.copy()
     vertices_on_edge = np.stack([right_tri, left_tri], axis=1)
 
-    # Per-vertex (triangle) tables, counter-clockwise.
+    # Per-vertex (triangle) tables, counter-clockwise.  Pairs are looked up
+    # through sorted int64 keys a*n + b.
     cells_on_vertex = tri.copy()
-    edge_lookup = {}
-    for e in range(n_edges):
-        edge_lookup[(int(edges[e, 0]), int(edges[e, 1]))] = e
+    n = np.int64(n_cells)
+    edge_keys = edges[:, 0].astype(np.int64) * n + edges[:, 1]  # sorted (np.unique order)
     edges_on_vertex = np.empty((n_vertices, 3), dtype=np.int64)
     for j in range(3):
         a = tri[:, j]
         b = tri[:, (j + 1) % 3]
-        lo = np.minimum(a, b)
-        hi = np.maximum(a, b)
-        edges_on_vertex[:, j] = [edge_lookup[(int(x), int(y))] for x, y in zip(lo, hi)]
+        k = np.minimum(a, b).astype(np.int64) * n + np.maximum(a, b)
+        pos = np.searchsorted(edge_keys, k)
+        if np.any(edge_keys[np.minimum(pos, n_edges - 1)] != k):
+            raise ValueError("triangle side without an edge")
+        edges_on_vertex[:, j] = pos
     kite_areas = kite * radius * radius
     area_triangle = _spherical_triangle_area(points[tri[:, 0]], points[tri[:, 1]], points[tri[:, 2]]) * radius * radius
 
-    # Per-cell tables, counter-clockwise around the generator.
-    neighbors = [[] for _ in range(n_cells)]
-    for e in range(n_edges):
-        c1, c2 = int(edges[e, 0]), int(edges[e, 1])
-        neighbors[c1].append((c2, e))
-        neighbors[c2].append((c1, e))
-    max_edges = max(len(n) for n in neighbors)
-    edges_on_cell = np.full((n_cells, max_edges), -1, dtype=np.int64)
-    cells_on_cell = np.full((n_cells, max_edges), -1, dtype=np.int64)
-    vertices_on_cell = np.full((n_cells, max_edges), -1, dtype=np.int64)
-    n_edges_on_cell = np.zeros(n_cells, dtype=np.int64)
+    # Per-cell tables, counter-clockwise around the generator: every cell's
+    # (neighbour, edge) entries, ordered by the angle of the neighbour in the
+    # cell's tangent plane (east-north frame; x-based at the poles).
+    owner = np.concatenate([edges[:, 0], edges[:, 1]]).astype(np.int64)
+    other_cell = np.concatenate([edges[:, 1], edges[:, 0]]).astype(np.int64)
+    edge_id = np.concatenate([np.arange(n_edges), np.arange(n_edges)]).astype(np.int64)
+    order = np.lexsort((edge_id, owner))
+    owner, other_cell, edge_id = owner[order], other_cell[order], edge_id[order]
+    n_edges_on_cell = np.bincount(owner, minlength=n_cells).astype(np.int64)
+    max_edges = int(n_edges_on_cell.max())
+    start = np.concatenate([[0], np.cumsum(n_edges_on_cell)[:-1]])
+    slot = np.arange(owner.size) - start[owner]
+    nb_cell = np.full((n_cells, max_edges), -1, dtype=np.int64)
+    nb_edge = np.full((n_cells, max_edges), -1, dtype=np.int64)
+    nb_cell[owner, slot] = other_cell
+    nb_edge[owner, slot] = edge_id
     north = np.array([0.0, 0.0, 1.0])
-    # vertex lookup by its ordered cell triple
-    tri_lookup = {}
-    for v in range(n_vertices):
-        a, b, c = (int(x) for x in tri[v])
-        tri_lookup[(a, b)] = v
-        tri_lookup[(b, c)] = v
-        tri_lookup[(c, a)] = v
-    for c in range(n_cells):
-        p = points[c]
-        ref = north - p * p[2]
-        if np.linalg.norm(ref) < 1e-8:
-            ref = np.array([1.0, 0.0, 0.0]) - p * p[0]
-        ref = ref / np.linalg.norm(ref)
-        other = np.cross(p, ref)
-        nb = neighbors[c]
-        ang = [np.arctan2(np.dot(points[n] - p, other), np.dot(points[n] - p, ref)) for n, _ in nb]
-        order = np.argsort(ang)
-        nb = [nb[i] for i in order]
-        count = len(nb)
-        n_edges_on_cell[c] = count
-        for i, (n, e) in enumerate(nb):
-            cells_on_cell[c, i] = n
-            edges_on_cell[c, i] = e
-        for i in range(count):
-            prv = nb[(i - 1) % count][0]
-            # CCW triangle (c, n_{i-1}, n_i) is the vertex between edges i-1 and i
-            vertices_on_cell[c, i] = tri_lookup[(c, prv)]
-            if tri_lookup[(c, prv)] != tri_lookup[(prv, nb[i][0])]:
-                raise ValueError("cell neighbourhood is not a fan")
+    ref = north[None, :] - points * points[:, 2:3]
+    polar = np.linalg.norm(ref, axis=1) < 1e-8
+    ref[polar] = np.array([1.0, 0.0, 0.0])[None, :] - points[polar] * points[polar, 0:1]
+    ref = ref / np.linalg.norm(ref, axis=1, keepdims=True)
+    other = np.cross(points, ref)
+    valid = nb_cell >= 0
+    d = points[np.where(valid, nb_cell, 0)] - points[:, None, :]
+    ang = np.arctan2(np.einsum("csk,ck->cs", d, other), np.einsum("csk,ck->cs", d, ref))
+    ang = np.where(valid, ang, np.inf)
+    rank = np.argsort(ang, axis=1, kind="stable")
+    cells_on_cell = np.take_along_axis(nb_cell, rank, axis=1)
+    edges_on_cell = np.take_along_axis(nb_edge, rank, axis=1)
+    # vertex between edges i-1 and i: the CCW triangle (c, n_{i-1}, n_i)
+    tri_keys = np.concatenate([tri[:, 0].astype(np.int64) * n + tri[:, 1], tri[:, 1].astype(np.int64) * n + tri[:, 2],
+                               tri[:, 2].astype(np.int64) * n + tri[:, 0]])
+    tri_ids = np.concatenate([np.arange(n_vertices)] * 3).astype(np.int64)
+    korder = np.argsort(tri_keys)
+    tri_keys, tri_ids = tri_keys[korder], tri_ids[korder]
+
+    def tri_lookup(a, b):
+        k = a.astype(np.int64) * n + b
+        pos = np.minimum(np.searchsorted(tri_keys, k), tri_keys.size - 1)
+        if np.any(tri_keys[pos] != k):
+            raise ValueError("cell neighbourhood is not a fan")
+        return tri_ids[pos]
+
+    cidx = np.repeat(np.arange(n_cells), max_edges).reshape(n_cells, max_edges)
+    sl = np.arange(max_edges)[None, :]
+    prev_slot = (sl - 1) % n_edges_on_cell[:, None]
+    prv = np.take_along_axis(cells_on_cell, np.where(sl < n_edges_on_cell[:, None], prev_slot, 0), axis=1)
+    vertices_on_cell = np.full((n_cells, max_edges), -1, dtype=np.int64)
+    used = sl < n_edges_on_cell[:, None]
+    v_a = tri_lookup(cidx[used], prv[used])
+    v_b = tri_lookup(prv[used], cells_on_cell[used])
+    if np.any(v_a != v_b):
+        raise ValueError("cell neighbourhood is not a fan")
+    vertices_on_cell[used] = v_a
 
     xyz_cell = points * radius
     xyz_vertex = cc * radius

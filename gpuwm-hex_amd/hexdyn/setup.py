@@ -358,15 +358,15 @@ def deformation_weights(mesh: Mesh):
 
 def mesh_mixing_scaling(mesh: Mesh, *, scale_with_mesh: bool = True):
     """``meshScalingDel2/Del4`` per edge (``mixing.py:271-318``): 1/rho_e^(1/4) and 1/rho_e^(3/4) of the
-    binary32 mean cell mesh density, or ones, with Fortran's real ** as the host C library's powf (the
-    evaluation hexdyn_create makes).  On a quasi-uniform mesh (density 1) both are exactly 1."""
+    binary32 mean cell mesh density, or ones.  The power is binary64 pow of the binary32 mean, rounded
+    once (the reference's ``rkind_libm.powf_rkind``, the evaluation hexdyn_create makes).  On a
+    quasi-uniform mesh (density 1) both are exactly 1."""
     n_edges = mesh.cellsOnEdge.shape[0]
     if not scale_with_mesh:
         return np.ones(n_edges, F32), np.ones(n_edges, F32)
     density = np.asarray(mesh.arrays.get("meshDensity", np.ones(mesh.nEdgesOnCell.size)), dtype=F32)
-    mean = F32(0.5) * (density[mesh.cellsOnEdge[:, 0]] + density[mesh.cellsOnEdge[:, 1]])
-    powf = _libm_f32("powf", 2)
+    mean = (F32(0.5) * (density[mesh.cellsOnEdge[:, 0]] + density[mesh.cellsOnEdge[:, 1]])).astype(F32)
     one = F32(1.0)
-    del2 = np.array([one / F32(powf(float(m), 0.25)) for m in mean], dtype=F32)
-    del4 = np.array([one / F32(powf(float(m), 0.75)) for m in mean], dtype=F32)
+    del2 = (one / (mean.astype(np.float64) ** np.float64(F32(0.25))).astype(F32)).astype(F32)
+    del4 = (one / (mean.astype(np.float64) ** np.float64(F32(0.75))).astype(F32)).astype(F32)
     return del2, del4

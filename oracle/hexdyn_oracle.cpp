@@ -34,10 +34,11 @@
  *   scalar transport        transport.py:1649-1826 (stages 1-2),
  *                           transport.py:1829-2175 (monotonic FCT, stage 3)
  *   post-transport clamp    regional_v841.py:1004 (putmask q<0 -> 0)
- * Parity status: see the test suite.  The binary32 power function (exner,
- * mesh scaling) is the host C library's powf: the GNU powf, equal to the
- * reference's own transcription of it (native_powf.py) on every argument in
- * [2^-12, 4) by its golden vectors.
+ * Parity status: see the test suite.  The exner power function is the host
+ * C library's powf: the GNU powf, equal to the reference's own transcription
+ * of it (native_powf.py) on every argument in [2^-12, 4) by its golden
+ * vectors.  The mesh-scaling power is binary64 pow rounded once, the
+ * reference's rkind_libm.powf_rkind.
  */
 #include <algorithm>
 #include <cmath>
@@ -482,8 +483,9 @@ template <class T> struct Oracle {
     if (cfg.config_h_ScaleWithMesh)
       for (int e = 0; e < E; ++e) {
         const T mean = half * (density[coe[2 * (size_t)e]] + density[coe[2 * (size_t)e + 1]]);
-        del2[e] = one / power(mean, T(0.25));
-        del4[e] = one / power(mean, T(0.75));
+        // binary64 pow rounded once (mixing.py:313-318, rkind_libm.py:68-74)
+        del2[e] = one / T(std::pow((double)mean, (double)0.25f));
+        del4[e] = one / T(std::pow((double)mean, (double)0.75f));
       }
     auto vi = [&](int k, int vx) { return (size_t)k * V + vx; };
     auto rho_e = [&](int k, int e) {
@@ -1561,7 +1563,8 @@ int oracle_horizontal_mixing(void *h) {
 // The momentum terms of every edge and level without the vertical transport
 // and pools (dynamics.py:273-305's association) from the current u, rho_zz,
 // pv_edge, kinetic energy and the mass-flux divergence of the current ru.
-int oracle_momentum_terms(void *h, float *out, int64_t n) {
+}  // extern "C"
+template <class D> static int momentum_terms_out(void *h, D *out, int64_t n) {
   int rc = 0;
   DISPATCH(h, {
     if (n != (int64_t)o.eL()) {
@@ -1569,11 +1572,14 @@ int oracle_momentum_terms(void *h, float *out, int64_t n) {
     } else {
       o.mass_flux_divergence(o.c_ru, o.hdiv);
       for (int e = 0; e < o.E; ++e)
-        for (int k = 0; k < o.L; ++k) out[o.ei(k, e)] = (float)o.momentum_terms(e, k, nullptr);
+        for (int k = 0; k < o.L; ++k) out[o.ei(k, e)] = (D)o.momentum_terms(e, k, nullptr);
     }
   });
   return rc;
 }
+extern "C" {
+int oracle_momentum_terms(void *h, float *out, int64_t n) { return momentum_terms_out(h, out, n); }
+int oracle_momentum_terms_f64(void *h, double *out, int64_t n) { return momentum_terms_out(h, out, n); }
 
 // The omega conversion applied to a given w tendency [L+1][C] with a given
 // momentum tendency [L][E] (acoustic.py:246-281); out [L+1][C], rows 0 and L
@@ -1679,6 +1685,16 @@ int oracle_export_btri_ctri_f64(void *h, double *b, double *c, int64_t n) {
   });
   return rc;
 }
+int oracle_export_internal_f64(void *h, const char *name, double *dst, int64_t n) {
+  int rc = 0;
+  DISPATCH(h, {
+    auto *f = internal(o, name);
+    if (!f || (int64_t)f->size() != n) rc = 1;
+    else for (int64_t i = 0; i < n; ++i) dst[i] = (double)(*f)[i];
+  });
+  return rc;
+}
+
 int oracle_export_field_f64(void *h, int id, double *dst, int64_t n) {
   int rc = 0;
   DISPATCH(h, {

@@ -23,6 +23,16 @@ sys.path.insert(0, str(ROOT / "gpuwm-hex_amd"))
 
 from hexdyn import capi  # noqa: E402
 
+# planted mistakes (the PLANT_* enum of hexdyn_oracle.cpp)
+PLANT_CORIOLIS_TARGET_ANGLE = 1
+PLANT_CORIOLIS_NEIGHBOUR_F = 2
+PLANT_BLEND_SIGN = 3
+PLANT_ADV_SLOT = 4
+PLANT_CORRECTION_SIGN = 5
+PLANT_VERTICAL_CORRECTION = 6
+PLANT_W_INTERP = 7
+PLANT_OMEGA_SIGN = 8
+
 LIB = HERE / "liboracle.so"
 
 
@@ -74,6 +84,13 @@ def lib() -> ct.CDLL:
         _lib.oracle_export_internal.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_void_p, ct.c_int64]
         _lib.oracle_import_internal.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_void_p, ct.c_int64]
         _lib.oracle_vertical_transport_u.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64]
+        _lib.oracle_export_internal_f64.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_void_p, ct.c_int64]
+        _lib.oracle_set_plant.argtypes = [ct.c_void_p, ct.c_int]
+        for name in ("oracle_begin_subcycle_pools", "oracle_horizontal_mixing"):
+            getattr(_lib, name).argtypes = [ct.c_void_p]
+        _lib.oracle_momentum_terms.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64]
+        _lib.oracle_momentum_terms_f64.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64]
+        _lib.oracle_omega_conversion.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
         for name in ("oracle_libm_powf_many", "oracle_hexdyn_powf_many"):
             getattr(_lib, name).argtypes = [ct.c_void_p, ct.c_float, ct.c_void_p, ct.c_int64]
             getattr(_lib, name).restype = None
@@ -161,6 +178,56 @@ class Oracle:
         a = np.ascontiguousarray(values, dtype=np.float32).ravel()
         if lib().oracle_import_internal(self.h, name.encode(), a.ctypes.data, a.size):
             raise KeyError(name)
+
+    def export_internal_f64(self, name: str) -> np.ndarray:
+        n = lib().oracle_internal_size(self.h, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        out = np.empty(n, dtype=np.float64)
+        if lib().oracle_export_internal_f64(self.h, name.encode(), out.ctypes.data, n):
+            raise KeyError(name)
+        return out
+
+    def set_plant(self, code: int) -> None:
+        """Plant a deliberate mistake in one operator (PLANT_* in the C++
+        source; 0 removes it).  Test hook: shows that a check catches it."""
+        lib().oracle_set_plant(self.h, int(code))
+
+    def begin_subcycle_pools(self):
+        """begin_subcycle up to, not including, the horizontal mixing."""
+        return lib().oracle_begin_subcycle_pools(self.h)
+
+    def horizontal_mixing(self):
+        """The mixing half of begin_subcycle (after begin_subcycle_pools)."""
+        return lib().oracle_horizontal_mixing(self.h)
+
+    def momentum_terms(self) -> np.ndarray:
+        """rho_edge*(q - dKE/dc) - (u*0.5)*(hdiv0+hdiv1) of every edge and
+        level from the current u, rho_zz, pv_edge, KE and ru, flat [L][E]
+        (the per-element code dyn_tend runs, without its initial term)."""
+        n = self.case.n_levels * self.case.mesh.n_edges
+        out = np.empty(n, dtype=np.float32)
+        if lib().oracle_momentum_terms(self.h, out.ctypes.data, n):
+            raise ValueError("size mismatch")
+        return out
+
+    def momentum_terms_f64(self) -> np.ndarray:
+        """momentum_terms in the oracle's own precision, as float64."""
+        n = self.case.n_levels * self.case.mesh.n_edges
+        out = np.empty(n, dtype=np.float64)
+        if lib().oracle_momentum_terms_f64(self.h, out.ctypes.data, n):
+            raise ValueError("size mismatch")
+        return out
+
+    def omega_conversion(self, w_tend: np.ndarray, u_tend: np.ndarray) -> np.ndarray:
+        """The w -> omega conversion dyn_tend applies, on a given w tendency
+        [L+1][C] and momentum tendency [L][E]."""
+        L, C, E = self.case.n_levels, self.case.mesh.n_cells, self.case.mesh.n_edges
+        w = np.ascontiguousarray(w_tend, dtype=np.float32).reshape(L + 1, C)
+        u = np.ascontiguousarray(u_tend, dtype=np.float32).reshape(L, E)
+        out = np.empty((L + 1, C), dtype=np.float32)
+        lib().oracle_omega_conversion(self.h, w.ctypes.data, u.ctypes.data, out.ctypes.data)
+        return out
 
     def vertical_transport_u(self) -> np.ndarray:
         """The vertical transport term of the momentum tendency for the

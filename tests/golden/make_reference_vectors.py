@@ -32,9 +32,10 @@ ROOT = HERE.parent.parent
 sys.path.insert(0, str(ROOT / "gpuwm-hex_amd"))
 sys.path.insert(0, str(ROOT))
 
-from reference_walk import (ACOUSTIC_FORCING, ACOUSTIC_STATE, COEFFICIENTS, DAMPING, POOLS, RECOVERY_STATE,  # noqa: E402
-                            TRANSPORT, TRANSPORT_FLUX, VCOEF, build_case as build, case_digest, operand_digest, read,
-                            read_all, scalar_tendency, schedule, walk, walk_transport)
+from reference_walk import (ACOUSTIC_FORCING, ACOUSTIC_STATE, COEFFICIENTS, DAMPING, MIXING, MOMENTUM,  # noqa: E402
+                            OMEGA, POOLS, RECOVERY_STATE, TRANSPORT, TRANSPORT_FLUX, VCOEF, build_case as build,
+                            build_mixing_case, case_digest, operand_digest, read, read_all, scalar_tendency,
+                            schedule, walk, walk_mixing, walk_transport)
 
 F32 = np.float32
 
@@ -48,7 +49,7 @@ def mesh_view(case, geometry=F32):
              "kiteForCell", "edgesOnVertex", "cellsOnVertex", "edgesOnEdge", "nEdgesOnEdge", "weightsOnEdge",
              "dcEdge", "dvEdge", "areaCell", "areaTriangle", "kiteAreasOnVertex", "fEdge", "fVertex",
              "angleEdge", "xCell", "yCell", "zCell", "xEdge", "yEdge", "zEdge", "xVertex", "yVertex", "zVertex",
-             "latCell", "lonCell", "latEdge", "lonEdge"]
+             "latCell", "lonCell", "latEdge", "lonEdge", "meshDensity"]
     for n in names:
         try:
             a = np.asarray(getattr(m, n))
@@ -526,6 +527,99 @@ def moist_coefficient_vectors(case):
                 "moist.cqw": np.ascontiguousarray(cqw), "moist.cqu": np.ascontiguousarray(cqu)}
 
 
+def momentum_vectors(case):
+    """dynamics.vector_invariant_momentum_tendency (:273-305), the nonlinear
+    PV/KE terms, fed the reference's own mass_flux_divergence (:86-121) of
+    the current rho_u; reference wind zero (the v8.4.1 form then adds
+    nothing)."""
+    from hexcore.dynamics import mass_flux_divergence, vector_invariant_momentum_tendency
+
+    assert not np.any(case.u_init) and not np.any(case.v_init)
+    mesh = mesh_view(case)
+    c0, c1 = np.asarray(case.mesh.cellsOnEdge, dtype=np.int64).T
+    for label, o in walk(case):
+        if label != "vcoef_2":
+            continue
+        ops = read_all(o, case, MOMENTUM)
+        rho = ops["rho_zz"]
+        rho_edge = (F32(0.5) * (rho[:, c0] + rho[:, c1])).astype(F32)
+        hdiv = mass_flux_divergence(mesh, ops["rho_u"], inv_area_cell=case.inv_area_cell.astype(F32))
+        res = vector_invariant_momentum_tendency(
+            mesh, normal_velocity=ops["normal_velocity"], rho_edge=rho_edge, pv_edge=ops["pv_edge"],
+            kinetic_energy=ops["kinetic_energy"], horizontal_divergence=hdiv)
+        assert hdiv.dtype == F32 and res.dtype == F32
+        return {"momentum.operands": operand_digest(ops.values()), "momentum.hdiv": np.ascontiguousarray(hdiv),
+                "momentum.result": np.ascontiguousarray(res)}
+
+
+def omega_vectors(case):
+    """acoustic.convert_w_tendency_to_omega (:246-281) of a realistic w
+    tendency (the stage's Euler w pool) with the stage's momentum tendency."""
+    from hexcore.acoustic import convert_w_tendency_to_omega
+
+    mesh = mesh_view(case)
+    lev = _levels(case)
+    g = case.grid
+    for label, o in walk(case):
+        if label != "vcoef_2":
+            continue
+        ops = read_all(o, case, OMEGA)
+        res = convert_w_tendency_to_omega(mesh, ops["w_tendency"], ops["u_tendency"], fzm=lev["fzm"], fzp=lev["fzp"],
+                                          zz=np.asarray(g.zz, F32), zb_cell=np.asarray(g.zb_cell, F32),
+                                          zb3_cell=np.asarray(g.zb3_cell, F32))
+        assert res.dtype == F32
+        return {"omega.operands": operand_digest(ops.values()), "omega.result": np.ascontiguousarray(res)}
+
+
+def mixing_vectors(_case):
+    """mixing.compute_mesh_mixing_scaling (:271-323) on a variable mesh
+    density, and the del2/del4 filters the v8.4.1 mixing applies with its
+    kdiff (mixing_v841.py:709-790): momentum_horizontal_filter_tendency
+    (:391-530) onto the subcycle-start u pool,
+    vertical_momentum_horizontal_filter_tendency (:533-635) and
+    theta_horizontal_filter_tendency (:638-733), native stored-inverse
+    association.  kdiff is the oracle's (mixing_v841's viscosity needs the
+    absent Rust engine)."""
+    from hexcore.mixing import (compute_mesh_mixing_scaling, momentum_horizontal_filter_tendency,
+                                theta_horizontal_filter_tendency, vertical_momentum_horizontal_filter_tendency)
+
+    case = build_mixing_case()
+    mesh = mesh_view(case)
+    cfg = case.config
+    length = F32(case.len_disp)
+    h4 = F32(F32(cfg.config_visc4_2dsmag) * ((length * length) * length))
+    sc = compute_mesh_mixing_scaling(mesh, config_h_ScaleWithMesh=True, dtype=np.float32)
+    assert sc.del2.dtype == F32 and np.count_nonzero(sc.del2 != F32(1.0)) > 0
+    out = {"mixing_case_digest": case_digest(case),
+           "mixing_density": np.asarray(case.mesh.arrays["meshDensity"], F32),
+           "scaling.del2": np.ascontiguousarray(sc.del2), "scaling.del4": np.ascontiguousarray(sc.del4)}
+    c0, c1 = np.asarray(case.mesh.cellsOnEdge, dtype=np.int64).T
+    ops = None
+    for label, o in walk_mixing(case):
+        if label == "mixing":
+            ops = read_all(o, case, MIXING)
+            continue
+        kdiff = read(o, case, "I:kdiff", "cell")
+        rho = ops["rho_zz"]
+        rho_edge = (F32(0.5) * (rho[:, c0] + rho[:, c1])).astype(F32)
+        common = dict(rho_edge=rho_edge, kdiff=kdiff, mesh_scaling_del2=sc.del2, mesh_scaling_del4=sc.del4,
+                      native_stored_inverse=True)
+        mom = momentum_horizontal_filter_tendency(
+            mesh, divergence=ops["divergence"], vorticity=ops["vorticity"], h_mom_eddy_visc4=float(h4),
+            config_del4u_div_factor=float(F32(cfg.config_del4u_div_factor)), initial_tendency=ops["euler_ru"],
+            **common)
+        wf = vertical_momentum_horizontal_filter_tendency(mesh, vertical_velocity=ops["w"],
+                                                          h_mom_eddy_visc4=float(h4), **common)
+        th = theta_horizontal_filter_tendency(mesh, theta_m=ops["theta_m"], h_theta_eddy_visc4=float(h4), **common)
+        for a in (mom.tendency, wf.tendency, th.tendency):
+            assert a.dtype == F32
+        out.update({"mixing.operands": operand_digest(list(ops.values()) + [kdiff]),
+                    "mixing.euler_ru": np.ascontiguousarray(mom.tendency),
+                    "mixing.w_tendency": np.ascontiguousarray(wf.tendency),
+                    "mixing.theta_tendency": np.ascontiguousarray(th.tendency)})
+    return out
+
+
 VECTORS = {
     "solve_diagnostics": solve_diagnostics_vectors,
     "vertical_coefficients": vertical_coefficient_vectors,
@@ -542,6 +636,9 @@ VECTORS = {
     "vertical_transport": vertical_transport_vectors,
     "split_flux": split_flux_vectors,
     "moist_coefficients": moist_coefficient_vectors,
+    "momentum": momentum_vectors,
+    "omega": omega_vectors,
+    "mixing": mixing_vectors,
 }
 
 

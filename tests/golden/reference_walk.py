@@ -126,7 +126,44 @@ TRANSPORT_FLUX = [("uh_avg", "I:split_u", "edge"), ("ww_avg", "I:split_w", "w")]
 
 def shapes(case):
     L, C, E = case.n_levels, case.mesh.n_cells, case.mesh.n_edges
-    return {"cell": (L, C), "edge": (L, E), "w": (L + 1, C), "tracer": (case.n_scalars, L, C)}
+    return {"cell": (L, C), "edge": (L, E), "w": (L + 1, C), "tracer": (case.n_scalars, L, C),
+            "vertex": (L, case.mesh.n_vertices)}
+
+
+def mixing_density(mesh) -> np.ndarray:
+    """A smooth variable mesh density in [1, 4] (a variable-resolution mesh's
+    meshDensity), so the del2/del4 mesh scaling is not the identity."""
+    lat = np.asarray(mesh.latCell, dtype=np.float64)
+    lon = np.asarray(mesh.lonCell, dtype=np.float64)
+    return (1.0 + 1.5 * (1.0 + np.sin(lat) * np.cos(lon))).astype(F32)
+
+
+def build_mixing_case():
+    """The x1.162 moist case at 10 levels with 2-D Smagorinsky mixing
+    (visc4 0.05) and the variable mesh density above."""
+    from hexdyn.case import build_case as build
+    from hexdyn.config import DycoreConfig
+
+    case = build(CASE, nlev=NLEV, use_cache=False,
+                 config=DycoreConfig(config_dt=7200.0, config_horiz_mixing="2d_smagorinsky"))
+    case.mesh.arrays["meshDensity"] = mixing_density(case.mesh)
+    return case
+
+
+def walk_mixing(case):
+    """Yield ("mixing", oracle) after the subcycle-start pools of a developed
+    step and ("mixing_out", oracle) after the horizontal mixing added to them."""
+    from hexdyn.case import initial_state_arrays
+    from oracle.oracle import Oracle
+
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(1) == 0
+    o.begin_step()
+    o.begin_subcycle_pools()
+    yield "mixing", o
+    o.horizontal_mixing()
+    yield "mixing_out", o
 
 
 # what each reference operator reads, as (argument, source, shape); source is a
@@ -155,6 +192,16 @@ RECOVERY_OUT = [("rho_p", "I:c_rho_p", "cell"), ("rho_zz", "I:c_rho", "cell"), (
                 ("theta_m", "I:c_theta", "cell"), ("exner", "I:exner", "cell"), ("pressure_p", "I:pp", "cell"),
                 ("ru_avg", "F:FLUX_U", "edge"), ("ru", "I:c_ru", "edge"), ("u", "I:u", "edge")]
 
+
+# the v8.2.3-association momentum terms (dynamics.py:273-305) and the omega
+# conversion (acoustic.py:246-281) at the stage-2 tendency checkpoint
+MOMENTUM = [("normal_velocity", "I:u", "edge"), ("rho_zz", "I:c_rho", "cell"), ("pv_edge", "F:PV_EDGE", "edge"),
+            ("kinetic_energy", "F:KE", "cell"), ("rho_u", "I:c_ru", "edge")]
+OMEGA = [("w_tendency", "I:euler_rw", "w"), ("u_tendency", "F:TEND_RU", "edge")]
+# the mixing's operands at the subcycle start, and the pools it adds to
+MIXING = [("divergence", "I:divergence", "cell"), ("vorticity", "I:vorticity", "vertex"),
+          ("rho_zz", "I:c_rho", "cell"), ("theta_m", "I:s_theta", "cell"), ("w", "I:w", "w"),
+          ("euler_ru", "I:euler_ru", "edge"), ("euler_rw", "I:euler_rw", "w")]
 
 POOLS = [("rho_u", "I:s_ru", "edge"), ("rho_w", "I:s_rw", "w"), ("pressure_perturbation", "I:pp", "cell"),
          ("dpdz", "I:dpdz", "cell"), ("cqu", "F:CQU", "edge")]

@@ -4,7 +4,6 @@ Strict numerics: every phase and every whole step bit for bit equal to the
 binary32 restatement.  Fast numerics (FMA contraction): within relative L2
 1e-5 of the FP64 restatement on u, w, theta_m and rho_zz after one step.
 """
-import os
 
 import numpy as np
 import pytest
@@ -191,15 +190,16 @@ def test_headline_mesh_step_bitwise_and_conservative(get_case, mixing):
 
 
 @pytest.mark.parametrize("name", ["x1.10242-moist", "x1.163842-qv"])
-def test_other_mesh_sizes_step_bitwise(get_case, name):
+def test_other_mesh_sizes_step_bitwise(get_case, name, capsys, monkeypatch):
     """The ensemble mesh (x1.10242, T=6, dt 1440 s) and the scale-up mesh
     (x1.163842, water vapour only, dt 360 s), with 2-D Smagorinsky mixing:
     one large step bit for bit equal to the oracle, then 4 more steps with
-    dry and tracer masses held in FP64 integrals."""
-    if name == "x1.163842-qv" and not os.environ.get("HEXDYN_LARGE_MESH"):
-        pytest.skip("building x1.163842 takes about 2.5 min of host time; set HEXDYN_LARGE_MESH=1")
+    dry and tracer masses held in FP64 integrals.  Building the x1.163842
+    case takes a minute or two of host time; its stages are printed."""
     print(f"[gpu] {name} 1 step bitwise + 4 steps of budgets, mixing on", flush=True)
-    case = _mixing(get_case, name)
+    monkeypatch.setenv("HEXDYN_VERBOSE", "1")
+    with capsys.disabled():
+        case = _mixing(get_case, name)
     d = _driver(case)
     a = d.integrals()
     d.step_device(1)
@@ -362,6 +362,17 @@ def test_graph_replays_match_eager_steps(get_case):
             assert _bits_equal(g[k], e[k]), f"step {i + 1} {k}: {_diff(g[k], e[k])}"
 
 
+def test_graph_cache_captures_once_per_layout(get_case):
+    """Steps alternate between two buffer layouts: steps 1-2 capture one
+    graph each and steps 3-6 replay them (the cache key is stable)."""
+    print("[gpu] hipGraph cache x1.642-moist", flush=True)
+    d = _driver(get_case("x1.642-moist"))
+    d.step_device(2)
+    assert d.graph_count() == 2
+    d.step_device(4)
+    assert d.graph_count() == 2
+
+
 def test_restart_file_continues_bitwise(get_case, tmp_path):
     print("[gpu] restart file x1.642-moist", flush=True)
     case = get_case("x1.642-moist")
@@ -394,6 +405,9 @@ STEP_VARIANTS = [
     dict(config_monotonic=False),
     dict(config_time_integration_order=2),
     dict(config_horiz_mixing="2d_smagorinsky", config_visc4_2dsmag=0.0),
+    dict(config_rayleigh_damp_u=True, config_number_rayleigh_damp_u_levels=8,
+         config_rayleigh_damp_u_timescale_days=0.5),
+    dict(config_horiz_mixing="2d_smagorinsky", config_mpas_cam_coef=0.2, config_number_cam_damping_levels=4),
 ]
 
 
@@ -547,3 +561,78 @@ def test_phase_by_phase_bitwise_with_physics_tendencies(get_case):
         np.float32)
     bad = _phase_walk(case, physics)
     assert not bad, "\n".join(bad[:20])
+
+
+def test_phase_by_phase_bitwise_nonzero_reference_wind(get_case):
+    """A non-zero u_init/v_init profile: the perturbation-Coriolis term
+    (neighbour angle, target f) is live in every stage's momentum tendency."""
+    print("[gpu] phase walk x1.642-moist with reference wind (12, -7) m/s", flush=True)
+    case = get_case("x1.642-moist", reference_wind=(12.0, -7.0))
+    assert np.any(case.u_init) and np.any(case.v_init)
+    bad = _phase_walk(case)
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_phase_by_phase_bitwise_upper_damping(get_case):
+    """Rayleigh damping of u (8 levels, half a day) and the CAM kdiff floor
+    (4 levels) with 2-D Smagorinsky mixing, phase by phase."""
+    import dataclasses
+
+    print("[gpu] phase walk x1.642-moist with Rayleigh u damping and the CAM floor", flush=True)
+    base = get_case("x1.642-moist")
+    cfg = dataclasses.replace(base.config, config_horiz_mixing="2d_smagorinsky", config_mpas_cam_coef=0.2,
+                              config_number_cam_damping_levels=4, config_rayleigh_damp_u=True,
+                              config_number_rayleigh_damp_u_levels=8, config_rayleigh_damp_u_timescale_days=0.5)
+    bad = _phase_walk(get_case("x1.642-moist", config=cfg))
+    assert not bad, "\n".join(bad[:20])
+
+
+def test_variable_mesh_density_mixing_bitwise(get_case):
+    """A variable meshDensity in [1, 4]: del2/del4 scaled per edge; 2 steps
+    with mixing bit for bit."""
+    import copy
+    import dataclasses
+    import sys
+    from pathlib import Path
+
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    from reference_walk import mixing_density
+
+    print("[gpu] x1.642-moist variable mesh density, mixing, 2 steps", flush=True)
+    base = _mixing(get_case, "x1.642-moist")
+    mesh = copy.deepcopy(base.mesh)
+    mesh.arrays["meshDensity"] = mixing_density(mesh)
+    case = dataclasses.replace(base, mesh=mesh)
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(2) == 0
+    d = _driver(case)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+
+
+def test_passive_tracers_clamped_without_moisture_bitwise(get_case):
+    """n_moist = 0 with six carried species, unlimited transport and negative
+    patches: the end-of-step clamp of every species, 2 steps bit for bit."""
+    import dataclasses
+
+    print("[gpu] x1.642 passive tracers, unlimited transport, 2 steps", flush=True)
+    base = get_case("x1.642-moist")
+    case = dataclasses.replace(base, n_moist=0, config=dataclasses.replace(base.config, config_monotonic=False))
+    st = initial_state_arrays(case)
+    q = st["scalars"].copy()
+    q[1] = (q[1] - np.float32(2e-6)).astype(np.float32)
+    st["scalars"] = q
+    o = Oracle(case, 32)
+    o.set_state(st, rebuild=True)
+    assert o.step(2) == 0
+    from hexdyn.driver import HexDycoreDriver
+
+    d = HexDycoreDriver.from_host(case, state=st)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
+    assert (got["scalars"] >= 0).all()

@@ -24,7 +24,8 @@ sys.path.insert(0, str(GOLDEN))
 
 import reference_walk as rw  # noqa: E402
 
-NAMES = ["solve_diagnostics", "vertical_coefficients", "acoustic", "divergence_damping", "recovery", "advection"]
+NAMES = ["solve_diagnostics", "vertical_coefficients", "acoustic", "divergence_damping", "recovery", "advection",
+         "momentum", "omega"]
 
 
 def _load(name):
@@ -300,4 +301,91 @@ def test_moist_coefficients_match_reference_bits(case):
             _same_bits(rw.read(o, case, "F:CQW", "cell"), z["moist.cqw"], "cqw")
             _same_bits(rw.read(o, case, "F:CQU", "edge"), z["moist.cqu"], "cqu")
             return
+    raise AssertionError("checkpoint not reached")
+
+
+def _walk_checked(case, z, key, table, label):
+    for name, o in rw.walk(case):
+        if name == label:
+            ops = rw.read_all(o, case, table)
+            assert bytes(z[key]).decode() == bytes(rw.operand_digest(ops.values())).decode(), \
+                f"{label}: the walk reached different operands than the vectors were made from"
+            return o, ops
+    raise AssertionError("checkpoint not reached")
+
+
+def test_momentum_terms_match_reference_bits(case):
+    """vector_invariant_momentum_tendency (dynamics.py:273-305) fed the
+    reference's mass_flux_divergence (:86-121): the oracle's momentum terms
+    (the code of every stage's tendency, v8.4.1 form with zero reference
+    wind) and its mass-flux divergence equal the reference word for word."""
+    z = _load("momentum")
+    assert bytes(z["case_digest"]).decode() == bytes(rw.case_digest(case)).decode()
+    o, _ = _walk_checked(case, z, "momentum.operands", rw.MOMENTUM, "vcoef_2")
+    got = o.momentum_terms()
+    _same_bits(o.export_internal("hdiv"), z["momentum.hdiv"], "mass-flux divergence")
+    _same_bits(got, z["momentum.result"], "momentum terms")
+
+
+def test_omega_conversion_matches_reference_bits(case):
+    """convert_w_tendency_to_omega (acoustic.py:246-281): the terrain
+    zb/zb3 correction with the upwind sign of the momentum tendency and the
+    interface zz, on the stage's Euler w pool and momentum tendency."""
+    z = _load("omega")
+    assert bytes(z["case_digest"]).decode() == bytes(rw.case_digest(case)).decode()
+    o, ops = _walk_checked(case, z, "omega.operands", rw.OMEGA, "vcoef_2")
+    _same_bits(o.omega_conversion(ops["w_tendency"], ops["u_tendency"]), z["omega.result"], "omega tendency")
+
+
+def test_omega_vectors_catch_a_flipped_upwind_sign(case):
+    """The same comparison fails when the oracle's upwind sign is planted
+    reversed (the check is not blind to that mistake)."""
+    from oracle.oracle import PLANT_OMEGA_SIGN
+
+    z = _load("omega")
+    o, ops = _walk_checked(case, z, "omega.operands", rw.OMEGA, "vcoef_2")
+    o.set_plant(PLANT_OMEGA_SIGN)
+    got = o.omega_conversion(ops["w_tendency"], ops["u_tendency"])
+    assert np.count_nonzero(got.view(np.uint32) != z["omega.result"].view(np.uint32)) > 0
+
+
+@pytest.fixture(scope="module")
+def mixing_case():
+    return rw.build_mixing_case()
+
+
+def test_mesh_scaling_matches_reference_bits(mixing_case):
+    """compute_mesh_mixing_scaling (mixing.py:271-323) on a variable mesh
+    density: setup.mesh_mixing_scaling (the host evaluation hexdyn_create and
+    the oracle make) gives the same del2/del4 words."""
+    from hexdyn.setup import mesh_mixing_scaling
+
+    z = _load("mixing")
+    assert np.array_equal(z["mixing_density"], mixing_case.mesh.arrays["meshDensity"])
+    del2, del4 = mesh_mixing_scaling(mixing_case.mesh)
+    assert np.count_nonzero(del2 != np.float32(1)) > 0
+    _same_bits(del2, z["scaling.del2"], "meshScalingDel2")
+    _same_bits(del4, z["scaling.del4"], "meshScalingDel4")
+
+
+def test_mixing_filters_match_reference_bits(mixing_case):
+    """The del2/del4 filters of the v8.4.1 mixing on the oracle's kdiff
+    (mixing.py momentum_horizontal_filter_tendency :391-530 onto the u pool,
+    vertical_momentum_horizontal_filter_tendency :533-635 added to the w pool,
+    theta_horizontal_filter_tendency :638-733), variable mesh density,
+    visc4 0.05: every word equal."""
+    z = _load("mixing")
+    assert bytes(z["mixing_case_digest"]).decode() == bytes(rw.case_digest(mixing_case)).decode()
+    ops = None
+    for name, o in rw.walk_mixing(mixing_case):
+        if name == "mixing":
+            ops = rw.read_all(o, mixing_case, rw.MIXING)
+            continue
+        kdiff = rw.read(o, mixing_case, "I:kdiff", "cell")
+        assert bytes(z["mixing.operands"]).decode() == bytes(rw.operand_digest(list(ops.values()) + [kdiff])).decode()
+        _same_bits(rw.read(o, mixing_case, "I:euler_ru", "edge"), z["mixing.euler_ru"], "u pool with del2/del4 u")
+        _same_bits(rw.read(o, mixing_case, "I:euler_theta", "cell"), z["mixing.theta_tendency"], "theta del2/del4")
+        want_w = (ops["euler_rw"] + z["mixing.w_tendency"]).astype(np.float32)
+        _same_bits(rw.read(o, mixing_case, "I:euler_rw", "w"), want_w, "w pool with del2/del4 w")
+        return
     raise AssertionError("checkpoint not reached")
