@@ -363,14 +363,17 @@ def test_graph_replays_match_eager_steps(get_case):
 
 
 def test_graph_cache_captures_once_per_layout(get_case):
-    """Steps alternate between two buffer layouts: steps 1-2 capture one
-    graph each and steps 3-6 replay them (the cache key is stable)."""
+    """The dynamics buffers return to their layout every 2 steps (one
+    publish swap per step) and the scalar buffers every 3 (three stage swaps
+    and a publish swap make a 3-cycle), so the layouts repeat every 6 steps:
+    steps 1-6 capture one graph each and steps 7-12 replay them (the cache
+    key is stable)."""
     print("[gpu] hipGraph cache x1.642-moist", flush=True)
     d = _driver(get_case("x1.642-moist"))
-    d.step_device(2)
-    assert d.graph_count() == 2
-    d.step_device(4)
-    assert d.graph_count() == 2
+    d.step_device(6)
+    assert d.graph_count() == 6
+    d.step_device(6)
+    assert d.graph_count() == 6
 
 
 def test_restart_file_continues_bitwise(get_case, tmp_path):
