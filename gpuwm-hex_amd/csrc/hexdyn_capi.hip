@@ -172,6 +172,7 @@ struct hexdyn_handle {
   std::vector<void *> allocs;
   int *old_c = nullptr, *old_e = nullptr, *old_v = nullptr;  // device: old index of each new entity
   double *int_partial = nullptr, *int_out = nullptr;          // integral partials and results
+  int device = 0;                                             // the HIP device current at create
   // hipGraph-captured large steps, one executable per buffer-role layout at step start
   int use_graphs = 1;
   int post_clamp = 1;  // end-of-step q < 0 -> 0 (0: scalars handed over unclamped)
@@ -347,6 +348,35 @@ float hexdyn_powf(float x, float y) {
 
 namespace {
 thread_local std::string t_create_error;  // why this thread's last hexdyn_create failed
+
+// Scratch of hexdyn_integrals_noalloc: the per-block partial sums, held in a
+// file-scope device array so that entry point allocates nothing.  Each GPU
+// the library runs on holds its own copy (8 MiB) until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is
+// only a stand-in for the symbol): hexdyn_create resolves the array's device
+// address once per device with hipGetSymbolAddress and every later use goes
+// through that address.
+static __device__ __align__(256) double scratch[HEXDYN_NOALLOC_MAX_PARTIALS];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices];
+
+// Resolve this device's scratch address (device `dev` must be current).
+static int resolve_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -1;
+  g_scratch_ptr[dev] = (double *)ptr;
+  return 0;
+}
+
+static double *scratch_ptr(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
 }
 
 int hexdyn_last_create_error(char *buf, size_t len) {
@@ -755,6 +785,8 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
   rc |= h->alloc(&h->int_partial, (size_t)((C + 3) / 4) * (3 + n.NT));
   rc |= h->alloc(&h->int_out, (size_t)(3 + n.NT));
   if (rc) return hip_fail("device allocation or upload failed");
+  if (hipGetDevice(&h->device) != hipSuccess || resolve_scratch(h->device) != 0)
+    return hip_fail("resolving the device scratch of hexdyn_integrals_noalloc failed");
   if (n.NM == 0) {
     std::vector<float> ones(std::max(cr, er), 1.0f);
     if (hipMemcpy(B.cqw, ones.data(), cr * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1285,6 +1317,25 @@ int hexdyn_integrals(hexdyn_handle *h, double *out, int n_out, void *stream) {
   if (hipMemcpyAsync(out, h->int_out, NQ * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return set_error(h, HEXDYN_E_HIP, "integral copy failed");
+  return 0;
+}
+
+int hexdyn_integrals_noalloc(hexdyn_handle *h, double *out_device, int n_out, void *stream) {
+  if (!h || !out_device) return HEXDYN_E_INVALID;
+  const int NQ = 3 + h->I.n.NT;
+  if (n_out < NQ) return set_error(h, HEXDYN_E_INVALID, "hexdyn_integrals_noalloc: n_out < 3 + n_scalars");
+  if (h->in_step) return set_error(h, HEXDYN_E_INVALID, "hexdyn_integrals_noalloc: a step is in progress");
+  const size_t need = (size_t)((h->I.n.C + 3) / 4) * (size_t)NQ;
+  if (need > (size_t)HEXDYN_NOALLOC_MAX_PARTIALS)
+    return set_error(h, HEXDYN_E_INVALID,
+                     "hexdyn_integrals_noalloc: ceil(n_cells/4)*(3+n_scalars) = " + std::to_string(need) +
+                         " partial sums exceed the scratch of " + std::to_string(HEXDYN_NOALLOC_MAX_PARTIALS));
+  double *part = scratch_ptr(h->device);
+  if (!part) return set_error(h, HEXDYN_E_HIP, "hexdyn_integrals_noalloc: device scratch not resolved");
+  const Work &B = h->I.b;
+  if (hexdyn::integrals(h->I, B.rho, B.rt, B.q, B.p_theta, B.p_exner, B.p_u, h->I.z.zmid, part, out_device,
+                        (hipStream_t)stream))
+    return set_error(h, HEXDYN_E_HIP, "integral kernels failed to launch");
   return 0;
 }
 

@@ -67,6 +67,7 @@ def lib() -> ct.CDLL:
     L.hexdyn_powf.argtypes = [ct.c_float, ct.c_float]
     L.hexdyn_powf.restype = ct.c_float
     L.hexdyn_integrals.argtypes = [vp, ct.POINTER(ct.c_double), ct.c_int, vp]
+    L.hexdyn_integrals_noalloc.argtypes = [vp, vp, ct.c_int, vp]
     L.hexdyn_set_graphs.argtypes = [vp, ct.c_int]
     L.hexdyn_set_physics_tendencies.argtypes = [vp, ct.POINTER(capi.HexdynPhysicsView), vp]
     L.hexdyn_set_post_transport_clamp.argtypes = [vp, ct.c_int]
@@ -84,5 +85,5 @@ EXPORTED = [
     "hexdyn_end_subcycle", "hexdyn_scalars", "hexdyn_end_step", "hexdyn_profile_enable", "hexdyn_profile_read",
     "hexdyn_profile_kernels", "hexdyn_powf", "hexdyn_integrals", "hexdyn_set_graphs",
     "hexdyn_set_physics_tendencies", "hexdyn_set_post_transport_clamp", "hexdyn_last_create_error",
-    "hexdyn_graph_count",
+    "hexdyn_graph_count", "hexdyn_integrals_noalloc",
 ]

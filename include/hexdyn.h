@@ -276,6 +276,22 @@ int hexdyn_profile_kernels(hexdyn_handle *h, char *buf, size_t len);
  * with CuPy, cuda_fullstep_lts_active.py:1337-1360.) */
 int hexdyn_integrals(hexdyn_handle *h, double *out, int n_out, void *stream);
 
+/* hexdyn_integrals without allocation, synchronisation or host copy: the
+ * same kernels and reduction order (bitwise the same values), the per-block
+ * partial sums held in a file-scope device array of
+ * HEXDYN_NOALLOC_MAX_PARTIALS doubles (8 MiB per GPU, kept until the process
+ * ends) instead of the handle's workspace, and the n_out >= 3 + n_scalars
+ * results written to the caller's DEVICE memory `out_device` on `stream`.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Refuses meshes with ceil(n_cells/4)*(3+n_scalars) above the
+ * limit (HEXDYN_E_INVALID).  Every call on one GPU shares that GPU's array
+ * (it keeps the last call's partial sums; each call writes every partial it
+ * reads, so nothing is cleared): calls that overlap on one GPU (any handles,
+ * any streams) race. */
+#define HEXDYN_NOALLOC_MAX_PARTIALS (1 << 20)
+int hexdyn_integrals_noalloc(hexdyn_handle *h, double *out_device, int n_out, void *stream);
+
 /* The exner power function of recovery (cuda_driver.py:1466-1472: the GNU
  * powf), host copy, exposed for its tests: for y = R/cv and x in [2^-12, 4)
  * the GNU C library's bits, elsewhere the correctly rounded binary64 method.
