@@ -173,6 +173,7 @@ struct hexdyn_handle {
   int *old_c = nullptr, *old_e = nullptr, *old_v = nullptr;  // device: old index of each new entity
   double *int_partial = nullptr, *int_out = nullptr;          // integral partials and results
   int device = 0;                                             // the HIP device current at create
+  LaunchTune tune;                                            // gather kernels' wavefronts per block
   // hipGraph-captured large steps, one executable per buffer-role layout at step start
   int use_graphs = 1;
   int post_clamp = 1;  // end-of-step q < 0 -> 0 (0: scalars handed over unclamped)
@@ -415,6 +416,7 @@ int hexdyn_create(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_
 
 static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const hexdyn_config *c, hexdyn_handle **out) {
   hexdyn_handle *h = new hexdyn_handle();
+  h->I.tune = &h->tune;
   h->cfg = *c;
   Sizes &n = h->I.n;
   n.C = m->n_cells; n.E = m->n_edges; n.V = m->n_vertices; n.L = vt->n_levels;
@@ -1297,6 +1299,48 @@ int hexdyn_set_graphs(hexdyn_handle *h, int enable) {
   return 0;
 }
 
+namespace {
+void drop_graphs(hexdyn_handle *h) {
+  (void)hipDeviceSynchronize();
+  for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second.first);
+  h->graphs.clear();
+}
+// the column kernels' block shape is fixed (16 columns through LDS)
+bool column_kernel(const char *name) {
+  return std::strcmp(name, "k_vcoef") == 0 || std::strcmp(name, "k_ac_cell") == 0;
+}
+}  // namespace
+
+int hexdyn_set_launch(hexdyn_handle *h, const char *kernel, int waves_per_block) {
+  if (!h || !kernel) return HEXDYN_E_INVALID;
+  if (h->in_step) return set_error(h, HEXDYN_E_INVALID, "hexdyn_set_launch inside a step");
+  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
+    return set_error(h, HEXDYN_E_INVALID, "waves_per_block must be 1, 2 or 4");
+  LaunchTune &t = h->tune;
+  if (std::strcmp(kernel, "*") == 0) {
+    t.n = 0;
+    t.def = waves_per_block;
+  } else {
+    if (column_kernel(kernel) || std::strlen(kernel) >= 32 || std::strncmp(kernel, "k_", 2) != 0)
+      return set_error(h, HEXDYN_E_INVALID, std::string("not a tunable gather kernel: ") + kernel);
+    int i = 0;
+    while (i < t.n && std::strncmp(t.names[i], kernel, 31) != 0) ++i;
+    if (i == t.n) {
+      if (t.n == LaunchTune::kMax) return set_error(h, HEXDYN_E_INVALID, "launch table full");
+      std::snprintf(t.names[i], sizeof t.names[i], "%s", kernel);
+      ++t.n;
+    }
+    t.wpb[i] = waves_per_block;
+  }
+  drop_graphs(h);
+  return 0;
+}
+
+int hexdyn_get_launch(const hexdyn_handle *h, const char *kernel) {
+  if (!h || !kernel) return -1;
+  return h->tune.waves_per_block(kernel);
+}
+
 int hexdyn_set_post_transport_clamp(hexdyn_handle *h, int enable) {
   if (!h) return HEXDYN_E_INVALID;
   if (h->in_step) return set_error(h, HEXDYN_E_INVALID, "hexdyn_set_post_transport_clamp inside a step");
@@ -1336,6 +1380,192 @@ int hexdyn_integrals_noalloc(hexdyn_handle *h, double *out_device, int n_out, vo
   if (hexdyn::integrals(h->I, B.rho, B.rt, B.q, B.p_theta, B.p_exner, B.p_u, h->I.z.zmid, part, out_device,
                         (hipStream_t)stream))
     return set_error(h, HEXDYN_E_HIP, "integral kernels failed to launch");
+  return 0;
+}
+
+// ---------------------------------------------------------------- state copy
+namespace {
+bool same_mesh(const hexdyn_handle *a, const hexdyn_handle *b) {
+  const Sizes &x = a->I.n, &y = b->I.n;
+  return x.C == y.C && x.E == y.E && x.V == y.V && x.L == y.L && x.ME == y.ME && x.NT == y.NT &&
+         a->old_c_h == b->old_c_h && a->old_e_h == b->old_e_h && a->old_v_h == b->old_v_h;
+}
+}  // namespace
+
+int hexdyn_copy_state(hexdyn_handle *dst, hexdyn_handle *src, void *stream) {
+  if (!dst || !src) return HEXDYN_E_INVALID;
+  if (dst == src) return 0;
+  if (dst->in_step || src->in_step) return set_error(dst, HEXDYN_E_INVALID, "hexdyn_copy_state inside a step");
+  if (!same_mesh(dst, src)) return set_error(dst, HEXDYN_E_INVALID, "hexdyn_copy_state: handles of different meshes");
+  hipStream_t s = (hipStream_t)stream;
+  const Sizes &n = src->I.n;
+  const Work &S = src->I.b;
+  Work &D = dst->I.b;
+  int rc = 0;
+  rc |= copy_rows(D.rho, S.rho, n.C, s); rc |= copy_rows(D.rt, S.rt, n.C, s);
+  rc |= copy_rows(D.ru, S.ru, n.E, s); rc |= copy_rows(D.rw, S.rw, n.C, s);
+  if (n.NT > 0) rc |= copy_rows(D.q, S.q, (size_t)n.NT * n.C, s);
+  rc |= copy_rows(D.p_theta, S.p_theta, n.C, s); rc |= copy_rows(D.p_exner, S.p_exner, n.C, s);
+  rc |= copy_rows(D.p_rho_p, S.p_rho_p, n.C, s); rc |= copy_rows(D.p_rtheta_p, S.p_rtheta_p, n.C, s);
+  rc |= copy_rows(D.p_pp, S.p_pp, n.C, s); rc |= copy_rows(D.p_u, S.p_u, n.E, s);
+  rc |= copy_rows(D.p_w, S.p_w, n.C, s);
+  if (rc) return set_error(dst, HEXDYN_E_HIP, "hexdyn_copy_state: copy failed");
+  dst->time_s = src->time_s;
+  return 0;
+}
+
+// ---------------------------------------------------------------- corrector
+struct hexdyn_net {
+  hexdyn::net::Net *impl = nullptr;
+  hexdyn_handle *h = nullptr;  // the mesh the network was created on
+  int layers = 0;
+  std::string error;
+};
+
+namespace {
+int net_error(hexdyn_net *n, int rc, const std::string &msg) {
+  n->error = msg;
+  return rc;
+}
+int net_mesh_ok(hexdyn_net *n, hexdyn_handle *h) {
+  if (!h) return net_error(n, HEXDYN_E_INVALID, "null handle");
+  if (h->in_step) return net_error(n, HEXDYN_E_INVALID, "a step is in progress on the handle");
+  if (h != n->h && !same_mesh(h, n->h)) return net_error(n, HEXDYN_E_INVALID, "handle of another mesh");
+  return 0;
+}
+}  // namespace
+
+int hexdyn_net_create(hexdyn_handle *h, int layers, hexdyn_net **out) {
+  if (!h || !out || layers < 1 || layers > 8) return HEXDYN_E_INVALID;
+  *out = nullptr;
+  std::string err;
+  hexdyn::net::Net *impl = hexdyn::net::create(h->I, layers, &err);
+  if (!impl) return set_error(h, HEXDYN_E_HIP, err);
+  hexdyn_net *n = new hexdyn_net();
+  n->impl = impl;
+  n->h = h;
+  n->layers = layers;
+  *out = n;
+  return 0;
+}
+
+void hexdyn_net_destroy(hexdyn_net *net) {
+  if (!net) return;
+  (void)hipDeviceSynchronize();
+  hexdyn::net::destroy(net->impl);
+  delete net;
+}
+
+int64_t hexdyn_net_param_count(const hexdyn_net *net) {
+  return net ? (int64_t)hexdyn::net::param_count(net->impl) : 0;
+}
+
+int hexdyn_net_set_params(hexdyn_net *net, const float *host, int64_t n, void *stream) {
+  if (!net || !host) return HEXDYN_E_INVALID;
+  if (n != (int64_t)hexdyn::net::param_count(net->impl)) return net_error(net, HEXDYN_E_INVALID, "parameter count");
+  return hexdyn::net::set_params(*net->impl, host, (hipStream_t)stream) ? net_error(net, HEXDYN_E_HIP, "upload") : 0;
+}
+
+int hexdyn_net_get_params(hexdyn_net *net, float *host, int64_t n, void *stream) {
+  if (!net || !host) return HEXDYN_E_INVALID;
+  if (n != (int64_t)hexdyn::net::param_count(net->impl)) return net_error(net, HEXDYN_E_INVALID, "parameter count");
+  return hexdyn::net::get_vec(*net->impl, 0, host, (hipStream_t)stream) ? net_error(net, HEXDYN_E_HIP, "download") : 0;
+}
+
+int hexdyn_net_get_grads(hexdyn_net *net, float *host, int64_t n, void *stream) {
+  if (!net || !host) return HEXDYN_E_INVALID;
+  if (n != (int64_t)hexdyn::net::param_count(net->impl)) return net_error(net, HEXDYN_E_INVALID, "parameter count");
+  return hexdyn::net::get_vec(*net->impl, 1, host, (hipStream_t)stream) ? net_error(net, HEXDYN_E_HIP, "download") : 0;
+}
+
+int hexdyn_net_set_scales(hexdyn_net *net, const float *in_mean, const float *in_scale, const float *out_scale,
+                          void *stream) {
+  if (!net || !in_mean || !in_scale || !out_scale) return HEXDYN_E_INVALID;
+  return hexdyn::net::set_scales(*net->impl, in_mean, in_scale, out_scale, (hipStream_t)stream)
+             ? net_error(net, HEXDYN_E_HIP, "upload")
+             : 0;
+}
+
+int hexdyn_net_forward(hexdyn_net *net, hexdyn_handle *h, int keep, void *stream) {
+  if (!net) return HEXDYN_E_INVALID;
+  if (int rc = net_mesh_ok(net, h)) return rc;
+  return hexdyn::net::forward(*net->impl, h->I, keep != 0, (hipStream_t)stream)
+             ? net_error(net, HEXDYN_E_HIP, "forward kernels failed to launch")
+             : 0;
+}
+
+int hexdyn_net_set_target(hexdyn_net *net, hexdyn_handle *coarse, hexdyn_handle *fine, void *stream) {
+  if (!net) return HEXDYN_E_INVALID;
+  if (int rc = net_mesh_ok(net, coarse)) return rc;
+  if (int rc = net_mesh_ok(net, fine)) return rc;
+  return hexdyn::net::target(*net->impl, coarse->I, fine->I, (hipStream_t)stream)
+             ? net_error(net, HEXDYN_E_HIP, "target kernels failed to launch")
+             : 0;
+}
+
+int hexdyn_net_loss(hexdyn_net *net, double *out5, void *stream) {
+  if (!net || !out5) return HEXDYN_E_INVALID;
+  return hexdyn::net::loss(*net->impl, net->h->I, out5, (hipStream_t)stream)
+             ? net_error(net, HEXDYN_E_HIP, "loss kernels failed")
+             : 0;
+}
+
+int hexdyn_net_backward(hexdyn_net *net, void *stream) {
+  if (!net) return HEXDYN_E_INVALID;
+  return hexdyn::net::backward(*net->impl, net->h->I, (hipStream_t)stream)
+             ? net_error(net, HEXDYN_E_HIP, "backward kernels failed to launch")
+             : 0;
+}
+
+int hexdyn_net_adam(hexdyn_net *net, float lr, float beta1, float beta2, float eps, int step, void *stream) {
+  if (!net || step < 1) return HEXDYN_E_INVALID;
+  return hexdyn::net::adam(*net->impl, lr, beta1, beta2, eps, step, (hipStream_t)stream)
+             ? net_error(net, HEXDYN_E_HIP, "adam kernels failed to launch")
+             : 0;
+}
+
+int hexdyn_net_apply(hexdyn_net *net, hexdyn_handle *h, float gain, void *stream) {
+  if (!net) return HEXDYN_E_INVALID;
+  if (int rc = net_mesh_ok(net, h)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (hexdyn::net::apply(*net->impl, h->I, gain, s)) return net_error(net, HEXDYN_E_HIP, "apply kernels failed");
+  if (NS_CALL(h, rebuild_saved, h->I, s)) return net_error(net, HEXDYN_E_HIP, "diagnostic rebuild failed");
+  Work &B = h->I.b;
+  const Sizes &n = h->I.n;
+  int rc = 0;
+  rc |= copy_rows(B.p_theta, B.theta, n.C, s); rc |= copy_rows(B.p_exner, B.exner, n.C, s);
+  rc |= copy_rows(B.p_rho_p, B.rho_p, n.C, s); rc |= copy_rows(B.p_rtheta_p, B.rtheta_p, n.C, s);
+  rc |= copy_rows(B.p_pp, B.pp, n.C, s); rc |= copy_rows(B.p_u, B.u, n.E, s);
+  rc |= copy_rows(B.p_w, B.w, n.C, s);
+  return rc ? net_error(net, HEXDYN_E_HIP, "diagnostic copy failed") : 0;
+}
+
+int hexdyn_net_checks(hexdyn_net *net, hexdyn_handle *h, float *out3, void *stream) {
+  if (!net || !out3) return HEXDYN_E_INVALID;
+  if (int rc = net_mesh_ok(net, h)) return rc;
+  return hexdyn::net::checks(*net->impl, h->I, (float)h->cfg.config_dt, out3, (hipStream_t)stream)
+             ? net_error(net, HEXDYN_E_HIP, "check kernels failed")
+             : 0;
+}
+
+int hexdyn_net_features(hexdyn_net *net, hexdyn_handle *h, float *dst, void *stream) {
+  if (!net || !dst) return HEXDYN_E_INVALID;
+  if (int rc = net_mesh_ok(net, h)) return rc;
+  return hexdyn::net::raw_features(*net->impl, h->I, h->old_c, dst, (hipStream_t)stream)
+             ? net_error(net, HEXDYN_E_HIP, "feature kernels failed")
+             : 0;
+}
+
+int hexdyn_net_export(hexdyn_net *net, int what, float *dst, void *stream) {
+  if (!net || !dst) return HEXDYN_E_INVALID;
+  const int rc = hexdyn::net::export_buf(*net->impl, what, net->h->old_c, net->h->old_e, dst, (hipStream_t)stream);
+  if (rc > 0) return net_error(net, HEXDYN_E_INVALID, "unknown export");
+  return rc ? net_error(net, HEXDYN_E_HIP, "export failed") : 0;
+}
+
+int hexdyn_net_last_error(const hexdyn_net *net, char *buf, size_t len) {
+  if (!net || !buf || len == 0) return HEXDYN_E_INVALID;
+  std::snprintf(buf, len, "%s", net->error.c_str());
   return 0;
 }
 

@@ -13,6 +13,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
+#include <string>
+
 #define HX_ROW 64
 
 namespace hexdyn {
@@ -147,12 +150,28 @@ static_assert(sizeof(Consts) == sizeof(double) + 46 * 4, "Consts must have no pa
 void prof_mark(const char *name, hipStream_t s, int start);
 extern int g_prof_on;
 
+// Launch settings of the gather kernels (host side; kernels never read it):
+// wavefronts (entities) per block, 1, 2 or 4, by kernel name.
+struct LaunchTune {
+  static constexpr int kMax = 64;
+  int n = 0;
+  int def = 4;  // kernels not listed
+  char names[kMax][32];
+  int wpb[kMax];
+  int waves_per_block(const char *name) const {
+    for (int i = 0; i < n; ++i)
+      if (std::strncmp(names[i], name, 31) == 0) return wpb[i];
+    return def;
+  }
+};
+
 struct Instance {
   Sizes n;
   Mesh m;
   Vert z;
   Work b;
   Consts k;
+  const LaunchTune *tune;  // host pointer to the handle's launch table
 };
 
 // Phase entry points, compiled once with strict binary32 numerics
@@ -178,6 +197,28 @@ HEXDYN_PHASES(fast)
 int integrals(const Instance &I, const float *rho, const float *rt, const float *q, const float *theta,
               const float *exner, const float *u, const float *zmid, double *partial, double *out,
               hipStream_t s);
+
+// Learned large-step corrector (hexdyn_net.hip).  Every entry point works on
+// the published state rows of an Instance; the mesh connectivity must be the
+// one the network was created with.
+namespace net {
+struct Net;
+Net *create(const Instance &I, int layers, std::string *err);
+void destroy(Net *n);
+size_t param_count(const Net *n);
+int set_params(Net &n, const float *host, hipStream_t s);
+int get_vec(const Net &n, int which, float *host, hipStream_t s);  // 0 params, 1 grads
+int set_scales(Net &n, const float *in_mean, const float *in_scale, const float *out_scale, hipStream_t s);
+int forward(Net &n, const Instance &I, bool keep, hipStream_t s);
+int target(Net &n, const Instance &Ic, const Instance &If, hipStream_t s);
+int loss(Net &n, const Instance &I, double *out5, hipStream_t s);
+int backward(Net &n, const Instance &I, hipStream_t s);
+int adam(Net &n, float lr, float b1, float b2, float eps, int t, hipStream_t s);
+int apply(Net &n, const Instance &I, float gain, hipStream_t s);
+int checks(Net &n, const Instance &I, float dt, float out[3], hipStream_t s);
+int raw_features(Net &n, const Instance &I, const int *old_c, float *dst, hipStream_t s);
+int export_buf(Net &n, int what, const int *old_c, const int *old_e, float *dst, hipStream_t s);
+}  // namespace net
 
 }  // namespace hexdyn
 

@@ -51,16 +51,18 @@ __device__ __forceinline__ uint32_t row_off(int ent, int k) {
 #define KB __global__ void __launch_bounds__(256)
 
 // First entity of this block: blocks are renumbered so that XCD x (hardware
-// block b runs on XCD b % 8) owns a contiguous range of the curve.
+// block b runs on XCD b % 8) owns a contiguous range of the curve.  WPB is
+// the number of wavefronts (entities) per block; WPB = 0 takes it from the
+// launch (the gather kernels' block size is a per-kernel launch setting).
 template <int WPB>
 __device__ __forceinline__ int block_entity0() {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
   const int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
   const int lb = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-  return lb * WPB;
+  return lb * (WPB > 0 ? WPB : (int)(blockDim.x >> 6));
 }
-template <int WPB = 4>
+template <int WPB = 0>
 __device__ __forceinline__ int wave_entity() {
   return __builtin_amdgcn_readfirstlane(block_entity0<WPB>() + (int)(threadIdx.x >> 6));
 }
@@ -119,7 +121,7 @@ __device__ __forceinline__ void slots(T (&out)[N], const T *__restrict__ p) {
   for (int s = 0; s < N; ++s) out[s] = p[s];
 }
 
-inline int nblocks(int n) { return (n + 3) / 4; }
+inline int nblocks(int n, int wpb) { return (n + wpb - 1) / wpb; }
 
 // Column kernels: COLW cells (one wavefront each) per block.  The serial
 // vertical recurrences of the block's cells run together on the lanes of
@@ -1803,11 +1805,24 @@ KB k_clamp(const Instance I, const float *src, float *dst, int rows) {
   AT(dst, r, k) = x < 0.0f ? 0.0f : x;
 }
 
+// Wavefronts per block of a gather kernel: the handle's launch table when the
+// first kernel argument is the Instance, else 4.
+template <class T>
+inline int wpb_from(const char *, const T &) {
+  return 4;
+}
+inline int wpb_from(const char *name, const Instance &I) { return I.tune ? I.tune->waves_per_block(name) : 4; }
+template <class A0, class... A>
+inline int launch_wpb(const char *name, const A0 &a0, const A &...) {
+  return wpb_from(name, a0);
+}
+
 template <typename K, typename... A>
 inline void launch_named(const char *name, K kernel, int n, int gy, hipStream_t s, A... args) {
   if (n <= 0 || gy <= 0) return;
+  const int wpb = launch_wpb(name, args...);
   if (g_prof_on) prof_mark(name, s, 1);
-  hipLaunchKernelGGL(kernel, dim3(nblocks(n), gy), dim3(256), 0, s, args...);
+  hipLaunchKernelGGL(kernel, dim3(nblocks(n, wpb), gy), dim3(64 * wpb), 0, s, args...);
   if (g_prof_on) prof_mark(name, s, 0);
 }
 #define launch(kernel, ...) launch_named(#kernel, kernel, __VA_ARGS__)

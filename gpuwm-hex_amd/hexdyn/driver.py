@@ -72,6 +72,7 @@ class HexDycoreDriver:
         self._h = ct.c_void_p(handle)
         self._bundle = bundle
         self.shapes = self._state_shapes(case)
+        self.tuning = None
 
     @staticmethod
     def _state_shapes(case) -> Dict[str, tuple]:
@@ -98,11 +99,32 @@ class HexDycoreDriver:
                 raise ConfigurationRefusal("hexdyn_create", rc, why, "a supported mesh and configuration")
             raise RuntimeError(f"hexdyn_create failed ({rc}): {why}")
         drv = cls(case, h.value, bundle)
+        try:  # a stored launch tuning of this device and mesh (hexdyn.autotune)
+            from .autotune import apply_saved
+
+            drv.tuning = apply_saved(drv)
+        except (OSError, ValueError):
+            drv.tuning = None
         if state is None:
             s = case.state
             state = {"rho_zz": s.rho_zz, "rtheta_m": s.rtheta_m, "ru": s.ru, "rw": s.rw, "scalars": s.scalars}
         drv.set_state(state, saved)
         return drv
+
+    @classmethod
+    def from_reference(cls, mesh, state, vertical, reference, config, *, saved_diagnostics=None,
+                       terrain_metrics, advection_coefficients=None,
+                       reference_wind_profiles=None) -> "HexDycoreDriver":
+        """Drop-in for the reference's CudaDryDycoreDriver.from_host
+        (cuda_driver.py:2826-2842): the same arguments, read by attribute name
+        (hexdyn.reference_types), uploaded through hexdyn_create."""
+        from .reference_types import case_from_reference
+
+        case, st, saved = case_from_reference(
+            mesh, state, vertical, reference, config, saved_diagnostics=saved_diagnostics,
+            terrain_metrics=terrain_metrics, advection_coefficients=advection_coefficients,
+            reference_wind_profiles=reference_wind_profiles)
+        return cls.from_host(case, state=st, saved=saved)
 
     def close(self) -> None:
         if self._h:
@@ -244,6 +266,10 @@ class HexDycoreDriver:
     def use_graphs(self, enable: bool) -> None:
         """Step through captured hipGraphs (default) or kernel by kernel."""
         _check(self._h, lib().hexdyn_set_graphs(self._h, 1 if enable else 0), "hexdyn_set_graphs")
+
+    def model_time(self) -> float:
+        """Model time of the published state in seconds (hexdyn_model_time)."""
+        return float(lib().hexdyn_model_time(self._h))
 
     def integrals(self) -> Dict[str, float]:
         """FP64 global integrals of the published state (hexdyn_integrals)."""

@@ -68,6 +68,29 @@ def lib() -> ct.CDLL:
     L.hexdyn_powf.restype = ct.c_float
     L.hexdyn_integrals.argtypes = [vp, ct.POINTER(ct.c_double), ct.c_int, vp]
     L.hexdyn_integrals_noalloc.argtypes = [vp, vp, ct.c_int, vp]
+    L.hexdyn_copy_state.argtypes = [vp, vp, vp]
+    L.hexdyn_set_launch.argtypes = [vp, ct.c_char_p, ct.c_int]
+    L.hexdyn_get_launch.argtypes = [vp, ct.c_char_p]
+    # learned corrector
+    f32p = ct.POINTER(ct.c_float)
+    L.hexdyn_net_create.argtypes = [vp, ct.c_int, ct.POINTER(vp)]
+    L.hexdyn_net_destroy.argtypes = [vp]
+    L.hexdyn_net_destroy.restype = None
+    L.hexdyn_net_param_count.argtypes = [vp]
+    L.hexdyn_net_param_count.restype = ct.c_int64
+    for name in ("hexdyn_net_set_params", "hexdyn_net_get_params", "hexdyn_net_get_grads"):
+        getattr(L, name).argtypes = [vp, f32p, ct.c_int64, vp]
+    L.hexdyn_net_set_scales.argtypes = [vp, f32p, f32p, f32p, vp]
+    L.hexdyn_net_forward.argtypes = [vp, vp, ct.c_int, vp]
+    L.hexdyn_net_set_target.argtypes = [vp, vp, vp, vp]
+    L.hexdyn_net_loss.argtypes = [vp, ct.POINTER(ct.c_double), vp]
+    L.hexdyn_net_backward.argtypes = [vp, vp]
+    L.hexdyn_net_adam.argtypes = [vp, ct.c_float, ct.c_float, ct.c_float, ct.c_float, ct.c_int, vp]
+    L.hexdyn_net_apply.argtypes = [vp, vp, ct.c_float, vp]
+    L.hexdyn_net_checks.argtypes = [vp, vp, f32p, vp]
+    L.hexdyn_net_features.argtypes = [vp, vp, vp, vp]
+    L.hexdyn_net_export.argtypes = [vp, ct.c_int, vp, vp]
+    L.hexdyn_net_last_error.argtypes = [vp, ct.c_char_p, ct.c_size_t]
     L.hexdyn_set_graphs.argtypes = [vp, ct.c_int]
     L.hexdyn_set_physics_tendencies.argtypes = [vp, ct.POINTER(capi.HexdynPhysicsView), vp]
     L.hexdyn_set_post_transport_clamp.argtypes = [vp, ct.c_int]
@@ -85,5 +108,10 @@ EXPORTED = [
     "hexdyn_end_subcycle", "hexdyn_scalars", "hexdyn_end_step", "hexdyn_profile_enable", "hexdyn_profile_read",
     "hexdyn_profile_kernels", "hexdyn_powf", "hexdyn_integrals", "hexdyn_set_graphs",
     "hexdyn_set_physics_tendencies", "hexdyn_set_post_transport_clamp", "hexdyn_last_create_error",
-    "hexdyn_graph_count", "hexdyn_integrals_noalloc",
+    "hexdyn_graph_count", "hexdyn_integrals_noalloc", "hexdyn_copy_state", "hexdyn_set_launch",
+    "hexdyn_get_launch",
+    "hexdyn_net_create", "hexdyn_net_destroy", "hexdyn_net_param_count", "hexdyn_net_set_params",
+    "hexdyn_net_get_params", "hexdyn_net_get_grads", "hexdyn_net_set_scales", "hexdyn_net_forward",
+    "hexdyn_net_set_target", "hexdyn_net_loss", "hexdyn_net_backward", "hexdyn_net_adam", "hexdyn_net_apply",
+    "hexdyn_net_checks", "hexdyn_net_features", "hexdyn_net_export", "hexdyn_net_last_error",
 ]

@@ -236,6 +236,17 @@ int hexdyn_set_graphs(hexdyn_handle *h, int enable);
 /* Number of captured step graphs the handle holds (one per buffer layout). */
 int hexdyn_graph_count(const hexdyn_handle *h);
 
+/* Launch shape of the gather kernels (one wavefront per cell, edge or
+ * vertex row): wavefronts per block, 1, 2 or 4 (the default), by kernel name
+ * ("k_diag_edge_flux", ...; "*" resets every kernel to the given value).
+ * The column kernels (k_vcoef, k_ac_cell: 16 columns per block through LDS)
+ * are refused.  Results do not depend on it.  Drops the captured step graphs.
+ * The autotuner (hexdyn/autotune.py) picks it per kernel by timing.  (The
+ * reference tunes its level chunking instead, cuda_edge_hoist.py:63-101.) */
+int hexdyn_set_launch(hexdyn_handle *h, const char *kernel, int waves_per_block);
+/* the waves per block a kernel is launched with (-1 for a null argument) */
+int hexdyn_get_launch(const hexdyn_handle *h, const char *kernel);
+
 /* Install (view != NULL) or clear (view == NULL) held physics tendencies. */
 int hexdyn_set_physics_tendencies(hexdyn_handle *h, const hexdyn_physics_view *view, void *stream);
 
@@ -275,6 +286,82 @@ int hexdyn_profile_kernels(hexdyn_handle *h, char *buf, size_t len);
  * (No reference entry point: the reference sums rho*(area*dzw) in float64
  * with CuPy, cuda_fullstep_lts_active.py:1337-1360.) */
 int hexdyn_integrals(hexdyn_handle *h, double *out, int n_out, void *stream);
+
+/* Copy the published state (prognostic fields, carried diagnostics, model
+ * time) of one handle into another built from the same mesh, vertical grid
+ * and scalar count (the same internal numbering), device to device on
+ * `stream`.  Handles with different timesteps may exchange states this way:
+ * the learned corrector's coarse and fine handles do.  No reference
+ * counterpart (the reference re-uploads a host state, cuda_driver.py:2826). */
+int hexdyn_copy_state(hexdyn_handle *dst, hexdyn_handle *src, void *stream);
+
+/* ---- Learned large-step corrector.  No reference counterpart: the
+ * reference dycore has no learned component.  A graph network on the mesh
+ * (message passing over cellsOnCell, edge outputs over cellsOnEdge, cell
+ * inputs gathered over edgesOnCell) maps the published state after one
+ * coarse large step (doubled dt, fewer acoustic substeps) to the correction
+ * that brings it onto the fine-dt trajectory: corrections of u (edges),
+ * w, theta_m and rho_zz (cells).  bf16 MFMA GEMMs with fp32 accumulation,
+ * hand-written forward, backward and Adam kernels, fp32 master weights.
+ *
+ * Parameters, flat fp32 in this order (row-major [in][out] matrices):
+ *   W_in [384][128], b_in [128], then per layer W_l [256][128], b_l [128],
+ *   then W_out [128][320], b_out [320].
+ * Inputs per cell: 6 channels x 64 rows (east and north cell wind from a
+ * least-squares fit of the edge normal winds, w, theta_m, rho_zz, qv; the
+ * Coriolis parameter in row n_levels of the east-wind channel), normalised
+ * as (raw - in_mean) * in_scale.  Outputs per cell: 5 channels x 64 rows
+ * (east wind, north wind, theta_m, rho_zz, w corrections) in units of
+ * out_scale; the east and north channels share one scale per row.
+ *
+ * Calls on one network must not overlap; every entry point enqueues on
+ * `stream`, and those that return host values synchronise it. */
+typedef struct hexdyn_net hexdyn_net;
+#define HEXDYN_NET_HIDDEN 128
+#define HEXDYN_NET_INPUTS 384
+#define HEXDYN_NET_OUTPUTS 320
+enum { HEXDYN_NET_X = 0, HEXDYN_NET_Y = 1, HEXDYN_NET_TARGET_CELLS = 2, HEXDYN_NET_TARGET_EDGES = 3 };
+
+/* A network of `layers` (1..8) message-passing layers bound to the mesh of
+ * handle h (weights zero until set). */
+int hexdyn_net_create(hexdyn_handle *h, int layers, hexdyn_net **out);
+void hexdyn_net_destroy(hexdyn_net *net);
+int64_t hexdyn_net_param_count(const hexdyn_net *net);
+/* fp32 master weights from the host; resets the Adam moments */
+int hexdyn_net_set_params(hexdyn_net *net, const float *host, int64_t n, void *stream);
+int hexdyn_net_get_params(hexdyn_net *net, float *host, int64_t n, void *stream);
+int hexdyn_net_get_grads(hexdyn_net *net, float *host, int64_t n, void *stream);
+/* in_mean, in_scale [6][64]; out_scale [5][64] (host) */
+int hexdyn_net_set_scales(hexdyn_net *net, const float *in_mean, const float *in_scale, const float *out_scale,
+                          void *stream);
+/* features of h's published state and the forward pass; keep_activations
+ * = 1 keeps what hexdyn_net_backward needs */
+int hexdyn_net_forward(hexdyn_net *net, hexdyn_handle *h, int keep_activations, void *stream);
+/* training target: (fine - coarse) of the two handles' published u, w,
+ * theta_m, rho_zz, in units of out_scale */
+int hexdyn_net_set_target(hexdyn_net *net, hexdyn_handle *coarse, hexdyn_handle *fine, void *stream);
+/* loss of the last forward against the target: out5 = {total, u, theta_m,
+ * rho_zz, w} mean squared normalised errors; also forms the output gradient */
+int hexdyn_net_loss(hexdyn_net *net, double *out5, void *stream);
+int hexdyn_net_backward(hexdyn_net *net, void *stream);
+/* one Adam step (step >= 1 counts from 1) on the gradients of the last backward */
+int hexdyn_net_adam(hexdyn_net *net, float lr, float beta1, float beta2, float eps, int step, void *stream);
+/* add gain x the last forward's correction to h's published state: the
+ * density correction is shifted so that the dry mass sum area*dzw*drho is
+ * zero (binary64 sums), rho*theta is formed from the corrected rho and
+ * theta_m, ru from the corrected u and edge density, rw from the corrected
+ * w; then the carried diagnostics are recomputed from the prognostics. */
+int hexdyn_net_apply(hexdyn_net *net, hexdyn_handle *h, float gain, void *stream);
+/* per-step checks of h's published state: out3 = {max |u| dt/dcEdge,
+ * max |w| dt rdzw/zz, max |divergence| dt} (NaN if any value is NaN) */
+int hexdyn_net_checks(hexdyn_net *net, hexdyn_handle *h, float *out3, void *stream);
+/* raw (unnormalised) features of h's published state, [C][384] in the
+ * caller's cell order, device memory */
+int hexdyn_net_features(hexdyn_net *net, hexdyn_handle *h, float *dst_device, void *stream);
+/* export X [C][384], Y [C][320], target cells [C][3][64] (theta_m, rho_zz,
+ * w) or target edges [E][64] in the caller's order, device memory, fp32 */
+int hexdyn_net_export(hexdyn_net *net, int what, float *dst_device, void *stream);
+int hexdyn_net_last_error(const hexdyn_net *net, char *buf, size_t len);
 
 /* hexdyn_integrals without allocation, synchronisation or host copy: the
  * same kernels and reduction order (bitwise the same values), the per-block

@@ -11,7 +11,7 @@ HEADER = Path(__file__).resolve().parent.parent / "include" / "hexdyn.h"
 
 def declared():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|float|double)\s+\*?(hexdyn_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int64_t|int|void|float|double)\s+\*?(hexdyn_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_header_symbols():
