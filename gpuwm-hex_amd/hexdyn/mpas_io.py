@@ -251,7 +251,7 @@ VERTICAL_REQUIRED = ("zgrid", "rdzw", "dzu", "rdzu", "fzm", "fzp", "zz", "zxu", 
                      "cf1", "cf2", "cf3")
 
 
-def write_mpas_init(case, path, *, unit_sphere: bool = False) -> None:
+def write_mpas_init(case, path, *, unit_sphere: bool = False, public_w=None) -> None:
     """Write a case as an MPAS-style initialized file (mesh + vertical metrics +
     reference + state), 64-bit offset netCDF.
 
@@ -264,6 +264,11 @@ def write_mpas_init(case, path, *, unit_sphere: bool = False) -> None:
     ``rw`` and the binary32 ``rtheta_m``, ``rtheta_base``, ``exner_base`` are
     added as a model dump carries them, so a file read back reproduces the
     case bit for bit.
+
+    With ``public_w`` (the ``[L+1][C]`` vertical velocity of the state) the
+    file is written the way init_atmosphere writes one instead: physical
+    ``rho = rho_zz * zz`` and dry ``theta = theta_m / (1 + (R_v/R_d) qv)``,
+    ``u`` and ``w``, and none of ``rho_zz theta_m rtheta_m ru rw``.
     """
     from scipy.io import netcdf_file
 
@@ -305,12 +310,21 @@ def write_mpas_init(case, path, *, unit_sphere: bool = False) -> None:
         var("theta_base", cl, ref.theta_base.T)
         var("rtheta_base", cl, ref.rtheta_base.T)
         var("exner_base", cl, ref.exner_base.T)
-        var("rho_zz", cl, rho.T)
-        var("rtheta_m", cl, st.rtheta_m.T)
-        var("theta_m", cl, (st.rtheta_m / rho).astype(np.float32).T)
-        var("ru", ("Time", "nEdges", "nVertLevels"), st.ru.T)
+        theta_m = (st.rtheta_m / rho).astype(np.float32)
+        if public_w is None:
+            var("rho_zz", cl, rho.T)
+            var("rtheta_m", cl, st.rtheta_m.T)
+            var("theta_m", cl, theta_m.T)
+            var("ru", ("Time", "nEdges", "nVertLevels"), st.ru.T)
+            var("rw", ("Time", "nCells", "nVertLevelsP1"), st.rw.T)
+        else:
+            f32 = np.float32
+            rvord = f32(461.6) / f32(287.0)
+            qv = st.scalars[0] if case.n_scalars else np.zeros_like(rho)
+            var("rho", cl, (rho * g.zz).astype(f32).T)
+            var("theta", cl, (theta_m / (f32(1.0) + rvord * qv)).astype(f32).T)
+            var("w", ("Time", "nCells", "nVertLevelsP1"), np.asarray(public_w, dtype=f32).T)
         var("u", ("Time", "nEdges", "nVertLevels"), u.T)
-        var("rw", ("Time", "nCells", "nVertLevelsP1"), st.rw.T)
         for i in range(case.n_scalars):
             var(SCALAR_NAMES[i] if i < len(SCALAR_NAMES) else f"scalar{i}", cl, st.scalars[i].T)
         f.moist = 1 if case.n_moist > 0 else 0
@@ -321,7 +335,50 @@ def _levels_first(a: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a).T)
 
 
-def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=None, name: str = "mpas"):
+def w_to_rw(mesh, grid, rho_zz: np.ndarray, ru: np.ndarray, w: np.ndarray) -> np.ndarray:
+    """Vertical momentum of a vertical velocity on the terrain-following grid,
+    the inverse of the recovery of w (atm_init_coupled_diagnostics; the
+    reference's driver.py:1791-1854 with its correct_vertical_momentum):
+
+        rw(k) = (w(k) * rho_i(k)) * zz_i(k)
+                - sum_s ((sign_s * (zb_s(k) + sign(flux) zb3_s(k))) * flux) * zz_i(k)
+
+    for interior interfaces 1 <= k < L, with rho_i = fzm rho(k) + fzp rho(k-1),
+    zz_i = fzm zz(k) + fzp zz(k-1), flux = fzm ru(k) + fzp ru(k-1) on the
+    cell's edge in slot s, sign_s = +1 when the cell is the edge's first cell,
+    and zb3 the grid's slot metric (scaled by the third-order coefficient, as
+    the recovery reads it); rw = 0 at the surface and the top.  Binary32,
+    slots in edgesOnCell order.  The reference's correct_vertical_momentum
+    is a compiled engine that is not in the reference tree, so its operation
+    order is not pinned here; the conversion is pinned by its round trip
+    through the recovery of w."""
+    f32 = np.float32
+    L, C = rho_zz.shape
+    fzm = np.asarray(grid.fzm, dtype=f32)[:, None]
+    fzp = np.asarray(grid.fzp, dtype=f32)[:, None]
+    zz = np.asarray(grid.zz, dtype=f32)
+    rw = np.zeros((L + 1, C), dtype=f32)
+    rho_i = fzm[1:] * rho_zz[1:] + fzp[1:] * rho_zz[:-1]
+    zz_i = fzm[1:] * zz[1:] + fzp[1:] * zz[:-1]
+    rw[1:L] = (w[1:L] * rho_i) * zz_i
+    coe = np.asarray(mesh.cellsOnEdge)
+    eoc = np.asarray(mesh.edgesOnCell)
+    n = np.asarray(mesh.nEdgesOnCell)
+    cells = np.arange(C)
+    for s in range(eoc.shape[1]):
+        live = n > s
+        e = np.where(live, eoc[:, s], 0)
+        sign = np.where(coe[e, 0] == cells, f32(1.0), f32(-1.0)).astype(f32)
+        flux = fzm[1:] * ru[1:, e] + fzp[1:] * ru[:-1, e]
+        zb = np.asarray(grid.zb_cell[1:L, :, s], dtype=f32)
+        zb3 = np.asarray(grid.zb3_cell[1:L, :, s], dtype=f32)
+        term = ((sign[None, :] * (zb + np.copysign(f32(1.0), flux) * zb3)) * flux) * zz_i
+        rw[1:L] = np.where(live[None, :], rw[1:L] - term, rw[1:L]).astype(f32)
+    return rw
+
+
+def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=None, name: str = "mpas",
+                   cold_start: bool = False):
     """A dycore case from MPAS files: mesh (grid file, or the init file's own
     mesh variables), vertical metrics (the invariant file when given, the
     reference's authority order, driver.py:1418-1455), reference state and
@@ -330,9 +387,17 @@ def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=Non
     The per-edge ``zb``/``zb3`` become per-cell-slot metrics with zb3 scaled
     by ``config_coef_3rd_order`` in binary32.  ``rho_zz`` and ``theta_m`` are
     used as stored (``rtheta_m`` when the file carries it, else
-    ``rho_zz * theta_m`` in binary32); ``ru``/``rw`` are read when present,
-    else ``ru = 0.5 (rho_zz[c1] + rho_zz[c2]) u``; a file without ``rw`` is
-    refused (the terrain-following w -> rw conversion is not built).
+    ``rho_zz * theta_m`` in binary32).  An init_atmosphere file's public
+    fields are converted as the reference does (driver.py:1642-1681): physical
+    ``rho`` to ``rho_zz = rho / zz`` and dry ``theta`` to
+    ``theta_m = theta * (1 + rvord qv)`` with ``rvord = 461.6f / 287.0f``;
+    with ``cold_start`` they are used even where the file also holds
+    ``rho_zz``/``theta_m``/``ru``/``rw`` (atm_init_coupled_diagnostics
+    rebuilds the coupled state; the saved diagnostics are always recomputed
+    from the prognostics when a driver takes the state).  ``ru``/``rw`` are
+    read when present, else ``ru = (0.5 u) (rho_zz[c1] + rho_zz[c2])``
+    (driver.py:1760-1766, the half binding to u first) and ``rw`` from ``w``
+    by ``w_to_rw`` (driver.py:1791-1854).
     ``rtheta_base``/``exner_base`` are read when present, else formed as
     ``rho_base * theta_base`` and ``(zz R_d/p0 rtheta_base)^(R_d/c_v)``
     (binary64, rounded once: parity unpinned against MPAS's powf).
@@ -392,18 +457,38 @@ def case_from_mpas(init_path, *, grid_path=None, invariant_path=None, config=Non
         exner_base = np.power(arg, rgas / (cp - rgas)).astype(f32)
     reference = hinit.ReferenceState(rho_base=rho_base, rtheta_base=rtheta_base, exner_base=exner_base,
                                      theta_base=theta_base)
-    rho = cell_field("rho_zz")
-    rtheta = cell_field("rtheta_m") if "rtheta_m" in arrays else (rho * cell_field("theta_m")).astype(f32)
-    c1, c2 = mesh.cellsOnEdge[:, 0], mesh.cellsOnEdge[:, 1]
-    if "ru" in arrays:
-        ru = _levels_first(arrays["ru"]).astype(f32)
+    f32 = np.float32
+    if "rho_zz" in arrays and not (cold_start and "rho" in arrays):
+        rho = cell_field("rho_zz")
+    elif "rho" in arrays:
+        rho = (cell_field("rho") / zz).astype(f32)
     else:
+        raise MpasFileError("neither rho_zz nor the public rho is present")
+    if "theta_m" in arrays and not (cold_start and "theta" in arrays):
+        rtheta = cell_field("rtheta_m") if ("rtheta_m" in arrays and not cold_start) else \
+            (rho * cell_field("theta_m")).astype(f32)
+    elif "theta" in arrays:
+        if "qv" not in arrays:
+            raise MpasFileError("theta_m from the public theta needs qv")
+        rvord = f32(461.6) / f32(287.0)
+        theta_m = (cell_field("theta") * (f32(1.0) + rvord * cell_field("qv"))).astype(f32)
+        rtheta = (rho * theta_m).astype(f32)
+    else:
+        raise MpasFileError("neither theta_m nor the public theta is present")
+    c1, c2 = mesh.cellsOnEdge[:, 0], mesh.cellsOnEdge[:, 1]
+    if "ru" in arrays and not cold_start:
+        ru = _levels_first(arrays["ru"]).astype(f32)
+    elif "u" in arrays:
         u = _levels_first(arrays["u"]).astype(f32)
-        ru = ((f32(0.5) * (rho[:, c1] + rho[:, c2])) * u).astype(f32)
-    if "rw" not in arrays:
-        raise MpasFileError("the file carries no rw: the w -> rw conversion is not built; "
-                            "write rw (a model dump or hexdyn restart carries it)")
-    rw = _levels_first(arrays["rw"]).astype(f32)
+        ru = ((f32(0.5) * u) * (rho[:, c1] + rho[:, c2])).astype(f32)
+    else:
+        raise MpasFileError("neither ru nor u is present")
+    if "rw" in arrays and not cold_start:
+        rw = _levels_first(arrays["rw"]).astype(f32)
+    elif "w" in arrays:
+        rw = w_to_rw(mesh, grid, rho, ru, _levels_first(arrays["w"]).astype(f32))
+    else:
+        raise MpasFileError("neither rw nor w is present")
     names = [n for n in SCALAR_NAMES if n in arrays]
     names += sorted((n for n in arrays if n.startswith("scalar") and n[6:].isdigit()), key=lambda n: int(n[6:]))
     scalars = (np.stack([cell_field(n) for n in names]) if names

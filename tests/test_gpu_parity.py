@@ -532,14 +532,22 @@ def test_day_long_headline_run_conserves_mass(get_case):
     assert np.isfinite(st["rho_zz"]).all() and (st["scalars"] >= 0).all()
 
 
-def test_case_read_from_an_mpas_init_file_steps_bitwise(get_case, tmp_path):
-    print("[gpu] case from an MPAS init file x1.642-moist, 2 steps", flush=True)
+@pytest.mark.parametrize("public", [False, True])
+def test_case_read_from_an_mpas_init_file_steps_bitwise(get_case, tmp_path, public):
+    """A model-dump file (rho_zz, theta_m, ru, rw) and an init_atmosphere-style
+    file (rho, theta, u, w: converted, w -> rw with the terrain terms)."""
+    print(f"[gpu] case from an MPAS init file x1.642-moist (public fields: {public}), 2 steps", flush=True)
     from hexdyn import mpas_io
 
     case = get_case("x1.642-moist")
     p = tmp_path / "x1.642.init.nc"
-    mpas_io.write_mpas_init(case, p)
-    back = mpas_io.case_from_mpas(p, config=case.config)
+    w = None
+    if public:
+        o0 = Oracle(case, 32)
+        o0.set_state(initial_state_arrays(case), rebuild=True)
+        w = o0.get_state()["w"]
+    mpas_io.write_mpas_init(case, p, public_w=w)
+    back = mpas_io.case_from_mpas(p, config=case.config, cold_start=public)
     o = Oracle(back, 32)
     o.set_state(initial_state_arrays(back), rebuild=True)
     assert o.step(2) == 0

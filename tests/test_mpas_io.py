@@ -181,3 +181,84 @@ def test_file_deriv_two_builds_the_advection_stencils(get_case, tmp_path, unit_s
         assert "deriv_two" in back.mesh.arrays
         for name in ("adv_coefs", "adv_coefs_3rd"):
             assert _same_bits(getattr(case, name), getattr(back, name)) == same, (scale, name)
+
+
+def _recovered_w(case):
+    from oracle.oracle import Oracle
+    from hexdyn.case import initial_state_arrays
+
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    return o.get_state()["w"]
+
+
+def test_w_to_rw_inverts_the_recovery_of_w(get_case):
+    """rw -> w (the oracle's recovery, terrain terms included) -> rw again
+    returns the case's rw to binary32 rounding; a flipped terrain term does
+    not (the terrain slopes of the random 0-4 km orography matter)."""
+    case = get_case("x1.642-moist")
+    w = _recovered_w(case)
+    st = case.state
+    back = mpas_io.w_to_rw(case.mesh, case.grid, st.rho_zz, st.ru, w)
+    scale = np.abs(st.rw).max()
+    err = np.abs(back.astype(np.float64) - st.rw).max()
+    assert err <= 1e-5 * scale, (err, scale)
+    assert np.all(back[0] == 0) and np.all(back[-1] == 0)
+    # planted mistake: the terrain correction added instead of subtracted
+    import dataclasses
+
+    flipped = dataclasses.replace(case.grid, zb_cell=-case.grid.zb_cell, zb3_cell=-case.grid.zb3_cell)
+    wrong = mpas_io.w_to_rw(case.mesh, flipped, st.rho_zz, st.ru, w)
+    assert np.abs(wrong.astype(np.float64) - st.rw).max() > 100 * err
+
+
+def test_public_init_file_converts_rho_theta_u_w(get_case, tmp_path):
+    """An init_atmosphere-style file (physical rho, dry theta, u, w; no
+    rho_zz, theta_m, ru, rw) reads back to the case's prognostic state to
+    rounding, and the case read from it steps on the oracle."""
+    from oracle.oracle import Oracle
+
+    case = get_case("x1.642-moist")
+    p = tmp_path / "public.init.nc"
+    mpas_io.write_mpas_init(case, p, public_w=_recovered_w(case))
+    from scipy.io import netcdf_file
+
+    with netcdf_file(str(p), "r", mmap=False) as f:
+        names = set(f.variables)
+    assert {"rho", "theta", "u", "w"} <= names and not names & {"rho_zz", "theta_m", "rtheta_m", "ru", "rw"}
+    back = mpas_io.case_from_mpas(p, config=case.config)
+    a, b = case.state, back.state
+
+    def rel(x, y):
+        return float(np.abs(x.astype(np.float64) - y).max() / np.abs(y).max())
+
+    assert rel(b.rho_zz, a.rho_zz) <= 3e-7
+    assert rel(b.rtheta_m, a.rtheta_m) <= 1e-6
+    assert rel(b.ru, a.ru) <= 1e-6
+    assert rel(b.rw, a.rw) <= 1e-5
+    o = Oracle(back, 32)
+    from hexdyn.case import initial_state_arrays
+
+    o.set_state(initial_state_arrays(back), rebuild=True)
+    assert o.step(1) == 0
+
+
+def test_cold_start_prefers_public_fields(get_case, tmp_path):
+    """With cold_start the public rho/theta/u/w rebuild the coupled state even
+    where the file also holds rho_zz/theta_m/ru/rw (here scaled copies)."""
+    from scipy.io import netcdf_file
+
+    case = get_case("x1.642-moist")
+    p = tmp_path / "both.init.nc"
+    mpas_io.write_mpas_init(case, p, public_w=_recovered_w(case))
+    with netcdf_file(str(p), "a") as f:
+        L, C = case.state.rho_zz.shape
+        for name, a, dims in (("rho_zz", case.state.rho_zz * 2, ("Time", "nCells", "nVertLevels")),
+                              ("theta_m", case.state.rtheta_m / case.state.rho_zz * 2,
+                               ("Time", "nCells", "nVertLevels"))):
+            v = f.createVariable(name, "f4", dims)
+            v[0] = np.asarray(a, np.float32).T
+    warm = mpas_io.case_from_mpas(p, config=case.config)
+    cold = mpas_io.case_from_mpas(p, config=case.config, cold_start=True)
+    assert np.allclose(warm.state.rho_zz, 2 * case.state.rho_zz, rtol=1e-6)
+    assert np.allclose(cold.state.rho_zz, case.state.rho_zz, rtol=1e-6)
