@@ -1385,6 +1385,25 @@ int hexdyn_integrals_noalloc(hexdyn_handle *h, double *out_device, int n_out, vo
 
 // ---------------------------------------------------------------- state copy
 namespace {
+// Buffers of one state copy (every row is 256-byte aligned: float4 moves).
+struct CopyList {
+  int n;
+  float *dst[16];
+  const float *src[16];
+  size_t count[16];  // float4 elements
+};
+
+// all buffers of the list in one launch: each buffer is split over the grid
+__global__ void __launch_bounds__(256) k_copy_many(const CopyList cl) {
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int b = 0; b < cl.n; ++b) {
+    const float4 *src = (const float4 *)cl.src[b];
+    float4 *dst = (float4 *)cl.dst[b];
+    for (size_t i = tid; i < cl.count[b]; i += stride) dst[i] = src[i];
+  }
+}
+
 bool same_mesh(const hexdyn_handle *a, const hexdyn_handle *b) {
   const Sizes &x = a->I.n, &y = b->I.n;
   return x.C == y.C && x.E == y.E && x.V == y.V && x.L == y.L && x.ME == y.ME && x.NT == y.NT &&
@@ -1401,15 +1420,20 @@ int hexdyn_copy_state(hexdyn_handle *dst, hexdyn_handle *src, void *stream) {
   const Sizes &n = src->I.n;
   const Work &S = src->I.b;
   Work &D = dst->I.b;
-  int rc = 0;
-  rc |= copy_rows(D.rho, S.rho, n.C, s); rc |= copy_rows(D.rt, S.rt, n.C, s);
-  rc |= copy_rows(D.ru, S.ru, n.E, s); rc |= copy_rows(D.rw, S.rw, n.C, s);
-  if (n.NT > 0) rc |= copy_rows(D.q, S.q, (size_t)n.NT * n.C, s);
-  rc |= copy_rows(D.p_theta, S.p_theta, n.C, s); rc |= copy_rows(D.p_exner, S.p_exner, n.C, s);
-  rc |= copy_rows(D.p_rho_p, S.p_rho_p, n.C, s); rc |= copy_rows(D.p_rtheta_p, S.p_rtheta_p, n.C, s);
-  rc |= copy_rows(D.p_pp, S.p_pp, n.C, s); rc |= copy_rows(D.p_u, S.p_u, n.E, s);
-  rc |= copy_rows(D.p_w, S.p_w, n.C, s);
-  if (rc) return set_error(dst, HEXDYN_E_HIP, "hexdyn_copy_state: copy failed");
+  CopyList cl{};
+  auto add = [&](float *d, const float *p, size_t rows) {
+    cl.dst[cl.n] = d;
+    cl.src[cl.n] = p;
+    cl.count[cl.n] = rows * HX_ROW / 4;  // float4 per row quarter
+    ++cl.n;
+  };
+  add(D.rho, S.rho, n.C); add(D.rt, S.rt, n.C); add(D.ru, S.ru, n.E); add(D.rw, S.rw, n.C);
+  if (n.NT > 0) add(D.q, S.q, (size_t)n.NT * n.C);
+  add(D.p_theta, S.p_theta, n.C); add(D.p_exner, S.p_exner, n.C); add(D.p_rho_p, S.p_rho_p, n.C);
+  add(D.p_rtheta_p, S.p_rtheta_p, n.C); add(D.p_pp, S.p_pp, n.C); add(D.p_u, S.p_u, n.E);
+  add(D.p_w, S.p_w, n.C);
+  hipLaunchKernelGGL(k_copy_many, dim3(2048), dim3(256), 0, s, cl);
+  if (hipGetLastError() != hipSuccess) return set_error(dst, HEXDYN_E_HIP, "hexdyn_copy_state: copy failed");
   dst->time_s = src->time_s;
   return 0;
 }
@@ -1529,21 +1553,21 @@ int hexdyn_net_apply(hexdyn_net *net, hexdyn_handle *h, float gain, void *stream
   if (int rc = net_mesh_ok(net, h)) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (hexdyn::net::apply(*net->impl, h->I, gain, s)) return net_error(net, HEXDYN_E_HIP, "apply kernels failed");
-  if (NS_CALL(h, rebuild_saved, h->I, s)) return net_error(net, HEXDYN_E_HIP, "diagnostic rebuild failed");
-  Work &B = h->I.b;
-  const Sizes &n = h->I.n;
-  int rc = 0;
-  rc |= copy_rows(B.p_theta, B.theta, n.C, s); rc |= copy_rows(B.p_exner, B.exner, n.C, s);
-  rc |= copy_rows(B.p_rho_p, B.rho_p, n.C, s); rc |= copy_rows(B.p_rtheta_p, B.rtheta_p, n.C, s);
-  rc |= copy_rows(B.p_pp, B.pp, n.C, s); rc |= copy_rows(B.p_u, B.u, n.E, s);
-  rc |= copy_rows(B.p_w, B.w, n.C, s);
-  return rc ? net_error(net, HEXDYN_E_HIP, "diagnostic copy failed") : 0;
+  // the carried cell diagnostics recomputed from the corrected prognostics,
+  // written straight into the published buffers (the rebuild reads only
+  // prognostics); the edge pass of the apply already formed u from ru
+  Instance J = h->I;
+  Work &B = J.b;
+  B.theta = B.p_theta; B.exner = B.p_exner; B.rho_p = B.p_rho_p; B.rtheta_p = B.p_rtheta_p;
+  B.pp = B.p_pp; B.u = B.p_u; B.w = B.p_w;
+  if (NS_CALL(h, rebuild_cells, J, s)) return net_error(net, HEXDYN_E_HIP, "diagnostic rebuild failed");
+  return 0;
 }
 
-int hexdyn_net_checks(hexdyn_net *net, hexdyn_handle *h, float *out3, void *stream) {
-  if (!net || !out3) return HEXDYN_E_INVALID;
+int hexdyn_net_checks(hexdyn_net *net, hexdyn_handle *h, float *out3, double *dry_mass, void *stream) {
+  if (!net || !out3 || !dry_mass) return HEXDYN_E_INVALID;
   if (int rc = net_mesh_ok(net, h)) return rc;
-  return hexdyn::net::checks(*net->impl, h->I, (float)h->cfg.config_dt, out3, (hipStream_t)stream)
+  return hexdyn::net::checks(*net->impl, h->I, (float)h->cfg.config_dt, out3, dry_mass, (hipStream_t)stream)
              ? net_error(net, HEXDYN_E_HIP, "check kernels failed")
              : 0;
 }

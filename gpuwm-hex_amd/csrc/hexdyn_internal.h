@@ -187,6 +187,7 @@ struct Instance {
   int solve_diagnostics(const Instance &I, int rk, hipStream_t s);        \
   int scalars_stage(const Instance &I, int stage, hipStream_t s);         \
   int rebuild_saved(const Instance &I, hipStream_t s);                    \
+  int rebuild_cells(const Instance &I, hipStream_t s);                    \
   int clamp_scalars(const Instance &I, const float *src, float *dst, hipStream_t s); \
   }
 
@@ -215,7 +216,7 @@ int loss(Net &n, const Instance &I, double *out5, hipStream_t s);
 int backward(Net &n, const Instance &I, hipStream_t s);
 int adam(Net &n, float lr, float b1, float b2, float eps, int t, hipStream_t s);
 int apply(Net &n, const Instance &I, float gain, hipStream_t s);
-int checks(Net &n, const Instance &I, float dt, float out[3], hipStream_t s);
+int checks(Net &n, const Instance &I, float dt, float out[3], double *mass, hipStream_t s);
 int raw_features(Net &n, const Instance &I, const int *old_c, float *dst, hipStream_t s);
 int export_buf(Net &n, int what, const int *old_c, const int *old_e, float *dst, hipStream_t s);
 }  // namespace net

@@ -66,63 +66,78 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f3
 // and the 8 consecutive k values 8 * (l / 16) .. +7; the accumulator lane l
 // holds column l % 16, rows 4 * (l / 16) + i.
 enum { EPI_RELU = 0, EPI_RESID_RELU = 1, EPI_F32 = 2 };
+constexpr int GEMM_WAVES = 8;  // wavefronts per block (2 per SIMD beside the block's LDS copy of B)
+constexpr int GEMM_TR = 2;     // 16-row tiles per wavefront at a time (each B fragment feeds TR MFMAs)
 
 template <int K, int N, int K1, int EPI>
-__global__ void __launch_bounds__(256) k_gemm(const bf16 *__restrict__ A1, const bf16 *__restrict__ A2,
-                                              const bf16 *__restrict__ Bt, const float *__restrict__ bias,
-                                              const bf16 *__restrict__ resid, bf16 *__restrict__ out_b,
-                                              float *__restrict__ out_f, bf16 *__restrict__ relu_out, int M) {
+__global__ void __launch_bounds__(64 * GEMM_WAVES) k_gemm(const bf16 *__restrict__ A1, const bf16 *__restrict__ A2,
+                                                          const bf16 *__restrict__ Bt, const float *__restrict__ bias,
+                                                          const bf16 *__restrict__ resid, bf16 *__restrict__ out_b,
+                                                          float *__restrict__ out_f, bf16 *__restrict__ relu_out,
+                                                          int M) {
   static_assert(K % 32 == 0 && N % 16 == 0 && K1 % 32 == 0 && K1 <= K, "tile shape");
   constexpr int LDK = K + 8;  // row stride in LDS (16-byte pad against bank conflicts)
   constexpr int KA1 = K1;
   constexpr int KA2 = K - K1;
+  constexpr int NT = N / 16;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   bf16 *sB = (bf16 *)lds_raw;
-  for (int i = (int)threadIdx.x; i < N * (K / 8); i += 256) {
+  for (int i = (int)threadIdx.x; i < N * (K / 8); i += 64 * GEMM_WAVES) {
     const int n = i / (K / 8), kc = i % (K / 8);
     *(bf16x8 *)(sB + n * LDK + kc * 8) = *(const bf16x8 *)(Bt + (size_t)n * K + kc * 8);
   }
   __syncthreads();
   const int w = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
   const int lr = l & 15, lg = l >> 4;
-  const int ntiles = (M + 15) / 16;
-  for (int t = (int)blockIdx.x * 4 + w; t < ntiles; t += (int)gridDim.x * 4) {
-    const int row = min(t * 16 + lr, M - 1);
-    f32x4 acc[N / 16];
+  const int ngroups = (M + 16 * GEMM_TR - 1) / (16 * GEMM_TR);
+  for (int gt = (int)blockIdx.x * GEMM_WAVES + w; gt < ngroups; gt += (int)gridDim.x * GEMM_WAVES) {
+    int row[GEMM_TR];
 #pragma unroll
-    for (int j = 0; j < N / 16; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r < GEMM_TR; ++r) row[r] = min((gt * GEMM_TR + r) * 16 + lr, M - 1);
+    f32x4 acc[GEMM_TR][NT];
 #pragma unroll
+    for (int r = 0; r < GEMM_TR; ++r)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
     for (int k0 = 0; k0 < K; k0 += 32) {
       const int kk = k0 + lg * 8;
-      bf16x8 a;
-      if (KA2 == 0 || k0 < KA1)
-        a = *(const bf16x8 *)(A1 + (size_t)row * KA1 + kk);
-      else
-        a = *(const bf16x8 *)(A2 + (size_t)row * (KA2 > 0 ? KA2 : 1) + (kk - KA1));
+      bf16x8 a[GEMM_TR];
 #pragma unroll
-      for (int j = 0; j < N / 16; ++j) {
+      for (int r = 0; r < GEMM_TR; ++r) {
+        if (KA2 == 0 || k0 < KA1)
+          a[r] = *(const bf16x8 *)(A1 + (size_t)row[r] * KA1 + kk);
+        else
+          a[r] = *(const bf16x8 *)(A2 + (size_t)row[r] * (KA2 > 0 ? KA2 : 1) + (kk - KA1));
+      }
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
         const bf16x8 b = *(const bf16x8 *)(sB + (j * 16 + lr) * LDK + kk);
-        acc[j] = mfma(a, b, acc[j]);
+#pragma unroll
+        for (int r = 0; r < GEMM_TR; ++r) acc[r][j] = mfma(a[r], b, acc[r][j]);
       }
     }
 #pragma unroll
-    for (int j = 0; j < N / 16; ++j) {
+    for (int j = 0; j < NT; ++j) {
       const int col = j * 16 + lr;
       const float bj = bias ? bias[col] : 0.0f;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = t * 16 + lg * 4 + i;
-        if (r >= M) continue;
-        const float v = acc[j][i] + bj;
-        const size_t o = (size_t)r * N + col;
-        if (EPI == EPI_RELU) {
-          out_b[o] = (bf16)fmaxf(v, 0.0f);
-        } else if (EPI == EPI_RESID_RELU) {
-          const float rl = fmaxf(v, 0.0f);
-          if (relu_out) relu_out[o] = (bf16)rl;
-          out_b[o] = (bf16)((float)resid[o] + rl);
-        } else {
-          out_f[o] = v;
+      for (int r = 0; r < GEMM_TR; ++r) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int rr = (gt * GEMM_TR + r) * 16 + lg * 4 + i;
+          if (rr >= M) continue;
+          const float v = acc[r][j][i] + bj;
+          const size_t o = (size_t)rr * N + col;
+          if (EPI == EPI_RELU) {
+            out_b[o] = (bf16)fmaxf(v, 0.0f);
+          } else if (EPI == EPI_RESID_RELU) {
+            const float rl = fmaxf(v, 0.0f);
+            if (relu_out) relu_out[o] = (bf16)rl;
+            out_b[o] = (bf16)((float)resid[o] + rl);
+          } else {
+            out_f[o] = v;
+          }
         }
       }
     }
@@ -572,7 +587,9 @@ __global__ void __launch_bounds__(256) k_apply_cells(const Instance I, const flo
   }
 }
 
-// edges (after the cells): u' = u + du, ru' = (rho'_0 + rho'_1)/2 u'
+// edges (after the cells): u' = u + du, ru' = (rho'_0 + rho'_1)/2 u', and
+// the carried u recomputed from ru' as the diagnostic rebuild forms it,
+// 2 ru' / (rho'_0 + rho'_1)
 __global__ void __launch_bounds__(256) k_apply_edges(const Instance I, const float *__restrict__ Y,
                                                      const float *__restrict__ oscale, float gain) {
   const int e = ent4();
@@ -582,58 +599,101 @@ __global__ void __launch_bounds__(256) k_apply_edges(const Instance I, const flo
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const size_t o = (size_t)e * HX_ROW + k;
   const float u = I.b.p_u[o] + gain * (edge_pred(I, Y, e, k) * oscale[O_UE * 64 + k]);
-  I.b.ru[o] = (0.5f * (rowv(I.b.rho, c0, k) + rowv(I.b.rho, c1, k))) * u;
+  const float r0 = rowv(I.b.rho, c0, k), r1 = rowv(I.b.rho, c1, k);
+  const float ru = (0.5f * (r0 + r1)) * u;
+  I.b.ru[o] = ru;
+  I.b.p_u[o] = (2.0f * ru) / (r0 + r1);
 }
 
 // ---------------------------------------------------------- checks
-__device__ __forceinline__ void atomic_max_pos(float *p, float v) {
-  // non-negative floats (and NaN, which exceeds them all) order like their bits
-  atomicMax((unsigned int *)p, __float_as_uint(v));
-}
-
 __device__ __forceinline__ float nanmax(float a, float b) { return (b != b || b > a) ? b : a; }
 
-// out[0] = max |u| dt / dc (horizontal CFL), out[1] = max |w| dt rdzw / zz
-// (vertical CFL), out[2] = max |divergence| dt; a NaN anywhere wins
+// Per-block partials of the checks: max |u| dt / dc (horizontal CFL),
+// max |w| dt rdzw / zz (vertical CFL), max |divergence| dt (a NaN anywhere
+// wins), and the dry mass sum areaCell * dzw * rho_zz in binary64.
 template <int NE>
-__global__ void __launch_bounds__(256) k_checks_cells(const Instance I, float dt, float *__restrict__ out) {
+__global__ void __launch_bounds__(256) k_checks_cells(const Instance I, float dt, float *__restrict__ fpart,
+                                                      double *__restrict__ mpart) {
+  __shared__ float s_f[3][4];
+  __shared__ double s_m[4];
   const int c = ent4();
-  if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
   const Work &B = I.b;
-  const int n = I.m.neoc[c];
-  float div = 0.f, cfl = 0.f;
+  float div = 0.f, cfl = 0.f, vc = 0.f;
+  double mass = 0.0;
+  if (c < I.n.C) {
+    const int n = I.m.neoc[c];
 #pragma unroll
-  for (int s = 0; s < NE; ++s) {
-    if (s < n) {
-      const int e = I.m.eoc[c * NE + s];
-      const float u = rowv(B.p_u, e, k);
-      div += I.m.csdv[c * NE + s] * u;
-      cfl = nanmax(cfl, fabsf(u) * dt * I.m.invdc[e]);
+    for (int s = 0; s < NE; ++s) {
+      if (s < n) {
+        const int e = I.m.eoc[c * NE + s];
+        const float u = rowv(B.p_u, e, k);
+        div += I.m.csdv[c * NE + s] * u;
+        cfl = nanmax(cfl, fabsf(u) * dt * I.m.invdc[e]);
+      }
     }
-  }
-  div = fabsf(div * I.m.invarea[c]) * dt;
-  float vc = 0.f;
-  if (k >= 1 && k < L) vc = fabsf(rowv(B.p_w, c, k)) * dt * I.z.rdzw[k] / rowv(I.z.zz, c, k);
-  if (k >= L) {
-    div = 0.f;
-    cfl = 0.f;
+    div = fabsf(div * I.m.invarea[c]) * dt;
+    if (k >= 1 && k < L) vc = fabsf(rowv(B.p_w, c, k)) * dt * I.z.rdzw[k] / rowv(I.z.zz, c, k);
+    if (k < L) mass = (1.0 / (double)I.m.invarea[c]) * (1.0 / (double)I.z.rdzw[k]) * (double)rowv(B.rho, c, k);
+    if (k >= L) {
+      div = 0.f;
+      cfl = 0.f;
+    }
   }
   for (int off = 32; off >= 1; off >>= 1) {
     cfl = nanmax(cfl, __shfl_down(cfl, off, 64));
     vc = nanmax(vc, __shfl_down(vc, off, 64));
     div = nanmax(div, __shfl_down(div, off, 64));
+    mass += __shfl_down(mass, off, 64);
   }
+  const int w = (int)(threadIdx.x >> 6);
   if (k == 0) {
-    atomic_max_pos(out + 0, cfl);
-    atomic_max_pos(out + 1, vc);
-    atomic_max_pos(out + 2, div);
+    s_f[0][w] = cfl;
+    s_f[1][w] = vc;
+    s_f[2][w] = div;
+    s_m[w] = mass;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int v = (int)threadIdx.x;
+    fpart[(size_t)v * gridDim.x + blockIdx.x] = nanmax(nanmax(nanmax(s_f[v][0], s_f[v][1]), s_f[v][2]), s_f[v][3]);
+  } else if (threadIdx.x == 3) {
+    mpart[blockIdx.x] = ((s_m[0] + s_m[1]) + s_m[2]) + s_m[3];
   }
 }
 
-__global__ void k_zero3(float *p) {
-  if (threadIdx.x < 3) p[threadIdx.x] = 0.0f;
+// one block: out = {cfl_h, cfl_v, div (floats), dry mass (double)} from the
+// partials in a fixed order
+__global__ void __launch_bounds__(256) k_checks_final(const float *__restrict__ fpart, const double *__restrict__ mpart,
+                                                      int nb, float *__restrict__ fout, double *__restrict__ mout) {
+  __shared__ float s_f[3][256];
+  __shared__ double s_m[256];
+  const int t = (int)threadIdx.x;
+  float f[3] = {0.f, 0.f, 0.f};
+  double m = 0.0;
+  for (int b = t; b < nb; b += 256) {
+#pragma unroll
+    for (int v = 0; v < 3; ++v) f[v] = nanmax(f[v], fpart[(size_t)v * nb + b]);
+    m += mpart[b];
+  }
+#pragma unroll
+  for (int v = 0; v < 3; ++v) s_f[v][t] = f[v];
+  s_m[t] = m;
+  __syncthreads();
+  for (int off = 128; off >= 1; off >>= 1) {
+    if (t < off) {
+#pragma unroll
+      for (int v = 0; v < 3; ++v) s_f[v][t] = nanmax(s_f[v][t], s_f[v][t + off]);
+      s_m[t] = s_m[t] + s_m[t + off];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int v = 0; v < 3; ++v) fout[v] = s_f[v][0];
+    mout[0] = s_m[0];
+  }
 }
 
 // ---------------------------------------------------------- optimiser
@@ -708,7 +768,9 @@ struct Net {
   float *dH0 = nullptr, *dH1 = nullptr, *dZ = nullptr, *dH4 = nullptr;
   float *partial = nullptr, *lpart = nullptr;
   double *loss_out = nullptr, *mpart = nullptr;
-  float *shift = nullptr, *checks = nullptr, *raw = nullptr;
+  float *shift = nullptr, *checks = nullptr, *raw = nullptr, *cpart = nullptr;
+  double *mass_out = nullptr;
+  float check_host[6];  // three maxima, pad, then the binary64 mass
   int have_target = 0, have_forward = 0;
   std::string error;
 
@@ -754,11 +816,11 @@ int gemm(const Net &n, const bf16 *A1, const bf16 *A2, const bf16 *Bt, const flo
     if (set_lds(k_gemm<K, N, K1, EPI>, lds)) return -1;
     ready = 1;
   }
-  const int tiles = (M + 15) / 16;
-  const int grid = std::max(1, std::min(n.dev_blocks, (tiles + 3) / 4));
+  const int groups = (M + 16 * GEMM_TR - 1) / (16 * GEMM_TR);
+  const int grid = std::max(1, std::min(n.dev_blocks, (groups + GEMM_WAVES - 1) / GEMM_WAVES));
   prof_mark("k_net_gemm", s, 1);
-  hipLaunchKernelGGL((k_gemm<K, N, K1, EPI>), dim3(grid), dim3(256), lds, s, A1, A2, Bt, bias, resid, out_b, out_f,
-                     relu_out, M);
+  hipLaunchKernelGGL((k_gemm<K, N, K1, EPI>), dim3(grid), dim3(64 * GEMM_WAVES), lds, s, A1, A2, Bt, bias, resid,
+                     out_b, out_f, relu_out, M);
   prof_mark("k_net_gemm", s, 0);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -918,13 +980,21 @@ int apply(Net &n, const Instance &I, float gain, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int checks(Net &n, const Instance &I, float dt, float out[3], hipStream_t s) {
-  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, s, n.checks);
-  NE_LAUNCH(n.NE, k_checks_cells, nblk(n.C, 4), 256, s, I, dt, n.checks);
+int checks(Net &n, const Instance &I, float dt, float out[3], double *mass, hipStream_t s) {
+  const unsigned gc = nblk(n.C, 4);
+  prof_mark("k_net_checks", s, 1);
+  NE_LAUNCH(n.NE, k_checks_cells, gc, 256, s, I, dt, n.cpart, n.mpart);
+  hipLaunchKernelGGL(k_checks_final, dim3(1), dim3(256), 0, s, (const float *)n.cpart, (const double *)n.mpart,
+                     (int)gc, n.checks, n.mass_out);
+  prof_mark("k_net_checks", s, 0);
   if (hipGetLastError() != hipSuccess) return -1;
-  if (hipMemcpyAsync(out, n.checks, 3 * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+  // one host read: the three maxima and the binary64 mass side by side
+  if (hipMemcpyAsync(n.check_host, n.checks, 4 * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(&n.check_host[4], n.mass_out, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return -1;
+  for (int v = 0; v < 3; ++v) out[v] = n.check_host[v];
+  std::memcpy(mass, &n.check_host[4], sizeof(double));
   return 0;
 }
 
@@ -995,6 +1065,8 @@ Net *create(const Instance &I, int layers, std::string *err) {
   rc |= n->alloc(&n->mpart, 2 * (size_t)nblk(C, 4));
   rc |= n->alloc(&n->shift, 1);
   rc |= n->alloc(&n->checks, 4);
+  rc |= n->alloc(&n->cpart, 3 * (size_t)nblk(C, 4));
+  rc |= n->alloc(&n->mass_out, 1);
   if (rc) {
     *err = "device allocation for the corrector failed";
     delete n;

@@ -2013,6 +2013,11 @@ int rebuild_saved(const Instance &I, hipStream_t s) {
   return check();
 }
 
+int rebuild_cells(const Instance &I, hipStream_t s) {
+  LAUNCH_NE(I, k_rebuild_cell, I.n.C, 1, s, I);
+  return check();
+}
+
 int clamp_scalars(const Instance &I, const float *src, float *dst, hipStream_t s) {
   launch(k_clamp, I.n.NT * I.n.C, 1, s, I, src, dst, I.n.NT * I.n.C);
   return check();

@@ -82,8 +82,14 @@ def init_params(layers: int, seed: int = 0, out_gain: float = 0.01) -> np.ndarra
     return np.concatenate([p.ravel() for p in parts])
 
 
-def coarse_config(cfg: DycoreConfig, factor: int = 2, sub_steps: int = 3) -> DycoreConfig:
-    return replace(cfg, config_dt=cfg.config_dt * factor, config_number_of_sub_steps=sub_steps)
+COARSE_DEFAULT = (3, 3)   # (acoustic substeps per stage, dynamics subcycles) of the doubled-dt step
+
+
+def coarse_config(cfg: DycoreConfig, factor: int = 2, sub_steps: int = 3, splits: int = 3) -> DycoreConfig:
+    """The coarse step: dt x factor with its own acoustic substep count and
+    dynamics subcycles (1 or 3)."""
+    return replace(cfg, config_dt=cfg.config_dt * factor, config_number_of_sub_steps=sub_steps,
+                   config_dynamics_split_steps=splits)
 
 
 class NetError(RuntimeError):
@@ -106,6 +112,7 @@ class CorrectorNet:
         self.in_scale = np.ones(N_IN, np.float32)
         self.out_scale = np.ones(N_OUT, np.float32)
         self.envelope = np.zeros(3)  # max check values of the training states at the coarse dt
+        self.coarse = COARSE_DEFAULT  # (substeps, subcycles) of the coarse step it was trained after
 
     def close(self) -> None:
         if getattr(self, "_n", None):
@@ -178,9 +185,12 @@ class CorrectorNet:
         self._check(lib().hexdyn_net_apply(self._n, drv.handle, gain, self._s()), "hexdyn_net_apply")
 
     def checks(self, drv) -> tuple:
+        """(max horizontal CFL, max vertical CFL, max |divergence| dt, dry mass)"""
         out = (ct.c_float * 3)()
-        self._check(lib().hexdyn_net_checks(self._n, drv.handle, out, self._s()), "hexdyn_net_checks")
-        return float(out[0]), float(out[1]), float(out[2])
+        mass = ct.c_double()
+        self._check(lib().hexdyn_net_checks(self._n, drv.handle, out, ct.byref(mass), self._s()),
+                    "hexdyn_net_checks")
+        return float(out[0]), float(out[1]), float(out[2]), float(mass.value)
 
     def raw_features(self, drv):
         torch = _torch()
@@ -202,7 +212,7 @@ class CorrectorNet:
     def state_dict(self) -> Dict[str, np.ndarray]:
         return {"params": self.get_params(), "in_mean": self.in_mean, "in_scale": self.in_scale,
                 "out_scale": self.out_scale, "layers": np.array(self.layers),
-                "envelope": np.asarray(self.envelope, np.float64)}
+                "envelope": np.asarray(self.envelope, np.float64), "coarse": np.asarray(self.coarse)}
 
     def load_state(self, d) -> None:
         assert int(d["layers"]) == self.layers
@@ -210,6 +220,8 @@ class CorrectorNet:
         self.set_scales(d["in_mean"], d["in_scale"], d["out_scale"])
         if "envelope" in d:
             self.envelope = np.asarray(d["envelope"], np.float64)
+        if "coarse" in d:
+            self.coarse = tuple(int(x) for x in np.asarray(d["coarse"]).ravel())
 
 
 def copy_state(dst, src) -> None:
@@ -306,15 +318,19 @@ class TrainConfig:
     spinup_min: int = 2          # fine steps before a trajectory's first pair: seeded in [min, max]
     spinup_max: int = 8
     stats_pairs: int = 8         # pairs of a separate trajectory that set the scales
+    epochs: int = 1              # passes over the same seeded trajectories (the pairs are regenerated)
+    coarse: tuple = COARSE_DEFAULT  # (acoustic substeps, dynamics subcycles) of the doubled-dt step
 
 
-def make_pair_drivers(case):
-    """Fine and coarse drivers of one case (the fine one at the case's dt)."""
+def make_pair_drivers(case, coarse=COARSE_DEFAULT):
+    """Fine and coarse drivers of one case (the fine one at the case's dt;
+    coarse = (substeps, subcycles) of the doubled-dt step)."""
     from .driver import HexDycoreDriver
 
     fine = HexDycoreDriver.from_host(case)
-    coarse = HexDycoreDriver.from_host(replace(case, config=coarse_config(case.config)))
-    return fine, coarse
+    c = HexDycoreDriver.from_host(replace(case, config=coarse_config(case.config, 2, int(coarse[0]),
+                                                                     int(coarse[1]))))
+    return fine, c
 
 
 def make_pair(fine, coarse) -> None:
@@ -346,43 +362,45 @@ def train(case, tc: TrainConfig = TrainConfig(), log: Callable[[str], None] = pr
     by the dycore along seeded trajectories (fit: set the scales first).
     Returns (net, history)."""
     torch = _torch()
-    fine, coarse = drivers or make_pair_drivers(case)
+    fine, coarse = drivers or make_pair_drivers(case, tc.coarse)
     if net is None:
         net = CorrectorNet(coarse, tc.layers)
         net.set_params(init_params(tc.layers, tc.seed))
     if fit:
         fit_scales(net, case, fine, coarse, tc)
     rng = np.random.default_rng(tc.seed + 17)
+    spins = [int(rng.integers(tc.spinup_min, tc.spinup_max + 1)) for _ in range(tc.trajectories)]
     per = max(1, tc.pairs // tc.trajectories)
+    total = per * tc.trajectories * tc.epochs
     history = []
     step = 0
     t0 = time.perf_counter()
     ratio = coarse.case.config.config_dt / fine.case.config.config_dt
     envelope = np.zeros(3)
-    for traj in range(tc.trajectories):
-        fine.set_state(perturbed_state(case, seed=1000 * (tc.seed + 1) + traj))
-        spin = int(rng.integers(tc.spinup_min, tc.spinup_max + 1))
-        if spin:
-            fine.step_device(spin)
-        for _ in range(per):
-            if step >= tc.pairs:
-                break
-            make_pair(fine, coarse)
-            net.forward(coarse, keep=True)
-            net.set_target(coarse, fine)
-            lo = net.loss()
-            envelope = np.maximum(envelope, ratio * np.asarray(net.checks(fine)))
-            net.backward()
-            frac = step / max(1, tc.pairs - 1)
-            lr = tc.lr * (tc.lr_final / tc.lr) ** frac
-            net.adam(lr, step + 1)
-            step += 1
-            history.append(lo["total"])
-            if step % 50 == 0 or step == 1:
-                log(f"# pair {step}/{tc.pairs} loss {lo['total']:.4f} (u {lo['u']:.3f} theta {lo['theta_m']:.3f} "
-                    f"rho {lo['rho_zz']:.3f} w {lo['w']:.3f}) {time.perf_counter() - t0:.0f} s")
+    for epoch in range(tc.epochs):
+        for traj in range(tc.trajectories):
+            fine.set_state(perturbed_state(case, seed=1000 * (tc.seed + 1) + traj))
+            if spins[traj]:
+                fine.step_device(spins[traj])
+            for _ in range(per):
+                make_pair(fine, coarse)
+                net.forward(coarse, keep=True)
+                net.set_target(coarse, fine)
+                lo = net.loss()
+                if epoch == 0:
+                    envelope = np.maximum(envelope, ratio * np.asarray(net.checks(fine)[:3]))
+                net.backward()
+                frac = step / max(1, total - 1)
+                lr = tc.lr * (tc.lr_final / tc.lr) ** frac
+                net.adam(lr, step + 1)
+                step += 1
+                history.append(lo["total"])
+                if step % 50 == 0 or step == 1:
+                    log(f"# pair {step}/{total} loss {lo['total']:.4f} (u {lo['u']:.3f} theta {lo['theta_m']:.3f} "
+                        f"rho {lo['rho_zz']:.3f} w {lo['w']:.3f}) {time.perf_counter() - t0:.0f} s")
     torch.cuda.synchronize()
     net.envelope = envelope
+    net.coarse = tuple(int(x) for x in tc.coarse)
     return net, history
 
 
@@ -431,7 +449,9 @@ class LearnedStepper:
     def __init__(self, case, net_state=None, *, layers: int = 4, limits: Optional[Limits] = None,
                  drivers=None) -> None:
         self.case = case
-        self.fine, self.coarse = drivers or make_pair_drivers(case)
+        coarse = tuple(int(x) for x in np.asarray(net_state["coarse"]).ravel()) \
+            if net_state is not None and "coarse" in net_state else COARSE_DEFAULT
+        self.fine, self.coarse = drivers or make_pair_drivers(case, coarse)
         self.net = CorrectorNet(self.coarse, layers)
         if net_state is not None:
             self.net.load_state(net_state)
@@ -448,14 +468,11 @@ class LearnedStepper:
         self.coarse.set_state(state)
         self._mass = None
 
-    def _dry_mass(self, drv) -> float:
-        return drv.integrals()["dry_mass"]
-
     def step(self, n: int = 1) -> None:
         lim = self.limits
         for _ in range(n):
             if self._mass is None:
-                self._mass = self._dry_mass(self.coarse)
+                self._mass = self.net.checks(self.coarse)[3]
             copy_state(self.fine, self.coarse)
             why = None
             try:
@@ -465,8 +482,7 @@ class LearnedStepper:
             if why is None:
                 self.net.forward(self.coarse)
                 self.net.apply(self.coarse, self.gain)
-                ch, cv, dv = self.net.checks(self.coarse)
-                m1 = self._dry_mass(self.coarse)
+                ch, cv, dv, m1 = self.net.checks(self.coarse)
                 if not (ch <= lim.cfl_h):
                     why = "cfl_h"
                 elif not (cv <= lim.cfl_v):

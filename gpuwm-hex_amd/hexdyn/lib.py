@@ -87,7 +87,7 @@ def lib() -> ct.CDLL:
     L.hexdyn_net_backward.argtypes = [vp, vp]
     L.hexdyn_net_adam.argtypes = [vp, ct.c_float, ct.c_float, ct.c_float, ct.c_float, ct.c_int, vp]
     L.hexdyn_net_apply.argtypes = [vp, vp, ct.c_float, vp]
-    L.hexdyn_net_checks.argtypes = [vp, vp, f32p, vp]
+    L.hexdyn_net_checks.argtypes = [vp, vp, f32p, ct.POINTER(ct.c_double), vp]
     L.hexdyn_net_features.argtypes = [vp, vp, vp, vp]
     L.hexdyn_net_export.argtypes = [vp, ct.c_int, vp, vp]
     L.hexdyn_net_last_error.argtypes = [vp, ct.c_char_p, ct.c_size_t]

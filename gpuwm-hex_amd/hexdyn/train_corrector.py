@@ -31,6 +31,9 @@ def main(argv=None) -> int:
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--spinup-min", type=int, default=2)
     p.add_argument("--spinup", type=int, default=8)
+    p.add_argument("--epochs", type=int, default=1)
+    p.add_argument("--coarse-substeps", type=int, default=3)
+    p.add_argument("--coarse-subcycles", type=int, default=3, choices=[1, 3])
     p.add_argument("--mixing", default="2d_smagorinsky", choices=["2d_smagorinsky", "off"])
     p.add_argument("--out", required=True)
     a = p.parse_args(argv)
@@ -50,15 +53,17 @@ def main(argv=None) -> int:
     print(f"# case {a.workload} built in {time.perf_counter() - t0:.1f} s", flush=True)
     tc = learned.TrainConfig(pairs=a.pairs, trajectories=a.trajectories, layers=a.layers, lr=a.lr,
                              lr_final=a.lr_final, seed=a.seed, spinup_min=a.spinup_min,
-                             spinup_max=a.spinup)
+                             spinup_max=a.spinup, epochs=a.epochs,
+                             coarse=(a.coarse_substeps, a.coarse_subcycles))
     t0 = time.perf_counter()
     net, hist = learned.train(case, tc, log=lambda m: print(m, flush=True))
     wall = time.perf_counter() - t0
     record = {"workload": a.workload, "pairs": a.pairs, "trajectories": a.trajectories, "layers": a.layers,
-              "lr": a.lr, "lr_final": a.lr_final, "seed": a.seed, "spinup": [a.spinup_min, a.spinup], "mixing": a.mixing,
+              "lr": a.lr, "lr_final": a.lr_final, "epochs": a.epochs, "seed": a.seed, "spinup": [a.spinup_min, a.spinup], "mixing": a.mixing,
               "train_seconds": wall, "loss_first50": float(np.mean(hist[:50])),
               "loss_last50": float(np.mean(hist[-50:])), "envelope": [float(x) for x in net.envelope],
-              "coarse": {"dt": cfg.config_dt * 2, "number_of_sub_steps": 3}}
+              "coarse": {"dt": cfg.config_dt * 2, "number_of_sub_steps": a.coarse_substeps,
+                         "dynamics_split_steps": a.coarse_subcycles}}
     out = Path(a.out)
     out.parent.mkdir(parents=True, exist_ok=True)
     learned.save(net, out, history=np.asarray(hist, np.float32), record=json.dumps(record))

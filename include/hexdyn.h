@@ -353,8 +353,10 @@ int hexdyn_net_adam(hexdyn_net *net, float lr, float beta1, float beta2, float e
  * w; then the carried diagnostics are recomputed from the prognostics. */
 int hexdyn_net_apply(hexdyn_net *net, hexdyn_handle *h, float gain, void *stream);
 /* per-step checks of h's published state: out3 = {max |u| dt/dcEdge,
- * max |w| dt rdzw/zz, max |divergence| dt} (NaN if any value is NaN) */
-int hexdyn_net_checks(hexdyn_net *net, hexdyn_handle *h, float *out3, void *stream);
+ * max |w| dt rdzw/zz, max |divergence| dt} (NaN if any value is NaN) and the
+ * dry mass sum areaCell*dzw*rho_zz (binary64, fixed-order reduction), with
+ * one synchronisation of `stream` */
+int hexdyn_net_checks(hexdyn_net *net, hexdyn_handle *h, float *out3, double *dry_mass, void *stream);
 /* raw (unnormalised) features of h's published state, [C][384] in the
  * caller's cell order, device memory */
 int hexdyn_net_features(hexdyn_net *net, hexdyn_handle *h, float *dst_device, void *stream);

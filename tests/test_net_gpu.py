@@ -169,6 +169,9 @@ def test_apply_keeps_dry_mass(pair, trained_inputs):
     after = coarse.get_state()
     m1 = coarse.integrals()["dry_mass"]
     assert abs(m1 - m0) <= 1e-9 * abs(m0)
+    # the checks' own binary64 dry mass is the same sum in another order
+    ch = net.checks(coarse)
+    assert abs(ch[3] - m1) <= 1e-12 * abs(m1) and all(np.isfinite(ch[:3]))
     assert not np.array_equal(before["u"], after["u"])  # the correction is not empty
 
 
