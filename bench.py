@@ -46,6 +46,10 @@ def parse():
     p.add_argument("--profile-steps", type=int, default=2)
     p.add_argument("--calibration-copy-mib", type=int, default=0,
                    help="also run a device copy of this many MiB (counter calibration for rocprofv3 --pmc)")
+    p.add_argument("--form", default="both", choices=["both", "plain"],
+                   help="both: also time the learned-corrected form (needs the trained corrector file)")
+    p.add_argument("--error-hours", type=float, default=24.0,
+                   help="forecast length of the surface-pressure error against the stored FP64 run")
     return p.parse_args()
 
 
@@ -100,12 +104,165 @@ def fp64_error(case, got):
     return out
 
 
+def surface_pressure_rms(case, state, ps64):
+    sys.path.insert(0, str(ROOT))
+    from oracle.surface_pressure import surface_pressure
+    import numpy as np
+
+    d = surface_pressure(case, state) - ps64
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def learned_leg(case, args, plain_value, plain_state_24h):
+    """Time the learned-corrected form (coarse step + correction + checks,
+    plain fallback) on the same workload and state its 24 h error."""
+    import numpy as np
+    import torch
+
+    from hexdyn import learned
+    from hexdyn.case import initial_state_arrays
+
+    wfile = learned.DATA / f"corrector_{args.workload}.npz"
+    if not wfile.exists():
+        return {"error": f"no trained corrector {wfile.name}"}
+    net_state = learned.load(wfile)
+    st = learned.LearnedStepper(case, net_state, layers=int(net_state["layers"]))
+    ratio = 2
+    for _ in range(max(1, args.warmup // ratio)):
+        st.step(1)
+    st.set_state(initial_state_arrays(case))
+    torch.cuda.synchronize()
+    n = max(1, args.steps // ratio)
+    fb0 = st.fallbacks
+    w0 = time.perf_counter()
+    st.step(n)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    value = n * ratio * case.config.config_dt / wall
+    out = {"form": "learned", "value": round(value, 1), "unit": "sim-days/wall-day",
+           "ms_per_coarse_step": round(1000.0 * wall / n, 4), "coarse_steps": n,
+           "coarse_dt": case.config.config_dt * ratio,
+           "coarse_number_of_sub_steps": int(st.net.coarse[0]), "coarse_dynamics_split_steps": int(st.net.coarse[1]),
+           "speedup_vs_plain": round(value / plain_value, 4), "fallbacks": st.fallbacks - fb0,
+           "trips": sorted(set(st.trips)), "limits": st.limits.__dict__,
+           "network": {"layers": int(net_state["layers"]), "hidden": learned.HIDDEN,
+                       "params": int(np.asarray(net_state["params"]).size)}}
+    # MFMA roofline of the network's GEMMs (HIP events on the launch stream)
+    st.coarse.profile(True)
+    st.net.forward(st.coarse)
+    st.net.forward(st.coarse)
+    torch.cuda.synchronize()
+    avg, launches = st.coarse.profile_read("k_net_gemm")
+    st.coarse.profile(False)
+    C, L = case.mesh.n_cells, case.n_levels
+    layers = int(net_state["layers"])
+    flops = 2.0 * C * (learned.N_IN * learned.HIDDEN + layers * 2 * learned.HIDDEN * learned.HIDDEN
+                       + learned.HIDDEN * learned.N_OUT)
+    gemm_ms = avg * launches / 2.0
+    if gemm_ms > 0:
+        tf = flops / (gemm_ms * 1e-3) / 1e12
+        out["roofline_mfma"] = {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                                "frac": round(tf / 2500.0, 5), "flops_per_forward": flops,
+                                "gemm_ms_per_forward": round(gemm_ms, 5), "gemm_launches_per_forward": launches / 2}
+    # 24 h run of the learned form: FP64 integrals every step, and the
+    # surface-pressure error against the stored FP64 run beside plain's
+    if plain_state_24h is not None:
+        sys.path.insert(0, str(ROOT))
+        from oracle import surface_pressure as sp
+
+        st2 = learned.LearnedStepper(case, net_state, layers=layers, drivers=(st.fine, st.coarse))
+        st2.set_state(initial_state_arrays(case))
+        nsteps = int(round(args.error_hours * 3600.0 / (ratio * case.config.config_dt)))
+        first = st2.coarse.integrals()
+        last = first
+        for _ in range(nsteps):  # FP64 integrals every (coarse) step
+            st2.step(1)
+            last = st2.coarse.integrals()
+        out["learned_24h_fp64_integral_drift"] = {
+            "dry_mass": (last["dry_mass"] - first["dry_mass"]) / first["dry_mass"],
+            "total_energy": (last["total_energy"] - first["total_energy"]) / first["total_energy"],
+            "fallbacks": st2.fallbacks, "steps": st2.steps}
+        learned24 = st2.get_state()
+        # the same coarse steps without the correction (what the network adds)
+        st.coarse.set_state(initial_state_arrays(case))
+        st.coarse.step_device(nsteps)
+        coarse24 = st.coarse.get_state()
+        ps_plain = sp.surface_pressure(case, plain_state_24h)
+        out["ps_rms_vs_plain_pa"] = {"learned": surface_pressure_rms(case, learned24, ps_plain),
+                                     "uncorrected_coarse": surface_pressure_rms(case, coarse24, ps_plain)}
+        ps64 = sp.load(case, args.workload, args.error_hours)
+        if ps64 is not None:
+            lerr = surface_pressure_rms(case, learned24, ps64)
+            perr = surface_pressure_rms(case, plain_state_24h, ps64)
+            out["ps_rms_vs_fp64"] = {"hours": args.error_hours, "plain_pa": perr, "learned_pa": lerr,
+                                     "uncorrected_coarse_pa": surface_pressure_rms(case, coarse24, ps64),
+                                     "ratio": lerr / perr if perr > 0 else None}
+        else:
+            out["ps_rms_vs_fp64"] = {"error": "no stored FP64 surface pressure for this case"}
+    st.close()
+    return out
+
+
+ENSEMBLE = "ensemble-8x40962-24x10242"
+
+
+def ensemble_bench(args) -> None:
+    """Batched ensemble: 8 members on x1.40962 (dt 720 s) and 24 on x1.10242
+    (dt 1440 s), each mesh group one instance on the union of its members'
+    meshes (one launch sequence per group), advanced 12 h after one warm-up
+    interval.  Beside it, the same members as separate instances (timed on
+    one member of each mesh over 2 h and scaled by the member counts)."""
+    import torch
+
+    from hexdyn import ensemble
+    from hexdyn.driver import HexDycoreDriver
+
+    t0 = time.perf_counter()
+    ens = ensemble.config3()
+    setup = time.perf_counter() - t0
+    print(f"# ensemble set-up {setup:.1f} s", file=sys.stderr, flush=True)
+    ens.advance(ens.interval)
+    hours = 12.0
+    wall = ens.advance(hours * 3600.0)
+    members = sum(g.n for g in ens.groups)
+    value = members * hours / 24.0 / (wall / 86400.0)
+    sep = 0.0
+    detail = []
+    for g in ens.groups:
+        d = HexDycoreDriver.from_host(g.case, state=ensemble.member_state(g.case, 0))
+        d.step_device(1)
+        n = int(round(2 * 3600.0 / g.dt))
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        d.step_device(n)
+        torch.cuda.synchronize()
+        one = (time.perf_counter() - w0) / n
+        d.close()
+        per_member_12h = one * hours * 3600.0 / g.dt
+        sep += per_member_12h * g.n
+        detail.append({"mesh": g.case.name, "members": g.n, "dt": g.dt, "cells_per_member": g.case.mesh.n_cells,
+                       "ms_per_step_separate": round(1000 * one, 4)})
+    ens.close()
+    print(json.dumps({
+        "metric": "member-sim-days/wall-day, batched ensemble 8 x x1.40962 + 24 x x1.10242 L55, 1 GPU",
+        "value": round(value, 1), "unit": "member-sim-days/wall-day", "n_gpus": 1, "hours": hours,
+        "wall_s": round(wall, 4), "members": members, "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic (seeded meshes, JW jet, seeded moisture, seeded smoothed theta noise sigma 0.1 K per member)",
+        "config": {"workload": ENSEMBLE, "groups": detail, "form": "plain", "numerics": "strict",
+                   "horiz_mixing": "2d_smagorinsky", "hipgraph": True},
+        "separate_instances_wall_s_estimate": round(sep, 4),
+        "batching_speedup": round(sep / wall, 4)}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": f"--gpus {args.gpus}: this build runs on one GPU only; "
                                    "multi-GPU sharding is designed but not built"}))
         sys.exit(2)
+    if args.workload == ENSEMBLE:
+        ensemble_bench(args)
+        return
     import torch
 
     from hexdyn.bytes_model import HBM_PEAK_GBS, kernel_bytes, step_bytes
@@ -132,6 +289,17 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    from hexdyn import autotune
+
+    tuning = drv.tuning
+    if tuning is None:
+        tuning = autotune.tune(drv)
+        try:
+            autotune.save(tuning)
+        except OSError:
+            pass
+    print(f"# launch tuning: {tuning['table']} (default {tuning['default_ms_per_step']} ms/step, "
+          f"tuned {tuning['tuned_ms_per_step']} ms/step)", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         drv.step_device(1)
     torch.cuda.synchronize()
@@ -210,6 +378,30 @@ def main():
         except Exception as exc:
             err64 = {"error": repr(exc)}
 
+    learned_out = None
+    if args.form == "both":
+        plain24 = None
+        try:
+            hrs = int(round(args.error_hours * 3600.0 / case.config.config_dt))
+            d24 = HexDycoreDriver.from_host(case)
+            first = d24.integrals()
+            last = first
+            for _ in range(hrs):  # FP64 integrals every step (GPU, fixed-order sums)
+                d24.step_device(1)
+                last = d24.integrals()
+            plain24 = d24.get_state()
+            plain_drift = {"dry_mass": (last["dry_mass"] - first["dry_mass"]) / first["dry_mass"],
+                           "total_energy": (last["total_energy"] - first["total_energy"]) / first["total_energy"]}
+            d24.close()
+        except Exception as exc:
+            learned_out = {"error": f"plain 24 h run: {exc!r}"}
+        if learned_out is None:
+            try:
+                learned_out = learned_leg(case, args, value, plain24)
+                learned_out["plain_24h_fp64_integral_drift"] = plain_drift
+            except Exception as exc:  # reported, never silently substituted
+                learned_out = {"error": repr(exc)}
+
     out = {
         "metric": HEADLINE_METRIC if args.workload == "x1.40962-moist"
         else f"sim-days/wall-day {args.workload.split('-')[0]} L{L}, 1 GPU; %HBM peak",
@@ -228,11 +420,15 @@ def main():
                    "dt": cfg.config_dt, "form": "plain", "numerics": args.numerics,
                    "horiz_mixing": args.mixing,
                    "dynamics_split_steps": cfg.config_dynamics_split_steps,
-                   "number_of_sub_steps": cfg.config_number_of_sub_steps, "parallelism": "single-gpu"},
+                   "number_of_sub_steps": cfg.config_number_of_sub_steps, "parallelism": "single-gpu",
+                   "launch_tuning": {"waves_per_block": tuning["table"], "default": 4,
+                                     "default_ms_per_step": tuning["default_ms_per_step"],
+                                     "tuned_ms_per_step": tuning["tuned_ms_per_step"]}},
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,
         "fp64_rel_l2": err64,
+        "learned": learned_out,
         "kernels_top": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()} for r in kernels[:8]],
     }
     print(json.dumps(out))
