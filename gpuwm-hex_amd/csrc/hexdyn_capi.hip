@@ -1314,8 +1314,9 @@ bool column_kernel(const char *name) {
 int hexdyn_set_launch(hexdyn_handle *h, const char *kernel, int waves_per_block) {
   if (!h || !kernel) return HEXDYN_E_INVALID;
   if (h->in_step) return set_error(h, HEXDYN_E_INVALID, "hexdyn_set_launch inside a step");
-  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
-    return set_error(h, HEXDYN_E_INVALID, "waves_per_block must be 1, 2 or 4");
+  if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4 && waves_per_block != 8 &&
+      waves_per_block != 16)
+    return set_error(h, HEXDYN_E_INVALID, "waves_per_block must be 1, 2, 4, 8 or 16");
   LaunchTune &t = h->tune;
   if (std::strcmp(kernel, "*") == 0) {
     t.n = 0;

@@ -151,7 +151,7 @@ void prof_mark(const char *name, hipStream_t s, int start);
 extern int g_prof_on;
 
 // Launch settings of the gather kernels (host side; kernels never read it):
-// wavefronts (entities) per block, 1, 2 or 4, by kernel name.
+// wavefronts (entities) per block, 1, 2, 4, 8 or 16, by kernel name.
 struct LaunchTune {
   static constexpr int kMax = 64;
   int n = 0;

@@ -48,7 +48,8 @@ __device__ __forceinline__ uint32_t row_off(int ent, int k) {
   return (uint32_t)ent * (uint32_t)(HX_ROW * 4) + (uint32_t)k * 4u;
 }
 #define AT(p, ent, k) (*(float *)((char *)(p) + row_off((ent), (k))))
-#define KB __global__ void __launch_bounds__(256)
+// gather kernels take up to 16 rows (wavefronts) per block: a per-kernel launch setting
+#define KB __global__ void __launch_bounds__(1024)
 
 // First entity of this block: blocks are renumbered so that XCD x (hardware
 // block b runs on XCD b % 8) owns a contiguous range of the curve.  WPB is

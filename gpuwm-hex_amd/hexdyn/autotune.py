@@ -3,9 +3,11 @@
 Each gather kernel runs one wavefront per cell, edge or vertex row (level k
 on lane k, the whole column of a row in one wavefront, so the level block is
 the row).  What is left to tune is the tile: wavefronts (rows) per block,
-1, 2 or 4.  A block's rows are consecutive along the space-filling curve and
+1, 2, 4, 8 or 16.  A block's rows are consecutive along the space-filling curve and
 share the block's XCD, so the tile sets how much of a neighbourhood one
-scheduling unit covers, and how many blocks a CU holds.  The tuner times
+scheduling unit covers (the rows of one block share the CU's L1, and
+consecutive rows on the curve share most of their neighbour rows), and how
+many blocks a CU holds.  The tuner times
 every gather kernel at each tile with HIP events over profiled large steps
 of the handle's own mesh, keeps each kernel's fastest tile, checks that the
 whole step with the chosen tiles is not slower than with the default (4),
@@ -25,7 +27,7 @@ from typing import Dict, Optional
 
 from .lib import lib
 
-CANDIDATES = (1, 2, 4)
+CANDIDATES = (1, 2, 4, 8, 16)
 COLUMN_KERNELS = {"k_vcoef", "k_ac_cell"}
 
 

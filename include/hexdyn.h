@@ -237,7 +237,7 @@ int hexdyn_set_graphs(hexdyn_handle *h, int enable);
 int hexdyn_graph_count(const hexdyn_handle *h);
 
 /* Launch shape of the gather kernels (one wavefront per cell, edge or
- * vertex row): wavefronts per block, 1, 2 or 4 (the default), by kernel name
+ * vertex row): wavefronts per block, 1, 2, 4 (the default), 8 or 16, by kernel name
  * ("k_diag_edge_flux", ...; "*" resets every kernel to the given value).
  * The column kernels (k_vcoef, k_ac_cell: 16 columns per block through LDS)
  * are refused.  Results do not depend on it.  Drops the captured step graphs.
