@@ -12,7 +12,7 @@ every gather kernel at each tile with HIP events over profiled large steps
 of the handle's own mesh, keeps each kernel's fastest tile, checks that the
 whole step with the chosen tiles is not slower than with the default (4),
 and stores the choice in a JSON file keyed by device, mesh size, levels,
-scalars and mixing; ``apply_saved`` installs it after ``hexdyn_create``.
+scalars, mixing and numerics; ``apply_saved`` installs it after ``hexdyn_create``.
 (The reference tunes the level chunking of its hoisted edge kernels by
 longest-processing-time packing, cuda_edge_hoist.py:63-101.)
 """
@@ -39,11 +39,11 @@ def _cache_dir() -> Path:
 
 
 def tuning_key(drv) -> str:
-    import torch
-
+    """Mesh and configuration of a stored table (the cache directory belongs
+    to one machine; the device name is left out because a profiler attached
+    to the process can hide it)."""
     c = drv.case
-    dev = torch.cuda.get_device_name(0).replace(" ", "_") if torch.cuda.is_available() else "cpu"
-    return (f"{dev}-C{c.mesh.n_cells}-E{c.mesh.n_edges}-L{c.n_levels}-T{c.n_scalars}-"
+    return (f"C{c.mesh.n_cells}-E{c.mesh.n_edges}-L{c.n_levels}-T{c.n_scalars}-"
             f"{c.config.config_horiz_mixing}-{c.config.numerics}")
 
 

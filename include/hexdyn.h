@@ -323,7 +323,9 @@ typedef struct hexdyn_net hexdyn_net;
 enum { HEXDYN_NET_X = 0, HEXDYN_NET_Y = 1, HEXDYN_NET_TARGET_CELLS = 2, HEXDYN_NET_TARGET_EDGES = 3 };
 
 /* A network of `layers` (1..8) message-passing layers bound to the mesh of
- * handle h (weights zero until set). */
+ * handle h (weights zero until set).  h must outlive the network; any
+ * handle of the same mesh (same sizes and numbering) may be passed to the
+ * calls below. */
 int hexdyn_net_create(hexdyn_handle *h, int layers, hexdyn_net **out);
 void hexdyn_net_destroy(hexdyn_net *net);
 int64_t hexdyn_net_param_count(const hexdyn_net *net);

@@ -234,3 +234,47 @@ def test_fallback_on_tripped_checks_equals_plain_steps(get_case):
         for k in want:
             assert np.array_equal(np.asarray(got[k]).view(np.uint32), np.asarray(want[k]).view(np.uint32)), (why, k)
         st.close()
+
+
+def test_ragged_mesh_forward_and_gradients_match_statement(get_case):
+    """The maxEdges-7 instantiations (features, neighbour mean and its
+    adjoint, loss gather, checks) on the r642 mesh with heptagons, against
+    the binary64 statement."""
+    from hexdyn import learned
+
+    print("[gpu] corrector on the ragged r642 mesh (maxEdges 7)", flush=True)
+    case = get_case("r642-moist")
+    assert case.mesh.max_edges == 7
+    fine, coarse = learned.make_pair_drivers(case)
+    _stepped_pair(case, fine, coarse, seed=5)
+    net = learned.CorrectorNet(coarse, 2)
+    flat = learned.init_params(2, seed=9, out_gain=0.5)
+    net.set_params(flat)
+    stats = learned.ScaleStats()
+    stats.add(net, coarse, fine)
+    net.set_scales(*stats.scales(case.n_levels))
+    g = _graph(case)
+    got_raw = net.raw_features(coarse).cpu().numpy()
+    want_raw = hn.features(coarse.get_state(), g, case.mesh.arrays["fEdge"], case.n_levels, moist=True)
+    scale = np.abs(want_raw.reshape(-1, 6, 64)).max(axis=(0, 2))
+    assert np.all(np.abs(got_raw - want_raw).reshape(-1, 6, 64).max(axis=(0, 2)) <= 1e-5 * scale + 1e-30)
+    net.forward(coarse, keep=True)
+    net.set_target(coarse, fine)
+    lo = net.loss()
+    net.backward()
+    X, Y = net.export("X").cpu().numpy(), net.export("Y").cpu().numpy()
+    Tc, Te = net.export("target_cells").cpu().numpy(), net.export("target_edges").cpu().numpy()
+    p = hn.unpack(flat, 2)
+    cache = hn.forward(p, X, g, 2)
+    assert _rel(Y, cache["Y"]) <= 2e-2
+    parts, dY = hn.loss(cache["Y"], Tc, Te, g, case.n_levels)
+    assert abs(lo["total"] - parts["total"]) <= 2e-2 * parts["total"]
+    want = hn.backward(p, cache, dY, g, 2)
+    got = learned.unpack(net.get_grads(), 2)
+    bad = [(n, _rel(got[n], want[n])) for n, _ in hn.layout(2) if not _rel(got[n], want[n]) <= 5e-2]
+    assert not bad, bad
+    ch = net.checks(coarse)
+    assert all(np.isfinite(ch)) and ch[3] > 0
+    net.close()
+    fine.close()
+    coarse.close()
