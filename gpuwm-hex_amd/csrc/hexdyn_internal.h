@@ -22,6 +22,9 @@ namespace hexdyn {
 
 struct Sizes {
   int C, E, V, L, ME, ME2, MA, NT, NM;
+  // wavefronts (rows) per block of the launch in flight: each launch passes
+  // its own copy, so a kernel finds its tile with its other scalar arguments
+  int WPB = 4;
 };
 
 struct Mesh {

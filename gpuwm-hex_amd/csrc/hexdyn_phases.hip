@@ -52,20 +52,26 @@ __device__ __forceinline__ uint32_t row_off(int ent, int k) {
 #define KB __global__ void __launch_bounds__(1024)
 
 // First entity of this block: blocks are renumbered so that XCD x (hardware
-// block b runs on XCD b % 8) owns a contiguous range of the curve.  WPB is
-// the number of wavefronts (entities) per block; WPB = 0 takes it from the
-// launch (the gather kernels' block size is a per-kernel launch setting).
-template <int WPB>
-__device__ __forceinline__ int block_entity0() {
+// block b runs on XCD b % 8) owns a contiguous range of the curve.  wpb is
+// the number of wavefronts (entities) per block.
+__device__ __forceinline__ int block_entity0(int wpb) {
   const int nb = gridDim.x;
   const int b = blockIdx.x;
   const int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
   const int lb = (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-  return lb * (WPB > 0 ? WPB : (int)(blockDim.x >> 6));
+  return lb * wpb;
 }
-template <int WPB = 0>
+// Column kernels: WPB rows per block, fixed at compile time.
+template <int WPB>
 __device__ __forceinline__ int wave_entity() {
-  return __builtin_amdgcn_readfirstlane(block_entity0<WPB>() + (int)(threadIdx.x >> 6));
+  return __builtin_amdgcn_readfirstlane(block_entity0(WPB) + (int)(threadIdx.x >> 6));
+}
+// Gather kernels: the block size is a per-kernel launch setting, carried in
+// the kernel's own Instance argument (I.n.WPB) rather than read from the
+// dispatch packet, which would cost each wavefront a vector-memory round trip
+// before its first row load.
+__device__ __forceinline__ int wave_entity(const Instance &I) {
+  return __builtin_amdgcn_readfirstlane(block_entity0(I.n.WPB) + (int)(threadIdx.x >> 6));
 }
 __device__ __forceinline__ int lane() { return (int)(threadIdx.x & 63); }
 __device__ __forceinline__ float up1(float x) { return __shfl_up(x, 1, 64); }     // value of lane k-1
@@ -122,6 +128,49 @@ __device__ __forceinline__ void slots(T (&out)[N], const T *__restrict__ p) {
   for (int s = 0; s < N; ++s) out[s] = p[s];
 }
 
+// Per-slot coefficients of one entity (its row of N values in an [n][N]
+// table), read by one vector load with lane s taking slot s and issued with
+// the row loads; slot s is read from lane s where it is used.  The same
+// values as scalar loads miss the scalar cache for every entity, and the
+// compiler issues such loads next to their uses, after the rows have
+// arrived, so each becomes a dependent round trip of the wavefront.
+// Read a slot only in wave-uniform control flow: the compiler may move the
+// load into the first branch that reads the row, and in a lane-divergent
+// branch the lanes that hold the other slots would not execute it.  Code
+// that needs slots under a lane condition takes them with slot_vals first.
+template <int N>
+struct SlotRow {
+  float v;
+  __device__ __forceinline__ float operator[](int s) const { return rdl(v, s); }
+};
+template <int N>
+__device__ __forceinline__ SlotRow<N> slot_row(const float *__restrict__ p) {
+  const int l = lane();
+  return {p[l < N ? l : N - 1]};
+}
+// every slot as a wave-uniform value (call with all lanes active)
+template <int N>
+__device__ __forceinline__ void slot_vals(const SlotRow<N> &r, float (&o)[N]) {
+#pragma unroll
+  for (int s = 0; s < N; ++s) o[s] = r[s];
+}
+// Ends the load phase for x: the compiler may not sink the load of x into
+// the first (uniform) branch that uses it, which would issue it, and the
+// scalar index load it needs, one round trip after the others.
+__device__ __forceinline__ void keep(float &x) { asm volatile("" : "+v"(x)); }
+// Ends the index phase of a wavefront: every scalar index load issued before
+// the first fence completes in one wait, before any row load after it is
+// issued.  Without it the compiler may issue one index table's loads only
+// after waiting for another's, one more dependent round trip.
+__device__ __forceinline__ void fence(int x) { asm volatile("" ::"s"(x)); }
+// A per-entity value read by a vector load (every lane the same address) for
+// the same reason; the zero offset is opaque, so the load stays a vector load.
+__device__ __forceinline__ float uni_load(const float *__restrict__ p) {
+  int z;
+  asm("v_mov_b32 %0, 0" : "=v"(z));
+  return p[z];
+}
+
 inline int nblocks(int n, int wpb) { return (n + wpb - 1) / wpb; }
 
 // Column kernels: COLW cells (one wavefront each) per block.  The serial
@@ -153,7 +202,7 @@ __device__ __forceinline__ void stage_levels(float *s_lev, const float *const (&
 
 // ------------------------------------------------------------------ diagnostics
 KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
-  const int v = wave_entity();
+  const int v = wave_entity(I);
   if (v >= I.n.V) return;
   const int k = lane();
   int e[3];
@@ -248,7 +297,7 @@ __device__ __forceinline__ void store_cell_diag(const Instance &I, int c, int k,
 
 template <int NE>
 KB k_diag_cell(const Instance I, const float *__restrict__ u) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   store_cell_diag(I, c, k, diag_cell_compute<NE>(I, diag_cell_load<NE>(I, u, c, k)));
@@ -262,37 +311,70 @@ struct EdgeDiag {
 };
 template <int NE, bool RECON>
 struct EdgeDiagIn {
-  float un[RECON ? 2 * NE - 2 : 1], wt[RECON ? 2 * NE - 2 : 1];
-  float vrow, pv0, pv1, pc0, pc1, ue;
+  float un[RECON ? 2 * NE - 2 : 1];
+  SlotRow<2 * NE - 2> wt;
+  float vrow, pv0, pv1, pc0, pc1, ue, idv, idc;
   int n;
+  int v0, v1, c0, c1, nb[RECON ? 2 * NE - 2 : 1];  // scalar index loads
 };
-// every load of the edge diagnostics (pv_cell and u are read whether or not
-// APVM is on: both buffers always exist)
+// first phase: indices, slot counts and per-edge coefficients (none of
+// which needs another load first)
 template <int NE, bool RECON>
-__device__ __forceinline__ EdgeDiagIn<NE, RECON> diag_edge_load(const Instance &I, const float *__restrict__ u, int e,
-                                                                int k) {
+__device__ __forceinline__ void diag_edge_index(const Instance &I, int e, EdgeDiagIn<NE, RECON> &a) {
   constexpr int NX = 2 * NE - 2;
-  EdgeDiagIn<NE, RECON> a;
+  a.v0 = I.m.voe[2 * e];
+  a.v1 = I.m.voe[2 * e + 1];
+  a.c0 = I.m.coe[2 * e];
+  a.c1 = I.m.coe[2 * e + 1];
+  a.idv = uni_load(I.m.invdv + e);
+  a.idc = uni_load(I.m.invdc + e);
   if (RECON) {
-    int nb[NX];
 #pragma unroll
-    for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
-#pragma unroll
-    for (int s = 0; s < NX; ++s) a.un[s] = AT(u, nb[s], k);
+    for (int s = 0; s < NX; ++s) a.nb[s] = I.m.eoe[e * NX + s];
     a.n = I.m.neoe[e];
-    slots(a.wt, I.m.woe + e * NX);
+    a.wt = slot_row<NX>(I.m.woe + e * NX);
+  } else {
+    a.wt = {0.0f};
+    a.n = 0;
+  }
+}
+template <int NE, bool RECON>
+__device__ __forceinline__ void diag_edge_fence(const EdgeDiagIn<NE, RECON> &a) {
+  fence(a.v0);
+  fence(a.v1);
+  fence(a.c0);
+  fence(a.c1);
+  if (RECON) {
+#pragma unroll
+    for (int s = 0; s < 2 * NE - 2; ++s) fence(a.nb[s]);
+  }
+}
+// second phase: the rows (pv_cell and u are read whether or not APVM is on:
+// both buffers always exist)
+template <int NE, bool RECON>
+__device__ __forceinline__ void diag_edge_rows(const Instance &I, const float *__restrict__ u, int e, int k,
+                                               EdgeDiagIn<NE, RECON> &a) {
+  constexpr int NX = 2 * NE - 2;
+  if (RECON) {
+#pragma unroll
+    for (int s = 0; s < NX; ++s) a.un[s] = AT(u, a.nb[s], k);
     a.vrow = 0.0f;
   } else {
     a.vrow = AT(I.b.v, e, k);
-    a.n = 0;
   }
-  const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
-  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  a.pv0 = AT(I.b.pv_vertex, v0, k);
-  a.pv1 = AT(I.b.pv_vertex, v1, k);
-  a.pc0 = AT(I.b.pv_cell, c0, k);
-  a.pc1 = AT(I.b.pv_cell, c1, k);
+  a.pv0 = AT(I.b.pv_vertex, a.v0, k);
+  a.pv1 = AT(I.b.pv_vertex, a.v1, k);
+  a.pc0 = AT(I.b.pv_cell, a.c0, k);
+  a.pc1 = AT(I.b.pv_cell, a.c1, k);
   a.ue = AT(u, e, k);
+}
+template <int NE, bool RECON>
+__device__ __forceinline__ EdgeDiagIn<NE, RECON> diag_edge_load(const Instance &I, const float *__restrict__ u, int e,
+                                                                int k) {
+  EdgeDiagIn<NE, RECON> a;
+  diag_edge_index(I, e, a);
+  diag_edge_fence(a);
+  diag_edge_rows(I, u, e, k, a);
   return a;
 }
 template <int NE, bool RECON>
@@ -309,8 +391,8 @@ __device__ __forceinline__ EdgeDiag diag_edge_compute(const Instance &I, int e, 
   }
   const float pve = 0.5f * (a.pv0 + a.pv1);
   // computed either way (a select, not a branch, so the loads are not sunk)
-  const float gt = (a.pv1 - a.pv0) * I.m.invdv[e];
-  const float gn = (a.pc1 - a.pc0) * I.m.invdc[e];
+  const float gt = (a.pv1 - a.pv0) * a.idv;
+  const float gn = (a.pc1 - a.pc0) * a.idc;
   const float pva = pve - I.k.apvm_scale * (vv * gt + a.ue * gn);
   return {vv, I.k.apvm_on ? pva : pve};
 }
@@ -328,7 +410,7 @@ __device__ __forceinline__ void store_edge_diag(const Instance &I, int e, int k,
 // RECON: rk step 3, where v is reconstructed (cuda_horizontal_v841.py:1058-1097)
 template <int NE, bool RECON>
 KB k_diag_edge(const Instance I, const float *__restrict__ u) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   store_edge_diag(I, e, k, RECON, diag_edge_values<NE, RECON>(I, u, e, k));
@@ -341,7 +423,11 @@ struct EdgeFlux {
 template <int NE>
 struct EdgeFluxIn;
 template <int NE, bool CORR>
-__device__ __forceinline__ EdgeFluxIn<NE> tend_flux_load(const Instance &I, int e, int k);
+__device__ __forceinline__ void tend_flux_index(const Instance &I, int e, int k, EdgeFluxIn<NE> &b);
+template <int NE>
+__device__ __forceinline__ void tend_flux_fence(const EdgeFluxIn<NE> &b);
+template <int NE, bool CORR>
+__device__ __forceinline__ void tend_flux_rows(const Instance &I, int e, int k, EdgeFluxIn<NE> &b);
 template <int NE, bool CORR>
 __device__ __forceinline__ EdgeFlux tend_flux_compute(const Instance &I, int e, const EdgeFluxIn<NE> &b);
 template <bool CORR>
@@ -354,11 +440,20 @@ __device__ __forceinline__ void store_edge_flux(const Instance &I, int e, int k,
 // after rk steps 1 and 2 it is stage 2 or 3 (with the correction).
 template <int NE, bool RECON>
 KB k_diag_edge_flux(const Instance I, const float *__restrict__ u) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
-  const EdgeDiagIn<NE, RECON> a = diag_edge_load<NE, RECON>(I, u, e, k);
-  const EdgeFluxIn<NE> b = tend_flux_load<NE, !RECON>(I, e, k);
+  EdgeDiagIn<NE, RECON> a;
+  EdgeFluxIn<NE> b;
+  diag_edge_index(I, e, a);
+  tend_flux_index<NE, !RECON>(I, e, k, b);
+  diag_edge_fence(a);
+  tend_flux_fence(b);
+  diag_edge_rows(I, u, e, k, a);
+  tend_flux_rows<NE, !RECON>(I, e, k, b);
+  keep(b.thn[0]);
+  keep(b.wn[0]);
+  if (RECON) keep(a.un[0]);
   const EdgeDiag d = diag_edge_compute<NE, RECON>(I, e, a);
   const EdgeFlux f = tend_flux_compute<NE, !RECON>(I, e, b);
   store_edge_diag(I, e, k, RECON, d);
@@ -367,7 +462,7 @@ KB k_diag_edge_flux(const Instance I, const float *__restrict__ u) {
 
 // ---------------------------------------------------------- moist coefficients
 KB k_moist_cell(const Instance I) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const size_t plane = (size_t)I.n.C * HX_ROW;
@@ -382,7 +477,7 @@ KB k_moist_cell(const Instance I) {
 }
 
 KB k_moist_edge(const Instance I) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const size_t plane = (size_t)I.n.C * HX_ROW;
@@ -459,7 +554,7 @@ KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const 
       pa[j] = al;
       pb[j] = gm;
     }
-    if (singular && block_entity0<COLW>() + k < I.n.C) set_flag(I.b.flag);
+    if (singular && block_entity0(COLW) + k < I.n.C) set_flag(I.b.flag);
   }
   __syncthreads();
   const bool inner = k >= 1 && k < L;
@@ -564,7 +659,7 @@ __device__ __forceinline__ MixEdge mix_edge_value(const Instance &I, int e, int 
 // divergence of delsq_u (laplacian_divergence_v841_f32)
 template <int NE>
 KB k_mix_lapdiv(const Instance I) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int n = I.m.neoc[c];
@@ -584,7 +679,7 @@ KB k_mix_lapdiv(const Instance I) {
 
 // vorticity of delsq_u (laplacian_vorticity_v841_f32)
 KB k_mix_lapvort(const Instance I) {
-  const int v = wave_entity();
+  const int v = wave_entity(I);
   if (v >= I.n.V) return;
   const int k = lane();
   int e[3];
@@ -602,7 +697,7 @@ KB k_mix_lapvort(const Instance I) {
 
 // del4 of u (momentum_filter_lap4_v841_f32): pool - rho_e*((ddiv*r_dc) - (dvort*r_dv))
 KB k_mix_edge4(const Instance I) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
@@ -621,7 +716,7 @@ KB k_mix_edge4(const Instance I) {
 // Without the fourth-order part the results go straight to the Euler pools.
 template <int NE>
 KB k_mix_cell(const Instance I) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
@@ -689,7 +784,7 @@ KB k_mix_cell(const Instance I) {
 // del4 of theta and of w (theta_filter_lap4_v841_f32, w_filter_lap4_v841_f32), then the w pool
 template <int NE>
 KB k_mix_cell4(const Instance I) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
@@ -727,7 +822,7 @@ KB k_mix_cell4(const Instance I) {
 // MIX: also the 2-D Smagorinsky kdiff of the cell
 template <int NE, bool MIX>
 KB k_sub_cell(const Instance I) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
@@ -767,7 +862,7 @@ KB k_sub_cell(const Instance I) {
 // pool (mixing.py del2 u term, in the same pass; kdiff is final by then)
 template <bool MIX>
 KB k_sub_edge(const Instance I) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
@@ -798,8 +893,13 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   int nb[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
+  const int nn = I.m.neoe[e];
+  const SlotRow<NX> wt = slot_row<NX>(I.m.woe + e * NX), ca = slot_row<NX>(I.m.ecos + e * NX),
+                    sa = slot_row<NX>(I.m.esin + e * NX);
+  const float fe = uni_load(I.m.fedge + e), idc = uni_load(I.m.invdc + e);
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k), rdzw = AT(I.z.rdzw, 0, k);
+  const float ui = AT(I.z.uinit, 0, k), vi = AT(I.z.vinit, 0, k);
   const float u_k = AT(I.b.u, e, k);
-  const float u_m1 = lane_off(u_k, k, -1), u_m2 = lane_off(u_k, k, -2), u_p1 = lane_off(u_k, k, 1);
   const float rw0 = AT(I.b.c_rw, c0, k), rw1 = AT(I.b.c_rw, c1, k);
   const float pve = AT(I.b.pv_edge, e, k);
   float un[NX], pvn[NX];
@@ -812,9 +912,8 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   const float ke0 = AT(I.b.ke, c0, k), ke1 = AT(I.b.ke, c1, k);
   const float hd0 = AT(I.b.hdiv, c0, k), hd1 = AT(I.b.hdiv, c1, k);
   const float eru = AT(I.b.euler_ru, e, k);
-  const float pru = I.k.physics ? AT(I.b.phys_ru, e, k) : 0.0f;
-  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k), rdzw = AT(I.z.rdzw, 0, k);
-  const float ui = AT(I.z.uinit, 0, k), vi = AT(I.z.vinit, 0, k);
+  const float pru = I.k.physics ? AT(I.b.phys_ru, e, k) : 0.0f;  // the buffer exists with physics only
+  const float u_m1 = lane_off(u_k, k, -1), u_m2 = lane_off(u_k, k, -2), u_p1 = lane_off(u_k, k, 1);
   const float rwe = 0.5f * (rw0 + rw1);
   float wduz = 0.0f;
   if (k == 1 || k == L - 1) {
@@ -824,11 +923,6 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   }
   const float wduz_p = dn1(wduz);
   const float initial = (-rdzw) * (wduz_p - wduz);
-  const int nn = I.m.neoe[e];
-  float wt[NX], ca[NX], sa[NX];
-  slots(wt, I.m.woe + e * NX);
-  slots(ca, I.m.ecos + e * NX);
-  slots(sa, I.m.esin + e * NX);
   float qq = 0.0f;
 #pragma unroll
   for (int s = 0; s < NX; ++s)
@@ -836,7 +930,6 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
       const float work_pv = 0.5f * (pve + pvn[s]);
       qq = qq + (wt[s] * un[s]) * work_pv;
     }
-  const float fe = I.m.fedge[e];
 #pragma unroll
   for (int s = 0; s < NX; ++s)
     if (s < nn) {
@@ -844,7 +937,7 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
       qq = qq - (wt[s] * ref_u) * fe;
     }
   const float rho_edge = 0.5f * (rho0 + rho1);
-  const float horiz = rho_edge * (qq - (ke1 - ke0) * I.m.invdc[e]);
+  const float horiz = rho_edge * (qq - (ke1 - ke0) * idc);
   float t = (initial + horiz) - (u_k * 0.5f) * (hd0 + hd1);
   // upper-level Rayleigh damping of u, before the Euler pool is added
   // (native/upper_damping.rs:18-31; cuda_upper_damping_v841.py:12-18)
@@ -857,7 +950,7 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
 
 template <int NE>
 KB k_tend_mom(const Instance I) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   tend_mom_edge<NE>(I, e, lane());
 }
@@ -869,7 +962,7 @@ KB k_tend_mom(const Instance I) {
 // ru_p, with one the recovery reads ru_p itself).
 template <int NE>
 KB k_tend_mom_ac(const Instance I, float dts, int store_avg) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const float tr = tend_mom_edge<NE>(I, e, k);
@@ -886,35 +979,55 @@ KB k_tend_mom_ac(const Instance I, float dts, int store_avg) {
 // CORR: stages 2 and 3, which add the theta RK correction flux
 template <int NE>
 struct EdgeFluxIn {
-  float thn[2 * NE - 2], wn[2 * NE - 2], av[2 * NE - 2], a3v[2 * NE - 2];
-  float rue, sru, sth0, sth1, fzm, fzp;
+  float thn[2 * NE - 2], wn[2 * NE - 2];
+  SlotRow<2 * NE - 2> av, a3v;
+  float rue, sru, sth0, sth1, fzm, fzp, dv;
   int na;
+  int c0, c1, ac[2 * NE - 2];  // scalar index loads
 };
 template <int NE, bool CORR>
-__device__ __forceinline__ EdgeFluxIn<NE> tend_flux_load(const Instance &I, int e, int k) {
+__device__ __forceinline__ void tend_flux_index(const Instance &I, int e, int k, EdgeFluxIn<NE> &b) {
   constexpr int NX = 2 * NE - 2;
-  EdgeFluxIn<NE> b;
-  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  int ac[NX];
+  b.c0 = I.m.coe[2 * e];
+  b.c1 = I.m.coe[2 * e + 1];
 #pragma unroll
-  for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
+  for (int s = 0; s < NX; ++s) b.ac[s] = I.m.advc[e * NX + s];
+  b.na = I.m.nadv[e];
+  b.av = slot_row<NX>(I.m.adv + e * NX);
+  b.a3v = slot_row<NX>(I.m.adv3 + e * NX);
+  b.dv = CORR ? uni_load(I.m.dv + e) : 0.0f;
+  b.fzm = AT(I.z.fzm, 0, k);
+  b.fzp = AT(I.z.fzp, 0, k);
+}
+template <int NE>
+__device__ __forceinline__ void tend_flux_fence(const EdgeFluxIn<NE> &b) {
+  fence(b.c0);
+  fence(b.c1);
+#pragma unroll
+  for (int s = 0; s < 2 * NE - 2; ++s) fence(b.ac[s]);
+}
+template <int NE, bool CORR>
+__device__ __forceinline__ void tend_flux_rows(const Instance &I, int e, int k, EdgeFluxIn<NE> &b) {
+  constexpr int NX = 2 * NE - 2;
 #pragma unroll
   for (int s = 0; s < NX; ++s) {
-    b.thn[s] = AT(I.b.theta, ac[s], k);
-    b.wn[s] = AT(I.b.w, ac[s], k);
+    b.thn[s] = AT(I.b.theta, b.ac[s], k);
+    b.wn[s] = AT(I.b.w, b.ac[s], k);
   }
   b.rue = AT(I.b.c_ru, e, k);
   b.sru = b.sth0 = b.sth1 = 0.0f;
   if (CORR) {
     b.sru = AT(I.b.s_ru, e, k);
-    b.sth0 = AT(I.b.s_theta, c0, k);
-    b.sth1 = AT(I.b.s_theta, c1, k);
+    b.sth0 = AT(I.b.s_theta, b.c0, k);
+    b.sth1 = AT(I.b.s_theta, b.c1, k);
   }
-  b.fzm = AT(I.z.fzm, 0, k);
-  b.fzp = AT(I.z.fzp, 0, k);
-  b.na = I.m.nadv[e];
-  slots(b.av, I.m.adv + e * NX);
-  slots(b.a3v, I.m.adv3 + e * NX);
+}
+template <int NE, bool CORR>
+__device__ __forceinline__ EdgeFluxIn<NE> tend_flux_load(const Instance &I, int e, int k) {
+  EdgeFluxIn<NE> b;
+  tend_flux_index<NE, CORR>(I, e, k, b);
+  tend_flux_fence(b);
+  tend_flux_rows<NE, CORR>(I, e, k, b);
   return b;
 }
 template <int NE, bool CORR>
@@ -934,7 +1047,7 @@ __device__ __forceinline__ EdgeFlux tend_flux_compute(const Instance &I, int e, 
     }
   EdgeFlux f;
   f.th = b.rue * et;
-  f.corr = CORR ? ((I.m.dv[e] * (b.sru - b.rue)) * 0.5f) * (b.sth1 + b.sth0) : 0.0f;
+  f.corr = CORR ? ((b.dv * (b.sru - b.rue)) * 0.5f) * (b.sth1 + b.sth0) : 0.0f;
   f.w = rui * ew;
   return f;
 }
@@ -953,45 +1066,56 @@ __device__ __forceinline__ void store_edge_flux(const Instance &I, int e, int k,
 
 template <int NE, bool CORR>
 KB k_tend_flux(const Instance I) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   store_edge_flux<CORR>(I, e, k, tend_flux_values<NE, CORR>(I, e, k));
 }
 
-template <int NE>
-KB k_tend_cell(const Instance I, int stage) {
-  const int c = wave_entity();
+// CORR: stages 2 and 3 (the theta RK correction flux); MIX: horizontal
+// mixing on (its theta increment is added)
+template <int NE, bool CORR, bool MIX>
+KB k_tend_cell(const Instance I) {
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
   const float coef = I.k.coef3;
   const int n = I.m.neoc[c];
-  float sg[NE];
-  slots(sg, I.m.csign + c * NE);
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  const SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
+  const float ia = uni_load(I.m.invarea + c);
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
+  const float rdzw = AT(I.z.rdzw, 0, k), rdzu = AT(I.z.rdzu, 0, k);
+  // the column rows, read once; vertical neighbours by shuffles
+  const float th_k = AT(I.b.theta, c, k), ths_k = AT(I.b.s_theta, c, k);
+  const float rwc_k = AT(I.b.c_rw, c, k), rws_k = AT(I.b.s_rw, c, k), ww_k = AT(I.b.w, c, k);
+  const float zz_k = AT(I.z.zz, c, k), erw_k = AT(I.b.euler_rw, c, k);
+  const float eth_k = MIX ? AT(I.b.euler_theta, c, k) : 0.0f;
   float fth[NE], fco[NE], fwv[NE], tru[NE], zb[NE], zb3[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    zb[s] = AT(I.z.zb, c * NE + s, k);
+    zb3[s] = AT(I.z.zb3, c * NE + s, k);
+  }
+#pragma unroll
+  for (int s = 0; s < NE; ++s) fence(e[s]);
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     fth[s] = AT(I.b.fth, e[s], k);
     fwv[s] = AT(I.b.fw, e[s], k);
     tru[s] = AT(I.b.tend_ru, e[s], k);
-    zb[s] = AT(I.z.zb, c * NE + s, k);
-    zb3[s] = AT(I.z.zb3, c * NE + s, k);
+    fco[s] = CORR ? AT(I.b.fcorr, e[s], k) : 0.0f;
   }
-  if (stage > 1) {
-#pragma unroll
-    for (int s = 0; s < NE; ++s) fco[s] = AT(I.b.fcorr, e[s], k);
-  }
-  // the column rows, read once; vertical neighbours by shuffles
-  const float th_k = AT(I.b.theta, c, k), ths_k = AT(I.b.s_theta, c, k);
-  const float rwc_k = AT(I.b.c_rw, c, k), rws_k = AT(I.b.s_rw, c, k), ww_k = AT(I.b.w, c, k);
-  const float zz_k = AT(I.z.zz, c, k), erw_k = AT(I.b.euler_rw, c, k);
-  const float eth_k = I.k.mixing ? AT(I.b.euler_theta, c, k) : 0.0f;
-  const float prt_k = I.k.physics ? AT(I.b.phys_rt, c, k) : 0.0f;
-  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
+  const float prt_k = I.k.physics ? AT(I.b.phys_rt, c, k) : 0.0f;  // the buffer exists with physics only
+  keep(fth[0]);
+  keep(fwv[0]);
+  keep(tru[0]);
+  if (CORR) keep(fco[0]);
+  float sg[NE];
+  slot_vals(sgr, sg);
   const float th_m1 = lane_off(th_k, k, -1), th_m2 = lane_off(th_k, k, -2), th_p1 = lane_off(th_k, k, 1);
   const float ths_m1 = lane_off(ths_k, k, -1), rwc_m1 = lane_off(rwc_k, k, -1);
   const float ww_m1 = lane_off(ww_k, k, -1), ww_m2 = lane_off(ww_k, k, -2), ww_p1 = lane_off(ww_k, k, 1);
@@ -1014,18 +1138,17 @@ KB k_tend_cell(const Instance I, int stage) {
     wf = flux3(ww_m2, ww_m1, ww_k, ww_p1, 0.5f * (rwc_k + rwc_m1), 1.0f);
   }
   const float wf_p = dn1(wf);
-  const float ia = I.m.invarea[c];
   float r = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) r = r - sg[s] * fth[s];
-  if (stage > 1) {
+  if (CORR) {
 #pragma unroll
     for (int s = 0; s < NE; ++s)
       if (s < n) r = r - sg[s] * fco[s];
   }
   r = r * ia;
-  const float trt = r - AT(I.z.rdzw, 0, k) * (vf_p - vf);
+  const float trt = r - rdzw * (vf_p - vf);
   float tru_m[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) tru_m[s] = up1(tru[s]);
@@ -1036,7 +1159,7 @@ KB k_tend_cell(const Instance I, int stage) {
     for (int s = 0; s < NE; ++s)
       if (s < n) q = q - sg[s] * fwv[s];
     q = q * ia;
-    q = q - AT(I.z.rdzu, 0, k) * (wf_p - wf);
+    q = q - rdzu * (wf_p - wf);
     q = q + erw_k;
 #pragma unroll
     for (int s = 0; s < NE; ++s)
@@ -1048,7 +1171,7 @@ KB k_tend_cell(const Instance I, int stage) {
     rw_t = q * (fzm * zz_k + fzp * zz_m1);
   }
   if (k < L) {
-    const float tmix = I.k.mixing ? trt + eth_k : trt;
+    const float tmix = MIX ? trt + eth_k : trt;
     AT(I.b.tend_rt, c, k) = I.k.physics ? tmix + prt_k : tmix;
   }
   if (k <= L) AT(I.b.tend_rw, c, k) = rw_t;
@@ -1070,7 +1193,7 @@ __device__ __forceinline__ float damped_ru_p(const Instance &I, int e, int k, fl
 // the first, where it equals the undamped ru_p read here (p1 + p2, the same
 // operands in the same order as ru_avg + p).  Loads first, stores last.
 KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_coef, int avg_from_rp) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
@@ -1224,7 +1347,7 @@ KBCOL k_ac_cell(const Instance I, float dts) {
 }
 
 KB k_divdamp(const Instance I, float coefficient) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
@@ -1238,31 +1361,36 @@ template <int NE, class RU>
 __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int k, float rho_k, float rw_k, RU ru_row,
                                                   float *hdiv_out = nullptr) {
   const int L = I.n.L;
-  const float zk = AT(I.z.zz, c, k);
-  const float zkm = lane_off(zk, k, -1);
-  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
   const int n = I.m.neoc[c];
-  float sg[NE];
-  slots(sg, I.m.csign + c * NE);
-  float sd[NE];
-  slots(sd, I.m.csdv + c * NE);
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  const SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
+  const SlotRow<NE> sd = slot_row<NE>(I.m.csdv + c * NE);
+  const float ia = uni_load(I.m.invarea + c);
+  const float zk = AT(I.z.zz, c, k);
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
   float ru[NE], zb[NE], zb3[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    ru[s] = ru_row(e[s]);
     zb[s] = AT(I.z.zb, c * NE + s, k);
     zb3[s] = AT(I.z.zb3, c * NE + s, k);
   }
+#pragma unroll
+  for (int s = 0; s < NE; ++s) fence(e[s]);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) ru[s] = ru_row(e[s]);
+  keep(ru[0]);
+  float sg[NE];
+  slot_vals(sgr, sg);
+  const float zkm = lane_off(zk, k, -1);
   if (hdiv_out) {
     // mass-flux divergence of the recovered ru (dynamics.py:86-121), next stage's hdiv
     float acc = 0.0f;
 #pragma unroll
     for (int s = 0; s < NE; ++s)
       if (s < n) acc = acc + sd[s] * ru[s];
-    *hdiv_out = acc * I.m.invarea[c];
+    *hdiv_out = acc * ia;
   }
   float col = 0.0f;
   if (k >= 1 && k < L) col = rw_k / (fzm * zk + fzp * zkm);
@@ -1299,7 +1427,7 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
 // of the pass is in flight together; stores come last.
 template <int NE, bool S3>
 KB k_rec_cell(const Instance I, int sub, float inv_steps) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
@@ -1366,7 +1494,7 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps) {
 // first, stores last.
 template <bool S3, bool DAMP, bool AVGRP>
 KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
@@ -1396,7 +1524,7 @@ KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef) {
 // rebuild of the saved diagnostics from a prognostic state (driver.py:2655-2700)
 template <int NE>
 KB k_rebuild_cell(const Instance I) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
@@ -1420,7 +1548,7 @@ KB k_rebuild_cell(const Instance I) {
 }
 
 KB k_rebuild_edge(const Instance I) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
@@ -1444,7 +1572,7 @@ __device__ __forceinline__ float vflux(float q0, float q1, float q2, float q3, f
 template <int NE>
 KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__ ev) {
   constexpr int NX = 2 * NE - 2;
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const int NT = I.n.NT;
@@ -1486,7 +1614,7 @@ KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__
 template <int NE>
 KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__restrict__ ev, float *__restrict__ out,
              float dt, float wn, const float *__restrict__ src) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L, NT = I.n.NT;
@@ -1546,7 +1674,7 @@ KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__rest
 // old mixing ratio with the held scalar tendency, old + dt*source/rho_old
 // (transport.py:1906), every species of cell c
 KB k_fct_source(const Instance I, float dt) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   if (k >= I.n.L) return;
@@ -1559,7 +1687,7 @@ KB k_fct_source(const Instance I, float dt) {
 
 template <int NE>
 KB k_fct_target(const Instance I, float dt) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
@@ -1589,7 +1717,7 @@ KB k_fct_target(const Instance I, float dt) {
 template <int NE>
 KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt, const float *__restrict__ qold, float qadd) {
   constexpr int NX = 2 * NE - 2;
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const int NT = I.n.NT;
@@ -1656,7 +1784,7 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt, const f
 
 template <int NE>
 KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const float *__restrict__ qold, float qadd) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
@@ -1747,7 +1875,7 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const f
 }
 
 KB k_fct_edge2(const Instance I) {
-  const int e = wave_entity();
+  const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
@@ -1763,7 +1891,7 @@ KB k_fct_edge2(const Instance I) {
 
 template <int NE>
 KB k_fct_cell2(const Instance I, float *__restrict__ out) {
-  const int c = wave_entity();
+  const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
@@ -1799,7 +1927,7 @@ KB k_fct_cell2(const Instance I, float *__restrict__ out) {
 
 // q < 0 -> 0 as a where-assignment (-0.0 and NaN keep their bits); src may be dst
 KB k_clamp(const Instance I, const float *src, float *dst, int rows) {
-  const int r = wave_entity();
+  const int r = wave_entity(I);
   if (r >= rows) return;
   const int k = lane();
   const float x = AT(src, r, k);
@@ -1818,10 +1946,15 @@ inline int launch_wpb(const char *name, const A0 &a0, const A &...) {
   return wpb_from(name, a0);
 }
 
+template <class T>
+inline void set_wpb(T &, int) {}
+inline void set_wpb(Instance &I, int wpb) { I.n.WPB = wpb; }
+
 template <typename K, typename... A>
 inline void launch_named(const char *name, K kernel, int n, int gy, hipStream_t s, A... args) {
   if (n <= 0 || gy <= 0) return;
   const int wpb = launch_wpb(name, args...);
+  (set_wpb(args, wpb), ...);  // the launch's own copy of the Instance carries its tile
   if (g_prof_on) prof_mark(name, s, 1);
   hipLaunchKernelGGL(kernel, dim3(nblocks(n, wpb), gy), dim3(64 * wpb), 0, s, args...);
   if (g_prof_on) prof_mark(name, s, 0);
@@ -1930,7 +2063,16 @@ int dyn_tend(const Instance &I, int stage, hipStream_t s) {
       NEDISPATCH(I, launch_named("k_tend_flux", k_tend_flux<6, false>, I.n.E, 1, s, I),
                  launch_named("k_tend_flux", k_tend_flux<7, false>, I.n.E, 1, s, I));
   }
-  LAUNCH_NE(I, k_tend_cell, I.n.C, 1, s, I, stage);
+  {
+    const bool corr = stage > 1, mix = I.k.mixing != 0;
+#define TEND_CELL(NE, CO, MX) launch_named("k_tend_cell", k_tend_cell<NE, CO, MX>, I.n.C, 1, s, I)
+#define TEND_CELL_NE(NE)                           \
+  (corr ? (mix ? TEND_CELL(NE, true, true) : TEND_CELL(NE, true, false)) \
+        : (mix ? TEND_CELL(NE, false, true) : TEND_CELL(NE, false, false)))
+    NEDISPATCH(I, TEND_CELL_NE(6), TEND_CELL_NE(7));
+#undef TEND_CELL_NE
+#undef TEND_CELL
+  }
   return check();
 }
 
