@@ -140,7 +140,7 @@ __device__ __forceinline__ void slots(T (&out)[N], const T *__restrict__ p) {
 // that needs slots under a lane condition takes them with slot_vals first.
 template <int N>
 struct SlotRow {
-  float v;
+  float v;  // lane s: slot s
   __device__ __forceinline__ float operator[](int s) const { return rdl(v, s); }
 };
 template <int N>
@@ -150,7 +150,7 @@ __device__ __forceinline__ SlotRow<N> slot_row(const float *__restrict__ p) {
 }
 // every slot as a wave-uniform value (call with all lanes active)
 template <int N>
-__device__ __forceinline__ void slot_vals(const SlotRow<N> &r, float (&o)[N]) {
+__device__ __forceinline__ void slot_vals(SlotRow<N> &r, float (&o)[N]) {
 #pragma unroll
   for (int s = 0; s < N; ++s) o[s] = r[s];
 }
@@ -158,6 +158,32 @@ __device__ __forceinline__ void slot_vals(const SlotRow<N> &r, float (&o)[N]) {
 // the first (uniform) branch that uses it, which would issue it, and the
 // scalar index load it needs, one round trip after the others.
 __device__ __forceinline__ void keep(float &x) { asm volatile("" : "+v"(x)); }
+template <int N>
+__device__ __forceinline__ void keep(float (&x)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) keep(x[i]);
+}
+template <int G, int N>
+__device__ __forceinline__ void keep(float (&x)[G][N]) {
+#pragma unroll
+  for (int g = 0; g < G; ++g) keep(x[g]);
+}
+template <class... T>
+__device__ __forceinline__ void keep(T &...x) {
+  (keep(x), ...);
+}
+// Marks slot rows as read only after this point: called once the
+// wavefront's row loads are issued, so the lane reads of the slots (which
+// wait for the slot loads) are not scheduled ahead of the row loads.  Call
+// with all lanes active.  Emits nothing.
+template <class R>
+__device__ __forceinline__ void settle1(R &row) {
+  asm volatile("" : "+v"(row.v));
+}
+template <class... R>
+__device__ __forceinline__ void settle(R &...rows) {
+  (settle1(rows), ...);
+}
 // Ends the index phase of a wavefront: every scalar index load issued before
 // the first fence completes in one wait, before any row load after it is
 // issued.  Without it the compiler may issue one index table's loads only
@@ -209,18 +235,23 @@ KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
   float ue[3];
 #pragma unroll
   for (int s = 0; s < 3; ++s) e[s] = I.m.eov[3 * v + s];
+  SlotRow<3> dc = slot_row<3>(I.m.vdc + 3 * v), dcdv = slot_row<3>(I.m.vdcdv + 3 * v);
+  const float it = uni_load(I.m.invtri + v), fv = uni_load(I.m.fvert + v);
+#pragma unroll
+  for (int s = 0; s < 3; ++s) fence(e[s]);
 #pragma unroll
   for (int s = 0; s < 3; ++s) ue[s] = AT(u, e[s], k);
+  keep(ue, it, fv);
+  settle(dc, dcdv);
   float vort = 0.0f, kev = 0.0f;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    vort = vort + I.m.vdc[3 * v + s] * ue[s];
-    kev = kev + I.m.vdcdv[3 * v + s] * (ue[s] * ue[s]);
+    vort = vort + dc[s] * ue[s];
+    kev = kev + dcdv[s] * (ue[s] * ue[s]);
   }
-  const float it = I.m.invtri[v];
   vort = vort * it;
   kev = kev * (0.25f * it);
-  const float pvv = I.m.fvert[v] + vort;
+  const float pvv = fv + vort;
   if (k < I.n.L) {
     AT(I.b.vorticity, v, k) = vort;
     AT(I.b.ke_vertex, v, k) = kev;
@@ -234,7 +265,8 @@ struct CellDiag {
 };
 template <int NE>
 struct CellDiagIn {
-  float ue[NE], kv[NE], pv[NE], sd[NE], dd[NE], kt[NE];
+  float ue[NE], kv[NE], pv[NE];
+  SlotRow<NE> sd, dd, kt;
   float ia;
   int n;
 };
@@ -243,12 +275,20 @@ __device__ __forceinline__ CellDiagIn<NE> diag_cell_load(const Instance &I, cons
                                                          int k) {
   CellDiagIn<NE> a;
   a.n = I.m.neoc[c];
-  a.ia = I.m.invarea[c];
   int e[NE], vv[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     e[s] = I.m.eoc[c * NE + s];
     vv[s] = I.m.voc[c * NE + s];
+  }
+  a.sd = slot_row<NE>(I.m.csdv + c * NE);
+  a.dd = slot_row<NE>(I.m.cdcdv + c * NE);
+  a.kt = slot_row<NE>(I.m.ckite + c * NE);
+  a.ia = uni_load(I.m.invarea + c);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    fence(e[s]);
+    fence(vv[s]);
   }
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
@@ -256,9 +296,8 @@ __device__ __forceinline__ CellDiagIn<NE> diag_cell_load(const Instance &I, cons
     a.kv[s] = AT(I.b.ke_vertex, vv[s], k);
     a.pv[s] = AT(I.b.pv_vertex, vv[s], k);
   }
-  slots(a.sd, I.m.csdv + c * NE);
-  slots(a.dd, I.m.cdcdv + c * NE);
-  slots(a.kt, I.m.ckite + c * NE);
+  keep(a.ue, a.kv, a.pv, a.ia);
+  settle(a.sd, a.dd, a.kt);
   return a;
 }
 template <int NE>
@@ -375,6 +414,7 @@ __device__ __forceinline__ EdgeDiagIn<NE, RECON> diag_edge_load(const Instance &
   diag_edge_index(I, e, a);
   diag_edge_fence(a);
   diag_edge_rows(I, u, e, k, a);
+  settle(a.wt);
   return a;
 }
 template <int NE, bool RECON>
@@ -451,9 +491,9 @@ KB k_diag_edge_flux(const Instance I, const float *__restrict__ u) {
   tend_flux_fence(b);
   diag_edge_rows(I, u, e, k, a);
   tend_flux_rows<NE, !RECON>(I, e, k, b);
-  keep(b.thn[0]);
-  keep(b.wn[0]);
-  if (RECON) keep(a.un[0]);
+  keep(b.thn, b.wn, b.rue, b.sru, b.sth0, b.sth1, b.fzm, b.fzp, b.dv);
+  keep(a.un, a.vrow, a.pv0, a.pv1, a.pc0, a.pc1, a.ue, a.idv, a.idc);
+  settle(a.wt, b.av, b.a3v);
   const EdgeDiag d = diag_edge_compute<NE, RECON>(I, e, a);
   const EdgeFlux f = tend_flux_compute<NE, !RECON>(I, e, b);
   store_edge_diag(I, e, k, RECON, d);
@@ -600,6 +640,8 @@ __device__ __forceinline__ float cell_flux_div(const Instance &I, int c, int k, 
 // current rho_zz, as the subcycle-start solve formed it.
 
 // eddy viscosity from the deformation tensor (smagorinsky_v841_f32)
+__device__ __forceinline__ float kdiff_from_deformation(const Instance &I, int k, float dudx, float dudy, float dvdx,
+                                                       float dvdy);
 // 2-D Smagorinsky viscosity of cell c at level k (lane k), from the
 // deformation of the cell's edge winds
 template <int NE>
@@ -626,6 +668,11 @@ __device__ __forceinline__ float mix_kdiff_value(const Instance &I, int c, int k
       dvdx = (dvdx + cs[s] * un[s]) + c2[s] * vn[s];
       dvdy = (dvdy + s2[s] * un[s]) + cs[s] * vn[s];
     }
+  return kdiff_from_deformation(I, k, dudx, dudy, dvdx, dvdy);
+}
+// eddy viscosity of level k from the deformation sums of a cell
+__device__ __forceinline__ float kdiff_from_deformation(const Instance &I, int k, float dudx, float dudy, float dvdx,
+                                                       float dvdy) {
   const float d11 = 2.0f * dudx, d22 = 2.0f * dvdy, d12 = dudy + dvdx;
   const float diff = d11 - d22;
   const float strain = __builtin_sqrtf(0.25f * (diff * diff) + d12 * d12);
@@ -648,10 +695,11 @@ __device__ __forceinline__ MixEdge mix_edge_value(const Instance &I, int e, int 
   const float z0 = AT(I.b.vorticity, v0, k), z1 = AT(I.b.vorticity, v1, k);
   const float kd0 = AT(I.b.kdiff, c0, k), kd1 = AT(I.b.kdiff, c1, k);
   const float r0 = AT(I.b.c_rho, c0, k), r1 = AT(I.b.c_rho, c1, k);
-  const float lap = (d1 - d0) * I.m.mx_idc[e] - (z1 - z0) * I.m.mx_idv[e];
+  const float idc = uni_load(I.m.mx_idc + e), idv = uni_load(I.m.mx_idv + e), del2 = uni_load(I.m.mx_del2 + e);
+  const float lap = (d1 - d0) * idc - (z1 - z0) * idv;
   const float kedge = 0.5f * (kd0 + kd1);
   const float rho_e = 0.5f * (r0 + r1);
-  return {lap, ((rho_e * kedge) * lap) * I.m.mx_del2[e]};
+  return {lap, ((rho_e * kedge) * lap) * del2};
 }
 
 
@@ -667,9 +715,14 @@ KB k_mix_lapdiv(const Instance I) {
   float x[NE], f[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  SlotRow<NE> fr = slot_row<NE>(I.m.mx_fdiv + c * NE);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) fence(e[s]);
 #pragma unroll
   for (int s = 0; s < NE; ++s) x[s] = AT(I.b.delsq_u, e[s], k);
-  slots(f, I.m.mx_fdiv + c * NE);
+  keep(x);
+  settle(fr);
+  slot_vals(fr, f);
   float acc = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
@@ -686,9 +739,14 @@ KB k_mix_lapvort(const Instance I) {
   float x[3], f[3];
 #pragma unroll
   for (int s = 0; s < 3; ++s) e[s] = I.m.eov[3 * v + s];
+  SlotRow<3> fr = slot_row<3>(I.m.mx_fvort + 3 * v);
+#pragma unroll
+  for (int s = 0; s < 3; ++s) fence(e[s]);
 #pragma unroll
   for (int s = 0; s < 3; ++s) x[s] = AT(I.b.delsq_u, e[s], k);
-  slots(f, I.m.mx_fvort + 3 * v);
+  keep(x);
+  settle(fr);
+  slot_vals(fr, f);
   float acc = 0.0f;
 #pragma unroll
   for (int s = 0; s < 3; ++s) acc = acc + f[s] * x[s];
@@ -705,16 +763,17 @@ KB k_mix_edge4(const Instance I) {
   const float d0 = AT(I.b.delsq_div, c0, k), d1 = AT(I.b.delsq_div, c1, k);
   const float z0 = AT(I.b.delsq_vort, v0, k), z1 = AT(I.b.delsq_vort, v1, k);
   const float r0 = AT(I.b.c_rho, c0, k), r1 = AT(I.b.c_rho, c1, k);
+  const float eru = AT(I.b.euler_ru, e, k);
+  const float rdc = uni_load(I.m.mx_rdc + e), rdv = uni_load(I.m.mx_rdv + e);
   const float rho_e = 0.5f * (r0 + r1);
-  const float diffusion = rho_e * ((d1 - d0) * I.m.mx_rdc[e] - (z1 - z0) * I.m.mx_rdv[e]);
-  if (k < I.n.L) {
-    AT(I.b.euler_ru, e, k) = AT(I.b.euler_ru, e, k) - diffusion;
-  }
+  const float diffusion = rho_e * ((d1 - d0) * rdc - (z1 - z0) * rdv);
+  if (k < I.n.L) AT(I.b.euler_ru, e, k) = eru - diffusion;
 }
 
 // del2 of theta and of w (theta_filter_lap2_v841_f32, w_filter_lap2_v841_f32).
-// Without the fourth-order part the results go straight to the Euler pools.
-template <int NE>
+// Without the fourth-order part (MIX4 false) the results go straight to the
+// Euler pools.
+template <int NE, bool MIX4>
 KB k_mix_cell(const Instance I) {
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
@@ -727,22 +786,31 @@ KB k_mix_cell(const Instance I) {
     e[s] = I.m.eoc[c * NE + s];
     o[s] = I.m.ocell[c * NE + s];
   }
-  float tho[NE], rho[NE], wo[NE], kdo[NE];
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE), f2r = slot_row<NE>(I.m.mx_fth2 + c * NE),
+                    fwr = slot_row<NE>(I.m.mx_fw2 + c * NE);
+  const float thc = AT(I.b.theta, c, k), rhc = AT(I.b.c_rho, c, k), wc = AT(I.b.w, c, k);
+  const float kdc = AT(I.b.kdiff, c, k);
+  const float erw = MIX4 ? 0.0f : AT(I.b.euler_rw, c, k);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    fence(e[s]);
+    fence(o[s]);
+  }
+  float tho[NE], rho[NE], wo[NE], kdo[NE], d2[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     tho[s] = AT(I.b.theta, o[s], k);
     rho[s] = AT(I.b.c_rho, o[s], k);
     wo[s] = AT(I.b.w, o[s], k);
     kdo[s] = AT(I.b.kdiff, o[s], k);
+    d2[s] = uni_load(I.m.mx_del2 + e[s]);
   }
-  const float thc = AT(I.b.theta, c, k), rhc = AT(I.b.c_rho, c, k), wc = AT(I.b.w, c, k);
-  const float kdc = AT(I.b.kdiff, c, k);
-  float sg[NE], f2[NE], fw[NE], d2[NE];
-  slots(sg, I.m.csign + c * NE);
-  slots(f2, I.m.mx_fth2 + c * NE);
-  slots(fw, I.m.mx_fw2 + c * NE);
-#pragma unroll
-  for (int s = 0; s < NE; ++s) d2[s] = I.m.mx_del2[e[s]];
+  keep(tho, rho, wo, kdo, d2);
+  settle(sgr, f2r, fwr);
+  float sg[NE], f2[NE], fw[NE];
+  slot_vals(sgr, sg);
+  slot_vals(f2r, f2);
+  slot_vals(fwr, fw);
   float lap = 0.0f, tend = 0.0f, lapw = 0.0f, tendw = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
@@ -774,10 +842,10 @@ KB k_mix_cell(const Instance I) {
     AT(I.b.euler_theta, c, k) = tend;
     AT(I.b.delsq_w, c, k) = lapw;
   }
-  if (I.k.mix4) {
+  if (MIX4) {
     if (k <= L) AT(I.b.mix_tw, c, k) = tendw;
   } else if (k <= L) {
-    AT(I.b.euler_rw, c, k) = AT(I.b.euler_rw, c, k) + tendw;
+    AT(I.b.euler_rw, c, k) = erw + tendw;
   }
 }
 
@@ -792,18 +860,24 @@ KB k_mix_cell4(const Instance I) {
   int o[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) o[s] = I.m.ocell[c * NE + s];
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE), f4r = slot_row<NE>(I.m.mx_f4 + c * NE);
+  const float dtc = AT(I.b.delsq_theta, c, k), dwc = AT(I.b.delsq_w, c, k);
+  float tend = AT(I.b.euler_theta, c, k);
+  float tendw = AT(I.b.mix_tw, c, k);
+  const float erw = AT(I.b.euler_rw, c, k);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) fence(o[s]);
   float dto[NE], dwo[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     dto[s] = AT(I.b.delsq_theta, o[s], k);
     dwo[s] = AT(I.b.delsq_w, o[s], k);
   }
-  const float dtc = AT(I.b.delsq_theta, c, k), dwc = AT(I.b.delsq_w, c, k);
+  keep(dto, dwo);
+  settle(sgr, f4r);
   float sg[NE], f4[NE];
-  slots(sg, I.m.csign + c * NE);
-  slots(f4, I.m.mx_f4 + c * NE);
-  float tend = AT(I.b.euler_theta, c, k);
-  float tendw = AT(I.b.mix_tw, c, k);
+  slot_vals(sgr, sg);
+  slot_vals(f4r, f4);
   const bool wlev = k >= 1 && k < L;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
@@ -815,38 +889,79 @@ KB k_mix_cell4(const Instance I) {
       if (wlev) tendw = tendw - f4[s] * (w1 - w0);
     }
   if (k < L) AT(I.b.euler_theta, c, k) = tend;
-  if (k <= L) AT(I.b.euler_rw, c, k) = AT(I.b.euler_rw, c, k) + tendw;
+  if (k <= L) AT(I.b.euler_rw, c, k) = erw + tendw;
 }
 
 // ------------------------------------------------------------ subcycle start
-// MIX: also the 2-D Smagorinsky kdiff of the cell
-template <int NE, bool MIX>
+// MIX: also the 2-D Smagorinsky kdiff of the cell (mix_kdiff_value's
+// arithmetic); MOIST: moisture species present
+template <int NE, bool MIX, bool MOIST>
 KB k_sub_cell(const Instance I) {
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
-  const bool moist = I.n.NM > 0;
-  const float kd = MIX ? mix_kdiff_value<NE>(I, c, k) : 0.0f;
-  // every load first (the mass-flux divergence gathers among them)
+  const int n = I.m.neoc[c];
+  int e[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  SlotRow<NE> sd = slot_row<NE>(I.m.csdv + c * NE);
+  SlotRow<NE> c2 = {0.0f}, s2 = {0.0f}, cs = {0.0f};
+  if (MIX) {
+    c2 = slot_row<NE>(I.m.defc2 + c * NE);
+    s2 = slot_row<NE>(I.m.defs2 + c * NE);
+    cs = slot_row<NE>(I.m.defcs + c * NE);
+  }
+  const float ia = uni_load(I.m.invarea + c);
+  const float rdzw = AT(I.z.rdzw, 0, k), rdzu = AT(I.z.rdzu, 0, k);
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
   const float rho_p = AT(I.b.s_rho_p, c, k);
-  const float qt = moist ? AT(I.b.qtot, c, k) : 0.0f;
-  const float rb = moist ? AT(I.z.rb, c, k) : 0.0f;
-  const float cqw = moist ? AT(I.b.cqw, c, k) : 0.0f;
-  const float prho = I.k.physics ? AT(I.b.phys_rho, c, k) : 0.0f;
+  const float qt = MOIST ? AT(I.b.qtot, c, k) : 0.0f;
+  const float rb = MOIST ? AT(I.z.rb, c, k) : 0.0f;
+  const float cqw = MOIST ? AT(I.b.cqw, c, k) : 0.0f;
   const float rw_k = AT(I.b.c_rw, c, k);
   const float pp_k = AT(I.b.pp, c, k);
-  const float acc = cell_flux_div<NE>(I, c, k, I.b.c_ru);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) fence(e[s]);
+  float xv[NE], un[NE], vn[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    xv[s] = AT(I.b.c_ru, e[s], k);
+    un[s] = MIX ? AT(I.b.u, e[s], k) : 0.0f;
+    vn[s] = MIX ? AT(I.b.v, e[s], k) : 0.0f;
+  }
+  const float prho = I.k.physics ? AT(I.b.phys_rho, c, k) : 0.0f;  // the buffer exists with physics only
+  keep(xv);
+  if (MIX) keep(un, vn);
+  settle(sd, c2, s2, cs);
+  float kd = 0.0f;
+  if (MIX) {
+    float dudx = 0.0f, dudy = 0.0f, dvdx = 0.0f, dvdy = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) {
+        dudx = (dudx + c2[s] * un[s]) - cs[s] * vn[s];
+        dudy = (dudy + cs[s] * un[s]) - s2[s] * vn[s];
+        dvdx = (dvdx + cs[s] * un[s]) + c2[s] * vn[s];
+        dvdy = (dvdy + s2[s] * un[s]) + cs[s] * vn[s];
+      }
+    kd = kdiff_from_deformation(I, k, dudx, dudy, dvdx, dvdy);
+  }
+  float acc = 0.0f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) acc = acc + sd[s] * xv[s];
+  acc = acc * ia;
   const float g = I.k.gravity;
-  const float dpdz = moist ? (-g) * (rb * qt + rho_p * (1.0f + qt)) : (-g) * rho_p;
+  const float dpdz = MOIST ? (-g) * (rb * qt + rho_p * (1.0f + qt)) : (-g) * rho_p;
   // 0 - x, not -x: a zero divergence gives +0 (density_tendency_f32)
-  const float trho = (0.0f - acc) - AT(I.z.rdzw, 0, k) * (lane_off(rw_k, k, 1) - rw_k);
+  const float trho = (0.0f - acc) - rdzw * (lane_off(rw_k, k, 1) - rw_k);
   const float pp_m1 = lane_off(pp_k, k, -1);
   const float dpdzm = up1(dpdz);
   float erw = 0.0f;
   if (k >= 1 && k < L) {
-    const float inner = AT(I.z.rdzu, 0, k) * (pp_k - pp_m1) - (AT(I.z.fzm, 0, k) * dpdz + AT(I.z.fzp, 0, k) * dpdzm);
-    erw = moist ? 0.0f - cqw * inner : 0.0f - inner;  // euler_w(_moist_v841)_f32
+    const float inner = rdzu * (pp_k - pp_m1) - (fzm * dpdz + fzp * dpdzm);
+    erw = MOIST ? 0.0f - cqw * inner : 0.0f - inner;  // euler_w(_moist_v841)_f32
   }
   if (k < L) {
     const float tr = I.k.physics ? trho + prho : trho;
@@ -867,7 +982,7 @@ KB k_sub_edge(const Instance I) {
   const int k = lane();
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const float *pp = I.b.pp, *zz = I.z.zz, *dz = I.b.dpdz;
-  const float a = ((AT(pp, c1, k) - AT(pp, c0, k)) * I.m.invdc[e]) / (0.5f * (AT(zz, c1, k) + AT(zz, c0, k)));
+  const float a = ((AT(pp, c1, k) - AT(pp, c0, k)) * uni_load(I.m.invdc + e)) / (0.5f * (AT(zz, c1, k) + AT(zz, c0, k)));
   const float bb = (0.5f * AT(I.z.zxu, e, k)) * (AT(dz, c0, k) + AT(dz, c1, k));
   const float pool = (0.0f - AT(I.b.cqu, e, k)) * (a - bb);
   if (MIX) {
@@ -894,8 +1009,8 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
 #pragma unroll
   for (int s = 0; s < NX; ++s) nb[s] = I.m.eoe[e * NX + s];
   const int nn = I.m.neoe[e];
-  const SlotRow<NX> wt = slot_row<NX>(I.m.woe + e * NX), ca = slot_row<NX>(I.m.ecos + e * NX),
-                    sa = slot_row<NX>(I.m.esin + e * NX);
+  SlotRow<NX> wtr = slot_row<NX>(I.m.woe + e * NX), car = slot_row<NX>(I.m.ecos + e * NX),
+                    sar = slot_row<NX>(I.m.esin + e * NX);
   const float fe = uni_load(I.m.fedge + e), idc = uni_load(I.m.invdc + e);
   const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k), rdzw = AT(I.z.rdzw, 0, k);
   const float ui = AT(I.z.uinit, 0, k), vi = AT(I.z.vinit, 0, k);
@@ -913,6 +1028,10 @@ __device__ __forceinline__ float tend_mom_edge(const Instance &I, int e, int k) 
   const float hd0 = AT(I.b.hdiv, c0, k), hd1 = AT(I.b.hdiv, c1, k);
   const float eru = AT(I.b.euler_ru, e, k);
   const float pru = I.k.physics ? AT(I.b.phys_ru, e, k) : 0.0f;  // the buffer exists with physics only
+  keep(un, pvn);
+  keep(u_k, rw0, rw1, pve, rho0, rho1, ke0, ke1, hd0, hd1, eru, fe, idc);
+  SlotRow<NX> wt = wtr, ca = car, sa = sar;
+  settle(wt, ca, sa);
   const float u_m1 = lane_off(u_k, k, -1), u_m2 = lane_off(u_k, k, -2), u_p1 = lane_off(u_k, k, 1);
   const float rwe = 0.5f * (rw0 + rw1);
   float wduz = 0.0f;
@@ -1028,6 +1147,7 @@ __device__ __forceinline__ EdgeFluxIn<NE> tend_flux_load(const Instance &I, int 
   tend_flux_index<NE, CORR>(I, e, k, b);
   tend_flux_fence(b);
   tend_flux_rows<NE, CORR>(I, e, k, b);
+  settle(b.av, b.a3v);
   return b;
 }
 template <int NE, bool CORR>
@@ -1085,7 +1205,7 @@ KB k_tend_cell(const Instance I) {
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-  const SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
   const float ia = uni_load(I.m.invarea + c);
   const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
   const float rdzw = AT(I.z.rdzw, 0, k), rdzu = AT(I.z.rdzu, 0, k);
@@ -1110,12 +1230,12 @@ KB k_tend_cell(const Instance I) {
     fco[s] = CORR ? AT(I.b.fcorr, e[s], k) : 0.0f;
   }
   const float prt_k = I.k.physics ? AT(I.b.phys_rt, c, k) : 0.0f;  // the buffer exists with physics only
-  keep(fth[0]);
-  keep(fwv[0]);
-  keep(tru[0]);
-  if (CORR) keep(fco[0]);
+  keep(fth, fwv, tru, zb, zb3);
+  if (CORR) keep(fco);
+  SlotRow<NE> sgs = sgr;
+  settle(sgs);
   float sg[NE];
-  slot_vals(sgr, sg);
+  slot_vals(sgs, sg);
   const float th_m1 = lane_off(th_k, k, -1), th_m2 = lane_off(th_k, k, -2), th_p1 = lane_off(th_k, k, 1);
   const float ths_m1 = lane_off(ths_k, k, -1), rwc_m1 = lane_off(rwc_k, k, -1);
   const float ww_m1 = lane_off(ww_k, k, -1), ww_m2 = lane_off(ww_k, k, -2), ww_p1 = lane_off(ww_k, k, 1);
@@ -1187,31 +1307,50 @@ __device__ __forceinline__ float damped_ru_p(const Instance &I, int e, int k, fl
   return rp + ((coefficient * (d0 - d1)) * (1.0f - 0.0f)) / den;
 }
 
-// damp_prev: apply the previous substep's divergence damping to ru_p first
-// (the same update k_divdamp makes, fused into this pass over the edges)
-// avg_from_rp: this is the second of two substeps and ru_avg was not stored at
-// the first, where it equals the undamped ru_p read here (p1 + p2, the same
-// operands in the same order as ru_avg + p).  Loads first, stores last.
-KB k_ac_edge(const Instance I, float dts, int first, int damp_prev, float damp_coef, int avg_from_rp) {
+// FIRST: the first substep (ru_p = dts * tend_ru).  DAMP: apply the previous
+// substep's divergence damping to ru_p first (the same update k_divdamp
+// makes, fused into this pass over the edges).  AVGRP: this is the second of
+// two substeps and ru_avg was not stored at the first, where it equals the
+// undamped ru_p read here (p1 + p2, the same operands in the same order as
+// ru_avg + p).  Loads first, stores last.
+template <bool FIRST, bool DAMP, bool AVGRP>
+KB k_ac_edge(const Instance I, float dts, float damp_coef) {
   const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
-  const float tr = AT(I.b.tend_ru, e, k);
-  if (first) {
-    const float p = dts * tr;
+  if (FIRST) {
+    const float p = dts * AT(I.b.tend_ru, e, k);
     AT(I.b.ru_p, e, k) = p;
     AT(I.b.ru_avg, e, k) = p;
     return;
   }
-  const float rp_raw = AT(I.b.ru_p, e, k);
-  const float ra_old = avg_from_rp ? rp_raw : AT(I.b.ru_avg, e, k);
-  const float rp_in = damp_prev ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  float pg = ((AT(I.b.rtheta_pp, c1, k) - AT(I.b.rtheta_pp, c0, k)) * I.m.invdc[e]) /
-             (0.5f * (AT(I.z.zz, c1, k) + AT(I.z.zz, c0, k)));
-  pg = (((AT(I.b.cqu, e, k) * 0.5f) * I.k.c2) * (AT(I.b.s_exner, c0, k) + AT(I.b.s_exner, c1, k))) * pg;
-  pg = pg + ((0.5f * AT(I.z.zxu, e, k)) * I.k.gravity) * (AT(I.b.rho_pp, c0, k) + AT(I.b.rho_pp, c1, k));
+  const float idc = uni_load(I.m.invdc + e);
+  const float tr = AT(I.b.tend_ru, e, k);
+  const float rp_raw = AT(I.b.ru_p, e, k);
+  const float ra_in = AVGRP ? 0.0f : AT(I.b.ru_avg, e, k);
+  const float cqu = AT(I.b.cqu, e, k), zxu = AT(I.z.zxu, e, k);
+  fence(c0);
+  fence(c1);
+  const float tp0 = AT(I.b.rtheta_pp, c0, k), tp1 = AT(I.b.rtheta_pp, c1, k);
+  const float z0 = AT(I.z.zz, c0, k), z1 = AT(I.z.zz, c1, k);
+  const float x0 = AT(I.b.s_exner, c0, k), x1 = AT(I.b.s_exner, c1, k);
+  const float q0 = AT(I.b.rho_pp, c0, k), q1 = AT(I.b.rho_pp, c1, k);
+  float to0 = 0.0f, to1 = 0.0f, st0 = 0.0f, st1 = 0.0f;
+  if (DAMP) {
+    to0 = AT(I.b.rtheta_pp_old, c0, k);
+    to1 = AT(I.b.rtheta_pp_old, c1, k);
+    st0 = AT(I.b.s_theta, c0, k);
+    st1 = AT(I.b.s_theta, c1, k);
+  }
+  const float ra_old = AVGRP ? rp_raw : ra_in;
+  // ru_p + the divergence-damping increment (damped_ru_p's arithmetic)
+  const float rp_in = DAMP ? rp_raw + ((damp_coef * ((tp0 - to0) - (tp1 - to1))) * (1.0f - 0.0f)) / (st0 + st1)
+                           : rp_raw;
+  float pg = ((tp1 - tp0) * idc) / (0.5f * (z1 + z0));
+  pg = (((cqu * 0.5f) * I.k.c2) * (x0 + x1)) * pg;
+  pg = pg + ((0.5f * zxu) * I.k.gravity) * (q0 + q1);
   const float p = rp_in + dts * (tr - pg);
   AT(I.b.ru_p, e, k) = p;
   AT(I.b.ru_avg, e, k) = ra_old + p;
@@ -1365,8 +1504,8 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-  const SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
-  const SlotRow<NE> sd = slot_row<NE>(I.m.csdv + c * NE);
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
+  SlotRow<NE> sdr = slot_row<NE>(I.m.csdv + c * NE);
   const float ia = uni_load(I.m.invarea + c);
   const float zk = AT(I.z.zz, c, k);
   const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
@@ -1380,9 +1519,11 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
   for (int s = 0; s < NE; ++s) fence(e[s]);
 #pragma unroll
   for (int s = 0; s < NE; ++s) ru[s] = ru_row(e[s]);
-  keep(ru[0]);
+  keep(ru, zb, zb3);
+  SlotRow<NE> sgs = sgr, sd = sdr;
+  settle(sgs, sd);
   float sg[NE];
-  slot_vals(sgr, sg);
+  slot_vals(sgs, sg);
   const float zkm = lane_off(zk, k, -1);
   if (hdiv_out) {
     // mass-flux divergence of the recovered ru (dynamics.py:86-121), next stage's hdiv
@@ -1568,45 +1709,73 @@ __device__ __forceinline__ float vflux(float q0, float q1, float q2, float q3, f
 
 // Scalar kernels run one wavefront per entity over all species: the
 // connectivity, stencil weights and the species-independent rows are read
-// once, and two species' neighbour rows are in flight together.
+// once, and the species are taken in batches of G whose rows are all
+// requested before any of them is combined (each batch is one memory round
+// trip; a batch's stores come after its loads, and the next batch's loads
+// after those stores, since the kernels' input and output arrays may alias).
+// G trades round trips against registers per wavefront.
+#ifndef HEXDYN_G_SC_EDGE
+#define HEXDYN_G_SC_EDGE 3
+#endif
+#ifndef HEXDYN_G_SC_CELL
+#define HEXDYN_G_SC_CELL 3
+#endif
+#ifndef HEXDYN_G_FCT_EDGE1
+#define HEXDYN_G_FCT_EDGE1 3
+#endif
+#ifndef HEXDYN_G_FCT_CELL1
+#define HEXDYN_G_FCT_CELL1 2
+#endif
+#ifndef HEXDYN_G_FCT_EDGE2
+#define HEXDYN_G_FCT_EDGE2 6
+#endif
+#ifndef HEXDYN_G_FCT_CELL2
+#define HEXDYN_G_FCT_CELL2 3
+#endif
+__device__ __forceinline__ int clamp_sp(int sp, int nt) { return sp < nt ? sp : nt - 1; }
+
 template <int NE>
 KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__ ev) {
-  constexpr int NX = 2 * NE - 2;
+  constexpr int NX = 2 * NE - 2, G = HEXDYN_G_SC_EDGE;
   const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const int NT = I.n.NT;
-  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   int ac[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
-  const float sv = cs1(AT(I.b.split_u, e, k));
-  const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
-  float av[NX], a3v[NX], wt[NX];
-  slots(av, I.m.adv + e * NX);
-  slots(a3v, I.m.adv3 + e * NX);
+  SlotRow<NX> av = slot_row<NX>(I.m.adv + e * NX), a3v = slot_row<NX>(I.m.adv3 + e * NX);
+  const float uu = AT(I.b.split_u, e, k);
 #pragma unroll
-  for (int s = 0; s < NX; ++s) wt[s] = av[s] + (coef * sv) * a3v[s];
-  for (int sp = 0; sp < NT; sp += 2) {
-    const int sq = sp + 1 < NT ? sp + 1 : sp;
-    const int pa = sp * I.n.C, pb = sq * I.n.C;  // species rows start at s * C
-    float va[NX], vb[NX];
+  for (int s = 0; s < NX; ++s) fence(ac[s]);
+  const float coef = I.k.coef3;
+  for (int sp0 = 0; sp0 < NT; sp0 += G) {
+    float v[G][NX];
 #pragma unroll
-    for (int s = 0; s < NX; ++s) {
-      va[s] = AT(qs, pa + ac[s], k);
-      vb[s] = AT(qs, pb + ac[s], k);
+    for (int g = 0; g < G; ++g) {
+      const int r0 = clamp_sp(sp0 + g, NT) * I.n.C;  // species rows start at s * C
+#pragma unroll
+      for (int s = 0; s < NX; ++s) v[g][s] = AT(qs, r0 + ac[s], k);
     }
-    float ra = 0.0f, rb = 0.0f;
+    // the weights are formed after the batch's loads are issued
+    float uk = uu;
+    keep(uk);
+    keep(v);
+    SlotRow<NX> avs = av, a3s = a3v;
+    settle(avs, a3s);
+    const float sv = cs1(uk);
+    float wt[NX];
 #pragma unroll
-    for (int s = 0; s < NX; ++s)
-      if (s < na) {
-        ra = ra + wt[s] * va[s];
-        rb = rb + wt[s] * vb[s];
-      }
-    if (k < I.n.L) {
-      AT(ev, sp * I.n.E + e, k) = ra;
-      if (sq != sp) AT(ev, sq * I.n.E + e, k) = rb;
+    for (int s = 0; s < NX; ++s) wt[s] = avs[s] + (coef * sv) * a3s[s];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (sp0 + g >= NT) break;
+      float r = 0.0f;
+#pragma unroll
+      for (int s = 0; s < NX; ++s)
+        if (s < na) r = r + wt[s] * v[g][s];
+      if (k < I.n.L) AT(ev, (sp0 + g) * I.n.E + e, k) = r;
     }
   }
 }
@@ -1614,60 +1783,62 @@ KB k_sc_edge(const Instance I, const float *__restrict__ qs, float *__restrict__
 template <int NE>
 KB k_sc_cell(const Instance I, const float *__restrict__ qs, const float *__restrict__ ev, float *__restrict__ out,
              float dt, float wn, const float *__restrict__ src) {
+  constexpr int G = HEXDYN_G_SC_CELL;
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L, NT = I.n.NT;
-  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   const int n = I.m.neoc[c];
-  float sg[NE];
-  slots(sg, I.m.csign + c * NE);
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
+  const float ia = uni_load(I.m.invarea + c);
+  const float sw = AT(I.b.split_w, c, k);
+  const float orho = AT(I.b.rho, c, k), crho = AT(I.b.c_rho, c, k);
+  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k), rdzw = AT(I.z.rdzw, 0, k);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) fence(e[s]);
   float uh[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) uh[s] = AT(I.b.split_u, e[s], k);
-  const float sw = AT(I.b.split_w, c, k);
-  const float orho = AT(I.b.rho, c, k);
-  const float target = (1.0f - wn) * orho + wn * AT(I.b.c_rho, c, k);
-  const float fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k), rdzw = AT(I.z.rdzw, 0, k), ia = I.m.invarea[c];
-  // software-pipelined over species: the next species' rows are requested
-  // before the current one is combined (clamped index, in bounds)
-  struct Sp {
-    float qk, qo, src, evv[NE];
-  };
-  auto load_sp = [&](int sp) {
-    Sp a;
-    a.qk = AT(qs, sp * I.n.C + c, k);
+  for (int sp0 = 0; sp0 < NT; sp0 += G) {
+    float qk[G], qo[G], sr[G], evv[G][NE];
 #pragma unroll
-    for (int s = 0; s < NE; ++s) a.evv[s] = AT(ev, sp * I.n.E + e[s], k);
-    a.qo = AT(I.b.q, sp * I.n.C + c, k);
-    a.src = src ? AT(src, sp * I.n.C + c, k) : 0.0f;  // held scalar tendency (transport.py:1800-1808)
-    return a;
-  };
-  auto species = [&](int sp, const Sp &a) {
-    const float qk = a.qk, qo = a.qo;
-    const float *evv = a.evv;
-    const float q_m2 = lane_off(qk, k, -2), q_m1 = lane_off(qk, k, -1), q_p1 = lane_off(qk, k, 1);
-    float vf = 0.0f;
-    if (k >= 1 && k < L) vf = vflux(q_m2, q_m1, qk, q_p1, sw, k, L, fzm, fzp, I.k.coef3);
-    const float vf_p = dn1(vf);
-    float tc = 0.0f;
+    for (int g = 0; g < G; ++g) {
+      const int sq = clamp_sp(sp0 + g, NT);
+      qk[g] = AT(qs, sq * I.n.C + c, k);
+      qo[g] = AT(I.b.q, sq * I.n.C + c, k);
 #pragma unroll
-    for (int s = 0; s < NE; ++s)
-      if (s < n) tc = tc + ((-sg[s]) * uh[s]) * evv[s];
-    tc = tc * ia + a.src;
-    if (k < L) {
-      const float num = qo * orho + dt * (tc - rdzw * (vf_p - vf));
-      AT(out, sp * I.n.C + c, k) = num * (1.0f / target);
+      for (int s = 0; s < NE; ++s) evv[g][s] = AT(ev, sq * I.n.E + e[s], k);
     }
-  };
-  Sp cur = load_sp(0);
-  for (int sp = 0; sp < NT; ++sp) {
-    const Sp nxt = load_sp(sp + 1 < NT ? sp + 1 : sp);
-    species(sp, cur);
-    cur = nxt;
+    // held scalar tendency (transport.py:1800-1808)
+#pragma unroll
+    for (int g = 0; g < G; ++g) sr[g] = src ? AT(src, clamp_sp(sp0 + g, NT) * I.n.C + c, k) : 0.0f;
+    keep(qk, qo, evv);
+    keep(uh);
+    SlotRow<NE> sgs = sgr;
+    settle(sgs);
+    float sg[NE];
+    slot_vals(sgs, sg);
+    const float target = (1.0f - wn) * orho + wn * crho;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (sp0 + g >= NT) break;
+      const float q_m2 = lane_off(qk[g], k, -2), q_m1 = lane_off(qk[g], k, -1), q_p1 = lane_off(qk[g], k, 1);
+      float vf = 0.0f;
+      if (k >= 1 && k < L) vf = vflux(q_m2, q_m1, qk[g], q_p1, sw, k, L, fzm, fzp, I.k.coef3);
+      const float vf_p = dn1(vf);
+      float tc = 0.0f;
+#pragma unroll
+      for (int s = 0; s < NE; ++s)
+        if (s < n) tc = tc + ((-sg[s]) * uh[s]) * evv[g][s];
+      tc = tc * ia + sr[g];
+      if (k < L) {
+        const float num = qo[g] * orho + dt * (tc - rdzw * (vf_p - vf));
+        AT(out, (sp0 + g) * I.n.C + c, k) = num * (1.0f / target);
+      }
+    }
   }
 }
 
@@ -1691,24 +1862,29 @@ KB k_fct_target(const Instance I, float dt) {
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
-  const float ia = I.m.invarea[c];
   const int n = I.m.neoc[c];
-  float sd[NE];
-  slots(sd, I.m.csdv + c * NE);
   int e[NE];
-  float uh[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+  SlotRow<NE> sd = slot_row<NE>(I.m.csdv + c * NE);
+  const float ia = uni_load(I.m.invarea + c);
+  const float wa_k = AT(I.b.split_w, c, k);
+  const float rho = AT(I.b.rho, c, k), rdzw = AT(I.z.rdzw, 0, k);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) fence(e[s]);
+  float uh[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) uh[s] = AT(I.b.split_u, e[s], k);
+  keep(uh);
+  SlotRow<NE> sds = sd;
+  settle(sds);
   float hd = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
-    if (s < n) hd = hd + (-(uh[s] * sd[s])) * ia;  // ((-csign)*uh)*dv, csign = +-1
-  const float wa_k = AT(I.b.split_w, c, k);
+    if (s < n) hd = hd + (-(uh[s] * sds[s])) * ia;  // ((-csign)*uh)*dv, csign = +-1
   const float wa_p = lane_off(wa_k, k, 1);
   if (k < L) {
-    const float t = AT(I.b.rho, c, k) + dt * (hd - AT(I.z.rdzw, 0, k) * (wa_p - wa_k));
+    const float t = rho + dt * (hd - rdzw * (wa_p - wa_k));
     AT(I.b.target, c, k) = t;
     if (!(t > 0.0f)) set_flag(I.b.flag);
   }
@@ -1716,67 +1892,66 @@ KB k_fct_target(const Instance I, float dt) {
 
 template <int NE>
 KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt, const float *__restrict__ qold, float qadd) {
-  constexpr int NX = 2 * NE - 2;
+  constexpr int NX = 2 * NE - 2, G = HEXDYN_G_FCT_EDGE1;
   const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const int NT = I.n.NT;
-  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
   int ac[NX];
 #pragma unroll
   for (int s = 0; s < NX; ++s) ac[s] = I.m.advc[e * NX + s];
   const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
-  const float uu = AT(I.b.split_u, e, k);
-  const float sv = cs1(uu);
-  const float coef = I.k.coef3;
   const int na = I.m.nadv[e];
-  float av[NX], a3v[NX], wt[NX];
-  slots(av, I.m.adv + e * NX);
-  slots(a3v, I.m.adv3 + e * NX);
+  SlotRow<NX> av = slot_row<NX>(I.m.adv + e * NX), a3v = slot_row<NX>(I.m.adv3 + e * NX);
+  const float dve = uni_load(I.m.dv + e);
+  const float uu = AT(I.b.split_u, e, k);
 #pragma unroll
-  for (int s = 0; s < NX; ++s) wt[s] = av[s] + (coef * sv) * a3v[s];
-  const float dvdt = I.m.dv[e] * dt;
-  for (int sp = 0; sp < NT; sp += 2) {
-    const int sq = sp + 1 < NT ? sp + 1 : sp;
-    float va[NX], vb[NX];
+  for (int s = 0; s < NX; ++s) fence(ac[s]);
+  fence(c1);
+  fence(c2);
+  const float coef = I.k.coef3;
+  const float dvdt = dve * dt;
+  for (int sp0 = 0; sp0 < NT; sp0 += G) {
+    float v[G][NX], q1[G], q2[G];
 #pragma unroll
-    for (int s = 0; s < NX; ++s) {
-      va[s] = AT(qs, sp * I.n.C + ac[s], k);
-      vb[s] = AT(qs, sq * I.n.C + ac[s], k);
+    for (int g = 0; g < G; ++g) {
+      const int p = clamp_sp(sp0 + g, NT) * I.n.C;
+#pragma unroll
+      for (int s = 0; s < NX; ++s) v[g][s] = AT(qs, p + ac[s], k);
+      q1[g] = AT(qold, p + c1, k);
+      q2[g] = AT(qold, p + c2, k);
     }
-    const int pa = sp * I.n.C, pb = sq * I.n.C;
-    const float qa1 = AT(qold, pa + c1, k) + qadd, qa2 = AT(qold, pa + c2, k) + qadd;
-    const float qb1 = AT(qold, pb + c1, k) + qadd, qb2 = AT(qold, pb + c2, k) + qadd;
-    float ra, rb;
-    if (na == 10) {
-      // ten-cell stencils associate from the first product (transport.py n_adv == 10 branch)
-      float xa = wt[0] * va[0], xb = wt[0] * vb[0];
+    float uk = uu;
+    keep(uk);
+    keep(v);
+    SlotRow<NX> avs = av, a3s = a3v;
+    settle(avs, a3s);
+    const float sv = cs1(uk);
+    float wt[NX];
 #pragma unroll
-      for (int s = 1; s < NX; ++s)
-        if (s < 10) {
-          xa = xa + wt[s] * va[s];
-          xb = xb + wt[s] * vb[s];
-        }
-      ra = uu * xa;
-      rb = uu * xb;
-    } else {
-      ra = 0.0f;
-      rb = 0.0f;
+    for (int s = 0; s < NX; ++s) wt[s] = avs[s] + (coef * sv) * a3s[s];
 #pragma unroll
-      for (int s = 0; s < NX; ++s)
-        if (s < na) {
-          ra = ra + (uu * wt[s]) * va[s];
-          rb = rb + (uu * wt[s]) * vb[s];
-        }
-    }
-    const float upa = dvdt * (max0(uu) * qa1 + min0(uu) * qa2);
-    const float upb = dvdt * (max0(uu) * qb1 + min0(uu) * qb2);
-    if (k < I.n.L) {
-      AT(I.b.fct_upw, sp * I.n.E + e, k) = upa;
-      AT(I.b.fct_hres, sp * I.n.E + e, k) = dt * ra - upa;
-      if (sq != sp) {
-        AT(I.b.fct_upw, sq * I.n.E + e, k) = upb;
-        AT(I.b.fct_hres, sq * I.n.E + e, k) = dt * rb - upb;
+    for (int g = 0; g < G; ++g) {
+      if (sp0 + g >= NT) break;
+      const float qa1 = q1[g] + qadd, qa2 = q2[g] + qadd;
+      float ra;
+      if (na == 10) {
+        // ten-cell stencils associate from the first product (transport.py n_adv == 10 branch)
+        float xa = wt[0] * v[g][0];
+#pragma unroll
+        for (int s = 1; s < NX; ++s)
+          if (s < 10) xa = xa + wt[s] * v[g][s];
+        ra = uk * xa;
+      } else {
+        ra = 0.0f;
+#pragma unroll
+        for (int s = 0; s < NX; ++s)
+          if (s < na) ra = ra + (uk * wt[s]) * v[g][s];
+      }
+      const float upa = dvdt * (max0(uk) * qa1 + min0(uk) * qa2);
+      if (k < I.n.L) {
+        AT(I.b.fct_upw, (sp0 + g) * I.n.E + e, k) = upa;
+        AT(I.b.fct_hres, (sp0 + g) * I.n.E + e, k) = dt * ra - upa;
       }
     }
   }
@@ -1784,144 +1959,197 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt, const f
 
 template <int NE>
 KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const float *__restrict__ qold, float qadd) {
+  constexpr int G = HEXDYN_G_FCT_CELL1;
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L;
-  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int L = I.n.L, NT = I.n.NT;
   const int n = I.m.neoc[c];
-  float sg[NE];
-  slots(sg, I.m.csign + c * NE);
   int nb[NE], e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     nb[s] = I.m.coc[c * NE + s];
     e[s] = I.m.eoc[c * NE + s];
   }
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
+  const float ia = uni_load(I.m.invarea + c);
   const float orho = AT(I.b.rho, c, k);
   const float wk = AT(I.b.split_w, c, k);
   const float rdzw = AT(I.z.rdzw, 0, k), fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
-  const float ia = I.m.invarea[c];
   const float target = AT(I.b.target, c, k);
-  for (int sp = 0; sp < I.n.NT; ++sp) {
-    const float qs_k = AT(qs, sp * I.n.C + c, k);
-    const float qs_m2 = lane_off(qs_k, k, -2), qs_m1 = lane_off(qs_k, k, -1), qs_p1 = lane_off(qs_k, k, 1);
-    const int po = sp * I.n.C;
-    float qn[NE], upw[NE], hres[NE];
 #pragma unroll
-    for (int s = 0; s < NE; ++s) {
-      qn[s] = AT(qold, po + nb[s], k) + qadd;
-      upw[s] = AT(I.b.fct_upw, sp * I.n.E + e[s], k);
-      hres[s] = AT(I.b.fct_hres, sp * I.n.E + e[s], k);
-    }
-    const float qo = AT(qold, po + c, k) + qadd;
-    const float qo_m = up1(qo), qo_p = dn1(qo);
-    float a, b;
-    if (k == 0) {
-      a = qo; b = qo;
-      a = pmin(a, qo_p); b = pmax(b, qo_p);
-    } else if (k == L - 1) {
-      a = qo_m; b = qo_m;
-      a = pmin(a, qo); b = pmax(b, qo);
-    } else {
-      a = qo_m; b = qo_m;
-      a = pmin(a, qo); b = pmax(b, qo);
-      a = pmin(a, qo_p); b = pmax(b, qo_p);
-    }
+  for (int s = 0; s < NE; ++s) {
+    fence(nb[s]);
+    fence(e[s]);
+  }
+  for (int sp0 = 0; sp0 < NT; sp0 += G) {
+    float qsk[G], qol[G], qn[G][NE], upw[G][NE], hres[G][NE];
 #pragma unroll
-    for (int s = 0; s < NE; ++s)
-      if (s < n) {
-        a = pmin(a, qn[s]);
-        b = pmax(b, qn[s]);
+    for (int g = 0; g < G; ++g) {
+      const int sq = clamp_sp(sp0 + g, NT);
+      const int po = sq * I.n.C;
+      qsk[g] = AT(qs, po + c, k);
+#pragma unroll
+      for (int s = 0; s < NE; ++s) {
+        qn[g][s] = AT(qold, po + nb[s], k);
+        upw[g][s] = AT(I.b.fct_upw, sq * I.n.E + e[s], k);
+        hres[g][s] = AT(I.b.fct_hres, sq * I.n.E + e[s], k);
       }
-    float mass = qo * orho;
-    float vres = 0.0f;
-    float fcol = 0.0f;
-    if (k >= 1 && k < L) {
-      vres = dt * vflux(qs_m2, qs_m1, qs_k, qs_p1, wk, k, L, fzm, fzp, I.k.coef3);
-      fcol = dt * (max0(wk) * qo_m + min0(wk) * qo);
-    } else {
-      vres = dt * 0.0f;
+      qol[g] = AT(qold, po + c, k);
     }
-    const float fcol_p = dn1(fcol);
-    if (k <= L - 2) mass = mass - fcol_p * rdzw;
-    if (k >= 1 && k < L) {
-      mass = mass + fcol * rdzw;
-      vres = vres - fcol;
-    }
-    const float vres_p = dn1(vres);
-    float sin_ = (-rdzw) * (min0(vres_p) - max0(vres));
-    float sout = (-rdzw) * (max0(vres_p) - min0(vres));
+    keep(qsk, qol, qn, upw, hres);
+    SlotRow<NE> sgs = sgr;
+    settle(sgs);
+    float sg[NE];
+    slot_vals(sgs, sg);
 #pragma unroll
-    for (int s = 0; s < NE; ++s)
-      if (s < n) {
-        const float so = -sg[s];
-        mass = mass + (so * upw[s]) * ia;
-        const float sr = (-so) * hres[s];
-        sout = sout - max0(sr) * ia;
-        sin_ = sin_ - min0(sr) * ia;
+    for (int g = 0; g < G; ++g) {
+      if (sp0 + g >= NT) break;
+      const int sp = sp0 + g;
+      const float qs_k = qsk[g];
+      const float qs_m2 = lane_off(qs_k, k, -2), qs_m1 = lane_off(qs_k, k, -1), qs_p1 = lane_off(qs_k, k, 1);
+      const float qo = qol[g] + qadd;
+      const float qo_m = up1(qo), qo_p = dn1(qo);
+      float a, b;
+      if (k == 0) {
+        a = qo; b = qo;
+        a = pmin(a, qo_p); b = pmax(b, qo_p);
+      } else if (k == L - 1) {
+        a = qo_m; b = qo_m;
+        a = pmin(a, qo); b = pmax(b, qo);
+      } else {
+        a = qo_m; b = qo_m;
+        a = pmin(a, qo); b = pmax(b, qo);
+        a = pmin(a, qo_p); b = pmax(b, qo_p);
       }
-    float f = (b * target - mass) / (sin_ + 1.0e-20f);
-    sin_ = pmin(1.0f, max0(f));
-    f = (a * target - mass) / (sout - 1.0e-20f);
-    sout = pmin(1.0f, max0(f));
-    if (k < L) {
-      AT(I.b.fct_mass, sp * I.n.C + c, k) = mass;
-      AT(I.b.fct_sin, sp * I.n.C + c, k) = sin_;
-      AT(I.b.fct_sout, sp * I.n.C + c, k) = sout;
+#pragma unroll
+      for (int s = 0; s < NE; ++s)
+        if (s < n) {
+          const float q = qn[g][s] + qadd;
+          a = pmin(a, q);
+          b = pmax(b, q);
+        }
+      float mass = qo * orho;
+      float vres = 0.0f;
+      float fcol = 0.0f;
+      if (k >= 1 && k < L) {
+        vres = dt * vflux(qs_m2, qs_m1, qs_k, qs_p1, wk, k, L, fzm, fzp, I.k.coef3);
+        fcol = dt * (max0(wk) * qo_m + min0(wk) * qo);
+      } else {
+        vres = dt * 0.0f;
+      }
+      const float fcol_p = dn1(fcol);
+      if (k <= L - 2) mass = mass - fcol_p * rdzw;
+      if (k >= 1 && k < L) {
+        mass = mass + fcol * rdzw;
+        vres = vres - fcol;
+      }
+      const float vres_p = dn1(vres);
+      float sin_ = (-rdzw) * (min0(vres_p) - max0(vres));
+      float sout = (-rdzw) * (max0(vres_p) - min0(vres));
+#pragma unroll
+      for (int s = 0; s < NE; ++s)
+        if (s < n) {
+          const float so = -sg[s];
+          mass = mass + (so * upw[g][s]) * ia;
+          const float sr = (-so) * hres[g][s];
+          sout = sout - max0(sr) * ia;
+          sin_ = sin_ - min0(sr) * ia;
+        }
+      float f = (b * target - mass) / (sin_ + 1.0e-20f);
+      sin_ = pmin(1.0f, max0(f));
+      f = (a * target - mass) / (sout - 1.0e-20f);
+      sout = pmin(1.0f, max0(f));
+      if (k < L) {
+        AT(I.b.fct_mass, sp * I.n.C + c, k) = mass;
+        AT(I.b.fct_sin, sp * I.n.C + c, k) = sin_;
+        AT(I.b.fct_sout, sp * I.n.C + c, k) = sout;
+      }
+      if (k <= L) AT(I.b.fct_vres, sp * I.n.C + c, k) = vres;
     }
-    if (k <= L) AT(I.b.fct_vres, sp * I.n.C + c, k) = vres;
   }
 }
 
 KB k_fct_edge2(const Instance I) {
+  constexpr int G = HEXDYN_G_FCT_EDGE2;
   const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
-  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int NT = I.n.NT;
   const int c1 = I.m.coe[2 * e], c2 = I.m.coe[2 * e + 1];
-  for (int sp = 0; sp < I.n.NT; ++sp) {
-    const int ps = sp * I.n.C;
-    const float f = AT(I.b.fct_hres, sp * I.n.E + e, k);
-    AT(I.b.fct_hres, sp * I.n.E + e, k) = max0(f) * pmin(AT(I.b.fct_sout, ps + c1, k), AT(I.b.fct_sin, ps + c2, k)) +
-                                              min0(f) * pmin(AT(I.b.fct_sin, ps + c1, k), AT(I.b.fct_sout, ps + c2, k));
+  for (int sp0 = 0; sp0 < NT; sp0 += G) {
+    float f[G], so1[G], si2[G], si1[G], so2[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int sq = clamp_sp(sp0 + g, NT);
+      const int ps = sq * I.n.C;
+      f[g] = AT(I.b.fct_hres, sq * I.n.E + e, k);
+      so1[g] = AT(I.b.fct_sout, ps + c1, k);
+      si2[g] = AT(I.b.fct_sin, ps + c2, k);
+      si1[g] = AT(I.b.fct_sin, ps + c1, k);
+      so2[g] = AT(I.b.fct_sout, ps + c2, k);
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (sp0 + g >= NT) break;
+      AT(I.b.fct_hres, (sp0 + g) * I.n.E + e, k) =
+          max0(f[g]) * pmin(so1[g], si2[g]) + min0(f[g]) * pmin(si1[g], so2[g]);
+    }
   }
 }
 
 template <int NE>
 KB k_fct_cell2(const Instance I, float *__restrict__ out) {
+  constexpr int G = HEXDYN_G_FCT_CELL2;
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
-  const int L = I.n.L;
-  const size_t cplane = (size_t)I.n.C * HX_ROW, eplane = (size_t)I.n.E * HX_ROW;
+  const int L = I.n.L, NT = I.n.NT;
   const int n = I.m.neoc[c];
-  float sg[NE];
-  slots(sg, I.m.csign + c * NE);
   int e[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-  const float ia = I.m.invarea[c], rdzw = AT(I.z.rdzw, 0, k);
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
+  const float ia = uni_load(I.m.invarea + c);
+  const float rdzw = AT(I.z.rdzw, 0, k);
   const float target = AT(I.b.target, c, k);
-  for (int sp = 0; sp < I.n.NT; ++sp) {
-    float hres[NE];
 #pragma unroll
-    for (int s = 0; s < NE; ++s) hres[s] = AT(I.b.fct_hres, sp * I.n.E + e[s], k);
-    const float si = AT(I.b.fct_sin, sp * I.n.C + c, k), so = AT(I.b.fct_sout, sp * I.n.C + c, k);
-    const float si_m = up1(si), so_m = up1(so);
-    float v = AT(I.b.fct_vres, sp * I.n.C + c, k);
-    if (k >= 1 && k < L) v = max0(v) * pmin(so_m, si) + min0(v) * pmin(so, si_m);
-    const float v_p = dn1(v);
-    float mass = AT(I.b.fct_mass, sp * I.n.C + c, k);
+  for (int s = 0; s < NE; ++s) fence(e[s]);
+  for (int sp0 = 0; sp0 < NT; sp0 += G) {
+    float hres[G][NE], si[G], so[G], vr[G], ms[G];
 #pragma unroll
-    for (int s = 0; s < NE; ++s)
-      if (s < n) {
-        const float sgn = -sg[s];
-        mass = mass + (sgn * hres[s]) * ia;
-      }
-    mass = mass - rdzw * (v_p - v);
-    if (k < L) AT(out, sp * I.n.C + c, k) = max0(mass / target);
+    for (int g = 0; g < G; ++g) {
+      const int sq = clamp_sp(sp0 + g, NT);
+#pragma unroll
+      for (int s = 0; s < NE; ++s) hres[g][s] = AT(I.b.fct_hres, sq * I.n.E + e[s], k);
+      si[g] = AT(I.b.fct_sin, sq * I.n.C + c, k);
+      so[g] = AT(I.b.fct_sout, sq * I.n.C + c, k);
+      vr[g] = AT(I.b.fct_vres, sq * I.n.C + c, k);
+      ms[g] = AT(I.b.fct_mass, sq * I.n.C + c, k);
+    }
+    keep(hres, si, so, vr, ms);
+    SlotRow<NE> sgs = sgr;
+    settle(sgs);
+    float sg[NE];
+    slot_vals(sgs, sg);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (sp0 + g >= NT) break;
+      const float si_m = up1(si[g]), so_m = up1(so[g]);
+      float v = vr[g];
+      if (k >= 1 && k < L) v = max0(v) * pmin(so_m, si[g]) + min0(v) * pmin(so[g], si_m);
+      const float v_p = dn1(v);
+      float mass = ms[g];
+#pragma unroll
+      for (int s = 0; s < NE; ++s)
+        if (s < n) {
+          const float sgn = -sg[s];
+          mass = mass + (sgn * hres[g][s]) * ia;
+        }
+      mass = mass - rdzw * (v_p - v);
+      if (k < L) AT(out, (sp0 + g) * I.n.C + c, k) = max0(mass / target);
+    }
   }
 }
 
@@ -2025,12 +2253,20 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
   // with mixing, kdiff rides on the cell pass and the del2 u increment on the
   // edge pass (the pool is written once)
   if (I.k.mixing) {
-    NEDISPATCH(I, launch_named("k_sub_cell", k_sub_cell<6, true>, I.n.C, 1, s, I),
-               launch_named("k_sub_cell", k_sub_cell<7, true>, I.n.C, 1, s, I));
+    if (I.n.NM > 0)
+      NEDISPATCH(I, launch_named("k_sub_cell", k_sub_cell<6, true, true>, I.n.C, 1, s, I),
+                 launch_named("k_sub_cell", k_sub_cell<7, true, true>, I.n.C, 1, s, I));
+    else
+      NEDISPATCH(I, launch_named("k_sub_cell", k_sub_cell<6, true, false>, I.n.C, 1, s, I),
+                 launch_named("k_sub_cell", k_sub_cell<7, true, false>, I.n.C, 1, s, I));
     launch_named("k_sub_edge", k_sub_edge<true>, I.n.E, 1, s, I);
   } else {
-    NEDISPATCH(I, launch_named("k_sub_cell", k_sub_cell<6, false>, I.n.C, 1, s, I),
-               launch_named("k_sub_cell", k_sub_cell<7, false>, I.n.C, 1, s, I));
+    if (I.n.NM > 0)
+      NEDISPATCH(I, launch_named("k_sub_cell", k_sub_cell<6, false, true>, I.n.C, 1, s, I),
+                 launch_named("k_sub_cell", k_sub_cell<7, false, true>, I.n.C, 1, s, I));
+    else
+      NEDISPATCH(I, launch_named("k_sub_cell", k_sub_cell<6, false, false>, I.n.C, 1, s, I),
+                 launch_named("k_sub_cell", k_sub_cell<7, false, false>, I.n.C, 1, s, I));
     launch_named("k_sub_edge", k_sub_edge<false>, I.n.E, 1, s, I);
   }
   launch_col(k_vcoef, I.n.C, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
@@ -2041,7 +2277,12 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
       launch(k_mix_lapvort, I.n.V, 1, s, I);
       launch(k_mix_edge4, I.n.E, 1, s, I);
     }
-    LAUNCH_NE(I, k_mix_cell, I.n.C, 1, s, I);
+    if (I.k.mix4)
+      NEDISPATCH(I, launch_named("k_mix_cell", k_mix_cell<6, true>, I.n.C, 1, s, I),
+                 launch_named("k_mix_cell", k_mix_cell<7, true>, I.n.C, 1, s, I));
+    else
+      NEDISPATCH(I, launch_named("k_mix_cell", k_mix_cell<6, false>, I.n.C, 1, s, I),
+                 launch_named("k_mix_cell", k_mix_cell<7, false>, I.n.C, 1, s, I));
     if (I.k.mix4) LAUNCH_NE(I, k_mix_cell4, I.n.C, 1, s, I);
   }
   return check();
@@ -2080,8 +2321,20 @@ int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStr
   const float dts = I.k.dts[stage - 1];
   const int n = I.k.steps[stage - 1];
   if (!(I.k.fused && ss == 1))  // fused: the first edge update ran with the momentum tendency
-    launch(k_ac_edge, I.n.E, 1, s, I, dts, ss == 1 ? 1 : 0, damp_prev, I.k.dts_inv_coef[stage - 1],
-           (I.k.fused && n == 2) ? 1 : 0);
+  {
+    const float coef = I.k.dts_inv_coef[stage - 1];
+    const bool avgrp = I.k.fused && n == 2;
+    if (ss == 1)
+      launch_named("k_ac_edge", k_ac_edge<true, false, false>, I.n.E, 1, s, I, dts, coef);
+    else if (damp_prev && avgrp)
+      launch_named("k_ac_edge", k_ac_edge<false, true, true>, I.n.E, 1, s, I, dts, coef);
+    else if (damp_prev)
+      launch_named("k_ac_edge", k_ac_edge<false, true, false>, I.n.E, 1, s, I, dts, coef);
+    else if (avgrp)
+      launch_named("k_ac_edge", k_ac_edge<false, false, true>, I.n.E, 1, s, I, dts, coef);
+    else
+      launch_named("k_ac_edge", k_ac_edge<false, false, false>, I.n.E, 1, s, I, dts, coef);
+  }
   if (ss == 1)
     NEDISPATCH(I, launch_col_named("k_ac_cell", k_ac_cell<6, true>, I.n.C, s, I, dts),
                launch_col_named("k_ac_cell", k_ac_cell<7, true>, I.n.C, s, I, dts));
