@@ -1369,22 +1369,18 @@ KBCOL k_ac_cell(const Instance I, float dts) {
   const int k = lane();
   const int L = I.n.L;
   const int n = I.m.neoc[c];
-  float sg[NE];
-  slots(sg, I.m.csign + c * NE);
-  float sd[NE];
-  slots(sd, I.m.csdv + c * NE);
   int e[NE], o[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     e[s] = I.m.eoc[c * NE + s];
     o[s] = I.m.ocell[c * NE + s];  // the other cell of edge s
   }
-  float rup[NE], sto[NE];
-#pragma unroll
-  for (int s = 0; s < NE; ++s) {
-    rup[s] = AT(I.b.ru_p, e[s], k);
-    sto[s] = AT(I.b.s_theta, o[s], k);
-  }
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE), sdr = slot_row<NE>(I.m.csdv + c * NE);
+  const float ia = uni_load(I.m.invarea + c);
+  // wavefront w < 7 stages level row w for the block
+  const float *const lev[7] = {I.z.rdzw, I.z.ewm, I.z.ewp, I.z.etp, I.z.etm, I.z.fzm, I.z.fzp};
+  float lv = 0.0f;
+  if (w < 7) lv = AT(lev[w], 0, k);
   const float sthc = AT(I.b.s_theta, c, k);
   float rpp = 0.0f, rtpp = 0.0f, rwp = 0.0f, wwa = 0.0f;
   if (!FIRST) {
@@ -1400,15 +1396,27 @@ KBCOL k_ac_cell(const Instance I, float dts) {
   const float a = AT(I.b.atri, c, k), al = AT(I.b.alpha, c, k), ga = AT(I.b.gamma, c, k);
   const float srw = AT(I.b.s_rw, c, k), crw = AT(I.b.c_rw, c, k), damp = AT(I.z.dss, c, k);
   const float crho = AT(I.b.c_rho, c, k), wk = AT(I.b.w, c, k);
-  {
-    const float *const lev[7] = {I.z.rdzw, I.z.ewm, I.z.ewp, I.z.etp, I.z.etm, I.z.fzm, I.z.fzp};
-    stage_levels<7>(s_lev, lev, w, k);
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    fence(e[s]);
+    fence(o[s]);
   }
+  float rup[NE], sto[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) {
+    rup[s] = AT(I.b.ru_p, e[s], k);
+    sto[s] = AT(I.b.s_theta, o[s], k);
+  }
+  keep(rup, sto);
+  if (w < 7) s_lev[w * 64 + k] = lv;
+  settle(sgr, sdr);
+  float sg[NE], sd[NE];
+  slot_vals(sgr, sg);
+  slot_vals(sdr, sd);
   __syncthreads();
   const float rdzw = s_lev[0 * 64 + k], ewm = s_lev[1 * 64 + k], ewp = s_lev[2 * 64 + k];
   const float etp = s_lev[3 * 64 + k], etm = s_lev[4 * 64 + k], fzm = s_lev[5 * 64 + k], fzp = s_lev[6 * 64 + k];
   const float rtpp_old = rtpp;
-  const float ia = I.m.invarea[c];
   float r = 0.0f, t = 0.0f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
