@@ -276,28 +276,47 @@ __device__ __forceinline__ void lsq_frame(const Instance &I, int c, float &i11, 
 template <int NE>
 __device__ __forceinline__ void raw_features(const Instance &I, int c, int k, float raw[6]) {
   const Work &B = I.b;
+  const int n = I.m.neoc[c];
   int eo[NE];
-  float cs[NE], sn[NE];
-  float i11, i12, i22;
-  int n;
-  lsq_frame<NE>(I, c, i11, i12, i22, eo, cs, sn, n);
-  float b1 = 0.f, b2 = 0.f, fs = 0.f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s) eo[s] = I.m.eoc[c * NE + s];
+  // every load first (slot tables are padded, so each slot is readable); the
+  // least-squares frame and sums below are lsq_frame's, in the same order
+  float u[NE], ca[NE], sa[NE], fe[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    if (s < n) {
-      const float u = rowv(B.p_u, eo[s], k);
-      b1 += cs[s] * u;
-      b2 += sn[s] * u;
-      fs += I.m.fedge[eo[s]];
-    }
+    u[s] = rowv(B.p_u, eo[s], k);
+    ca[s] = I.m.cosang[eo[s]];
+    sa[s] = I.m.sinang[eo[s]];
+    fe[s] = I.m.fedge[eo[s]];
   }
+  const float wk = rowv(B.p_w, c, k), thk = rowv(B.p_theta, c, k), rk = rowv(B.rho, c, k);
+  const float qk = I.n.NT > 0 ? rowv(B.q, c, k) : 0.0f;
+  float m11 = 0.f, m12 = 0.f, m22 = 0.f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      m11 += ca[s] * ca[s];
+      m12 += ca[s] * sa[s];
+      m22 += sa[s] * sa[s];
+    }
+  const float det = m11 * m22 - m12 * m12;
+  const float i11 = m22 / det, i12 = -m12 / det, i22 = m11 / det;
+  float b1 = 0.f, b2 = 0.f, fs = 0.f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      b1 += ca[s] * u[s];
+      b2 += sa[s] * u[s];
+      fs += fe[s];
+    }
   const int L = I.n.L;
   raw[CH_UE] = (k < L) ? i11 * b1 + i12 * b2 : (k == L ? fs / (float)n : 0.0f);
   raw[CH_UN] = (k < L) ? i12 * b1 + i22 * b2 : 0.0f;
-  raw[CH_W] = (k <= L) ? rowv(B.p_w, c, k) : 0.0f;
-  raw[CH_TH] = (k < L) ? rowv(B.p_theta, c, k) : 0.0f;
-  raw[CH_RHO] = (k < L) ? rowv(B.rho, c, k) : 0.0f;
-  raw[CH_QV] = (k < L && I.n.NT > 0) ? rowv(B.q, c, k) : 0.0f;
+  raw[CH_W] = (k <= L) ? wk : 0.0f;
+  raw[CH_TH] = (k < L) ? thk : 0.0f;
+  raw[CH_RHO] = (k < L) ? rk : 0.0f;
+  raw[CH_QV] = (k < L && I.n.NT > 0) ? qk : 0.0f;
 }
 
 template <int NE>
@@ -325,15 +344,21 @@ __global__ void __launch_bounds__(256) k_aggregate(const Instance I, const bf16 
   if (c >= I.n.C) return;
   const int k = lane();
   const int n = I.m.neoc[c];
-  float s0 = 0.f, s1 = 0.f;
+  // every neighbour row first (the slot table is padded), then the sums in slot order
+  float h0[NE], h1[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    if (s < n) {
-      const int j = I.m.coc[c * NE + s];
-      s0 += (float)Hin[(size_t)j * HID + k];
-      s1 += (float)Hin[(size_t)j * HID + 64 + k];
-    }
+    const int j = I.m.coc[c * NE + s];
+    h0[s] = (float)Hin[(size_t)j * HID + k];
+    h1[s] = (float)Hin[(size_t)j * HID + 64 + k];
   }
+  float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      s0 += h0[s];
+      s1 += h1[s];
+    }
   Aout[(size_t)c * HID + k] = (bf16)(s0 / (float)n);
   Aout[(size_t)c * HID + 64 + k] = (bf16)(s1 / (float)n);
 }
@@ -347,16 +372,22 @@ __global__ void __launch_bounds__(256) k_aggregate_back(const Instance I, const 
   if (c >= I.n.C) return;
   const int k = lane();
   const int n = I.m.neoc[c];
-  float a0 = 0.f, a1 = 0.f;
+  float z0[NE], z1[NE], nj[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    if (s < n) {
-      const int j = I.m.coc[c * NE + s];
-      const float inv = 1.0f / (float)I.m.neoc[j];
-      a0 += dZ[(size_t)j * 2 * HID + HID + k] * inv;
-      a1 += dZ[(size_t)j * 2 * HID + HID + 64 + k] * inv;
-    }
+    const int j = I.m.coc[c * NE + s];
+    nj[s] = (float)I.m.neoc[j];
+    z0[s] = dZ[(size_t)j * 2 * HID + HID + k];
+    z1[s] = dZ[(size_t)j * 2 * HID + HID + 64 + k];
   }
+  float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+  for (int s = 0; s < NE; ++s)
+    if (s < n) {
+      const float inv = 1.0f / nj[s];
+      a0 += z0[s] * inv;
+      a1 += z1[s] * inv;
+    }
   const size_t o = (size_t)c * HID;
   dH[o + k] = (dHnext[o + k] + dZ[(size_t)c * 2 * HID + k]) + a0;
   dH[o + 64 + k] = (dHnext[o + 64 + k] + dZ[(size_t)c * 2 * HID + 64 + k]) + a1;
