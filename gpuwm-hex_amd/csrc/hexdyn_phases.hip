@@ -119,15 +119,6 @@ __device__ __forceinline__ float exner_pow(const Instance &I, float arg) {
   return hexdyn_gnu_powf_fix(arg, (float)d, d, I.m.powf_ties, I.k.powf_near_tie);
 }
 
-// Wave-uniform per-slot coefficients, loaded unconditionally (tables are
-// padded, so every slot is addressable) so the scalar loads issue together
-// instead of one round trip per slot under the slot-count test.
-template <int N, typename T>
-__device__ __forceinline__ void slots(T (&out)[N], const T *__restrict__ p) {
-#pragma unroll
-  for (int s = 0; s < N; ++s) out[s] = p[s];
-}
-
 // Per-slot coefficients of one entity (its row of N values in an [n][N]
 // table), read by one vector load with lane s taking slot s and issued with
 // the row loads; slot s is read from lane s where it is used.  The same
@@ -613,25 +604,6 @@ KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const 
   }
 }
 
-// sum_s csign*dv*x[e_s] / area over the cell's edges, in slot order
-template <int NE>
-__device__ __forceinline__ float cell_flux_div(const Instance &I, int c, int k, const float *__restrict__ x) {
-  const int n = I.m.neoc[c];
-  int e[NE];
-  float xv[NE];
-#pragma unroll
-  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-#pragma unroll
-  for (int s = 0; s < NE; ++s) xv[s] = AT(x, e[s], k);
-  float sd[NE];
-  slots(sd, I.m.csdv + c * NE);
-  float acc = 0.0f;
-#pragma unroll
-  for (int s = 0; s < NE; ++s)
-    if (s < n) acc = acc + sd[s] * xv[s];
-  return acc * I.m.invarea[c];
-}
-
 // ------------------------------------------------------- horizontal mixing
 // v8.4.1 2-D Smagorinsky mixing, once per dynamics subcycle from the
 // subcycle-start state (cuda_horizontal_v841.py:205-583, host sequence
@@ -639,38 +611,8 @@ __device__ __forceinline__ float cell_flux_div(const Instance &I, int c, int k, 
 // del2/del4 filters).  Edge density is 0.5*(rho[cell0]+rho[cell1]) of the
 // current rho_zz, as the subcycle-start solve formed it.
 
-// eddy viscosity from the deformation tensor (smagorinsky_v841_f32)
-__device__ __forceinline__ float kdiff_from_deformation(const Instance &I, int k, float dudx, float dudy, float dvdx,
-                                                       float dvdy);
-// 2-D Smagorinsky viscosity of cell c at level k (lane k), from the
-// deformation of the cell's edge winds
-template <int NE>
-__device__ __forceinline__ float mix_kdiff_value(const Instance &I, int c, int k) {
-  const int n = I.m.neoc[c];
-  int e[NE];
-#pragma unroll
-  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-  float un[NE], vn[NE], c2[NE], s2[NE], cs[NE];
-#pragma unroll
-  for (int s = 0; s < NE; ++s) {
-    un[s] = AT(I.b.u, e[s], k);
-    vn[s] = AT(I.b.v, e[s], k);
-  }
-  slots(c2, I.m.defc2 + c * NE);
-  slots(s2, I.m.defs2 + c * NE);
-  slots(cs, I.m.defcs + c * NE);
-  float dudx = 0.0f, dudy = 0.0f, dvdx = 0.0f, dvdy = 0.0f;
-#pragma unroll
-  for (int s = 0; s < NE; ++s)
-    if (s < n) {
-      dudx = (dudx + c2[s] * un[s]) - cs[s] * vn[s];
-      dudy = (dudy + cs[s] * un[s]) - s2[s] * vn[s];
-      dvdx = (dvdx + cs[s] * un[s]) + c2[s] * vn[s];
-      dvdy = (dvdy + s2[s] * un[s]) + cs[s] * vn[s];
-    }
-  return kdiff_from_deformation(I, k, dudx, dudy, dvdx, dvdy);
-}
-// eddy viscosity of level k from the deformation sums of a cell
+// 2-D Smagorinsky eddy viscosity of level k (lane k) from the deformation
+// sums of a cell's edge winds (smagorinsky_v841_f32)
 __device__ __forceinline__ float kdiff_from_deformation(const Instance &I, int k, float dudx, float dudy, float dvdx,
                                                        float dvdy) {
   const float d11 = 2.0f * dudx, d22 = 2.0f * dvdy, d12 = dudy + dvdx;
@@ -1677,22 +1619,22 @@ KB k_rebuild_cell(const Instance I) {
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L;
-  float rho = 1.0f;
-  if (k < L) {
-    rho = AT(I.b.rho, c, k);
-    const float rt = AT(I.b.rt, c, k);
-    const float zk = AT(I.z.zz, c, k);
-    const float ex = exner_pow(I, (zk * I.k.cfac) * rt);
-    const float rtp = rt - AT(I.z.rtb, c, k);
-    AT(I.b.theta, c, k) = rt / rho;
-    AT(I.b.exner, c, k) = ex;
-    AT(I.b.rho_p, c, k) = rho - AT(I.z.rb, c, k);
-    AT(I.b.rtheta_p, c, k) = rtp;
-    AT(I.b.pp, c, k) = (zk * I.k.rgas) * (ex * rtp + AT(I.z.rtb, c, k) * (ex - AT(I.z.exb, c, k)));
-  }
+  // every load before any store (the terrain w recovery gathers among them)
+  const float rho_k = AT(I.b.rho, c, k), rt = AT(I.b.rt, c, k), zk = AT(I.z.zz, c, k);
+  const float rtb = AT(I.z.rtb, c, k), rb = AT(I.z.rb, c, k), exb = AT(I.z.exb, c, k);
   const float rwk = AT(I.b.rw, c, k);
+  const float rho = k < L ? rho_k : 1.0f;
   const float *ru = I.b.ru;
   const float wv = recover_w_column<NE>(I, c, k, rho, rwk, [&](int e) { return AT(ru, e, k); });
+  if (k < L) {
+    const float ex = exner_pow(I, (zk * I.k.cfac) * rt);
+    const float rtp = rt - rtb;
+    AT(I.b.theta, c, k) = rt / rho;
+    AT(I.b.exner, c, k) = ex;
+    AT(I.b.rho_p, c, k) = rho - rb;
+    AT(I.b.rtheta_p, c, k) = rtp;
+    AT(I.b.pp, c, k) = (zk * I.k.rgas) * (ex * rtp + rtb * (ex - exb));
+  }
   if (k <= L) AT(I.b.w, c, k) = wv;
 }
 
