@@ -693,6 +693,39 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
   rc |= h->upload(&M.fvert, fvert); rc |= h->upload(&M.cosang, cosang); rc |= h->upload(&M.sinang, sinang);
   rc |= h->upload(&M.adv, adv); rc |= h->upload(&M.adv3, adv3);
   {
+    // edge tiles of the staged flux gathers: distinct advection cells in
+    // order of first use, and each slot's index among them
+    const int ntiles = (E + HX_TILE_E - 1) / HX_TILE_E;
+    std::vector<int> tcell((size_t)ntiles * HX_TILE_U, 0), ntcell(ntiles, 0), advl((size_t)E * MA, 0);
+    int tu = 0;
+    bool fits = true;
+    for (int t = 0; t < ntiles && fits; ++t) {
+      int cnt = 0;
+      int *row = &tcell[(size_t)t * HX_TILE_U];
+      for (int ne = t * HX_TILE_E; ne < std::min(E, (t + 1) * HX_TILE_E) && fits; ++ne)
+        for (int sl = 0; sl < MA; ++sl) {
+          const int cell = advc[(size_t)ne * MA + sl];  // padded slots hold cell 0 and are staged too
+          int j = 0;
+          while (j < cnt && row[j] != cell) ++j;
+          if (j == cnt) {
+            if (cnt == HX_TILE_U) {
+              fits = false;
+              break;
+            }
+            row[cnt++] = cell;
+          }
+          advl[(size_t)ne * MA + sl] = j;
+        }
+      for (int j = cnt; j < HX_TILE_U; ++j) row[j] = row[0];
+      ntcell[t] = cnt;
+      tu = std::max(tu, cnt);
+    }
+    n.TU = fits ? tu : 0;
+    rc |= h->upload(&M.tcell, tcell);
+    rc |= h->upload(&M.ntcell, ntcell);
+    rc |= h->upload(&M.advl, advl);
+  }
+  {
     std::vector<int> a(h->old_c_h), b(h->old_e_h), cvec(h->old_v_h);
     const int *pa, *pb, *pc;
     rc |= h->upload(&pa, a); rc |= h->upload(&pb, b); rc |= h->upload(&pc, cvec);
@@ -1306,8 +1339,11 @@ void drop_graphs(hexdyn_handle *h) {
   h->graphs.clear();
 }
 // the column kernels' block shape is fixed (16 columns through LDS)
+// kernels with a fixed block shape: the column kernels (16 columns per block
+// through LDS) and the edge-tiled flux pass (HX_TILE_E edges per block)
 bool column_kernel(const char *name) {
-  return std::strcmp(name, "k_vcoef") == 0 || std::strcmp(name, "k_ac_cell") == 0;
+  return std::strcmp(name, "k_vcoef") == 0 || std::strcmp(name, "k_ac_cell") == 0 ||
+         std::strcmp(name, "k_diag_edge_flux_t") == 0;
 }
 }  // namespace
 

@@ -22,10 +22,17 @@ namespace hexdyn {
 
 struct Sizes {
   int C, E, V, L, ME, ME2, MA, NT, NM;
+  int TU;  // distinct advection cells per edge tile (0: the tiled flux gathers are off)
   // wavefronts (rows) per block of the launch in flight: each launch passes
   // its own copy, so a kernel finds its tile with its other scalar arguments
   int WPB = 4;
 };
+
+// Edge tiles of the staged flux gathers: HX_TILE_E edges per block, at most
+// HX_TILE_U distinct advection cells per tile (larger tiles turn the staged
+// kernels off for the mesh).
+constexpr int HX_TILE_E = 8;
+constexpr int HX_TILE_U = 56;
 
 struct Mesh {
   // connectivity (renumbered)
@@ -41,6 +48,12 @@ struct Mesh {
   const int *neoe;   // [E]
   const int *nadv;   // [E]
   const int *advc;   // [E][MA]
+  // tiles of HX_TILE_E consecutive edges: the distinct cells of their
+  // advection stencils (padded with the tile's first cell) and each edge
+  // slot's index in that list
+  const int *tcell;  // [E / HX_TILE_E rounded up][HX_TILE_U]
+  const int *ntcell; // [E / HX_TILE_E rounded up] distinct cells of the tile
+  const int *advl;   // [E][MA]
   // geometry
   const float *csign;   // [C][ME] +1 first cell of the edge, -1 otherwise
   const float *vsign;   // [V][3]
@@ -161,10 +174,14 @@ struct LaunchTune {
   int def = 4;  // kernels not listed
   char names[kMax][32];
   int wpb[kMax];
-  int waves_per_block(const char *name) const {
+  int find(const char *name) const {
     for (int i = 0; i < n; ++i)
-      if (std::strncmp(names[i], name, 31) == 0) return wpb[i];
-    return def;
+      if (std::strncmp(names[i], name, 31) == 0) return i;
+    return -1;
+  }
+  int waves_per_block(const char *name) const {
+    const int i = find(name);
+    return i < 0 ? def : wpb[i];
   }
 };
 

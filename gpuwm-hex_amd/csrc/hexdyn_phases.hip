@@ -491,6 +491,85 @@ KB k_diag_edge_flux(const Instance I, const float *__restrict__ u) {
   store_edge_flux<!RECON>(I, e, k, f);
 }
 
+// The same pass over a tile of HX_TILE_E edges, one wavefront each: the
+// theta and w rows of the tile's distinct advection cells cross the texture
+// path once per tile into LDS, and the ten-cell stencils read them there.
+// The gathers are bound by the texture path's rate per row load (about ten
+// cycles per 256-byte row per CU), and consecutive edges along the curve
+// share most stencil cells (about 2.8 distinct cells per edge instead of 10).
+template <int NE, bool RECON>
+__global__ void __launch_bounds__(64 * HX_TILE_E) k_diag_edge_flux_t(const Instance I, const float *__restrict__ u) {
+  constexpr int NX = 2 * NE - 2, J = (HX_TILE_U + HX_TILE_E - 1) / HX_TILE_E;
+  constexpr bool CORR = !RECON;
+  __shared__ float s_th[HX_TILE_U * 64], s_w[HX_TILE_U * 64];
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), k = lane();
+  const int tile = block_entity0(1);
+  const int e0 = tile * HX_TILE_E + w;
+  const bool live = e0 < I.n.E;
+  const int e = live ? e0 : I.n.E - 1;  // every wavefront stages its share and reaches the barrier
+  const int ncell = I.m.ntcell[tile];
+  EdgeDiagIn<NE, RECON> a;
+  EdgeFluxIn<NE> b;
+  diag_edge_index(I, e, a);
+  b.c0 = I.m.coe[2 * e];
+  b.c1 = I.m.coe[2 * e + 1];
+#pragma unroll
+  for (int sl = 0; sl < NX; ++sl) b.ac[sl] = I.m.advl[e * NX + sl];  // slot -> row of the tile's cell list
+  b.na = I.m.nadv[e];
+  b.av = slot_row<NX>(I.m.adv + e * NX);
+  b.a3v = slot_row<NX>(I.m.adv3 + e * NX);
+  b.dv = CORR ? uni_load(I.m.dv + e) : 0.0f;
+  b.fzm = AT(I.z.fzm, 0, k);
+  b.fzp = AT(I.z.fzp, 0, k);
+  int cr[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) cr[j] = I.m.tcell[tile * HX_TILE_U + min(w + j * HX_TILE_E, HX_TILE_U - 1)];
+  diag_edge_fence(a);
+  fence(b.c0);
+  fence(b.c1);
+#pragma unroll
+  for (int j = 0; j < J; ++j) fence(cr[j]);
+  float st[J], sw[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    st[j] = sw[j] = 0.0f;
+    if (w + j * HX_TILE_E < ncell) {  // this wavefront's rows of the tile's list
+      st[j] = AT(I.b.theta, cr[j], k);
+      sw[j] = AT(I.b.w, cr[j], k);
+    }
+  }
+  diag_edge_rows(I, u, e, k, a);
+  b.rue = AT(I.b.c_ru, e, k);
+  b.sru = b.sth0 = b.sth1 = 0.0f;
+  if (CORR) {
+    b.sru = AT(I.b.s_ru, e, k);
+    b.sth0 = AT(I.b.s_theta, b.c0, k);
+    b.sth1 = AT(I.b.s_theta, b.c1, k);
+  }
+  keep(st, sw);
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+    if (w + j * HX_TILE_E < ncell) {
+      s_th[(w + j * HX_TILE_E) * 64 + k] = st[j];
+      s_w[(w + j * HX_TILE_E) * 64 + k] = sw[j];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int sl = 0; sl < NX; ++sl) {
+    b.thn[sl] = s_th[b.ac[sl] * 64 + k];
+    b.wn[sl] = s_w[b.ac[sl] * 64 + k];
+  }
+  keep(b.thn, b.wn, b.rue, b.sru, b.sth0, b.sth1, b.fzm, b.fzp, b.dv);
+  keep(a.un, a.vrow, a.pv0, a.pv1, a.pc0, a.pc1, a.ue, a.idv, a.idc);
+  settle(a.wt, b.av, b.a3v);
+  const EdgeDiag d = diag_edge_compute<NE, RECON>(I, e, a);
+  const EdgeFlux f = tend_flux_compute<NE, CORR>(I, e, b);
+  if (live) {
+    store_edge_diag(I, e, k, RECON, d);
+    store_edge_flux<CORR>(I, e, k, f);
+  }
+}
+
 // ---------------------------------------------------------- moist coefficients
 KB k_moist_cell(const Instance I) {
   const int c = wave_entity(I);
@@ -2169,12 +2248,26 @@ inline int check() {
 #define LAUNCH_NE(I, kernel, ...) \
   NEDISPATCH(I, launch_named(#kernel, kernel<6>, __VA_ARGS__), launch_named(#kernel, kernel<7>, __VA_ARGS__))
 
+// The edge-tiled flux pass runs unless the mesh has a tile with more than
+// HX_TILE_U distinct stencil cells or the caller set a launch shape for the
+// gather form (k_diag_edge_flux) explicitly.
+inline bool tiled_flux_on(const Instance &I) {
+  return I.n.TU > 0 && !(I.tune && I.tune->find("k_diag_edge_flux") >= 0);
+}
+
 int solve_diagnostics(const Instance &I, int rk, hipStream_t s) {
   launch(k_diag_vertex, I.n.V, 1, s, I, (const float *)I.b.u);
   LAUNCH_NE(I, k_diag_cell, I.n.C, 1, s, I, (const float *)I.b.u);
   const float *u = I.b.u;
   const bool six = I.n.ME == 6;
-  if (I.k.fused) {  // diagnostics after stage rk feed stage rk % 3 + 1 (the next subcycle's first after rk 3)
+  if (I.k.fused && tiled_flux_on(I)) {
+    const int ntiles = (I.n.E + HX_TILE_E - 1) / HX_TILE_E;
+    if (g_prof_on) prof_mark("k_diag_edge_flux_t", s, 1);
+    auto kt = rk == 3 ? (six ? k_diag_edge_flux_t<6, true> : k_diag_edge_flux_t<7, true>)
+                      : (six ? k_diag_edge_flux_t<6, false> : k_diag_edge_flux_t<7, false>);
+    hipLaunchKernelGGL(kt, dim3(ntiles), dim3(64 * HX_TILE_E), 0, s, I, u);
+    if (g_prof_on) prof_mark("k_diag_edge_flux_t", s, 0);
+  } else if (I.k.fused) {  // diagnostics after stage rk feed stage rk % 3 + 1 (the next subcycle's first after rk 3)
     if (rk == 3)
       launch_named("k_diag_edge_flux", six ? k_diag_edge_flux<6, true> : k_diag_edge_flux<7, true>, I.n.E, 1, s, I,
                    u);

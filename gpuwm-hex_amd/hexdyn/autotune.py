@@ -28,7 +28,7 @@ from typing import Dict, Optional
 from .lib import lib
 
 CANDIDATES = (1, 2, 4, 8, 16)
-COLUMN_KERNELS = {"k_vcoef", "k_ac_cell"}
+COLUMN_KERNELS = {"k_vcoef", "k_ac_cell", "k_diag_edge_flux_t"}  # fixed block shapes
 
 
 def _cache_dir() -> Path:

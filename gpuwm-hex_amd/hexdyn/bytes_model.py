@@ -115,6 +115,7 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     # delsq_theta, delsq_w, euler_theta, tend_w, euler_rw | euler_theta, euler_rw
     "k_mix_cell4": [("C", "L"), ("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("C", "L"), ("C", "W")],
 }
+_K["k_diag_edge_flux_t"] = _K["k_diag_edge_flux"]  # the edge-tiled form moves the same arrays
 
 
 def kernel_bytes(kernel: str, *, C: int, E: int, V: int, L: int, T: int, max_edges: int = 6) -> float:

@@ -240,7 +240,10 @@ int hexdyn_graph_count(const hexdyn_handle *h);
  * vertex row): wavefronts per block, 1, 2, 4 (the default), 8 or 16, by kernel name
  * ("k_diag_edge_flux", ...; "*" resets every kernel to the given value).
  * The column kernels (k_vcoef, k_ac_cell: 16 columns per block through LDS)
- * are refused.  Results do not depend on it.  Drops the captured step graphs.
+ * and the edge-tiled flux pass (k_diag_edge_flux_t: 8 edges per block
+ * through LDS) are refused; setting k_diag_edge_flux selects that pass's
+ * gather form (one block of whole-edge wavefronts, no LDS) instead of the
+ * tiled one.  Results do not depend on it.  Drops the captured step graphs.
  * The autotuner (hexdyn/autotune.py) picks it per kernel by timing.  (The
  * reference tunes its level chunking instead, cuda_edge_hoist.py:63-101.) */
 int hexdyn_set_launch(hexdyn_handle *h, const char *kernel, int waves_per_block);

@@ -22,6 +22,8 @@ def test_launch_shapes_give_the_same_bits(get_case):
     print("[gpu] launch shapes x1.2562-moist", flush=True)
     case = get_case("x1.2562-moist", config=DycoreConfig(config_dt=1800.0, config_horiz_mixing="2d_smagorinsky"))
     runs = []
+    # the last table sets the edge-flux pass's gather form explicitly (the
+    # others run its edge-tiled form): both give the same bits
     for table in ({"*": 4}, {"*": 1}, {"*": 2}, {"*": 8}, {"*": 16},
                   {"*": 4, "k_diag_edge_flux": 16, "k_tend_mom_ac": 2, "k_sc_edge": 8}):
         d = HexDycoreDriver.from_host(case)
@@ -41,7 +43,8 @@ def test_refusals(get_case):
     from hexdyn.driver import HexDycoreDriver
 
     d = HexDycoreDriver.from_host(get_case("x1.642-moist"))
-    for k, w in (("k_vcoef", 2), ("k_ac_cell", 1), ("k_rec_cell", 3), ("k_rec_cell", 32), ("notakernel", 2)):
+    for k, w in (("k_vcoef", 2), ("k_ac_cell", 1), ("k_diag_edge_flux_t", 4), ("k_rec_cell", 3), ("k_rec_cell", 32),
+                 ("notakernel", 2)):
         with pytest.raises(ValueError):
             autotune.set_launch(d, k, w)
     d.close()
