@@ -59,7 +59,8 @@ _K: Dict[str, List[Tuple[str, str]]] = {
                          ("E", "L"), ("E", "L"), ("E", "W")],
     "k_moist_cell": [("SC", "L"), ("C", "L"), ("C", "L")],
     "k_moist_edge": [("SC", "L"), ("E", "L")],
-    "k_vcoef": [("C", "L")] * 10 + [("C", "W")],
+    # zz, exner, theta, cqw, qtot, rtheta_p, rb, rtb, exb | cofwr, cofwz, cofwt, atri, alpha, gamma, coftz
+    "k_vcoef": [("C", "L")] * 15 + [("C", "W")],
     # with 2-D Smagorinsky mixing (the default): + u, v | kdiff
     "k_sub_cell": [("C", "L")] * 12 + [("C", "W")] * 3 + [("E", "L")] * 3 + [("C", "L")],
     # with mixing: + divergence, vorticity, kdiff, c_rho | delsq_u
