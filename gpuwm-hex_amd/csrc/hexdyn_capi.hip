@@ -795,7 +795,7 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
   float **cell_bufs[] = {&B.rho, &B.rt, &B.rw, &B.theta, &B.exner, &B.rho_p, &B.rtheta_p, &B.pp, &B.w,
                          &B.p_theta, &B.p_exner, &B.p_rho_p, &B.p_rtheta_p, &B.p_pp, &B.p_w,
                          &B.c_rho, &B.c_rt, &B.c_rw, &B.s_rho, &B.s_rt, &B.s_rw, &B.s_theta, &B.s_rho_p,
-                         &B.s_rtheta_p, &B.s_exner, &B.qtot, &B.cqw, &B.cofwr, &B.cofwz, &B.cofwt, &B.coftz,
+                         &B.s_rtheta_p, &B.s_exner, &B.qtot, &B.cqw, &B.cofwz, &B.cofwt, &B.coftz,
                          &B.atri, &B.alpha, &B.gamma, &B.dpdz, &B.euler_rw, &B.tend_rho, &B.tend_rt, &B.tend_rw,
                          &B.hdiv, &B.rw_p, &B.ww_avg, &B.rho_pp, &B.rtheta_pp, &B.rtheta_pp_old, &B.ke,
                          &B.divergence, &B.pv_cell, &B.flux_w, &B.split_w, &B.target};

@@ -109,7 +109,7 @@ struct Work {
   // moist coefficients
   float *qtot, *cqw, *cqu;
   // implicit coefficients
-  float *cofwr, *cofwz, *cofwt, *coftz, *atri, *alpha, *gamma;
+  float *cofwz, *cofwt, *coftz, *atri, *alpha, *gamma;
   // subcycle pools
   float *dpdz, *euler_ru, *euler_rw, *tend_rho;
   // stage tendencies and edge fluxes

@@ -673,7 +673,6 @@ KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const 
   if (live && k <= L) {
     AT(I.b.coftz, c, k) = coftz;
     if (k < L) {
-      AT(I.b.cofwr, c, k) = cofwr;
       AT(I.b.cofwz, c, k) = cofwz;
       AT(I.b.cofwt, c, k) = cofwt;
       AT(I.b.atri, c, k) = a;
@@ -1411,10 +1410,10 @@ KBCOL k_ac_cell(const Instance I, float dts) {
     wwa = AT(I.b.ww_avg, c, k);
   }
   const float coftz = AT(I.b.coftz, c, k), cofwt = AT(I.b.cofwt, c, k);
-  const float cofwz = AT(I.b.cofwz, c, k), cofwr = AT(I.b.cofwr, c, k);
+  const float cofwz = AT(I.b.cofwz, c, k);
   const float trho = AT(I.b.tend_rho, c, k), trt = AT(I.b.tend_rt, c, k), trw = AT(I.b.tend_rw, c, k);
   const float zk = AT(I.z.zz, c, k);
-  const float a = AT(I.b.atri, c, k), al = AT(I.b.alpha, c, k), ga = AT(I.b.gamma, c, k);
+  const float al = AT(I.b.alpha, c, k), ga = AT(I.b.gamma, c, k);
   const float srw = AT(I.b.s_rw, c, k), crw = AT(I.b.c_rw, c, k), damp = AT(I.z.dss, c, k);
   const float crho = AT(I.b.c_rho, c, k), wk = AT(I.b.w, c, k);
 #pragma unroll
@@ -1460,6 +1459,17 @@ KBCOL k_ac_cell(const Instance I, float dts) {
   const float cofwt_m = up1(cofwt);
   const float etp_m = lane_off(etp, k, -1), etm_m = lane_off(etm, k, -1);
   const float crho_m = up1(crho);
+  // cofwr and the sub-diagonal a of the implicit w system at levels 1..L-1,
+  // formed from zz, the level rows and the stored coefficients of levels k
+  // and k-1 by the same operations as in k_vcoef (which stores cofwz, coftz
+  // and cofwt for this kernel, and a only for export)
+  const float rdzw_m = lane_off(rdzw, k, -1), ewp_m = lane_off(ewp, k, -1), coftz_m = up1(coftz);
+  float cofwr = 0.0f, a = 0.0f;
+  if (k >= 1 && k < L) {
+    cofwr = (0.5f * I.k.gravity) * (fzm * zk + fzp * zk_m);
+    a = (((-cofwz) * coftz_m) * rdzw_m) * zk_m + cofwr * rdzw_m - (cofwt_m * coftz_m) * rdzw_m;
+    a = (a * etp_m) * ewp_m;
+  }
   float col = rwp;
   if (k >= 1 && k < L) {
     float x = trw - cofwz * (((etp * zk) * ts - (etp_m * zk_m) * ts_m) + ((etm * zk) * rtpp - (etm_m * zk_m) * rtpp_m));

@@ -59,8 +59,8 @@ _K: Dict[str, List[Tuple[str, str]]] = {
                          ("E", "L"), ("E", "L"), ("E", "W")],
     "k_moist_cell": [("SC", "L"), ("C", "L"), ("C", "L")],
     "k_moist_edge": [("SC", "L"), ("E", "L")],
-    # zz, exner, theta, cqw, qtot, rtheta_p, rb, rtb, exb | cofwr, cofwz, cofwt, atri, alpha, gamma, coftz
-    "k_vcoef": [("C", "L")] * 15 + [("C", "W")],
+    # zz, exner, theta, cqw, qtot, rtheta_p, rb, rtb, exb | cofwz, cofwt, atri, alpha, gamma, coftz
+    "k_vcoef": [("C", "L")] * 14 + [("C", "W")],
     # with 2-D Smagorinsky mixing (the default): + u, v | kdiff
     "k_sub_cell": [("C", "L")] * 12 + [("C", "W")] * 3 + [("E", "L")] * 3 + [("C", "L")],
     # with mixing: + divergence, vorticity, kdiff, c_rho | delsq_u
@@ -83,10 +83,11 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     # tend_ru, ru_p, rtheta_pp, rtheta_pp_old, s_theta, zz, cqu, s_exner, zxu, rho_pp | ru_p, ru_avg
     "k_ac_edge": [("E", "L"), ("E", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"),
                   ("C", "L"), ("E", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
-    # rho_pp, rtheta_pp, rw_p, ww_avg, ru_p, s_theta, tend_rho, tend_rt, tend_rw, coftz, cofwz, cofwr,
-    # cofwt, atri, alpha, gamma, zz, dss, s_rw, c_rw, c_rho, w | rw_p, ww_avg, rho_pp, rtheta_pp, rtheta_pp_old
+    # rho_pp, rtheta_pp, rw_p, ww_avg, ru_p, s_theta, tend_rho, tend_rt, tend_rw, coftz, cofwz,
+    # cofwt, alpha, gamma, zz, dss, s_rw, c_rw, c_rho, w | rw_p, ww_avg, rho_pp, rtheta_pp, rtheta_pp_old
+    # (cofwr and the sub-diagonal are formed from zz and the coefficients read)
     "k_ac_cell": [("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("E", "L"), ("C", "L"), ("C", "L"),
-                  ("C", "L"), ("C", "W"), ("C", "W"), ("C", "L"), ("C", "L"), ("C", "L"), ("C", "L"),
+                  ("C", "L"), ("C", "W"), ("C", "W"), ("C", "L"), ("C", "L"),
                   ("C", "L"), ("C", "L"), ("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("C", "L"),
                   ("C", "W"), ("C", "W"), ("C", "W"), ("C", "L"), ("C", "L"), ("C", "L")],
     "k_divdamp": [("C", "L"), ("C", "L"), ("C", "L"), ("E", "L"), ("E", "L")],
