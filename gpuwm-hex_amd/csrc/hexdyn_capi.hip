@@ -585,11 +585,13 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
   std::vector<float> csdv((size_t)C * ME, 0.0f), cdcdv((size_t)C * ME, 0.0f), ckite((size_t)C * ME, 0.0f),
       vdc(3 * V), vdcdv(3 * V), ecos((size_t)E * ME2, 0.0f), esin((size_t)E * ME2, 0.0f);
   std::vector<int> ocell((size_t)C * ME, 0);
+  int ocp = 1;
   for (int nc = 0; nc < C; ++nc)
     for (int s = 0; s < neoc[nc]; ++s) {
       const size_t o = (size_t)nc * ME + s;
       const int a0 = coe[2 * eoc[o]], a1 = coe[2 * eoc[o] + 1];
       ocell[o] = a0 == nc ? a1 : a0;
+      if (ocell[o] != coc[o]) ocp = 0;
       csdv[o] = csign[o] * dv[eoc[o]];
       cdcdv[o] = dcdv[eoc[o]];
       ckite[o] = kite[3 * voc[o] + kfc[o]];
@@ -610,6 +612,7 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
   rc |= h->upload(&M.csdv, csdv); rc |= h->upload(&M.cdcdv, cdcdv); rc |= h->upload(&M.ckite, ckite);
   rc |= h->upload(&M.vdc, vdc); rc |= h->upload(&M.vdcdv, vdcdv);
   rc |= h->upload(&M.ecos, ecos); rc |= h->upload(&M.esin, esin); rc |= h->upload(&M.ocell, ocell);
+  n.OCP = ocp;
   if (mixing) {
     // binary32 scalars staged as the reference host does (cuda_horizontal_v841.py:1336-1350)
     const float len = c->config_len_disp, csm = c->config_smagorinsky_coef;
@@ -808,7 +811,7 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
   float **scal_bufs[] = {&B.q, &B.q_stage, &B.q_next, &B.fct_mn, &B.fct_mx, &B.fct_mass, &B.fct_sin,
                          &B.fct_sout, &B.fct_vres};
   for (float **p : scal_bufs) rc |= h->alloc(p, sr);
-  float **sedge_bufs[] = {&B.ev, &B.fct_hres, &B.fct_upw};
+  float **sedge_bufs[] = {&B.ev, &B.fct_hres};
   for (float **p : sedge_bufs) rc |= h->alloc(p, ser);
   if (mixing) {
     float **mix_cell[] = {&B.kdiff, &B.delsq_div, &B.delsq_theta, &B.delsq_w, &B.mix_tw, &B.euler_theta};

@@ -23,6 +23,7 @@ namespace hexdyn {
 struct Sizes {
   int C, E, V, L, ME, ME2, MA, NT, NM;
   int TU;  // distinct advection cells per edge tile (0: the tiled flux gathers are off)
+  int OCP;  // 1: in every cell, cellsOnCell slot s is the other cell of edgesOnCell slot s
   // wavefronts (rows) per block of the launch in flight: each launch passes
   // its own copy, so a kernel finds its tile with its other scalar arguments
   int WPB = 4;
@@ -122,7 +123,7 @@ struct Work {
   float *flux_u, *flux_w, *split_u, *split_w;
   // transport
   float *q_stage, *q_next, *target, *ev, *fct_mn, *fct_mx, *fct_mass, *fct_sin, *fct_sout, *fct_vres,
-      *fct_hres, *fct_upw;
+      *fct_hres;
   // horizontal mixing: viscosity, Laplacians and the theta Euler pool
   float *kdiff, *delsq_u, *delsq_div, *delsq_vort, *delsq_theta, *delsq_w, *mix_tw, *euler_theta;
   // held physics tendencies (rows; allocated when first installed)

@@ -1987,30 +1987,35 @@ KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt, const f
         for (int s = 0; s < NX; ++s)
           if (s < na) ra = ra + (uk * wt[s]) * v[g][s];
       }
-      const float upa = dvdt * (max0(uk) * qa1 + min0(uk) * qa2);
-      if (k < I.n.L) {
-        AT(I.b.fct_upw, (sp0 + g) * I.n.E + e, k) = upa;
-        AT(I.b.fct_hres, (sp0 + g) * I.n.E + e, k) = dt * ra - upa;
-      }
+      const float upa = dvdt * (max0(uk) * qa1 + min0(uk) * qa2);  // formed again by k_fct_cell1
+      if (k < I.n.L) AT(I.b.fct_hres, (sp0 + g) * I.n.E + e, k) = dt * ra - upa;
     }
   }
 }
 
-template <int NE>
+// The upwind flux of each edge, dv*dt*(max(u,0)*q[cell0] + min(u,0)*q[cell1])
+// (k_fct_edge1 forms the same value for the high-order residual), is formed
+// here from the edge's split u and the old mixing ratios of its two cells,
+// which the bounds read anyway: one u row per edge instead of one flux row
+// per edge and species.  PAIR: cellsOnCell slot s is the other cell of
+// edgesOnCell slot s in every cell (as in MPAS meshes); otherwise that cell's
+// row is read separately.
+template <int NE, bool PAIR>
 KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const float *__restrict__ qold, float qadd) {
-  constexpr int G = HEXDYN_G_FCT_CELL1;
+  constexpr int G = HEXDYN_G_FCT_CELL1, NP = PAIR ? 1 : NE;
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
   const int L = I.n.L, NT = I.n.NT;
   const int n = I.m.neoc[c];
-  int nb[NE], e[NE];
+  int nb[NE], e[NE], oc[NP];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     nb[s] = I.m.coc[c * NE + s];
     e[s] = I.m.eoc[c * NE + s];
+    if (!PAIR) oc[s < NP ? s : 0] = I.m.ocell[c * NE + s];
   }
-  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE);
+  SlotRow<NE> sgr = slot_row<NE>(I.m.csign + c * NE), sdr = slot_row<NE>(I.m.csdv + c * NE);
   const float ia = uni_load(I.m.invarea + c);
   const float orho = AT(I.b.rho, c, k);
   const float wk = AT(I.b.split_w, c, k);
@@ -2021,8 +2026,14 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const f
     fence(nb[s]);
     fence(e[s]);
   }
+  if (!PAIR)
+#pragma unroll
+    for (int s = 0; s < NP; ++s) fence(oc[s]);
+  float uh[NE];
+#pragma unroll
+  for (int s = 0; s < NE; ++s) uh[s] = AT(I.b.split_u, e[s], k);
   for (int sp0 = 0; sp0 < NT; sp0 += G) {
-    float qsk[G], qol[G], qn[G][NE], upw[G][NE], hres[G][NE];
+    float qsk[G], qol[G], qn[G][NE], qp[G][NP], hres[G][NE];
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       const int sq = clamp_sp(sp0 + g, NT);
@@ -2031,16 +2042,20 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const f
 #pragma unroll
       for (int s = 0; s < NE; ++s) {
         qn[g][s] = AT(qold, po + nb[s], k);
-        upw[g][s] = AT(I.b.fct_upw, sq * I.n.E + e[s], k);
         hres[g][s] = AT(I.b.fct_hres, sq * I.n.E + e[s], k);
       }
+      if (!PAIR)
+#pragma unroll
+        for (int s = 0; s < NP; ++s) qp[g][s] = AT(qold, po + oc[s], k);
       qol[g] = AT(qold, po + c, k);
     }
-    keep(qsk, qol, qn, upw, hres);
-    SlotRow<NE> sgs = sgr;
-    settle(sgs);
-    float sg[NE];
+    keep(qsk, qol, qn, hres, uh);
+    if (!PAIR) keep(qp);
+    SlotRow<NE> sgs = sgr, sds = sdr;
+    settle(sgs, sds);
+    float sg[NE], sd[NE];
     slot_vals(sgs, sg);
+    slot_vals(sds, sd);
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       if (sp0 + g >= NT) break;
@@ -2090,7 +2105,13 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const f
       for (int s = 0; s < NE; ++s)
         if (s < n) {
           const float so = -sg[s];
-          mass = mass + (so * upw[g][s]) * ia;
+          // edge s's cells in edge order: this one first where csign = +1
+          const float qx = (PAIR ? qn[g][s] : qp[g][s < NP ? s : 0]) + qadd;
+          const bool first = sg[s] > 0.0f;
+          const float qa1 = first ? qo : qx, qa2 = first ? qx : qo;
+          const float dvdt = __builtin_fabsf(sd[s]) * dt;  // |csign * dv| = dv
+          const float upa = dvdt * (max0(uh[s]) * qa1 + min0(uh[s]) * qa2);
+          mass = mass + (so * upa) * ia;
           const float sr = (-so) * hres[g][s];
           sout = sout - max0(sr) * ia;
           sin_ = sin_ - min0(sr) * ia;
@@ -2450,7 +2471,12 @@ int scalars_stage(const Instance &I, int stage, hipStream_t s) {
   }
   LAUNCH_NE(I, k_fct_target, I.n.C, 1, s, I, dt);
   LAUNCH_NE(I, k_fct_edge1, I.n.E, 1, s, I, qin, dt, qold, qadd);
-  LAUNCH_NE(I, k_fct_cell1, I.n.C, 1, s, I, qin, dt, qold, qadd);
+  if (I.n.OCP)
+    NEDISPATCH(I, launch_named("k_fct_cell1", k_fct_cell1<6, true>, I.n.C, 1, s, I, qin, dt, qold, qadd),
+               launch_named("k_fct_cell1", k_fct_cell1<7, true>, I.n.C, 1, s, I, qin, dt, qold, qadd));
+  else
+    NEDISPATCH(I, launch_named("k_fct_cell1", k_fct_cell1<6, false>, I.n.C, 1, s, I, qin, dt, qold, qadd),
+               launch_named("k_fct_cell1", k_fct_cell1<7, false>, I.n.C, 1, s, I, qin, dt, qold, qadd));
   launch(k_fct_edge2, I.n.E, 1, s, I);
   LAUNCH_NE(I, k_fct_cell2, I.n.C, 1, s, I, I.b.q_next);
   return check();

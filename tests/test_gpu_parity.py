@@ -647,3 +647,32 @@ def test_passive_tracers_clamped_without_moisture_bitwise(get_case):
     for k in ref:
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
     assert (got["scalars"] >= 0).all()
+
+
+def test_cells_on_cell_out_of_edge_order_bitwise(get_case):
+    """cellsOnCell rotated one slot against edgesOnCell in every cell (a
+    table order MPAS files do not use, so slot s no longer names the cell
+    across edge s): the FCT cell pass then reads each edge's other cell on
+    its own, and the limiter bounds keep the given order; 2 moist steps bit
+    for bit."""
+    import copy
+    import dataclasses
+
+    print("[gpu] x1.642-moist cellsOnCell rotated one slot, 2 steps", flush=True)
+    base = get_case("x1.642-moist")
+    assert base.config.config_monotonic
+    mesh = copy.deepcopy(base.mesh)
+    coc = mesh.arrays["cellsOnCell"].copy()
+    n = mesh.arrays["nEdgesOnCell"]
+    for c in range(coc.shape[0]):
+        coc[c, : n[c]] = np.roll(coc[c, : n[c]], 1)
+    mesh.arrays["cellsOnCell"] = coc
+    case = dataclasses.replace(base, mesh=mesh)
+    o = Oracle(case, 32)
+    o.set_state(initial_state_arrays(case), rebuild=True)
+    assert o.step(2) == 0
+    d = _driver(case)
+    d.step_device(2)
+    ref, got = o.get_state(), d.get_state()
+    for k in ref:
+        assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
