@@ -150,8 +150,9 @@ struct Consts {
   int mixing, mix4;              // Smagorinsky mixing on; fourth-order part on (h4 > 0)
   int physics;                   // held physics tendencies are added
   int physics_q;                 // a held scalar tendency enters the transport
-  int store_flux;                // recovery also stores the stage's flux_u/flux_w (read only by
-                                 // hexdyn_export_field; stage 3 folds them into the split sums)
+  int store_flux;                // the phase API: recovery also stores what only exports read between
+                                 // phases (the stage's flux_u/flux_w, which stage 3 folds into the
+                                 // split sums, and rho_p/rtheta_m at stages 1-2)
   int fused;                     // whole-step launch sequence: the next stage's theta/w edge fluxes
                                  // ride on the edge diagnostics, the first acoustic edge update on
                                  // the momentum tendency, and ru_avg is kept only where read back

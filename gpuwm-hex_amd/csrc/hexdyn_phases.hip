@@ -1648,10 +1648,14 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps) {
     split = sub == I.k.splits ? acc * I.k.split_inv : acc;
   }
   if (lay) {
-    AT(I.b.rho_p, c, k) = dp;
+    // rho_p and rtheta_m are read only as the next subcycle's start values
+    // (stage 3) or by an export between phases
+    if (S3 || I.k.store_flux) {
+      AT(I.b.rho_p, c, k) = dp;
+      AT(I.b.c_rt, c, k) = rtv;
+    }
     AT(I.b.rtheta_p, c, k) = rp;
     AT(I.b.c_rho, c, k) = rho;
-    AT(I.b.c_rt, c, k) = rtv;
     AT(I.b.theta, c, k) = th;
     if (S3) {
       AT(I.b.exner, c, k) = ex;
