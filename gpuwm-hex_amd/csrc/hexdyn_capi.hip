@@ -102,20 +102,22 @@ std::vector<int> order_by_keys(const std::vector<uint64_t> &keys) {
   return idx;  // idx[new] = old
 }
 
+// logical [level][entity] <-> rows of rs floats in curve order (levels
+// nlev .. rs-1 of a row are zero)
 __global__ void k_import(float *__restrict__ dst, const float *__restrict__ src, const int *__restrict__ old_of_new,
-                         int n, int nlev) {
+                         int n, int nlev, int rs) {
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int k = threadIdx.x & 63;
-  if (e >= n) return;
-  dst[(size_t)e * HX_ROW + k] = k < nlev ? src[(size_t)k * n + old_of_new[e]] : 0.0f;
+  if (e >= n || k >= rs) return;
+  dst[(size_t)e * rs + k] = k < nlev ? src[(size_t)k * n + old_of_new[e]] : 0.0f;
 }
 
 __global__ void k_export(float *__restrict__ dst, const float *__restrict__ src, const int *__restrict__ old_of_new,
-                         int n, int nlev) {
+                         int n, int nlev, int rs) {
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int k = threadIdx.x & 63;
   if (e >= n || k >= nlev) return;
-  dst[(size_t)k * n + old_of_new[e]] = src[(size_t)e * HX_ROW + k];
+  dst[(size_t)k * n + old_of_new[e]] = src[(size_t)e * rs + k];
 }
 
 struct ProfEntry {
@@ -435,6 +437,7 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
   };
   if (n.C <= 0 || n.E <= 0 || n.V <= 0) return refuse("empty mesh");
   if (n.L < 4 || n.L + 1 > HX_ROW) return refuse("n_levels must satisfy 4 <= L and L + 1 <= 64");
+  n.RS = std::min(HX_ROW, (n.L + 1 + 7) / 8 * 8);
   if (n.ME < 6 || n.ME > HEXDYN_MAX_EDGES) return refuse("max_edges must be 6 or 7");
   // edge stencils are stored at the fixed width 2*maxEdges-2 (kernels unroll over it)
   if (m->max_edges2 > 2 * n.ME - 2 || m->max_adv > 2 * n.ME - 2)
@@ -743,16 +746,17 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
     for (int i = 0; i < count; ++i) v[i] = src[i];
     return v;
   };
-  auto rows_cells = [&](const float *src, int nlev) {  // [nlev][C] old -> [C][64] new
+  const int RS = n.RS;
+  auto rows_cells = [&](const float *src, int nlev) {  // [nlev][C] old -> [C][RS] new
     std::vector<float> v((size_t)C * HX_ROW, 0.0f);
     for (int nc = 0; nc < C; ++nc)
-      for (int k = 0; k < nlev; ++k) v[(size_t)nc * HX_ROW + k] = src[(size_t)k * C + h->old_c_h[nc]];
+      for (int k = 0; k < nlev; ++k) v[(size_t)nc * RS + k] = src[(size_t)k * C + h->old_c_h[nc]];
     return v;
   };
   auto rows_edges = [&](const float *src, int nlev) {
     std::vector<float> v((size_t)E * HX_ROW, 0.0f);
     for (int ne = 0; ne < E; ++ne)
-      for (int k = 0; k < nlev; ++k) v[(size_t)ne * HX_ROW + k] = src[(size_t)k * E + h->old_e_h[ne]];
+      for (int k = 0; k < nlev; ++k) v[(size_t)ne * RS + k] = src[(size_t)k * E + h->old_e_h[ne]];
     return v;
   };
   rc |= h->upload(&Z.rdzw, pad(vt->rdzw, L)); rc |= h->upload(&Z.rdzu, pad(vt->rdzu, L));
@@ -770,7 +774,7 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
     for (int nc = 0; nc < C; ++nc)
       for (int k = 0; k < L; ++k) {
         const int cc = h->old_c_h[nc];
-        zm[(size_t)nc * HX_ROW + k] =
+        zm[(size_t)nc * RS + k] =
             (float)(0.5 * ((double)vt->zgrid[(size_t)k * C + cc] + (double)vt->zgrid[(size_t)(k + 1) * C + cc]));
       }
     rc |= h->upload(&Z.zmid, zm);
@@ -782,7 +786,7 @@ static int create_impl(const hexdyn_mesh *m, const hexdyn_vertical *vt, const he
       for (int s = 0; s < ME; ++s)
         for (int k = 0; k <= L; ++k) {
           const size_t src = ((size_t)k * C + cc) * ME + s;
-          const size_t dst = ((size_t)nc * ME + s) * HX_ROW + k;
+          const size_t dst = ((size_t)nc * ME + s) * RS + k;
           zb[dst] = vt->zb_cell[src];
           zb3[dst] = vt->zb3_cell[src];
         }
@@ -951,13 +955,15 @@ FieldRef field_ref(hexdyn_handle *h, int id) {
 
 int transfer(hexdyn_handle *h, const FieldRef &f, float *dev, bool to_internal, hipStream_t s) {
   for (int p = 0; p < f.planes; ++p) {
-    float *in_rows = f.ptr + (size_t)p * f.n * HX_ROW;
+    float *in_rows = f.ptr + (size_t)p * f.n * h->I.n.RS;
     float *logical = dev + (size_t)p * f.nlev * f.n;
     const int nb = (f.n + 3) / 4;
     if (to_internal)
-      hipLaunchKernelGGL(k_import, dim3(nb), dim3(256), 0, s, in_rows, (const float *)logical, f.old, f.n, f.nlev);
+      hipLaunchKernelGGL(k_import, dim3(nb), dim3(256), 0, s, in_rows, (const float *)logical, f.old, f.n, f.nlev,
+                         h->I.n.RS);
     else
-      hipLaunchKernelGGL(k_export, dim3(nb), dim3(256), 0, s, logical, (const float *)in_rows, f.old, f.n, f.nlev);
+      hipLaunchKernelGGL(k_export, dim3(nb), dim3(256), 0, s, logical, (const float *)in_rows, f.old, f.n, f.nlev,
+                         h->I.n.RS);
   }
   return hipGetLastError() == hipSuccess ? 0 : HEXDYN_E_HIP;
 }

@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(256) k_integrals(const Instance I, const Integ
   const int k = (int)(threadIdx.x & 63);
   const bool live = c0 < I.n.C && k < I.n.L;
   const int c = c0 < I.n.C ? c0 : 0;
-  const size_t row = (size_t)c * HX_ROW + k;
+  const int rs = I.n.RS;
+  const size_t row = (size_t)c * rs + (k < rs ? k : rs - 1);  // rows rs floats apart
   double wgt = 0.0;
   double rho = 0.0;
   if (live) {
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(256) k_integrals(const Instance I, const Integ
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     const int e = I.m.eoc[c * NE + s];
-    const double ue = (double)a.u[(size_t)e * HX_ROW + k];
+    const double ue = (double)a.u[(size_t)e * rs + (k < rs ? k : rs - 1)];
     if (s < n) ke += 0.25 * (double)I.m.dcdv[e] * ue * ue;
   }
   ke *= (double)I.m.invarea[c];
@@ -79,7 +80,7 @@ __global__ void __launch_bounds__(256) k_integrals(const Instance I, const Integ
     if (k == 0) s_part[w * NQ + i] = t;
   }
   for (int sp = 0; sp < a.NT; ++sp) {
-    const double x = live ? wgt * rho * (double)a.q[(size_t)sp * I.n.C * HX_ROW + row] : 0.0;
+    const double x = live ? wgt * rho * (double)a.q[(size_t)sp * I.n.C * rs + row] : 0.0;
     const double t = wave_sum(x);
     if (k == 0) s_part[w * NQ + 3 + sp] = t;
   }

@@ -16,12 +16,15 @@
 #include <cstring>
 #include <string>
 
+// lanes of a level row (one wavefront); rows are stored Sizes::RS <= HX_ROW
+// floats apart and allocated HX_ROW floats each
 #define HX_ROW 64
 
 namespace hexdyn {
 
 struct Sizes {
   int C, E, V, L, ME, ME2, MA, NT, NM;
+  int RS;  // floats from one level row to the next: L + 1 rounded up to 8 (at most HX_ROW)
   int TU;  // distinct advection cells per edge tile (0: the tiled flux gathers are off)
   int OCP;  // 1: in every cell, cellsOnCell slot s is the other cell of edgesOnCell slot s
   // wavefronts (rows) per block of the launch in flight: each launch passes

@@ -244,7 +244,10 @@ __global__ void __launch_bounds__(256) k_reduce(const float *__restrict__ partia
 // Lane k of wavefront (blockIdx*4 + w) owns row k of one cell or edge.
 __device__ __forceinline__ int ent4() { return (int)(blockIdx.x * 4 + (threadIdx.x >> 6)); }
 __device__ __forceinline__ int lane() { return (int)(threadIdx.x & 63); }
-__device__ __forceinline__ float rowv(const float *p, int e, int k) { return p[(size_t)e * HX_ROW + k]; }
+// element k of row e of a dycore array (rows rs floats apart; lanes k >= rs
+// read the row's last element, as the dycore kernels do)
+__device__ __forceinline__ size_t rowi(int e, int k, int rs) { return (size_t)e * rs + (k < rs ? k : rs - 1); }
+__device__ __forceinline__ float rowv(const float *p, int e, int k, int rs) { return p[rowi(e, k, rs)]; }
 
 // Least-squares cell wind from the normal velocities of the cell's edges:
 // minimise sum_s (n_s . U - u_s)^2, n_s = (cos a_s, sin a_s).
@@ -285,13 +288,13 @@ __device__ __forceinline__ void raw_features(const Instance &I, int c, int k, fl
   float u[NE], ca[NE], sa[NE], fe[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    u[s] = rowv(B.p_u, eo[s], k);
+    u[s] = rowv(B.p_u, eo[s], k, I.n.RS);
     ca[s] = I.m.cosang[eo[s]];
     sa[s] = I.m.sinang[eo[s]];
     fe[s] = I.m.fedge[eo[s]];
   }
-  const float wk = rowv(B.p_w, c, k), thk = rowv(B.p_theta, c, k), rk = rowv(B.rho, c, k);
-  const float qk = I.n.NT > 0 ? rowv(B.q, c, k) : 0.0f;
+  const float wk = rowv(B.p_w, c, k, I.n.RS), thk = rowv(B.p_theta, c, k, I.n.RS), rk = rowv(B.rho, c, k, I.n.RS);
+  const float qk = I.n.NT > 0 ? rowv(B.q, c, k, I.n.RS) : 0.0f;
   float m11 = 0.f, m12 = 0.f, m22 = 0.f;
 #pragma unroll
   for (int s = 0; s < NE; ++s)
@@ -409,7 +412,7 @@ __global__ void __launch_bounds__(256) k_f32_to_bf16(const float *__restrict__ x
 
 // Published fields of the coarse and fine handles that the target reads.
 struct TargetArgs {
-  int C, E, L;
+  int C, E, L, RS;
   const float *th_c, *rho_c, *w_c, *u_c;
   const float *th_f, *rho_f, *w_f, *u_f;
 };
@@ -421,9 +424,9 @@ __global__ void __launch_bounds__(256) k_target_cells(const TargetArgs a, const 
   if (c >= a.C) return;
   const int k = lane();
   const int L = a.L;
-  const float th = k < L ? (rowv(a.th_f, c, k) - rowv(a.th_c, c, k)) / oscale[O_TH * 64 + k] : 0.f;
-  const float rh = k < L ? (rowv(a.rho_f, c, k) - rowv(a.rho_c, c, k)) / oscale[O_RHO * 64 + k] : 0.f;
-  const float ww = (k >= 1 && k < L) ? (rowv(a.w_f, c, k) - rowv(a.w_c, c, k)) / oscale[O_W * 64 + k] : 0.f;
+  const float th = k < L ? (rowv(a.th_f, c, k, a.RS) - rowv(a.th_c, c, k, a.RS)) / oscale[O_TH * 64 + k] : 0.f;
+  const float rh = k < L ? (rowv(a.rho_f, c, k, a.RS) - rowv(a.rho_c, c, k, a.RS)) / oscale[O_RHO * 64 + k] : 0.f;
+  const float ww = (k >= 1 && k < L) ? (rowv(a.w_f, c, k, a.RS) - rowv(a.w_c, c, k, a.RS)) / oscale[O_W * 64 + k] : 0.f;
   float *t = Tc + (size_t)c * 3 * HX_ROW;
   t[k] = th;
   t[HX_ROW + k] = rh;
@@ -435,7 +438,7 @@ __global__ void __launch_bounds__(256) k_target_edges(const TargetArgs a, const 
   const int e = ent4();
   if (e >= a.E) return;
   const int k = lane();
-  Te[(size_t)e * HX_ROW + k] = k < a.L ? (rowv(a.u_f, e, k) - rowv(a.u_c, e, k)) / oscale[O_UE * 64 + k] : 0.f;
+  Te[(size_t)e * HX_ROW + k] = k < a.L ? (rowv(a.u_f, e, k, a.RS) - rowv(a.u_c, e, k, a.RS)) / oscale[O_UE * 64 + k] : 0.f;
 }
 
 __device__ __forceinline__ float wave_sum_f(float x) {
@@ -599,7 +602,7 @@ __global__ void __launch_bounds__(256) k_apply_cells(const Instance I, const flo
   const int L = I.n.L;
   const Work &B = I.b;
   const float *y = Y + (size_t)c * NOUT;
-  const size_t o = (size_t)c * HX_ROW + k;
+  const size_t o = rowi(c, k, I.n.RS);
   float rho = B.rho[o];
   if (k < L) {
     const float drho = gain * (y[O_RHO * 64 + k] * oscale[O_RHO * 64 + k] - shift[0]);
@@ -628,9 +631,9 @@ __global__ void __launch_bounds__(256) k_apply_edges(const Instance I, const flo
   const int k = lane();
   if (k >= I.n.L) return;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  const size_t o = (size_t)e * HX_ROW + k;
+  const size_t o = rowi(e, k, I.n.RS);
   const float u = I.b.p_u[o] + gain * (edge_pred(I, Y, e, k) * oscale[O_UE * 64 + k]);
-  const float r0 = rowv(I.b.rho, c0, k), r1 = rowv(I.b.rho, c1, k);
+  const float r0 = rowv(I.b.rho, c0, k, I.n.RS), r1 = rowv(I.b.rho, c1, k, I.n.RS);
   const float ru = (0.5f * (r0 + r1)) * u;
   I.b.ru[o] = ru;
   I.b.p_u[o] = (2.0f * ru) / (r0 + r1);
@@ -659,14 +662,14 @@ __global__ void __launch_bounds__(256) k_checks_cells(const Instance I, float dt
     for (int s = 0; s < NE; ++s) {
       if (s < n) {
         const int e = I.m.eoc[c * NE + s];
-        const float u = rowv(B.p_u, e, k);
+        const float u = rowv(B.p_u, e, k, I.n.RS);
         div += I.m.csdv[c * NE + s] * u;
         cfl = nanmax(cfl, fabsf(u) * dt * I.m.invdc[e]);
       }
     }
     div = fabsf(div * I.m.invarea[c]) * dt;
-    if (k >= 1 && k < L) vc = fabsf(rowv(B.p_w, c, k)) * dt * I.z.rdzw[k] / rowv(I.z.zz, c, k);
-    if (k < L) mass = (1.0 / (double)I.m.invarea[c]) * (1.0 / (double)I.z.rdzw[k]) * (double)rowv(B.rho, c, k);
+    if (k >= 1 && k < L) vc = fabsf(rowv(B.p_w, c, k, I.n.RS)) * dt * I.z.rdzw[k] / rowv(I.z.zz, c, k, I.n.RS);
+    if (k < L) mass = (1.0 / (double)I.m.invarea[c]) * (1.0 / (double)I.z.rdzw[k]) * (double)rowv(B.rho, c, k, I.n.RS);
     if (k >= L) {
       div = 0.f;
       cfl = 0.f;
@@ -989,7 +992,7 @@ int adam(Net &n, float lr, float b1, float b2, float eps, int t, hipStream_t s) 
 }
 
 int target(Net &n, const Instance &Ic, const Instance &If, hipStream_t s) {
-  const TargetArgs a{n.C, n.E, Ic.n.L, Ic.b.p_theta, Ic.b.rho, Ic.b.p_w, Ic.b.p_u,
+  const TargetArgs a{n.C, n.E, Ic.n.L, Ic.n.RS, Ic.b.p_theta, Ic.b.rho, Ic.b.p_w, Ic.b.p_u,
                      If.b.p_theta, If.b.rho, If.b.p_w, If.b.p_u};
   hipLaunchKernelGGL(k_target_cells, dim3(nblk(n.C, 4)), dim3(256), 0, s, a, (const float *)n.out_scale, n.Tc);
   hipLaunchKernelGGL(k_target_edges, dim3(nblk(n.E, 4)), dim3(256), 0, s, a, (const float *)n.out_scale, n.Te);

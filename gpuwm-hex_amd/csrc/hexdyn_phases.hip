@@ -40,14 +40,21 @@ namespace hexdyn {
 namespace HEXDYN_NS {
 namespace {
 
-#define ROW(p, ent) ((p) + (size_t)(ent) * HX_ROW)
 // Element k of row ent of p, addressed as a 32-bit byte offset from the
 // wave-uniform base (every array is below 4 GiB), so loads and stores take
 // the SGPR-base + VGPR-offset form instead of a per-lane 64-bit address.
-__device__ __forceinline__ uint32_t row_off(int ent, int k) {
-  return (uint32_t)ent * (uint32_t)(HX_ROW * 4) + (uint32_t)k * 4u;
+// Rows are rs floats apart (the level count plus one, rounded up to 8:
+// 224-byte rows at 55 levels, so a streamed array moves 12.5 % fewer bytes
+// than with 64-float rows, while a gathered row still spans four 64-byte
+// sectors).  Lanes k >= rs read the row's last element instead of the next
+// row: for layer arrays that element is padding (zero), so every lane sees
+// what a 64-float row would hold; for interface arrays it is the top value,
+// which no lane at or below the top reads through a shuffle.  Stores are
+// made for k <= L < rs only.
+__device__ __forceinline__ uint32_t row_off(int rs, int ent, int k) {
+  return (uint32_t)ent * (uint32_t)(rs * 4) + (uint32_t)(k < rs ? k : rs - 1) * 4u;
 }
-#define AT(p, ent, k) (*(float *)((char *)(p) + row_off((ent), (k))))
+#define AT(p, ent, k) (*(float *)((char *)(p) + row_off(I.n.RS, (ent), (k))))
 // gather kernels take up to 16 rows (wavefronts) per block: a per-kernel launch setting
 #define KB __global__ void __launch_bounds__(1024)
 
@@ -575,7 +582,6 @@ KB k_moist_cell(const Instance I) {
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
-  const size_t plane = (size_t)I.n.C * HX_ROW;
   float t = 0.0f;
   for (int s = 0; s < I.n.NM; ++s) t = t + AT(I.b.q, s * I.n.C + c, k);
   const float tm = up1(t);
@@ -590,7 +596,6 @@ KB k_moist_edge(const Instance I) {
   const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
-  const size_t plane = (size_t)I.n.C * HX_ROW;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   float t = 0.0f;
   for (int s = 0; s < I.n.NM; ++s)
@@ -2222,6 +2227,7 @@ KB k_clamp(const Instance I, const float *src, float *dst, int rows) {
   const int r = wave_entity(I);
   if (r >= rows) return;
   const int k = lane();
+  if (k >= I.n.RS) return;
   const float x = AT(src, r, k);
   AT(dst, r, k) = x < 0.0f ? 0.0f : x;
 }

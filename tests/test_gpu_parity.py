@@ -130,10 +130,11 @@ def test_ragged_mesh_three_steps_bitwise(get_case):
         assert _bits_equal(ref[k], got[k]), f"{k}: {_diff(ref[k], got[k])}"
 
 
-@pytest.mark.parametrize("nlev", [10, 63])
+@pytest.mark.parametrize("nlev", [10, 50, 63])
 def test_level_counts_two_steps_bitwise(get_case, nlev):
-    """A short column and the deepest one a 64-lane row holds (L + 1 = 64,
-    the interface row on the last lane), with mixing: 2 steps bit for bit."""
+    """A short column, one whose rows (56 floats apart) end in padding
+    lanes, and the deepest one a 64-lane row holds (L + 1 = 64, the
+    interface row on the last lane), with mixing: 2 steps bit for bit."""
     print(f"[gpu] x1.642-moist at {nlev} levels, mixing on, 2 steps", flush=True)
     import dataclasses
 
