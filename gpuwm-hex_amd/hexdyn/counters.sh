@@ -14,7 +14,9 @@ WL=${1:-x1.40962-moist}
 OUT=${2:-$ROOT/gpurun_out/pmc_$WL}
 mkdir -p "$OUT"
 BENCH="python3 $ROOT/bench.py --workload $WL --steps 2 --warmup 0 --profile-steps 1 --no-cpu-baseline --form plain"
-KERNELS="k_diag_edge_flux|k_tend_mom_ac|k_rec_cell|k_ac_cell|k_tend_cell|k_rec_edge|k_sc_edge|k_diag_cell"
+# the edge-flux pass under both names: its tiled form runs unless a launch
+# table selects the gather form
+KERNELS="k_diag_edge_flux_t|k_diag_edge_flux|k_tend_mom_ac|k_rec_cell|k_ac_cell|k_tend_cell|k_rec_edge|k_sc_edge|k_diag_cell"
 export HEXDYN_CACHE=$OUT/cache
 echo "# tune" && timeout -k 10 300 $BENCH > "$OUT/tune.log" 2>&1 || exit $?
 cd /tmp && export TMPDIR=/tmp
