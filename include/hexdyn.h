@@ -26,7 +26,8 @@
  * Static mesh/vertical/reference data are host pointers copied at create
  * time.  State views are device pointers.  Internally the library renumbers
  * entities along a space-filling curve and stores every field entity-major
- * with the level index fastest, each entity row padded to 64 floats.
+ * with the level index fastest, entity rows L+1 rounded up to 8 floats
+ * apart (56 at 55 levels, at most 64).
  *
  * Return codes: 0 ok, >0 refusal or numerical flag, <0 HIP error.  A set
  * validation flag makes hexdyn_step/hexdyn_end_step return
