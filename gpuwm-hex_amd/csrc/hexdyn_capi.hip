@@ -1347,9 +1347,8 @@ void drop_graphs(hexdyn_handle *h) {
   for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second.first);
   h->graphs.clear();
 }
-// the column kernels' block shape is fixed (16 columns through LDS)
-// kernels with a fixed block shape: the column kernels (16 columns per block
-// through LDS) and the edge-tiled flux pass (HX_TILE_E edges per block)
+// kernels with a fixed block shape: the column kernels (16 or 8 columns per
+// block through LDS) and the edge-tiled flux pass (HX_TILE_E edges per block)
 bool column_kernel(const char *name) {
   return std::strcmp(name, "k_vcoef") == 0 || std::strcmp(name, "k_ac_cell") == 0 ||
          std::strcmp(name, "k_diag_edge_flux_t") == 0;

@@ -197,15 +197,20 @@ __device__ __forceinline__ float uni_load(const float *__restrict__ p) {
 
 inline int nblocks(int n, int wpb) { return (n + wpb - 1) / wpb; }
 
-// Column kernels: COLW cells (one wavefront each) per block.  The serial
+// Column kernels: W cells (one wavefront each) per block.  The serial
 // vertical recurrences of the block's cells run together on the lanes of
 // one wavefront (lane = cell) through LDS rows padded to COLP floats, so
-// 16 columns cost one column's dependent chain instead of sixteen.
-constexpr int COLW = 16;
+// W columns cost one column's dependent chain instead of W.  The implicit
+// coefficients take 16 columns per block; the acoustic column update, whose
+// two sweeps hold the block for longer, takes 8 (four blocks per CU
+// overlap their loads with the others' sweeps: 57 -> 53 us; the
+// coefficients were 1 us slower at 8).
+constexpr int COLW_VC = 16;  // k_vcoef
+constexpr int COLW_AC = 8;   // k_ac_cell (at least 7: its waves 0-6 stage the level rows)
 constexpr int COLP = 65;
-// Two column blocks per CU need every SIMD to hold 8 wavefronts: the
-// register budget is capped for that (at 7 a CU would hold one block).
-#define KBCOL __global__ void __launch_bounds__(64 * COLW) __attribute__((amdgpu_waves_per_eu(8, 8)))
+// The column blocks of a CU need every SIMD to hold 8 wavefronts: the
+// register budget is capped for that (at 7 a CU would hold one block of 16).
+#define KBCOL(W) __global__ void __launch_bounds__(64 * (W)) __attribute__((amdgpu_waves_per_eu(8, 8)))
 
 // Per-level coefficient rows are the same for every column of a block: wave
 // j < N of a column block copies row j into LDS once, instead of each of the
@@ -604,8 +609,9 @@ KB k_moist_edge(const Instance I) {
 }
 
 // ------------------------------------------------------ implicit coefficients
-KBCOL k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const float *__restrict__ theta,
+KBCOL(COLW_VC) k_vcoef(const Instance I, float dts, const float *__restrict__ exn, const float *__restrict__ theta,
               const float *__restrict__ rtp) {
+  constexpr int COLW = COLW_VC;
   __shared__ float s_a[COLW * COLP], s_b[COLW * COLP], s_c[COLW * COLP];
   __shared__ float s_lev[6 * 64];
   const int w = (int)(threadIdx.x >> 6);
@@ -1384,7 +1390,9 @@ KB k_ac_edge(const Instance I, float dts, float damp_coef) {
 // FIRST: the first substep of a stage (the perturbation fields start at
 // zero).  Every global load is issued before the column sweep.
 template <int NE, bool FIRST>
-KBCOL k_ac_cell(const Instance I, float dts) {
+KBCOL(COLW_AC) k_ac_cell(const Instance I, float dts) {
+  constexpr int COLW = COLW_AC;
+  static_assert(COLW >= 7, "waves 0-6 stage the level rows");
   __shared__ float s_x[COLW * COLP], s_d[COLW * COLP], s_l[COLW * COLP], s_g[COLW * COLP];
   __shared__ float s_lev[7 * 64];
   const int w = (int)(threadIdx.x >> 6);
@@ -2259,15 +2267,14 @@ inline void launch_named(const char *name, K kernel, int n, int gy, hipStream_t 
 }
 #define launch(kernel, ...) launch_named(#kernel, kernel, __VA_ARGS__)
 
-// column kernels: COLW cells per block of 64*COLW threads
-template <class K, class... A>
+// column kernels: W cells per block of 64*W threads
+template <int W, class K, class... A>
 inline void launch_col_named(const char *name, K kernel, int n, hipStream_t s, A... args) {
   if (n <= 0) return;
   if (g_prof_on) prof_mark(name, s, 1);
-  hipLaunchKernelGGL(kernel, dim3((n + COLW - 1) / COLW), dim3(64 * COLW), 0, s, args...);
+  hipLaunchKernelGGL(kernel, dim3((n + W - 1) / W), dim3(64 * W), 0, s, args...);
   if (g_prof_on) prof_mark(name, s, 0);
 }
-#define launch_col(kernel, ...) launch_col_named(#kernel, kernel, __VA_ARGS__)
 
 inline int check() {
   const hipError_t e = hipGetLastError();
@@ -2353,7 +2360,7 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
                  launch_named("k_sub_cell", k_sub_cell<7, false, false>, I.n.C, 1, s, I));
     launch_named("k_sub_edge", k_sub_edge<false>, I.n.E, 1, s, I);
   }
-  launch_col(k_vcoef, I.n.C, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
+  launch_col_named<COLW_VC>("k_vcoef", k_vcoef, I.n.C, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
          (const float *)I.b.s_rtheta_p);
   if (I.k.mixing) {
     if (I.k.mix4) {
@@ -2374,7 +2381,7 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
 
 int dyn_tend(const Instance &I, int stage, hipStream_t s) {
   if (stage == 2 && I.k.order == 3)
-    launch_col(k_vcoef, I.n.C, s, I, I.k.dts[1], (const float *)I.b.s_exner, (const float *)I.b.theta,
+    launch_col_named<COLW_VC>("k_vcoef", k_vcoef, I.n.C, s, I, I.k.dts[1], (const float *)I.b.s_exner, (const float *)I.b.theta,
            (const float *)I.b.rtheta_p);
   if (I.k.fused) {
     // the edge fluxes came with the preceding diagnostics
@@ -2420,11 +2427,11 @@ int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStr
       launch_named("k_ac_edge", k_ac_edge<false, false, false>, I.n.E, 1, s, I, dts, coef);
   }
   if (ss == 1)
-    NEDISPATCH(I, launch_col_named("k_ac_cell", k_ac_cell<6, true>, I.n.C, s, I, dts),
-               launch_col_named("k_ac_cell", k_ac_cell<7, true>, I.n.C, s, I, dts));
+    NEDISPATCH(I, launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<6, true>, I.n.C, s, I, dts),
+               launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<7, true>, I.n.C, s, I, dts));
   else
-    NEDISPATCH(I, launch_col_named("k_ac_cell", k_ac_cell<6, false>, I.n.C, s, I, dts),
-               launch_col_named("k_ac_cell", k_ac_cell<7, false>, I.n.C, s, I, dts));
+    NEDISPATCH(I, launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<6, false>, I.n.C, s, I, dts),
+               launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<7, false>, I.n.C, s, I, dts));
   return check();
 }
 

@@ -243,7 +243,7 @@ int hexdyn_graph_count(const hexdyn_handle *h);
 /* Launch shape of the gather kernels (one wavefront per cell, edge or
  * vertex row): wavefronts per block, 1, 2, 4 (the default), 8 or 16, by kernel name
  * ("k_diag_edge_flux", ...; "*" resets every kernel to the given value).
- * The column kernels (k_vcoef, k_ac_cell: 16 columns per block through LDS)
+ * The column kernels (k_vcoef, k_ac_cell: 16 and 8 columns per block through LDS)
  * and the edge-tiled flux pass (k_diag_edge_flux_t: 8 edges per block
  * through LDS) are refused; setting k_diag_edge_flux selects that pass's
  * gather form (one block of whole-edge wavefronts, no LDS) instead of the
