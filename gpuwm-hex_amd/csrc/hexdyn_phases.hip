@@ -1109,16 +1109,19 @@ KB k_tend_mom(const Instance I) {
 // (ru_p = dts * tend_ru, acoustic_v841.py:296-330) in one pass.  The running
 // sum ru_avg equals ru_p after one substep; it is stored only when a later
 // substep reads it back (with two substeps the second forms p1 + p2 from
-// ru_p, with one the recovery reads ru_p itself).
+// ru_p, with one the recovery reads ru_p itself).  ru_p itself is stored
+// only for a second substep's edge update or, at stage 3, for the step's
+// output: the first substep's column update and a one-substep stage's
+// recovery form dts * tend_ru from tend_ru.
 template <int NE>
-KB k_tend_mom_ac(const Instance I, float dts, int store_avg) {
+KB k_tend_mom_ac(const Instance I, float dts, int store_avg, int store_rup) {
   const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   const float tr = tend_mom_edge<NE>(I, e, k);
   if (k < I.n.L) {
     const float p = dts * tr;
-    AT(I.b.ru_p, e, k) = p;
+    if (store_rup) AT(I.b.ru_p, e, k) = p;
     if (store_avg) AT(I.b.ru_avg, e, k) = p;
   }
 }
@@ -1437,7 +1440,7 @@ KBCOL(COLW_AC) k_ac_cell(const Instance I, float dts) {
   float rup[NE], sto[NE];
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
-    rup[s] = AT(I.b.ru_p, e[s], k);
+    rup[s] = FIRST ? AT(I.b.tend_ru, e[s], k) : AT(I.b.ru_p, e[s], k);
     sto[s] = AT(I.b.s_theta, o[s], k);
   }
   keep(rup, sto);
@@ -1454,7 +1457,9 @@ KBCOL(COLW_AC) k_ac_cell(const Instance I, float dts) {
 #pragma unroll
   for (int s = 0; s < NE; ++s)
     if (s < n) {
-      const float flux = ((dts * sd[s]) * rup[s]) * ia;  // (csign*dts)*dv, csign = +-1
+      // the first substep's ru_p is dts * tend_ru (the momentum pass's product)
+      const float rp = FIRST ? dts * rup[s] : rup[s];
+      const float flux = ((dts * sd[s]) * rp) * ia;  // (csign*dts)*dv, csign = +-1
       // theta at the edge: (theta[cell1] + theta[cell0]) in that order
       const bool first_cell = sg[s] > 0.0f;
       const float th0 = first_cell ? sthc : sto[s];
@@ -1690,18 +1695,21 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps) {
 // ru_p of that substep.  S3: accumulate the split-transport sums.  Loads
 // first, stores last.
 template <bool S3, bool DAMP, bool AVGRP>
-KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef) {
+KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef, float dts) {
   const int e = wave_entity(I);
   if (e >= I.n.E) return;
   const int k = lane();
   if (k >= I.n.L) return;
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
   const float sru = AT(I.b.s_ru, e, k);
-  const float rp_raw = AT(I.b.ru_p, e, k);
-  const float avg = AVGRP ? rp_raw : AT(I.b.ru_avg, e, k);
+  // one substep: its ru_p is dts * tend_ru, formed here as the momentum pass formed it
+  const float rp_ld = AVGRP ? AT(I.b.tend_ru, e, k) : AT(I.b.ru_p, e, k);
+  const float avg_ld = AVGRP ? 0.0f : AT(I.b.ru_avg, e, k);
   const float spu = S3 ? AT(I.b.split_u, e, k) : 0.0f;
   const float r0 = (AT(I.b.s_rho_p, c0, k) + AT(I.b.rho_pp, c0, k)) + AT(I.z.rb, c0, k);
   const float r1 = (AT(I.b.s_rho_p, c1, k) + AT(I.b.rho_pp, c1, k)) + AT(I.z.rb, c1, k);
+  const float rp_raw = AVGRP ? dts * rp_ld : rp_ld;
+  const float avg = AVGRP ? rp_raw : avg_ld;
   const float rp = DAMP ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
   const float ru = sru + rp;
   const float fu = sru + avg * inv_steps;
@@ -1711,7 +1719,9 @@ KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef) {
     split = sub == I.k.splits ? acc * I.k.split_inv : acc;
   }
   const float uu = (2.0f * ru) / (r0 + r1);
-  if (DAMP) AT(I.b.ru_p, e, k) = rp;
+  // the damped ru_p is read again only as the step's output (stage 3) or by
+  // an export between phases; the next stage forms its own
+  if (DAMP && (S3 || I.k.store_flux)) AT(I.b.ru_p, e, k) = rp;
   AT(I.b.c_ru, e, k) = ru;
   if (I.k.store_flux) AT(I.b.flux_u, e, k) = fu;
   if (S3) AT(I.b.split_u, e, k) = split;
@@ -2385,7 +2395,8 @@ int dyn_tend(const Instance &I, int stage, hipStream_t s) {
            (const float *)I.b.rtheta_p);
   if (I.k.fused) {
     // the edge fluxes came with the preceding diagnostics
-    LAUNCH_NE(I, k_tend_mom_ac, I.n.E, 1, s, I, I.k.dts[stage - 1], I.k.steps[stage - 1] > 2 ? 1 : 0);
+    LAUNCH_NE(I, k_tend_mom_ac, I.n.E, 1, s, I, I.k.dts[stage - 1], I.k.steps[stage - 1] > 2 ? 1 : 0,
+              I.k.steps[stage - 1] > 1 || stage == 3 ? 1 : 0);
   } else {
     LAUNCH_NE(I, k_tend_mom, I.n.E, 1, s, I);
     if (stage > 1)
@@ -2441,15 +2452,16 @@ int divergence_damping(const Instance &I, int stage, hipStream_t s) {
 }
 
 template <bool S3>
-void recover_edges(const Instance &I, int sub, float inv, bool damp, bool avgrp, float coef, hipStream_t s) {
+void recover_edges(const Instance &I, int sub, float inv, bool damp, bool avgrp, float coef, float dts,
+                   hipStream_t s) {
   if (damp && avgrp)
-    launch_named("k_rec_edge", k_rec_edge<S3, true, true>, I.n.E, 1, s, I, sub, inv, coef);
+    launch_named("k_rec_edge", k_rec_edge<S3, true, true>, I.n.E, 1, s, I, sub, inv, coef, dts);
   else if (damp)
-    launch_named("k_rec_edge", k_rec_edge<S3, true, false>, I.n.E, 1, s, I, sub, inv, coef);
+    launch_named("k_rec_edge", k_rec_edge<S3, true, false>, I.n.E, 1, s, I, sub, inv, coef, dts);
   else if (avgrp)
-    launch_named("k_rec_edge", k_rec_edge<S3, false, true>, I.n.E, 1, s, I, sub, inv, coef);
+    launch_named("k_rec_edge", k_rec_edge<S3, false, true>, I.n.E, 1, s, I, sub, inv, coef, dts);
   else
-    launch_named("k_rec_edge", k_rec_edge<S3, false, false>, I.n.E, 1, s, I, sub, inv, coef);
+    launch_named("k_rec_edge", k_rec_edge<S3, false, false>, I.n.E, 1, s, I, sub, inv, coef, dts);
 }
 
 int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
@@ -2457,11 +2469,11 @@ int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
   const bool avgrp = I.k.fused && I.k.steps[stage - 1] == 1;
   const float coef = I.k.dts_inv_coef[stage - 1];
   if (stage == 3) {
-    recover_edges<true>(I, sub, inv, damp != 0, avgrp, coef, s);
+    recover_edges<true>(I, sub, inv, damp != 0, avgrp, coef, I.k.dts[stage - 1], s);
     NEDISPATCH(I, launch_named("k_rec_cell", k_rec_cell<6, true>, I.n.C, 1, s, I, sub, inv),
                launch_named("k_rec_cell", k_rec_cell<7, true>, I.n.C, 1, s, I, sub, inv));
   } else {
-    recover_edges<false>(I, sub, inv, damp != 0, avgrp, coef, s);
+    recover_edges<false>(I, sub, inv, damp != 0, avgrp, coef, I.k.dts[stage - 1], s);
     NEDISPATCH(I, launch_named("k_rec_cell", k_rec_cell<6, false>, I.n.C, 1, s, I, sub, inv),
                launch_named("k_rec_cell", k_rec_cell<7, false>, I.n.C, 1, s, I, sub, inv));
   }
