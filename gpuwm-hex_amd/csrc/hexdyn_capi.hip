@@ -1169,7 +1169,9 @@ int hexdyn_recover(hexdyn_handle *h, int stage, void *stream) {
 
 int hexdyn_solve_diagnostics(hexdyn_handle *h, int rk_step, void *stream) {
   if (!h || rk_step < 1 || rk_step > 3) return HEXDYN_E_INVALID;
-  return NS_CALL(h, solve_diagnostics, view(h, false), rk_step, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
+  Instance J = view(h, false);
+  J.k.store_flux = 1;  // the phase API keeps divergence and vorticity exportable at every stage
+  return NS_CALL(h, solve_diagnostics, J, rk_step, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
 }
 
 int hexdyn_end_subcycle(hexdyn_handle *h, int subcycle, void *stream) {

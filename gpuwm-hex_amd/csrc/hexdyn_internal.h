@@ -153,9 +153,10 @@ struct Consts {
   int mixing, mix4;              // Smagorinsky mixing on; fourth-order part on (h4 > 0)
   int physics;                   // held physics tendencies are added
   int physics_q;                 // a held scalar tendency enters the transport
-  int store_flux;                // the phase API: recovery also stores what only exports read between
+  int store_flux;                // the phase API: kernels also store what only exports read between
                                  // phases (the stage's flux_u/flux_w, which stage 3 folds into the
-                                 // split sums, and rho_p/rtheta_m at stages 1-2)
+                                 // split sums; rho_p/rtheta_m at stages 1-2; divergence/vorticity
+                                 // of rk 1-2; the damped ru_p of stages 1-2)
   int fused;                     // whole-step launch sequence: the next stage's theta/w edge fluxes
                                  // ride on the edge diagnostics, the first acoustic edge update on
                                  // the momentum tendency, and ru_avg is kept only where read back

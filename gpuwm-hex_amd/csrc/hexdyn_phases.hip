@@ -230,7 +230,7 @@ __device__ __forceinline__ void stage_levels(float *s_lev, const float *const (&
 // system overlap every gather of the wave.
 
 // ------------------------------------------------------------------ diagnostics
-KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
+KB k_diag_vertex(const Instance I, const float *__restrict__ u, int store_mix) {
   const int v = wave_entity(I);
   if (v >= I.n.V) return;
   const int k = lane();
@@ -256,7 +256,7 @@ KB k_diag_vertex(const Instance I, const float *__restrict__ u) {
   kev = kev * (0.25f * it);
   const float pvv = fv + vort;
   if (k < I.n.L) {
-    AT(I.b.vorticity, v, k) = vort;
+    if (store_mix) AT(I.b.vorticity, v, k) = vort;
     AT(I.b.ke_vertex, v, k) = kev;
     AT(I.b.pv_vertex, v, k) = pvv;
   }
@@ -329,20 +329,20 @@ __device__ __forceinline__ CellDiag diag_cell_compute(const Instance &I, const C
     }
   return {d, kc, pvc};
 }
-__device__ __forceinline__ void store_cell_diag(const Instance &I, int c, int k, CellDiag r) {
+__device__ __forceinline__ void store_cell_diag(const Instance &I, int c, int k, CellDiag r, int store_mix) {
   if (k < I.n.L) {
-    AT(I.b.divergence, c, k) = r.div;
+    if (store_mix) AT(I.b.divergence, c, k) = r.div;
     AT(I.b.ke, c, k) = r.ke;
     if (I.k.apvm_on) AT(I.b.pv_cell, c, k) = r.pv;
   }
 }
 
 template <int NE>
-KB k_diag_cell(const Instance I, const float *__restrict__ u) {
+KB k_diag_cell(const Instance I, const float *__restrict__ u, int store_mix) {
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
-  store_cell_diag(I, c, k, diag_cell_compute<NE>(I, diag_cell_load<NE>(I, u, c, k)));
+  store_cell_diag(I, c, k, diag_cell_compute<NE>(I, diag_cell_load<NE>(I, u, c, k)), store_mix);
 }
 
 // pv_edge (with APVM) and, at rk step 3, the TRiSK tangential velocity v of
@@ -1333,10 +1333,13 @@ KB k_tend_cell(const Instance I) {
 
 // ----------------------------------------------------------- acoustic substep
 // ru_p + divergence-damping increment of edge e at level k (mixing.py:851-945)
+// OLD0: the damped substep was a stage's first, whose rtheta_pp_old is zero
+// (then neither stored nor read; x - 0 keeps the bits of x)
+template <bool OLD0 = false>
 __device__ __forceinline__ float damped_ru_p(const Instance &I, int e, int k, float rp, float coefficient) {
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  const float d0 = AT(I.b.rtheta_pp, c0, k) - AT(I.b.rtheta_pp_old, c0, k);
-  const float d1 = AT(I.b.rtheta_pp, c1, k) - AT(I.b.rtheta_pp_old, c1, k);
+  const float d0 = AT(I.b.rtheta_pp, c0, k) - (OLD0 ? 0.0f : AT(I.b.rtheta_pp_old, c0, k));
+  const float d1 = AT(I.b.rtheta_pp, c1, k) - (OLD0 ? 0.0f : AT(I.b.rtheta_pp_old, c1, k));
   const float den = AT(I.b.s_theta, c0, k) + AT(I.b.s_theta, c1, k);
   return rp + ((coefficient * (d0 - d1)) * (1.0f - 0.0f)) / den;
 }
@@ -1373,8 +1376,10 @@ KB k_ac_edge(const Instance I, float dts, float damp_coef) {
   const float q0 = AT(I.b.rho_pp, c0, k), q1 = AT(I.b.rho_pp, c1, k);
   float to0 = 0.0f, to1 = 0.0f, st0 = 0.0f, st1 = 0.0f;
   if (DAMP) {
-    to0 = AT(I.b.rtheta_pp_old, c0, k);
-    to1 = AT(I.b.rtheta_pp_old, c1, k);
+    if (!AVGRP) {  // after a stage's first substep (AVGRP: the second of two) rtheta_pp_old is zero
+      to0 = AT(I.b.rtheta_pp_old, c0, k);
+      to1 = AT(I.b.rtheta_pp_old, c1, k);
+    }
     st0 = AT(I.b.s_theta, c0, k);
     st1 = AT(I.b.s_theta, c1, k);
   }
@@ -1393,7 +1398,7 @@ KB k_ac_edge(const Instance I, float dts, float damp_coef) {
 // FIRST: the first substep of a stage (the perturbation fields start at
 // zero).  Every global load is issued before the column sweep.
 template <int NE, bool FIRST>
-KBCOL(COLW_AC) k_ac_cell(const Instance I, float dts) {
+KBCOL(COLW_AC) k_ac_cell(const Instance I, float dts, int store_old) {
   constexpr int COLW = COLW_AC;
   static_assert(COLW >= 7, "waves 0-6 stage the level rows");
   __shared__ float s_x[COLW * COLP], s_d[COLW * COLP], s_l[COLW * COLP], s_g[COLW * COLP];
@@ -1538,7 +1543,7 @@ KBCOL(COLW_AC) k_ac_cell(const Instance I, float dts) {
   if (live && k < L) {
     AT(I.b.rho_pp, c, k) = rpp_new;
     AT(I.b.rtheta_pp, c, k) = rtpp_new;
-    AT(I.b.rtheta_pp_old, c, k) = rtpp_old;
+    if (store_old) AT(I.b.rtheta_pp_old, c, k) = rtpp_old;
   }
 }
 
@@ -1710,7 +1715,7 @@ KB k_rec_edge(const Instance I, int sub, float inv_steps, float damp_coef, float
   const float r1 = (AT(I.b.s_rho_p, c1, k) + AT(I.b.rho_pp, c1, k)) + AT(I.z.rb, c1, k);
   const float rp_raw = AVGRP ? dts * rp_ld : rp_ld;
   const float avg = AVGRP ? rp_raw : avg_ld;
-  const float rp = DAMP ? damped_ru_p(I, e, k, rp_raw, damp_coef) : rp_raw;
+  const float rp = DAMP ? damped_ru_p<AVGRP>(I, e, k, rp_raw, damp_coef) : rp_raw;
   const float ru = sru + rp;
   const float fu = sru + avg * inv_steps;
   float split = 0.0f;
@@ -2314,8 +2319,12 @@ inline bool tiled_flux_on(const Instance &I) {
 }
 
 int solve_diagnostics(const Instance &I, int rk, hipStream_t s) {
-  launch(k_diag_vertex, I.n.V, 1, s, I, (const float *)I.b.u);
-  LAUNCH_NE(I, k_diag_cell, I.n.C, 1, s, I, (const float *)I.b.u);
+  // divergence and vorticity are read only by the horizontal mixing, from
+  // the diagnostics that open a subcycle (rk 3, and the step's first), or by
+  // an export between phases
+  const int store_mix = rk == 3 || I.k.store_flux ? 1 : 0;
+  launch(k_diag_vertex, I.n.V, 1, s, I, (const float *)I.b.u, store_mix);
+  LAUNCH_NE(I, k_diag_cell, I.n.C, 1, s, I, (const float *)I.b.u, store_mix);
   const float *u = I.b.u;
   const bool six = I.n.ME == 6;
   if (I.k.fused && tiled_flux_on(I)) {
@@ -2437,12 +2446,16 @@ int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStr
     else
       launch_named("k_ac_edge", k_ac_edge<false, false, false>, I.n.E, 1, s, I, dts, coef);
   }
+  // a stage's first substep leaves rtheta_pp_old zero; on the step path the
+  // damping after it takes that zero without a store and a reload (the
+  // second of two substeps' edge update, or a one-substep stage's recovery)
+  const int store_old = I.k.fused && ss == 1 && n <= 2 ? 0 : 1;
   if (ss == 1)
-    NEDISPATCH(I, launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<6, true>, I.n.C, s, I, dts),
-               launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<7, true>, I.n.C, s, I, dts));
+    NEDISPATCH(I, launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<6, true>, I.n.C, s, I, dts, store_old),
+               launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<7, true>, I.n.C, s, I, dts, store_old));
   else
-    NEDISPATCH(I, launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<6, false>, I.n.C, s, I, dts),
-               launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<7, false>, I.n.C, s, I, dts));
+    NEDISPATCH(I, launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<6, false>, I.n.C, s, I, dts, store_old),
+               launch_col_named<COLW_AC>("k_ac_cell", k_ac_cell<7, false>, I.n.C, s, I, dts, store_old));
   return check();
 }
 
