@@ -284,6 +284,9 @@ int run_phase_begin_step(hexdyn_handle *h, hipStream_t s, bool fused);
 // The device pointers the kernels of the current phase see (buffer roles above).
 Instance view(const hexdyn_handle *h, bool recovery) {
   Instance J = h->I;
+  // the phase entry points keep every exportable working field current;
+  // the whole-step sequence (step_view) stores only what a later kernel reads
+  J.k.store_flux = 1;
   Work &B = J.b;
   if (h->start_is_pub) {
     B.s_rho = B.rho; B.s_rt = B.rt; B.s_ru = B.ru; B.s_rw = B.rw;
@@ -990,6 +993,7 @@ int run_phase_begin_step(hexdyn_handle *h, hipStream_t s, bool fused) {
   h->pp_pub = 1;
   Instance J = view(h, false);
   J.k.fused = fused ? 1 : 0;
+  J.k.store_flux = fused ? 0 : 1;
   return NS_CALL(h, begin_step, J, s);
 }
 
@@ -1161,7 +1165,6 @@ int hexdyn_recover(hexdyn_handle *h, int stage, void *stream) {
   // At stage 3 the recovered fluxes are also accumulated into the split-transport
   // sums (copy on the first subcycle, add afterwards: integration.py:27-56).
   Instance J = view(h, true);
-  J.k.store_flux = 1;  // the phase API keeps the stage fluxes exportable
   const int rc = NS_CALL(h, recover, J, stage, h->cur_sub, 0, (hipStream_t)stream);
   recovered(h, stage);
   return rc ? HEXDYN_E_HIP : 0;
@@ -1170,7 +1173,6 @@ int hexdyn_recover(hexdyn_handle *h, int stage, void *stream) {
 int hexdyn_solve_diagnostics(hexdyn_handle *h, int rk_step, void *stream) {
   if (!h || rk_step < 1 || rk_step > 3) return HEXDYN_E_INVALID;
   Instance J = view(h, false);
-  J.k.store_flux = 1;  // the phase API keeps divergence and vorticity exportable at every stage
   return NS_CALL(h, solve_diagnostics, J, rk_step, (hipStream_t)stream) ? HEXDYN_E_HIP : 0;
 }
 
@@ -1200,6 +1202,7 @@ namespace {
 // The instance as the whole-step sequence sees it: fused launches on.
 Instance step_view(const hexdyn_handle *h, bool recovery) {
   Instance J = view(h, recovery);
+  J.k.store_flux = 0;
   J.k.fused = 1;
   return J;
 }

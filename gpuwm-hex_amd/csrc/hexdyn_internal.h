@@ -153,10 +153,11 @@ struct Consts {
   int mixing, mix4;              // Smagorinsky mixing on; fourth-order part on (h4 > 0)
   int physics;                   // held physics tendencies are added
   int physics_q;                 // a held scalar tendency enters the transport
-  int store_flux;                // the phase API: kernels also store what only exports read between
-                                 // phases (the stage's flux_u/flux_w, which stage 3 folds into the
-                                 // split sums; rho_p/rtheta_m at stages 1-2; divergence/vorticity
-                                 // of rk 1-2; the damped ru_p of stages 1-2)
+  int store_flux;                // set by every phase entry point, clear on the whole-step path:
+                                 // kernels also store what only exports read between phases (the
+                                 // stage's flux_u/flux_w, which stage 3 folds into the split sums;
+                                 // rho_p/rtheta_m at stages 1-2, rtheta_p at stage 2; divergence and
+                                 // vorticity of rk 1-2; the damped ru_p of stages 1-2; atri)
   int fused;                     // whole-step launch sequence: the next stage's theta/w edge fluxes
                                  // ride on the edge diagnostics, the first acoustic edge update on
                                  // the momentum tendency, and ru_avg is kept only where read back

@@ -686,7 +686,7 @@ KBCOL(COLW_VC) k_vcoef(const Instance I, float dts, const float *__restrict__ ex
     if (k < L) {
       AT(I.b.cofwz, c, k) = cofwz;
       AT(I.b.cofwt, c, k) = cofwt;
-      AT(I.b.atri, c, k) = a;
+      if (I.k.store_flux) AT(I.b.atri, c, k) = a;  // export only: k_ac_cell forms a itself
       AT(I.b.alpha, c, k) = alpha;
       AT(I.b.gamma, c, k) = gamma;
     }
@@ -1629,7 +1629,7 @@ __device__ __forceinline__ float recover_w_column(const Instance &I, int c, int 
 // loads and the terrain w recovery come first, branch-free, so every gather
 // of the pass is in flight together; stores come last.
 template <int NE, bool S3>
-KB k_rec_cell(const Instance I, int sub, float inv_steps) {
+KB k_rec_cell(const Instance I, int sub, float inv_steps, int store_rtp) {
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
@@ -1677,7 +1677,7 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps) {
       AT(I.b.rho_p, c, k) = dp;
       AT(I.b.c_rt, c, k) = rtv;
     }
-    AT(I.b.rtheta_p, c, k) = rp;
+    if (store_rtp) AT(I.b.rtheta_p, c, k) = rp;
     AT(I.b.c_rho, c, k) = rho;
     AT(I.b.theta, c, k) = th;
     if (S3) {
@@ -2481,14 +2481,17 @@ int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s) {
   const float inv = I.k.inv_steps[stage - 1];
   const bool avgrp = I.k.fused && I.k.steps[stage - 1] == 1;
   const float coef = I.k.dts_inv_coef[stage - 1];
+  // rtheta_p is read by the implicit coefficients of stage 2 (from stage 1)
+  // and as the next subcycle's start (from stage 3)
+  const int store_rtp = stage == 3 || (stage == 1 && I.k.order == 3) || I.k.store_flux ? 1 : 0;
   if (stage == 3) {
     recover_edges<true>(I, sub, inv, damp != 0, avgrp, coef, I.k.dts[stage - 1], s);
-    NEDISPATCH(I, launch_named("k_rec_cell", k_rec_cell<6, true>, I.n.C, 1, s, I, sub, inv),
-               launch_named("k_rec_cell", k_rec_cell<7, true>, I.n.C, 1, s, I, sub, inv));
+    NEDISPATCH(I, launch_named("k_rec_cell", k_rec_cell<6, true>, I.n.C, 1, s, I, sub, inv, store_rtp),
+               launch_named("k_rec_cell", k_rec_cell<7, true>, I.n.C, 1, s, I, sub, inv, store_rtp));
   } else {
     recover_edges<false>(I, sub, inv, damp != 0, avgrp, coef, I.k.dts[stage - 1], s);
-    NEDISPATCH(I, launch_named("k_rec_cell", k_rec_cell<6, false>, I.n.C, 1, s, I, sub, inv),
-               launch_named("k_rec_cell", k_rec_cell<7, false>, I.n.C, 1, s, I, sub, inv));
+    NEDISPATCH(I, launch_named("k_rec_cell", k_rec_cell<6, false>, I.n.C, 1, s, I, sub, inv, store_rtp),
+               launch_named("k_rec_cell", k_rec_cell<7, false>, I.n.C, 1, s, I, sub, inv, store_rtp));
   }
   return check();
 }
