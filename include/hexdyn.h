@@ -171,8 +171,10 @@ typedef struct hexdyn_state_view {
  * (logical layout, device pointers).  Used by parity tests to compare every
  * phase against the CPU restatement.  The working fields (after
  * HEXDYN_F_W) hold what the phase entry points leave; hexdyn_step stores
- * only what a later kernel of the step reads, so after it FLUX_U and FLUX_W
- * are not current. */
+ * only what a later kernel of the step reads, so after it FLUX_U, FLUX_W,
+ * A_TRI and the diagnostics (V, PV_EDGE, KE, DIVERGENCE, VORTICITY, which
+ * the next step's start recomputes from the published state) are not
+ * current. */
 enum {
   HEXDYN_F_RHO_ZZ = 0, HEXDYN_F_RTHETA_M, HEXDYN_F_RU, HEXDYN_F_RW, HEXDYN_F_SCALARS,
   HEXDYN_F_THETA_M, HEXDYN_F_EXNER, HEXDYN_F_RHO_P, HEXDYN_F_RTHETA_P, HEXDYN_F_PRESSURE_P,
