@@ -1652,7 +1652,9 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps, int store_rtp) {
   const float rwv = inner ? srw + rwp : 0.0f;
   const float *cru = I.b.c_ru;
   float hd = 0.0f;
-  const float wv = recover_w_column<NE>(I, c, k, rho, rwv, [&](int e) { return AT(cru, e, k); }, &hd);
+  // hdiv feeds the next stage's momentum tendency; after stage 3 the next
+  // subcycle's opening pass forms it again
+  const float wv = recover_w_column<NE>(I, c, k, rho, rwv, [&](int e) { return AT(cru, e, k); }, S3 ? nullptr : &hd);
   int bad = (lay && (!(rho > 0.0f) || !(rtv > 0.0f))) ? 1 : 0;
   const float th = rtv / rho;
   float ex = 0.0f, ppv = 0.0f;
@@ -1684,7 +1686,7 @@ KB k_rec_cell(const Instance I, int sub, float inv_steps, int store_rtp) {
       AT(I.b.exner, c, k) = ex;
       AT(I.b.pp, c, k) = ppv;
     }
-    AT(I.b.hdiv, c, k) = hd;
+    if (!S3) AT(I.b.hdiv, c, k) = hd;
   }
   if (itf) {
     AT(I.b.c_rw, c, k) = rwv;
@@ -1927,39 +1929,6 @@ KB k_fct_source(const Instance I, float dt) {
   }
 }
 
-template <int NE>
-KB k_fct_target(const Instance I, float dt) {
-  const int c = wave_entity(I);
-  if (c >= I.n.C) return;
-  const int k = lane();
-  const int L = I.n.L;
-  const int n = I.m.neoc[c];
-  int e[NE];
-#pragma unroll
-  for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
-  SlotRow<NE> sd = slot_row<NE>(I.m.csdv + c * NE);
-  const float ia = uni_load(I.m.invarea + c);
-  const float wa_k = AT(I.b.split_w, c, k);
-  const float rho = AT(I.b.rho, c, k), rdzw = AT(I.z.rdzw, 0, k);
-#pragma unroll
-  for (int s = 0; s < NE; ++s) fence(e[s]);
-  float uh[NE];
-#pragma unroll
-  for (int s = 0; s < NE; ++s) uh[s] = AT(I.b.split_u, e[s], k);
-  keep(uh);
-  SlotRow<NE> sds = sd;
-  settle(sds);
-  float hd = 0.0f;
-#pragma unroll
-  for (int s = 0; s < NE; ++s)
-    if (s < n) hd = hd + (-(uh[s] * sds[s])) * ia;  // ((-csign)*uh)*dv, csign = +-1
-  const float wa_p = lane_off(wa_k, k, 1);
-  if (k < L) {
-    const float t = rho + dt * (hd - rdzw * (wa_p - wa_k));
-    AT(I.b.target, c, k) = t;
-    if (!(t > 0.0f)) set_flag(I.b.flag);
-  }
-}
 
 template <int NE>
 KB k_fct_edge1(const Instance I, const float *__restrict__ qs, float dt, const float *__restrict__ qold, float qadd) {
@@ -2052,7 +2021,6 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const f
   const float orho = AT(I.b.rho, c, k);
   const float wk = AT(I.b.split_w, c, k);
   const float rdzw = AT(I.z.rdzw, 0, k), fzm = AT(I.z.fzm, 0, k), fzp = AT(I.z.fzp, 0, k);
-  const float target = AT(I.b.target, c, k);
 #pragma unroll
   for (int s = 0; s < NE; ++s) {
     fence(nb[s]);
@@ -2088,6 +2056,19 @@ KB k_fct_cell1(const Instance I, const float *__restrict__ qs, float dt, const f
     float sg[NE], sd[NE];
     slot_vals(sgs, sg);
     slot_vals(sds, sd);
+    // the target density (transport.py:1964-1989): the old density plus dt
+    // times the split mass-flux convergence, from rows this pass holds; the
+    // first batch stores it for the second pass and checks it
+    float hd = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+      if (s < n) hd = hd + (-(uh[s] * sd[s])) * ia;  // ((-csign)*uh)*dv, csign = +-1
+    const float wk_p = lane_off(wk, k, 1);
+    const float target = orho + dt * (hd - rdzw * (wk_p - wk));
+    if (sp0 == 0 && k < L) {
+      AT(I.b.target, c, k) = target;
+      if (!(target > 0.0f)) set_flag(I.b.flag);
+    }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
       if (sp0 + g >= NT) break;
@@ -2514,7 +2495,6 @@ int scalars_stage(const Instance &I, int stage, hipStream_t s) {
     qold = I.b.q_src;
     qadd = -0.0f;
   }
-  LAUNCH_NE(I, k_fct_target, I.n.C, 1, s, I, dt);
   LAUNCH_NE(I, k_fct_edge1, I.n.E, 1, s, I, qin, dt, qold, qadd);
   if (I.n.OCP)
     NEDISPATCH(I, launch_named("k_fct_cell1", k_fct_cell1<6, true>, I.n.C, 1, s, I, qin, dt, qold, qadd),

@@ -101,10 +101,9 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     "k_sc_edge": [("SC", "L"), ("E", "L"), ("SE", "L")],
     "k_sc_cell": [("SC", "L"), ("C", "W"), ("E", "L"), ("SE", "L"), ("C", "L"), ("C", "L"), ("SC", "L"),
                   ("SC", "L")],
-    "k_fct_target": [("E", "L"), ("C", "W"), ("C", "L"), ("C", "L")],
     # qs, qold, split_u | hres
     "k_fct_edge1": [("SC", "L"), ("SC", "L"), ("E", "L"), ("SE", "L")],
-    # qs, qold, rho, split_w, split_u, hres, target | mass, sin, sout, vres
+    # qs, qold, rho, split_w, split_u, hres | target, mass, sin, sout, vres
     "k_fct_cell1": [("SC", "L"), ("SC", "L"), ("C", "L"), ("C", "W"), ("E", "L"), ("SE", "L"), ("C", "L"),
                     ("SC", "L"), ("SC", "L"), ("SC", "L"), ("SC", "W")],
     "k_fct_edge2": [("SC", "L"), ("SC", "L"), ("SE", "L"), ("SE", "L")],
