@@ -949,7 +949,7 @@ template <class T> struct Oracle {
   // --------------------------------------------------------------- recovery
   void recover(int stage) {
     const T inv_steps = T(1) / T(dyn[stage - 1].steps);
-    const T two = T(2), one = T(1);
+    const T two = T(2);
     for (size_t i = 0; i < cL(); ++i) {
       const T dp = s_rho_p[i] + rho_pp[i];
       const T rp = s_rtheta_p[i] + rtheta_pp[i];
@@ -1051,7 +1051,7 @@ template <class T> struct Oracle {
   void scalars_stage(int stage) {
     const T dt_s = T(sca[stage - 1].large);
     const T coef = T(cfg.config_coef_3rd_order);
-    const T one = T(1), zero = T(0);
+    const T one = T(1);
     const std::vector<T> &uh = split_u;
     const std::vector<T> &wa = split_w;
     std::vector<T> out((size_t)NT * cL());
