@@ -180,6 +180,7 @@ struct hexdyn_handle {
   int use_graphs = 1;
   int post_clamp = 1;  // end-of-step q < 0 -> 0 (0: scalars handed over unclamped)
   hipStream_t capture_stream = nullptr;
+  Side side{};  // the whole step's second stream and its fork/join events (created with the first step)
   struct Roles {  // host-side buffer roles and step flags
     Instance I;
     int in_step, cur_sub, start_is_pub, fresh_sub, fresh_step, exner_start, pp_pub;
@@ -403,6 +404,9 @@ void hexdyn_destroy(hexdyn_handle *h) {
   if (!h) return;
   for (auto &g : h->graphs) (void)hipGraphExecDestroy(g.second.first);
   if (h->capture_stream) (void)hipStreamDestroy(h->capture_stream);
+  if (h->side.s) (void)hipStreamDestroy(h->side.s);
+  if (h->side.fork) (void)hipEventDestroy(h->side.fork);
+  if (h->side.join) (void)hipEventDestroy(h->side.join);
   for (void *p : h->allocs) (void)hipFree(p);
   delete h;
 }
@@ -1212,13 +1216,20 @@ Instance step_view(const hexdyn_handle *h, bool recovery) {
 // (begin_step's for the first stage of the first subcycle).
 int enqueue_step(hexdyn_handle *h, hipStream_t s) {
   const hexdyn_config &c = h->cfg;
+  if (!h->side.s &&
+      (hipStreamCreateWithFlags(&h->side.s, hipStreamNonBlocking) != hipSuccess ||
+       hipEventCreateWithFlags(&h->side.fork, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&h->side.join, hipEventDisableTiming) != hipSuccess))
+    return HEXDYN_E_HIP;
   int rc = run_phase_begin_step(h, s, true);
+  // profiled steps stay on one stream, so each kernel's HIP-event time is its own
+  const Side *side = g_prof_on ? nullptr : &h->side;
   for (int sub = 1; rc == 0 && sub <= c.config_dynamics_split_steps; ++sub) {
     h->cur_sub = sub;
     rotate_subcycle(h, sub);
-    rc |= NS_CALL(h, begin_subcycle, step_view(h, false), sub, s);
+    rc |= NS_CALL(h, begin_subcycle, step_view(h, false), sub, s, side);
     for (int st = 1; rc == 0 && st <= 3; ++st) {
-      rc |= NS_CALL(h, dyn_tend, step_view(h, false), st, s);
+      rc |= NS_CALL(h, dyn_tend, step_view(h, false), st, s, side);
       // divergence damping of substep ss is fused into the next edge pass:
       // the acoustic edge update of ss+1, or the edge recovery after the last substep
       const int dd = c.config_divergence_damping ? 1 : 0;

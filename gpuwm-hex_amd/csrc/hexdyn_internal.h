@@ -201,13 +201,22 @@ struct Instance {
   const LaunchTune *tune;  // host pointer to the handle's launch table
 };
 
+// A second stream (default priority) for the implicit coefficients, which
+// depend on none of the kernels launched beside them: forked from the
+// caller's stream and joined back to it with the two events, so in a
+// captured step they are a parallel branch of the graph.  Null: one stream.
+struct Side {
+  hipStream_t s;
+  hipEvent_t fork, join;
+};
+
 // Phase entry points, compiled once with strict binary32 numerics
 // (no contraction) and once with fused multiply-adds allowed.
 #define HEXDYN_PHASES(NS)                                                 \
   namespace NS {                                                          \
   int begin_step(const Instance &I, hipStream_t s);                       \
-  int begin_subcycle(const Instance &I, int sub, hipStream_t s);          \
-  int dyn_tend(const Instance &I, int stage, hipStream_t s);              \
+  int begin_subcycle(const Instance &I, int sub, hipStream_t s, const Side *side = nullptr); \
+  int dyn_tend(const Instance &I, int stage, hipStream_t s, const Side *side = nullptr);     \
   int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStream_t s); \
   int divergence_damping(const Instance &I, int stage, hipStream_t s);    \
   int recover(const Instance &I, int stage, int sub, int damp, hipStream_t s); \

@@ -2339,8 +2339,25 @@ int begin_step(const Instance &I, hipStream_t s) {
   return solve_diagnostics(I, 3, s);
 }
 
-int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
+// The side stream (or s itself): ordered after everything s has queued.
+inline hipStream_t fork_side(const Side *side, hipStream_t s, int &rc) {
+  if (!side) return s;
+  if (hipEventRecord(side->fork, s) != hipSuccess || hipStreamWaitEvent(side->s, side->fork, 0) != hipSuccess) rc = -1;
+  return side->s;
+}
+// s waits for what the side stream has queued
+inline void join_side(const Side *side, hipStream_t s, int &rc) {
+  if (!side) return;
+  if (hipEventRecord(side->join, side->s) != hipSuccess || hipStreamWaitEvent(s, side->join, 0) != hipSuccess) rc = -1;
+}
+
+int begin_subcycle(const Instance &I, int sub, hipStream_t s, const Side *side) {
   (void)sub;
+  // the implicit coefficients read only the subcycle-start state, the
+  // moisture and reference rows: beside the pools and the mixing
+  int rc = 0;
+  launch_col_named<COLW_VC>("k_vcoef", k_vcoef, I.n.C, fork_side(side, s, rc), I, I.k.dts[0],
+                            (const float *)I.b.s_exner, (const float *)I.b.s_theta, (const float *)I.b.s_rtheta_p);
   // with mixing, kdiff rides on the cell pass and the del2 u increment on the
   // edge pass (the pool is written once)
   if (I.k.mixing) {
@@ -2360,8 +2377,6 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
                  launch_named("k_sub_cell", k_sub_cell<7, false, false>, I.n.C, 1, s, I));
     launch_named("k_sub_edge", k_sub_edge<false>, I.n.E, 1, s, I);
   }
-  launch_col_named<COLW_VC>("k_vcoef", k_vcoef, I.n.C, s, I, I.k.dts[0], (const float *)I.b.s_exner, (const float *)I.b.s_theta,
-         (const float *)I.b.s_rtheta_p);
   if (I.k.mixing) {
     if (I.k.mix4) {
       LAUNCH_NE(I, k_mix_lapdiv, I.n.C, 1, s, I);
@@ -2376,13 +2391,18 @@ int begin_subcycle(const Instance &I, int sub, hipStream_t s) {
                  launch_named("k_mix_cell", k_mix_cell<7, false>, I.n.C, 1, s, I));
     if (I.k.mix4) LAUNCH_NE(I, k_mix_cell4, I.n.C, 1, s, I);
   }
-  return check();
+  join_side(side, s, rc);
+  return rc ? rc : check();
 }
 
-int dyn_tend(const Instance &I, int stage, hipStream_t s) {
-  if (stage == 2 && I.k.order == 3)
-    launch_col_named<COLW_VC>("k_vcoef", k_vcoef, I.n.C, s, I, I.k.dts[1], (const float *)I.b.s_exner, (const float *)I.b.theta,
-           (const float *)I.b.rtheta_p);
+int dyn_tend(const Instance &I, int stage, hipStream_t s, const Side *side) {
+  // stage 2's implicit coefficients beside the stage's tendencies (which
+  // neither read nor write them); joined before the acoustic substeps
+  int rc = 0;
+  const bool vc = stage == 2 && I.k.order == 3;
+  if (vc)
+    launch_col_named<COLW_VC>("k_vcoef", k_vcoef, I.n.C, fork_side(side, s, rc), I, I.k.dts[1],
+                              (const float *)I.b.s_exner, (const float *)I.b.theta, (const float *)I.b.rtheta_p);
   if (I.k.fused) {
     // the edge fluxes came with the preceding diagnostics
     LAUNCH_NE(I, k_tend_mom_ac, I.n.E, 1, s, I, I.k.dts[stage - 1], I.k.steps[stage - 1] > 2 ? 1 : 0,
@@ -2406,7 +2426,8 @@ int dyn_tend(const Instance &I, int stage, hipStream_t s) {
 #undef TEND_CELL_NE
 #undef TEND_CELL
   }
-  return check();
+  if (vc) join_side(side, s, rc);
+  return rc ? rc : check();
 }
 
 int acoustic_substep(const Instance &I, int stage, int ss, int damp_prev, hipStream_t s) {
