@@ -583,12 +583,24 @@ __global__ void __launch_bounds__(64 * HX_TILE_E) k_diag_edge_flux_t(const Insta
 }
 
 // ---------------------------------------------------------- moist coefficients
+// the moist species are summed in order, their rows loaded HX_MOIST_G at a
+// time before any of them is added (one round trip per batch, not per species)
+constexpr int HX_MOIST_G = 8;
 KB k_moist_cell(const Instance I) {
   const int c = wave_entity(I);
   if (c >= I.n.C) return;
   const int k = lane();
+  const int NM = I.n.NM;
   float t = 0.0f;
-  for (int s = 0; s < I.n.NM; ++s) t = t + AT(I.b.q, s * I.n.C + c, k);
+  for (int s0 = 0; s0 < NM; s0 += HX_MOIST_G) {
+    float q[HX_MOIST_G];
+#pragma unroll
+    for (int g = 0; g < HX_MOIST_G; ++g) q[g] = s0 + g < NM ? AT(I.b.q, (s0 + g) * I.n.C + c, k) : 0.0f;
+    keep(q);
+#pragma unroll
+    for (int g = 0; g < HX_MOIST_G; ++g)
+      if (s0 + g < NM) t = t + q[g];
+  }
   const float tm = up1(t);
   const float cqw = k >= 1 ? 1.0f / (1.0f + 0.5f * (t + tm)) : 1.0f;
   if (k < I.n.L) {
@@ -602,9 +614,22 @@ KB k_moist_edge(const Instance I) {
   if (e >= I.n.E) return;
   const int k = lane();
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
+  const int NM = I.n.NM;
+  fence(c0);
+  fence(c1);
   float t = 0.0f;
-  for (int s = 0; s < I.n.NM; ++s)
-    t = t + 0.5f * (AT(I.b.q, s * I.n.C + c0, k) + AT(I.b.q, s * I.n.C + c1, k));
+  for (int s0 = 0; s0 < NM; s0 += HX_MOIST_G) {
+    float q0[HX_MOIST_G], q1[HX_MOIST_G];
+#pragma unroll
+    for (int g = 0; g < HX_MOIST_G; ++g) {
+      q0[g] = s0 + g < NM ? AT(I.b.q, (s0 + g) * I.n.C + c0, k) : 0.0f;
+      q1[g] = s0 + g < NM ? AT(I.b.q, (s0 + g) * I.n.C + c1, k) : 0.0f;
+    }
+    keep(q0, q1);
+#pragma unroll
+    for (int g = 0; g < HX_MOIST_G; ++g)
+      if (s0 + g < NM) t = t + 0.5f * (q0[g] + q1[g]);
+  }
   if (k < I.n.L) AT(I.b.cqu, e, k) = 1.0f / (1.0f + t);
 }
 
