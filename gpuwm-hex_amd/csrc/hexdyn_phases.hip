@@ -1948,9 +1948,19 @@ KB k_fct_source(const Instance I, float dt) {
   const int k = lane();
   if (k >= I.n.L) return;
   const float rho = AT(I.b.rho, c, k);
-  for (int sp = 0; sp < I.n.NT; ++sp) {
-    const int r = sp * I.n.C + c;
-    AT(I.b.q_src, r, k) = AT(I.b.q, r, k) + (dt * AT(I.b.phys_q, r, k)) / rho;
+  const int NT = I.n.NT;
+  for (int s0 = 0; s0 < NT; s0 += HX_MOIST_G) {  // every row of a batch loaded before its stores
+    float q[HX_MOIST_G], f[HX_MOIST_G];
+#pragma unroll
+    for (int g = 0; g < HX_MOIST_G; ++g) {
+      const int r = (s0 + g < NT ? s0 + g : NT - 1) * I.n.C + c;
+      q[g] = AT(I.b.q, r, k);
+      f[g] = AT(I.b.phys_q, r, k);
+    }
+    keep(q, f);
+#pragma unroll
+    for (int g = 0; g < HX_MOIST_G; ++g)
+      if (s0 + g < NT) AT(I.b.q_src, (s0 + g) * I.n.C + c, k) = q[g] + (dt * f[g]) / rho;
   }
 }
 
