@@ -658,18 +658,30 @@ __global__ void __launch_bounds__(256) k_checks_cells(const Instance I, float dt
   double mass = 0.0;
   if (c < I.n.C) {
     const int n = I.m.neoc[c];
+    // every load first (slot tables are padded, so each slot is readable),
+    // then the sums in slot order
+    int e[NE];
+#pragma unroll
+    for (int s = 0; s < NE; ++s) e[s] = I.m.eoc[c * NE + s];
+    float u[NE], sd[NE], idc[NE];
+#pragma unroll
+    for (int s = 0; s < NE; ++s) {
+      u[s] = rowv(B.p_u, e[s], k, I.n.RS);
+      sd[s] = I.m.csdv[c * NE + s];
+      idc[s] = I.m.invdc[e[s]];
+    }
+    const float wk = rowv(B.p_w, c, k, I.n.RS), zk = rowv(I.z.zz, c, k, I.n.RS), rk = rowv(B.rho, c, k, I.n.RS);
+    const float rdzw = I.z.rdzw[k];
 #pragma unroll
     for (int s = 0; s < NE; ++s) {
       if (s < n) {
-        const int e = I.m.eoc[c * NE + s];
-        const float u = rowv(B.p_u, e, k, I.n.RS);
-        div += I.m.csdv[c * NE + s] * u;
-        cfl = nanmax(cfl, fabsf(u) * dt * I.m.invdc[e]);
+        div += sd[s] * u[s];
+        cfl = nanmax(cfl, fabsf(u[s]) * dt * idc[s]);
       }
     }
     div = fabsf(div * I.m.invarea[c]) * dt;
-    if (k >= 1 && k < L) vc = fabsf(rowv(B.p_w, c, k, I.n.RS)) * dt * I.z.rdzw[k] / rowv(I.z.zz, c, k, I.n.RS);
-    if (k < L) mass = (1.0 / (double)I.m.invarea[c]) * (1.0 / (double)I.z.rdzw[k]) * (double)rowv(B.rho, c, k, I.n.RS);
+    if (k >= 1 && k < L) vc = fabsf(wk) * dt * rdzw / zk;
+    if (k < L) mass = (1.0 / (double)I.m.invarea[c]) * (1.0 / (double)rdzw) * (double)rk;
     if (k >= L) {
       div = 0.f;
       cfl = 0.f;
