@@ -4,7 +4,8 @@
  * A scalar C++ restatement of the reference's MPAS-A v8.4.2 (v8.4.1 lane)
  * dry/moist split-explicit step, in the reference's operation order, for
  * T = float (strict binary32: build with -ffp-contract=off, no fast-math) and
- * T = double (the FP64 error judge).  Only tests/, __graft_entry__.smoke() and
+ * T = double (the FP64 reference the fast numerics are measured against).
+ * Only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg load it; the product never does.
  *
  * Layout: MPAS logical order, level first, entity fastest ([L][C], [L][E],
