@@ -740,23 +740,6 @@ __device__ __forceinline__ float kdiff_from_deformation(const Instance &I, int k
   return kd;
 }
 
-// del2 u of edge e at level k: the increment added to the u pool and delsq_u
-struct MixEdge {
-  float lap, inc;
-};
-__device__ __forceinline__ MixEdge mix_edge_value(const Instance &I, int e, int k) {
-  const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  const int v0 = I.m.voe[2 * e], v1 = I.m.voe[2 * e + 1];
-  const float d0 = AT(I.b.divergence, c0, k), d1 = AT(I.b.divergence, c1, k);
-  const float z0 = AT(I.b.vorticity, v0, k), z1 = AT(I.b.vorticity, v1, k);
-  const float kd0 = AT(I.b.kdiff, c0, k), kd1 = AT(I.b.kdiff, c1, k);
-  const float r0 = AT(I.b.c_rho, c0, k), r1 = AT(I.b.c_rho, c1, k);
-  const float idc = uni_load(I.m.mx_idc + e), idv = uni_load(I.m.mx_idv + e), del2 = uni_load(I.m.mx_del2 + e);
-  const float lap = (d1 - d0) * idc - (z1 - z0) * idv;
-  const float kedge = 0.5f * (kd0 + kd1);
-  const float rho_e = 0.5f * (r0 + r1);
-  return {lap, ((rho_e * kedge) * lap) * del2};
-}
 
 
 
@@ -1037,15 +1020,50 @@ KB k_sub_edge(const Instance I) {
   if (e >= I.n.E) return;
   const int k = lane();
   const int c0 = I.m.coe[2 * e], c1 = I.m.coe[2 * e + 1];
-  const float *pp = I.b.pp, *zz = I.z.zz, *dz = I.b.dpdz;
-  const float a = ((AT(pp, c1, k) - AT(pp, c0, k)) * uni_load(I.m.invdc + e)) / (0.5f * (AT(zz, c1, k) + AT(zz, c0, k)));
-  const float bb = (0.5f * AT(I.z.zxu, e, k)) * (AT(dz, c0, k) + AT(dz, c1, k));
-  const float pool = (0.0f - AT(I.b.cqu, e, k)) * (a - bb);
+  int v0 = 0, v1 = 0;
   if (MIX) {
-    const MixEdge m = mix_edge_value(I, e, k);
+    v0 = I.m.voe[2 * e];
+    v1 = I.m.voe[2 * e + 1];
+  }
+  fence(c0);
+  fence(c1);
+  fence(v0);
+  fence(v1);
+  // every row first: the pool's, then (with mixing) del2 u's: divergence,
+  // vorticity, kdiff and rho of the edge's cells and vertices
+  const float *pp = I.b.pp, *zz = I.z.zz, *dz = I.b.dpdz;
+  const float idc = uni_load(I.m.invdc + e);
+  const float pp0 = AT(pp, c0, k), pp1 = AT(pp, c1, k), zz0 = AT(zz, c0, k), zz1 = AT(zz, c1, k);
+  const float dz0 = AT(dz, c0, k), dz1 = AT(dz, c1, k), zxu = AT(I.z.zxu, e, k), cqu = AT(I.b.cqu, e, k);
+  float d0 = 0.0f, d1 = 0.0f, z0 = 0.0f, z1 = 0.0f, kd0 = 0.0f, kd1 = 0.0f, r0 = 0.0f, r1 = 0.0f;
+  float midc = 0.0f, midv = 0.0f, del2 = 0.0f;
+  if (MIX) {
+    d0 = AT(I.b.divergence, c0, k);
+    d1 = AT(I.b.divergence, c1, k);
+    z0 = AT(I.b.vorticity, v0, k);
+    z1 = AT(I.b.vorticity, v1, k);
+    kd0 = AT(I.b.kdiff, c0, k);
+    kd1 = AT(I.b.kdiff, c1, k);
+    r0 = AT(I.b.c_rho, c0, k);
+    r1 = AT(I.b.c_rho, c1, k);
+    midc = uni_load(I.m.mx_idc + e);
+    midv = uni_load(I.m.mx_idv + e);
+    del2 = uni_load(I.m.mx_del2 + e);
+  }
+  keep(pp0, pp1, zz0, zz1, dz0, dz1, zxu, cqu);
+  if (MIX) keep(d0, d1, z0, z1, kd0, kd1, r0, r1);
+  const float a = ((pp1 - pp0) * idc) / (0.5f * (zz1 + zz0));
+  const float bb = (0.5f * zxu) * (dz0 + dz1);
+  const float pool = (0.0f - cqu) * (a - bb);
+  if (MIX) {
+    // del2 u at level k: the increment added to the u pool, and delsq_u
+    const float lap = (d1 - d0) * midc - (z1 - z0) * midv;
+    const float kedge = 0.5f * (kd0 + kd1);
+    const float rho_e = 0.5f * (r0 + r1);
+    const float inc = ((rho_e * kedge) * lap) * del2;
     if (k < I.n.L) {
-      if (I.k.mix4) AT(I.b.delsq_u, e, k) = m.lap;
-      AT(I.b.euler_ru, e, k) = pool + m.inc;
+      if (I.k.mix4) AT(I.b.delsq_u, e, k) = lap;
+      AT(I.b.euler_ru, e, k) = pool + inc;
     }
   } else {
     if (k < I.n.L) AT(I.b.euler_ru, e, k) = pool;
