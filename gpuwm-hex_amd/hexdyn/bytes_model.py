@@ -118,6 +118,12 @@ _K: Dict[str, List[Tuple[str, str]]] = {
     # delsq_theta, delsq_w, euler_theta, tend_w, euler_rw | euler_theta, euler_rw
     "k_mix_cell4": [("C", "L"), ("C", "L"), ("C", "L"), ("C", "W"), ("C", "W"), ("C", "L"), ("C", "W")],
 }
+# state rebuild (after a state is set): rho, rtheta_m, zz, rtb, rb, exb, rw, ru, zb, zb3 |
+# theta, exner, rho_p, rtheta_p, p', w
+_K["k_rebuild_cell"] = ([("C", "L")] * 6 + [("C", "W"), ("E", "L"), ("CM", "W"), ("CM", "W")]
+                        + [("C", "L")] * 5 + [("C", "W")])
+# ru, rho | u
+_K["k_rebuild_edge"] = [("E", "L"), ("C", "L"), ("E", "L")]
 _K["k_diag_edge_flux_t"] = _K["k_diag_edge_flux"]  # the edge-tiled form moves the same arrays
 
 
